@@ -1,0 +1,110 @@
+/*
+ * CPU FP64 restatement of ITK's dense Mattes mutual-information v4 metric.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
+ * compared against (pytest, the smoke check and the CPU-baseline leg of
+ * bench.py).  Product code never links, loads or calls it.
+ *
+ * Every function follows a reference file named in mattes_oracle.cpp, in the
+ * same floating-point operation order, compiled without FMA contraction.
+ */
+#ifndef ORC_MATTES_ORACLE_H
+#define ORC_MATTES_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Image (or B-spline coefficient grid) geometry, ITK conventions: x fastest,
+ * physical = origin + index_to_physical * index,
+ * continuous index = physical_to_index * (physical - origin). */
+typedef struct {
+  int32_t dim;
+  int32_t pad_;
+  int64_t size[3];
+  int64_t start[3];
+  double origin[3];
+  double spacing[3];
+  double direction[9];         /* row-major dim x dim */
+  double index_to_physical[9]; /* row-major dim x dim = direction * diag(spacing) */
+  double physical_to_index[9]; /* row-major dim x dim */
+} orc_geometry;
+
+enum { ORC_TF_IDENTITY = 0, ORC_TF_AFFINE = 1, ORC_TF_BSPLINE = 2 };
+
+typedef struct {
+  int32_t kind;
+  int32_t pad_;
+  double matrix[9];  /* affine: row-major */
+  double offset[3];  /* affine: y = matrix * x + offset */
+  double center[3];  /* affine: Jacobian is taken about this point */
+  orc_geometry grid; /* B-spline coefficient grid, size = mesh + 3 */
+  const double* coeffs; /* B-spline parameters, dim * prod(grid.size), dimension-major */
+} orc_transform;
+
+typedef struct {
+  int32_t bins;
+  int32_t pixel_f64;        /* 1: pixels are double (interpolated values not rounded to float) */
+  int32_t gradient_source;  /* 0: central differences of the moving image, 1: gradient image */
+  int32_t compute_derivative;
+  int32_t threads;          /* slabs along the slowest virtual axis */
+  int32_t pad_;
+  double fixed_min, fixed_max, moving_min, moving_max;
+} orc_options;
+
+typedef struct {
+  double value;
+  double joint_pdf_sum;
+  int64_t n_valid;
+  double* derivative;      /* [P] or NULL */
+  double* joint_pdf;       /* [B*B] normalised, row = fixed bin; or NULL */
+  double* fixed_marginal;  /* [B] normalised; or NULL */
+  double* moving_marginal; /* [B]; or NULL */
+  int16_t* fixed_bin;      /* [N virtual], -1 when the sample is not counted; or NULL */
+  int16_t* moving_bin;     /* [N virtual], Parzen window index, -1 when not counted; or NULL */
+  double* moving_value;    /* [N virtual], interpolated moving value (NaN when outside); or NULL */
+} orc_result;
+
+/* Mattes MI value (and derivative) over every voxel of the virtual domain.
+ * Returns 0 on success, 1 with the reference's exception text in err. */
+int orc_mattes_evaluate(const orc_geometry* virt, const orc_geometry* fixed_geom, const void* fixed_pixels,
+                        const orc_geometry* moving_geom, const void* moving_pixels,
+                        const double* moving_gradient, const orc_transform* fixed_tf,
+                        const orc_transform* moving_tf, const orc_options* opt, orc_result* res, char* err,
+                        int errlen);
+
+/* Bin geometry of MattesMutualInformationImageToImageMetricv4::Initialize. */
+int orc_bin_parameters(double true_min, double true_max, int32_t bins, double* bin_size, double* normalized_min,
+                       char* err, int errlen);
+
+/* Parzen bin index of one intensity (ComputeSingleFixedImageParzenWindowIndex). */
+int64_t orc_parzen_index(double value, double bin_size, double normalized_min, int32_t bins);
+
+double orc_bspline3(double u);
+double orc_bspline3_derivative(double u);
+int64_t orc_itk_floor(double x);
+
+/* Linear interpolation of a scalar image at a physical point; returns 0 when
+ * the point is outside the buffer. */
+int orc_interpolate(const orc_geometry* g, const void* pixels, int32_t pixel_f64, const double* point, double* out);
+
+/* Transform a point (affine or B-spline), as the moving transform does. */
+void orc_transform_point(const orc_transform* tf, int32_t dim, const double* in, double* out);
+
+/* GradientRecursiveGaussianImageFilter (sigma, normalise across scale, image
+ * direction applied): out has dim doubles per voxel. */
+int orc_gradient_recursive_gaussian(const orc_geometry* g, const void* pixels, int32_t pixel_f64, double sigma,
+                                    double* out, int32_t threads);
+
+/* SmoothingRecursiveGaussianImageFilter with one sigma for all axes
+ * (normalise across scale off); out is float or double like the input. */
+int orc_smooth_recursive_gaussian(const orc_geometry* g, const void* pixels, int32_t pixel_f64, double sigma,
+                                  void* out, int32_t threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

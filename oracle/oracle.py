@@ -1,0 +1,264 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg import
+this module; product code never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from itk_amd.geometry import CGeometry, ImageGeometry
+from itk_amd.transforms import TF_AFFINE, TF_BSPLINE, TF_IDENTITY
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libmattes_oracle.so")
+_lib = None
+
+
+class CTransform(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
+        ("matrix", ctypes.c_double * 9),
+        ("offset", ctypes.c_double * 3),
+        ("center", ctypes.c_double * 3),
+        ("grid", CGeometry),
+        ("coeffs", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+class COptions(ctypes.Structure):
+    _fields_ = [
+        ("bins", ctypes.c_int32),
+        ("pixel_f64", ctypes.c_int32),
+        ("gradient_source", ctypes.c_int32),
+        ("compute_derivative", ctypes.c_int32),
+        ("threads", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
+        ("fixed_min", ctypes.c_double),
+        ("fixed_max", ctypes.c_double),
+        ("moving_min", ctypes.c_double),
+        ("moving_max", ctypes.c_double),
+    ]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i16p = ctypes.POINTER(ctypes.c_int16)
+
+
+class CResult(ctypes.Structure):
+    _fields_ = [
+        ("value", ctypes.c_double),
+        ("joint_pdf_sum", ctypes.c_double),
+        ("n_valid", ctypes.c_int64),
+        ("derivative", _dp),
+        ("joint_pdf", _dp),
+        ("fixed_marginal", _dp),
+        ("moving_marginal", _dp),
+        ("fixed_bin", _i16p),
+        ("moving_bin", _i16p),
+        ("moving_value", _dp),
+    ]
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (g++ only)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_mattes_evaluate.restype = ctypes.c_int
+        L.orc_mattes_evaluate.argtypes = [
+            ctypes.POINTER(CGeometry), ctypes.POINTER(CGeometry), ctypes.c_void_p,
+            ctypes.POINTER(CGeometry), ctypes.c_void_p, _dp,
+            ctypes.POINTER(CTransform), ctypes.POINTER(CTransform), ctypes.POINTER(COptions),
+            ctypes.POINTER(CResult), ctypes.c_char_p, ctypes.c_int,
+        ]
+        L.orc_bin_parameters.restype = ctypes.c_int
+        L.orc_bin_parameters.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int32, _dp, _dp,
+                                         ctypes.c_char_p, ctypes.c_int]
+        L.orc_parzen_index.restype = ctypes.c_int64
+        L.orc_parzen_index.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int32]
+        L.orc_bspline3.restype = ctypes.c_double
+        L.orc_bspline3.argtypes = [ctypes.c_double]
+        L.orc_bspline3_derivative.restype = ctypes.c_double
+        L.orc_bspline3_derivative.argtypes = [ctypes.c_double]
+        L.orc_itk_floor.restype = ctypes.c_int64
+        L.orc_itk_floor.argtypes = [ctypes.c_double]
+        L.orc_interpolate.restype = ctypes.c_int
+        L.orc_interpolate.argtypes = [ctypes.POINTER(CGeometry), ctypes.c_void_p, ctypes.c_int32, _dp, _dp]
+        L.orc_transform_point.restype = None
+        L.orc_transform_point.argtypes = [ctypes.POINTER(CTransform), ctypes.c_int32, _dp, _dp]
+        L.orc_gradient_recursive_gaussian.restype = ctypes.c_int
+        L.orc_gradient_recursive_gaussian.argtypes = [ctypes.POINTER(CGeometry), ctypes.c_void_p, ctypes.c_int32,
+                                                      ctypes.c_double, _dp, ctypes.c_int32]
+        L.orc_smooth_recursive_gaussian.restype = ctypes.c_int
+        L.orc_smooth_recursive_gaussian.argtypes = [ctypes.POINTER(CGeometry), ctypes.c_void_p, ctypes.c_int32,
+                                                    ctypes.c_double, ctypes.c_void_p, ctypes.c_int32]
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _ptr(a, ctype=ctypes.c_double):
+    return a.ctypes.data_as(ctypes.POINTER(ctype)) if a is not None else None
+
+
+def make_transform(tf) -> tuple:
+    """(CTransform, keepalive) for an itk_amd transform or None (identity)."""
+    c = CTransform()
+    keep = []
+    if tf is None:
+        c.kind = TF_IDENTITY
+        return c, keep
+    d = tf.dim
+    c.kind = tf.kind
+    if tf.kind == TF_AFFINE:
+        m, off = tf.matrix, tf.offset
+        for i in range(d):
+            c.offset[i] = off[i]
+            c.center[i] = tf.center[i]
+            for j in range(d):
+                c.matrix[i * d + j] = m[i, j]
+    elif tf.kind == TF_BSPLINE:
+        c.grid = tf.grid.to_c()
+        coeffs = np.ascontiguousarray(tf.parameters, dtype=np.float64)
+        keep.append(coeffs)
+        c.coeffs = _ptr(coeffs)
+    else:
+        raise ValueError("unknown transform kind")
+    return c, keep
+
+
+def image_range(img: np.ndarray) -> tuple:
+    return float(img.min()), float(img.max())
+
+
+def bin_parameters(tmin, tmax, bins):
+    bs, nm = ctypes.c_double(), ctypes.c_double()
+    err = ctypes.create_string_buffer(512)
+    if lib().orc_bin_parameters(tmin, tmax, bins, ctypes.byref(bs), ctypes.byref(nm), err, 512):
+        raise OracleError(err.value.decode())
+    return bs.value, nm.value
+
+
+def mattes(fixed: np.ndarray, fixed_geom: ImageGeometry, moving: np.ndarray, moving_geom: ImageGeometry,
+           moving_tf, *, bins: int = 50, virtual_geom: ImageGeometry = None, fixed_tf=None,
+           moving_gradient: np.ndarray = None, derivative: bool = True, threads: int = 1,
+           want_samples: bool = False, ranges=None) -> dict:
+    """Dense Mattes MI (value, derivative) exactly as the reference computes it.
+
+    ``fixed``/``moving`` are float32 or float64 numpy arrays in buffer order
+    (numpy shape = reversed ITK size).  ``moving_gradient`` (float64, shape +
+    (D,)) selects the gradient-image source; ``None`` selects central
+    differences.
+    """
+    L = lib()
+    if fixed.dtype != moving.dtype or fixed.dtype not in (np.float32, np.float64):
+        raise ValueError("fixed and moving must share a float32 or float64 dtype")
+    f64 = 1 if fixed.dtype == np.float64 else 0
+    fixed = np.ascontiguousarray(fixed)
+    moving = np.ascontiguousarray(moving)
+    virtual_geom = virtual_geom or fixed_geom
+    vg, fg, mg = virtual_geom.to_c(), fixed_geom.to_c(), moving_geom.to_c()
+    ftf, k1 = make_transform(fixed_tf)
+    mtf, k2 = make_transform(moving_tf)
+    opt = COptions()
+    opt.bins = bins
+    opt.pixel_f64 = f64
+    opt.gradient_source = 1 if moving_gradient is not None else 0
+    opt.compute_derivative = 1 if derivative else 0
+    opt.threads = threads
+    if ranges is None:
+        ranges = (image_range(fixed), image_range(moving))
+    (opt.fixed_min, opt.fixed_max), (opt.moving_min, opt.moving_max) = ranges
+    P = moving_tf.number_of_parameters
+    n = virtual_geom.num_pixels
+    out = {
+        "derivative": np.zeros(P) if derivative else None,
+        "joint_pdf": np.zeros(bins * bins),
+        "fixed_marginal": np.zeros(bins),
+        "moving_marginal": np.zeros(bins),
+    }
+    if want_samples:
+        out["fixed_bin"] = np.zeros(n, dtype=np.int16)
+        out["moving_bin"] = np.zeros(n, dtype=np.int16)
+        out["moving_value"] = np.zeros(n)
+    res = CResult()
+    res.derivative = _ptr(out["derivative"])
+    res.joint_pdf = _ptr(out["joint_pdf"])
+    res.fixed_marginal = _ptr(out["fixed_marginal"])
+    res.moving_marginal = _ptr(out["moving_marginal"])
+    if want_samples:
+        res.fixed_bin = _ptr(out["fixed_bin"], ctypes.c_int16)
+        res.moving_bin = _ptr(out["moving_bin"], ctypes.c_int16)
+        res.moving_value = _ptr(out["moving_value"])
+    grad = None
+    if moving_gradient is not None:
+        grad = np.ascontiguousarray(moving_gradient, dtype=np.float64)
+    err = ctypes.create_string_buffer(1024)
+    rc = L.orc_mattes_evaluate(ctypes.byref(vg), ctypes.byref(fg), fixed.ctypes.data, ctypes.byref(mg),
+                               moving.ctypes.data, _ptr(grad), ctypes.byref(ftf), ctypes.byref(mtf),
+                               ctypes.byref(opt), ctypes.byref(res), err, 1024)
+    if rc != 0:
+        raise OracleError(err.value.decode())
+    out["value"] = res.value
+    out["n_valid"] = res.n_valid
+    out["joint_pdf_sum"] = res.joint_pdf_sum
+    out["joint_pdf"] = out["joint_pdf"].reshape(bins, bins)
+    return out
+
+
+def gradient_image(img: np.ndarray, geom: ImageGeometry, sigma: float = None, threads: int = 4) -> np.ndarray:
+    """GradientRecursiveGaussianImageFilter as ImageToImageMetricv4 configures it
+    (sigma = max spacing, normalise across scale, image direction on)."""
+    if sigma is None:
+        sigma = max(geom.spacing)
+    img = np.ascontiguousarray(img)
+    f64 = 1 if img.dtype == np.float64 else 0
+    out = np.zeros(img.shape + (geom.dim,))
+    g = geom.to_c()
+    if lib().orc_gradient_recursive_gaussian(ctypes.byref(g), img.ctypes.data, f64, sigma, _ptr(out), threads):
+        raise OracleError("gradient filter failed (each axis needs at least 4 pixels)")
+    return out
+
+
+def smooth(img: np.ndarray, geom: ImageGeometry, sigma: float, threads: int = 4) -> np.ndarray:
+    img = np.ascontiguousarray(img)
+    f64 = 1 if img.dtype == np.float64 else 0
+    out = np.zeros_like(img)
+    g = geom.to_c()
+    if lib().orc_smooth_recursive_gaussian(ctypes.byref(g), img.ctypes.data, f64, sigma, out.ctypes.data, threads):
+        raise OracleError("smoothing failed")
+    return out
+
+
+def interpolate(img: np.ndarray, geom: ImageGeometry, point):
+    img = np.ascontiguousarray(img)
+    f64 = 1 if img.dtype == np.float64 else 0
+    p = np.asarray(point, dtype=np.float64)
+    out = ctypes.c_double()
+    g = geom.to_c()
+    ok = lib().orc_interpolate(ctypes.byref(g), img.ctypes.data, f64, _ptr(p), ctypes.byref(out))
+    return out.value if ok else None
+
+
+def transform_point(tf, point) -> np.ndarray:
+    c, keep = make_transform(tf)
+    p = np.asarray(point, dtype=np.float64)
+    out = np.zeros(tf.dim)
+    lib().orc_transform_point(ctypes.byref(c), tf.dim, _ptr(p), _ptr(out))
+    return out
