@@ -1,0 +1,147 @@
+/*
+ * C ABI of the MI355X Mattes mutual-information v4 metric.
+ *
+ * Plain C types only; host pointers are borrowed for the duration of a call.
+ * Every function returns MMI_OK (0) or a negative status; the message of the
+ * last failure on the calling thread is returned by mmi_last_error().  The
+ * C++ ITK subclass converts non-zero statuses into itk::ExceptionObject with
+ * the same text the reference throws.
+ *
+ * The entry points replace, inside ITK, everything from
+ * ImageToImageMetricv4::GetValueAndDerivativeExecute down
+ * (Modules/Registration/Metricsv4/include/itkImageToImageMetricv4.hxx:239-259):
+ * the dense threader, ProcessVirtualPoint, Mattes ProcessPoint and the
+ * post-processing in MattesMutualInformationImageToImageMetricv4::ComputeResults.
+ * A context is not re-entrant: one metric instance drives it from one thread.
+ */
+#ifndef ITK_MATTES_HIP_H
+#define ITK_MATTES_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MMI_OK 0
+#define MMI_ERR_ARGUMENT -1   /* bad argument or call order */
+#define MMI_ERR_DEVICE -2     /* HIP runtime failure */
+#define MMI_ERR_DEGENERATE -3 /* the reference would throw (constant image, no overlap, ...) */
+#define MMI_ERR_MEMORY -4     /* device memory exhausted */
+
+enum mmi_pixel_type { MMI_F32 = 0, MMI_F64 = 1, MMI_BF16 = 2, MMI_F16 = 3 };
+enum mmi_transform_kind { MMI_TF_IDENTITY = 0, MMI_TF_AFFINE = 1, MMI_TF_BSPLINE = 2 };
+enum mmi_gradient_source { MMI_GRADIENT_CENTRAL_DIFFERENCE = 0, MMI_GRADIENT_IMAGE = 1 };
+
+/* Image geometry in ITK conventions (x fastest in memory).
+ * physical = origin + index_to_physical * index;
+ * continuous index = physical_to_index * (physical - origin).
+ * index_to_physical and physical_to_index are passed as ITK computed them
+ * (ImageBase::GetIndexToPhysicalPoint / GetPhysicalPointToIndex,
+ * Modules/Core/Common/include/itkImageBase.h:517-531,594-604) so that the
+ * device reproduces ITK's continuous indices bit for bit. */
+typedef struct mmi_geometry {
+  int32_t dim; /* 2 or 3 */
+  int32_t pad_;
+  int64_t size[3];
+  int64_t start[3];
+  double origin[3];
+  double spacing[3];
+  double direction[9];
+  double index_to_physical[9];
+  double physical_to_index[9];
+} mmi_geometry;
+
+typedef struct mmi_context mmi_context;
+
+/* Last error message of the calling thread (never NULL). */
+const char* mmi_last_error(void);
+
+/* Create a context on a HIP device.  flags: reserved, pass 0. */
+int mmi_context_create(int device, uint32_t flags, mmi_context** out);
+int mmi_context_destroy(mmi_context* ctx);
+
+/* Images: ImageToImageMetricv4::SetFixedImage / SetMovingImage
+ * (itkImageToImageMetricv4.h:362-390).  pixels are host memory in buffer
+ * order; pixel_type MMI_F32 or MMI_F64 (converted to the device storage type
+ * chosen by mmi_set_storage). */
+int mmi_set_fixed_image(mmi_context* ctx, const mmi_geometry* g, int32_t pixel_type, const void* pixels);
+int mmi_set_moving_image(mmi_context* ctx, const mmi_geometry* g, int32_t pixel_type, const void* pixels);
+
+/* Moving-image gradient: dim doubles per voxel in buffer order, as
+ * GradientRecursiveGaussianImageFilter produces it
+ * (ImageToImageMetricv4::ComputeMovingImageGradientFilterImage,
+ * itkImageToImageMetricv4.hxx:448-455). */
+int mmi_set_moving_gradient(mmi_context* ctx, const double* gradient);
+
+/* Compute the moving-image gradient on the device with the recursive Gaussian
+ * derivative filter, sigma = maximum spacing, normalised across scale
+ * (ImageToImageMetricv4::InitializeDefaultMovingImageGradientFilter,
+ * itkImageToImageMetricv4.hxx:487-503). */
+int mmi_compute_moving_gradient(mmi_context* ctx);
+
+/* MMI_GRADIENT_IMAGE (UseMovingImageGradientFilter on, the metric default) or
+ * MMI_GRADIENT_CENTRAL_DIFFERENCE (filter off: CentralDifferenceImageFunction). */
+int mmi_set_gradient_source(mmi_context* ctx, int32_t source);
+
+/* Device storage of fixed/moving/gradient images: MMI_F32 (default),
+ * MMI_BF16 or MMI_F16.  Accumulation stays FP32/fixed point, normalisation
+ * and the entropy sum FP64.  Must precede the image uploads. */
+int mmi_set_storage(mmi_context* ctx, int32_t pixel_type);
+
+/* Virtual domain (ObjectToObjectMetric::SetVirtualDomain,
+ * Modules/Numerics/Optimizersv4/include/itkObjectToObjectMetric.hxx:210-244).
+ * If never called, the fixed image's geometry is used. */
+int mmi_set_virtual_domain(mmi_context* ctx, const mmi_geometry* g);
+
+/* MattesMutualInformationImageToImageMetricv4::SetNumberOfHistogramBins
+ * (clamped to >= 5, itkMattesMutualInformationImageToImageMetricv4.h:146). */
+int mmi_set_number_of_histogram_bins(mmi_context* ctx, int32_t bins);
+
+/* MattesMutualInformationImageToImageMetricv4::Initialize
+ * (itkMattesMutualInformationImageToImageMetricv4.hxx:70-327): intensity
+ * ranges over the whole buffers, degenerate-range errors with the
+ * reference's messages, bin size and normalised minimum. */
+int mmi_initialize(mmi_context* ctx);
+
+/* Fixed transform (identity by default) and moving transform.
+ * MMI_TF_AFFINE:  params = matrix row-major then translation (P = D*D + D),
+ *                 fixed_params = centre (D values).
+ * MMI_TF_BSPLINE: params = coefficients, dimension-major (P = D * prod(grid)),
+ *                 fixed_params = ITK's BSplineTransform fixed parameters:
+ *                 grid size (D), grid origin (D), grid spacing (D), grid
+ *                 direction (D*D); optionally followed by the grid's
+ *                 physical-to-index matrix (D*D) as ITK computed it. */
+int mmi_set_fixed_transform(mmi_context* ctx, int32_t kind, const double* fixed_params, int32_t n_fixed,
+                            const double* params, int64_t n_params);
+int mmi_set_moving_transform(mmi_context* ctx, int32_t kind, const double* fixed_params, int32_t n_fixed,
+                             const double* params, int64_t n_params);
+/* Replace only the moving transform's parameters (optimizer step). */
+int mmi_set_moving_parameters(mmi_context* ctx, const double* params, int64_t n_params);
+
+/* GetValueAndDerivative (want_derivative = 1) or GetValue (0):
+ * ImageToImageMetricv4::GetValueAndDerivative (itkImageToImageMetricv4.hxx:217-231).
+ * value = -MI, derivative[P] = -d value / d p (the reference's sign),
+ * n_valid = NumberOfValidPoints.  joint_pdf (B*B, normalised) and
+ * fixed_marginal (B) are optional (NULL to skip). */
+int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, double* derivative, int64_t* n_valid,
+                 double* joint_pdf, double* fixed_marginal);
+
+/* Per-sample Parzen bins of the last evaluation (int16, -1 = sample not
+ * counted), in virtual buffer order; for parity tests.  Either may be NULL. */
+int mmi_get_sample_bins(mmi_context* ctx, int16_t* fixed_bin, int16_t* moving_bin);
+
+/* Bin geometry chosen by mmi_initialize. */
+int mmi_get_histogram_parameters(mmi_context* ctx, double* fixed_min, double* fixed_max, double* moving_min,
+                                 double* moving_max, double* fixed_bin_size, double* moving_bin_size);
+
+/* Timing support: average device time (ms) of the main evaluation kernels of
+ * the last mmi_evaluate, measured with HIP events on the context stream. */
+int mmi_get_last_timing(mmi_context* ctx, float* total_ms, float* pass1_ms, float* pass2_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

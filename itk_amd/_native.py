@@ -1,0 +1,85 @@
+"""ctypes binding of the C ABI in include/itk_mattes_hip.h.
+
+The HIP library is the only implementation of the metric: if it is missing,
+or no HIP device is visible, every entry point raises instead of falling
+back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from .geometry import CGeometry
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libitk_mattes_hip.so")
+
+MMI_OK = 0
+MMI_ERR_ARGUMENT = -1
+MMI_ERR_DEVICE = -2
+MMI_ERR_DEGENERATE = -3
+MMI_ERR_MEMORY = -4
+
+MMI_F32, MMI_F64, MMI_BF16, MMI_F16 = 0, 1, 2, 3
+MMI_TF_IDENTITY, MMI_TF_AFFINE, MMI_TF_BSPLINE = 0, 1, 2
+MMI_GRADIENT_CENTRAL_DIFFERENCE, MMI_GRADIENT_IMAGE = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_gp = ctypes.POINTER(CGeometry)
+
+# name -> (restype, argtypes); the exported C ABI, declaration for declaration.
+SIGNATURES = {
+    "mmi_last_error": (ctypes.c_char_p, []),
+    "mmi_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    "mmi_context_destroy": (ctypes.c_int, [_vp]),
+    "mmi_set_fixed_image": (ctypes.c_int, [_vp, _gp, ctypes.c_int32, _vp]),
+    "mmi_set_moving_image": (ctypes.c_int, [_vp, _gp, ctypes.c_int32, _vp]),
+    "mmi_set_moving_gradient": (ctypes.c_int, [_vp, _dp]),
+    "mmi_compute_moving_gradient": (ctypes.c_int, [_vp]),
+    "mmi_set_gradient_source": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mmi_set_storage": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mmi_set_virtual_domain": (ctypes.c_int, [_vp, _gp]),
+    "mmi_set_number_of_histogram_bins": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mmi_initialize": (ctypes.c_int, [_vp]),
+    "mmi_set_fixed_transform": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, ctypes.c_int32, _dp, ctypes.c_int64]),
+    "mmi_set_moving_transform": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, ctypes.c_int32, _dp, ctypes.c_int64]),
+    "mmi_set_moving_parameters": (ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
+    "mmi_evaluate": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp, ctypes.POINTER(ctypes.c_int64), _dp, _dp]),
+    "mmi_get_sample_bins": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int16), ctypes.POINTER(ctypes.c_int16)]),
+    "mmi_get_histogram_parameters": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "mmi_get_last_timing": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                           ctypes.POINTER(ctypes.c_float)]),
+}
+
+_lib = None
+
+
+class MetricError(RuntimeError):
+    """Raised for every non-zero status; the message is the C ABI's (the
+    reference's exception text for degenerate inputs)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library (no device work).  Raises if it was never built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("the HIP metric library %s is missing: run `python -m itk_amd.build`" % path)
+    L = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int):
+    if rc != MMI_OK:
+        raise MetricError(rc, load().mmi_last_error().decode())

@@ -1,0 +1,777 @@
+// C ABI of the MI355X Mattes MI v4 metric: device context, image and
+// transform uploads, MattesMutualInformationImageToImageMetricv4::Initialize
+// and GetValue / GetValueAndDerivative on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/itk_mattes_hip.h"
+#include "mattes_kernels.h"
+
+using namespace mmi;
+
+namespace {
+
+thread_local std::string g_last_error = "";
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define MMI_HIP(call)                                                                                  \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess) {                                                                            \
+      return fail(e_ == hipErrorOutOfMemory ? MMI_ERR_MEMORY : MMI_ERR_DEVICE,                        \
+                  std::string("HIP error in ") + #call + ": " + hipGetErrorString(e_));                \
+    }                                                                                                  \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      if (e != hipSuccess) return e;
+      p = nullptr;
+      bytes = 0;
+    }
+    hipError_t e = hipMalloc(&p, n ? n : 1);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+DevGeom to_dev(const mmi_geometry& g) {
+  DevGeom d;
+  std::memset(&d, 0, sizeof(d));
+  d.dim = g.dim;
+  for (int i = 0; i < 3; ++i) {
+    d.size[i] = i < g.dim ? g.size[i] : 1;
+    d.start[i] = i < g.dim ? g.start[i] : 0;
+    d.end[i] = d.start[i] + d.size[i] - 1;
+    d.origin[i] = i < g.dim ? g.origin[i] : 0.0;
+    d.spacing[i] = i < g.dim ? g.spacing[i] : 1.0;
+    // ImageFunction::SetInputImage bounds (itkImageFunction.hxx:57-64)
+    d.cilo[i] = (double)d.start[i] - 0.5;
+    d.cihi[i] = (double)d.end[i] + 0.5;
+  }
+  for (int i = 0; i < 9; ++i) {
+    d.i2p[i] = g.index_to_physical[i];
+    d.p2i[i] = g.physical_to_index[i];
+  }
+  d.sy = d.size[0];
+  d.sz = d.size[0] * d.size[1];
+  return d;
+}
+
+int64_t num_pixels(const mmi_geometry& g) {
+  int64_t n = 1;
+  for (int i = 0; i < g.dim; ++i) n *= g.size[i];
+  return n;
+}
+
+int check_geometry(const mmi_geometry* g) {
+  if (!g) return fail(MMI_ERR_ARGUMENT, "geometry is NULL");
+  if (g->dim != 2 && g->dim != 3) return fail(MMI_ERR_ARGUMENT, "only 2-D and 3-D images are supported");
+  for (int i = 0; i < g->dim; ++i)
+    if (g->size[i] < 1) return fail(MMI_ERR_ARGUMENT, "image size must be positive");
+  return MMI_OK;
+}
+
+// RecursiveGaussianImageFilter::SetUp for orders 0 and 1
+// (Modules/Filtering/Smoothing/include/itkRecursiveGaussianImageFilter.hxx:56-300).
+// The exponential-series parameters are the published table of Farneback &
+// Westin (2006), identical to the reference's.
+bool rec_setup(double sigma, double spacing, int order, bool normalizeAcrossScale, RecCoeffs* c) {
+  double direction = 1.0;
+  if (spacing < 0.0) {
+    direction = -1.0;
+    spacing *= -1;
+  }
+  if (spacing < 1e-8 || !(sigma > 0.0)) return false;
+  const double sigmad = sigma / spacing;
+  double across = 1.0;
+  const double A1[2] = {1.3530, -0.6724};
+  const double B1[2] = {1.8151, -3.4327};
+  const double W1 = 0.6681, L1 = -1.3932;
+  const double A2[2] = {-0.3531, 0.6724};
+  const double B2[2] = {0.0902, 0.6100};
+  const double W2 = 2.0787, L2 = -1.3732;
+  const double Cos1 = std::cos(W1 / sigmad), Cos2 = std::cos(W2 / sigmad);
+  const double Sin1 = std::sin(W1 / sigmad), Sin2 = std::sin(W2 / sigmad);
+  const double Exp1 = std::exp(L1 / sigmad), Exp2 = std::exp(L2 / sigmad);
+  c->D4 = Exp1 * Exp1 * Exp2 * Exp2;
+  c->D3 = -2 * Cos1 * Exp1 * Exp2 * Exp2;
+  c->D3 += -2 * Cos2 * Exp2 * Exp1 * Exp1;
+  c->D2 = 4 * Cos2 * Cos1 * Exp1 * Exp2;
+  c->D2 += Exp1 * Exp1 + Exp2 * Exp2;
+  c->D1 = -2 * (Exp2 * Cos2 + Exp1 * Cos1);
+  const double SD = 1.0 + c->D1 + c->D2 + c->D3 + c->D4;
+  const double DD = c->D1 + 2 * c->D2 + 3 * c->D3 + 4 * c->D4;
+  const int o = order;
+  const double a1 = A1[o], b1 = B1[o], a2 = A2[o], b2 = B2[o];
+  c->N0 = a1 + a2;
+  c->N1 = Exp2 * (b2 * Sin2 - (a2 + 2 * a1) * Cos2);
+  c->N1 += Exp1 * (b1 * Sin1 - (a1 + 2 * a2) * Cos1);
+  c->N2 = (a1 + a2) * Cos2 * Cos1;
+  c->N2 -= b1 * Cos2 * Sin1 + b2 * Cos1 * Sin2;
+  c->N2 *= 2 * Exp1 * Exp2;
+  c->N2 += a2 * Exp1 * Exp1 + a1 * Exp2 * Exp2;
+  c->N3 = Exp2 * Exp1 * Exp1 * (b2 * Sin2 - a2 * Cos2);
+  c->N3 += Exp1 * Exp2 * Exp2 * (b1 * Sin1 - a1 * Cos1);
+  const double SN = c->N0 + c->N1 + c->N2 + c->N3;
+  const double DN = c->N1 + 2 * c->N2 + 3 * c->N3;
+  bool symmetric;
+  if (order == 0) {
+    const double alpha0 = 2 * SN / SD - c->N0;
+    c->N0 *= across / alpha0;
+    c->N1 *= across / alpha0;
+    c->N2 *= across / alpha0;
+    c->N3 *= across / alpha0;
+    symmetric = true;
+  } else {
+    if (normalizeAcrossScale) across = sigma;
+    double alpha1 = 2 * (SN * DD - DN * SD) / (SD * SD);
+    alpha1 *= direction;
+    c->N0 *= across / alpha1;
+    c->N1 *= across / alpha1;
+    c->N2 *= across / alpha1;
+    c->N3 *= across / alpha1;
+    symmetric = false;
+  }
+  if (symmetric) {
+    c->M1 = c->N1 - c->D1 * c->N0;
+    c->M2 = c->N2 - c->D2 * c->N0;
+    c->M3 = c->N3 - c->D3 * c->N0;
+    c->M4 = -c->D4 * c->N0;
+  } else {
+    c->M1 = -(c->N1 - c->D1 * c->N0);
+    c->M2 = -(c->N2 - c->D2 * c->N0);
+    c->M3 = -(c->N3 - c->D3 * c->N0);
+    c->M4 = c->D4 * c->N0;
+  }
+  const double SN2 = c->N0 + c->N1 + c->N2 + c->N3;
+  const double SM = c->M1 + c->M2 + c->M3 + c->M4;
+  const double SD2 = 1.0 + c->D1 + c->D2 + c->D3 + c->D4;
+  c->BN1 = c->D1 * SN2 / SD2;
+  c->BN2 = c->D2 * SN2 / SD2;
+  c->BN3 = c->D3 * SN2 / SD2;
+  c->BN4 = c->D4 * SN2 / SD2;
+  c->BM1 = c->D1 * SM / SD2;
+  c->BM2 = c->D2 * SM / SD2;
+  c->BM3 = c->D3 * SM / SD2;
+  c->BM4 = c->D4 * SM / SD2;
+  return true;
+}
+
+// Inverse of a small matrix: exact reciprocal for axis-permutation matrices
+// (what ITK's SVD pseudo-inverse gives there), Gauss-Jordan otherwise.
+void invert(const double* m, int D, double* inv) {
+  bool perm = true;
+  for (int i = 0; i < D && perm; ++i) {
+    int nz = 0;
+    for (int j = 0; j < D; ++j) nz += m[i * D + j] != 0.0;
+    perm = nz == 1;
+  }
+  for (int j = 0; j < D && perm; ++j) {
+    int nz = 0;
+    for (int i = 0; i < D; ++i) nz += m[i * D + j] != 0.0;
+    perm = nz == 1;
+  }
+  if (perm) {
+    for (int i = 0; i < D * D; ++i) inv[i] = 0.0;
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j)
+        if (m[i * D + j] != 0.0) inv[j * D + i] = 1.0 / m[i * D + j];
+    return;
+  }
+  double a[3][6];
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < 2 * D; ++j) a[i][j] = j < D ? m[i * D + j] : (j - D == i ? 1.0 : 0.0);
+  for (int c = 0; c < D; ++c) {
+    int p = c;
+    for (int r = c + 1; r < D; ++r)
+      if (std::fabs(a[r][c]) > std::fabs(a[p][c])) p = r;
+    for (int j = 0; j < 2 * D; ++j) std::swap(a[c][j], a[p][j]);
+    const double piv = a[c][c];
+    for (int j = 0; j < 2 * D; ++j) a[c][j] /= piv;
+    for (int r = 0; r < D; ++r) {
+      if (r == c) continue;
+      const double f = a[r][c];
+      for (int j = 0; j < 2 * D; ++j) a[r][j] -= f * a[c][j];
+    }
+  }
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j) inv[i * D + j] = a[i][j + D];
+}
+
+struct HostTransform {
+  int32_t kind = MMI_TF_IDENTITY;
+  std::vector<double> fixed;
+  std::vector<double> params;
+};
+
+}  // namespace
+
+struct mmi_context {
+  int device = 0;
+  uint32_t flags = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 256;
+  int32_t bins = 50;
+  int32_t gradient_source = MMI_GRADIENT_IMAGE;
+  int32_t storage = MMI_F32;
+
+  bool have_fixed = false, have_moving = false, have_virtual = false, have_texels = false;
+  mmi_geometry fixed_g{}, moving_g{}, virtual_g{};
+  DevBuf fixed_px, moving_px, texels;
+
+  bool initialized = false;
+  double fixed_min = 0, fixed_max = 0, moving_min = 0, moving_max = 0;
+  double fixed_bin_size = 0, fixed_norm_min = 0, moving_bin_size = 0, moving_norm_min = 0;
+
+  HostTransform ftf, mtf;
+  DevTransform dftf{}, dmtf{};
+  DevBuf coeffs;
+
+  DevBuf hist, fmarg, nvalid, pdf, fm, mm, pratio, rowsum, result, partials, deriv, sample_fbin, sample_mbin, scratch;
+  int grid = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  float last_total_ms = 0, last_pass1_ms = 0, last_pass2_ms = 0;
+};
+
+namespace {
+
+int build_transform(mmi_context* ctx, int D, HostTransform& h, DevTransform& d, bool moving) {
+  std::memset(&d, 0, sizeof(d));
+  d.kind = h.kind;
+  if (h.kind == MMI_TF_IDENTITY) return MMI_OK;
+  if (h.kind == MMI_TF_AFFINE) {
+    if ((int)h.params.size() != D * D + D) return fail(MMI_ERR_ARGUMENT, "affine transform needs D*D+D parameters");
+    if ((int)h.fixed.size() != D) return fail(MMI_ERR_ARGUMENT, "affine transform needs D fixed parameters (centre)");
+    // MatrixOffsetTransformBase::SetParameters + ComputeOffset (MOTB:518-556, 615-631)
+    for (int i = 0; i < D * D; ++i) d.m[i] = h.params[i];
+    for (int i = 0; i < D; ++i) {
+      const double t = h.params[D * D + i];
+      const double c = h.fixed[i];
+      double off = t + c;
+      for (int j = 0; j < D; ++j) off -= d.m[i * D + j] * h.fixed[j];
+      d.off[i] = off;
+      d.center[i] = c;
+    }
+    return MMI_OK;
+  }
+  if (h.kind == MMI_TF_BSPLINE) {
+    if (!moving) return fail(MMI_ERR_ARGUMENT, "the fixed transform must be identity or affine");
+    const int nf = (int)h.fixed.size();
+    if (nf != 3 * D + D * D && nf != 3 * D + 2 * D * D)
+      return fail(MMI_ERR_ARGUMENT, "B-spline fixed parameters: size, origin, spacing, direction [, physical_to_index]");
+    mmi_geometry g{};
+    g.dim = D;
+    int64_t nodes = 1;
+    for (int i = 0; i < D; ++i) {
+      g.size[i] = (int64_t)std::llround(h.fixed[i]);
+      g.start[i] = 0;
+      g.origin[i] = h.fixed[D + i];
+      g.spacing[i] = h.fixed[2 * D + i];
+      if (g.size[i] < 4) return fail(MMI_ERR_ARGUMENT, "B-spline grid needs at least 4 nodes per axis");
+      nodes *= g.size[i];
+    }
+    for (int i = 0; i < D * D; ++i) g.direction[i] = h.fixed[3 * D + i];
+    // ImageBase::ComputeIndexToPhysicalPointMatrices: direction * diag(spacing)
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < D; ++k) s += g.direction[i * D + k] * (k == j ? g.spacing[j] : 0.0);
+        g.index_to_physical[i * D + j] = s;
+      }
+    if (nf == 3 * D + 2 * D * D) {
+      for (int i = 0; i < D * D; ++i) g.physical_to_index[i] = h.fixed[3 * D + D * D + i];
+    } else {
+      invert(g.index_to_physical, D, g.physical_to_index);
+    }
+    if ((int64_t)h.params.size() != D * nodes) return fail(MMI_ERR_ARGUMENT, "B-spline needs D * prod(grid) parameters");
+    d.grid = to_dev(g);
+    d.pdim = nodes;
+    MMI_HIP(ctx->coeffs.ensure(sizeof(double) * h.params.size()));
+    MMI_HIP(hipMemcpyAsync(ctx->coeffs.p, h.params.data(), sizeof(double) * h.params.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    d.coeffs = ctx->coeffs.as<double>();
+    return MMI_OK;
+  }
+  return fail(MMI_ERR_ARGUMENT, "unknown transform kind");
+}
+
+int upload_image(mmi_context* ctx, DevBuf& buf, const mmi_geometry* g, int32_t pixel_type, const void* pixels) {
+  if (int rc = check_geometry(g)) return rc;
+  if (!pixels) return fail(MMI_ERR_ARGUMENT, "pixels is NULL");
+  const int64_t n = num_pixels(*g);
+  MMI_HIP(buf.ensure(sizeof(float) * n));
+  if (pixel_type == MMI_F32) {
+    MMI_HIP(hipMemcpyAsync(buf.p, pixels, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    MMI_HIP(hipStreamSynchronize(ctx->stream));
+  } else if (pixel_type == MMI_F64) {
+    std::vector<float> tmp(n);
+    const double* src = static_cast<const double*>(pixels);
+    for (int64_t i = 0; i < n; ++i) tmp[i] = (float)src[i];
+    MMI_HIP(hipMemcpyAsync(buf.p, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    MMI_HIP(hipStreamSynchronize(ctx->stream));
+  } else {
+    return fail(MMI_ERR_ARGUMENT, "pixel_type must be MMI_F32 or MMI_F64");
+  }
+  return MMI_OK;
+}
+
+int device_minmax(mmi_context* ctx, const float* px, int64_t n, double* lo, double* hi) {
+  const int blocks = (int)std::min<int64_t>(1024, (n + 255) / 256);
+  MMI_HIP(ctx->scratch.ensure(sizeof(float) * 2 * blocks));
+  MMI_HIP(launch_minmax(px, n, ctx->scratch.as<float>(), blocks, ctx->stream));
+  std::vector<float> h(2 * blocks);
+  MMI_HIP(hipMemcpyAsync(h.data(), ctx->scratch.p, sizeof(float) * 2 * blocks, hipMemcpyDeviceToHost, ctx->stream));
+  MMI_HIP(hipStreamSynchronize(ctx->stream));
+  float a = std::numeric_limits<float>::max(), b = -std::numeric_limits<float>::max();
+  for (int i = 0; i < blocks; ++i) {
+    a = std::min(a, h[2 * i]);
+    b = std::max(b, h[2 * i + 1]);
+  }
+  *lo = a;
+  *hi = b;
+  return MMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mmi_last_error(void) { return g_last_error.c_str(); }
+
+int mmi_context_create(int device, uint32_t flags, mmi_context** out) {
+  if (!out) return fail(MMI_ERR_ARGUMENT, "out is NULL");
+  int count = 0;
+  MMI_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) return fail(MMI_ERR_ARGUMENT, "no such HIP device");
+  MMI_HIP(hipSetDevice(device));
+  auto* ctx = new mmi_context();
+  ctx->device = device;
+  ctx->flags = flags;
+  hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return fail(MMI_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    ctx->cu_count = prop.multiProcessorCount;
+  for (auto& ev : ctx->ev) {
+    if (hipEventCreate(&ev) != hipSuccess) {
+      mmi_context_destroy(ctx);
+      return fail(MMI_ERR_DEVICE, "hipEventCreate failed");
+    }
+  }
+  *out = ctx;
+  return MMI_OK;
+}
+
+int mmi_context_destroy(mmi_context* ctx) {
+  if (!ctx) return MMI_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  DevBuf* bufs[] = {&ctx->fixed_px, &ctx->moving_px, &ctx->texels, &ctx->coeffs, &ctx->hist, &ctx->fmarg,
+                    &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->rowsum, &ctx->result,
+                    &ctx->partials, &ctx->deriv, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch};
+  for (DevBuf* b : bufs) b->release();
+  for (auto& ev : ctx->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return MMI_OK;
+}
+
+int mmi_set_storage(mmi_context* ctx, int32_t pixel_type) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (pixel_type != MMI_F32)
+    return fail(MMI_ERR_ARGUMENT, "this build stores images as MMI_F32 only");
+  ctx->storage = pixel_type;
+  return MMI_OK;
+}
+
+int mmi_set_fixed_image(mmi_context* ctx, const mmi_geometry* g, int32_t pixel_type, const void* pixels) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  MMI_HIP(hipSetDevice(ctx->device));
+  if (int rc = upload_image(ctx, ctx->fixed_px, g, pixel_type, pixels)) return rc;
+  ctx->fixed_g = *g;
+  ctx->have_fixed = true;
+  ctx->initialized = false;
+  return MMI_OK;
+}
+
+int mmi_set_moving_image(mmi_context* ctx, const mmi_geometry* g, int32_t pixel_type, const void* pixels) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  MMI_HIP(hipSetDevice(ctx->device));
+  if (int rc = upload_image(ctx, ctx->moving_px, g, pixel_type, pixels)) return rc;
+  ctx->moving_g = *g;
+  ctx->have_moving = true;
+  ctx->have_texels = false;
+  ctx->initialized = false;
+  return MMI_OK;
+}
+
+int mmi_set_moving_gradient(mmi_context* ctx, const double* gradient) {
+  if (!ctx || !gradient) return fail(MMI_ERR_ARGUMENT, "NULL argument");
+  if (!ctx->have_moving) return fail(MMI_ERR_ARGUMENT, "set the moving image first");
+  MMI_HIP(hipSetDevice(ctx->device));
+  const int D = ctx->moving_g.dim;
+  const int64_t n = num_pixels(ctx->moving_g);
+  DevBuf tmp;
+  MMI_HIP(tmp.ensure(sizeof(double) * n * D));
+  MMI_HIP(hipMemcpyAsync(tmp.p, gradient, sizeof(double) * n * D, hipMemcpyHostToDevice, ctx->stream));
+  MMI_HIP(ctx->texels.ensure(sizeof(float4) * n));
+  Direction ident{};
+  for (int i = 0; i < D; ++i) ident.m[i * D + i] = 1.0;
+  MMI_HIP(launch_pack_texels(ctx->moving_px.as<float>(), tmp.as<double>(), n, D, ident, ctx->texels.as<float4>(),
+                             ctx->stream));
+  MMI_HIP(hipStreamSynchronize(ctx->stream));
+  tmp.release();
+  ctx->have_texels = true;
+  return MMI_OK;
+}
+
+int mmi_compute_moving_gradient(mmi_context* ctx) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (!ctx->have_moving) return fail(MMI_ERR_ARGUMENT, "set the moving image first");
+  MMI_HIP(hipSetDevice(ctx->device));
+  const mmi_geometry& g = ctx->moving_g;
+  const int D = g.dim;
+  const int64_t n = num_pixels(g);
+  for (int i = 0; i < D; ++i)
+    if (g.size[i] < 4)
+      return fail(MMI_ERR_ARGUMENT,
+                  "The number of pixels along direction " + std::to_string(i) +
+                      " is less than 4. This filter requires a minimum of four pixels along the dimension to be "
+                      "processed.");
+  // GradientRecursiveGaussianImageFilter as the metric configures it:
+  // sigma = maximum spacing, normalised across scale (I2IM:487-503).
+  double sigma = 0.0;
+  for (int i = 0; i < D; ++i) sigma = std::max(sigma, g.spacing[i]);
+  DevBuf a, b, grad;
+  MMI_HIP(a.ensure(sizeof(double) * n));
+  MMI_HIP(b.ensure(sizeof(double) * n));
+  MMI_HIP(grad.ensure(sizeof(double) * n * D));
+  for (int dim = 0; dim < D; ++dim) {
+    MMI_HIP(launch_float_to_double(ctx->moving_px.as<float>(), a.as<double>(), n, ctx->stream));
+    double* src = a.as<double>();
+    double* dst = b.as<double>();
+    // derivative along dim, then zero order along the other axes in
+    // increasing order (GradRG:214-238)
+    int order[3] = {dim, -1, -1};
+    int k = 1;
+    for (int o = 0; o < D; ++o)
+      if (o != dim) order[k++] = o;
+    for (int step = 0; step < D; ++step) {
+      const int axis = order[step];
+      RecCoeffs c;
+      if (!rec_setup(sigma, g.spacing[axis], step == 0 ? 1 : 0, true, &c))
+        return fail(MMI_ERR_ARGUMENT, "recursive Gaussian: spacing or sigma out of range");
+      int64_t stride = 1;
+      for (int j = 0; j < axis; ++j) stride *= g.size[j];
+      const int64_t ln = g.size[axis];
+      MMI_HIP(launch_recursive_gaussian(src, dst, ln, stride, n / ln, c, ctx->stream));
+      std::swap(src, dst);
+    }
+    MMI_HIP(launch_gradient_component(src, n, g.spacing[dim], dim, grad.as<double>(), D, ctx->stream));
+  }
+  Direction dir{};
+  for (int i = 0; i < D * D; ++i) dir.m[i] = g.direction[i];
+  MMI_HIP(ctx->texels.ensure(sizeof(float4) * n));
+  MMI_HIP(launch_pack_texels(ctx->moving_px.as<float>(), grad.as<double>(), n, D, dir, ctx->texels.as<float4>(),
+                             ctx->stream));
+  MMI_HIP(hipStreamSynchronize(ctx->stream));
+  a.release();
+  b.release();
+  grad.release();
+  ctx->have_texels = true;
+  return MMI_OK;
+}
+
+int mmi_set_gradient_source(mmi_context* ctx, int32_t source) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (source != MMI_GRADIENT_IMAGE && source != MMI_GRADIENT_CENTRAL_DIFFERENCE)
+    return fail(MMI_ERR_ARGUMENT, "unknown gradient source");
+  ctx->gradient_source = source;
+  return MMI_OK;
+}
+
+int mmi_set_virtual_domain(mmi_context* ctx, const mmi_geometry* g) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (int rc = check_geometry(g)) return rc;
+  ctx->virtual_g = *g;
+  ctx->have_virtual = true;
+  return MMI_OK;
+}
+
+int mmi_set_number_of_histogram_bins(mmi_context* ctx, int32_t bins) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  ctx->bins = std::max<int32_t>(5, bins);  // itkSetClampMacro(..., 5, max)
+  ctx->initialized = false;
+  return MMI_OK;
+}
+
+int mmi_initialize(mmi_context* ctx) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (!ctx->have_fixed) return fail(MMI_ERR_ARGUMENT, "FixedImage is not present");
+  if (!ctx->have_moving) return fail(MMI_ERR_ARGUMENT, "MovingImage is not present");
+  if (ctx->fixed_g.dim != ctx->moving_g.dim) return fail(MMI_ERR_ARGUMENT, "fixed and moving dimensions differ");
+  MMI_HIP(hipSetDevice(ctx->device));
+  if (ctx->gradient_source == MMI_GRADIENT_IMAGE && !ctx->have_texels) {
+    if (int rc = mmi_compute_moving_gradient(ctx)) return rc;
+  }
+  if (int rc = device_minmax(ctx, ctx->fixed_px.as<float>(), num_pixels(ctx->fixed_g), &ctx->fixed_min,
+                             &ctx->fixed_max))
+    return rc;
+  if (int rc = device_minmax(ctx, ctx->moving_px.as<float>(), num_pixels(ctx->moving_g), &ctx->moving_min,
+                             &ctx->moving_max))
+    return rc;
+  // Mattes:273-317
+  const double eps = std::numeric_limits<double>::epsilon();
+  char buf[512];
+  if (ctx->fixed_max - ctx->fixed_min <= eps) {
+    std::snprintf(buf, sizeof(buf),
+                  "All fixed-image samples in the analysis region have the same intensity value (%g). Mattes mutual "
+                  "information is undefined for a constant image: the marginal entropy is zero and no meaningful "
+                  "gradient can be computed.",
+                  ctx->fixed_min);
+    return fail(MMI_ERR_DEGENERATE, buf);
+  }
+  if (ctx->moving_max - ctx->moving_min <= eps) {
+    std::snprintf(buf, sizeof(buf),
+                  "All moving-image samples in the analysis region have the same intensity value (%g). Mattes "
+                  "mutual information is undefined for a constant image: the marginal entropy is zero and no "
+                  "meaningful gradient can be computed.",
+                  ctx->moving_min);
+    return fail(MMI_ERR_DEGENERATE, buf);
+  }
+  const int padding = 2;
+  ctx->fixed_bin_size = (ctx->fixed_max - ctx->fixed_min) / (double)(ctx->bins - 2 * padding);
+  ctx->fixed_norm_min = ctx->fixed_min / ctx->fixed_bin_size - (double)padding;
+  ctx->moving_bin_size = (ctx->moving_max - ctx->moving_min) / (double)(ctx->bins - 2 * padding);
+  ctx->moving_norm_min = ctx->moving_min / ctx->moving_bin_size - (double)padding;
+  if (!ctx->have_virtual) ctx->virtual_g = ctx->fixed_g;
+  const int B = ctx->bins;
+  MMI_HIP(ctx->hist.ensure(sizeof(unsigned long long) * B * B));
+  MMI_HIP(ctx->fmarg.ensure(sizeof(unsigned long long) * B));
+  MMI_HIP(ctx->nvalid.ensure(sizeof(unsigned long long)));
+  MMI_HIP(ctx->pdf.ensure(sizeof(double) * B * B));
+  MMI_HIP(ctx->pratio.ensure(sizeof(double) * B * B));
+  MMI_HIP(ctx->fm.ensure(sizeof(double) * B));
+  MMI_HIP(ctx->mm.ensure(sizeof(double) * B));
+  MMI_HIP(ctx->rowsum.ensure(sizeof(double) * B));
+  MMI_HIP(ctx->result.ensure(sizeof(DevResult)));
+  const int64_t n = num_pixels(ctx->virtual_g);
+  ctx->grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->cu_count * 4));
+  if (ctx->flags & 1u) {
+    MMI_HIP(ctx->sample_fbin.ensure(sizeof(int16_t) * n));
+    MMI_HIP(ctx->sample_mbin.ensure(sizeof(int16_t) * n));
+  }
+  ctx->initialized = true;
+  return MMI_OK;
+}
+
+static int set_transform(mmi_context* ctx, HostTransform& h, DevTransform& d, bool moving, int32_t kind,
+                         const double* fixed_params, int32_t n_fixed, const double* params, int64_t n_params) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (!ctx->have_fixed) return fail(MMI_ERR_ARGUMENT, "set the fixed image before the transforms");
+  MMI_HIP(hipSetDevice(ctx->device));
+  h.kind = kind;
+  h.fixed.assign(fixed_params, fixed_params + std::max(0, n_fixed));
+  h.params.assign(params, params + std::max<int64_t>(0, n_params));
+  return build_transform(ctx, ctx->fixed_g.dim, h, d, moving);
+}
+
+int mmi_set_fixed_transform(mmi_context* ctx, int32_t kind, const double* fixed_params, int32_t n_fixed,
+                            const double* params, int64_t n_params) {
+  return set_transform(ctx, ctx->ftf, ctx->dftf, false, kind, fixed_params, n_fixed, params, n_params);
+}
+
+int mmi_set_moving_transform(mmi_context* ctx, int32_t kind, const double* fixed_params, int32_t n_fixed,
+                             const double* params, int64_t n_params) {
+  return set_transform(ctx, ctx->mtf, ctx->dmtf, true, kind, fixed_params, n_fixed, params, n_params);
+}
+
+int mmi_set_moving_parameters(mmi_context* ctx, const double* params, int64_t n_params) {
+  if (!ctx || !params) return fail(MMI_ERR_ARGUMENT, "NULL argument");
+  if ((int64_t)ctx->mtf.params.size() != n_params)
+    return fail(MMI_ERR_ARGUMENT, "parameter count differs from the moving transform's");
+  ctx->mtf.params.assign(params, params + n_params);
+  return build_transform(ctx, ctx->fixed_g.dim, ctx->mtf, ctx->dmtf, true);
+}
+
+int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, double* derivative, int64_t* n_valid,
+                 double* joint_pdf, double* fixed_marginal) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (!ctx->initialized) return fail(MMI_ERR_ARGUMENT, "mmi_initialize must precede mmi_evaluate");
+  if (ctx->mtf.kind == MMI_TF_IDENTITY && want_derivative)
+    return fail(MMI_ERR_ARGUMENT, "MovingTransform is not present");
+  if (ctx->virtual_g.dim != ctx->fixed_g.dim) return fail(MMI_ERR_ARGUMENT, "virtual domain dimension differs");
+  MMI_HIP(hipSetDevice(ctx->device));
+  const int D = ctx->fixed_g.dim;
+  const int B = ctx->bins;
+  const int64_t n = num_pixels(ctx->virtual_g);
+  const int64_t P = (int64_t)ctx->mtf.params.size();
+  hipStream_t st = ctx->stream;
+
+  EvalArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.virt = to_dev(ctx->virtual_g);
+  a.fixed = to_dev(ctx->fixed_g);
+  a.moving = to_dev(ctx->moving_g);
+  a.ftf = ctx->dftf;
+  a.mtf = ctx->dmtf;
+  a.fixed_px = ctx->fixed_px.as<float>();
+  a.moving_px = ctx->moving_px.as<float>();
+  a.texels = ctx->texels.as<float4>();
+  a.n = n;
+  a.bins = B;
+  a.hist_in_lds = (size_t)B * B * 8 + (B + 1) * 4 <= 64 * 1024 ? 1 : 0;
+  a.gradient_source = ctx->gradient_source;
+  a.moving_min = ctx->moving_min;
+  a.moving_max = ctx->moving_max;
+  a.moving_bin_size = ctx->moving_bin_size;
+  a.moving_norm_min = ctx->moving_norm_min;
+  a.fixed_bin_size = ctx->fixed_bin_size;
+  a.fixed_norm_min = ctx->fixed_norm_min;
+  a.hist = ctx->hist.as<unsigned long long>();
+  a.fmarg = ctx->fmarg.as<unsigned long long>();
+  a.nvalid = ctx->nvalid.as<unsigned long long>();
+  if (ctx->flags & 1u) {
+    a.sample_fbin = ctx->sample_fbin.as<int16_t>();
+    a.sample_mbin = ctx->sample_mbin.as<int16_t>();
+  }
+  a.pratio = ctx->pratio.as<double>();
+  if (want_derivative && ctx->gradient_source == MMI_GRADIENT_IMAGE && !ctx->have_texels)
+    return fail(MMI_ERR_ARGUMENT, "moving gradient image missing");
+
+  MMI_HIP(hipMemsetAsync(ctx->hist.p, 0, sizeof(unsigned long long) * B * B, st));
+  MMI_HIP(hipMemsetAsync(ctx->fmarg.p, 0, sizeof(unsigned long long) * B, st));
+  MMI_HIP(hipMemsetAsync(ctx->nvalid.p, 0, sizeof(unsigned long long), st));
+  MMI_HIP(hipEventRecord(ctx->ev[0], st));
+  MMI_HIP(launch_pass1(a, ctx->grid, st));
+  MMI_HIP(hipEventRecord(ctx->ev[1], st));
+
+  FinalizeArgs f;
+  std::memset(&f, 0, sizeof(f));
+  f.bins = B;
+  f.hist = a.hist;
+  f.fmarg = a.fmarg;
+  f.nvalid = a.nvalid;
+  f.pdf = ctx->pdf.as<double>();
+  f.fm = ctx->fm.as<double>();
+  f.mm = ctx->mm.as<double>();
+  f.pratio = ctx->pratio.as<double>();
+  f.rowsum = ctx->rowsum.as<double>();
+  f.result = ctx->result.as<DevResult>();
+  f.moving_bin_size = ctx->moving_bin_size;
+  MMI_HIP(launch_finalize(f, st));
+
+  if (want_derivative) {
+    MMI_HIP(ctx->deriv.ensure(sizeof(double) * P));
+    MMI_HIP(hipEventRecord(ctx->ev[2], st));
+    if (ctx->mtf.kind == MMI_TF_AFFINE) {
+      MMI_HIP(ctx->partials.ensure(sizeof(double) * P * ctx->grid));
+      a.partials = ctx->partials.as<double>();
+      MMI_HIP(launch_pass2(a, ctx->grid, st));
+      MMI_HIP(hipEventRecord(ctx->ev[3], st));
+      MMI_HIP(launch_derivative_reduce(a.partials, ctx->grid, (int)P, f.result, ctx->deriv.as<double>(), st));
+    } else {
+      MMI_HIP(hipMemsetAsync(ctx->deriv.p, 0, sizeof(double) * P, st));
+      a.deriv_accum = ctx->deriv.as<double>();
+      MMI_HIP(launch_pass2(a, ctx->grid, st));
+      MMI_HIP(hipEventRecord(ctx->ev[3], st));
+      MMI_HIP(launch_derivative_scale(ctx->deriv.as<double>(), P, f.result, st));
+    }
+  }
+  DevResult r;
+  MMI_HIP(hipMemcpyAsync(&r, ctx->result.p, sizeof(DevResult), hipMemcpyDeviceToHost, st));
+  if (want_derivative && derivative)
+    MMI_HIP(hipMemcpyAsync(derivative, ctx->deriv.p, sizeof(double) * P, hipMemcpyDeviceToHost, st));
+  if (joint_pdf) MMI_HIP(hipMemcpyAsync(joint_pdf, ctx->pdf.p, sizeof(double) * B * B, hipMemcpyDeviceToHost, st));
+  if (fixed_marginal)
+    MMI_HIP(hipMemcpyAsync(fixed_marginal, ctx->fm.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+  MMI_HIP(hipStreamSynchronize(st));
+  float t1 = 0, t2 = 0;
+  MMI_HIP(hipEventElapsedTime(&t1, ctx->ev[0], ctx->ev[1]));
+  ctx->last_pass1_ms = t1;
+  ctx->last_pass2_ms = 0;
+  if (want_derivative) {
+    MMI_HIP(hipEventElapsedTime(&t2, ctx->ev[2], ctx->ev[3]));
+    ctx->last_pass2_ms = t2;
+  }
+  float tt = 0;
+  MMI_HIP(hipEventElapsedTime(&tt, ctx->ev[0], want_derivative ? ctx->ev[3] : ctx->ev[1]));
+  ctx->last_total_ms = tt;
+  if (n_valid) *n_valid = (int64_t)r.n_valid;
+  if (r.status == 1)
+    return fail(MMI_ERR_DEGENERATE,
+                "All samples map outside moving image buffer. The images do not sufficiently overlap. They need to "
+                "be initialized to have more overlap before this metric will work. For instance, you can align the "
+                "image centers by translation.");
+  if (r.status == 2) return fail(MMI_ERR_DEGENERATE, "Joint PDF summed to zero");
+  if (value) *value = r.value;
+  (void)D;
+  return MMI_OK;
+}
+
+int mmi_get_sample_bins(mmi_context* ctx, int16_t* fixed_bin, int16_t* moving_bin) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (!(ctx->flags & 1u)) return fail(MMI_ERR_ARGUMENT, "create the context with flag 1 to record sample bins");
+  if (!ctx->initialized) return fail(MMI_ERR_ARGUMENT, "not initialized");
+  const int64_t n = num_pixels(ctx->virtual_g);
+  MMI_HIP(hipSetDevice(ctx->device));
+  if (fixed_bin)
+    MMI_HIP(hipMemcpyAsync(fixed_bin, ctx->sample_fbin.p, sizeof(int16_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+  if (moving_bin)
+    MMI_HIP(hipMemcpyAsync(moving_bin, ctx->sample_mbin.p, sizeof(int16_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+  MMI_HIP(hipStreamSynchronize(ctx->stream));
+  return MMI_OK;
+}
+
+int mmi_get_histogram_parameters(mmi_context* ctx, double* fixed_min, double* fixed_max, double* moving_min,
+                                 double* moving_max, double* fixed_bin_size, double* moving_bin_size) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (!ctx->initialized) return fail(MMI_ERR_ARGUMENT, "not initialized");
+  if (fixed_min) *fixed_min = ctx->fixed_min;
+  if (fixed_max) *fixed_max = ctx->fixed_max;
+  if (moving_min) *moving_min = ctx->moving_min;
+  if (moving_max) *moving_max = ctx->moving_max;
+  if (fixed_bin_size) *fixed_bin_size = ctx->fixed_bin_size;
+  if (moving_bin_size) *moving_bin_size = ctx->moving_bin_size;
+  return MMI_OK;
+}
+
+int mmi_get_last_timing(mmi_context* ctx, float* total_ms, float* pass1_ms, float* pass2_ms) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (total_ms) *total_ms = ctx->last_total_ms;
+  if (pass1_ms) *pass1_ms = ctx->last_pass1_ms;
+  if (pass2_ms) *pass2_ms = ctx->last_pass2_ms;
+  return MMI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,319 @@
+// Device-side building blocks of the Mattes MI v4 hot path (gfx950).
+//
+// The coordinate path (index -> physical point -> transform -> continuous
+// index -> linear interpolation -> pixel rounding -> Parzen bin) runs in FP64
+// in exactly the operation order of the reference C++ (this file is compiled
+// with -ffp-contract=off), so Parzen bin indices and valid-sample counts are
+// bit-identical to ITK's.  Everything that only feeds the derivative (image
+// gradient interpolation, per-sample weights) runs in FP32/FP64 without that
+// constraint.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mmi {
+
+constexpr int kMaxDim = 3;
+
+// Geometry of an image or B-spline grid, with the ImageFunction bounds
+// precomputed (Modules/Core/ImageFunction/include/itkImageFunction.hxx:50-67).
+struct DevGeom {
+  int32_t dim;
+  int32_t pad_;
+  int64_t size[3];
+  int64_t start[3];
+  int64_t end[3];  // start + size - 1
+  double origin[3];
+  double spacing[3];
+  double i2p[9];
+  double p2i[9];
+  double cilo[3];  // start - 0.5
+  double cihi[3];  // end + 0.5
+  int64_t sy;      // size[0]
+  int64_t sz;      // size[0] * size[1]
+};
+
+struct DevTransform {
+  int32_t kind;  // 0 identity, 1 affine, 2 B-spline
+  int32_t pad_;
+  double m[9];
+  double off[3];
+  double center[3];
+  DevGeom grid;
+  const double* coeffs;  // B-spline parameters on the device, dimension-major
+  int64_t pdim;          // nodes per dimension
+};
+
+// Math::Floor<int64>(double) = RoundHalfIntegerToEven(2x - 0.5) >> 1
+// (Modules/Core/Common/include/itkMathDetail.h:393-396).  Not a plain floor
+// for tiny negative x, so it is reproduced literally.
+__device__ __forceinline__ int64_t itk_floor(double x) {
+  return ((int64_t)__builtin_rint(2.0 * x - 0.5)) >> 1;
+}
+
+// BSplineKernelFunction<3> (Modules/Core/Common/include/itkBSplineKernelFunction.h:153-175).
+__device__ __forceinline__ double bspline3(double u) {
+  const double a = fabs(u);
+  if (a < 1.0) {
+    const double s = a * a;
+    return (4.0 - 6.0 * s + 3.0 * s * a) / 6.0;
+  }
+  if (a < 2.0) {
+    const double s = a * a;
+    return (8.0 - 12.0 * a + 6.0 * s - s * a) / 6.0;
+  }
+  return 0.0;
+}
+
+// BSplineDerivativeKernelFunction<3> (itkBSplineDerivativeKernelFunction.h:157-180).
+__device__ __forceinline__ double bspline3_deriv(double u) {
+  if (u >= 0.0 && u < 1.0) return -2.0 * u + 1.5 * u * u;
+  if (u > -1.0 && u < 0.0) return -2.0 * u - 1.5 * u * u;
+  if (u >= 1.0 && u < 2.0) return -2.0 + 2.0 * u - 0.5 * u * u;
+  if (u > -2.0 && u <= -1.0) return 2.0 + 2.0 * u + 0.5 * u * u;
+  return 0.0;
+}
+
+// ImageBase::TransformIndexToPhysicalPoint (itkImageBase.h:594-604).
+template <int D>
+__device__ __forceinline__ void index_to_point(const DevGeom& g, const int64_t* idx, double* p) {
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    double s = g.origin[i];
+#pragma unroll
+    for (int j = 0; j < D; ++j) s += g.i2p[i * D + j] * (double)idx[j];
+    p[i] = s;
+  }
+}
+
+// ImageBase::TransformPhysicalPointToContinuousIndex (itkImageBase.h:517-531)
+// with Matrix * Vector summation order (itkMatrix.hxx:30-41).
+template <int D>
+__device__ __forceinline__ void point_to_cindex(const DevGeom& g, const double* p, double* ci) {
+  double v[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) v[k] = p[k] - g.origin[k];
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) s += g.p2i[r * D + c] * v[c];
+    ci[r] = s;
+  }
+}
+
+// ImageFunction::IsInsideBuffer(ContinuousIndex) (itkImageFunction.h:159-170).
+template <int D>
+__device__ __forceinline__ bool inside_buffer(const DevGeom& g, const double* ci) {
+  bool in = true;
+#pragma unroll
+  for (int j = 0; j < D; ++j) in = in && (ci[j] >= g.cilo[j] && ci[j] < g.cihi[j]);
+  return in;
+}
+
+// The interpolation cell of LinearInterpolateImageFunction::EvaluateOptimized
+// (itkLinearInterpolateImageFunction.h:157-471).  The reference skips an axis
+// when its distance is <= 0 or when the upper neighbour leaves the buffer;
+// duplicating the lower corner on such an axis makes the full multilinear
+// formula return the same bits (v + (v - v) * d == v), so one branch-free
+// formula reproduces every branch.
+struct Cell {
+  int64_t base;
+  int64_t ox, oy, oz;  // offsets of the upper corners (0 on inactive axes)
+  double d[3];
+};
+
+template <int D>
+__device__ __forceinline__ Cell make_cell(const DevGeom& g, const double* ci) {
+  Cell c;
+  int64_t b[3] = {0, 0, 0};
+  bool act[3] = {false, false, false};
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    int64_t f = itk_floor(ci[j]);
+    b[j] = f > g.start[j] ? f : g.start[j];
+    c.d[j] = ci[j] - (double)b[j];
+    act[j] = (c.d[j] > 0.0) && (b[j] + 1 <= g.end[j]);
+  }
+  if (D == 2) {
+    b[2] = 0;
+    c.d[2] = 0.0;
+    act[2] = false;
+  }
+  c.base = (b[0] - g.start[0]) + g.sy * (b[1] - g.start[1]) + (D == 3 ? g.sz * (b[2] - g.start[2]) : 0);
+  c.ox = act[0] ? 1 : 0;
+  c.oy = act[1] ? g.sy : 0;
+  c.oz = act[2] ? g.sz : 0;
+  return c;
+}
+
+__device__ __forceinline__ double lerp_exact(double a, double b, double t) { return a + (b - a) * t; }
+
+// Exact FP64 multilinear value of a float image (the reference's RealType is
+// double for float pixels).
+template <int D>
+__device__ __forceinline__ double interp_exact(const float* __restrict__ img, const Cell& c) {
+  const double v000 = (double)img[c.base];
+  const double v100 = (double)img[c.base + c.ox];
+  const double v010 = (double)img[c.base + c.oy];
+  const double v110 = (double)img[c.base + c.oy + c.ox];
+  const double vx00 = lerp_exact(v000, v100, c.d[0]);
+  const double vx10 = lerp_exact(v010, v110, c.d[0]);
+  const double vxx0 = lerp_exact(vx00, vx10, c.d[1]);
+  if (D == 2) return vxx0;
+  const double v001 = (double)img[c.base + c.oz];
+  const double v101 = (double)img[c.base + c.oz + c.ox];
+  const double v011 = (double)img[c.base + c.oz + c.oy];
+  const double v111 = (double)img[c.base + c.oz + c.oy + c.ox];
+  const double vx01 = lerp_exact(v001, v101, c.d[0]);
+  const double vx11 = lerp_exact(v011, v111, c.d[0]);
+  const double vxx1 = lerp_exact(vx01, vx11, c.d[1]);
+  return lerp_exact(vxx0, vxx1, c.d[2]);
+}
+
+// Gradient texels: (moving value, dI/dx, dI/dy, dI/dz) as float4.  Only the
+// gradient components are interpolated here (FP32), with the same cell.
+template <int D>
+__device__ __forceinline__ void interp_gradient(const float4* __restrict__ tex, const Cell& c, float* g) {
+  const float t0 = (float)c.d[0], t1 = (float)c.d[1], t2 = (float)c.d[2];
+  const float4 a000 = tex[c.base];
+  const float4 a100 = tex[c.base + c.ox];
+  const float4 a010 = tex[c.base + c.oy];
+  const float4 a110 = tex[c.base + c.oy + c.ox];
+  float r[3];
+  {
+    const float gx0 = a000.y + (a100.y - a000.y) * t0, gx1 = a010.y + (a110.y - a010.y) * t0;
+    const float gy0 = a000.z + (a100.z - a000.z) * t0, gy1 = a010.z + (a110.z - a010.z) * t0;
+    const float gz0 = a000.w + (a100.w - a000.w) * t0, gz1 = a010.w + (a110.w - a010.w) * t0;
+    r[0] = gx0 + (gx1 - gx0) * t1;
+    r[1] = gy0 + (gy1 - gy0) * t1;
+    r[2] = gz0 + (gz1 - gz0) * t1;
+  }
+  if (D == 3) {
+    const float4 a001 = tex[c.base + c.oz];
+    const float4 a101 = tex[c.base + c.oz + c.ox];
+    const float4 a011 = tex[c.base + c.oz + c.oy];
+    const float4 a111 = tex[c.base + c.oz + c.oy + c.ox];
+    const float gx0 = a001.y + (a101.y - a001.y) * t0, gx1 = a011.y + (a111.y - a011.y) * t0;
+    const float gy0 = a001.z + (a101.z - a001.z) * t0, gy1 = a011.z + (a111.z - a011.z) * t0;
+    const float gz0 = a001.w + (a101.w - a001.w) * t0, gz1 = a011.w + (a111.w - a011.w) * t0;
+    const float s0 = gx0 + (gx1 - gx0) * t1, s1 = gy0 + (gy1 - gy0) * t1, s2 = gz0 + (gz1 - gz0) * t1;
+    r[0] = r[0] + (s0 - r[0]) * t2;
+    r[1] = r[1] + (s1 - r[1]) * t2;
+    r[2] = r[2] + (s2 - r[2]) * t2;
+  }
+#pragma unroll
+  for (int k = 0; k < D; ++k) g[k] = r[k];
+}
+
+// Math::FloatAlmostEqual(x, y, 4 ulps, 0.1 eps) (itkMath.h:331-356).
+__device__ __forceinline__ bool float_almost_equal(double x, double y) {
+  if (fabs(x - y) <= 0.1 * 2.220446049250313e-16) return true;
+  if (signbit(x) != signbit(y)) return false;
+  auto asulp = [](int64_t i) -> int64_t { return i < 0 ? (int64_t)(0x8000000000000000ULL) - i : i; };
+  int64_t u = asulp(__double_as_longlong(x)) - asulp(__double_as_longlong(y));
+  if (u < 0) u = -u;
+  return u <= 4;
+}
+
+// B-spline transform support of one point: BSplineTransform::TransformPoint
+// (Modules/Core/Transform/include/itkBSplineTransform.hxx:476-590) with
+// BSplineInterpolationWeightFunction::Evaluate
+// (Modules/Core/Common/include/itkBSplineInterpolationWeightFunction.hxx:43-94).
+template <int D>
+struct BSplineSupport {
+  bool inside;
+  int64_t start[3];
+  double w1[3][4];
+};
+
+template <int D>
+__device__ __forceinline__ void bspline_support(const DevGeom& grid, const double* p, BSplineSupport<D>& s) {
+  double ci[3];
+  point_to_cindex<D>(grid, p, ci);
+  s.inside = true;
+  const double minLimit = 0.5 * (double)(3 - 1);
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    if (!s.inside) break;
+    const double maxLimit = (double)grid.size[j] - 0.5 * (double)(3 - 1) - 1.0;
+    if (float_almost_equal(ci[j], maxLimit)) {
+      ci[j] = __longlong_as_double(__double_as_longlong(maxLimit) - 6);  // FloatAddULP(maxLimit, -6)
+    } else if (ci[j] >= maxLimit) {
+      s.inside = false;
+    } else if (ci[j] < minLimit) {
+      s.inside = false;
+    }
+  }
+  if (!s.inside) return;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    s.start[j] = itk_floor(ci[j] + 0.5 - 3 / 2.0);
+    double x = ci[j] - (double)s.start[j];
+#pragma unroll
+    for (int k = 0; k <= 3; ++k) {
+      s.w1[j][k] = bspline3(x);
+      x -= 1.0;
+    }
+  }
+}
+
+// Weight and node index of support point k (x fastest), as the reference's
+// weights[k] = 1 * w0 * w1 (* w2) and coefficient scanline order.
+template <int D>
+__device__ __forceinline__ double bspline_weight(const BSplineSupport<D>& s, int k) {
+  double w = 1.0;
+  w *= s.w1[0][k & 3];
+  w *= s.w1[1][(k >> 2) & 3];
+  if (D == 3) w *= s.w1[2][(k >> 4) & 3];
+  return w;
+}
+
+template <int D>
+__device__ __forceinline__ int64_t bspline_node(const DevGeom& grid, const BSplineSupport<D>& s, int k) {
+  int64_t n = s.start[0] + (k & 3);
+  n += grid.size[0] * (s.start[1] + ((k >> 2) & 3));
+  if (D == 3) n += grid.size[0] * grid.size[1] * (s.start[2] + ((k >> 4) & 3));
+  return n;
+}
+
+// Transform::TransformPoint for identity, affine (MatrixOffsetTransformBase,
+// itkMatrixOffsetTransformBase.hxx:124-129) and cubic B-spline.
+template <int D>
+__device__ __forceinline__ void transform_point(const DevTransform& t, const double* in, double* out,
+                                                BSplineSupport<D>* sup) {
+  if (t.kind == 1) {
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      double s = 0.0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) s += t.m[r * D + c] * in[c];
+      out[r] = s + t.off[r];
+    }
+    return;
+  }
+  if (t.kind == 2) {
+    bspline_support<D>(t.grid, in, *sup);
+    if (!sup->inside) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) out[i] = in[i];
+      return;
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    constexpr int NW = D == 3 ? 64 : 16;
+    for (int k = 0; k < NW; ++k) {
+      const double w = bspline_weight<D>(*sup, k);
+      const int64_t node = bspline_node<D>(t.grid, *sup, k);
+#pragma unroll
+      for (int j = 0; j < D; ++j) acc[j] += w * t.coeffs[j * t.pdim + node];
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) out[j] = acc[j] + in[j];
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < D; ++i) out[i] = in[i];
+}
+
+}  // namespace mmi

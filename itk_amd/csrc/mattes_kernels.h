@@ -1,0 +1,85 @@
+// Kernel argument blocks and launch wrappers shared by the kernels and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mattes_device.h"
+
+namespace mmi {
+
+struct DevResult {
+  unsigned long long n_valid;
+  int32_t status;  // 0 ok, 1 no valid sample, 2 joint PDF summed to zero
+  int32_t pad_;
+  double value;
+  double joint_pdf_sum;
+  double n_factor;  // -1 / (moving bin size * NValid)
+};
+
+struct EvalArgs {
+  DevGeom virt;
+  DevGeom fixed;
+  DevGeom moving;
+  DevTransform ftf;
+  DevTransform mtf;
+  const float* fixed_px;
+  const float* moving_px;
+  const float4* texels;
+  int64_t n;  // virtual samples
+  int32_t bins;
+  int32_t hist_in_lds;
+  int32_t gradient_source;
+  int32_t pad_;
+  double moving_min, moving_max;
+  double moving_bin_size, moving_norm_min;
+  double fixed_bin_size, fixed_norm_min;
+  unsigned long long* hist;   // [B*B] fixed point 2^-32
+  unsigned long long* fmarg;  // [B]
+  unsigned long long* nvalid;
+  int16_t* sample_fbin;  // optional [n]
+  int16_t* sample_mbin;  // optional [n]
+  const double* pratio;  // [B*B]
+  double* partials;      // [grid * P] (affine)
+  double* deriv_accum;   // [P] (B-spline)
+};
+
+struct FinalizeArgs {
+  int32_t bins;
+  int32_t pad_;
+  const unsigned long long* hist;
+  const unsigned long long* fmarg;
+  const unsigned long long* nvalid;
+  double* pdf;     // [B*B] normalised joint PDF
+  double* fm;      // [B]
+  double* mm;      // [B]
+  double* pratio;  // [B*B]
+  double* rowsum;  // [B]
+  DevResult* result;
+  double moving_bin_size;
+};
+
+struct RecCoeffs {
+  double N0, N1, N2, N3, D1, D2, D3, D4, M1, M2, M3, M4, BN1, BN2, BN3, BN4, BM1, BM2, BM3, BM4;
+};
+
+struct Direction {
+  double m[9];
+};
+
+hipError_t launch_pass1(const EvalArgs& a, int grid, hipStream_t st);
+hipError_t launch_finalize(const FinalizeArgs& f, hipStream_t st);
+hipError_t launch_pass2(const EvalArgs& a, int grid, hipStream_t st);
+hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, const DevResult* r, double* deriv,
+                                    hipStream_t st);
+hipError_t launch_derivative_scale(double* deriv, int64_t P, const DevResult* r, hipStream_t st);
+hipError_t launch_minmax(const float* px, int64_t n, float* out, int blocks, hipStream_t st);
+hipError_t launch_recursive_gaussian(const double* in, double* out, int64_t ln, int64_t stride, int64_t nlines,
+                                     const RecCoeffs& c, hipStream_t st);
+hipError_t launch_float_to_double(const float* in, double* out, int64_t n, hipStream_t st);
+hipError_t launch_gradient_component(const double* vol, int64_t n, double spacing, int comp, double* grad, int D,
+                                     hipStream_t st);
+hipError_t launch_pack_texels(const float* moving, const double* grad, int64_t n, int D, const Direction& dir,
+                              float4* tex, hipStream_t st);
+
+}  // namespace mmi

@@ -1,0 +1,207 @@
+"""Parity of the HIP metric (through the C ABI) with the CPU oracle.
+
+Bar: Parzen bin indices and valid-sample counts bit-exact; value within 1e-5
+relative; derivative within 1e-4 relative (L2).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from _cases import REFERENCE_KNOWN_VALUES, affine_for, affine_params, blob_problem, bspline_for, reference_test_images
+from itk_amd import synth
+from itk_amd._native import MetricError
+from itk_amd.geometry import ImageGeometry
+from itk_amd.metric import MattesMutualInformationImageToImageMetricv4 as Metric
+from itk_amd.transforms import AffineTransform
+
+pytestmark = pytest.mark.gpu
+
+VALUE_RTOL = 1e-5
+DERIV_RTOL = 1e-4
+
+
+def _gpu_metric(fixed, moving, geom, tf, *, gradient=None, use_filter=True, virtual=None, bins=50, record=True):
+    m = Metric(record_sample_bins=record)
+    m.SetFixedImage(fixed, geom)
+    m.SetMovingImage(moving, geom)
+    m.SetUseMovingImageGradientFilter(use_filter)
+    if gradient is not None:
+        m.SetMovingImageGradient(gradient)
+    if virtual is not None:
+        m.SetVirtualDomain(virtual)
+    m.SetNumberOfHistogramBins(bins)
+    m.SetMovingTransform(tf)
+    m.Initialize()
+    return m
+
+
+def _compare(oracle_mod, m, fixed, moving, geom, tf, *, gradient=None, virtual=None, bins=50, check_bins=True):
+    v, d = m.GetValueAndDerivative()
+    r = oracle_mod.mattes(fixed, geom, moving, geom, tf, bins=bins, virtual_geom=virtual,
+                          moving_gradient=gradient, want_samples=check_bins, threads=8)
+    print("value gpu %.17g oracle %.17g  nvalid %d/%d" % (v, r["value"], m.GetNumberOfValidPoints(), r["n_valid"]))
+    assert m.GetNumberOfValidPoints() == r["n_valid"]
+    if check_bins:
+        fb, mb = m.GetSampleBins()
+        assert np.array_equal(fb, r["fixed_bin"])
+        assert np.array_equal(mb, r["moving_bin"])
+    assert abs(v - r["value"]) <= VALUE_RTOL * abs(r["value"])
+    err = np.linalg.norm(d - r["derivative"]) / np.linalg.norm(r["derivative"])
+    print("derivative rel L2 error %.3e" % err)
+    assert err <= DERIV_RTOL
+    return v, d, r
+
+
+def test_reference_unit_test_case(oracle_mod):
+    """The reference test's images and settings (central differences, 50 bins,
+    spacing (3,2), origin (0.001,-0.002)) on float32 storage."""
+    fixed, moving, geom = reference_test_images(np.float32)
+    tf = AffineTransform(2)
+    m = _gpu_metric(fixed, moving, geom, tf, use_filter=False)
+    for k, dd in [(4, -3.5), (5, 1e-5), (0, 0.02), (1, -0.03)]:
+        p = np.array([1.0, 0, 0, 1, 0, 0])
+        p[k] += dd
+        tf.set_parameters(p)
+        _compare(oracle_mod, m, fixed, moving, geom, tf)
+    p = np.array([1.0, 0, 0, 1, 0, 1e-5])
+    tf.set_parameters(p)
+    assert m.GetValue() == pytest.approx(REFERENCE_KNOWN_VALUES[0], rel=1e-6)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_affine_3d_gradient_image(oracle_mod, seed):
+    fixed, moving, geom = blob_problem((40, 36, 32), seed)
+    grad = oracle_mod.gradient_image(moving, geom)
+    tf = affine_for(geom, affine_params(3, np.random.default_rng(seed)))
+    m = _gpu_metric(fixed, moving, geom, tf, gradient=grad)
+    _compare(oracle_mod, m, fixed, moving, geom, tf, gradient=grad)
+    tf.set_parameters(affine_params(3, np.random.default_rng(100 + seed), scale=2.0))
+    _compare(oracle_mod, m, fixed, moving, geom, tf, gradient=grad)
+
+
+def test_affine_2d_config0(oracle_mod):
+    """2-D 256x256 blob registration problem (30 blobs), dense, 50 bins."""
+    p = synth.make_affine_problem((256, 256), 42)
+    grad = oracle_mod.gradient_image(p.moving, p.geometry)
+    tf = affine_for(p.geometry, affine_params(2, np.random.default_rng(5)))
+    m = _gpu_metric(p.fixed, p.moving, p.geometry, tf, gradient=grad)
+    _compare(oracle_mod, m, p.fixed, p.moving, p.geometry, tf, gradient=grad)
+
+
+def test_device_gradient_filter_matches_oracle(oracle_mod):
+    """GradientRecursiveGaussianImageFilter on the device vs the oracle's, seen
+    through the metric derivative (the value does not use the gradient)."""
+    fixed, moving, geom = blob_problem((36, 30, 28), 4)
+    geom = ImageGeometry(size=geom.size, spacing=(1.0, 1.25, 0.8), origin=(3.0, -2.0, 1.0))
+    grad = oracle_mod.gradient_image(moving, geom)
+    tf = affine_for(geom, affine_params(3, np.random.default_rng(8)))
+    m = _gpu_metric(fixed, moving, geom, tf)  # no gradient given: computed on the device
+    _compare(oracle_mod, m, fixed, moving, geom, tf, gradient=grad)
+
+
+def test_central_difference_3d(oracle_mod):
+    fixed, moving, geom = blob_problem((30, 28, 26), 6)
+    tf = affine_for(geom, affine_params(3, np.random.default_rng(9)))
+    m = _gpu_metric(fixed, moving, geom, tf, use_filter=False)
+    _compare(oracle_mod, m, fixed, moving, geom, tf)
+
+
+def test_anisotropic_geometry_and_virtual_domain(oracle_mod):
+    fixed, moving, _ = blob_problem((34, 30, 26), 10)
+    geom = ImageGeometry(size=(34, 30, 26), spacing=(0.9, 1.1, 1.7), origin=(-5.0, 2.5, 11.0))
+    virt = ImageGeometry(size=(20, 18, 12), start=(5, 4, 6), spacing=(1.3, 1.2, 2.1), origin=(-4.0, 3.0, 12.5))
+    grad = oracle_mod.gradient_image(moving, geom)
+    tf = affine_for(geom, affine_params(3, np.random.default_rng(11)))
+    m = _gpu_metric(fixed, moving, geom, tf, gradient=grad, virtual=virt)
+    _compare(oracle_mod, m, fixed, moving, geom, tf, gradient=grad, virtual=virt)
+
+
+@pytest.mark.parametrize("bins", [5, 32, 100])
+def test_histogram_sizes(oracle_mod, bins):
+    fixed, moving, geom = blob_problem((24, 22, 20), 12)
+    grad = oracle_mod.gradient_image(moving, geom)
+    tf = affine_for(geom, affine_params(3, np.random.default_rng(12)))
+    m = _gpu_metric(fixed, moving, geom, tf, gradient=grad, bins=bins)
+    _compare(oracle_mod, m, fixed, moving, geom, tf, gradient=grad, bins=bins)
+
+
+def test_bspline_3d(oracle_mod):
+    fixed, moving, geom = blob_problem((32, 30, 28), 13)
+    grad = oracle_mod.gradient_image(moving, geom)
+    tf = bspline_for(geom, (4, 4, 4), seed=3, sigma=0.8)
+    m = _gpu_metric(fixed, moving, geom, tf, gradient=grad)
+    _compare(oracle_mod, m, fixed, moving, geom, tf, gradient=grad)
+
+
+def test_bspline_2d(oracle_mod):
+    p = synth.make_affine_problem((96, 80), 14)
+    grad = oracle_mod.gradient_image(p.moving, p.geometry)
+    tf = bspline_for(p.geometry, (6, 5), seed=4, sigma=1.0)
+    m = _gpu_metric(p.fixed, p.moving, p.geometry, tf, gradient=grad)
+    _compare(oracle_mod, m, p.fixed, p.moving, p.geometry, tf, gradient=grad)
+
+
+def test_value_only_matches_value_and_derivative(oracle_mod):
+    fixed, moving, geom = blob_problem((24, 24, 24), 15)
+    tf = affine_for(geom, affine_params(3, np.random.default_rng(13)))
+    m = _gpu_metric(fixed, moving, geom, tf)
+    v1 = m.GetValue()
+    v2, _ = m.GetValueAndDerivative()
+    assert v1 == v2
+
+
+def test_repeatable_bitwise(oracle_mod):
+    """Integer histogram accumulation and ordered partial sums: two
+    evaluations of the affine metric agree to the last bit."""
+    fixed, moving, geom = blob_problem((40, 40, 40), 16)
+    tf = affine_for(geom, affine_params(3, np.random.default_rng(14)))
+    m = _gpu_metric(fixed, moving, geom, tf)
+    a = m.GetValueAndDerivative()
+    b = m.GetValueAndDerivative()
+    assert a[0] == b[0]
+    assert np.array_equal(a[1], b[1])
+
+
+def test_degenerate_inputs_raise():
+    fixed, moving, geom = blob_problem((16, 16, 16), 17)
+    tf = affine_for(geom)
+    m = Metric()
+    m.SetFixedImage(np.full_like(fixed, 2.0), geom)
+    m.SetMovingImage(moving, geom)
+    m.SetMovingTransform(tf)
+    with pytest.raises(MetricError, match="All fixed-image samples in the analysis region have the same intensity"):
+        m.Initialize()
+    m = Metric()
+    m.SetFixedImage(fixed, geom)
+    m.SetMovingImage(np.zeros_like(moving), geom)
+    m.SetMovingTransform(tf)
+    with pytest.raises(MetricError, match="All moving-image samples in the analysis region have the same intensity"):
+        m.Initialize()
+    m = _gpu_metric(fixed, moving, geom, tf)
+    p = tf.get_parameters()
+    p[9] = 1e6
+    tf.set_parameters(p)
+    with pytest.raises(MetricError, match="do not sufficiently overlap"):
+        m.GetValueAndDerivative()
+    assert m.GetNumberOfValidPoints() == 0
+
+
+def test_joint_pdf(oracle_mod):
+    fixed, moving, geom = blob_problem((28, 26, 24), 18)
+    tf = affine_for(geom, affine_params(3, np.random.default_rng(15)))
+    m = _gpu_metric(fixed, moving, geom, tf)
+    pdf = m.GetJointPDF()
+    r = oracle_mod.mattes(fixed, geom, moving, geom, tf, derivative=False)
+    assert pdf.sum() == pytest.approx(1.0, abs=1e-12)
+    assert np.abs(pdf - r["joint_pdf"]).max() <= 1e-9
+
+
+def test_full_size_config1_affine(oracle_mod):
+    """256^3 float32, 80 blobs, affine: bins, count, value and derivative
+    against the multithreaded oracle at full size."""
+    p = synth.make_affine_problem((256, 256, 256), 2024)
+    tf = affine_for(p.geometry, affine_params(3, np.random.default_rng(16)))
+    m = _gpu_metric(p.fixed, p.moving, p.geometry, tf, record=True)
+    grad = oracle_mod.gradient_image(p.moving, p.geometry, threads=16)
+    _compare(oracle_mod, m, p.fixed, p.moving, p.geometry, tf, gradient=grad)
