@@ -58,7 +58,11 @@ typedef struct mmi_context mmi_context;
 /* Last error message of the calling thread (never NULL). */
 const char* mmi_last_error(void);
 
-/* Create a context on a HIP device.  flags: reserved, pass 0. */
+/* Context flags. */
+#define MMI_FLAG_RECORD_SAMPLE_BINS 1u /* keep per-sample Parzen bins for mmi_get_sample_bins */
+#define MMI_FLAG_EXACT_PATH_ONLY 2u    /* evaluate every sample on the FP64 path (no certified FP32 fast path) */
+
+/* Create a context on a HIP device with the flags above (0 for defaults). */
 int mmi_context_create(int device, uint32_t flags, mmi_context** out);
 int mmi_context_destroy(mmi_context* ctx);
 

@@ -341,6 +341,143 @@ int upload_image(mmi_context* ctx, DevBuf& buf, const mmi_geometry* g, int32_t p
   return MMI_OK;
 }
 
+bool is_diagonal(const double* m, int D) {
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j)
+      if (i != j && m[i * D + j] != 0.0) return false;
+  return true;
+}
+
+double itk_floor_host(double x) { return (double)(((int64_t)std::nearbyint(2.0 * x - 0.5)) >> 1); }
+
+// Fixed side of the certified fast path: with an identity fixed transform and
+// axis-aligned virtual and fixed grids, the reference's continuous index of
+// every virtual sample along axis a depends on that axis' index only.  If, for
+// every index, it lands inside the buffer on a pixel centre (or on an axis the
+// interpolator skips), the interpolated fixed value is exactly one pixel, at
+// a constant index offset.  Checked here with the reference's expressions.
+bool fixed_direct_offsets(const mmi_geometry& v, const mmi_geometry& f, int64_t* offset) {
+  const int D = v.dim;
+  if (!is_diagonal(v.index_to_physical, D) || !is_diagonal(f.physical_to_index, D)) return false;
+  for (int a = 0; a < D; ++a) {
+    const double s = v.index_to_physical[a * D + a];
+    const double q = f.physical_to_index[a * D + a];
+    const int64_t fend = f.start[a] + f.size[a] - 1;
+    bool have = false;
+    for (int64_t i = v.start[a]; i < v.start[a] + v.size[a]; ++i) {
+      const double p = v.origin[a] + s * (double)i;
+      const double ci = q * (p - f.origin[a]);
+      if (!(ci >= (double)f.start[a] - 0.5 && ci < (double)fend + 0.5)) return false;
+      int64_t b = (int64_t)itk_floor_host(ci);
+      if (b < f.start[a]) b = f.start[a];
+      const double d = ci - (double)b;
+      const bool active = d > 0.0 && b + 1 <= fend;
+      if (active) return false;
+      if (!have) {
+        offset[a] = b - i;
+        have = true;
+      } else if (b - i != offset[a]) {
+        return false;
+      }
+    }
+  }
+  for (int a = D; a < 3; ++a) offset[a] = 0;
+  return true;
+}
+
+// Parameters and error bounds of the certified fast path (FastAffine in
+// mattes_kernels.h).  Returns false when the path does not apply.
+bool setup_fast(const mmi_context* ctx, FastAffine* F) {
+  std::memset(F, 0, sizeof(*F));
+  const int D = ctx->fixed_g.dim;
+  const int B = ctx->bins;
+  if (ctx->flags & 2u) return false;  // exact path requested
+  if (ctx->mtf.kind != MMI_TF_AFFINE || ctx->ftf.kind != MMI_TF_IDENTITY) return false;
+  if (ctx->gradient_source != MMI_GRADIENT_IMAGE) return false;
+  if ((size_t)B * B * 8 + (B + 1) * 4 > 64 * 1024) return false;
+  const mmi_geometry& v = ctx->virtual_g;
+  const mmi_geometry& m = ctx->moving_g;
+  const DevTransform& t = ctx->dmtf;
+  // ci = P2I_m (M (o_v + I2P_v idx) + off - o_m) = A idx + b, in long double
+  long double MI[9] = {0}, A[9] = {0}, bb[3] = {0}, w[3] = {0};
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j)
+      for (int k = 0; k < D; ++k) MI[i * D + j] += (long double)t.m[i * D + k] * v.index_to_physical[k * D + j];
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j)
+      for (int k = 0; k < D; ++k) A[i * D + j] += (long double)m.physical_to_index[i * D + k] * MI[k * D + j];
+  for (int i = 0; i < D; ++i) {
+    long double s = (long double)t.off[i] - m.origin[i];
+    for (int k = 0; k < D; ++k) s += (long double)t.m[i * D + k] * v.origin[k];
+    w[i] = s;
+  }
+  for (int i = 0; i < D; ++i)
+    for (int k = 0; k < D; ++k) bb[i] += (long double)m.physical_to_index[i * D + k] * w[k];
+  for (int i = 0; i < D * D; ++i) F->A[i] = (double)A[i];
+  for (int i = 0; i < D; ++i) F->b[i] = (double)bb[i];
+  // magnitude of every intermediate of the reference's chain over the domain
+  double S = 0.0;
+  for (int corner = 0; corner < (1 << D); ++corner) {
+    double idx[3], vp[3] = {0, 0, 0}, mp[3] = {0, 0, 0}, ci[3] = {0, 0, 0}, mag = 0.0;
+    for (int k = 0; k < D; ++k) idx[k] = (double)(v.start[k] + ((corner >> k) & 1 ? v.size[k] - 1 : 0));
+    for (int i = 0; i < D; ++i) {
+      double s = std::fabs(v.origin[i]);
+      vp[i] = v.origin[i];
+      for (int j = 0; j < D; ++j) {
+        vp[i] += v.index_to_physical[i * D + j] * idx[j];
+        s += std::fabs(v.index_to_physical[i * D + j] * idx[j]);
+      }
+      mag = std::max(mag, s);
+    }
+    for (int i = 0; i < D; ++i) {
+      double s = std::fabs(t.off[i]);
+      mp[i] = t.off[i];
+      for (int j = 0; j < D; ++j) {
+        mp[i] += t.m[i * D + j] * vp[j];
+        s += std::fabs(t.m[i * D + j] * vp[j]);
+      }
+      mag = std::max(mag, s);
+    }
+    for (int i = 0; i < D; ++i) {
+      double s = 0.0;
+      for (int j = 0; j < D; ++j) {
+        ci[i] += m.physical_to_index[i * D + j] * (mp[j] - m.origin[j]);
+        s += std::fabs(m.physical_to_index[i * D + j]) * (std::fabs(mp[j]) + std::fabs(m.origin[j]));
+      }
+      mag = std::max(mag, s);
+    }
+    double rowA = 0.0;
+    for (int i = 0; i < D * D; ++i) rowA = std::max(rowA, std::fabs(F->A[i]));
+    S = std::max(S, mag * (1.0 + rowA));
+  }
+  F->e_ci = 64.0 * S * std::ldexp(1.0, -52) + 1e-300;
+  if (F->e_ci > 1e-6) return false;
+  // intensity bounds
+  const double Mm = std::max(std::fabs(ctx->moving_min), std::fabs(ctx->moving_max));
+  const double Mf = std::max(std::fabs(ctx->fixed_min), std::fabs(ctx->fixed_max));
+  const double ev = Mm * std::ldexp(1.0, -19) + 12.0 * Mm * F->e_ci;
+  const double u = std::ldexp(1.0, -22);
+  const double bt_m = 2.0 * ((ev + Mm * u) / ctx->moving_bin_size + u * (Mm / ctx->moving_bin_size +
+                                                                          std::fabs(ctx->moving_norm_min) + B));
+  const double bt_f = 2.0 * (Mf * u / ctx->fixed_bin_size + u * (Mf / ctx->fixed_bin_size +
+                                                                std::fabs(ctx->fixed_norm_min) + B));
+  if (bt_m > 0.05 || bt_f > 0.05) return false;
+  F->ev = (float)ev;
+  F->bt_m = (float)bt_m;
+  F->bt_f = (float)bt_f;
+  F->inv_mbs = (float)(1.0 / ctx->moving_bin_size);
+  F->mnm = (float)ctx->moving_norm_min;
+  F->inv_fbs = (float)(1.0 / ctx->fixed_bin_size);
+  F->fnm = (float)ctx->fixed_norm_min;
+  F->mmin = (float)ctx->moving_min;
+  F->mmax = (float)ctx->moving_max;
+  int64_t off[3] = {0, 0, 0};
+  F->fixed_direct = fixed_direct_offsets(v, ctx->fixed_g, off) ? 1 : 0;
+  for (int k = 0; k < 3; ++k) F->fixed_offset[k] = off[k];
+  F->enabled = 1;
+  return true;
+}
+
 int device_minmax(mmi_context* ctx, const float* px, int64_t n, double* lo, double* hi) {
   const int blocks = (int)std::min<int64_t>(1024, (n + 255) / 256);
   MMI_HIP(ctx->scratch.ensure(sizeof(float) * 2 * blocks));
@@ -672,8 +809,9 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
   MMI_HIP(hipMemsetAsync(ctx->hist.p, 0, sizeof(unsigned long long) * B * B, st));
   MMI_HIP(hipMemsetAsync(ctx->fmarg.p, 0, sizeof(unsigned long long) * B, st));
   MMI_HIP(hipMemsetAsync(ctx->nvalid.p, 0, sizeof(unsigned long long), st));
+  const bool fast = setup_fast(ctx, &a.fast);
   MMI_HIP(hipEventRecord(ctx->ev[0], st));
-  MMI_HIP(launch_pass1(a, ctx->grid, st));
+  MMI_HIP(fast ? launch_pass1_fast(a, ctx->grid, st) : launch_pass1(a, ctx->grid, st));
   MMI_HIP(hipEventRecord(ctx->ev[1], st));
 
   FinalizeArgs f;
@@ -697,7 +835,7 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
     if (ctx->mtf.kind == MMI_TF_AFFINE) {
       MMI_HIP(ctx->partials.ensure(sizeof(double) * P * ctx->grid));
       a.partials = ctx->partials.as<double>();
-      MMI_HIP(launch_pass2(a, ctx->grid, st));
+      MMI_HIP(fast ? launch_pass2_fast(a, ctx->grid, st) : launch_pass2(a, ctx->grid, st));
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
       MMI_HIP(launch_derivative_reduce(a.partials, ctx->grid, (int)P, f.result, ctx->deriv.as<double>(), st));
     } else {

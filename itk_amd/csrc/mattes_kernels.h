@@ -17,7 +17,30 @@ struct DevResult {
   double n_factor;  // -1 / (moving bin size * NValid)
 };
 
+// Certified fast path for an affine moving transform.  The moving continuous
+// index is one FP64 affine map of the virtual index (|error| << e_ci against
+// the reference's chain); interpolation and Parzen terms run in FP32 with
+// precomputed error bounds, and any sample whose inside-test, range test or
+// bin index could differ from the reference's within those bounds is redone
+// on the exact FP64 path.
+struct FastAffine {
+  int32_t enabled;
+  int32_t fixed_direct;  // fixed value = fixed pixel at (virtual index + fixed_offset), bit-exact
+  double A[9];           // virtual index -> moving continuous index
+  double b[3];
+  double e_ci;           // bound on |ci_fast - ci_reference|
+  int64_t fixed_offset[3];
+  float inv_mbs, mnm;    // 1 / moving bin size, moving normalised minimum
+  float inv_fbs, fnm;
+  float mmin, mmax;      // moving TrueMin / TrueMax
+  float ev;              // bound on |v_fast - v_reference| for moving values
+  float bt_m;            // bound on |mterm_fast - mterm_reference|
+  float bt_f;            // bound on |fterm_fast - fterm_reference|
+  float pad_;
+};
+
 struct EvalArgs {
+  FastAffine fast;
   DevGeom virt;
   DevGeom fixed;
   DevGeom moving;
@@ -68,6 +91,8 @@ struct Direction {
 };
 
 hipError_t launch_pass1(const EvalArgs& a, int grid, hipStream_t st);
+hipError_t launch_pass1_fast(const EvalArgs& a, int grid, hipStream_t st);
+hipError_t launch_pass2_fast(const EvalArgs& a, int grid, hipStream_t st);
 hipError_t launch_finalize(const FinalizeArgs& f, hipStream_t st);
 hipError_t launch_pass2(const EvalArgs& a, int grid, hipStream_t st);
 hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, const DevResult* r, double* deriv,

@@ -159,43 +159,44 @@ __global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f) {
   __syncthreads();
   if (nvalid == 0 || jointPDFSum < 2.220446049250313e-16) return;
   const double nf = s_norm;
+  const double closeToZero = 2.220446049250313e-16;
   for (int i = t; i < BB; i += blockDim.x) f.pdf[i] = ((double)f.hist[i] / kFixedPointScale) * nf;
-  for (int i = t; i < B; i += blockDim.x) f.fm[i] = (double)f.fmarg[i] * nf;
+  for (int i = t; i < B; i += blockDim.x) {
+    const double fmv = (double)f.fmarg[i] * nf;
+    f.fm[i] = fmv;
+    f.rowsum[i] = fmv > closeToZero ? log(fmv) : 0.0;  // log of the fixed marginal, once per row
+  }
   __syncthreads();
+  // moving marginal: column sums in the reference's row order (Mattes:405-417)
   for (int m = t; m < B; m += blockDim.x) {
     double acc = 0.0;
     for (int r = 0; r < B; ++r) acc += f.pdf[r * B + m];
     f.mm[m] = acc;
   }
   __syncthreads();
-  const double closeToZero = 2.220446049250313e-16;
-  double rowsum_all = 0.0;
-  for (int r = t; r < B; r += blockDim.x) {
+  // entropy terms and pRatio over all (f, m) pairs in parallel (Mattes:419-476)
+  double part_sum = 0.0;
+  for (int i = t; i < BB; i += blockDim.x) {
+    const int r = i / B, m = i - (i / B) * B;
     const double fmv = f.fm[r];
-    double rs = 0.0;
-    if (fmv > closeToZero) {
-      const double logf = log(fmv);
-      for (int m = 0; m < B; ++m) {
-        const double mmv = f.mm[m];
-        const double jp = f.pdf[r * B + m];
-        double pr = 0.0;
-        if (mmv > closeToZero && jp > closeToZero) {
-          pr = log(jp / mmv);
-          rs += jp * (pr - logf);
-        }
-        f.pratio[r * B + m] = pr;
-      }
-    } else {
-      for (int m = 0; m < B; ++m) f.pratio[r * B + m] = 0.0;
+    const double mmv = f.mm[m];
+    const double jp = f.pdf[i];
+    double pr = 0.0;
+    if (fmv > closeToZero && mmv > closeToZero && jp > closeToZero) {
+      pr = log(jp / mmv);
+      part_sum += jp * (pr - f.rowsum[r]);
     }
-    f.rowsum[r] = rs;
+    f.pratio[i] = pr;
   }
-  (void)rowsum_all;
+  __shared__ double reds[256];
+  reds[t] = part_sum;
   __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (t < w) reds[t] += reds[t + w];
+    __syncthreads();
+  }
   if (t == 0) {
-    double sum = 0.0;
-    for (int r = 0; r < B; ++r) sum += f.rowsum[r];
-    f.result->value = -1.0 * sum;
+    f.result->value = -1.0 * reds[0];
     f.result->n_factor = -1.0 / (f.moving_bin_size * (double)nvalid);
   }
 }
@@ -328,14 +329,257 @@ __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Certified fast path (affine moving transform)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float bspline3f(float u) {
+  const float a = fabsf(u);
+  if (a < 1.0f) return fmaf(a * a, fmaf(0.5f, a, -1.0f), 2.0f / 3.0f);
+  if (a < 2.0f) {
+    const float t = 2.0f - a;
+    return t * t * t * (1.0f / 6.0f);
+  }
+  return 0.0f;
+}
+
+__device__ __forceinline__ float bspline3_derivf(float u) {
+  if (u >= 0.0f && u < 1.0f) return u * fmaf(1.5f, u, -2.0f);
+  if (u > -1.0f && u < 0.0f) return -u * fmaf(1.5f, u, 2.0f);
+  if (u >= 1.0f && u < 2.0f) return fmaf(u, fmaf(-0.5f, u, 2.0f), -2.0f);
+  if (u > -2.0f && u <= -1.0f) return fmaf(u, fmaf(0.5f, u, 2.0f), 2.0f);
+  return 0.0f;
+}
+
+template <int D>
+struct FastSample {
+  int fbin;
+  int mbin;
+  float arg0;  // (mbin - 1) - mterm
+  Cell cell;   // moving interpolation cell
+  int64_t idx[3];
+};
+
+// FP32 multilinear value with the reference's duplicated-corner convention.
+template <int D>
+__device__ __forceinline__ float interp_fast(const float* __restrict__ img, const Cell& c, const float* t) {
+  const float v000 = img[c.base], v100 = img[c.base + c.ox];
+  const float v010 = img[c.base + c.oy], v110 = img[c.base + c.oy + c.ox];
+  const float vx00 = fmaf(v100 - v000, t[0], v000);
+  const float vx10 = fmaf(v110 - v010, t[0], v010);
+  const float vxx0 = fmaf(vx10 - vx00, t[1], vx00);
+  if (D == 2) return vxx0;
+  const float v001 = img[c.base + c.oz], v101 = img[c.base + c.oz + c.ox];
+  const float v011 = img[c.base + c.oz + c.oy], v111 = img[c.base + c.oz + c.oy + c.ox];
+  const float vx01 = fmaf(v101 - v001, t[0], v001);
+  const float vx11 = fmaf(v111 - v011, t[0], v011);
+  const float vxx1 = fmaf(vx11 - vx01, t[1], vx01);
+  return fmaf(vxx1 - vxx0, t[2], vxx0);
+}
+
+// Bin of an FP32 Parzen term; *uncertain when the reference's FP64 term could
+// fall on the other side of an unclamped bin edge.
+__device__ __forceinline__ int certified_bin(float term, float bound, int B, bool* uncertain) {
+  if (term < 2.0f) {
+    *uncertain = *uncertain || (term >= 3.0f - bound);
+    return 2;
+  }
+  if (term >= (float)(B - 2)) {
+    *uncertain = *uncertain || (term < (float)(B - 3) + bound);
+    return B - 3;
+  }
+  const float n = rintf(term);
+  if (n >= 3.0f && n <= (float)(B - 3) && fabsf(term - n) <= bound) *uncertain = true;
+  return (int)term;
+}
+
+template <int D>
+__device__ __forceinline__ bool fast_sample(const EvalArgs& a, int64_t s, FastSample<D>& o) {
+  const FastAffine& F = a.fast;
+  const int64_t sx = a.virt.size[0];
+  int64_t lin[3];
+  lin[0] = s % sx;
+  if (D == 2) {
+    lin[1] = s / sx;
+    lin[2] = 0;
+  } else {
+    const int64_t sy = a.virt.size[1];
+    lin[1] = (s / sx) % sy;
+    lin[2] = s / (sx * sy);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) o.idx[k] = a.virt.start[k] + lin[k];
+  bool uncertain = false;
+
+  // fixed value
+  float fv = 0.0f;
+  if (F.fixed_direct) {
+    const int64_t fx = o.idx[0] + F.fixed_offset[0] - a.fixed.start[0];
+    const int64_t fy = o.idx[1] + F.fixed_offset[1] - a.fixed.start[1];
+    const int64_t fz = D == 3 ? o.idx[2] + F.fixed_offset[2] - a.fixed.start[2] : 0;
+    fv = a.fixed_px[fx + a.fixed.sy * fy + a.fixed.sz * fz];
+  } else {
+    uncertain = true;  // general fixed geometry: exact path
+  }
+
+  // moving continuous index, one FP64 affine map
+  double ci[3];
+  const double x = (double)o.idx[0], y = (double)o.idx[1], z = (double)o.idx[2];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    double c = F.b[k];
+    if (D == 3) c = fma(F.A[k * 3 + 2], z, c);
+    c = fma(F.A[k * D + 1], y, c);
+    ci[k] = fma(F.A[k * D + 0], x, c);
+  }
+  bool out = false;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    out = out || (ci[k] < a.moving.cilo[k] - F.e_ci) || (ci[k] >= a.moving.cihi[k] + F.e_ci);
+    uncertain = uncertain || (fabs(ci[k] - a.moving.cilo[k]) <= F.e_ci) || (fabs(ci[k] - a.moving.cihi[k]) <= F.e_ci);
+  }
+  if (out) return false;  // outside the moving buffer: not counted whatever the fixed side says
+
+  if (!uncertain) {
+    o.cell = make_cell<D>(a.moving, ci);
+    const float t[3] = {(float)o.cell.d[0], (float)o.cell.d[1], (float)o.cell.d[2]};
+    const float v = interp_fast<D>(a.moving_px, o.cell, t);
+    if (v < F.mmin - F.ev || v > F.mmax + F.ev) return false;
+    uncertain = (v < F.mmin + F.ev) || (v > F.mmax - F.ev);
+    const float mterm = fmaf(v, F.inv_mbs, -F.mnm);
+    o.mbin = certified_bin(mterm, F.bt_m, a.bins, &uncertain);
+    o.arg0 = (float)(o.mbin - 1) - mterm;
+    bool fu = false;
+    const float fterm = fmaf(fv, F.inv_fbs, -F.fnm);
+    o.fbin = certified_bin(fterm, F.bt_f, a.bins, &fu);
+    if (fu) {
+      // exact FP64 fixed term (ComputeSingleFixedImageParzenWindowIndex)
+      const double ft = (double)fv / a.fixed_bin_size - a.fixed_norm_min;
+      o.fbin = ft < 2.0 ? 2 : (ft >= (double)(a.bins - 2) ? a.bins - 3 : (int)ft);
+    }
+    if (!uncertain) return true;
+  }
+  // exact FP64 path of the reference for this sample
+  SampleOut<D> e;
+  if (!eval_sample<D>(a, s, e)) return false;
+  o.fbin = e.fbin;
+  o.mbin = e.mbin;
+  o.arg0 = (float)((double)(e.mbin - 1) - e.mterm);
+  o.cell = e.cell;
+  return true;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) pass1_fast_kernel(EvalArgs a) {
+  extern __shared__ unsigned long long lds_u64[];
+  const int B = a.bins;
+  const int BB = B * B;
+  unsigned long long* hist = a.hist_in_lds ? lds_u64 : a.hist;
+  unsigned int* fmarg = reinterpret_cast<unsigned int*>(lds_u64 + (a.hist_in_lds ? BB : 0));
+  unsigned int* nvalid = fmarg + B;
+  if (a.hist_in_lds) {
+    for (int i = threadIdx.x; i < BB; i += blockDim.x) hist[i] = 0ull;
+  }
+  for (int i = threadIdx.x; i < B + 1; i += blockDim.x) fmarg[i] = 0u;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < a.n; s += stride) {
+    FastSample<D> o;
+    const bool ok = fast_sample<D>(a, s, o);
+    if (a.sample_fbin) {
+      a.sample_fbin[s] = ok ? (int16_t)o.fbin : (int16_t)-1;
+      a.sample_mbin[s] = ok ? (int16_t)o.mbin : (int16_t)-1;
+    }
+    const unsigned long long vm = __ballot(ok);
+    if ((threadIdx.x & 63) == 0 && vm) atomicAdd(nvalid, (unsigned int)__popcll(vm));
+    if (!ok) continue;
+    atomicAdd(&fmarg[o.fbin], 1u);
+    float arg = o.arg0;
+    unsigned long long* row = hist + (int64_t)o.fbin * B + (o.mbin - 1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float w = bspline3f(arg);
+      const unsigned long long q = (unsigned long long)(unsigned int)rintf(w * 4294967296.0f);
+      if (q) atomicAdd(&row[k], q);
+      arg += 1.0f;
+    }
+  }
+  __syncthreads();
+  if (a.hist_in_lds) {
+    for (int i = threadIdx.x; i < BB; i += blockDim.x) {
+      const unsigned long long v = hist[i];
+      if (v) atomicAdd(&a.hist[i], v);
+    }
+  }
+  for (int i = threadIdx.x; i < B; i += blockDim.x) {
+    const unsigned int v = fmarg[i];
+    if (v) atomicAdd(&a.fmarg[i], (unsigned long long)v);
+  }
+  if (threadIdx.x == 0 && *nvalid) atomicAdd(a.nvalid, (unsigned long long)*nvalid);
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) pass2_fast_kernel(EvalArgs a) {
+  extern __shared__ float lds_f32[];
+  const int B = a.bins;
+  const int BB = B * B;
+  double* scratch = reinterpret_cast<double*>(lds_f32);
+  float* pr = lds_f32 + 16;
+  for (int i = threadIdx.x; i < BB; i += blockDim.x) pr[i] = (float)a.pratio[i];
+  __syncthreads();
+  constexpr int PA = D * D + D;
+  double acc[PA];
+#pragma unroll
+  for (int i = 0; i < PA; ++i) acc[i] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < a.n; s += stride) {
+    FastSample<D> o;
+    if (!fast_sample<D>(a, s, o)) continue;
+    float arg = o.arg0;
+    const float* prow = pr + o.fbin * B + (o.mbin - 1);
+    float w = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      w = fmaf(bspline3_derivf(arg), prow[k], w);
+      arg += 1.0f;
+    }
+    if (w == 0.0f) continue;
+    float g[3];
+    interp_gradient<D>(a.texels, o.cell, g);
+    double vp[3];
+    index_to_point<D>(a.virt, o.idx, vp);
+#pragma unroll
+    for (int b = 0; b < D; ++b) {
+      const double wg = (double)(w * g[b]);
+#pragma unroll
+      for (int d = 0; d < D; ++d) acc[b * D + d] = fma(wg, vp[d] - a.mtf.center[d], acc[b * D + d]);
+      acc[D * D + b] += wg;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < PA; ++i) {
+    const double r = block_reduce_sum<D>(acc[i], scratch);
+    if (threadIdx.x == 0) a.partials[(int64_t)blockIdx.x * PA + i] = r;
+  }
+}
+
 // Deterministic sum of per-workgroup partials, then the nFactor scaling.
-__global__ void derivative_reduce_kernel(const double* __restrict__ partials, int nblocks, int P,
-                                         const DevResult* __restrict__ result, double* __restrict__ deriv) {
-  const int mu = blockIdx.x * blockDim.x + threadIdx.x;
-  if (mu >= P) return;
+// One workgroup per parameter; strided partial sums then a fixed-shape tree,
+// so the result does not depend on scheduling.
+__global__ void __launch_bounds__(256) derivative_reduce_kernel(const double* __restrict__ partials, int nblocks,
+                                                                int P, const DevResult* __restrict__ result,
+                                                                double* __restrict__ deriv) {
+  __shared__ double red[256];
+  const int mu = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += partials[(int64_t)b * P + mu];
-  deriv[mu] = s * result->n_factor;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partials[(int64_t)b * P + mu];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) deriv[mu] = red[0] * result->n_factor;
 }
 
 __global__ void derivative_scale_kernel(double* __restrict__ deriv, int64_t P, const DevResult* __restrict__ result) {
@@ -499,6 +743,24 @@ hipError_t launch_pass1(const EvalArgs& a, int grid, hipStream_t st) {
   return hipGetLastError();
 }
 
+hipError_t launch_pass1_fast(const EvalArgs& a, int grid, hipStream_t st) {
+  const size_t lds = (a.hist_in_lds ? (size_t)a.bins * a.bins * 8 : 0) + (size_t)(a.bins + 1) * 4;
+  if (a.virt.dim == 2)
+    hipLaunchKernelGGL(pass1_fast_kernel<2>, dim3(grid), dim3(256), lds, st, a);
+  else
+    hipLaunchKernelGGL(pass1_fast_kernel<3>, dim3(grid), dim3(256), lds, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_pass2_fast(const EvalArgs& a, int grid, hipStream_t st) {
+  const size_t lds = ((size_t)a.bins * a.bins + 16) * 4;
+  if (a.virt.dim == 2)
+    hipLaunchKernelGGL(pass2_fast_kernel<2>, dim3(grid), dim3(256), lds, st, a);
+  else
+    hipLaunchKernelGGL(pass2_fast_kernel<3>, dim3(grid), dim3(256), lds, st, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_finalize(const FinalizeArgs& f, hipStream_t st) {
   hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, f);
   return hipGetLastError();
@@ -515,7 +777,7 @@ hipError_t launch_pass2(const EvalArgs& a, int grid, hipStream_t st) {
 
 hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, const DevResult* r, double* deriv,
                                     hipStream_t st) {
-  hipLaunchKernelGGL(derivative_reduce_kernel, dim3((P + 63) / 64), dim3(64), 0, st, partials, nblocks, P, r, deriv);
+  hipLaunchKernelGGL(derivative_reduce_kernel, dim3(P), dim3(256), 0, st, partials, nblocks, P, r, deriv);
   return hipGetLastError();
 }
 

@@ -39,10 +39,12 @@ def transform_fixed_parameters(tf) -> np.ndarray:
 class MattesMutualInformationImageToImageMetricv4:
     """Dense-sampling Mattes MI metric evaluated on one HIP device."""
 
-    def __init__(self, device: int = 0, record_sample_bins: bool = False):
+    def __init__(self, device: int = 0, record_sample_bins: bool = False, exact_path_only: bool = False):
         self._L = N.load()
         self._ctx = ctypes.c_void_p()
-        N.check(self._L.mmi_context_create(device, 1 if record_sample_bins else 0, ctypes.byref(self._ctx)))
+        flags = (N.MMI_FLAG_RECORD_SAMPLE_BINS if record_sample_bins else 0) | (
+            N.MMI_FLAG_EXACT_PATH_ONLY if exact_path_only else 0)
+        N.check(self._L.mmi_context_create(device, flags, ctypes.byref(self._ctx)))
         self._bins = 50
         self._fixed = self._moving = None
         self._virtual = None
