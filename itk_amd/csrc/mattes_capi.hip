@@ -65,8 +65,8 @@ DevGeom to_dev(const mmi_geometry& g) {
   std::memset(&d, 0, sizeof(d));
   d.dim = g.dim;
   for (int i = 0; i < 3; ++i) {
-    d.size[i] = i < g.dim ? g.size[i] : 1;
-    d.start[i] = i < g.dim ? g.start[i] : 0;
+    d.size[i] = (int32_t)(i < g.dim ? g.size[i] : 1);
+    d.start[i] = (int32_t)(i < g.dim ? g.start[i] : 0);
     d.end[i] = d.start[i] + d.size[i] - 1;
     d.origin[i] = i < g.dim ? g.origin[i] : 0.0;
     d.spacing[i] = i < g.dim ? g.spacing[i] : 1.0;
@@ -80,6 +80,8 @@ DevGeom to_dev(const mmi_geometry& g) {
   }
   d.sy = d.size[0];
   d.sz = d.size[0] * d.size[1];
+  d.div_x = make_fastdiv((uint32_t)d.size[0]);
+  d.div_y = make_fastdiv((uint32_t)d.size[1]);
   return d;
 }
 
@@ -92,8 +94,14 @@ int64_t num_pixels(const mmi_geometry& g) {
 int check_geometry(const mmi_geometry* g) {
   if (!g) return fail(MMI_ERR_ARGUMENT, "geometry is NULL");
   if (g->dim != 2 && g->dim != 3) return fail(MMI_ERR_ARGUMENT, "only 2-D and 3-D images are supported");
-  for (int i = 0; i < g->dim; ++i)
+  int64_t n = 1;
+  for (int i = 0; i < g->dim; ++i) {
     if (g->size[i] < 1) return fail(MMI_ERR_ARGUMENT, "image size must be positive");
+    if (g->start[i] < -(1ll << 29) || g->start[i] > (1ll << 29))
+      return fail(MMI_ERR_ARGUMENT, "image start index out of range");
+    n *= g->size[i];
+    if (n >= (1ll << 31)) return fail(MMI_ERR_ARGUMENT, "images must hold fewer than 2^31 pixels");
+  }
   return MMI_OK;
 }
 
@@ -253,7 +261,8 @@ struct mmi_context {
   DevTransform dftf{}, dmtf{};
   DevBuf coeffs;
 
-  DevBuf hist, fmarg, nvalid, pdf, fm, mm, pratio, rowsum, result, partials, deriv, sample_fbin, sample_mbin, scratch;
+  DevBuf hist, fmarg, nvalid, pdf, fm, mm, pratio, pratio_f, rowsum, result, partials, deriv, sample_fbin, sample_mbin,
+      scratch;
   int grid = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float last_total_ms = 0, last_pass1_ms = 0, last_pass2_ms = 0;
@@ -311,7 +320,7 @@ int build_transform(mmi_context* ctx, int D, HostTransform& h, DevTransform& d, 
     }
     if ((int64_t)h.params.size() != D * nodes) return fail(MMI_ERR_ARGUMENT, "B-spline needs D * prod(grid) parameters");
     d.grid = to_dev(g);
-    d.pdim = nodes;
+    d.pdim = (int32_t)nodes;
     MMI_HIP(ctx->coeffs.ensure(sizeof(double) * h.params.size()));
     MMI_HIP(hipMemcpyAsync(ctx->coeffs.p, h.params.data(), sizeof(double) * h.params.size(), hipMemcpyHostToDevice,
                            ctx->stream));
@@ -533,7 +542,7 @@ int mmi_context_destroy(mmi_context* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   DevBuf* bufs[] = {&ctx->fixed_px, &ctx->moving_px, &ctx->texels, &ctx->coeffs, &ctx->hist, &ctx->fmarg,
-                    &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->rowsum, &ctx->result,
+                    &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->result,
                     &ctx->partials, &ctx->deriv, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch};
   for (DevBuf* b : bufs) b->release();
   for (auto& ev : ctx->ev)
@@ -718,6 +727,7 @@ int mmi_initialize(mmi_context* ctx) {
   MMI_HIP(ctx->nvalid.ensure(sizeof(unsigned long long)));
   MMI_HIP(ctx->pdf.ensure(sizeof(double) * B * B));
   MMI_HIP(ctx->pratio.ensure(sizeof(double) * B * B));
+  MMI_HIP(ctx->pratio_f.ensure(sizeof(float) * B * B));
   MMI_HIP(ctx->fm.ensure(sizeof(double) * B));
   MMI_HIP(ctx->mm.ensure(sizeof(double) * B));
   MMI_HIP(ctx->rowsum.ensure(sizeof(double) * B));
@@ -803,6 +813,8 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
     a.sample_mbin = ctx->sample_mbin.as<int16_t>();
   }
   a.pratio = ctx->pratio.as<double>();
+  a.pratio_f = ctx->pratio_f.as<float>();
+  a.pratio_in_lds = (size_t)B * B * 4 + 64 <= 64 * 1024 ? 1 : 0;
   if (want_derivative && ctx->gradient_source == MMI_GRADIENT_IMAGE && !ctx->have_texels)
     return fail(MMI_ERR_ARGUMENT, "moving gradient image missing");
 
@@ -811,7 +823,7 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
   MMI_HIP(hipMemsetAsync(ctx->nvalid.p, 0, sizeof(unsigned long long), st));
   const bool fast = setup_fast(ctx, &a.fast);
   MMI_HIP(hipEventRecord(ctx->ev[0], st));
-  MMI_HIP(fast ? launch_pass1_fast(a, ctx->grid, st) : launch_pass1(a, ctx->grid, st));
+  MMI_HIP(launch_pass1(a, ctx->grid, fast, st));
   MMI_HIP(hipEventRecord(ctx->ev[1], st));
 
   FinalizeArgs f;
@@ -824,6 +836,7 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
   f.fm = ctx->fm.as<double>();
   f.mm = ctx->mm.as<double>();
   f.pratio = ctx->pratio.as<double>();
+  f.pratio_f = ctx->pratio_f.as<float>();
   f.rowsum = ctx->rowsum.as<double>();
   f.result = ctx->result.as<DevResult>();
   f.moving_bin_size = ctx->moving_bin_size;
@@ -835,13 +848,13 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
     if (ctx->mtf.kind == MMI_TF_AFFINE) {
       MMI_HIP(ctx->partials.ensure(sizeof(double) * P * ctx->grid));
       a.partials = ctx->partials.as<double>();
-      MMI_HIP(fast ? launch_pass2_fast(a, ctx->grid, st) : launch_pass2(a, ctx->grid, st));
+      MMI_HIP(launch_pass2(a, ctx->grid, fast, st));
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
       MMI_HIP(launch_derivative_reduce(a.partials, ctx->grid, (int)P, f.result, ctx->deriv.as<double>(), st));
     } else {
       MMI_HIP(hipMemsetAsync(ctx->deriv.p, 0, sizeof(double) * P, st));
       a.deriv_accum = ctx->deriv.as<double>();
-      MMI_HIP(launch_pass2(a, ctx->grid, st));
+      MMI_HIP(launch_pass2(a, ctx->grid, false, st));
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
       MMI_HIP(launch_derivative_scale(ctx->deriv.as<double>(), P, f.result, st));
     }

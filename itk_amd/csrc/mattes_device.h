@@ -6,7 +6,8 @@
 // with -ffp-contract=off), so Parzen bin indices and valid-sample counts are
 // bit-identical to ITK's.  Everything that only feeds the derivative (image
 // gradient interpolation, per-sample weights) runs in FP32/FP64 without that
-// constraint.
+// constraint.  Index arithmetic is 32-bit: images and virtual domains hold
+// fewer than 2^31 voxels (checked on the host).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,24 +15,45 @@
 
 namespace mmi {
 
-constexpr int kMaxDim = 3;
+// Unsigned division by a runtime constant: multiply-high and shifts
+// (Granlund & Montgomery 1994), valid for every 32-bit numerator.
+struct FastDiv {
+  uint32_t d, m, s1, s2;
+};
+
+__host__ __device__ inline FastDiv make_fastdiv(uint32_t d) {
+  uint32_t l = 0;
+  while (l < 32 && (1ull << l) < d) ++l;
+  FastDiv f;
+  f.d = d;
+  f.m = (uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+  f.s1 = l < 1 ? l : 1;
+  f.s2 = l > 0 ? l - 1 : 0;
+  return f;
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
+  const uint32_t t = __umulhi(f.m, n);
+  return (t + ((n - t) >> f.s1)) >> f.s2;
+}
 
 // Geometry of an image or B-spline grid, with the ImageFunction bounds
 // precomputed (Modules/Core/ImageFunction/include/itkImageFunction.hxx:50-67).
 struct DevGeom {
   int32_t dim;
-  int32_t pad_;
-  int64_t size[3];
-  int64_t start[3];
-  int64_t end[3];  // start + size - 1
+  int32_t size[3];
+  int32_t start[3];
+  int32_t end[3];  // start + size - 1
+  int32_t sy;      // size[0]
+  int32_t sz;      // size[0] * size[1]
   double origin[3];
   double spacing[3];
   double i2p[9];
   double p2i[9];
   double cilo[3];  // start - 0.5
   double cihi[3];  // end + 0.5
-  int64_t sy;      // size[0]
-  int64_t sz;      // size[0] * size[1]
+  FastDiv div_x;   // division by size[0]
+  FastDiv div_y;   // division by size[1]
 };
 
 struct DevTransform {
@@ -42,14 +64,31 @@ struct DevTransform {
   double center[3];
   DevGeom grid;
   const double* coeffs;  // B-spline parameters on the device, dimension-major
-  int64_t pdim;          // nodes per dimension
+  int32_t pdim;          // nodes per dimension
+  int32_t pad2_;
 };
+
+// Virtual index of linear sample s (x fastest).
+template <int D>
+__device__ __forceinline__ void sample_index(const DevGeom& v, uint32_t s, int32_t* idx) {
+  const uint32_t r = fdiv(s, v.div_x);
+  idx[0] = v.start[0] + (int32_t)(s - r * (uint32_t)v.size[0]);
+  if (D == 2) {
+    idx[1] = v.start[1] + (int32_t)r;
+    idx[2] = 0;
+  } else {
+    const uint32_t z = fdiv(r, v.div_y);
+    idx[1] = v.start[1] + (int32_t)(r - z * (uint32_t)v.size[1]);
+    idx[2] = v.start[2] + (int32_t)z;
+  }
+}
 
 // Math::Floor<int64>(double) = RoundHalfIntegerToEven(2x - 0.5) >> 1
 // (Modules/Core/Common/include/itkMathDetail.h:393-396).  Not a plain floor
-// for tiny negative x, so it is reproduced literally.
-__device__ __forceinline__ int64_t itk_floor(double x) {
-  return ((int64_t)__builtin_rint(2.0 * x - 0.5)) >> 1;
+// for tiny negative x, so it is reproduced literally.  Callers only pass
+// indices of points already inside a buffer or grid (|x| < 2^30).
+__device__ __forceinline__ int32_t itk_floor(double x) {
+  return ((int32_t)__builtin_rint(2.0 * x - 0.5)) >> 1;
 }
 
 // BSplineKernelFunction<3> (Modules/Core/Common/include/itkBSplineKernelFunction.h:153-175).
@@ -77,7 +116,7 @@ __device__ __forceinline__ double bspline3_deriv(double u) {
 
 // ImageBase::TransformIndexToPhysicalPoint (itkImageBase.h:594-604).
 template <int D>
-__device__ __forceinline__ void index_to_point(const DevGeom& g, const int64_t* idx, double* p) {
+__device__ __forceinline__ void index_to_point(const DevGeom& g, const int32_t* idx, double* p) {
 #pragma unroll
   for (int i = 0; i < D; ++i) {
     double s = g.origin[i];
@@ -117,34 +156,30 @@ __device__ __forceinline__ bool inside_buffer(const DevGeom& g, const double* ci
 // when its distance is <= 0 or when the upper neighbour leaves the buffer;
 // duplicating the lower corner on such an axis makes the full multilinear
 // formula return the same bits (v + (v - v) * d == v), so one branch-free
-// formula reproduces every branch.
+// formula reproduces every branch.  Only called for points inside the buffer.
 struct Cell {
-  int64_t base;
-  int64_t ox, oy, oz;  // offsets of the upper corners (0 on inactive axes)
+  int32_t base;
+  int32_t ox, oy, oz;  // offsets of the upper corners (0 on inactive axes)
   double d[3];
 };
 
 template <int D>
 __device__ __forceinline__ Cell make_cell(const DevGeom& g, const double* ci) {
   Cell c;
-  int64_t b[3] = {0, 0, 0};
+  int32_t b[3] = {0, 0, 0};
   bool act[3] = {false, false, false};
+  c.d[2] = 0.0;
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    int64_t f = itk_floor(ci[j]);
+    const int32_t f = itk_floor(ci[j]);
     b[j] = f > g.start[j] ? f : g.start[j];
     c.d[j] = ci[j] - (double)b[j];
     act[j] = (c.d[j] > 0.0) && (b[j] + 1 <= g.end[j]);
   }
-  if (D == 2) {
-    b[2] = 0;
-    c.d[2] = 0.0;
-    act[2] = false;
-  }
   c.base = (b[0] - g.start[0]) + g.sy * (b[1] - g.start[1]) + (D == 3 ? g.sz * (b[2] - g.start[2]) : 0);
   c.ox = act[0] ? 1 : 0;
   c.oy = act[1] ? g.sy : 0;
-  c.oz = act[2] ? g.sz : 0;
+  c.oz = (D == 3 && act[2]) ? g.sz : 0;
   return c;
 }
 
@@ -183,25 +218,25 @@ __device__ __forceinline__ void interp_gradient(const float4* __restrict__ tex, 
   const float4 a110 = tex[c.base + c.oy + c.ox];
   float r[3];
   {
-    const float gx0 = a000.y + (a100.y - a000.y) * t0, gx1 = a010.y + (a110.y - a010.y) * t0;
-    const float gy0 = a000.z + (a100.z - a000.z) * t0, gy1 = a010.z + (a110.z - a010.z) * t0;
-    const float gz0 = a000.w + (a100.w - a000.w) * t0, gz1 = a010.w + (a110.w - a010.w) * t0;
-    r[0] = gx0 + (gx1 - gx0) * t1;
-    r[1] = gy0 + (gy1 - gy0) * t1;
-    r[2] = gz0 + (gz1 - gz0) * t1;
+    const float gx0 = fmaf(a100.y - a000.y, t0, a000.y), gx1 = fmaf(a110.y - a010.y, t0, a010.y);
+    const float gy0 = fmaf(a100.z - a000.z, t0, a000.z), gy1 = fmaf(a110.z - a010.z, t0, a010.z);
+    const float gz0 = fmaf(a100.w - a000.w, t0, a000.w), gz1 = fmaf(a110.w - a010.w, t0, a010.w);
+    r[0] = fmaf(gx1 - gx0, t1, gx0);
+    r[1] = fmaf(gy1 - gy0, t1, gy0);
+    r[2] = fmaf(gz1 - gz0, t1, gz0);
   }
   if (D == 3) {
     const float4 a001 = tex[c.base + c.oz];
     const float4 a101 = tex[c.base + c.oz + c.ox];
     const float4 a011 = tex[c.base + c.oz + c.oy];
     const float4 a111 = tex[c.base + c.oz + c.oy + c.ox];
-    const float gx0 = a001.y + (a101.y - a001.y) * t0, gx1 = a011.y + (a111.y - a011.y) * t0;
-    const float gy0 = a001.z + (a101.z - a001.z) * t0, gy1 = a011.z + (a111.z - a011.z) * t0;
-    const float gz0 = a001.w + (a101.w - a001.w) * t0, gz1 = a011.w + (a111.w - a011.w) * t0;
-    const float s0 = gx0 + (gx1 - gx0) * t1, s1 = gy0 + (gy1 - gy0) * t1, s2 = gz0 + (gz1 - gz0) * t1;
-    r[0] = r[0] + (s0 - r[0]) * t2;
-    r[1] = r[1] + (s1 - r[1]) * t2;
-    r[2] = r[2] + (s2 - r[2]) * t2;
+    const float gx0 = fmaf(a101.y - a001.y, t0, a001.y), gx1 = fmaf(a111.y - a011.y, t0, a011.y);
+    const float gy0 = fmaf(a101.z - a001.z, t0, a001.z), gy1 = fmaf(a111.z - a011.z, t0, a011.z);
+    const float gz0 = fmaf(a101.w - a001.w, t0, a001.w), gz1 = fmaf(a111.w - a011.w, t0, a011.w);
+    const float s0 = fmaf(gx1 - gx0, t1, gx0), s1 = fmaf(gy1 - gy0, t1, gy0), s2 = fmaf(gz1 - gz0, t1, gz0);
+    r[0] = fmaf(s0 - r[0], t2, r[0]);
+    r[1] = fmaf(s1 - r[1], t2, r[1]);
+    r[2] = fmaf(s2 - r[2], t2, r[2]);
   }
 #pragma unroll
   for (int k = 0; k < D; ++k) g[k] = r[k];
@@ -224,7 +259,7 @@ __device__ __forceinline__ bool float_almost_equal(double x, double y) {
 template <int D>
 struct BSplineSupport {
   bool inside;
-  int64_t start[3];
+  int32_t start[3];
   double w1[3][4];
 };
 
@@ -271,8 +306,8 @@ __device__ __forceinline__ double bspline_weight(const BSplineSupport<D>& s, int
 }
 
 template <int D>
-__device__ __forceinline__ int64_t bspline_node(const DevGeom& grid, const BSplineSupport<D>& s, int k) {
-  int64_t n = s.start[0] + (k & 3);
+__device__ __forceinline__ int32_t bspline_node(const DevGeom& grid, const BSplineSupport<D>& s, int k) {
+  int32_t n = s.start[0] + (k & 3);
   n += grid.size[0] * (s.start[1] + ((k >> 2) & 3));
   if (D == 3) n += grid.size[0] * grid.size[1] * (s.start[2] + ((k >> 4) & 3));
   return n;
@@ -304,7 +339,7 @@ __device__ __forceinline__ void transform_point(const DevTransform& t, const dou
     constexpr int NW = D == 3 ? 64 : 16;
     for (int k = 0; k < NW; ++k) {
       const double w = bspline_weight<D>(*sup, k);
-      const int64_t node = bspline_node<D>(t.grid, *sup, k);
+      const int32_t node = bspline_node<D>(t.grid, *sup, k);
 #pragma unroll
       for (int j = 0; j < D; ++j) acc[j] += w * t.coeffs[j * t.pdim + node];
     }

@@ -29,7 +29,8 @@ struct FastAffine {
   double A[9];           // virtual index -> moving continuous index
   double b[3];
   double e_ci;           // bound on |ci_fast - ci_reference|
-  int64_t fixed_offset[3];
+  int32_t fixed_offset[3];
+  int32_t pad0_;
   float inv_mbs, mnm;    // 1 / moving bin size, moving normalised minimum
   float inv_fbs, fnm;
   float mmin, mmax;      // moving TrueMin / TrueMax
@@ -63,6 +64,9 @@ struct EvalArgs {
   int16_t* sample_fbin;  // optional [n]
   int16_t* sample_mbin;  // optional [n]
   const double* pratio;  // [B*B]
+  const float* pratio_f; // [B*B] FP32 copy for pass 2 when it does not fit in LDS
+  int32_t pratio_in_lds;
+  int32_t pad1_;
   double* partials;      // [grid * P] (affine)
   double* deriv_accum;   // [P] (B-spline)
 };
@@ -77,6 +81,7 @@ struct FinalizeArgs {
   double* fm;      // [B]
   double* mm;      // [B]
   double* pratio;  // [B*B]
+  float* pratio_f; // [B*B]
   double* rowsum;  // [B]
   DevResult* result;
   double moving_bin_size;
@@ -90,11 +95,9 @@ struct Direction {
   double m[9];
 };
 
-hipError_t launch_pass1(const EvalArgs& a, int grid, hipStream_t st);
-hipError_t launch_pass1_fast(const EvalArgs& a, int grid, hipStream_t st);
-hipError_t launch_pass2_fast(const EvalArgs& a, int grid, hipStream_t st);
+hipError_t launch_pass1(const EvalArgs& a, int grid, bool fast, hipStream_t st);
 hipError_t launch_finalize(const FinalizeArgs& f, hipStream_t st);
-hipError_t launch_pass2(const EvalArgs& a, int grid, hipStream_t st);
+hipError_t launch_pass2(const EvalArgs& a, int grid, bool fast, hipStream_t st);
 hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, const DevResult* r, double* deriv,
                                     hipStream_t st);
 hipError_t launch_derivative_scale(double* deriv, int64_t P, const DevResult* r, hipStream_t st);

@@ -1,0 +1,220 @@
+// Preparation kernels (once per level) and small reductions of the Mattes MI
+// v4 evaluation: intensity range, recursive-Gaussian gradient image, texel
+// packing, derivative reduction.  Compiled with -ffp-contract=off like the
+// evaluation kernels, so the gradient image rounds like the reference filter.
+#include "mattes_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace mmi {
+
+// Deterministic sum of per-workgroup partials, then the nFactor scaling.
+// One workgroup per parameter; strided partial sums then a fixed-shape tree,
+// so the result does not depend on scheduling.
+__global__ void __launch_bounds__(256) derivative_reduce_kernel(const double* __restrict__ partials, int nblocks,
+                                                                int P, const DevResult* __restrict__ result,
+                                                                double* __restrict__ deriv) {
+  __shared__ double red[256];
+  const int mu = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partials[(int64_t)b * P + mu];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) deriv[mu] = red[0] * result->n_factor;
+}
+
+__global__ void derivative_scale_kernel(double* __restrict__ deriv, int64_t P, const DevResult* __restrict__ result) {
+  const int64_t mu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (mu < P) deriv[mu] = deriv[mu] * result->n_factor;
+}
+
+// Intensity range over a whole buffer (MattesMI::Initialize, Mattes:97-229),
+// 2 doubles per workgroup, reduced on the host.
+__global__ void minmax_kernel(const float* __restrict__ px, int64_t n, float* __restrict__ out) {
+  __shared__ float smin[256], smax[256];
+  float lo = 3.402823466e38f, hi = -3.402823466e38f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = px[i];
+    lo = v < lo ? v : lo;
+    hi = v > hi ? v : hi;
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + w]);
+      smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = smin[0];
+    out[2 * blockIdx.x + 1] = smax[0];
+  }
+}
+
+// Recursive Gaussian along one axis of a double volume, one line per thread
+// (RecursiveSeparableImageFilter::FilterDataArray,
+// Modules/Filtering/ImageFilterBase/include/itkRecursiveSeparableImageFilter.hxx:62-143):
+// the causal pass is written to `out`, the anti-causal pass is added in a
+// second sweep; both keep their 4-tap histories in registers.
+__global__ void recursive_gaussian_kernel(const double* __restrict__ in, double* __restrict__ out, int64_t ln,
+                                          int64_t stride, int64_t nlines, RecCoeffs c) {
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nlines) return;
+  const int64_t lo = l % stride, hi = l / stride;
+  const double* x = in + lo + hi * stride * ln;
+  double* y = out + lo + hi * stride * ln;
+  auto X = [&](int64_t i) { return x[i * stride]; };
+  // causal
+  const double v1 = X(0);
+  double o0, o1, o2, o3;  // outs[i-1..i-4]
+  {
+    double d0 = X(0), d1 = X(1), d2 = X(2), d3 = X(3);
+    double q0 = v1 * c.N0 + v1 * c.N1 + v1 * c.N2 + v1 * c.N3;
+    double q1 = d1 * c.N0 + v1 * c.N1 + v1 * c.N2 + v1 * c.N3;
+    double q2 = d2 * c.N0 + d1 * c.N1 + v1 * c.N2 + v1 * c.N3;
+    double q3 = d3 * c.N0 + d2 * c.N1 + d1 * c.N2 + v1 * c.N3;
+    (void)d0;
+    q0 -= v1 * c.BN1 + v1 * c.BN2 + v1 * c.BN3 + v1 * c.BN4;
+    q1 -= q0 * c.D1 + v1 * c.BN2 + v1 * c.BN3 + v1 * c.BN4;
+    q2 -= q1 * c.D1 + q0 * c.D2 + v1 * c.BN3 + v1 * c.BN4;
+    q3 -= q2 * c.D1 + q1 * c.D2 + q0 * c.D3 + v1 * c.BN4;
+    y[0] = q0;
+    y[stride] = q1;
+    y[2 * stride] = q2;
+    y[3 * stride] = q3;
+    o0 = q3;
+    o1 = q2;
+    o2 = q1;
+    o3 = q0;
+  }
+  double xm1 = X(3), xm2 = X(2), xm3 = X(1);
+  for (int64_t i = 4; i < ln; ++i) {
+    const double xi = X(i);
+    double q = xi * c.N0 + xm1 * c.N1 + xm2 * c.N2 + xm3 * c.N3;
+    q -= o0 * c.D1 + o1 * c.D2 + o2 * c.D3 + o3 * c.D4;
+    y[i * stride] = q;
+    o3 = o2;
+    o2 = o1;
+    o1 = o0;
+    o0 = q;
+    xm3 = xm2;
+    xm2 = xm1;
+    xm1 = xi;
+  }
+  // anti-causal
+  const double v2 = X(ln - 1);
+  double s0, s1, s2, s3;  // scratch[i+1..i+4]
+  {
+    const double e1 = X(ln - 1), e2 = X(ln - 2), e3 = X(ln - 3);
+    double r1 = v2 * c.M1 + v2 * c.M2 + v2 * c.M3 + v2 * c.M4;
+    double r2 = e1 * c.M1 + v2 * c.M2 + v2 * c.M3 + v2 * c.M4;
+    double r3 = e2 * c.M1 + e1 * c.M2 + v2 * c.M3 + v2 * c.M4;
+    double r4 = e3 * c.M1 + e2 * c.M2 + e1 * c.M3 + v2 * c.M4;
+    r1 -= v2 * c.BM1 + v2 * c.BM2 + v2 * c.BM3 + v2 * c.BM4;
+    r2 -= r1 * c.D1 + v2 * c.BM2 + v2 * c.BM3 + v2 * c.BM4;
+    r3 -= r2 * c.D1 + r1 * c.D2 + v2 * c.BM3 + v2 * c.BM4;
+    r4 -= r3 * c.D1 + r2 * c.D2 + r1 * c.D3 + v2 * c.BM4;
+    y[(ln - 1) * stride] += r1;
+    y[(ln - 2) * stride] += r2;
+    y[(ln - 3) * stride] += r3;
+    y[(ln - 4) * stride] += r4;
+    s0 = r4;
+    s1 = r3;
+    s2 = r2;
+    s3 = r1;
+  }
+  double xp1 = X(ln - 3), xp2 = X(ln - 2), xp3 = X(ln - 1);  // data[i+1..i+3] for i = ln-4
+  for (int64_t i = ln - 4; i > 0; --i) {
+    const double xi = X(i);
+    // scratch[i-1] = data[i]*M1 + data[i+1]*M2 + data[i+2]*M3 + data[i+3]*M4 - (...)
+    double q = xi * c.M1 + xp1 * c.M2 + xp2 * c.M3 + xp3 * c.M4;
+    q -= s0 * c.D1 + s1 * c.D2 + s2 * c.D3 + s3 * c.D4;
+    y[(i - 1) * stride] += q;
+    s3 = s2;
+    s2 = s1;
+    s1 = s0;
+    s0 = q;
+    xp3 = xp2;
+    xp2 = xp1;
+    xp1 = xi;
+  }
+}
+
+__global__ void float_to_double_kernel(const float* __restrict__ in, double* __restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (double)in[i];
+}
+
+// gradient component `comp` = vol / spacing into the texel's slot comp+1.
+__global__ void gradient_component_kernel(const double* __restrict__ vol, int64_t n, double spacing, int comp,
+                                          double* __restrict__ grad, int D) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) grad[i * D + comp] = vol[i] / spacing;
+}
+
+// TransformLocalVectorToPhysicalVector (itkImageBase.h:636-653) + packing into
+// float4 texels (value, gx, gy, gz).
+__global__ void pack_texels_kernel(const float* __restrict__ moving, const double* __restrict__ grad, int64_t n,
+                                   int D, Direction dir, float4* __restrict__ tex) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double in[3] = {0, 0, 0}, o[3] = {0, 0, 0};
+  for (int j = 0; j < D; ++j) in[j] = grad[i * D + j];
+  for (int r = 0; r < D; ++r) {
+    double s = 0.0;
+    for (int c = 0; c < D; ++c) s += dir.m[r * D + c] * in[c];
+    o[r] = s;
+  }
+  tex[i] = make_float4(moving[i], (float)o[0], (float)o[1], (float)o[2]);
+}
+
+hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, const DevResult* r, double* deriv,
+                                    hipStream_t st) {
+  hipLaunchKernelGGL(derivative_reduce_kernel, dim3(P), dim3(256), 0, st, partials, nblocks, P, r, deriv);
+  return hipGetLastError();
+}
+
+hipError_t launch_derivative_scale(double* deriv, int64_t P, const DevResult* r, hipStream_t st) {
+  hipLaunchKernelGGL(derivative_scale_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, deriv, P, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_minmax(const float* px, int64_t n, float* out, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL(minmax_kernel, dim3(blocks), dim3(256), 0, st, px, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_recursive_gaussian(const double* in, double* out, int64_t ln, int64_t stride, int64_t nlines,
+                                     const RecCoeffs& c, hipStream_t st) {
+  hipLaunchKernelGGL(recursive_gaussian_kernel, dim3((unsigned)((nlines + 127) / 128)), dim3(128), 0, st, in, out,
+                     ln, stride, nlines, c);
+  return hipGetLastError();
+}
+
+hipError_t launch_float_to_double(const float* in, double* out, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(float_to_double_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_gradient_component(const double* vol, int64_t n, double spacing, int comp, double* grad, int D,
+                                     hipStream_t st) {
+  hipLaunchKernelGGL(gradient_component_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vol, n, spacing,
+                     comp, grad, D);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_texels(const float* moving, const double* grad, int64_t n, int D, const Direction& dir,
+                              float4* tex, hipStream_t st) {
+  hipLaunchKernelGGL(pack_texels_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, moving, grad, n, D, dir,
+                     tex);
+  return hipGetLastError();
+}
+
+}  // namespace mmi

@@ -195,7 +195,8 @@ def test_joint_pdf(oracle_mod):
     pdf = m.GetJointPDF()
     r = oracle_mod.mattes(fixed, geom, moving, geom, tf, derivative=False)
     assert pdf.sum() == pytest.approx(1.0, abs=1e-12)
-    assert np.abs(pdf - r["joint_pdf"]).max() <= 1e-9
+    # FP32 Parzen weights on the fast path: ~1e-7 relative per bin
+    assert np.abs(pdf - r["joint_pdf"]).max() <= 1e-6 * r["joint_pdf"].max()
 
 
 @pytest.mark.parametrize("shape,seed", [((64, 60, 56), 21), ((200, 180), 22)])
