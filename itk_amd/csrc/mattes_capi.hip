@@ -814,7 +814,7 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
   }
   a.pratio = ctx->pratio.as<double>();
   a.pratio_f = ctx->pratio_f.as<float>();
-  a.pratio_in_lds = (size_t)B * B * 4 + 64 <= 64 * 1024 ? 1 : 0;
+  a.pratio_in_lds = (size_t)B * B * 4 + 512 <= 64 * 1024 ? 1 : 0;
   if (want_derivative && ctx->gradient_source == MMI_GRADIENT_IMAGE && !ctx->have_texels)
     return fail(MMI_ERR_ARGUMENT, "moving gradient image missing");
 

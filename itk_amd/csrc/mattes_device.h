@@ -210,6 +210,33 @@ __device__ __forceinline__ double interp_exact(const float* __restrict__ img, co
 // Gradient texels: (moving value, dI/dx, dI/dy, dI/dz) as float4.  Only the
 // gradient components are interpolated here (FP32), with the same cell.
 template <int D>
+__device__ __forceinline__ float3 interp_gradient3(const float4* __restrict__ tex, const Cell& c) {
+  const float t0 = (float)c.d[0], t1 = (float)c.d[1], t2 = (float)c.d[2];
+  const float4 a000 = tex[c.base];
+  const float4 a100 = tex[c.base + c.ox];
+  const float4 a010 = tex[c.base + c.oy];
+  const float4 a110 = tex[c.base + c.oy + c.ox];
+  const float gx0 = fmaf(a100.y - a000.y, t0, a000.y), gx1 = fmaf(a110.y - a010.y, t0, a010.y);
+  const float gy0 = fmaf(a100.z - a000.z, t0, a000.z), gy1 = fmaf(a110.z - a010.z, t0, a010.z);
+  const float gz0 = fmaf(a100.w - a000.w, t0, a000.w), gz1 = fmaf(a110.w - a010.w, t0, a010.w);
+  float3 r = make_float3(fmaf(gx1 - gx0, t1, gx0), fmaf(gy1 - gy0, t1, gy0), fmaf(gz1 - gz0, t1, gz0));
+  if (D == 3) {
+    const float4 a001 = tex[c.base + c.oz];
+    const float4 a101 = tex[c.base + c.oz + c.ox];
+    const float4 a011 = tex[c.base + c.oz + c.oy];
+    const float4 a111 = tex[c.base + c.oz + c.oy + c.ox];
+    const float hx0 = fmaf(a101.y - a001.y, t0, a001.y), hx1 = fmaf(a111.y - a011.y, t0, a011.y);
+    const float hy0 = fmaf(a101.z - a001.z, t0, a001.z), hy1 = fmaf(a111.z - a011.z, t0, a011.z);
+    const float hz0 = fmaf(a101.w - a001.w, t0, a001.w), hz1 = fmaf(a111.w - a011.w, t0, a011.w);
+    const float s0 = fmaf(hx1 - hx0, t1, hx0), s1 = fmaf(hy1 - hy0, t1, hy0), s2 = fmaf(hz1 - hz0, t1, hz0);
+    r.x = fmaf(s0 - r.x, t2, r.x);
+    r.y = fmaf(s1 - r.y, t2, r.y);
+    r.z = fmaf(s2 - r.z, t2, r.z);
+  }
+  return r;
+}
+
+template <int D>
 __device__ __forceinline__ void interp_gradient(const float4* __restrict__ tex, const Cell& c, float* g) {
   const float t0 = (float)c.d[0], t1 = (float)c.d[1], t2 = (float)c.d[2];
   const float4 a000 = tex[c.base];
@@ -314,11 +341,13 @@ __device__ __forceinline__ int32_t bspline_node(const DevGeom& grid, const BSpli
 }
 
 // Transform::TransformPoint for identity, affine (MatrixOffsetTransformBase,
-// itkMatrixOffsetTransformBase.hxx:124-129) and cubic B-spline.
-template <int D>
+// itkMatrixOffsetTransformBase.hxx:124-129) and cubic B-spline.  KIND is the
+// transform kind when known at compile time, -1 to dispatch on t.kind.
+template <int D, int KIND = -1>
 __device__ __forceinline__ void transform_point(const DevTransform& t, const double* in, double* out,
                                                 BSplineSupport<D>* sup) {
-  if (t.kind == 1) {
+  const int kind = KIND >= 0 ? KIND : t.kind;
+  if (kind == 1) {
 #pragma unroll
     for (int r = 0; r < D; ++r) {
       double s = 0.0;
@@ -328,7 +357,7 @@ __device__ __forceinline__ void transform_point(const DevTransform& t, const dou
     }
     return;
   }
-  if (t.kind == 2) {
+  if (kind == 2) {
     bspline_support<D>(t.grid, in, *sup);
     if (!sup->inside) {
 #pragma unroll

@@ -45,20 +45,36 @@ struct SampleOut {
   BSplineSupport<D> sup;
 };
 
-// ProcessVirtualPoint + Mattes ProcessPoint up to the bins
-// (ThrBase:174-273, I2IM:289-373, MattesT:215-275), exact FP64.
+// Evaluation modes of the pass kernels.
+constexpr int kExactAffine = 0;   // FP64 reference path, affine moving transform
+constexpr int kFastAffine = 1;    // certified fast path, affine moving transform
+constexpr int kExactBSpline = 2;  // FP64 reference path, cubic B-spline moving transform
+
+// Fixed transform: identity or affine.
 template <int D>
+__device__ __forceinline__ void fixed_point(const DevTransform& t, const double* in, double* out) {
+  if (t.kind == 1) {
+    transform_point<D, 1>(t, in, out, nullptr);
+  } else {
+#pragma unroll
+    for (int i = 0; i < D; ++i) out[i] = in[i];
+  }
+}
+
+// ProcessVirtualPoint + Mattes ProcessPoint up to the bins
+// (ThrBase:174-273, I2IM:289-373, MattesT:215-275), exact FP64.  MK is the
+// moving transform kind (1 affine, 2 B-spline).
+template <int D, int MK>
 __device__ __forceinline__ bool eval_sample(const EvalArgs& a, const int32_t* idx, SampleOut<D>& o) {
   index_to_point<D>(a.virt, idx, o.vp);
   double fp[3], ci[3];
-  BSplineSupport<D> dummy;
-  transform_point<D>(a.ftf, o.vp, fp, &dummy);
+  fixed_point<D>(a.ftf, o.vp, fp);
   point_to_cindex<D>(a.fixed, fp, ci);
   if (!inside_buffer<D>(a.fixed, ci)) return false;
   const Cell fc = make_cell<D>(a.fixed, ci);
   const double fval = (double)(float)interp_exact<D>(a.fixed_px, fc);
 
-  transform_point<D>(a.mtf, o.vp, o.mp, &o.sup);
+  transform_point<D, MK>(a.mtf, o.vp, o.mp, &o.sup);
   point_to_cindex<D>(a.moving, o.mp, ci);
   if (!inside_buffer<D>(a.moving, ci)) return false;
   o.cell = make_cell<D>(a.moving, ci);
@@ -217,7 +233,7 @@ __device__ __forceinline__ bool fast_sample(const EvalArgs& a, uint32_t s, FastS
   }
   // exact FP64 path of the reference for this sample
   SampleOut<D> e;
-  if (!eval_sample<D>(a, o.idx, e)) return false;
+  if (!eval_sample<D, 1>(a, o.idx, e)) return false;
   o.fbin = e.fbin;
   o.mbin = e.mbin;
   o.arg0 = (float)((double)(e.mbin - 1) - e.mterm);
@@ -229,8 +245,10 @@ __device__ __forceinline__ bool fast_sample(const EvalArgs& a, uint32_t s, FastS
 // pass 1: joint histogram
 // ---------------------------------------------------------------------------
 
-template <int D, bool FAST>
+template <int D, int MODE>
 __global__ void __launch_bounds__(256) pass1_kernel(EvalArgs a) {
+  constexpr bool FAST = MODE == kFastAffine;
+  constexpr int MK = MODE == kExactBSpline ? 2 : 1;
   extern __shared__ unsigned long long lds_u64[];
   const int B = a.bins;
   const int BB = B * B;
@@ -249,7 +267,7 @@ __global__ void __launch_bounds__(256) pass1_kernel(EvalArgs a) {
     int fbin = 0, mbin = 0;
     float arg0 = 0.0f;
     bool ok;
-    if (FAST) {
+    if constexpr (FAST) {
       FastSample<D> o;
       ok = fast_sample<D>(a, s, o);
       fbin = o.fbin;
@@ -259,7 +277,7 @@ __global__ void __launch_bounds__(256) pass1_kernel(EvalArgs a) {
       int32_t idx[3];
       sample_index<D>(a.virt, s, idx);
       SampleOut<D> o;
-      ok = eval_sample<D>(a, idx, o);
+      ok = eval_sample<D, MK>(a, idx, o);
       fbin = o.fbin;
       mbin = o.mbin;
       arg0 = (float)((double)(o.mbin - 1) - o.mterm);
@@ -408,19 +426,25 @@ __device__ __forceinline__ void central_difference(const EvalArgs& a, const doub
   }
 }
 
-__device__ __forceinline__ double block_reduce_sum(double v, double* scratch) {
-  // wavefront (64 lanes) shuffle reduction, then across wavefronts in LDS
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+// Sum of N per-thread values over a 256-thread workgroup, written by thread 0
+// to out[0..N): wavefront shuffles, then the 4 wavefront results in LDS, in a
+// fixed order.  scratch holds 4 * N doubles.
+template <int N>
+__device__ __forceinline__ void block_reduce_array(double (&v)[N], double* scratch, double* out) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) scratch[wid] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0) {
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += scratch[w];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double x = v[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    if (lane == 0) scratch[wid * N + i] = x;
   }
   __syncthreads();
-  return r;
+  if (threadIdx.x < N) {
+    double r = 0.0;
+    for (int w = 0; w < 4; ++w) r += scratch[w * N + threadIdx.x];
+    out[threadIdx.x] = r;
+  }
 }
 
 // sum_k B3'(arg0 + k) * pRatio[f][m - 1 + k] (MattesT:329-378 arguments).
@@ -434,17 +458,23 @@ __device__ __forceinline__ float parzen_deriv_weight(float arg0, const float* pr
   return w;
 }
 
-template <int D, bool FAST>
+template <int D, int MODE>
 __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
+  constexpr bool FAST = MODE == kFastAffine;
+  constexpr int MK = MODE == kExactBSpline ? 2 : 1;
   extern __shared__ float lds_f32[];
   const int B = a.bins;
   const int BB = B * B;
-  double* scratch = reinterpret_cast<double*>(lds_f32);  // 8 doubles
-  const float* pr = a.pratio_f;
-  if (a.pratio_in_lds) {
-    float* t = lds_f32 + 16;
-    for (int i = threadIdx.x; i < BB; i += blockDim.x) t[i] = a.pratio_f[i];
-    pr = t;
+  double* scratch = reinterpret_cast<double*>(lds_f32);  // 4 wavefronts x 12 doubles (128 floats)
+  // pRatio table: always in LDS on the fast path (the host only selects it
+  // when B*B floats fit), else in LDS when it fits and through the caches
+  // otherwise.
+  float* lds_pr = lds_f32 + 128;
+  const float* pr = lds_pr;
+  if (FAST || a.pratio_in_lds) {
+    for (int i = threadIdx.x; i < BB; i += blockDim.x) lds_pr[i] = a.pratio_f[i];
+  } else {
+    pr = a.pratio_f;
   }
   __syncthreads();
 
@@ -460,20 +490,29 @@ __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
     double gd[3] = {0.0, 0.0, 0.0};
     double vp[3];
     float w;
-    if (FAST) {
+    if constexpr (FAST) {
       FastSample<D> o;
       if (!fast_sample<D>(a, s, o)) continue;
       w = parzen_deriv_weight(o.arg0, pr + o.fbin * B + (o.mbin - 1));
       if (w == 0.0f) continue;
-      interp_gradient<D>(a.texels, o.cell, g);
-      index_to_point<D>(a.virt, o.idx, vp);
+      const float3 g3 = interp_gradient3<D>(a.texels, o.cell);
+      gd[0] = (double)g3.x;
+      gd[1] = (double)g3.y;
+      gd[2] = (double)g3.z;
+      const double i0 = (double)o.idx[0], i1 = (double)o.idx[1], i2 = (double)o.idx[2];
 #pragma unroll
-      for (int k = 0; k < D; ++k) gd[k] = (double)g[k];
+      for (int k = 0; k < D; ++k) {
+        double p = a.virt.origin[k];
+        p += a.virt.i2p[k * D + 0] * i0;
+        p += a.virt.i2p[k * D + 1] * i1;
+        if (D == 3) p += a.virt.i2p[k * D + 2] * i2;
+        vp[k] = p;
+      }
     } else {
       int32_t idx[3];
       sample_index<D>(a.virt, s, idx);
       SampleOut<D> o;
-      if (!eval_sample<D>(a, idx, o)) continue;
+      if (!eval_sample<D, MK>(a, idx, o)) continue;
       w = parzen_deriv_weight((float)((double)(o.mbin - 1) - o.mterm), pr + o.fbin * B + (o.mbin - 1));
       if (w == 0.0f) continue;
       if (a.gradient_source == 1) {
@@ -485,7 +524,7 @@ __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
       }
 #pragma unroll
       for (int k = 0; k < D; ++k) vp[k] = o.vp[k];
-      if (a.mtf.kind == 2) {
+      if constexpr (MK == 2) {
         // B-spline Jacobian: J[d][node + d*pdim] = weight(node) (BSplTf:594-650)
         if (o.sup.inside) {
           constexpr int NW = D == 3 ? 64 : 16;
@@ -508,48 +547,45 @@ __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
       acc[D * D + b] += wg;
     }
   }
-  if (a.mtf.kind == 1) {
-#pragma unroll
-    for (int i = 0; i < PA; ++i) {
-      const double r = block_reduce_sum(acc[i], scratch);
-      if (threadIdx.x == 0) a.partials[(int64_t)blockIdx.x * PA + i] = r;
-    }
-  }
+  if constexpr (MK == 1) block_reduce_array<PA>(acc, scratch, a.partials + (int64_t)blockIdx.x * PA);
 }
 
 // ---------------------------------------------------------------------------
 // Launch wrappers
 // ---------------------------------------------------------------------------
 
+
+
+
+#define MMI_DISPATCH(KERNEL, a, grid, lds, st, fast)                                                          \
+  do {                                                                                                        \
+    const int mode_ = (a).mtf.kind == 2 ? kExactBSpline : ((fast) ? kFastAffine : kExactAffine);             \
+    if ((a).virt.dim == 2) {                                                                                  \
+      if (mode_ == kFastAffine)                                                                               \
+        hipLaunchKernelGGL((KERNEL<2, kFastAffine>), dim3(grid), dim3(256), lds, st, a);                      \
+      else if (mode_ == kExactBSpline)                                                                        \
+        hipLaunchKernelGGL((KERNEL<2, kExactBSpline>), dim3(grid), dim3(256), lds, st, a);                    \
+      else                                                                                                    \
+        hipLaunchKernelGGL((KERNEL<2, kExactAffine>), dim3(grid), dim3(256), lds, st, a);                     \
+    } else {                                                                                                  \
+      if (mode_ == kFastAffine)                                                                               \
+        hipLaunchKernelGGL((KERNEL<3, kFastAffine>), dim3(grid), dim3(256), lds, st, a);                      \
+      else if (mode_ == kExactBSpline)                                                                        \
+        hipLaunchKernelGGL((KERNEL<3, kExactBSpline>), dim3(grid), dim3(256), lds, st, a);                    \
+      else                                                                                                    \
+        hipLaunchKernelGGL((KERNEL<3, kExactAffine>), dim3(grid), dim3(256), lds, st, a);                     \
+    }                                                                                                         \
+  } while (0)
+
 hipError_t launch_pass1(const EvalArgs& a, int grid, bool fast, hipStream_t st) {
   const size_t lds = (a.hist_in_lds ? (size_t)a.bins * a.bins * 8 : 0) + (size_t)(a.bins + 1) * 4;
-  if (a.virt.dim == 2) {
-    if (fast)
-      hipLaunchKernelGGL((pass1_kernel<2, true>), dim3(grid), dim3(256), lds, st, a);
-    else
-      hipLaunchKernelGGL((pass1_kernel<2, false>), dim3(grid), dim3(256), lds, st, a);
-  } else {
-    if (fast)
-      hipLaunchKernelGGL((pass1_kernel<3, true>), dim3(grid), dim3(256), lds, st, a);
-    else
-      hipLaunchKernelGGL((pass1_kernel<3, false>), dim3(grid), dim3(256), lds, st, a);
-  }
+  MMI_DISPATCH(pass1_kernel, a, grid, lds, st, fast);
   return hipGetLastError();
 }
 
 hipError_t launch_pass2(const EvalArgs& a, int grid, bool fast, hipStream_t st) {
-  const size_t lds = ((a.pratio_in_lds ? (size_t)a.bins * a.bins : 0) + 16) * 4;
-  if (a.virt.dim == 2) {
-    if (fast)
-      hipLaunchKernelGGL((pass2_kernel<2, true>), dim3(grid), dim3(256), lds, st, a);
-    else
-      hipLaunchKernelGGL((pass2_kernel<2, false>), dim3(grid), dim3(256), lds, st, a);
-  } else {
-    if (fast)
-      hipLaunchKernelGGL((pass2_kernel<3, true>), dim3(grid), dim3(256), lds, st, a);
-    else
-      hipLaunchKernelGGL((pass2_kernel<3, false>), dim3(grid), dim3(256), lds, st, a);
-  }
+  const size_t lds = ((a.pratio_in_lds ? (size_t)a.bins * a.bins : 0) + 128) * 4;
+  MMI_DISPATCH(pass2_kernel, a, grid, lds, st, fast);
   return hipGetLastError();
 }
 
