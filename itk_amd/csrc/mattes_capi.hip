@@ -733,7 +733,9 @@ int mmi_initialize(mmi_context* ctx) {
   MMI_HIP(ctx->rowsum.ensure(sizeof(double) * B));
   MMI_HIP(ctx->result.ensure(sizeof(DevResult)));
   const int64_t n = num_pixels(ctx->virtual_g);
-  ctx->grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->cu_count * 4));
+  // a multiple of 8: the fast kernels split the samples into one share per XCD
+  const int64_t want = std::min<int64_t>((n + 255) / 256, (int64_t)ctx->cu_count * 4);
+  ctx->grid = (int)std::max<int64_t>(8, (want + 7) / 8 * 8);
   if (ctx->flags & 1u) {
     MMI_HIP(ctx->sample_fbin.ensure(sizeof(int16_t) * n));
     MMI_HIP(ctx->sample_mbin.ensure(sizeof(int16_t) * n));

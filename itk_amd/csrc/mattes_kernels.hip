@@ -131,6 +131,38 @@ __device__ __forceinline__ void parzen_fixed(float arg0, unsigned int* q) {
   if (arg0 <= -1.0f && arg0 > -2.0f) q[1] = 0u - q[0] - q[2] - q[3];
 }
 
+// sum_k B3'(arg0 + k) * pRatio[f][m - 1 + k] (MattesT:329-378 arguments).
+__device__ __forceinline__ float parzen_deriv_weight(float arg0, const float* prow) {
+  float w = 0.0f, arg = arg0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    w = fmaf(bspline3_derivf(arg), prow[k], w);
+    arg += 1.0f;
+  }
+  return w;
+}
+
+// Sum of N per-thread values over a 256-thread workgroup, written by thread 0
+// to out[0..N): wavefront shuffles, then the 4 wavefront results in LDS, in a
+// fixed order.  scratch holds 4 * N doubles.
+template <int N>
+__device__ __forceinline__ void block_reduce_array(double (&v)[N], double* scratch, double* out) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double x = v[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    if (lane == 0) scratch[wid * N + i] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    double r = 0.0;
+    for (int w = 0; w < 4; ++w) r += scratch[w * N + threadIdx.x];
+    out[threadIdx.x] = r;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Certified fast path (affine moving transform)
 // ---------------------------------------------------------------------------
@@ -239,6 +271,222 @@ __device__ __forceinline__ bool fast_sample(const EvalArgs& a, uint32_t s, FastS
   o.arg0 = (float)((double)(e.mbin - 1) - e.mterm);
   o.cell = e.cell;
   return true;
+}
+
+// ---------------------------------------------------------------------------
+// Fast-path kernels: 4 consecutive samples per thread, XCD-local ranges
+// ---------------------------------------------------------------------------
+//
+// Each thread evaluates a run of 4 consecutive virtual samples (x fastest).
+// The continuous index is advanced by one FP64 column add per step (re-derived
+// from the index at every row wrap), consecutive samples with the same
+// (fixed bin, moving bin) are merged in registers before the LDS atomics, and
+// the valid count stays in a register until the end of the kernel.  Workgroup
+// b runs on XCD b % 8 (round-robin dispatch); the sample range is cut into 8
+// contiguous chunks so each XCD's L2 sees one slab of the moving texels.
+
+constexpr int kRun = 4;
+constexpr int kXcd = 8;
+
+// Fast evaluation of one sample given its index and FP64 continuous index.
+template <int D>
+__device__ __forceinline__ bool fast_eval(const EvalArgs& a, const int32_t* idx, const double* ci, int& fbin,
+                                          int& mbin, float& arg0, Cell& cell) {
+  const FastAffine& F = a.fast;
+  bool uncertain = !F.fixed_direct;
+  bool out = false;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    out = out || (ci[k] < a.moving.cilo[k] - F.e_ci) || (ci[k] >= a.moving.cihi[k] + F.e_ci);
+    uncertain = uncertain || (fabs(ci[k] - a.moving.cilo[k]) <= F.e_ci) || (fabs(ci[k] - a.moving.cihi[k]) <= F.e_ci);
+  }
+  if (out) return false;
+  if (!uncertain) {
+    const int32_t fx = idx[0] + F.fixed_offset[0] - a.fixed.start[0];
+    const int32_t fy = idx[1] + F.fixed_offset[1] - a.fixed.start[1];
+    const int32_t fz = D == 3 ? idx[2] + F.fixed_offset[2] - a.fixed.start[2] : 0;
+    const float fv = a.fixed_px[fx + a.fixed.sy * fy + a.fixed.sz * fz];
+    cell = make_cell<D>(a.moving, ci);
+    const float t[3] = {(float)cell.d[0], (float)cell.d[1], (float)cell.d[2]};
+    const float v = interp_fast<D>(a.moving_px, cell, t);
+    if (v < F.mmin - F.ev || v > F.mmax + F.ev) return false;
+    uncertain = (v < F.mmin + F.ev) || (v > F.mmax - F.ev);
+    const float mterm = fmaf(v, F.inv_mbs, -F.mnm);
+    mbin = certified_bin(mterm, F.bt_m, a.bins, &uncertain);
+    arg0 = (float)(mbin - 1) - mterm;
+    bool fu = false;
+    const float fterm = fmaf(fv, F.inv_fbs, -F.fnm);
+    fbin = certified_bin(fterm, F.bt_f, a.bins, &fu);
+    if (fu) {
+      const double ft = (double)fv / a.fixed_bin_size - a.fixed_norm_min;
+      fbin = ft < 2.0 ? 2 : (ft >= (double)(a.bins - 2) ? a.bins - 3 : (int)ft);
+    }
+    if (!uncertain) return true;
+  }
+  SampleOut<D> e;
+  if (!eval_sample<D, 1>(a, idx, e)) return false;
+  fbin = e.fbin;
+  mbin = e.mbin;
+  arg0 = (float)((double)(e.mbin - 1) - e.mterm);
+  cell = e.cell;
+  return true;
+}
+
+// Iterates the runs of this thread: calls f(idx, ci) for every sample.
+template <int D, typename F>
+__device__ __forceinline__ void for_each_run_sample(const EvalArgs& a, F&& f) {
+  const FastAffine& FA = a.fast;
+  const uint32_t n = (uint32_t)a.n;
+  const uint32_t nruns = (n + kRun - 1) / kRun;
+  const uint32_t xcd = blockIdx.x % kXcd;
+  const uint32_t lb = blockIdx.x / kXcd;
+  const uint32_t bpx = gridDim.x / kXcd;
+  const uint32_t chunk = (nruns + kXcd - 1) / kXcd;
+  const uint32_t r0 = xcd * chunk;
+  const uint32_t r1 = min(nruns, r0 + chunk);
+  for (uint32_t r = r0 + lb * blockDim.x + threadIdx.x; r < r1; r += bpx * blockDim.x) {
+    uint32_t s = r * kRun;
+    int32_t idx[3];
+    sample_index<D>(a.virt, s, idx);
+    double ci[3];
+    const double x = (double)idx[0], y = (double)idx[1], z = (double)idx[2];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      double c = FA.b[k];
+      if (D == 3) c = fma(FA.A[k * 3 + 2], z, c);
+      c = fma(FA.A[k * D + 1], y, c);
+      ci[k] = fma(FA.A[k * D + 0], x, c);
+    }
+#pragma unroll
+    for (int j = 0; j < kRun; ++j) {
+      if (s >= n) break;
+      f(s, idx, ci);
+      ++s;
+      if (++idx[0] > a.virt.end[0]) {
+        sample_index<D>(a.virt, s, idx);
+        const double x2 = (double)idx[0], y2 = (double)idx[1], z2 = (double)idx[2];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          double c = FA.b[k];
+          if (D == 3) c = fma(FA.A[k * 3 + 2], z2, c);
+          c = fma(FA.A[k * D + 1], y2, c);
+          ci[k] = fma(FA.A[k * D + 0], x2, c);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < D; ++k) ci[k] += FA.A[k * D + 0];
+      }
+    }
+  }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) pass1_fast_kernel(EvalArgs a) {
+  extern __shared__ unsigned long long lds_u64[];
+  const int B = a.bins;
+  const int BB = B * B;
+  unsigned long long* hist = lds_u64;
+  unsigned int* fmarg = reinterpret_cast<unsigned int*>(lds_u64 + BB);
+  unsigned int* nvalid_lds = fmarg + B;
+  for (int i = threadIdx.x; i < BB; i += blockDim.x) hist[i] = 0ull;
+  for (int i = threadIdx.x; i < B + 1; i += blockDim.x) fmarg[i] = 0u;
+  __syncthreads();
+  unsigned int nvalid = 0;
+  int cur = -1;  // key of the pending run: fbin * B + mbin
+  unsigned long long acc[4] = {0, 0, 0, 0};
+  unsigned int cnt = 0;
+  auto flush = [&]() {
+    if (cur < 0) return;
+    const int fb = cur / B, mb = cur - fb * B;
+    atomicAdd(&fmarg[fb], cnt);
+    unsigned long long* row = hist + fb * B + (mb - 1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (acc[k]) atomicAdd(&row[k], acc[k]);
+  };
+  for_each_run_sample<D>(a, [&](uint32_t s, const int32_t* idx, const double* ci) {
+    int fbin = 0, mbin = 0;
+    float arg0 = 0.0f;
+    Cell cell;
+    const bool ok = fast_eval<D>(a, idx, ci, fbin, mbin, arg0, cell);
+    if (a.sample_fbin) {
+      a.sample_fbin[s] = ok ? (int16_t)fbin : (int16_t)-1;
+      a.sample_mbin[s] = ok ? (int16_t)mbin : (int16_t)-1;
+    }
+    if (!ok) return;
+    ++nvalid;
+    unsigned int q[4];
+    parzen_fixed(arg0, q);
+    const int key = fbin * B + mbin;
+    if (key != cur) {
+      flush();
+      cur = key;
+      cnt = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc[k] = 0ull;
+    }
+    ++cnt;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[k] += q[k];
+  });
+  flush();
+  // valid count: wavefront sum, one LDS atomic per wavefront
+  unsigned int v = nvalid;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(nvalid_lds, v);
+  __syncthreads();
+  for (int i = threadIdx.x; i < BB; i += blockDim.x) {
+    const unsigned long long h = hist[i];
+    if (h) atomicAdd(&a.hist[i], h);
+  }
+  for (int i = threadIdx.x; i < B; i += blockDim.x) {
+    const unsigned int m = fmarg[i];
+    if (m) atomicAdd(&a.fmarg[i], (unsigned long long)m);
+  }
+  if (threadIdx.x == 0 && *nvalid_lds) atomicAdd(a.nvalid, (unsigned long long)*nvalid_lds);
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) pass2_fast_kernel(EvalArgs a) {
+  extern __shared__ float lds_f32[];
+  const int B = a.bins;
+  const int BB = B * B;
+  double* scratch = reinterpret_cast<double*>(lds_f32);  // 4 wavefronts x 12 doubles
+  float* pr = lds_f32 + 128;
+  for (int i = threadIdx.x; i < BB; i += blockDim.x) pr[i] = a.pratio_f[i];
+  __syncthreads();
+  constexpr int PA = D * D + D;
+  double acc[PA];
+#pragma unroll
+  for (int i = 0; i < PA; ++i) acc[i] = 0.0;
+  for_each_run_sample<D>(a, [&](uint32_t, const int32_t* idx, const double* ci) {
+    int fbin = 0, mbin = 0;
+    float arg0 = 0.0f;
+    Cell cell;
+    if (!fast_eval<D>(a, idx, ci, fbin, mbin, arg0, cell)) return;
+    const float w = parzen_deriv_weight(arg0, pr + fbin * B + (mbin - 1));
+    if (w == 0.0f) return;
+    const float3 g3 = interp_gradient3<D>(a.texels, cell);
+    const double gd[3] = {(double)g3.x, (double)g3.y, (double)g3.z};
+    double vp[3];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      double p = a.virt.origin[k];
+      p += a.virt.i2p[k * D + 0] * (double)idx[0];
+      p += a.virt.i2p[k * D + 1] * (double)idx[1];
+      if (D == 3) p += a.virt.i2p[k * D + 2] * (double)idx[2];
+      vp[k] = p - a.mtf.center[k];
+    }
+#pragma unroll
+    for (int b = 0; b < D; ++b) {
+      const double wg = (double)w * gd[b];
+#pragma unroll
+      for (int d = 0; d < D; ++d) acc[b * D + d] = fma(wg, vp[d], acc[b * D + d]);
+      acc[D * D + b] += wg;
+    }
+  });
+  block_reduce_array<PA>(acc, scratch, a.partials + (int64_t)blockIdx.x * PA);
 }
 
 // ---------------------------------------------------------------------------
@@ -426,38 +674,6 @@ __device__ __forceinline__ void central_difference(const EvalArgs& a, const doub
   }
 }
 
-// Sum of N per-thread values over a 256-thread workgroup, written by thread 0
-// to out[0..N): wavefront shuffles, then the 4 wavefront results in LDS, in a
-// fixed order.  scratch holds 4 * N doubles.
-template <int N>
-__device__ __forceinline__ void block_reduce_array(double (&v)[N], double* scratch, double* out) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    double x = v[i];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
-    if (lane == 0) scratch[wid * N + i] = x;
-  }
-  __syncthreads();
-  if (threadIdx.x < N) {
-    double r = 0.0;
-    for (int w = 0; w < 4; ++w) r += scratch[w * N + threadIdx.x];
-    out[threadIdx.x] = r;
-  }
-}
-
-// sum_k B3'(arg0 + k) * pRatio[f][m - 1 + k] (MattesT:329-378 arguments).
-__device__ __forceinline__ float parzen_deriv_weight(float arg0, const float* prow) {
-  float w = 0.0f, arg = arg0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    w = fmaf(bspline3_derivf(arg), prow[k], w);
-    arg += 1.0f;
-  }
-  return w;
-}
-
 template <int D, int MODE>
 __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
   constexpr bool FAST = MODE == kFastAffine;
@@ -577,15 +793,31 @@ __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
     }                                                                                                         \
   } while (0)
 
+// grid: a multiple of 8 (one share per XCD) on the fast path.
 hipError_t launch_pass1(const EvalArgs& a, int grid, bool fast, hipStream_t st) {
   const size_t lds = (a.hist_in_lds ? (size_t)a.bins * a.bins * 8 : 0) + (size_t)(a.bins + 1) * 4;
-  MMI_DISPATCH(pass1_kernel, a, grid, lds, st, fast);
+  if (fast && a.mtf.kind == 1) {
+    if (a.virt.dim == 2)
+      hipLaunchKernelGGL(pass1_fast_kernel<2>, dim3(grid), dim3(256), lds, st, a);
+    else
+      hipLaunchKernelGGL(pass1_fast_kernel<3>, dim3(grid), dim3(256), lds, st, a);
+    return hipGetLastError();
+  }
+  MMI_DISPATCH(pass1_kernel, a, grid, lds, st, false);
   return hipGetLastError();
 }
 
 hipError_t launch_pass2(const EvalArgs& a, int grid, bool fast, hipStream_t st) {
+  if (fast && a.mtf.kind == 1) {
+    const size_t lds = ((size_t)a.bins * a.bins + 128) * 4;
+    if (a.virt.dim == 2)
+      hipLaunchKernelGGL(pass2_fast_kernel<2>, dim3(grid), dim3(256), lds, st, a);
+    else
+      hipLaunchKernelGGL(pass2_fast_kernel<3>, dim3(grid), dim3(256), lds, st, a);
+    return hipGetLastError();
+  }
   const size_t lds = ((a.pratio_in_lds ? (size_t)a.bins * a.bins : 0) + 128) * 4;
-  MMI_DISPATCH(pass2_kernel, a, grid, lds, st, fast);
+  MMI_DISPATCH(pass2_kernel, a, grid, lds, st, false);
   return hipGetLastError();
 }
 
