@@ -16,6 +16,10 @@
 
 using namespace mmi;
 
+// Uncertain samples one fast-path workgroup may defer to the exact path (one
+// per thread of the sorting workgroup in exceptions_pass2_kernel).
+constexpr int kExceptionCapacity = 256;
+
 namespace {
 
 thread_local std::string g_last_error = "";
@@ -263,6 +267,8 @@ struct mmi_context {
 
   DevBuf hist, fmarg, nvalid, pdf, fm, mm, pratio, pratio_f, rowsum, result, partials, deriv, sample_fbin, sample_mbin,
       scratch;
+  DevBuf exc_count, exc_list, exc_overflow;  // fast-path exception lists
+  int xcd_chunks = 1;                        // fast path: one contiguous sample share per XCD
   int grid = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float last_total_ms = 0, last_pass1_ms = 0, last_pass2_ms = 0;
@@ -394,12 +400,14 @@ bool fixed_direct_offsets(const mmi_geometry& v, const mmi_geometry& f, int64_t*
   return true;
 }
 
-// Parameters and error bounds of the certified fast path (FastAffine in
-// mattes_kernels.h).  Returns false when the path does not apply.
-bool setup_fast(const mmi_context* ctx, FastAffine* F) {
+// Parameters and error bounds of the certified fast path (FastArgs in
+// mattes_kernels.h; buffers are filled in by the caller).  Returns false when
+// the path does not apply.
+bool setup_fast(const mmi_context* ctx, FastArgs* F) {
   std::memset(F, 0, sizeof(*F));
   const int D = ctx->fixed_g.dim;
   const int B = ctx->bins;
+  if (num_pixels(ctx->virtual_g) >= (1ll << 31)) return false;
   if (ctx->flags & 2u) return false;  // exact path requested
   if (ctx->mtf.kind != MMI_TF_AFFINE || ctx->ftf.kind != MMI_TF_IDENTITY) return false;
   if (ctx->gradient_source != MMI_GRADIENT_IMAGE) return false;
@@ -481,9 +489,29 @@ bool setup_fast(const mmi_context* ctx, FastAffine* F) {
   F->mmin = (float)ctx->moving_min;
   F->mmax = (float)ctx->moving_max;
   int64_t off[3] = {0, 0, 0};
-  F->fixed_direct = fixed_direct_offsets(v, ctx->fixed_g, off) ? 1 : 0;
-  for (int k = 0; k < 3; ++k) F->fixed_offset[k] = off[k];
-  F->enabled = 1;
+  if (!fixed_direct_offsets(v, ctx->fixed_g, off)) return false;
+  const mmi_geometry& fg = ctx->fixed_g;
+  for (int k = 0; k < 3; ++k) {
+    F->foff[k] = k < D ? (int32_t)(off[k] - fg.start[k]) : 0;
+    F->vstart[k] = k < D ? (int32_t)v.start[k] : 0;
+    F->mstart[k] = k < D ? (int32_t)m.start[k] : 0;
+    F->mend[k] = k < D ? (int32_t)(m.start[k] + m.size[k] - 1) : 0;
+    F->cilo[k] = k < D ? (float)((double)m.start[k] - 0.5) : 0.0f;
+    F->cihi[k] = k < D ? (float)((double)(m.start[k] + m.size[k] - 1) + 0.5) : 0.0f;
+    if (k < D && ((double)F->cilo[k] != (double)m.start[k] - 0.5 ||
+                  (double)F->cihi[k] != (double)(m.start[k] + m.size[k] - 1) + 0.5))
+      return false;  // bounds not exact in FP32 (|index| >= 2^23)
+  }
+  F->vsize0 = (int32_t)v.size[0];
+  F->vsize1 = (int32_t)(D > 1 ? v.size[1] : 1);
+  F->vdx = make_fastdiv((uint32_t)F->vsize0);
+  F->vdy = make_fastdiv((uint32_t)F->vsize1);
+  F->fsy = (int32_t)fg.size[0];
+  F->fsz = (int32_t)(fg.size[0] * (D > 1 ? fg.size[1] : 1));
+  F->msy = (int32_t)m.size[0];
+  F->msz = (int32_t)(m.size[0] * (D > 1 ? m.size[1] : 1));
+  F->n = (uint32_t)num_pixels(v);
+  F->bins = B;
   return true;
 }
 
@@ -519,6 +547,7 @@ int mmi_context_create(int device, uint32_t flags, mmi_context** out) {
   auto* ctx = new mmi_context();
   ctx->device = device;
   ctx->flags = flags;
+  if (const char* e = std::getenv("MMI_XCD_CHUNKS")) ctx->xcd_chunks = std::atoi(e) != 0;
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete ctx;
@@ -543,7 +572,8 @@ int mmi_context_destroy(mmi_context* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   DevBuf* bufs[] = {&ctx->fixed_px, &ctx->moving_px, &ctx->texels, &ctx->coeffs, &ctx->hist, &ctx->fmarg,
                     &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->result,
-                    &ctx->partials, &ctx->deriv, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch};
+                    &ctx->partials, &ctx->deriv, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
+                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_overflow};
   for (DevBuf* b : bufs) b->release();
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -820,12 +850,39 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
   if (want_derivative && ctx->gradient_source == MMI_GRADIENT_IMAGE && !ctx->have_texels)
     return fail(MMI_ERR_ARGUMENT, "moving gradient image missing");
 
+  FastArgs fa;
+  bool fast = setup_fast(ctx, &fa);
+  if (fast) {
+    MMI_HIP(ctx->exc_count.ensure(sizeof(uint32_t) * ctx->grid));
+    MMI_HIP(ctx->exc_list.ensure(sizeof(uint32_t) * ctx->grid * kExceptionCapacity));
+    MMI_HIP(ctx->exc_overflow.ensure(sizeof(uint32_t)));
+    fa.xcd_chunks = ctx->xcd_chunks;
+    fa.exc_cap = kExceptionCapacity;
+    fa.fixed_px = a.fixed_px;
+    fa.moving_px = a.moving_px;
+    fa.texels = a.texels;
+    fa.hist = a.hist;
+    fa.fmarg = a.fmarg;
+    fa.nvalid = a.nvalid;
+    fa.sample_fbin = a.sample_fbin;
+    fa.sample_mbin = a.sample_mbin;
+    fa.pratio_f = a.pratio_f;
+    fa.exc_count = ctx->exc_count.as<uint32_t>();
+    fa.exc_list = ctx->exc_list.as<uint32_t>();
+    fa.exc_overflow = ctx->exc_overflow.as<uint32_t>();
+  }
+retry:
   MMI_HIP(hipMemsetAsync(ctx->hist.p, 0, sizeof(unsigned long long) * B * B, st));
   MMI_HIP(hipMemsetAsync(ctx->fmarg.p, 0, sizeof(unsigned long long) * B, st));
   MMI_HIP(hipMemsetAsync(ctx->nvalid.p, 0, sizeof(unsigned long long), st));
-  const bool fast = setup_fast(ctx, &a.fast);
+  if (fast) MMI_HIP(hipMemsetAsync(ctx->exc_overflow.p, 0, sizeof(uint32_t), st));
   MMI_HIP(hipEventRecord(ctx->ev[0], st));
-  MMI_HIP(launch_pass1(a, ctx->grid, fast, st));
+  if (fast) {
+    MMI_HIP(launch_fast_pass1(fa, D, ctx->grid, st));
+    MMI_HIP(launch_fast_exceptions1(a, fa, ctx->grid, st));
+  } else {
+    MMI_HIP(launch_pass1(a, ctx->grid, st));
+  }
   MMI_HIP(hipEventRecord(ctx->ev[1], st));
 
   FinalizeArgs f;
@@ -847,21 +904,34 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
   if (want_derivative) {
     MMI_HIP(ctx->deriv.ensure(sizeof(double) * P));
     MMI_HIP(hipEventRecord(ctx->ev[2], st));
-    if (ctx->mtf.kind == MMI_TF_AFFINE) {
+    if (fast) {
+      MMI_HIP(ctx->partials.ensure(sizeof(double) * P * 2 * ctx->grid));
+      fa.partials = ctx->partials.as<double>();
+      MMI_HIP(launch_fast_pass2(fa, D, ctx->grid, st));
+      MMI_HIP(launch_fast_exceptions2(a, fa, ctx->grid, st));
+      MMI_HIP(hipEventRecord(ctx->ev[3], st));
+      FastMap map;
+      std::memset(&map, 0, sizeof(map));
+      for (int i = 0; i < D * D; ++i) map.V[i] = ctx->virtual_g.index_to_physical[i];
+      for (int i = 0; i < D; ++i) map.c0[i] = ctx->virtual_g.origin[i] - ctx->dmtf.center[i];
+      MMI_HIP(launch_fast_derivative(fa.partials, 2 * ctx->grid, map, D, f.result, ctx->deriv.as<double>(), st));
+    } else if (ctx->mtf.kind == MMI_TF_AFFINE) {
       MMI_HIP(ctx->partials.ensure(sizeof(double) * P * ctx->grid));
       a.partials = ctx->partials.as<double>();
-      MMI_HIP(launch_pass2(a, ctx->grid, fast, st));
+      MMI_HIP(launch_pass2(a, ctx->grid, st));
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
       MMI_HIP(launch_derivative_reduce(a.partials, ctx->grid, (int)P, f.result, ctx->deriv.as<double>(), st));
     } else {
       MMI_HIP(hipMemsetAsync(ctx->deriv.p, 0, sizeof(double) * P, st));
       a.deriv_accum = ctx->deriv.as<double>();
-      MMI_HIP(launch_pass2(a, ctx->grid, false, st));
+      MMI_HIP(launch_pass2(a, ctx->grid, st));
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
       MMI_HIP(launch_derivative_scale(ctx->deriv.as<double>(), P, f.result, st));
     }
   }
   DevResult r;
+  uint32_t overflow = 0;
+  if (fast) MMI_HIP(hipMemcpyAsync(&overflow, ctx->exc_overflow.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   MMI_HIP(hipMemcpyAsync(&r, ctx->result.p, sizeof(DevResult), hipMemcpyDeviceToHost, st));
   if (want_derivative && derivative)
     MMI_HIP(hipMemcpyAsync(derivative, ctx->deriv.p, sizeof(double) * P, hipMemcpyDeviceToHost, st));
@@ -869,6 +939,11 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
   if (fixed_marginal)
     MMI_HIP(hipMemcpyAsync(fixed_marginal, ctx->fm.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
   MMI_HIP(hipStreamSynchronize(st));
+  if (overflow) {
+    // more uncertain samples in one workgroup than its list holds: redo exactly
+    fast = false;
+    goto retry;
+  }
   float t1 = 0, t2 = 0;
   MMI_HIP(hipEventElapsedTime(&t1, ctx->ev[0], ctx->ev[1]));
   ctx->last_pass1_ms = t1;

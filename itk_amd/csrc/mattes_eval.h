@@ -1,0 +1,136 @@
+// Per-sample evaluators shared by the exact and fast Mattes MI kernels.
+#pragma once
+
+#include "mattes_kernels.h"
+
+namespace mmi {
+
+constexpr double kFixedPointScale = 4294967296.0;  // 2^32
+
+template <int D>
+struct SampleOut {
+  int fbin;
+  int mbin;
+  double mterm;
+  double vp[3];  // virtual point (the Jacobian is taken there)
+  double mp[3];  // mapped moving point
+  Cell cell;     // moving interpolation cell
+  BSplineSupport<D> sup;
+};
+
+// Fixed transform: identity or affine.
+template <int D>
+__device__ __forceinline__ void fixed_point(const DevTransform& t, const double* in, double* out) {
+  if (t.kind == 1) {
+    transform_point<D, 1>(t, in, out, nullptr);
+  } else {
+#pragma unroll
+    for (int i = 0; i < D; ++i) out[i] = in[i];
+  }
+}
+
+// ProcessVirtualPoint + Mattes ProcessPoint up to the bins
+// (itkImageToImageMetricv4GetValueAndDerivativeThreaderBase.hxx:174-273,
+// itkImageToImageMetricv4.hxx:289-373,
+// itkMattesMutualInformationImageToImageMetricv4GetValueAndDerivativeThreader.hxx:215-275),
+// exact FP64.  MK is the moving transform kind (1 affine, 2 B-spline).
+template <int D, int MK>
+__device__ __forceinline__ bool eval_sample(const EvalArgs& a, const int32_t* idx, SampleOut<D>& o) {
+  index_to_point<D>(a.virt, idx, o.vp);
+  double fp[3], ci[3];
+  fixed_point<D>(a.ftf, o.vp, fp);
+  point_to_cindex<D>(a.fixed, fp, ci);
+  if (!inside_buffer<D>(a.fixed, ci)) return false;
+  const Cell fc = make_cell<D>(a.fixed, ci);
+  const double fval = (double)(float)interp_exact<D>(a.fixed_px, fc);
+
+  transform_point<D, MK>(a.mtf, o.vp, o.mp, &o.sup);
+  point_to_cindex<D>(a.moving, o.mp, ci);
+  if (!inside_buffer<D>(a.moving, ci)) return false;
+  o.cell = make_cell<D>(a.moving, ci);
+  const double mval = (double)(float)interp_exact<D>(a.moving_px, o.cell);
+
+  if (mval < a.moving_min) return false;
+  if (mval > a.moving_max) return false;
+  const double mterm = mval / a.moving_bin_size - a.moving_norm_min;
+  const int mb = mterm < 2.0 ? 2 : (mterm >= (double)(a.bins - 2) ? a.bins - 3 : (int)mterm);
+  const double fterm = fval / a.fixed_bin_size - a.fixed_norm_min;
+  const int fb = fterm < 2.0 ? 2 : (fterm >= (double)(a.bins - 2) ? a.bins - 3 : (int)fterm);
+  o.fbin = fb;
+  o.mbin = mb;
+  o.mterm = mterm;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Parzen weights
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float bspline3f(float u) {
+  const float a = fabsf(u);
+  if (a < 1.0f) return fmaf(a * a, fmaf(0.5f, a, -1.0f), 2.0f / 3.0f);
+  if (a < 2.0f) {
+    const float t = 2.0f - a;
+    return t * t * t * (1.0f / 6.0f);
+  }
+  return 0.0f;
+}
+
+__device__ __forceinline__ float bspline3_derivf(float u) {
+  if (u >= 0.0f && u < 1.0f) return u * fmaf(1.5f, u, -2.0f);
+  if (u > -1.0f && u < 0.0f) return -u * fmaf(1.5f, u, 2.0f);
+  if (u >= 1.0f && u < 2.0f) return fmaf(u, fmaf(-0.5f, u, 2.0f), -2.0f);
+  if (u > -2.0f && u <= -1.0f) return fmaf(u, fmaf(0.5f, u, 2.0f), 2.0f);
+  return 0.0f;
+}
+
+// The four Parzen weights B3(arg0 + k) as 2^32 fixed point, summing to exactly
+// 2^32 (partition of unity; the rounding remainder goes to weight 1, which is
+// >= 1/6 whenever the window is not clamped).
+__device__ __forceinline__ void parzen_fixed(float arg0, unsigned int* q) {
+  float w[4];
+  float arg = arg0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    w[k] = bspline3f(arg);
+    arg += 1.0f;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) q[k] = (unsigned int)rintf(w[k] * 4294967296.0f);
+  // arg0 in (-2, -1]: unclamped window, the weights are a partition of unity
+  if (arg0 <= -1.0f && arg0 > -2.0f) q[1] = 0u - q[0] - q[2] - q[3];
+}
+
+// sum_k B3'(arg0 + k) * pRatio[f][m - 1 + k] (MattesT:329-378 arguments).
+__device__ __forceinline__ float parzen_deriv_weight(float arg0, const float* prow) {
+  float w = 0.0f, arg = arg0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    w = fmaf(bspline3_derivf(arg), prow[k], w);
+    arg += 1.0f;
+  }
+  return w;
+}
+
+// Sum of N per-thread values over a 256-thread workgroup, written by thread 0
+// to out[0..N): wavefront shuffles, then the 4 wavefront results in LDS, in a
+// fixed order.  scratch holds 4 * N doubles.
+template <int N>
+__device__ __forceinline__ void block_reduce_array(double (&v)[N], double* scratch, double* out) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double x = v[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    if (lane == 0) scratch[wid * N + i] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    double r = 0.0;
+    for (int w = 0; w < 4; ++w) r += scratch[w * N + threadIdx.x];
+    out[threadIdx.x] = r;
+  }
+}
+
+}  // namespace mmi

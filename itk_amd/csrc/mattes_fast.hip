@@ -1,0 +1,400 @@
+// Certified fast path of the Mattes MI v4 evaluation (affine moving transform,
+// identity fixed transform, fixed samples on fixed pixel centres).
+//
+// Per sample: the moving continuous index is one FP64 affine map of the
+// virtual index (|error| <= e_ci against the reference's chain of
+// index -> point -> transform -> continuous index), interpolation and Parzen
+// terms run in FP32 with precomputed error bounds.  A sample is "certain" when
+// its inside test, intensity-range test and both bin indices provably equal
+// the reference's; every other sample goes to a per-workgroup exception list
+// and is evaluated afterwards on the exact FP64 path (mattes_eval.h), so the
+// bins and the valid count are bit-identical to the reference's.
+//
+// Kernels: pass1_fast (histogram of certain samples) -> exceptions_pass1
+// (exact samples, integer atomics) -> finalize (mattes_kernels.hip) ->
+// pass2_fast (derivative moments of certain samples) -> exceptions_pass2
+// (exact samples, sorted per list so sums are in a fixed order) ->
+// fast_derivative (fixed-order reduction, index-space moments mapped to the
+// affine parameters).
+//
+// The hot kernels take a compact argument block so that the uniform values
+// they use fit the scalar register file; the exception kernels take the full
+// EvalArgs.
+#include "mattes_eval.h"
+
+#include <hip/hip_runtime.h>
+
+namespace mmi {
+
+constexpr int kXcd = 8;
+
+template <int D>
+__device__ __forceinline__ void run_range(const FastArgs& f, uint32_t& begin, uint32_t& end, uint32_t& step) {
+  if (f.xcd_chunks) {
+    // workgroup b runs on XCD b % 8: give each XCD one contiguous share
+    const uint32_t xcd = blockIdx.x % kXcd;
+    const uint32_t lb = blockIdx.x / kXcd;
+    const uint32_t bpx = gridDim.x / kXcd;
+    const uint32_t chunk = (f.n + kXcd - 1) / kXcd;
+    const uint32_t c0 = xcd * chunk;
+    begin = c0 + lb * blockDim.x + threadIdx.x;
+    end = min(f.n, c0 + chunk);
+    step = bpx * blockDim.x;
+  } else {
+    begin = blockIdx.x * blockDim.x + threadIdx.x;
+    end = f.n;
+    step = gridDim.x * blockDim.x;
+  }
+}
+
+template <int D>
+__device__ __forceinline__ void fast_index(const FastArgs& f, uint32_t s, int32_t* idx) {
+  const uint32_t r = fdiv(s, f.vdx);
+  idx[0] = f.vstart[0] + (int32_t)(s - r * (uint32_t)f.vsize0);
+  if (D == 2) {
+    idx[1] = f.vstart[1] + (int32_t)r;
+    idx[2] = 0;
+  } else {
+    const uint32_t z = fdiv(r, f.vdy);
+    idx[1] = f.vstart[1] + (int32_t)(r - z * (uint32_t)f.vsize1);
+    idx[2] = f.vstart[2] + (int32_t)z;
+  }
+}
+
+__device__ __forceinline__ int fast_bin(float term, float bound, int B, bool* uncertain) {
+  if (term < 2.0f) {
+    *uncertain = *uncertain || (term >= 3.0f - bound);
+    return 2;
+  }
+  if (term >= (float)(B - 2)) {
+    *uncertain = *uncertain || (term < (float)(B - 3) + bound);
+    return B - 3;
+  }
+  const float n = rintf(term);
+  if (n >= 3.0f && n <= (float)(B - 3) && fabsf(term - n) <= bound) *uncertain = true;
+  return (int)term;
+}
+
+// 0: not counted, 1: counted (bins, arg0, cell set), 2: uncertain.
+template <int D>
+__device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* idx, int& fbin, int& mbin, float& arg0,
+                                             Cell& cell) {
+  double ci[3];
+  const double x = (double)idx[0], y = (double)idx[1], z = (double)idx[2];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    double c = f.b[k];
+    if (D == 3) c = fma(f.A[k * 3 + 2], z, c);
+    c = fma(f.A[k * D + 1], y, c);
+    ci[k] = fma(f.A[k * D + 0], x, c);
+  }
+  bool uncertain = false;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const double lo = (double)f.cilo[k], hi = (double)f.cihi[k];
+    if (ci[k] < lo - f.e_ci || ci[k] >= hi + f.e_ci) return 0;
+    uncertain = uncertain || (ci[k] < lo + f.e_ci) || (ci[k] >= hi - f.e_ci);
+  }
+  if (uncertain) return 2;
+  // moving cell (the reference's branch rules as duplicated corners)
+  int32_t bb[3] = {0, 0, 0};
+  bool act[3] = {false, false, false};
+  cell.d[2] = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    const int32_t fl = itk_floor(ci[j]);
+    bb[j] = fl > f.mstart[j] ? fl : f.mstart[j];
+    cell.d[j] = ci[j] - (double)bb[j];
+    act[j] = (cell.d[j] > 0.0) && (bb[j] + 1 <= f.mend[j]);
+  }
+  cell.base = (bb[0] - f.mstart[0]) + f.msy * (bb[1] - f.mstart[1]) + (D == 3 ? f.msz * (bb[2] - f.mstart[2]) : 0);
+  cell.ox = act[0] ? 1 : 0;
+  cell.oy = act[1] ? f.msy : 0;
+  cell.oz = (D == 3 && act[2]) ? f.msz : 0;
+  const float t0 = (float)cell.d[0], t1 = (float)cell.d[1], t2 = (float)cell.d[2];
+  const float* img = f.moving_px;
+  const float v000 = img[cell.base], v100 = img[cell.base + cell.ox];
+  const float v010 = img[cell.base + cell.oy], v110 = img[cell.base + cell.oy + cell.ox];
+  const float vx00 = fmaf(v100 - v000, t0, v000);
+  const float vx10 = fmaf(v110 - v010, t0, v010);
+  float v = fmaf(vx10 - vx00, t1, vx00);
+  if (D == 3) {
+    const float v001 = img[cell.base + cell.oz], v101 = img[cell.base + cell.oz + cell.ox];
+    const float v011 = img[cell.base + cell.oz + cell.oy], v111 = img[cell.base + cell.oz + cell.oy + cell.ox];
+    const float vx01 = fmaf(v101 - v001, t0, v001);
+    const float vx11 = fmaf(v111 - v011, t0, v011);
+    const float vxx1 = fmaf(vx11 - vx01, t1, vx01);
+    v = fmaf(vxx1 - v, t2, v);
+  }
+  if (v < f.mmin - f.ev || v > f.mmax + f.ev) return 0;
+  if (v < f.mmin + f.ev || v > f.mmax - f.ev) return 2;
+  const float mterm = fmaf(v, f.inv_mbs, -f.mnm);
+  mbin = fast_bin(mterm, f.bt_m, f.bins, &uncertain);
+  arg0 = (float)(mbin - 1) - mterm;
+  // fixed value: the pixel the fixed sample sits on (verified on the host)
+  const int32_t fl = (idx[0] + f.foff[0]) + f.fsy * (idx[1] + f.foff[1]) + (D == 3 ? f.fsz * (idx[2] + f.foff[2]) : 0);
+  const float fv = f.fixed_px[fl];
+  const float fterm = fmaf(fv, f.inv_fbs, -f.fnm);
+  fbin = fast_bin(fterm, f.bt_f, f.bins, &uncertain);
+  return uncertain ? 2 : 1;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f) {
+  extern __shared__ unsigned long long lds_u64[];
+  const int B = f.bins;
+  const int BB = B * B;
+  unsigned long long* hist = lds_u64;
+  unsigned int* fmarg = reinterpret_cast<unsigned int*>(lds_u64 + BB);
+  unsigned int* counters = fmarg + B;  // [0] valid count, [1] exception count
+  for (int i = threadIdx.x; i < BB; i += blockDim.x) hist[i] = 0ull;
+  for (int i = threadIdx.x; i < B + 2; i += blockDim.x) fmarg[i] = 0u;
+  __syncthreads();
+  unsigned int nvalid = 0;
+  uint32_t s, end, step;
+  run_range<D>(f, s, end, step);
+  for (; s < end; s += step) {
+    int32_t idx[3];
+    fast_index<D>(f, s, idx);
+    int fbin = 0, mbin = 0;
+    float arg0 = 0.0f;
+    Cell cell;
+    const int c = fast_classify<D>(f, idx, fbin, mbin, arg0, cell);
+    if (c == 2) {
+      const unsigned int slot = atomicAdd(&counters[1], 1u);
+      if (slot < (unsigned int)f.exc_cap) f.exc_list[blockIdx.x * f.exc_cap + slot] = s;
+      continue;
+    }
+    if (f.sample_fbin) {
+      f.sample_fbin[s] = c == 1 ? (int16_t)fbin : (int16_t)-1;
+      f.sample_mbin[s] = c == 1 ? (int16_t)mbin : (int16_t)-1;
+    }
+    if (c == 0) continue;
+    ++nvalid;
+    atomicAdd(&fmarg[fbin], 1u);
+    unsigned int q[4];
+    parzen_fixed(arg0, q);
+    unsigned long long* row = hist + fbin * B + (mbin - 1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (q[k]) atomicAdd(&row[k], (unsigned long long)q[k]);
+  }
+  unsigned int v = nvalid;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(&counters[0], v);
+  __syncthreads();
+  for (int i = threadIdx.x; i < BB; i += blockDim.x) {
+    const unsigned long long h = hist[i];
+    if (h) atomicAdd(&f.hist[i], h);
+  }
+  for (int i = threadIdx.x; i < B; i += blockDim.x) {
+    const unsigned int m = fmarg[i];
+    if (m) atomicAdd(&f.fmarg[i], (unsigned long long)m);
+  }
+  if (threadIdx.x == 0) {
+    if (counters[0]) atomicAdd(f.nvalid, (unsigned long long)counters[0]);
+    const unsigned int ne = counters[1];
+    f.exc_count[blockIdx.x] = ne;
+    if (ne > (unsigned int)f.exc_cap) atomicOr(f.exc_overflow, 1u);
+  }
+}
+
+// Exact FP64 evaluation of the listed samples of one pass-1 workgroup.
+template <int D>
+__global__ void __launch_bounds__(256) exceptions_pass1_kernel(EvalArgs a, FastArgs f) {
+  const uint32_t ne = min(f.exc_count[blockIdx.x], (uint32_t)f.exc_cap);
+  for (uint32_t t = threadIdx.x; t < ne; t += blockDim.x) {
+    const uint32_t s = f.exc_list[blockIdx.x * f.exc_cap + t];
+    int32_t idx[3];
+    sample_index<D>(a.virt, s, idx);
+    SampleOut<D> o;
+    const bool ok = eval_sample<D, 1>(a, idx, o);
+    if (a.sample_fbin) {
+      a.sample_fbin[s] = ok ? (int16_t)o.fbin : (int16_t)-1;
+      a.sample_mbin[s] = ok ? (int16_t)o.mbin : (int16_t)-1;
+    }
+    if (!ok) continue;
+    atomicAdd(a.nvalid, 1ull);
+    atomicAdd(&a.fmarg[o.fbin], 1ull);
+    unsigned int q[4];
+    parzen_fixed((float)((double)(o.mbin - 1) - o.mterm), q);
+    unsigned long long* row = a.hist + o.fbin * a.bins + (o.mbin - 1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (q[k]) atomicAdd(&row[k], (unsigned long long)q[k]);
+  }
+}
+
+// Derivative moments in index space, per sample:
+//   M[b][j] += w * g_b * idx_j,  M1[b] += w * g_b
+// The physical affine derivative is J(M) = V M + c0 M1 (fast_derivative_kernel).
+template <int D>
+__device__ __forceinline__ void add_moments(double* acc, float w, float3 g, const int32_t* idx) {
+  const double gw[3] = {(double)(w * g.x), (double)(w * g.y), (double)(w * g.z)};
+#pragma unroll
+  for (int b = 0; b < D; ++b) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) acc[b * D + j] = fma(gw[b], (double)idx[j], acc[b * D + j]);
+    acc[D * D + b] += gw[b];
+  }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) pass2_fast_kernel(FastArgs f) {
+  extern __shared__ float lds_f32[];
+  const int B = f.bins;
+  const int BB = B * B;
+  double* scratch = reinterpret_cast<double*>(lds_f32);  // 4 wavefronts x 12 doubles (128 floats)
+  float* pr = lds_f32 + 128;
+  for (int i = threadIdx.x; i < BB; i += blockDim.x) pr[i] = f.pratio_f[i];
+  __syncthreads();
+  constexpr int PA = D * D + D;
+  double acc[PA];
+#pragma unroll
+  for (int i = 0; i < PA; ++i) acc[i] = 0.0;
+  uint32_t s, end, step;
+  run_range<D>(f, s, end, step);
+  for (; s < end; s += step) {
+    int32_t idx[3];
+    fast_index<D>(f, s, idx);
+    int fbin = 0, mbin = 0;
+    float arg0 = 0.0f;
+    Cell cell;
+    if (fast_classify<D>(f, idx, fbin, mbin, arg0, cell) != 1) continue;
+    const float w = parzen_deriv_weight(arg0, pr + fbin * B + (mbin - 1));
+    if (w == 0.0f) continue;
+    add_moments<D>(acc, w, interp_gradient3<D>(f.texels, cell), idx);
+  }
+  block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)blockIdx.x * PA);
+}
+
+// Exception samples of one pass-1 workgroup, in sample order (bitonic sort of
+// the list in LDS), summed in a fixed tree: deterministic.
+template <int D>
+__global__ void __launch_bounds__(256) exceptions_pass2_kernel(EvalArgs a, FastArgs f) {
+  __shared__ uint32_t keys[256];
+  __shared__ double scratch[4 * 12];
+  constexpr int PA = D * D + D;
+  const uint32_t ne = min(f.exc_count[blockIdx.x], (uint32_t)f.exc_cap);
+  double acc[PA];
+#pragma unroll
+  for (int i = 0; i < PA; ++i) acc[i] = 0.0;
+  if (ne > 0) {
+    keys[threadIdx.x] = threadIdx.x < ne ? f.exc_list[blockIdx.x * f.exc_cap + threadIdx.x] : 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint32_t k = 2; k <= 256; k <<= 1) {
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        const uint32_t i = threadIdx.x, l = i ^ j;
+        if (l > i) {
+          const uint32_t ki = keys[i], kl = keys[l];
+          const bool up = (i & k) == 0;
+          if ((ki > kl) == up) {
+            keys[i] = kl;
+            keys[l] = ki;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (threadIdx.x < ne) {
+      const uint32_t s = keys[threadIdx.x];
+      int32_t idx[3];
+      sample_index<D>(a.virt, s, idx);
+      SampleOut<D> o;
+      if (eval_sample<D, 1>(a, idx, o)) {
+        const float w = parzen_deriv_weight((float)((double)(o.mbin - 1) - o.mterm),
+                                            f.pratio_f + o.fbin * a.bins + (o.mbin - 1));
+        if (w != 0.0f) add_moments<D>(acc, w, interp_gradient3<D>(a.texels, o.cell), idx);
+      }
+    }
+  }
+  block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)(gridDim.x + blockIdx.x) * PA);
+}
+
+// Fixed-order reduction of 2 * nblocks rows of index-space moments, then the
+// map to the affine derivative:
+//   d[b*D + d]   = nFactor * (sum_j V[d][j] M[b][j] + c0[d] M1[b])
+//   d[D*D + b]   = nFactor * M1[b]
+// with V the virtual index-to-physical matrix and c0 = virtual origin - centre.
+template <int D>
+__global__ void __launch_bounds__(256) fast_derivative_kernel(const double* __restrict__ partials, int rows,
+                                                              FastMap map, const DevResult* __restrict__ result,
+                                                              double* __restrict__ deriv) {
+  constexpr int PA = D * D + D;
+  __shared__ double red[256];
+  __shared__ double mom[PA];
+  for (int mu = 0; mu < PA; ++mu) {
+    double s = 0.0;
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) s += partials[(int64_t)r * PA + mu];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) mom[mu] = red[0];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double nf = result->n_factor;
+    for (int b = 0; b < D; ++b) {
+      const double m1 = mom[D * D + b];
+      for (int d = 0; d < D; ++d) {
+        double s = map.c0[d] * m1;
+        for (int j = 0; j < D; ++j) s += map.V[d * D + j] * mom[b * D + j];
+        deriv[b * D + d] = s * nf;
+      }
+      deriv[D * D + b] = m1 * nf;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers
+// ---------------------------------------------------------------------------
+
+hipError_t launch_fast_pass1(const FastArgs& f, int dim, int grid, hipStream_t st) {
+  const size_t lds = (size_t)f.bins * f.bins * 8 + (size_t)(f.bins + 2) * 4;
+  if (dim == 2)
+    hipLaunchKernelGGL(pass1_fast_kernel<2>, dim3(grid), dim3(256), lds, st, f);
+  else
+    hipLaunchKernelGGL(pass1_fast_kernel<3>, dim3(grid), dim3(256), lds, st, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st) {
+  if (a.virt.dim == 2)
+    hipLaunchKernelGGL(exceptions_pass1_kernel<2>, dim3(grid), dim3(256), 0, st, a, f);
+  else
+    hipLaunchKernelGGL(exceptions_pass1_kernel<3>, dim3(grid), dim3(256), 0, st, a, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_fast_pass2(const FastArgs& f, int dim, int grid, hipStream_t st) {
+  const size_t lds = ((size_t)f.bins * f.bins + 128) * 4;
+  if (dim == 2)
+    hipLaunchKernelGGL(pass2_fast_kernel<2>, dim3(grid), dim3(256), lds, st, f);
+  else
+    hipLaunchKernelGGL(pass2_fast_kernel<3>, dim3(grid), dim3(256), lds, st, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_fast_exceptions2(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st) {
+  if (a.virt.dim == 2)
+    hipLaunchKernelGGL(exceptions_pass2_kernel<2>, dim3(grid), dim3(256), 0, st, a, f);
+  else
+    hipLaunchKernelGGL(exceptions_pass2_kernel<3>, dim3(grid), dim3(256), 0, st, a, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_fast_derivative(const double* partials, int rows, const FastMap& map, int dim,
+                                  const DevResult* r, double* deriv, hipStream_t st) {
+  if (dim == 2)
+    hipLaunchKernelGGL(fast_derivative_kernel<2>, dim3(1), dim3(256), 0, st, partials, rows, map, r, deriv);
+  else
+    hipLaunchKernelGGL(fast_derivative_kernel<3>, dim3(1), dim3(256), 0, st, partials, rows, map, r, deriv);
+  return hipGetLastError();
+}
+
+}  // namespace mmi

@@ -17,31 +17,57 @@ struct DevResult {
   double n_factor;  // -1 / (moving bin size * NValid)
 };
 
-// Certified fast path for an affine moving transform.  The moving continuous
-// index is one FP64 affine map of the virtual index (|error| << e_ci against
-// the reference's chain); interpolation and Parzen terms run in FP32 with
-// precomputed error bounds, and any sample whose inside-test, range test or
-// bin index could differ from the reference's within those bounds is redone
-// on the exact FP64 path.
-struct FastAffine {
-  int32_t enabled;
-  int32_t fixed_direct;  // fixed value = fixed pixel at (virtual index + fixed_offset), bit-exact
-  double A[9];           // virtual index -> moving continuous index
+// Compact arguments of the certified fast path (mattes_fast.hip): an affine
+// moving transform, identity fixed transform and fixed samples on fixed pixel
+// centres.  The moving continuous index is one FP64 affine map of the virtual
+// index (|error| <= e_ci against the reference's chain); interpolation and
+// Parzen terms run in FP32 with the error bounds below, and any sample whose
+// inside test, range test or bin index could differ from the reference's
+// within those bounds is listed and redone on the exact FP64 path.
+struct FastArgs {
+  uint32_t n;  // virtual samples
+  int32_t bins;
+  int32_t vstart[3];
+  int32_t vsize0, vsize1;
+  FastDiv vdx, vdy;
+  const float* fixed_px;
+  int32_t fsy, fsz;
+  int32_t foff[3];  // fixed pixel index = virtual index + foff (buffer-relative)
+  const float* moving_px;
+  const float4* texels;
+  int32_t mstart[3], mend[3];
+  int32_t msy, msz;
+  double A[9];  // virtual index -> moving continuous index
   double b[3];
-  double e_ci;           // bound on |ci_fast - ci_reference|
-  int32_t fixed_offset[3];
-  int32_t pad0_;
-  float inv_mbs, mnm;    // 1 / moving bin size, moving normalised minimum
+  double e_ci;  // bound on |ci_fast - ci_reference|
+  float cilo[3], cihi[3];
+  float inv_mbs, mnm;  // 1 / moving bin size, moving normalised minimum
   float inv_fbs, fnm;
-  float mmin, mmax;      // moving TrueMin / TrueMax
-  float ev;              // bound on |v_fast - v_reference| for moving values
-  float bt_m;            // bound on |mterm_fast - mterm_reference|
-  float bt_f;            // bound on |fterm_fast - fterm_reference|
-  float pad_;
+  float mmin, mmax;  // moving TrueMin / TrueMax
+  float ev;          // bound on |v_fast - v_reference| for moving values
+  float bt_m;        // bound on |mterm_fast - mterm_reference|
+  float bt_f;        // bound on |fterm_fast - fterm_reference|
+  int32_t xcd_chunks;
+  int32_t exc_cap;
+  unsigned long long* hist;
+  unsigned long long* fmarg;
+  unsigned long long* nvalid;
+  int16_t* sample_fbin;
+  int16_t* sample_mbin;
+  const float* pratio_f;
+  double* partials;  // [2 * grid * P]: fast rows then exception rows
+  uint32_t* exc_count;
+  uint32_t* exc_list;
+  uint32_t* exc_overflow;
+};
+
+// Map from index-space derivative moments to the affine parameters.
+struct FastMap {
+  double V[9];   // virtual index_to_physical
+  double c0[3];  // virtual origin - transform centre
 };
 
 struct EvalArgs {
-  FastAffine fast;
   DevGeom virt;
   DevGeom fixed;
   DevGeom moving;
@@ -95,9 +121,15 @@ struct Direction {
   double m[9];
 };
 
-hipError_t launch_pass1(const EvalArgs& a, int grid, bool fast, hipStream_t st);
+hipError_t launch_pass1(const EvalArgs& a, int grid, hipStream_t st);
 hipError_t launch_finalize(const FinalizeArgs& f, hipStream_t st);
-hipError_t launch_pass2(const EvalArgs& a, int grid, bool fast, hipStream_t st);
+hipError_t launch_pass2(const EvalArgs& a, int grid, hipStream_t st);
+hipError_t launch_fast_pass1(const FastArgs& f, int dim, int grid, hipStream_t st);
+hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st);
+hipError_t launch_fast_pass2(const FastArgs& f, int dim, int grid, hipStream_t st);
+hipError_t launch_fast_exceptions2(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st);
+hipError_t launch_fast_derivative(const double* partials, int rows, const FastMap& map, int dim,
+                                  const DevResult* r, double* deriv, hipStream_t st);
 hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, const DevResult* r, double* deriv,
                                     hipStream_t st);
 hipError_t launch_derivative_scale(double* deriv, int64_t P, const DevResult* r, hipStream_t st);
