@@ -112,6 +112,45 @@ __device__ __forceinline__ float parzen_deriv_weight(float arg0, const float* pr
   return w;
 }
 
+// Unclamped Parzen window (arg0 = -1 - t, t in [0, 1)): the four cubic
+// B-spline weights and derivatives in closed form, branch-free.  They equal
+// B3(arg0 + k) and B3'(arg0 + k) of the piecewise definitions above.
+__device__ __forceinline__ void parzen_weights_t(float t, float* w) {
+  const float u = 1.0f - t;
+  const float t2 = t * t, t3 = t2 * t;
+  w[0] = u * u * u * (1.0f / 6.0f);
+  w[1] = fmaf(t2, fmaf(0.5f, t, -1.0f), 2.0f / 3.0f);  // (3t^3 - 6t^2 + 4) / 6
+  w[2] = fmaf(u * u, fmaf(0.5f, u, -1.0f), 2.0f / 3.0f);  // B3(1 - t)
+  w[3] = t3 * (1.0f / 6.0f);
+}
+
+__device__ __forceinline__ float parzen_deriv_weight_t(float t, const float* prow) {
+  const float u = 1.0f - t;
+  const float d0 = 0.5f * u * u;
+  const float d1 = t * fmaf(-1.5f, t, 2.0f);
+  const float d2 = u * fmaf(1.5f, u, -2.0f);
+  const float d3 = -0.5f * t * t;
+  return fmaf(d0, prow[0], fmaf(d1, prow[1], fmaf(d2, prow[2], d3 * prow[3])));
+}
+
+// Fixed-point Parzen weights for either window: closed form when unclamped.
+__device__ __forceinline__ void parzen_fixed_fast(float arg0, unsigned int* q) {
+  if (arg0 <= -1.0f && arg0 > -2.0f) {
+    float w[4];
+    parzen_weights_t(-1.0f - arg0, w);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = (unsigned int)rintf(w[k] * 4294967296.0f);
+    q[1] = 0u - q[0] - q[2] - q[3];
+  } else {
+    parzen_fixed(arg0, q);
+  }
+}
+
+__device__ __forceinline__ float parzen_deriv_weight_fast(float arg0, const float* prow) {
+  if (arg0 <= -1.0f && arg0 > -2.0f) return parzen_deriv_weight_t(-1.0f - arg0, prow);
+  return parzen_deriv_weight(arg0, prow);
+}
+
 // Sum of N per-thread values over a 256-thread workgroup, written by thread 0
 // to out[0..N): wavefront shuffles, then the 4 wavefront results in LDS, in a
 // fixed order.  scratch holds 4 * N doubles.

@@ -61,21 +61,18 @@ __device__ __forceinline__ void fast_index(const FastArgs& f, uint32_t s, int32_
   }
 }
 
+// Bin of an FP32 Parzen term, clamped to [2, B-3] like the reference; sets
+// *uncertain when the reference's FP64 term could fall on the other side of
+// an unclamped bin edge (|term - n| <= bound for an edge n in [3, B-3]).
 __device__ __forceinline__ int fast_bin(float term, float bound, int B, bool* uncertain) {
-  if (term < 2.0f) {
-    *uncertain = *uncertain || (term >= 3.0f - bound);
-    return 2;
-  }
-  if (term >= (float)(B - 2)) {
-    *uncertain = *uncertain || (term < (float)(B - 3) + bound);
-    return B - 3;
-  }
   const float n = rintf(term);
-  if (n >= 3.0f && n <= (float)(B - 3) && fabsf(term - n) <= bound) *uncertain = true;
-  return (int)term;
+  *uncertain = *uncertain || ((fabsf(term - n) <= bound) & (n >= 3.0f) & (n <= (float)(B - 3)));
+  return (int)fminf(fmaxf(term, 2.0f), (float)(B - 3));
 }
 
-// 0: not counted, 1: counted (bins, arg0, cell set), 2: uncertain.
+// 0: not counted, 1: counted (bins, arg0, cell set), 2: uncertain.  Straight
+// line code: the cell is clamped into the buffer so every load is in bounds,
+// and the decisions are combined at the end.
 template <int D>
 __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* idx, int& fbin, int& mbin, float& arg0,
                                              Cell& cell) {
@@ -88,24 +85,22 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
     c = fma(f.A[k * D + 1], y, c);
     ci[k] = fma(f.A[k * D + 0], x, c);
   }
-  bool uncertain = false;
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    const double lo = (double)f.cilo[k], hi = (double)f.cihi[k];
-    if (ci[k] < lo - f.e_ci || ci[k] >= hi + f.e_ci) return 0;
-    uncertain = uncertain || (ci[k] < lo + f.e_ci) || (ci[k] >= hi - f.e_ci);
-  }
-  if (uncertain) return 2;
-  // moving cell (the reference's branch rules as duplicated corners)
+  bool out = false, uncertain = false;
   int32_t bb[3] = {0, 0, 0};
   bool act[3] = {false, false, false};
   cell.d[2] = 0.0;
 #pragma unroll
-  for (int j = 0; j < D; ++j) {
-    const int32_t fl = itk_floor(ci[j]);
-    bb[j] = fl > f.mstart[j] ? fl : f.mstart[j];
-    cell.d[j] = ci[j] - (double)bb[j];
-    act[j] = (cell.d[j] > 0.0) && (bb[j] + 1 <= f.mend[j]);
+  for (int k = 0; k < D; ++k) {
+    const double lo = (double)f.cilo[k], hi = (double)f.cihi[k];
+    out = out | (ci[k] < lo - f.e_ci) | (ci[k] >= hi + f.e_ci);
+    uncertain = uncertain | (ci[k] < lo + f.e_ci) | (ci[k] >= hi - f.e_ci);
+    // moving cell (the reference's branch rules as duplicated corners)
+    const double cc = fmin(fmax(ci[k], lo), hi);
+    int32_t fl = itk_floor(cc);
+    fl = fl < f.mstart[k] ? f.mstart[k] : (fl > f.mend[k] ? f.mend[k] : fl);
+    bb[k] = fl;
+    cell.d[k] = ci[k] - (double)fl;
+    act[k] = (cell.d[k] > 0.0) & (fl + 1 <= f.mend[k]);
   }
   cell.base = (bb[0] - f.mstart[0]) + f.msy * (bb[1] - f.mstart[1]) + (D == 3 ? f.msz * (bb[2] - f.mstart[2]) : 0);
   cell.ox = act[0] ? 1 : 0;
@@ -126,8 +121,8 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
     const float vxx1 = fmaf(vx11 - vx01, t1, vx01);
     v = fmaf(vxx1 - v, t2, v);
   }
-  if (v < f.mmin - f.ev || v > f.mmax + f.ev) return 0;
-  if (v < f.mmin + f.ev || v > f.mmax - f.ev) return 2;
+  out = out | (v < f.mmin - f.ev) | (v > f.mmax + f.ev);
+  uncertain = uncertain | (v < f.mmin + f.ev) | (v > f.mmax - f.ev);
   const float mterm = fmaf(v, f.inv_mbs, -f.mnm);
   mbin = fast_bin(mterm, f.bt_m, f.bins, &uncertain);
   arg0 = (float)(mbin - 1) - mterm;
@@ -136,7 +131,7 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
   const float fv = f.fixed_px[fl];
   const float fterm = fmaf(fv, f.inv_fbs, -f.fnm);
   fbin = fast_bin(fterm, f.bt_f, f.bins, &uncertain);
-  return uncertain ? 2 : 1;
+  return out ? 0 : (uncertain ? 2 : 1);
 }
 
 template <int D>
@@ -173,7 +168,7 @@ __global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f) {
     ++nvalid;
     atomicAdd(&fmarg[fbin], 1u);
     unsigned int q[4];
-    parzen_fixed(arg0, q);
+    parzen_fixed_fast(arg0, q);
     unsigned long long* row = hist + fbin * B + (mbin - 1);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -262,7 +257,7 @@ __global__ void __launch_bounds__(256) pass2_fast_kernel(FastArgs f) {
     float arg0 = 0.0f;
     Cell cell;
     if (fast_classify<D>(f, idx, fbin, mbin, arg0, cell) != 1) continue;
-    const float w = parzen_deriv_weight(arg0, pr + fbin * B + (mbin - 1));
+    const float w = parzen_deriv_weight_fast(arg0, pr + fbin * B + (mbin - 1));
     if (w == 0.0f) continue;
     add_moments<D>(acc, w, interp_gradient3<D>(f.texels, cell), idx);
   }
