@@ -269,6 +269,7 @@ struct mmi_context {
       scratch;
   DevBuf exc_count, exc_list, exc_overflow;  // fast-path exception lists
   int xcd_chunks = 1;                        // fast path: one contiguous sample share per XCD
+  int blocks_per_cu = 4;                     // evaluation grid = CUs x blocks_per_cu workgroups of 256
   int grid = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float last_total_ms = 0, last_pass1_ms = 0, last_pass2_ms = 0;
@@ -548,6 +549,7 @@ int mmi_context_create(int device, uint32_t flags, mmi_context** out) {
   ctx->device = device;
   ctx->flags = flags;
   if (const char* e = std::getenv("MMI_XCD_CHUNKS")) ctx->xcd_chunks = std::atoi(e) != 0;
+  if (const char* e = std::getenv("MMI_BLOCKS_PER_CU")) ctx->blocks_per_cu = std::max(1, std::min(64, std::atoi(e)));
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete ctx;
@@ -764,7 +766,7 @@ int mmi_initialize(mmi_context* ctx) {
   MMI_HIP(ctx->result.ensure(sizeof(DevResult)));
   const int64_t n = num_pixels(ctx->virtual_g);
   // a multiple of 8: the fast kernels split the samples into one share per XCD
-  const int64_t want = std::min<int64_t>((n + 255) / 256, (int64_t)ctx->cu_count * 4);
+  const int64_t want = std::min<int64_t>((n + 255) / 256, (int64_t)ctx->cu_count * ctx->blocks_per_cu);
   ctx->grid = (int)std::max<int64_t>(8, (want + 7) / 8 * 8);
   if (ctx->flags & 1u) {
     MMI_HIP(ctx->sample_fbin.ensure(sizeof(int16_t) * n));
