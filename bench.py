@@ -8,6 +8,9 @@ resident in HBM before timing starts; each step sets new transform
 parameters (as an optimizer iteration does) and returns value and the
 12-entry derivative to the host.
 
+--workload bspline measures the 192^3 cubic B-spline case instead (mesh 16^3,
+19^3 x 3 = 20,577 parameters, moving image warped by an 8^3-mesh B-spline).
+
 Prints one JSON line.  --gpus above 1 is rejected: this build measures one GPU.
 """
 from __future__ import annotations
@@ -25,6 +28,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_SAMPLE_F32_3D = 20  # fixed 4 + moving 4 + gradient 3 x 4 (one compulsory read each)
+# HBM bytes per evaluation (FETCH_SIZE + WRITE_SIZE summed over the evaluation's
+# kernels, scaled by the counters' reading for a copy of known size), from the
+# rocprofv3 --pmc runs summarised in profiles/.  None where not yet profiled.
+TRAFFIC_BYTES_PER_EVAL = {}
 
 
 def parse():
@@ -32,33 +39,80 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--workload", choices=("affine", "bspline"), default="affine")
+    ap.add_argument("--size", type=int, default=0, help="cube edge (default 256 affine, 192 B-spline)")
+    ap.add_argument("--mesh", type=int, default=16, help="B-spline mesh size per axis")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
 
 
-def cpu_baseline(problem, tf, seconds: float):
+class Workload:
+    """Problem, transform and the parameter sets the timed steps cycle through."""
+
+    def __init__(self, args):
+        from itk_amd import synth
+        from itk_amd.transforms import AffineTransform, BSplineTransform
+
+        rng = np.random.default_rng(args.seed)
+        self.kind = args.workload
+        t0 = time.perf_counter()
+        if args.workload == "affine":
+            s = args.size or 256
+            self.problem = synth.make_affine_problem((s, s, s), args.seed)
+            geom = self.problem.geometry
+            self.tf = AffineTransform(3, center=geom.index_to_point([(x - 1) / 2.0 for x in geom.size]))
+            base = self.tf.get_parameters()
+            self.params = [base + np.concatenate([rng.uniform(-0.02, 0.02, 9), rng.uniform(-1.5, 1.5, 3)])
+                           for _ in range(16)]
+            self.name = "mattes_mi_dense_value_and_derivative_affine_%dcube" % s
+            self.extra = {}
+        else:
+            s = args.size or 192
+            self.problem = synth.make_bspline_problem((s, s, s), args.seed, mesh=8, sigma_vox=2.0)
+            geom = self.problem.geometry
+            self.tf = BSplineTransform.for_image(geom, [args.mesh] * 3)
+            P = self.tf.number_of_parameters
+            self.params = [rng.normal(0.0, 0.5, P) for _ in range(4)]
+            self.name = "mattes_mi_dense_value_and_derivative_bspline_%dcube_mesh%d" % (s, args.mesh)
+            self.extra = {"parameters": P}
+        self.size = s
+        print("# synthesized %s in %.1f s" % (self.name, time.perf_counter() - t0), file=sys.stderr, flush=True)
+
+
+def cpu_baseline(wl: Workload, seconds: float):
     """The oracle (CPU FP64 restatement of the reference path, multithreaded
-    like the reference's slab threader) on the same 256^3 evaluation."""
+    like the reference's slab threader) on the same evaluation.  The B-spline
+    case is sampled on a slab of the virtual domain (its per-sample cost is the
+    whole cost) and scaled to full evaluations."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
+    from itk_amd.geometry import ImageGeometry
 
+    problem, tf = wl.problem, wl.tf
+    geom = problem.geometry
     threads = min(16, os.cpu_count() or 1)
-    grad = oracle.gradient_image(problem.moving, problem.geometry, threads=threads)
+    grad = oracle.gradient_image(problem.moving, geom, threads=threads)
+    virtual, scale, what = geom, 1.0, "the full %d^3 problem" % wl.size
+    if wl.kind == "bspline":
+        nz = max(2, geom.size[2] // 16)
+        virtual = ImageGeometry(size=[geom.size[0], geom.size[1], nz], origin=geom.origin, spacing=geom.spacing,
+                                direction=geom.direction, start=geom.start)
+        scale = nz / float(geom.size[2])
+        what = "a %d-slice slab (%.1f%% of the samples) of the %d^3 problem" % (nz, 100 * scale, wl.size)
     n = 0
     t0 = time.perf_counter()
     while True:
-        oracle.mattes(problem.fixed, problem.geometry, problem.moving, problem.geometry, tf, moving_gradient=grad,
-                      threads=threads)
+        oracle.mattes(problem.fixed, geom, problem.moving, geom, tf, moving_gradient=grad, threads=threads,
+                      virtual_geom=virtual)
         n += 1
         el = time.perf_counter() - t0
         if el >= seconds or n >= 50:
             break
-    return {"value": n / el, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": "%d full 256^3 GetValueAndDerivative evaluation(s) of the same problem in %.1f s "
-                      "(oracle, %d threads, slab partition)" % (n, el, threads)}
+    return {"value": n / el * scale, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": "%d GetValueAndDerivative evaluation(s) over %s in %.1f s, scaled to full "
+                      "evaluations (oracle, %d threads, slab partition)" % (n, what, el, threads)}
 
 
 def main():
@@ -66,21 +120,11 @@ def main():
     if args.gpus != 1:
         print("bench.py measures one GPU only (this build has no multi-GPU path); run with --gpus 1", file=sys.stderr)
         return 2
-    from itk_amd import synth
     from itk_amd.metric import MattesMutualInformationImageToImageMetricv4
-    from itk_amd.transforms import AffineTransform
 
-    s = args.size
-    t0 = time.perf_counter()
-    problem = synth.make_affine_problem((s, s, s), args.seed)
+    wl = Workload(args)
+    problem, tf = wl.problem, wl.tf
     geom = problem.geometry
-    print("# synthesized %d^3 problem in %.1f s" % (s, time.perf_counter() - t0), file=sys.stderr, flush=True)
-    tf = AffineTransform(3, center=geom.index_to_point([(x - 1) / 2.0 for x in geom.size]))
-    base = tf.get_parameters()
-    rng = np.random.default_rng(args.seed)
-    params = [base + np.concatenate([rng.uniform(-0.02, 0.02, 9), rng.uniform(-1.5, 1.5, 3)])
-              for _ in range(16)]
-
     m = MattesMutualInformationImageToImageMetricv4(device=0)
     m.SetFixedImage(problem.fixed, geom)
     m.SetMovingImage(problem.moving, geom)
@@ -92,12 +136,12 @@ def main():
           flush=True)
 
     for i in range(args.warmup):
-        tf.set_parameters(params[i % len(params)])
+        tf.set_parameters(wl.params[i % len(wl.params)])
         m.GetValueAndDerivative()
     t_pass1 = t_pass2 = t_total = 0.0
     t0 = time.perf_counter()
     for i in range(args.steps):
-        tf.set_parameters(params[i % len(params)])
+        tf.set_parameters(wl.params[i % len(wl.params)])
         v, d = m.GetValueAndDerivative()
         tm = m.GetLastTiming()
         t_total += tm["total_ms"]
@@ -123,11 +167,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 images, f64 coordinates/PDF",
         "data": "synthetic (seeded blob volumes, see itk_amd/synth.py)",
-        "config": {"workload": "mattes_mi_dense_value_and_derivative_affine_%dcube" % s, "bins": 50,
-                   "samples_per_eval": n, "gradient_source": "recursive Gaussian gradient image (device)",
-                   "parallelism": "single GPU"},
+        "config": dict({"workload": wl.name, "bins": 50, "samples_per_eval": n,
+                        "gradient_source": "recursive Gaussian gradient image (device)",
+                        "parallelism": "single GPU"}, **wl.extra),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES_PER_EVAL.get(wl.name),
                      "kernel": "whole evaluation (pass1 + finalize + pass2), HIP events on the metric stream",
                      "algorithmic_bytes_per_eval": alg_bytes, "device_ms_per_eval": dev_ms,
                      "pass1_ms": t_pass1 / K, "pass2_ms": t_pass2 / K},
@@ -135,7 +179,7 @@ def main():
     }
     m.close()
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(problem, tf, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -268,6 +268,11 @@ struct mmi_context {
   DevBuf hist, fmarg, nvalid, pdf, fm, mm, pratio, pratio_f, rowsum, result, partials, deriv, sample_fbin, sample_mbin,
       scratch;
   DevBuf exc_count, exc_list, exc_overflow;  // fast-path exception lists
+  // separable B-spline derivative: per-sample factors, contraction stages, axis tables
+  DevBuf bs_g, bs_t1, bs_t2;
+  DevBuf bs_start[3], bs_w[3], bs_inside[3], bs_lo[3], bs_hi[3];
+  bool bs_tables = false;
+  DevGeom bs_key_virt{}, bs_key_grid{};
   int xcd_chunks = 1;                        // fast path: one contiguous sample share per XCD
   int blocks_per_cu = 4;                     // evaluation grid = CUs x blocks_per_cu workgroups of 256
   int grid = 0;
@@ -533,9 +538,64 @@ int device_minmax(mmi_context* ctx, const float* px, int64_t n, double* lo, doub
   return MMI_OK;
 }
 
+// Axis tables of the separable B-spline derivative for this virtual domain and
+// grid, rebuilt when either changes (they do not depend on the coefficients).
+int bspline_axis_tables(mmi_context* ctx, const DevGeom& virt, const DevGeom& grid, int D, AxisTable* tabs) {
+  const bool same = ctx->bs_tables && std::memcmp(&ctx->bs_key_virt, &virt, sizeof(DevGeom)) == 0 &&
+                    std::memcmp(&ctx->bs_key_grid, &grid, sizeof(DevGeom)) == 0;
+  if (!same) {
+    ctx->bs_tables = false;
+    for (int j = 0; j < D; ++j) {
+      const int len = virt.size[j], nodes = grid.size[j];
+      MMI_HIP(ctx->bs_start[j].ensure(sizeof(int32_t) * len));
+      MMI_HIP(ctx->bs_w[j].ensure(sizeof(double) * 4 * len));
+      MMI_HIP(ctx->bs_inside[j].ensure(len));
+      MMI_HIP(ctx->bs_lo[j].ensure(sizeof(int32_t) * nodes));
+      MMI_HIP(ctx->bs_hi[j].ensure(sizeof(int32_t) * nodes));
+      MMI_HIP(launch_bspline_axis_table(virt, grid, j, ctx->bs_start[j].as<int32_t>(), ctx->bs_w[j].as<double>(),
+                                        ctx->bs_inside[j].as<uint8_t>(), ctx->stream));
+      std::vector<int32_t> start(len);
+      std::vector<uint8_t> inside(len);
+      MMI_HIP(hipMemcpyAsync(start.data(), ctx->bs_start[j].p, sizeof(int32_t) * len, hipMemcpyDeviceToHost,
+                             ctx->stream));
+      MMI_HIP(hipMemcpyAsync(inside.data(), ctx->bs_inside[j].p, len, hipMemcpyDeviceToHost, ctx->stream));
+      MMI_HIP(hipStreamSynchronize(ctx->stream));
+      std::vector<int32_t> lo(nodes, 0), hi(nodes, -1);
+      std::vector<bool> seen(nodes, false);
+      for (int i = 0; i < len; ++i) {
+        if (!inside[i]) continue;
+        for (int k = 0; k < 4; ++k) {
+          const int g = start[i] + k;
+          if (g < 0 || g >= nodes) continue;
+          if (!seen[g]) {
+            lo[g] = i;
+            seen[g] = true;
+          }
+          hi[g] = i;
+        }
+      }
+      MMI_HIP(hipMemcpyAsync(ctx->bs_lo[j].p, lo.data(), sizeof(int32_t) * nodes, hipMemcpyHostToDevice, ctx->stream));
+      MMI_HIP(hipMemcpyAsync(ctx->bs_hi[j].p, hi.data(), sizeof(int32_t) * nodes, hipMemcpyHostToDevice, ctx->stream));
+      MMI_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    ctx->bs_key_virt = virt;
+    ctx->bs_key_grid = grid;
+    ctx->bs_tables = true;
+  }
+  for (int j = 0; j < D; ++j) {
+    tabs[j].start = ctx->bs_start[j].as<int32_t>();
+    tabs[j].w = ctx->bs_w[j].as<double>();
+    tabs[j].inside = ctx->bs_inside[j].as<uint8_t>();
+    tabs[j].lo = ctx->bs_lo[j].as<int32_t>();
+    tabs[j].hi = ctx->bs_hi[j].as<int32_t>();
+  }
+  return MMI_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
 
 const char* mmi_last_error(void) { return g_last_error.c_str(); }
 
@@ -575,8 +635,15 @@ int mmi_context_destroy(mmi_context* ctx) {
   DevBuf* bufs[] = {&ctx->fixed_px, &ctx->moving_px, &ctx->texels, &ctx->coeffs, &ctx->hist, &ctx->fmarg,
                     &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->result,
                     &ctx->partials, &ctx->deriv, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
-                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_overflow};
+                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_overflow, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2};
   for (DevBuf* b : bufs) b->release();
+  for (int j = 0; j < 3; ++j) {
+    ctx->bs_start[j].release();
+    ctx->bs_w[j].release();
+    ctx->bs_inside[j].release();
+    ctx->bs_lo[j].release();
+    ctx->bs_hi[j].release();
+  }
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -911,24 +978,52 @@ retry:
       fa.partials = ctx->partials.as<double>();
       MMI_HIP(launch_fast_pass2(fa, D, ctx->grid, st));
       MMI_HIP(launch_fast_exceptions2(a, fa, ctx->grid, st));
-      MMI_HIP(hipEventRecord(ctx->ev[3], st));
       FastMap map;
       std::memset(&map, 0, sizeof(map));
       for (int i = 0; i < D * D; ++i) map.V[i] = ctx->virtual_g.index_to_physical[i];
       for (int i = 0; i < D; ++i) map.c0[i] = ctx->virtual_g.origin[i] - ctx->dmtf.center[i];
       MMI_HIP(launch_fast_derivative(fa.partials, 2 * ctx->grid, map, D, f.result, ctx->deriv.as<double>(), st));
+      MMI_HIP(hipEventRecord(ctx->ev[3], st));
     } else if (ctx->mtf.kind == MMI_TF_AFFINE) {
       MMI_HIP(ctx->partials.ensure(sizeof(double) * P * ctx->grid));
       a.partials = ctx->partials.as<double>();
       MMI_HIP(launch_pass2(a, ctx->grid, st));
-      MMI_HIP(hipEventRecord(ctx->ev[3], st));
       MMI_HIP(launch_derivative_reduce(a.partials, ctx->grid, (int)P, f.result, ctx->deriv.as<double>(), st));
+      MMI_HIP(hipEventRecord(ctx->ev[3], st));
+    } else if (is_diagonal(ctx->virtual_g.index_to_physical, D) &&
+               is_diagonal(ctx->dmtf.grid.p2i, D) && D == ctx->dmtf.grid.dim) {
+      // Separable geometry: the B-spline Jacobian weight of node (a, b, c) at
+      // virtual index (x, y, z) is wx[x][a] wy[y][b] wz[z][c], so the
+      // derivative is a tensor-product contraction of the per-sample factors,
+      // done axis by axis without atomics.
+      AxisTable tabs[3];
+      std::memset(tabs, 0, sizeof(tabs));
+      if (int rc = bspline_axis_tables(ctx, a.virt, ctx->dmtf.grid, D, tabs)) return rc;
+      MMI_HIP(ctx->bs_g.ensure(sizeof(float) * D * n));
+      a.bspline_g = ctx->bs_g.as<float>();
+      MMI_HIP(launch_pass2(a, ctx->grid, st));
+      const int64_t Nx = a.virt.size[0], Ny = a.virt.size[1], Nz = D == 3 ? a.virt.size[2] : 1;
+      const int Gx = ctx->dmtf.grid.size[0], Gy = ctx->dmtf.grid.size[1], Gz = D == 3 ? ctx->dmtf.grid.size[2] : 1;
+      MMI_HIP(ctx->bs_t1.ensure(sizeof(double) * D * Nz * Ny * Gx));
+      MMI_HIP(launch_separable_contract(a.bspline_g, nullptr, ctx->bs_t1.as<double>(), D * Nz * Ny, (int)Nx, 1, Gx,
+                                        tabs[0], nullptr, st));
+      if (D == 3) {
+        MMI_HIP(ctx->bs_t2.ensure(sizeof(double) * D * Nz * Gy * Gx));
+        MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t1.as<double>(), ctx->bs_t2.as<double>(), D * Nz, (int)Ny,
+                                          Gx, Gy, tabs[1], nullptr, st));
+        MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t2.as<double>(), ctx->deriv.as<double>(), D, (int)Nz,
+                                          (int64_t)Gy * Gx, Gz, tabs[2], f.result, st));
+      } else {
+        MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t1.as<double>(), ctx->deriv.as<double>(), D, (int)Ny, Gx,
+                                          Gy, tabs[1], f.result, st));
+      }
+      MMI_HIP(hipEventRecord(ctx->ev[3], st));
     } else {
       MMI_HIP(hipMemsetAsync(ctx->deriv.p, 0, sizeof(double) * P, st));
       a.deriv_accum = ctx->deriv.as<double>();
       MMI_HIP(launch_pass2(a, ctx->grid, st));
-      MMI_HIP(hipEventRecord(ctx->ev[3], st));
       MMI_HIP(launch_derivative_scale(ctx->deriv.as<double>(), P, f.result, st));
+      MMI_HIP(hipEventRecord(ctx->ev[3], st));
     }
   }
   DevResult r;

@@ -290,35 +290,49 @@ struct BSplineSupport {
   double w1[3][4];
 };
 
+// One axis of BSplineTransform::InsideValidRegion (itkBSplineTransform.hxx:478-506):
+// false when the continuous grid index c is outside [minLimit, maxLimit); c is
+// moved 6 ulp below maxLimit when it is within 4 ulp of it.  The region test
+// is the conjunction of the axes.
+__device__ __forceinline__ bool bspline_axis_inside(const DevGeom& grid, int j, double& c) {
+  const double minLimit = 0.5 * (double)(3 - 1);
+  const double maxLimit = (double)grid.size[j] - 0.5 * (double)(3 - 1) - 1.0;
+  if (float_almost_equal(c, maxLimit)) {
+    c = __longlong_as_double(__double_as_longlong(maxLimit) - 6);  // FloatAddULP(maxLimit, -6)
+    return true;
+  }
+  if (c >= maxLimit) return false;
+  if (c < minLimit) return false;
+  return true;
+}
+
+// One axis of BSplineInterpolationWeightFunction::Evaluate
+// (itkBSplineInterpolationWeightFunction.hxx:43-94): support start and the
+// four 1-D cubic weights.
+__device__ __forceinline__ int32_t bspline_axis_weights(double c, double* w) {
+  const int32_t start = itk_floor(c + 0.5 - 3 / 2.0);
+  double x = c - (double)start;
+#pragma unroll
+  for (int k = 0; k <= 3; ++k) {
+    w[k] = bspline3(x);
+    x -= 1.0;
+  }
+  return start;
+}
+
 template <int D>
 __device__ __forceinline__ void bspline_support(const DevGeom& grid, const double* p, BSplineSupport<D>& s) {
   double ci[3];
   point_to_cindex<D>(grid, p, ci);
   s.inside = true;
-  const double minLimit = 0.5 * (double)(3 - 1);
 #pragma unroll
   for (int j = 0; j < D; ++j) {
     if (!s.inside) break;
-    const double maxLimit = (double)grid.size[j] - 0.5 * (double)(3 - 1) - 1.0;
-    if (float_almost_equal(ci[j], maxLimit)) {
-      ci[j] = __longlong_as_double(__double_as_longlong(maxLimit) - 6);  // FloatAddULP(maxLimit, -6)
-    } else if (ci[j] >= maxLimit) {
-      s.inside = false;
-    } else if (ci[j] < minLimit) {
-      s.inside = false;
-    }
+    s.inside = bspline_axis_inside(grid, j, ci[j]);
   }
   if (!s.inside) return;
 #pragma unroll
-  for (int j = 0; j < D; ++j) {
-    s.start[j] = itk_floor(ci[j] + 0.5 - 3 / 2.0);
-    double x = ci[j] - (double)s.start[j];
-#pragma unroll
-    for (int k = 0; k <= 3; ++k) {
-      s.w1[j][k] = bspline3(x);
-      x -= 1.0;
-    }
-  }
+  for (int j = 0; j < D; ++j) s.start[j] = bspline_axis_weights(ci[j], s.w1[j]);
 }
 
 // Weight and node index of support point k (x fastest), as the reference's

@@ -198,7 +198,13 @@ __global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f) {
 // Exact FP64 evaluation of the listed samples of one pass-1 workgroup.
 template <int D>
 __global__ void __launch_bounds__(256) exceptions_pass1_kernel(EvalArgs a, FastArgs f) {
+  extern __shared__ unsigned int lds_u32[];
   const uint32_t ne = min(f.exc_count[blockIdx.x], (uint32_t)f.exc_cap);
+  if (ne == 0) return;  // uniform over the workgroup
+  const int B = a.bins;
+  unsigned int* fmarg = lds_u32;  // [B] fixed marginal counts, [B] valid count
+  for (int i = threadIdx.x; i <= B; i += blockDim.x) fmarg[i] = 0u;
+  __syncthreads();
   for (uint32_t t = threadIdx.x; t < ne; t += blockDim.x) {
     const uint32_t s = f.exc_list[blockIdx.x * f.exc_cap + t];
     int32_t idx[3];
@@ -210,8 +216,8 @@ __global__ void __launch_bounds__(256) exceptions_pass1_kernel(EvalArgs a, FastA
       a.sample_mbin[s] = ok ? (int16_t)o.mbin : (int16_t)-1;
     }
     if (!ok) continue;
-    atomicAdd(a.nvalid, 1ull);
-    atomicAdd(&a.fmarg[o.fbin], 1ull);
+    atomicAdd(&fmarg[B], 1u);
+    atomicAdd(&fmarg[o.fbin], 1u);
     unsigned int q[4];
     parzen_fixed((float)((double)(o.mbin - 1) - o.mterm), q);
     unsigned long long* row = a.hist + o.fbin * a.bins + (o.mbin - 1);
@@ -219,6 +225,11 @@ __global__ void __launch_bounds__(256) exceptions_pass1_kernel(EvalArgs a, FastA
     for (int k = 0; k < 4; ++k)
       if (q[k]) atomicAdd(&row[k], (unsigned long long)q[k]);
   }
+  __syncthreads();
+  // one global atomic per non-zero count of the workgroup
+  for (int i = threadIdx.x; i < B; i += blockDim.x)
+    if (fmarg[i]) atomicAdd(&a.fmarg[i], (unsigned long long)fmarg[i]);
+  if (threadIdx.x == 0 && fmarg[B]) atomicAdd(a.nvalid, (unsigned long long)fmarg[B]);
 }
 
 // Derivative moments in index space, per sample:
@@ -317,20 +328,17 @@ __global__ void __launch_bounds__(256) fast_derivative_kernel(const double* __re
                                                               FastMap map, const DevResult* __restrict__ result,
                                                               double* __restrict__ deriv) {
   constexpr int PA = D * D + D;
-  __shared__ double red[256];
+  __shared__ double scratch[4 * PA];
   __shared__ double mom[PA];
-  for (int mu = 0; mu < PA; ++mu) {
-    double s = 0.0;
-    for (int r = threadIdx.x; r < rows; r += blockDim.x) s += partials[(int64_t)r * PA + mu];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) mom[mu] = red[0];
-    __syncthreads();
+  double acc[PA];
+#pragma unroll
+  for (int mu = 0; mu < PA; ++mu) acc[mu] = 0.0;
+  for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+#pragma unroll
+    for (int mu = 0; mu < PA; ++mu) acc[mu] += partials[(int64_t)r * PA + mu];
   }
+  block_reduce_array<PA>(acc, scratch, mom);
+  __syncthreads();
   if (threadIdx.x == 0) {
     const double nf = result->n_factor;
     for (int b = 0; b < D; ++b) {
@@ -360,9 +368,9 @@ hipError_t launch_fast_pass1(const FastArgs& f, int dim, int grid, hipStream_t s
 
 hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st) {
   if (a.virt.dim == 2)
-    hipLaunchKernelGGL(exceptions_pass1_kernel<2>, dim3(grid), dim3(256), 0, st, a, f);
+    hipLaunchKernelGGL(exceptions_pass1_kernel<2>, dim3(grid), dim3(256), (a.bins + 1) * 4, st, a, f);
   else
-    hipLaunchKernelGGL(exceptions_pass1_kernel<3>, dim3(grid), dim3(256), 0, st, a, f);
+    hipLaunchKernelGGL(exceptions_pass1_kernel<3>, dim3(grid), dim3(256), (a.bins + 1) * 4, st, a, f);
   return hipGetLastError();
 }
 

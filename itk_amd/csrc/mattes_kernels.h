@@ -94,12 +94,26 @@ struct EvalArgs {
   int32_t pratio_in_lds;
   int32_t pad1_;
   double* partials;      // [grid * P] (affine)
-  double* deriv_accum;   // [P] (B-spline)
+  double* deriv_accum;   // [P] (B-spline, non-separable geometry)
+  float* bspline_g;      // [D][n] (B-spline, separable geometry): per-sample w * gradient, 0 if not counted
+};
+
+// Support of the cubic B-spline along one axis of a separable geometry (virtual
+// index_to_physical and grid physical_to_index both diagonal): for virtual
+// index i along the axis, its support start, the four 1-D weights and the
+// axis' part of the valid-region test; for grid node g, the virtual index range
+// [lo[g], hi[g]] whose support can hold g (lo > hi when empty).
+struct AxisTable {
+  const int32_t* start;  // [len]
+  const double* w;       // [len][4]
+  const uint8_t* inside; // [len]
+  const int32_t* lo;     // [nodes]
+  const int32_t* hi;     // [nodes]
 };
 
 struct FinalizeArgs {
   int32_t bins;
-  int32_t pad_;
+  int32_t lds;  // set by launch_finalize: PDF and marginals staged in LDS
   const unsigned long long* hist;
   const unsigned long long* fmarg;
   const unsigned long long* nvalid;
@@ -133,6 +147,12 @@ hipError_t launch_fast_derivative(const double* partials, int rows, const FastMa
 hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, const DevResult* r, double* deriv,
                                     hipStream_t st);
 hipError_t launch_derivative_scale(double* deriv, int64_t P, const DevResult* r, hipStream_t st);
+hipError_t launch_bspline_axis_table(const DevGeom& virt, const DevGeom& grid, int axis, int32_t* start, double* w,
+                                     uint8_t* inside, hipStream_t st);
+// out[o][g][i] = sum_l w[l][g - start[l]] * in[o][l][i] over the virtual
+// indices l whose support holds node g, times scale->n_factor when scale is set.
+hipError_t launch_separable_contract(const float* in_f, const double* in_d, double* out, int64_t outer, int L,
+                                     int64_t inner, int G, const AxisTable& t, const DevResult* scale, hipStream_t st);
 hipError_t launch_minmax(const float* px, int64_t n, float* out, int blocks, hipStream_t st);
 hipError_t launch_recursive_gaussian(const double* in, double* out, int64_t ln, int64_t stride, int64_t nlines,
                                      const RecCoeffs& c, hipStream_t st);

@@ -91,73 +91,100 @@ __global__ void __launch_bounds__(256) pass1_kernel(EvalArgs a) {
 // finalize: ComputeResults (Mattes:353-499) in FP64, one workgroup
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(256) finalize_kernel(FinalizeArgs f) {
+// One workgroup of kFinalizeThreads.  The joint PDF, marginals and log rows
+// live in LDS when they fit (f.lds != 0), else in the global output buffers.
+// Every sum has a fixed shape (per-thread strided sums, wavefront shuffle
+// tree, wavefront results in order), so the result does not depend on
+// scheduling.
+constexpr int kFinalizeThreads = 1024;
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+  return x;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+  return x;
+}
+
+__global__ void __launch_bounds__(kFinalizeThreads) finalize_kernel(FinalizeArgs f) {
+  extern __shared__ double lds_d[];
+  __shared__ unsigned long long redu[kFinalizeThreads / 64];
+  __shared__ double reds[kFinalizeThreads / 64];
   __shared__ double s_norm;
-  __shared__ unsigned long long redu[256];
-  __shared__ double reds[256];
   const int B = f.bins;
   const int BB = B * B;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6, nw = blockDim.x >> 6;
+  double* pdf = f.lds ? lds_d : f.pdf;
+  double* fm = f.lds ? lds_d + BB : f.fm;
+  double* mm = f.lds ? lds_d + BB + B : f.mm;
+  double* lrow = f.lds ? lds_d + BB + 2 * B : f.rowsum;
   // Joint PDF sum: exact in 64-bit integers.
   unsigned long long part = 0;
   for (int i = t; i < BB; i += blockDim.x) part += f.hist[i];
-  redu[t] = part;
+  part = wave_sum(part);
+  if (lane == 0) redu[wid] = part;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if (t < w) redu[t] += redu[t + w];
-    __syncthreads();
-  }
-  const unsigned long long total = redu[0];
-  const unsigned long long nvalid = *f.nvalid;
-  const double jointPDFSum = (double)total / kFixedPointScale;
   if (t == 0) {
+    unsigned long long total = 0;
+    for (int w = 0; w < nw; ++w) total += redu[w];
+    const unsigned long long nvalid = *f.nvalid;
+    const double jointPDFSum = (double)total / kFixedPointScale;
     f.result->n_valid = nvalid;
     f.result->joint_pdf_sum = jointPDFSum;
     f.result->status = (nvalid == 0) ? 1 : (jointPDFSum < 2.220446049250313e-16 ? 2 : 0);
-    s_norm = 1.0 / jointPDFSum;
+    s_norm = (nvalid == 0 || jointPDFSum < 2.220446049250313e-16) ? 0.0 : 1.0 / jointPDFSum;
   }
   __syncthreads();
-  if (nvalid == 0 || jointPDFSum < 2.220446049250313e-16) return;
   const double nf = s_norm;
+  if (nf == 0.0) return;
   const double closeToZero = 2.220446049250313e-16;
-  for (int i = t; i < BB; i += blockDim.x) f.pdf[i] = ((double)f.hist[i] / kFixedPointScale) * nf;
+  for (int i = t; i < BB; i += blockDim.x) {
+    const double v = ((double)f.hist[i] / kFixedPointScale) * nf;
+    pdf[i] = v;
+    if (f.lds) f.pdf[i] = v;
+  }
   for (int i = t; i < B; i += blockDim.x) {
     const double fmv = (double)f.fmarg[i] * nf;
-    f.fm[i] = fmv;
-    f.rowsum[i] = fmv > closeToZero ? log(fmv) : 0.0;  // log of the fixed marginal, once per row
+    fm[i] = fmv;
+    if (f.lds) f.fm[i] = fmv;
+    lrow[i] = fmv > closeToZero ? log(fmv) : 0.0;  // log of the fixed marginal, once per row
   }
   __syncthreads();
   // moving marginal: column sums in the reference's row order (Mattes:405-417)
   for (int m = t; m < B; m += blockDim.x) {
     double acc = 0.0;
-    for (int r = 0; r < B; ++r) acc += f.pdf[r * B + m];
-    f.mm[m] = acc;
+    for (int r = 0; r < B; ++r) acc += pdf[r * B + m];
+    mm[m] = acc;
+    if (f.lds) f.mm[m] = acc;
   }
   __syncthreads();
   // entropy terms and pRatio over all (f, m) pairs in parallel (Mattes:419-476)
   double part_sum = 0.0;
   for (int i = t; i < BB; i += blockDim.x) {
-    const int r = i / B, m = i - (i / B) * B;
-    const double fmv = f.fm[r];
-    const double mmv = f.mm[m];
-    const double jp = f.pdf[i];
+    const int r = i / B, m = i - r * B;
+    const double fmv = fm[r];
+    const double mmv = mm[m];
+    const double jp = pdf[i];
     double pr = 0.0;
     if (fmv > closeToZero && mmv > closeToZero && jp > closeToZero) {
       pr = log(jp / mmv);
-      part_sum += jp * (pr - f.rowsum[r]);
+      part_sum += jp * (pr - lrow[r]);
     }
     f.pratio[i] = pr;
     f.pratio_f[i] = (float)pr;
   }
-  reds[t] = part_sum;
+  part_sum = wave_sum(part_sum);
+  if (lane == 0) reds[wid] = part_sum;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if (t < w) reds[t] += reds[t + w];
-    __syncthreads();
-  }
   if (t == 0) {
-    f.result->value = -1.0 * reds[0];
-    f.result->n_factor = -1.0 / (f.moving_bin_size * (double)nvalid);
+    double sum = 0.0;
+    for (int w = 0; w < nw; ++w) sum += reds[w];
+    f.result->value = -1.0 * sum;
+    f.result->n_factor = -1.0 / (f.moving_bin_size * (double)*f.nvalid);
   }
 }
 
@@ -230,21 +257,28 @@ __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
     int32_t idx[3];
     sample_index<D>(a.virt, s, idx);
     SampleOut<D> o;
-    if (!eval_sample<D, MK>(a, idx, o)) continue;
-    const float w = parzen_deriv_weight((float)((double)(o.mbin - 1) - o.mterm), pr + o.fbin * B + (o.mbin - 1));
-    if (w == 0.0f) continue;
+    const bool ok = eval_sample<D, MK>(a, idx, o);
+    float w = 0.0f;
+    if (ok) w = parzen_deriv_weight((float)((double)(o.mbin - 1) - o.mterm), pr + o.fbin * B + (o.mbin - 1));
     double gd[3] = {0.0, 0.0, 0.0};
-    if (a.gradient_source == 1) {
-      const float3 g3 = interp_gradient3<D>(a.texels, o.cell);
-      gd[0] = (double)g3.x;
-      gd[1] = (double)g3.y;
-      gd[2] = (double)g3.z;
-    } else {
-      central_difference<D>(a, o.mp, gd);
+    if (w != 0.0f) {
+      if (a.gradient_source == 1) {
+        const float3 g3 = interp_gradient3<D>(a.texels, o.cell);
+        gd[0] = (double)g3.x;
+        gd[1] = (double)g3.y;
+        gd[2] = (double)g3.z;
+      } else {
+        central_difference<D>(a, o.mp, gd);
+      }
     }
     if constexpr (MK == 2) {
       // B-spline Jacobian: J[d][node + d*pdim] = weight(node) (BSplTf:594-650)
-      if (o.sup.inside) {
+      const bool use = w != 0.0f && o.sup.inside;
+      if (a.bspline_g) {
+        // separable geometry: per-sample factors, contracted axis by axis later
+#pragma unroll
+        for (int d = 0; d < D; ++d) a.bspline_g[(int64_t)d * n + s] = use ? (float)((double)w * gd[d]) : 0.0f;
+      } else if (use) {
         constexpr int NW = D == 3 ? 64 : 16;
         for (int k = 0; k < NW; ++k) {
           const double wk = (double)w * bspline_weight<D>(o.sup, k);
@@ -253,7 +287,7 @@ __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
           for (int d = 0; d < D; ++d) atomicAdd(&a.deriv_accum[node + d * a.mtf.pdim], wk * gd[d]);
         }
       }
-    } else {
+    } else if (w != 0.0f) {
       // affine Jacobian: J[b][b*D+d] = x_d - c_d, J[d][D*D+d] = 1 (MOTB:563-591)
 #pragma unroll
       for (int b = 0; b < D; ++b) {
@@ -299,7 +333,10 @@ hipError_t launch_pass2(const EvalArgs& a, int grid, hipStream_t st) {
 }
 
 hipError_t launch_finalize(const FinalizeArgs& f, hipStream_t st) {
-  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, st, f);
+  FinalizeArgs g = f;
+  const size_t lds = ((size_t)f.bins * f.bins + 3 * (size_t)f.bins) * sizeof(double);
+  g.lds = lds <= 48 * 1024 ? 1 : 0;
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kFinalizeThreads), g.lds ? lds : 0, st, g);
   return hipGetLastError();
 }
 

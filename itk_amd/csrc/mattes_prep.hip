@@ -175,6 +175,60 @@ __global__ void pack_texels_kernel(const float* __restrict__ moving, const doubl
   tex[i] = make_float4(moving[i], (float)o[0], (float)o[1], (float)o[2]);
 }
 
+// Per-axis B-spline support of the virtual samples for a separable geometry.
+// The grid continuous index along `axis` is computed from the full virtual
+// point with the reference's expressions; the other axes' terms are exact
+// zeros there, so it equals the per-sample value for every sample with this
+// index along the axis.
+__global__ void bspline_axis_table_kernel(DevGeom virt, DevGeom grid, int axis, int32_t* __restrict__ start,
+                                          double* __restrict__ w, uint8_t* __restrict__ inside) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= virt.size[axis]) return;
+  int32_t idx[3] = {virt.start[0], virt.start[1], virt.start[2]};
+  idx[axis] += i;
+  double p[3], ci[3];
+  if (virt.dim == 3) {
+    index_to_point<3>(virt, idx, p);
+    point_to_cindex<3>(grid, p, ci);
+  } else {
+    index_to_point<2>(virt, idx, p);
+    point_to_cindex<2>(grid, p, ci);
+  }
+  double c = ci[axis];
+  const bool in = bspline_axis_inside(grid, axis, c);
+  inside[i] = in ? 1 : 0;
+  double w4[4] = {0.0, 0.0, 0.0, 0.0};
+  start[i] = in ? bspline_axis_weights(c, w4) : 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) w[i * 4 + k] = w4[k];
+}
+
+// One axis of the tensor-product contraction of the B-spline derivative:
+// out[o][g][i] = sum over virtual indices l in [lo[g], hi[g]] with g in the
+// support of l of w[l][g - start[l]] * in[o][l][i].  Every output element is one
+// thread summing in index order, so the result does not depend on scheduling.
+template <typename T>
+__global__ void __launch_bounds__(256) separable_contract_kernel(const T* __restrict__ in, double* __restrict__ out,
+                                                                 int64_t total, int L, int64_t inner, int G,
+                                                                 AxisTable t, const DevResult* __restrict__ scale) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int64_t i = e % inner;
+  const int64_t r = e / inner;
+  const int g = (int)(r % G);
+  const int64_t o = r / G;
+  const T* src = in + o * L * inner + i;
+  double acc = 0.0;
+  const int hi = t.hi[g];
+  for (int l = t.lo[g]; l <= hi; ++l) {
+    const int k = g - t.start[l];
+    if (!t.inside[l] || k < 0 || k > 3) continue;
+    acc += t.w[l * 4 + k] * (double)src[(int64_t)l * inner];
+  }
+  if (scale) acc *= scale->n_factor;
+  out[e] = acc;
+}
+
 hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, const DevResult* r, double* deriv,
                                     hipStream_t st) {
   hipLaunchKernelGGL(derivative_reduce_kernel, dim3(P), dim3(256), 0, st, partials, nblocks, P, r, deriv);
@@ -214,6 +268,28 @@ hipError_t launch_pack_texels(const float* moving, const double* grad, int64_t n
                               float4* tex, hipStream_t st) {
   hipLaunchKernelGGL(pack_texels_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, moving, grad, n, D, dir,
                      tex);
+  return hipGetLastError();
+}
+
+hipError_t launch_bspline_axis_table(const DevGeom& virt, const DevGeom& grid, int axis, int32_t* start, double* w,
+                                     uint8_t* inside, hipStream_t st) {
+  const int n = virt.size[axis];
+  hipLaunchKernelGGL(bspline_axis_table_kernel, dim3((n + 127) / 128), dim3(128), 0, st, virt, grid, axis, start, w,
+                     inside);
+  return hipGetLastError();
+}
+
+hipError_t launch_separable_contract(const float* in_f, const double* in_d, double* out, int64_t outer, int L,
+                                     int64_t inner, int G, const AxisTable& t, const DevResult* scale, hipStream_t st) {
+  const int64_t total = outer * G * inner;
+  if (total <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((total + 255) / 256));
+  if (in_f)
+    hipLaunchKernelGGL(separable_contract_kernel<float>, grid, dim3(256), 0, st, in_f, out, total, L, inner, G, t,
+                       scale);
+  else
+    hipLaunchKernelGGL(separable_contract_kernel<double>, grid, dim3(256), 0, st, in_d, out, total, L, inner, G, t,
+                       scale);
   return hipGetLastError();
 }
 

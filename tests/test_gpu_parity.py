@@ -13,7 +13,7 @@ from itk_amd import synth
 from itk_amd._native import MetricError
 from itk_amd.geometry import ImageGeometry
 from itk_amd.metric import MattesMutualInformationImageToImageMetricv4 as Metric
-from itk_amd.transforms import AffineTransform
+from itk_amd.transforms import AffineTransform, BSplineTransform
 
 pytestmark = pytest.mark.gpu
 
@@ -141,6 +141,33 @@ def test_bspline_2d(oracle_mod):
     tf = bspline_for(p.geometry, (6, 5), seed=4, sigma=1.0)
     m = _gpu_metric(p.fixed, p.moving, p.geometry, tf, gradient=grad)
     _compare(oracle_mod, m, p.fixed, p.moving, p.geometry, tf, gradient=grad)
+
+
+def test_bspline_partial_domain_anisotropic(oracle_mod):
+    """Separable B-spline derivative where the transform domain covers part of
+    an anisotropic, offset virtual domain: samples outside the valid grid region
+    keep zero displacement and no Jacobian, per axis of the support tables."""
+    fixed, moving, _ = blob_problem((36, 30, 26), 21)
+    geom = ImageGeometry(size=(36, 30, 26), spacing=(0.9, 1.1, 1.7), origin=(-5.0, 2.5, 11.0))
+    virt = ImageGeometry(size=(28, 22, 14), start=(3, 4, 5), spacing=(1.0, 1.2, 1.9), origin=(-4.0, 3.0, 12.5))
+    grad = oracle_mod.gradient_image(moving, geom)
+    tf = BSplineTransform.from_domain(origin=(-1.0, 6.0, 16.0), physical_dimensions=(18.0, 17.0, 16.0),
+                                      mesh_size=(3, 4, 2))
+    tf.set_parameters(np.random.default_rng(22).normal(0.0, 0.7, tf.number_of_parameters))
+    m = _gpu_metric(fixed, moving, geom, tf, gradient=grad, virtual=virt)
+    _compare(oracle_mod, m, fixed, moving, geom, tf, gradient=grad, virtual=virt)
+
+
+def test_bspline_oblique_direction(oracle_mod):
+    """Rotated image and grid directions: the per-sample (non-separable)
+    B-spline derivative path."""
+    fixed, moving, _ = blob_problem((30, 28, 24), 23)
+    rot = synth.rotation_matrix([8.0, -5.0, 12.0])
+    geom = ImageGeometry(size=(30, 28, 24), spacing=(1.0, 1.1, 1.3), origin=(2.0, -3.0, 1.0), direction=rot)
+    grad = oracle_mod.gradient_image(moving, geom)
+    tf = bspline_for(geom, (3, 3, 3), seed=24, sigma=0.7)
+    m = _gpu_metric(fixed, moving, geom, tf, gradient=grad)
+    _compare(oracle_mod, m, fixed, moving, geom, tf, gradient=grad)
 
 
 def test_value_only_matches_value_and_derivative(oracle_mod):
