@@ -540,7 +540,9 @@ int device_minmax(mmi_context* ctx, const float* px, int64_t n, double* lo, doub
 
 // Axis tables of the separable B-spline derivative for this virtual domain and
 // grid, rebuilt when either changes (they do not depend on the coefficients).
-int bspline_axis_tables(mmi_context* ctx, const DevGeom& virt, const DevGeom& grid, int D, AxisTable* tabs) {
+int bspline_axis_tables(mmi_context* ctx, const DevGeom& virt, const DevGeom& grid, int D, AxisTable* tabs,
+                        bool* usable) {
+  *usable = false;
   const bool same = ctx->bs_tables && std::memcmp(&ctx->bs_key_virt, &virt, sizeof(DevGeom)) == 0 &&
                     std::memcmp(&ctx->bs_key_grid, &grid, sizeof(DevGeom)) == 0;
   if (!same) {
@@ -574,6 +576,12 @@ int bspline_axis_tables(mmi_context* ctx, const DevGeom& virt, const DevGeom& gr
           hi[g] = i;
         }
       }
+      // The contraction kernels sum every index of [lo, hi]: each must hold g
+      // in its support (true when the support start is monotone along the axis
+      // and the valid indices form one interval, as for any axis-aligned grid).
+      for (int g = 0; g < nodes; ++g)
+        for (int i = lo[g]; i <= hi[g]; ++i)
+          if (!inside[i] || g < start[i] || g > start[i] + 3) return MMI_OK;  // not usable: per-sample path
       MMI_HIP(hipMemcpyAsync(ctx->bs_lo[j].p, lo.data(), sizeof(int32_t) * nodes, hipMemcpyHostToDevice, ctx->stream));
       MMI_HIP(hipMemcpyAsync(ctx->bs_hi[j].p, hi.data(), sizeof(int32_t) * nodes, hipMemcpyHostToDevice, ctx->stream));
       MMI_HIP(hipStreamSynchronize(ctx->stream));
@@ -582,6 +590,7 @@ int bspline_axis_tables(mmi_context* ctx, const DevGeom& virt, const DevGeom& gr
     ctx->bs_key_grid = grid;
     ctx->bs_tables = true;
   }
+  *usable = true;
   for (int j = 0; j < D; ++j) {
     tabs[j].start = ctx->bs_start[j].as<int32_t>();
     tabs[j].w = ctx->bs_w[j].as<double>();
@@ -590,6 +599,15 @@ int bspline_axis_tables(mmi_context* ctx, const DevGeom& virt, const DevGeom& gr
     tabs[j].hi = ctx->bs_hi[j].as<int32_t>();
   }
   return MMI_OK;
+}
+
+// Whether the B-spline derivative of this evaluation can be contracted axis by
+// axis: axis-aligned virtual domain and grid, monotone supports.
+int bspline_separable(mmi_context* ctx, const DevGeom& virt, int D, AxisTable* tabs, bool* usable) {
+  *usable = false;
+  if (ctx->mtf.kind != MMI_TF_BSPLINE || ctx->dmtf.grid.dim != D) return MMI_OK;
+  if (!is_diagonal(ctx->virtual_g.index_to_physical, D) || !is_diagonal(ctx->dmtf.grid.p2i, D)) return MMI_OK;
+  return bspline_axis_tables(ctx, virt, ctx->dmtf.grid, D, tabs, usable);
 }
 
 }  // namespace
@@ -970,6 +988,11 @@ retry:
   f.moving_bin_size = ctx->moving_bin_size;
   MMI_HIP(launch_finalize(f, st));
 
+  AxisTable tabs[3];
+  std::memset(tabs, 0, sizeof(tabs));
+  bool separable = false;
+  if (want_derivative && !fast && ctx->mtf.kind == MMI_TF_BSPLINE)
+    if (int rc = bspline_separable(ctx, a.virt, D, tabs, &separable)) return rc;
   if (want_derivative) {
     MMI_HIP(ctx->deriv.ensure(sizeof(double) * P));
     MMI_HIP(hipEventRecord(ctx->ev[2], st));
@@ -990,15 +1013,11 @@ retry:
       MMI_HIP(launch_pass2(a, ctx->grid, st));
       MMI_HIP(launch_derivative_reduce(a.partials, ctx->grid, (int)P, f.result, ctx->deriv.as<double>(), st));
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
-    } else if (is_diagonal(ctx->virtual_g.index_to_physical, D) &&
-               is_diagonal(ctx->dmtf.grid.p2i, D) && D == ctx->dmtf.grid.dim) {
+    } else if (separable) {
       // Separable geometry: the B-spline Jacobian weight of node (a, b, c) at
       // virtual index (x, y, z) is wx[x][a] wy[y][b] wz[z][c], so the
       // derivative is a tensor-product contraction of the per-sample factors,
       // done axis by axis without atomics.
-      AxisTable tabs[3];
-      std::memset(tabs, 0, sizeof(tabs));
-      if (int rc = bspline_axis_tables(ctx, a.virt, ctx->dmtf.grid, D, tabs)) return rc;
       MMI_HIP(ctx->bs_g.ensure(sizeof(float) * D * n));
       a.bspline_g = ctx->bs_g.as<float>();
       MMI_HIP(launch_pass2(a, ctx->grid, st));

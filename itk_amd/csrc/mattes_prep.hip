@@ -6,6 +6,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 namespace mmi {
 
 // Deterministic sum of per-workgroup partials, then the nFactor scaling.
@@ -204,9 +206,11 @@ __global__ void bspline_axis_table_kernel(DevGeom virt, DevGeom grid, int axis, 
 }
 
 // One axis of the tensor-product contraction of the B-spline derivative:
-// out[o][g][i] = sum over virtual indices l in [lo[g], hi[g]] with g in the
-// support of l of w[l][g - start[l]] * in[o][l][i].  Every output element is one
-// thread summing in index order, so the result does not depend on scheduling.
+// out[o][g][i] = sum over virtual indices l in [lo[g], hi[g]] of
+// w[l][g - start[l]] * in[o][l][i].  The support start is monotone in l and the
+// valid indices along an axis are one interval, so every l of the range holds
+// g in its support.  Every output element is one thread summing in index
+// order, so the result does not depend on scheduling.
 template <typename T>
 __global__ void __launch_bounds__(256) separable_contract_kernel(const T* __restrict__ in, double* __restrict__ out,
                                                                  int64_t total, int L, int64_t inner, int G,
@@ -220,13 +224,51 @@ __global__ void __launch_bounds__(256) separable_contract_kernel(const T* __rest
   const T* src = in + o * L * inner + i;
   double acc = 0.0;
   const int hi = t.hi[g];
-  for (int l = t.lo[g]; l <= hi; ++l) {
-    const int k = g - t.start[l];
-    if (!t.inside[l] || k < 0 || k > 3) continue;
-    acc += t.w[l * 4 + k] * (double)src[(int64_t)l * inner];
-  }
+#pragma unroll 4
+  for (int l = t.lo[g]; l <= hi; ++l) acc += t.w[l * 4 + (g - t.start[l])] * (double)src[(int64_t)l * inner];
   if (scale) acc *= scale->n_factor;
   out[e] = acc;
+}
+
+// The first (x) axis of the contraction.  Persistent workgroups of four
+// wavefronts take four rows at a time; the axis table and the rows are staged
+// in LDS (coalesced loads), then each output node sums its range from LDS in
+// index order.
+constexpr int kRowThreads = 256;
+constexpr int kRowsPerStep = kRowThreads / 64;
+
+__global__ void __launch_bounds__(kRowThreads) separable_rows_kernel(const float* __restrict__ in,
+                                                                     double* __restrict__ out, int64_t rows, int L,
+                                                                     int G, AxisTable t) {
+  extern __shared__ double lds_rows[];
+  double* w = lds_rows;                                       // [L][4]
+  int32_t* start = reinterpret_cast<int32_t*>(w + 4 * L);     // [L]
+  int32_t* lo = start + L;                                    // [G]
+  int32_t* hi = lo + G;                                       // [G]
+  float* row = reinterpret_cast<float*>(hi + G);              // [kRowsPerStep][L]
+  for (int i = threadIdx.x; i < 4 * L; i += kRowThreads) w[i] = t.w[i];
+  for (int i = threadIdx.x; i < L; i += kRowThreads) start[i] = t.start[i];
+  for (int i = threadIdx.x; i < G; i += kRowThreads) {
+    lo[i] = t.lo[i];
+    hi[i] = t.hi[i];
+  }
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int64_t r0 = (int64_t)blockIdx.x * kRowsPerStep; r0 < rows; r0 += (int64_t)gridDim.x * kRowsPerStep) {
+    __syncthreads();  // previous rows consumed (and the tables staged, first time)
+    const int64_t nr = min((int64_t)kRowsPerStep, rows - r0);
+    for (int i = threadIdx.x; i < nr * L; i += kRowThreads) row[i] = in[r0 * L + i];
+    __syncthreads();
+    const int64_t r = r0 + wid;
+    if (wid < nr) {
+      const float* x = row + wid * L;
+      for (int g = lane; g < G; g += 64) {
+        double acc = 0.0;
+        const int h = hi[g];
+        for (int l = lo[g]; l <= h; ++l) acc += w[l * 4 + (g - start[l])] * (double)x[l];
+        out[r * G + g] = acc;
+      }
+    }
+  }
 }
 
 hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, const DevResult* r, double* deriv,
@@ -284,7 +326,13 @@ hipError_t launch_separable_contract(const float* in_f, const double* in_d, doub
   const int64_t total = outer * G * inner;
   if (total <= 0) return hipSuccess;
   const dim3 grid((unsigned)((total + 255) / 256));
-  if (in_f)
+  const size_t row_lds = (size_t)L * (4 * sizeof(double) + sizeof(int32_t) + kRowsPerStep * sizeof(float)) +
+                         (size_t)G * 2 * sizeof(int32_t);
+  if (in_f && inner == 1 && !scale && row_lds <= 48 * 1024) {
+    const int64_t steps = (outer + kRowsPerStep - 1) / kRowsPerStep;
+    const unsigned blocks = (unsigned)std::min<int64_t>(steps, 2048);
+    hipLaunchKernelGGL(separable_rows_kernel, dim3(blocks), dim3(kRowThreads), row_lds, st, in_f, out, outer, L, G, t);
+  } else if (in_f)
     hipLaunchKernelGGL(separable_contract_kernel<float>, grid, dim3(256), 0, st, in_f, out, total, L, inner, G, t,
                        scale);
   else
