@@ -27,7 +27,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_SAMPLE_F32_3D = 20  # fixed 4 + moving 4 + gradient 3 x 4 (one compulsory read each)
+# One compulsory read each of the fixed, moving and gradient (3 components)
+# images per sample: 4 + 4 + 12 bytes in FP32 storage, half in BF16/FP16.
+BYTES_PER_SAMPLE_3D = {"f32": 20, "bf16": 10, "f16": 10}
 # HBM bytes per evaluation (FETCH_SIZE + WRITE_SIZE summed over the evaluation's
 # kernels, scaled by the counters' reading for a copy of known size), from the
 # rocprofv3 --pmc runs summarised in profiles/.  None where not yet profiled.
@@ -42,6 +44,8 @@ def parse():
     ap.add_argument("--workload", choices=("affine", "bspline"), default="affine")
     ap.add_argument("--size", type=int, default=0, help="cube edge (default 256 affine, 192 B-spline)")
     ap.add_argument("--mesh", type=int, default=16, help="B-spline mesh size per axis")
+    ap.add_argument("--storage", choices=("f32", "bf16", "f16"), default="f32",
+                    help="device storage of the fixed, moving and gradient images")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -126,6 +130,7 @@ def main():
     problem, tf = wl.problem, wl.tf
     geom = problem.geometry
     m = MattesMutualInformationImageToImageMetricv4(device=0)
+    m.SetImageStorage(args.storage)
     m.SetFixedImage(problem.fixed, geom)
     m.SetMovingImage(problem.moving, geom)
     m.SetNumberOfHistogramBins(50)
@@ -151,7 +156,7 @@ def main():
     K = args.steps
     ms_step = wall * 1e3 / K
     n = geom.num_pixels
-    alg_bytes = n * BYTES_PER_SAMPLE_F32_3D
+    alg_bytes = n * BYTES_PER_SAMPLE_3D[args.storage]
     dev_ms = t_total / K
     achieved = alg_bytes / (dev_ms * 1e-3) / 1e9
     line = {
@@ -165,13 +170,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 images, f64 coordinates/PDF",
+        "dtype": "%s images, f64 coordinates/PDF" % args.storage,
         "data": "synthetic (seeded blob volumes, see itk_amd/synth.py)",
-        "config": dict({"workload": wl.name, "bins": 50, "samples_per_eval": n,
+        "config": dict({"workload": wl.name + ("" if args.storage == "f32" else "_" + args.storage), "bins": 50,
+                        "samples_per_eval": n, "storage": args.storage,
                         "gradient_source": "recursive Gaussian gradient image (device)",
                         "parallelism": "single GPU"}, **wl.extra),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES_PER_EVAL.get(wl.name),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC_BYTES_PER_EVAL.get((wl.name, args.storage)),
                      "kernel": "whole evaluation (pass1 + finalize + pass2), HIP events on the metric stream",
                      "algorithmic_bytes_per_eval": alg_bytes, "device_ms_per_eval": dev_ms,
                      "pass1_ms": t_pass1 / K, "pass2_ms": t_pass2 / K},

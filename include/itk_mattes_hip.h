@@ -90,8 +90,12 @@ int mmi_compute_moving_gradient(mmi_context* ctx);
 int mmi_set_gradient_source(mmi_context* ctx, int32_t source);
 
 /* Device storage of fixed/moving/gradient images: MMI_F32 (default),
- * MMI_BF16 or MMI_F16.  Accumulation stays FP32/fixed point, normalisation
- * and the entropy sum FP64.  Must precede the image uploads. */
+ * MMI_BF16 or MMI_F16 (round to nearest even at mmi_initialize; FP16 holds
+ * |v| <= 65504).  With 16-bit storage every read of the images (intensity
+ * range, bins, interpolation, gradient) sees the rounded values; Parzen
+ * weights stay FP32 accumulated in fixed point, coordinates, normalisation and
+ * the entropy sum FP64.  Takes effect at the next mmi_initialize.  No
+ * counterpart in the reference (its images are the pixel type itself). */
 int mmi_set_storage(mmi_context* ctx, int32_t pixel_type);
 
 /* Virtual domain (ObjectToObjectMetric::SetVirtualDomain,

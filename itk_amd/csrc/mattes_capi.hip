@@ -255,7 +255,9 @@ struct mmi_context {
 
   bool have_fixed = false, have_moving = false, have_virtual = false, have_texels = false;
   mmi_geometry fixed_g{}, moving_g{}, virtual_g{};
-  DevBuf fixed_px, moving_px, texels;
+  DevBuf fixed_px, moving_px, texels;  // FP32 uploads and FP32 gradient texels
+  DevBuf fixed_lp, moving_lp, texels_lp;  // 16-bit storage copies (MMI_BF16 / MMI_F16)
+  int32_t active_storage = MMI_F32;       // storage the evaluation reads, fixed at mmi_initialize
 
   bool initialized = false;
   double fixed_min = 0, fixed_max = 0, moving_min = 0, moving_max = 0;
@@ -521,10 +523,32 @@ bool setup_fast(const mmi_context* ctx, FastArgs* F) {
   return true;
 }
 
-int device_minmax(mmi_context* ctx, const float* px, int64_t n, double* lo, double* hi) {
+// Device images the evaluation reads: the FP32 uploads or their 16-bit copies.
+const void* fixed_pixels(const mmi_context* ctx) {
+  return ctx->active_storage == MMI_F32 ? ctx->fixed_px.p : ctx->fixed_lp.p;
+}
+const void* moving_pixels(const mmi_context* ctx) {
+  return ctx->active_storage == MMI_F32 ? ctx->moving_px.p : ctx->moving_lp.p;
+}
+const void* texel_pixels(const mmi_context* ctx) {
+  return ctx->active_storage == MMI_F32 ? ctx->texels.p : ctx->texels_lp.p;
+}
+
+// A new gradient image after mmi_initialize with 16-bit storage: refresh the
+// 16-bit texels the evaluation reads.
+int refresh_storage_texels(mmi_context* ctx) {
+  if (ctx->active_storage == MMI_F32) return MMI_OK;
+  const int64_t nm = num_pixels(ctx->moving_g);
+  MMI_HIP(ctx->texels_lp.ensure(sizeof(uint2) * nm));
+  MMI_HIP(launch_texels_to_storage(ctx->texels.as<float4>(), ctx->texels_lp.p, ctx->active_storage, nm, ctx->stream));
+  MMI_HIP(hipStreamSynchronize(ctx->stream));
+  return MMI_OK;
+}
+
+int device_minmax(mmi_context* ctx, const void* px, int storage, int64_t n, double* lo, double* hi) {
   const int blocks = (int)std::min<int64_t>(1024, (n + 255) / 256);
   MMI_HIP(ctx->scratch.ensure(sizeof(float) * 2 * blocks));
-  MMI_HIP(launch_minmax(px, n, ctx->scratch.as<float>(), blocks, ctx->stream));
+  MMI_HIP(launch_minmax(px, storage, n, ctx->scratch.as<float>(), blocks, ctx->stream));
   std::vector<float> h(2 * blocks);
   MMI_HIP(hipMemcpyAsync(h.data(), ctx->scratch.p, sizeof(float) * 2 * blocks, hipMemcpyDeviceToHost, ctx->stream));
   MMI_HIP(hipStreamSynchronize(ctx->stream));
@@ -650,7 +674,8 @@ int mmi_context_destroy(mmi_context* ctx) {
   if (!ctx) return MMI_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  DevBuf* bufs[] = {&ctx->fixed_px, &ctx->moving_px, &ctx->texels, &ctx->coeffs, &ctx->hist, &ctx->fmarg,
+  DevBuf* bufs[] = {&ctx->fixed_px, &ctx->moving_px, &ctx->texels, &ctx->fixed_lp, &ctx->moving_lp, &ctx->texels_lp,
+                    &ctx->coeffs, &ctx->hist, &ctx->fmarg,
                     &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->result,
                     &ctx->partials, &ctx->deriv, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
                     &ctx->exc_count, &ctx->exc_list, &ctx->exc_overflow, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2};
@@ -671,8 +696,9 @@ int mmi_context_destroy(mmi_context* ctx) {
 
 int mmi_set_storage(mmi_context* ctx, int32_t pixel_type) {
   if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
-  if (pixel_type != MMI_F32)
-    return fail(MMI_ERR_ARGUMENT, "this build stores images as MMI_F32 only");
+  if (pixel_type != MMI_F32 && pixel_type != MMI_BF16 && pixel_type != MMI_F16)
+    return fail(MMI_ERR_ARGUMENT, "storage must be MMI_F32, MMI_BF16 or MMI_F16");
+  if (pixel_type != ctx->storage) ctx->initialized = false;
   ctx->storage = pixel_type;
   return MMI_OK;
 }
@@ -715,7 +741,7 @@ int mmi_set_moving_gradient(mmi_context* ctx, const double* gradient) {
   MMI_HIP(hipStreamSynchronize(ctx->stream));
   tmp.release();
   ctx->have_texels = true;
-  return MMI_OK;
+  return refresh_storage_texels(ctx);
 }
 
 int mmi_compute_moving_gradient(mmi_context* ctx) {
@@ -772,7 +798,7 @@ int mmi_compute_moving_gradient(mmi_context* ctx) {
   b.release();
   grad.release();
   ctx->have_texels = true;
-  return MMI_OK;
+  return refresh_storage_texels(ctx);
 }
 
 int mmi_set_gradient_source(mmi_context* ctx, int32_t source) {
@@ -807,12 +833,30 @@ int mmi_initialize(mmi_context* ctx) {
   if (ctx->gradient_source == MMI_GRADIENT_IMAGE && !ctx->have_texels) {
     if (int rc = mmi_compute_moving_gradient(ctx)) return rc;
   }
-  if (int rc = device_minmax(ctx, ctx->fixed_px.as<float>(), num_pixels(ctx->fixed_g), &ctx->fixed_min,
+  // 16-bit storage: every later read of the images (intensity range, bins,
+  // interpolation, gradient) sees the rounded copies.
+  ctx->active_storage = MMI_F32;
+  if (ctx->storage != MMI_F32) {
+    const int64_t nf = num_pixels(ctx->fixed_g), nm = num_pixels(ctx->moving_g);
+    MMI_HIP(ctx->fixed_lp.ensure(sizeof(uint16_t) * nf));
+    MMI_HIP(ctx->moving_lp.ensure(sizeof(uint16_t) * nm));
+    MMI_HIP(launch_to_storage(ctx->fixed_px.as<float>(), ctx->fixed_lp.p, ctx->storage, nf, ctx->stream));
+    MMI_HIP(launch_to_storage(ctx->moving_px.as<float>(), ctx->moving_lp.p, ctx->storage, nm, ctx->stream));
+    if (ctx->have_texels) {
+      MMI_HIP(ctx->texels_lp.ensure(sizeof(uint2) * nm));
+      MMI_HIP(launch_texels_to_storage(ctx->texels.as<float4>(), ctx->texels_lp.p, ctx->storage, nm, ctx->stream));
+    }
+    ctx->active_storage = ctx->storage;
+  }
+  if (int rc = device_minmax(ctx, fixed_pixels(ctx), ctx->active_storage, num_pixels(ctx->fixed_g), &ctx->fixed_min,
                              &ctx->fixed_max))
     return rc;
-  if (int rc = device_minmax(ctx, ctx->moving_px.as<float>(), num_pixels(ctx->moving_g), &ctx->moving_min,
-                             &ctx->moving_max))
+  if (int rc = device_minmax(ctx, moving_pixels(ctx), ctx->active_storage, num_pixels(ctx->moving_g),
+                             &ctx->moving_min, &ctx->moving_max))
     return rc;
+  if (!std::isfinite(ctx->fixed_min) || !std::isfinite(ctx->fixed_max) || !std::isfinite(ctx->moving_min) ||
+      !std::isfinite(ctx->moving_max))
+    return fail(MMI_ERR_ARGUMENT, "image intensities are not finite in the storage format (FP16 holds |v| <= 65504)");
   // Mattes:273-317
   const double eps = std::numeric_limits<double>::epsilon();
   char buf[512];
@@ -911,9 +955,10 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
   a.moving = to_dev(ctx->moving_g);
   a.ftf = ctx->dftf;
   a.mtf = ctx->dmtf;
-  a.fixed_px = ctx->fixed_px.as<float>();
-  a.moving_px = ctx->moving_px.as<float>();
-  a.texels = ctx->texels.as<float4>();
+  a.fixed_px = fixed_pixels(ctx);
+  a.moving_px = moving_pixels(ctx);
+  a.texels = texel_pixels(ctx);
+  a.storage = ctx->active_storage;
   a.n = n;
   a.bins = B;
   a.hist_in_lds = (size_t)B * B * 8 + (B + 1) * 4 <= 64 * 1024 ? 1 : 0;
@@ -948,6 +993,7 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
     fa.fixed_px = a.fixed_px;
     fa.moving_px = a.moving_px;
     fa.texels = a.texels;
+    fa.storage = a.storage;
     fa.hist = a.hist;
     fa.fmarg = a.fmarg;
     fa.nvalid = a.nvalid;

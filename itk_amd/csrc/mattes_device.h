@@ -185,46 +185,82 @@ __device__ __forceinline__ Cell make_cell(const DevGeom& g, const double* ci) {
 
 __device__ __forceinline__ double lerp_exact(double a, double b, double t) { return a + (b - a) * t; }
 
+// Image storage formats.  Each gives the pixel and gradient-texel types and
+// their exact conversion to FP32 (every 16-bit value is exact in FP32).
+// Texels hold (moving value, dI/dx, dI/dy, dI/dz): float4 for FP32 storage,
+// four 16-bit halves packed in a uint2 for BF16 and FP16 storage (x = value |
+// dI/dx << 16, y = dI/dy | dI/dz << 16).
+struct PxF32 {
+  using pixel = float;
+  using texel = float4;
+  static __device__ __forceinline__ float ld(const pixel* p, int32_t i) { return p[i]; }
+  static __device__ __forceinline__ float4 ldt(const texel* t, int32_t i) { return t[i]; }
+};
+
+struct PxBF16 {
+  using pixel = uint16_t;
+  using texel = uint2;
+  static __device__ __forceinline__ float ld(const pixel* p, int32_t i) {
+    return __uint_as_float((uint32_t)p[i] << 16);
+  }
+  static __device__ __forceinline__ float4 ldt(const texel* t, int32_t i) {
+    const uint2 u = t[i];
+    return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xFFFF0000u), __uint_as_float(u.y << 16),
+                       __uint_as_float(u.y & 0xFFFF0000u));
+  }
+};
+
+struct PxF16 {
+  using pixel = uint16_t;
+  using texel = uint2;
+  static __device__ __forceinline__ float h2f(uint32_t h) { return (float)__builtin_bit_cast(_Float16, (uint16_t)h); }
+  static __device__ __forceinline__ float ld(const pixel* p, int32_t i) { return h2f(p[i]); }
+  static __device__ __forceinline__ float4 ldt(const texel* t, int32_t i) {
+    const uint2 u = t[i];
+    return make_float4(h2f(u.x & 0xFFFFu), h2f(u.x >> 16), h2f(u.y & 0xFFFFu), h2f(u.y >> 16));
+  }
+};
+
 // Exact FP64 multilinear value of a float image (the reference's RealType is
 // double for float pixels).
-template <int D>
-__device__ __forceinline__ double interp_exact(const float* __restrict__ img, const Cell& c) {
-  const double v000 = (double)img[c.base];
-  const double v100 = (double)img[c.base + c.ox];
-  const double v010 = (double)img[c.base + c.oy];
-  const double v110 = (double)img[c.base + c.oy + c.ox];
+template <int D, typename P>
+__device__ __forceinline__ double interp_exact(const typename P::pixel* __restrict__ img, const Cell& c) {
+  const double v000 = (double)P::ld(img, c.base);
+  const double v100 = (double)P::ld(img, c.base + c.ox);
+  const double v010 = (double)P::ld(img, c.base + c.oy);
+  const double v110 = (double)P::ld(img, c.base + c.oy + c.ox);
   const double vx00 = lerp_exact(v000, v100, c.d[0]);
   const double vx10 = lerp_exact(v010, v110, c.d[0]);
   const double vxx0 = lerp_exact(vx00, vx10, c.d[1]);
   if (D == 2) return vxx0;
-  const double v001 = (double)img[c.base + c.oz];
-  const double v101 = (double)img[c.base + c.oz + c.ox];
-  const double v011 = (double)img[c.base + c.oz + c.oy];
-  const double v111 = (double)img[c.base + c.oz + c.oy + c.ox];
+  const double v001 = (double)P::ld(img, c.base + c.oz);
+  const double v101 = (double)P::ld(img, c.base + c.oz + c.ox);
+  const double v011 = (double)P::ld(img, c.base + c.oz + c.oy);
+  const double v111 = (double)P::ld(img, c.base + c.oz + c.oy + c.ox);
   const double vx01 = lerp_exact(v001, v101, c.d[0]);
   const double vx11 = lerp_exact(v011, v111, c.d[0]);
   const double vxx1 = lerp_exact(vx01, vx11, c.d[1]);
   return lerp_exact(vxx0, vxx1, c.d[2]);
 }
 
-// Gradient texels: (moving value, dI/dx, dI/dy, dI/dz) as float4.  Only the
-// gradient components are interpolated here (FP32), with the same cell.
-template <int D>
-__device__ __forceinline__ float3 interp_gradient3(const float4* __restrict__ tex, const Cell& c) {
+// Gradient at a moving cell: only the gradient components of the texels are
+// interpolated here (FP32), with the same cell as the value.
+template <int D, typename P>
+__device__ __forceinline__ float3 interp_gradient3(const typename P::texel* __restrict__ tex, const Cell& c) {
   const float t0 = (float)c.d[0], t1 = (float)c.d[1], t2 = (float)c.d[2];
-  const float4 a000 = tex[c.base];
-  const float4 a100 = tex[c.base + c.ox];
-  const float4 a010 = tex[c.base + c.oy];
-  const float4 a110 = tex[c.base + c.oy + c.ox];
+  const float4 a000 = P::ldt(tex, c.base);
+  const float4 a100 = P::ldt(tex, c.base + c.ox);
+  const float4 a010 = P::ldt(tex, c.base + c.oy);
+  const float4 a110 = P::ldt(tex, c.base + c.oy + c.ox);
   const float gx0 = fmaf(a100.y - a000.y, t0, a000.y), gx1 = fmaf(a110.y - a010.y, t0, a010.y);
   const float gy0 = fmaf(a100.z - a000.z, t0, a000.z), gy1 = fmaf(a110.z - a010.z, t0, a010.z);
   const float gz0 = fmaf(a100.w - a000.w, t0, a000.w), gz1 = fmaf(a110.w - a010.w, t0, a010.w);
   float3 r = make_float3(fmaf(gx1 - gx0, t1, gx0), fmaf(gy1 - gy0, t1, gy0), fmaf(gz1 - gz0, t1, gz0));
   if (D == 3) {
-    const float4 a001 = tex[c.base + c.oz];
-    const float4 a101 = tex[c.base + c.oz + c.ox];
-    const float4 a011 = tex[c.base + c.oz + c.oy];
-    const float4 a111 = tex[c.base + c.oz + c.oy + c.ox];
+    const float4 a001 = P::ldt(tex, c.base + c.oz);
+    const float4 a101 = P::ldt(tex, c.base + c.oz + c.ox);
+    const float4 a011 = P::ldt(tex, c.base + c.oz + c.oy);
+    const float4 a111 = P::ldt(tex, c.base + c.oz + c.oy + c.ox);
     const float hx0 = fmaf(a101.y - a001.y, t0, a001.y), hx1 = fmaf(a111.y - a011.y, t0, a011.y);
     const float hy0 = fmaf(a101.z - a001.z, t0, a001.z), hy1 = fmaf(a111.z - a011.z, t0, a011.z);
     const float hz0 = fmaf(a101.w - a001.w, t0, a001.w), hz1 = fmaf(a111.w - a011.w, t0, a011.w);
@@ -234,39 +270,6 @@ __device__ __forceinline__ float3 interp_gradient3(const float4* __restrict__ te
     r.z = fmaf(s2 - r.z, t2, r.z);
   }
   return r;
-}
-
-template <int D>
-__device__ __forceinline__ void interp_gradient(const float4* __restrict__ tex, const Cell& c, float* g) {
-  const float t0 = (float)c.d[0], t1 = (float)c.d[1], t2 = (float)c.d[2];
-  const float4 a000 = tex[c.base];
-  const float4 a100 = tex[c.base + c.ox];
-  const float4 a010 = tex[c.base + c.oy];
-  const float4 a110 = tex[c.base + c.oy + c.ox];
-  float r[3];
-  {
-    const float gx0 = fmaf(a100.y - a000.y, t0, a000.y), gx1 = fmaf(a110.y - a010.y, t0, a010.y);
-    const float gy0 = fmaf(a100.z - a000.z, t0, a000.z), gy1 = fmaf(a110.z - a010.z, t0, a010.z);
-    const float gz0 = fmaf(a100.w - a000.w, t0, a000.w), gz1 = fmaf(a110.w - a010.w, t0, a010.w);
-    r[0] = fmaf(gx1 - gx0, t1, gx0);
-    r[1] = fmaf(gy1 - gy0, t1, gy0);
-    r[2] = fmaf(gz1 - gz0, t1, gz0);
-  }
-  if (D == 3) {
-    const float4 a001 = tex[c.base + c.oz];
-    const float4 a101 = tex[c.base + c.oz + c.ox];
-    const float4 a011 = tex[c.base + c.oz + c.oy];
-    const float4 a111 = tex[c.base + c.oz + c.oy + c.ox];
-    const float gx0 = fmaf(a101.y - a001.y, t0, a001.y), gx1 = fmaf(a111.y - a011.y, t0, a011.y);
-    const float gy0 = fmaf(a101.z - a001.z, t0, a001.z), gy1 = fmaf(a111.z - a011.z, t0, a011.z);
-    const float gz0 = fmaf(a101.w - a001.w, t0, a001.w), gz1 = fmaf(a111.w - a011.w, t0, a011.w);
-    const float s0 = fmaf(gx1 - gx0, t1, gx0), s1 = fmaf(gy1 - gy0, t1, gy0), s2 = fmaf(gz1 - gz0, t1, gz0);
-    r[0] = fmaf(s0 - r[0], t2, r[0]);
-    r[1] = fmaf(s1 - r[1], t2, r[1]);
-    r[2] = fmaf(s2 - r[2], t2, r[2]);
-  }
-#pragma unroll
-  for (int k = 0; k < D; ++k) g[k] = r[k];
 }
 
 // Math::FloatAlmostEqual(x, y, 4 ulps, 0.1 eps) (itkMath.h:331-356).

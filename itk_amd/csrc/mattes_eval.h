@@ -33,8 +33,9 @@ __device__ __forceinline__ void fixed_point(const DevTransform& t, const double*
 // (itkImageToImageMetricv4GetValueAndDerivativeThreaderBase.hxx:174-273,
 // itkImageToImageMetricv4.hxx:289-373,
 // itkMattesMutualInformationImageToImageMetricv4GetValueAndDerivativeThreader.hxx:215-275),
-// exact FP64.  MK is the moving transform kind (1 affine, 2 B-spline).
-template <int D, int MK>
+// exact FP64.  MK is the moving transform kind (1 affine, 2 B-spline); P the
+// image storage format.
+template <int D, int MK, typename P>
 __device__ __forceinline__ bool eval_sample(const EvalArgs& a, const int32_t* idx, SampleOut<D>& o) {
   index_to_point<D>(a.virt, idx, o.vp);
   double fp[3], ci[3];
@@ -42,13 +43,13 @@ __device__ __forceinline__ bool eval_sample(const EvalArgs& a, const int32_t* id
   point_to_cindex<D>(a.fixed, fp, ci);
   if (!inside_buffer<D>(a.fixed, ci)) return false;
   const Cell fc = make_cell<D>(a.fixed, ci);
-  const double fval = (double)(float)interp_exact<D>(a.fixed_px, fc);
+  const double fval = (double)(float)interp_exact<D, P>(static_cast<const typename P::pixel*>(a.fixed_px), fc);
 
   transform_point<D, MK>(a.mtf, o.vp, o.mp, &o.sup);
   point_to_cindex<D>(a.moving, o.mp, ci);
   if (!inside_buffer<D>(a.moving, ci)) return false;
   o.cell = make_cell<D>(a.moving, ci);
-  const double mval = (double)(float)interp_exact<D>(a.moving_px, o.cell);
+  const double mval = (double)(float)interp_exact<D, P>(static_cast<const typename P::pixel*>(a.moving_px), o.cell);
 
   if (mval < a.moving_min) return false;
   if (mval > a.moving_max) return false;

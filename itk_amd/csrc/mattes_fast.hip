@@ -73,7 +73,7 @@ __device__ __forceinline__ int fast_bin(float term, float bound, int B, bool* un
 // 0: not counted, 1: counted (bins, arg0, cell set), 2: uncertain.  Straight
 // line code: the cell is clamped into the buffer so every load is in bounds,
 // and the decisions are combined at the end.
-template <int D>
+template <int D, typename P>
 __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* idx, int& fbin, int& mbin, float& arg0,
                                              Cell& cell) {
   double ci[3];
@@ -107,15 +107,16 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
   cell.oy = act[1] ? f.msy : 0;
   cell.oz = (D == 3 && act[2]) ? f.msz : 0;
   const float t0 = (float)cell.d[0], t1 = (float)cell.d[1], t2 = (float)cell.d[2];
-  const float* img = f.moving_px;
-  const float v000 = img[cell.base], v100 = img[cell.base + cell.ox];
-  const float v010 = img[cell.base + cell.oy], v110 = img[cell.base + cell.oy + cell.ox];
+  const typename P::pixel* img = static_cast<const typename P::pixel*>(f.moving_px);
+  const float v000 = P::ld(img, cell.base), v100 = P::ld(img, cell.base + cell.ox);
+  const float v010 = P::ld(img, cell.base + cell.oy), v110 = P::ld(img, cell.base + cell.oy + cell.ox);
   const float vx00 = fmaf(v100 - v000, t0, v000);
   const float vx10 = fmaf(v110 - v010, t0, v010);
   float v = fmaf(vx10 - vx00, t1, vx00);
   if (D == 3) {
-    const float v001 = img[cell.base + cell.oz], v101 = img[cell.base + cell.oz + cell.ox];
-    const float v011 = img[cell.base + cell.oz + cell.oy], v111 = img[cell.base + cell.oz + cell.oy + cell.ox];
+    const float v001 = P::ld(img, cell.base + cell.oz), v101 = P::ld(img, cell.base + cell.oz + cell.ox);
+    const float v011 = P::ld(img, cell.base + cell.oz + cell.oy);
+    const float v111 = P::ld(img, cell.base + cell.oz + cell.oy + cell.ox);
     const float vx01 = fmaf(v101 - v001, t0, v001);
     const float vx11 = fmaf(v111 - v011, t0, v011);
     const float vxx1 = fmaf(vx11 - vx01, t1, vx01);
@@ -128,13 +129,13 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
   arg0 = (float)(mbin - 1) - mterm;
   // fixed value: the pixel the fixed sample sits on (verified on the host)
   const int32_t fl = (idx[0] + f.foff[0]) + f.fsy * (idx[1] + f.foff[1]) + (D == 3 ? f.fsz * (idx[2] + f.foff[2]) : 0);
-  const float fv = f.fixed_px[fl];
+  const float fv = P::ld(static_cast<const typename P::pixel*>(f.fixed_px), fl);
   const float fterm = fmaf(fv, f.inv_fbs, -f.fnm);
   fbin = fast_bin(fterm, f.bt_f, f.bins, &uncertain);
   return out ? 0 : (uncertain ? 2 : 1);
 }
 
-template <int D>
+template <int D, typename P>
 __global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f) {
   extern __shared__ unsigned long long lds_u64[];
   const int B = f.bins;
@@ -154,7 +155,7 @@ __global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f) {
     int fbin = 0, mbin = 0;
     float arg0 = 0.0f;
     Cell cell;
-    const int c = fast_classify<D>(f, idx, fbin, mbin, arg0, cell);
+    const int c = fast_classify<D, P>(f, idx, fbin, mbin, arg0, cell);
     if (c == 2) {
       const unsigned int slot = atomicAdd(&counters[1], 1u);
       if (slot < (unsigned int)f.exc_cap) f.exc_list[blockIdx.x * f.exc_cap + slot] = s;
@@ -196,7 +197,7 @@ __global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f) {
 }
 
 // Exact FP64 evaluation of the listed samples of one pass-1 workgroup.
-template <int D>
+template <int D, typename P>
 __global__ void __launch_bounds__(256) exceptions_pass1_kernel(EvalArgs a, FastArgs f) {
   extern __shared__ unsigned int lds_u32[];
   const uint32_t ne = min(f.exc_count[blockIdx.x], (uint32_t)f.exc_cap);
@@ -210,7 +211,7 @@ __global__ void __launch_bounds__(256) exceptions_pass1_kernel(EvalArgs a, FastA
     int32_t idx[3];
     sample_index<D>(a.virt, s, idx);
     SampleOut<D> o;
-    const bool ok = eval_sample<D, 1>(a, idx, o);
+    const bool ok = eval_sample<D, 1, P>(a, idx, o);
     if (a.sample_fbin) {
       a.sample_fbin[s] = ok ? (int16_t)o.fbin : (int16_t)-1;
       a.sample_mbin[s] = ok ? (int16_t)o.mbin : (int16_t)-1;
@@ -246,7 +247,7 @@ __device__ __forceinline__ void add_moments(double* acc, float w, float3 g, cons
   }
 }
 
-template <int D>
+template <int D, typename P>
 __global__ void __launch_bounds__(256) pass2_fast_kernel(FastArgs f) {
   extern __shared__ float lds_f32[];
   const int B = f.bins;
@@ -267,17 +268,17 @@ __global__ void __launch_bounds__(256) pass2_fast_kernel(FastArgs f) {
     int fbin = 0, mbin = 0;
     float arg0 = 0.0f;
     Cell cell;
-    if (fast_classify<D>(f, idx, fbin, mbin, arg0, cell) != 1) continue;
+    if (fast_classify<D, P>(f, idx, fbin, mbin, arg0, cell) != 1) continue;
     const float w = parzen_deriv_weight_fast(arg0, pr + fbin * B + (mbin - 1));
     if (w == 0.0f) continue;
-    add_moments<D>(acc, w, interp_gradient3<D>(f.texels, cell), idx);
+    add_moments<D>(acc, w, interp_gradient3<D, P>(static_cast<const typename P::texel*>(f.texels), cell), idx);
   }
   block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)blockIdx.x * PA);
 }
 
 // Exception samples of one pass-1 workgroup, in sample order (bitonic sort of
 // the list in LDS), summed in a fixed tree: deterministic.
-template <int D>
+template <int D, typename P>
 __global__ void __launch_bounds__(256) exceptions_pass2_kernel(EvalArgs a, FastArgs f) {
   __shared__ uint32_t keys[256];
   __shared__ double scratch[4 * 12];
@@ -308,10 +309,11 @@ __global__ void __launch_bounds__(256) exceptions_pass2_kernel(EvalArgs a, FastA
       int32_t idx[3];
       sample_index<D>(a.virt, s, idx);
       SampleOut<D> o;
-      if (eval_sample<D, 1>(a, idx, o)) {
+      if (eval_sample<D, 1, P>(a, idx, o)) {
         const float w = parzen_deriv_weight((float)((double)(o.mbin - 1) - o.mterm),
                                             f.pratio_f + o.fbin * a.bins + (o.mbin - 1));
-        if (w != 0.0f) add_moments<D>(acc, w, interp_gradient3<D>(a.texels, o.cell), idx);
+        if (w != 0.0f)
+          add_moments<D>(acc, w, interp_gradient3<D, P>(static_cast<const typename P::texel*>(a.texels), o.cell), idx);
       }
     }
   }
@@ -357,37 +359,45 @@ __global__ void __launch_bounds__(256) fast_derivative_kernel(const double* __re
 // Launch wrappers
 // ---------------------------------------------------------------------------
 
+#define MMI_DISPATCH_FAST(KERNEL, storage, dim, grid, lds, st, ...)                                  \
+  do {                                                                                              \
+    if ((storage) == kStorageBF16) {                                                                \
+      if ((dim) == 2)                                                                               \
+        hipLaunchKernelGGL((KERNEL<2, PxBF16>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);       \
+      else                                                                                          \
+        hipLaunchKernelGGL((KERNEL<3, PxBF16>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);       \
+    } else if ((storage) == kStorageF16) {                                                          \
+      if ((dim) == 2)                                                                               \
+        hipLaunchKernelGGL((KERNEL<2, PxF16>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);        \
+      else                                                                                          \
+        hipLaunchKernelGGL((KERNEL<3, PxF16>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);        \
+    } else {                                                                                        \
+      if ((dim) == 2)                                                                               \
+        hipLaunchKernelGGL((KERNEL<2, PxF32>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);        \
+      else                                                                                          \
+        hipLaunchKernelGGL((KERNEL<3, PxF32>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);        \
+    }                                                                                               \
+  } while (0)
+
 hipError_t launch_fast_pass1(const FastArgs& f, int dim, int grid, hipStream_t st) {
   const size_t lds = (size_t)f.bins * f.bins * 8 + (size_t)(f.bins + 2) * 4;
-  if (dim == 2)
-    hipLaunchKernelGGL(pass1_fast_kernel<2>, dim3(grid), dim3(256), lds, st, f);
-  else
-    hipLaunchKernelGGL(pass1_fast_kernel<3>, dim3(grid), dim3(256), lds, st, f);
+  MMI_DISPATCH_FAST(pass1_fast_kernel, f.storage, dim, grid, lds, st, f);
   return hipGetLastError();
 }
 
 hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st) {
-  if (a.virt.dim == 2)
-    hipLaunchKernelGGL(exceptions_pass1_kernel<2>, dim3(grid), dim3(256), (a.bins + 1) * 4, st, a, f);
-  else
-    hipLaunchKernelGGL(exceptions_pass1_kernel<3>, dim3(grid), dim3(256), (a.bins + 1) * 4, st, a, f);
+  MMI_DISPATCH_FAST(exceptions_pass1_kernel, f.storage, a.virt.dim, grid, (a.bins + 1) * 4, st, a, f);
   return hipGetLastError();
 }
 
 hipError_t launch_fast_pass2(const FastArgs& f, int dim, int grid, hipStream_t st) {
   const size_t lds = ((size_t)f.bins * f.bins + 128) * 4;
-  if (dim == 2)
-    hipLaunchKernelGGL(pass2_fast_kernel<2>, dim3(grid), dim3(256), lds, st, f);
-  else
-    hipLaunchKernelGGL(pass2_fast_kernel<3>, dim3(grid), dim3(256), lds, st, f);
+  MMI_DISPATCH_FAST(pass2_fast_kernel, f.storage, dim, grid, lds, st, f);
   return hipGetLastError();
 }
 
 hipError_t launch_fast_exceptions2(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st) {
-  if (a.virt.dim == 2)
-    hipLaunchKernelGGL(exceptions_pass2_kernel<2>, dim3(grid), dim3(256), 0, st, a, f);
-  else
-    hipLaunchKernelGGL(exceptions_pass2_kernel<3>, dim3(grid), dim3(256), 0, st, a, f);
+  MMI_DISPATCH_FAST(exceptions_pass2_kernel, f.storage, a.virt.dim, grid, 0, st, a, f);
   return hipGetLastError();
 }
 

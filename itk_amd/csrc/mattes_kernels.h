@@ -8,6 +8,11 @@
 
 namespace mmi {
 
+// Image storage codes (the C ABI's MMI_F32 / MMI_BF16 / MMI_F16).
+constexpr int kStorageF32 = 0;
+constexpr int kStorageBF16 = 2;
+constexpr int kStorageF16 = 3;
+
 struct DevResult {
   unsigned long long n_valid;
   int32_t status;  // 0 ok, 1 no valid sample, 2 joint PDF summed to zero
@@ -30,11 +35,12 @@ struct FastArgs {
   int32_t vstart[3];
   int32_t vsize0, vsize1;
   FastDiv vdx, vdy;
-  const float* fixed_px;
+  const void* fixed_px;  // storage type given by `storage` (MMI_F32 / MMI_BF16 / MMI_F16)
   int32_t fsy, fsz;
   int32_t foff[3];  // fixed pixel index = virtual index + foff (buffer-relative)
-  const float* moving_px;
-  const float4* texels;
+  int32_t storage;
+  const void* moving_px;
+  const void* texels;
   int32_t mstart[3], mend[3];
   int32_t msy, msz;
   double A[9];  // virtual index -> moving continuous index
@@ -73,10 +79,12 @@ struct EvalArgs {
   DevGeom moving;
   DevTransform ftf;
   DevTransform mtf;
-  const float* fixed_px;
-  const float* moving_px;
-  const float4* texels;
+  const void* fixed_px;   // storage type given by `storage`
+  const void* moving_px;
+  const void* texels;
   int64_t n;  // virtual samples
+  int32_t storage;        // MMI_F32, MMI_BF16 or MMI_F16
+  int32_t pad2_;
   int32_t bins;
   int32_t hist_in_lds;
   int32_t gradient_source;
@@ -153,7 +161,10 @@ hipError_t launch_bspline_axis_table(const DevGeom& virt, const DevGeom& grid, i
 // indices l whose support holds node g, times scale->n_factor when scale is set.
 hipError_t launch_separable_contract(const float* in_f, const double* in_d, double* out, int64_t outer, int L,
                                      int64_t inner, int G, const AxisTable& t, const DevResult* scale, hipStream_t st);
-hipError_t launch_minmax(const float* px, int64_t n, float* out, int blocks, hipStream_t st);
+hipError_t launch_minmax(const void* px, int storage, int64_t n, float* out, int blocks, hipStream_t st);
+// FP32 image -> 16-bit storage (round to nearest even), and FP32 texels -> 16-bit texels.
+hipError_t launch_to_storage(const float* in, void* out, int storage, int64_t n, hipStream_t st);
+hipError_t launch_texels_to_storage(const float4* in, void* out, int storage, int64_t n, hipStream_t st);
 hipError_t launch_recursive_gaussian(const double* in, double* out, int64_t ln, int64_t stride, int64_t nlines,
                                      const RecCoeffs& c, hipStream_t st);
 hipError_t launch_float_to_double(const float* in, double* out, int64_t n, hipStream_t st);

@@ -36,7 +36,7 @@ constexpr int kExactBSpline = 2;  // FP64 reference path, cubic B-spline moving 
 // pass 1: joint histogram
 // ---------------------------------------------------------------------------
 
-template <int D, int MODE>
+template <int D, int MODE, typename P>
 __global__ void __launch_bounds__(256) pass1_kernel(EvalArgs a) {
   constexpr int MK = MODE == kExactBSpline ? 2 : 1;
   extern __shared__ unsigned long long lds_u64[];
@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(256) pass1_kernel(EvalArgs a) {
     int32_t idx[3];
     sample_index<D>(a.virt, s, idx);
     SampleOut<D> o;
-    const bool ok = eval_sample<D, MK>(a, idx, o);
+    const bool ok = eval_sample<D, MK, P>(a, idx, o);
     if (a.sample_fbin) {
       a.sample_fbin[s] = ok ? (int16_t)o.fbin : (int16_t)-1;
       a.sample_mbin[s] = ok ? (int16_t)o.mbin : (int16_t)-1;
@@ -195,8 +195,9 @@ __global__ void __launch_bounds__(kFinalizeThreads) finalize_kernel(FinalizeArgs
 // Central differences of the moving image at a mapped point
 // (CentralDifferenceImageFunction::EvaluateSpecialized,
 // Modules/Core/ImageFunction/include/itkCentralDifferenceImageFunction.hxx:252-318).
-template <int D>
+template <int D, typename P>
 __device__ __forceinline__ void central_difference(const EvalArgs& a, const double* p, double* grad) {
+  const typename P::pixel* img = static_cast<const typename P::pixel*>(a.moving_px);
   double n1[3], n2[3];
 #pragma unroll
   for (int i = 0; i < D; ++i) n1[i] = n2[i] = p[i];
@@ -217,8 +218,8 @@ __device__ __forceinline__ void central_difference(const EvalArgs& a, const doub
     } else {
       const double delta = n2[dim] - n1[dim];
       if (delta > 10.0 * 2.220446049250313e-16) {
-        const double v2 = interp_exact<D>(a.moving_px, make_cell<D>(a.moving, c2));
-        const double v1 = interp_exact<D>(a.moving_px, make_cell<D>(a.moving, c1));
+        const double v2 = interp_exact<D, P>(img, make_cell<D>(a.moving, c2));
+        const double v1 = interp_exact<D, P>(img, make_cell<D>(a.moving, c1));
         grad[dim] = (v2 - v1) / delta;
       } else {
         grad[dim] = 0.0;
@@ -229,7 +230,7 @@ __device__ __forceinline__ void central_difference(const EvalArgs& a, const doub
   }
 }
 
-template <int D, int MODE>
+template <int D, int MODE, typename P>
 __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
   constexpr int MK = MODE == kExactBSpline ? 2 : 1;
   extern __shared__ float lds_f32[];
@@ -257,18 +258,18 @@ __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
     int32_t idx[3];
     sample_index<D>(a.virt, s, idx);
     SampleOut<D> o;
-    const bool ok = eval_sample<D, MK>(a, idx, o);
+    const bool ok = eval_sample<D, MK, P>(a, idx, o);
     float w = 0.0f;
     if (ok) w = parzen_deriv_weight((float)((double)(o.mbin - 1) - o.mterm), pr + o.fbin * B + (o.mbin - 1));
     double gd[3] = {0.0, 0.0, 0.0};
     if (w != 0.0f) {
       if (a.gradient_source == 1) {
-        const float3 g3 = interp_gradient3<D>(a.texels, o.cell);
+        const float3 g3 = interp_gradient3<D, P>(static_cast<const typename P::texel*>(a.texels), o.cell);
         gd[0] = (double)g3.x;
         gd[1] = (double)g3.y;
         gd[2] = (double)g3.z;
       } else {
-        central_difference<D>(a, o.mp, gd);
+        central_difference<D, P>(a, o.mp, gd);
       }
     }
     if constexpr (MK == 2) {
@@ -305,19 +306,29 @@ __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
 // Launch wrappers
 // ---------------------------------------------------------------------------
 
-#define MMI_DISPATCH_EXACT(KERNEL, a, grid, lds, st)                                            \
-  do {                                                                                          \
-    if ((a).virt.dim == 2) {                                                                    \
-      if ((a).mtf.kind == 2)                                                                    \
-        hipLaunchKernelGGL((KERNEL<2, kExactBSpline>), dim3(grid), dim3(256), lds, st, a);      \
-      else                                                                                      \
-        hipLaunchKernelGGL((KERNEL<2, kExactAffine>), dim3(grid), dim3(256), lds, st, a);       \
-    } else {                                                                                    \
-      if ((a).mtf.kind == 2)                                                                    \
-        hipLaunchKernelGGL((KERNEL<3, kExactBSpline>), dim3(grid), dim3(256), lds, st, a);      \
-      else                                                                                      \
-        hipLaunchKernelGGL((KERNEL<3, kExactAffine>), dim3(grid), dim3(256), lds, st, a);       \
-    }                                                                                           \
+#define MMI_DISPATCH_EXACT_P(KERNEL, P, a, grid, lds, st)                                         \
+  do {                                                                                            \
+    if ((a).virt.dim == 2) {                                                                      \
+      if ((a).mtf.kind == 2)                                                                      \
+        hipLaunchKernelGGL((KERNEL<2, kExactBSpline, P>), dim3(grid), dim3(256), lds, st, a);     \
+      else                                                                                        \
+        hipLaunchKernelGGL((KERNEL<2, kExactAffine, P>), dim3(grid), dim3(256), lds, st, a);      \
+    } else {                                                                                      \
+      if ((a).mtf.kind == 2)                                                                      \
+        hipLaunchKernelGGL((KERNEL<3, kExactBSpline, P>), dim3(grid), dim3(256), lds, st, a);     \
+      else                                                                                        \
+        hipLaunchKernelGGL((KERNEL<3, kExactAffine, P>), dim3(grid), dim3(256), lds, st, a);      \
+    }                                                                                             \
+  } while (0)
+
+#define MMI_DISPATCH_EXACT(KERNEL, a, grid, lds, st)                                              \
+  do {                                                                                            \
+    if ((a).storage == kStorageBF16)                                                              \
+      MMI_DISPATCH_EXACT_P(KERNEL, PxBF16, a, grid, lds, st);                                     \
+    else if ((a).storage == kStorageF16)                                                          \
+      MMI_DISPATCH_EXACT_P(KERNEL, PxF16, a, grid, lds, st);                                      \
+    else                                                                                          \
+      MMI_DISPATCH_EXACT_P(KERNEL, PxF32, a, grid, lds, st);                                      \
   } while (0)
 
 hipError_t launch_pass1(const EvalArgs& a, int grid, hipStream_t st) {

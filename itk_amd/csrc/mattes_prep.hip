@@ -36,11 +36,12 @@ __global__ void derivative_scale_kernel(double* __restrict__ deriv, int64_t P, c
 
 // Intensity range over a whole buffer (MattesMI::Initialize, Mattes:97-229),
 // 2 doubles per workgroup, reduced on the host.
-__global__ void minmax_kernel(const float* __restrict__ px, int64_t n, float* __restrict__ out) {
+template <typename P>
+__global__ void minmax_kernel(const typename P::pixel* __restrict__ px, int64_t n, float* __restrict__ out) {
   __shared__ float smin[256], smax[256];
   float lo = 3.402823466e38f, hi = -3.402823466e38f;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const float v = px[i];
+    const float v = P::ld(px + i, 0);
     lo = v < lo ? v : lo;
     hi = v > hi ? v : hi;
   }
@@ -177,6 +178,39 @@ __global__ void pack_texels_kernel(const float* __restrict__ moving, const doubl
   tex[i] = make_float4(moving[i], (float)o[0], (float)o[1], (float)o[2]);
 }
 
+// FP32 -> BF16 / FP16, round to nearest even (NaN stays NaN; FP16 overflows to
+// infinity, which the host rejects through the intensity range).
+__device__ __forceinline__ uint32_t to_bf16(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (u >> 16) | 0x40u;
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+__device__ __forceinline__ uint32_t to_f16(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+
+template <bool BF16>
+__global__ void to_storage_kernel(const float* __restrict__ in, uint16_t* __restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (uint16_t)(BF16 ? to_bf16(in[i]) : to_f16(in[i]));
+}
+
+template <bool BF16>
+__global__ void texels_to_storage_kernel(const float4* __restrict__ in, uint2* __restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 t = in[i];
+  uint2 u;
+  if (BF16) {
+    u.x = to_bf16(t.x) | (to_bf16(t.y) << 16);
+    u.y = to_bf16(t.z) | (to_bf16(t.w) << 16);
+  } else {
+    u.x = to_f16(t.x) | (to_f16(t.y) << 16);
+    u.y = to_f16(t.z) | (to_f16(t.w) << 16);
+  }
+  out[i] = u;
+}
+
 // Per-axis B-spline support of the virtual samples for a separable geometry.
 // The grid continuous index along `axis` is computed from the full virtual
 // point with the reference's expressions; the other axes' terms are exact
@@ -282,8 +316,36 @@ hipError_t launch_derivative_scale(double* deriv, int64_t P, const DevResult* r,
   return hipGetLastError();
 }
 
-hipError_t launch_minmax(const float* px, int64_t n, float* out, int blocks, hipStream_t st) {
-  hipLaunchKernelGGL(minmax_kernel, dim3(blocks), dim3(256), 0, st, px, n, out);
+hipError_t launch_minmax(const void* px, int storage, int64_t n, float* out, int blocks, hipStream_t st) {
+  if (storage == kStorageBF16)
+    hipLaunchKernelGGL(minmax_kernel<PxBF16>, dim3(blocks), dim3(256), 0, st, static_cast<const uint16_t*>(px), n,
+                       out);
+  else if (storage == kStorageF16)
+    hipLaunchKernelGGL(minmax_kernel<PxF16>, dim3(blocks), dim3(256), 0, st, static_cast<const uint16_t*>(px), n, out);
+  else
+    hipLaunchKernelGGL(minmax_kernel<PxF32>, dim3(blocks), dim3(256), 0, st, static_cast<const float*>(px), n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_storage(const float* in, void* out, int storage, int64_t n, hipStream_t st) {
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if (storage == kStorageBF16)
+    hipLaunchKernelGGL(to_storage_kernel<true>, grid, dim3(256), 0, st, in, static_cast<uint16_t*>(out), n);
+  else if (storage == kStorageF16)
+    hipLaunchKernelGGL(to_storage_kernel<false>, grid, dim3(256), 0, st, in, static_cast<uint16_t*>(out), n);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_texels_to_storage(const float4* in, void* out, int storage, int64_t n, hipStream_t st) {
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if (storage == kStorageBF16)
+    hipLaunchKernelGGL(texels_to_storage_kernel<true>, grid, dim3(256), 0, st, in, static_cast<uint2*>(out), n);
+  else if (storage == kStorageF16)
+    hipLaunchKernelGGL(texels_to_storage_kernel<false>, grid, dim3(256), 0, st, in, static_cast<uint2*>(out), n);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from _cases import REFERENCE_KNOWN_VALUES, affine_for, affine_params, blob_problem, bspline_for, reference_test_images
+from _gpu import _compare, _gpu_metric
 from itk_amd import synth
 from itk_amd._native import MetricError
 from itk_amd.geometry import ImageGeometry
@@ -16,43 +17,6 @@ from itk_amd.metric import MattesMutualInformationImageToImageMetricv4 as Metric
 from itk_amd.transforms import AffineTransform, BSplineTransform
 
 pytestmark = pytest.mark.gpu
-
-VALUE_RTOL = 1e-5
-DERIV_RTOL = 1e-4
-
-
-def _gpu_metric(fixed, moving, geom, tf, *, gradient=None, use_filter=True, virtual=None, bins=50, record=True):
-    m = Metric(record_sample_bins=record)
-    m.SetFixedImage(fixed, geom)
-    m.SetMovingImage(moving, geom)
-    m.SetUseMovingImageGradientFilter(use_filter)
-    if gradient is not None:
-        m.SetMovingImageGradient(gradient)
-    if virtual is not None:
-        m.SetVirtualDomain(virtual)
-    m.SetNumberOfHistogramBins(bins)
-    m.SetMovingTransform(tf)
-    m.Initialize()
-    return m
-
-
-def _compare(oracle_mod, m, fixed, moving, geom, tf, *, gradient=None, virtual=None, bins=50, check_bins=True):
-    v, d = m.GetValueAndDerivative()
-    r = oracle_mod.mattes(fixed, geom, moving, geom, tf, bins=bins, virtual_geom=virtual,
-                          moving_gradient=gradient, want_samples=check_bins, threads=8)
-    print("value gpu %.17g oracle %.17g  nvalid %d/%d" % (v, r["value"], m.GetNumberOfValidPoints(), r["n_valid"]))
-    assert m.GetNumberOfValidPoints() == r["n_valid"]
-    if check_bins:
-        fb, mb = m.GetSampleBins()
-        assert np.array_equal(fb, r["fixed_bin"])
-        assert np.array_equal(mb, r["moving_bin"])
-    # relative bar, with an absolute floor for metrics that are ~0 (e.g. 5 bins)
-    assert abs(v - r["value"]) <= VALUE_RTOL * abs(r["value"]) + 1e-11
-    err = np.linalg.norm(d - r["derivative"]) / max(np.linalg.norm(r["derivative"]), 1e-9)
-    print("derivative rel L2 error %.3e" % err)
-    assert err <= DERIV_RTOL
-    return v, d, r
-
 
 def test_reference_unit_test_case(oracle_mod):
     """The reference test's images and settings (central differences, 50 bins,

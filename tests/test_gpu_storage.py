@@ -1,0 +1,107 @@
+"""16-bit image storage (BF16 / FP16) of the HIP metric.
+
+With 16-bit storage the metric is that of the rounded images: the fixed,
+moving and gradient images are rounded to nearest even once, and every later
+read sees the rounded values.  Pinned two ways:
+  * bit-identical to the FP32-storage metric run on images (and a gradient)
+    rounded on the host, which pins the storage plumbing exactly;
+  * parity with the oracle on those rounded images (bins and counts
+    bit-exact, value 1e-5, derivative 1e-4), fast and exact paths alike.
+The error against the FP64 oracle on the unrounded images is reported, with a
+loose bound that a wrong conversion (swapped halves, truncation instead of
+rounding, a dropped gradient component) would exceed.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from _cases import affine_for, affine_params, blob_problem, bspline_for
+from _gpu import _compare, _gpu_metric
+
+pytestmark = pytest.mark.gpu
+
+
+def round_bf16(a: np.ndarray) -> np.ndarray:
+    """FP32 -> BF16 -> FP32, round to nearest even."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+def round_f16(a: np.ndarray) -> np.ndarray:
+    return np.asarray(a, dtype=np.float32).astype(np.float16).astype(np.float32)
+
+
+ROUND = {"bf16": round_bf16, "f16": round_f16}
+
+
+def test_rounding_helpers():
+    x = np.array([1.0, 1.00390625, 1.005859375, -2.5, 3.0e-3], dtype=np.float32)
+    # 1 + 2^-8 is halfway between bf16 neighbours 1 and 1 + 2^-7: ties to even (1.0)
+    assert round_bf16(x)[1] == 1.0
+    assert round_bf16(x)[2] == np.float32(1.0078125)
+    assert round_bf16(x)[3] == -2.5
+    assert round_f16(x)[1] == np.float32(1.00390625)
+
+
+def _transform(kind, geom, seed):
+    if kind == "affine":
+        return affine_for(geom, affine_params(3, np.random.default_rng(seed), scale=2.0))
+    return bspline_for(geom, (4, 3, 3), seed=seed, sigma=0.8)
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+@pytest.mark.parametrize("kind", ["affine", "bspline"])
+def test_storage_equals_f32_on_rounded_images(oracle_mod, storage, kind):
+    fixed, moving, geom = blob_problem((40, 36, 32), 31)
+    rnd = ROUND[storage]
+    grad = rnd(oracle_mod.gradient_image(moving, geom)).astype(np.float64)
+    tf = _transform(kind, geom, 32)
+    m16 = _gpu_metric(fixed, moving, geom, tf, gradient=grad, storage=storage)
+    m32 = _gpu_metric(rnd(fixed), rnd(moving), geom, tf, gradient=grad)
+    v16, d16 = m16.GetValueAndDerivative()
+    v32, d32 = m32.GetValueAndDerivative()
+    assert m16.GetNumberOfValidPoints() == m32.GetNumberOfValidPoints()
+    for a, b in zip(m16.GetSampleBins(), m32.GetSampleBins()):
+        assert np.array_equal(a, b)
+    assert v16 == v32
+    assert np.array_equal(d16, d32)
+    _compare(oracle_mod, m16, rnd(fixed), rnd(moving), geom, tf, gradient=grad)
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_storage_exact_path_parity(oracle_mod, storage):
+    """The all-FP64 path reading 16-bit storage, against the oracle on the
+    rounded images."""
+    fixed, moving, geom = blob_problem((36, 34, 30), 33)
+    rnd = ROUND[storage]
+    grad = rnd(oracle_mod.gradient_image(moving, geom)).astype(np.float64)
+    tf = _transform("affine", geom, 34)
+    m = _gpu_metric(fixed, moving, geom, tf, gradient=grad, storage=storage, exact=True)
+    _compare(oracle_mod, m, rnd(fixed), rnd(moving), geom, tf, gradient=grad)
+
+
+@pytest.mark.parametrize("storage,value_tol,deriv_tol", [("f32", 1e-4, 1e-3), ("bf16", 2e-2, 0.15),
+                                                         ("f16", 2e-3, 0.02)])
+def test_storage_error_vs_fp64_oracle(oracle_mod, storage, value_tol, deriv_tol):
+    """Error of each storage against the FP64 oracle (float64 images, the
+    gradient filter in FP64) at 10 random affine parameter sets."""
+    fixed, moving, geom = blob_problem((48, 44, 40), 35)
+    grad = oracle_mod.gradient_image(moving, geom)
+    tf = _transform("affine", geom, 36)
+    m = _gpu_metric(fixed, moving, geom, tf, storage=storage, record=False)
+    rng = np.random.default_rng(37)
+    worst_v = worst_d = 0.0
+    for _ in range(10):
+        tf.set_parameters(affine_params(3, rng, scale=2.0))
+        v, d = m.GetValueAndDerivative()
+        r = oracle_mod.mattes(fixed.astype(np.float64), geom, moving.astype(np.float64), geom, tf,
+                              moving_gradient=grad, threads=8)
+        ev = abs(v - r["value"]) / abs(r["value"])
+        ed = np.linalg.norm(d - r["derivative"]) / np.linalg.norm(r["derivative"])
+        worst_v, worst_d = max(worst_v, ev), max(worst_d, ed)
+    print("%s storage vs FP64 oracle: worst value rel err %.3e, worst derivative rel L2 err %.3e"
+          % (storage, worst_v, worst_d))
+    assert worst_v <= value_tol
+    assert worst_d <= deriv_tol
