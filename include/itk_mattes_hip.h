@@ -98,6 +98,14 @@ int mmi_set_gradient_source(mmi_context* ctx, int32_t source);
  * counterpart in the reference (its images are the pixel type itself). */
 int mmi_set_storage(mmi_context* ctx, int32_t pixel_type);
 
+/* Per-level smoothing of the fixed and moving images, as the registration
+ * method prepares each level (itkImageRegistrationMethodv4.hxx:591-637 with
+ * SmoothingRecursiveGaussianImageFilter): physical sigma per axis, NULL or all
+ * zero for none.  The uploaded images stay resident; each mmi_initialize
+ * smooths them on the device, and the gradient filter, intensity range and
+ * 16-bit storage use the smoothed images. */
+int mmi_set_smoothing_sigmas(mmi_context* ctx, const double* fixed_sigma, const double* moving_sigma);
+
 /* Virtual domain (ObjectToObjectMetric::SetVirtualDomain,
  * Modules/Numerics/Optimizersv4/include/itkObjectToObjectMetric.hxx:210-244).
  * If never called, the fixed image's geometry is used. */

@@ -41,6 +41,7 @@ SIGNATURES = {
     "mmi_compute_moving_gradient": (ctypes.c_int, [_vp]),
     "mmi_set_gradient_source": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "mmi_set_storage": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mmi_set_smoothing_sigmas": (ctypes.c_int, [_vp, _dp, _dp]),
     "mmi_set_virtual_domain": (ctypes.c_int, [_vp, _gp]),
     "mmi_set_number_of_histogram_bins": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "mmi_initialize": (ctypes.c_int, [_vp]),

@@ -254,10 +254,16 @@ struct mmi_context {
   int32_t storage = MMI_F32;
 
   bool have_fixed = false, have_moving = false, have_virtual = false, have_texels = false;
+  bool user_texels = false;  // gradient given by mmi_set_moving_gradient (not recomputed per level)
   mmi_geometry fixed_g{}, moving_g{}, virtual_g{};
   DevBuf fixed_px, moving_px, texels;  // FP32 uploads and FP32 gradient texels
   DevBuf fixed_lp, moving_lp, texels_lp;  // 16-bit storage copies (MMI_BF16 / MMI_F16)
   int32_t active_storage = MMI_F32;       // storage the evaluation reads, fixed at mmi_initialize
+  // per-level smoothing (physical sigma per axis, 0 = none) and the smoothed images
+  double fixed_sigma[3] = {0, 0, 0}, moving_sigma[3] = {0, 0, 0};
+  double texel_sigma[3] = {0, 0, 0};  // moving smoothing the filter texels were computed from
+  DevBuf fixed_work, moving_work;
+  bool fixed_smoothed = false, moving_smoothed = false;
 
   bool initialized = false;
   double fixed_min = 0, fixed_max = 0, moving_min = 0, moving_max = 0;
@@ -523,15 +529,61 @@ bool setup_fast(const mmi_context* ctx, FastArgs* F) {
   return true;
 }
 
-// Device images the evaluation reads: the FP32 uploads or their 16-bit copies.
+// FP32 images of the current level: the uploads, or their smoothed copies.
+const float* level_fixed(const mmi_context* ctx) {
+  return ctx->fixed_smoothed ? ctx->fixed_work.as<float>() : ctx->fixed_px.as<float>();
+}
+const float* level_moving(const mmi_context* ctx) {
+  return ctx->moving_smoothed ? ctx->moving_work.as<float>() : ctx->moving_px.as<float>();
+}
+
+// Device images the evaluation reads: the level's FP32 images or their 16-bit copies.
 const void* fixed_pixels(const mmi_context* ctx) {
-  return ctx->active_storage == MMI_F32 ? ctx->fixed_px.p : ctx->fixed_lp.p;
+  return ctx->active_storage == MMI_F32 ? (const void*)level_fixed(ctx) : ctx->fixed_lp.p;
 }
 const void* moving_pixels(const mmi_context* ctx) {
-  return ctx->active_storage == MMI_F32 ? ctx->moving_px.p : ctx->moving_lp.p;
+  return ctx->active_storage == MMI_F32 ? (const void*)level_moving(ctx) : ctx->moving_lp.p;
 }
 const void* texel_pixels(const mmi_context* ctx) {
   return ctx->active_storage == MMI_F32 ? ctx->texels.p : ctx->texels_lp.p;
+}
+
+// SmoothingRecursiveGaussianImageFilter on the device
+// (itkSmoothingRecursiveGaussianImageFilter.hxx:30-70, 119-131, 199-260):
+// zero-order recursive Gaussian along the last axis, then axes 0 .. D-2, in
+// FP64 (normalise across scale off), cast to FP32 at the end.
+int smooth_image(mmi_context* ctx, const float* in, const mmi_geometry& g, const double* sigma, DevBuf& out) {
+  const int D = g.dim;
+  const int64_t n = num_pixels(g);
+  for (int i = 0; i < D; ++i)
+    if (g.size[i] < 4)
+      return fail(MMI_ERR_ARGUMENT,
+                  "The number of pixels along dimension " + std::to_string(i) +
+                      " is less than 4. This filter requires a minimum of four pixels along the dimension to be "
+                      "processed.");
+  DevBuf a, b;
+  MMI_HIP(a.ensure(sizeof(double) * n));
+  MMI_HIP(b.ensure(sizeof(double) * n));
+  MMI_HIP(out.ensure(sizeof(float) * n));
+  MMI_HIP(launch_float_to_double(in, a.as<double>(), n, ctx->stream));
+  double* src = a.as<double>();
+  double* dst = b.as<double>();
+  for (int step = 0; step < D; ++step) {
+    const int axis = step == 0 ? D - 1 : step - 1;
+    RecCoeffs c;
+    if (!rec_setup(sigma[axis], g.spacing[axis], 0, false, &c))
+      return fail(MMI_ERR_ARGUMENT, "recursive Gaussian: spacing or sigma out of range");
+    int64_t stride = 1;
+    for (int j = 0; j < axis; ++j) stride *= g.size[j];
+    const int64_t ln = g.size[axis];
+    MMI_HIP(launch_recursive_gaussian(src, dst, ln, stride, n / ln, c, ctx->stream));
+    std::swap(src, dst);
+  }
+  MMI_HIP(launch_double_to_float(src, out.as<float>(), n, ctx->stream));
+  MMI_HIP(hipStreamSynchronize(ctx->stream));
+  a.release();
+  b.release();
+  return MMI_OK;
 }
 
 // A new gradient image after mmi_initialize with 16-bit storage: refresh the
@@ -675,6 +727,7 @@ int mmi_context_destroy(mmi_context* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   DevBuf* bufs[] = {&ctx->fixed_px, &ctx->moving_px, &ctx->texels, &ctx->fixed_lp, &ctx->moving_lp, &ctx->texels_lp,
+                    &ctx->fixed_work, &ctx->moving_work,
                     &ctx->coeffs, &ctx->hist, &ctx->fmarg,
                     &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->result,
                     &ctx->partials, &ctx->deriv, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
@@ -720,6 +773,8 @@ int mmi_set_moving_image(mmi_context* ctx, const mmi_geometry* g, int32_t pixel_
   ctx->moving_g = *g;
   ctx->have_moving = true;
   ctx->have_texels = false;
+  ctx->user_texels = false;
+  ctx->moving_smoothed = false;
   ctx->initialized = false;
   return MMI_OK;
 }
@@ -741,6 +796,7 @@ int mmi_set_moving_gradient(mmi_context* ctx, const double* gradient) {
   MMI_HIP(hipStreamSynchronize(ctx->stream));
   tmp.release();
   ctx->have_texels = true;
+  ctx->user_texels = true;
   return refresh_storage_texels(ctx);
 }
 
@@ -766,7 +822,7 @@ int mmi_compute_moving_gradient(mmi_context* ctx) {
   MMI_HIP(b.ensure(sizeof(double) * n));
   MMI_HIP(grad.ensure(sizeof(double) * n * D));
   for (int dim = 0; dim < D; ++dim) {
-    MMI_HIP(launch_float_to_double(ctx->moving_px.as<float>(), a.as<double>(), n, ctx->stream));
+    MMI_HIP(launch_float_to_double(level_moving(ctx), a.as<double>(), n, ctx->stream));
     double* src = a.as<double>();
     double* dst = b.as<double>();
     // derivative along dim, then zero order along the other axes in
@@ -791,13 +847,14 @@ int mmi_compute_moving_gradient(mmi_context* ctx) {
   Direction dir{};
   for (int i = 0; i < D * D; ++i) dir.m[i] = g.direction[i];
   MMI_HIP(ctx->texels.ensure(sizeof(float4) * n));
-  MMI_HIP(launch_pack_texels(ctx->moving_px.as<float>(), grad.as<double>(), n, D, dir, ctx->texels.as<float4>(),
-                             ctx->stream));
+  MMI_HIP(launch_pack_texels(level_moving(ctx), grad.as<double>(), n, D, dir, ctx->texels.as<float4>(), ctx->stream));
   MMI_HIP(hipStreamSynchronize(ctx->stream));
   a.release();
   b.release();
   grad.release();
   ctx->have_texels = true;
+  ctx->user_texels = false;
+  for (int i = 0; i < 3; ++i) ctx->texel_sigma[i] = ctx->moving_smoothed ? ctx->moving_sigma[i] : 0.0;
   return refresh_storage_texels(ctx);
 }
 
@@ -817,6 +874,26 @@ int mmi_set_virtual_domain(mmi_context* ctx, const mmi_geometry* g) {
   return MMI_OK;
 }
 
+int mmi_set_smoothing_sigmas(mmi_context* ctx, const double* fixed_sigma, const double* moving_sigma) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (!ctx->have_fixed || !ctx->have_moving) return fail(MMI_ERR_ARGUMENT, "set the fixed and moving images first");
+  const int D = ctx->fixed_g.dim;
+  const double* src[2] = {fixed_sigma, moving_sigma};
+  double* dst[2] = {ctx->fixed_sigma, ctx->moving_sigma};
+  for (int k = 0; k < 2; ++k) {
+    int positive = 0;
+    for (int i = 0; i < 3; ++i) dst[k][i] = (src[k] && i < D) ? src[k][i] : 0.0;
+    for (int i = 0; i < D; ++i) {
+      if (!(dst[k][i] >= 0.0)) return fail(MMI_ERR_ARGUMENT, "smoothing sigmas must be >= 0");
+      positive += dst[k][i] > 0.0;
+    }
+    if (positive != 0 && positive != D)
+      return fail(MMI_ERR_ARGUMENT, "smoothing sigmas must be all zero or all positive");
+  }
+  ctx->initialized = false;
+  return MMI_OK;
+}
+
 int mmi_set_number_of_histogram_bins(mmi_context* ctx, int32_t bins) {
   if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
   ctx->bins = std::max<int32_t>(5, bins);  // itkSetClampMacro(..., 5, max)
@@ -830,7 +907,25 @@ int mmi_initialize(mmi_context* ctx) {
   if (!ctx->have_moving) return fail(MMI_ERR_ARGUMENT, "MovingImage is not present");
   if (ctx->fixed_g.dim != ctx->moving_g.dim) return fail(MMI_ERR_ARGUMENT, "fixed and moving dimensions differ");
   MMI_HIP(hipSetDevice(ctx->device));
-  if (ctx->gradient_source == MMI_GRADIENT_IMAGE && !ctx->have_texels) {
+  // Per-level smoothing of the fixed and moving images
+  // (ImageRegistrationMethodv4::InitializeRegistrationAtEachLevel,
+  // itkImageRegistrationMethodv4.hxx:591-637).
+  const int D = ctx->fixed_g.dim;
+  ctx->fixed_smoothed = ctx->moving_smoothed = false;
+  if (ctx->fixed_sigma[0] > 0.0) {
+    if (int rc = smooth_image(ctx, ctx->fixed_px.as<float>(), ctx->fixed_g, ctx->fixed_sigma, ctx->fixed_work))
+      return rc;
+    ctx->fixed_smoothed = true;
+  }
+  if (ctx->moving_sigma[0] > 0.0) {
+    if (int rc = smooth_image(ctx, ctx->moving_px.as<float>(), ctx->moving_g, ctx->moving_sigma, ctx->moving_work))
+      return rc;
+    ctx->moving_smoothed = true;
+  }
+  bool stale = !ctx->have_texels;
+  if (ctx->have_texels && !ctx->user_texels)
+    for (int i = 0; i < D; ++i) stale = stale || ctx->texel_sigma[i] != (ctx->moving_smoothed ? ctx->moving_sigma[i] : 0.0);
+  if (ctx->gradient_source == MMI_GRADIENT_IMAGE && stale) {
     if (int rc = mmi_compute_moving_gradient(ctx)) return rc;
   }
   // 16-bit storage: every later read of the images (intensity range, bins,
@@ -840,8 +935,8 @@ int mmi_initialize(mmi_context* ctx) {
     const int64_t nf = num_pixels(ctx->fixed_g), nm = num_pixels(ctx->moving_g);
     MMI_HIP(ctx->fixed_lp.ensure(sizeof(uint16_t) * nf));
     MMI_HIP(ctx->moving_lp.ensure(sizeof(uint16_t) * nm));
-    MMI_HIP(launch_to_storage(ctx->fixed_px.as<float>(), ctx->fixed_lp.p, ctx->storage, nf, ctx->stream));
-    MMI_HIP(launch_to_storage(ctx->moving_px.as<float>(), ctx->moving_lp.p, ctx->storage, nm, ctx->stream));
+    MMI_HIP(launch_to_storage(level_fixed(ctx), ctx->fixed_lp.p, ctx->storage, nf, ctx->stream));
+    MMI_HIP(launch_to_storage(level_moving(ctx), ctx->moving_lp.p, ctx->storage, nm, ctx->stream));
     if (ctx->have_texels) {
       MMI_HIP(ctx->texels_lp.ensure(sizeof(uint2) * nm));
       MMI_HIP(launch_texels_to_storage(ctx->texels.as<float4>(), ctx->texels_lp.p, ctx->storage, nm, ctx->stream));

@@ -168,6 +168,7 @@ hipError_t launch_texels_to_storage(const float4* in, void* out, int storage, in
 hipError_t launch_recursive_gaussian(const double* in, double* out, int64_t ln, int64_t stride, int64_t nlines,
                                      const RecCoeffs& c, hipStream_t st);
 hipError_t launch_float_to_double(const float* in, double* out, int64_t n, hipStream_t st);
+hipError_t launch_double_to_float(const double* in, float* out, int64_t n, hipStream_t st);
 hipError_t launch_gradient_component(const double* vol, int64_t n, double spacing, int comp, double* grad, int D,
                                      hipStream_t st);
 hipError_t launch_pack_texels(const float* moving, const double* grad, int64_t n, int D, const Direction& dir,

@@ -155,6 +155,12 @@ __global__ void float_to_double_kernel(const float* __restrict__ in, double* __r
   if (i < n) out[i] = (double)in[i];
 }
 
+// Cast of the smoothing mini-pipeline's FP64 result to the FP32 pixel type.
+__global__ void double_to_float_kernel(const double* __restrict__ in, float* __restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
+
 // gradient component `comp` = vol / spacing into the texel's slot comp+1.
 __global__ void gradient_component_kernel(const double* __restrict__ vol, int64_t n, double spacing, int comp,
                                           double* __restrict__ grad, int D) {
@@ -358,6 +364,11 @@ hipError_t launch_recursive_gaussian(const double* in, double* out, int64_t ln, 
 
 hipError_t launch_float_to_double(const float* in, double* out, int64_t n, hipStream_t st) {
   hipLaunchKernelGGL(float_to_double_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_double_to_float(const double* in, float* out, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(double_to_float_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, out, n);
   return hipGetLastError();
 }
 

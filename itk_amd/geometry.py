@@ -15,6 +15,7 @@ buffered-region start index, an origin, a spacing and a direction matrix.
 from __future__ import annotations
 
 import ctypes
+import math
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -129,3 +130,35 @@ class ImageGeometry:
     def center_point(self) -> np.ndarray:
         """Physical centre of the buffer (the usual affine centre)."""
         return self.index_to_point([(s - 1) / 2.0 for s in self.size])
+
+    def continuous_index_to_point(self, ci) -> np.ndarray:
+        """ImageBase::TransformContinuousIndexToPhysicalPoint (itkImageBase.h:560-572):
+        the matrix product first, then the origin."""
+        d = self.dim
+        p = np.zeros(d)
+        for r in range(d):
+            s = 0.0
+            for c in range(d):
+                s += self.index_to_physical[r, c] * float(ci[c])
+            p[r] = s + self.origin[r]
+        return p
+
+    def shrink(self, factors) -> "ImageGeometry":
+        """Output geometry of ShrinkImageFilter (itkShrinkImageFilter.hxx:236-298):
+        spacing times the factor, size rounded down (at least 1), start rounded
+        up, and the origin moved so the physical centres coincide."""
+        d = self.dim
+        f = [int(x) for x in (factors if not np.isscalar(factors) else [factors] * d)]
+        if any(x < 1 for x in f):
+            raise ValueError("shrink factors must be >= 1")
+        spacing = [self.spacing[i] * float(f[i]) for i in range(d)]
+        size = [max(1, int(math.floor(float(self.size[i]) / float(f[i])))) for i in range(d)]
+        start = [int(math.ceil(float(self.start[i]) / float(f[i]))) for i in range(d)]
+        in_center = [self.start[i] + (self.size[i] - 1) / 2.0 for i in range(d)]
+        out_center = [start[i] + (size[i] - 1) / 2.0 for i in range(d)]
+        provisional = ImageGeometry(size=size, origin=self.origin, spacing=spacing, direction=self.direction,
+                                    start=start)
+        a = self.continuous_index_to_point(in_center)
+        b = provisional.continuous_index_to_point(out_center)
+        origin = [self.origin[i] + (a[i] - b[i]) for i in range(d)]
+        return ImageGeometry(size=size, origin=origin, spacing=spacing, direction=self.direction, start=start)

@@ -56,6 +56,9 @@ class MattesMutualInformationImageToImageMetricv4:
         self._n_valid = 0
         self._last_pdf = None
         self._storage = N.MMI_F32
+        self._fixed_sigma = self._moving_sigma = None
+        self._images_dirty = True    # host images not yet on the device
+        self._gradient_dirty = True  # user gradient not yet on the device
         self._initialized = False
 
     # -- lifetime ---------------------------------------------------------
@@ -73,16 +76,33 @@ class MattesMutualInformationImageToImageMetricv4:
     # -- inputs -----------------------------------------------------------
     def SetFixedImage(self, pixels: np.ndarray, geometry: ImageGeometry):
         self._fixed = (self._as_pixels(pixels, geometry), geometry)
+        self._images_dirty = True
         self._initialized = False
 
     def SetMovingImage(self, pixels: np.ndarray, geometry: ImageGeometry):
         self._moving = (self._as_pixels(pixels, geometry), geometry)
         self._gradient = None
+        self._images_dirty = True
         self._initialized = False
 
     def SetMovingImageGradient(self, gradient: np.ndarray):
         """Use a precomputed moving-image gradient (dim doubles per voxel)."""
         self._gradient = np.ascontiguousarray(gradient, dtype=np.float64)
+        self._gradient_dirty = True
+        self._initialized = False
+
+    def SetSmoothingSigmas(self, fixed_sigma=None, moving_sigma=None):
+        """Smooth the fixed and moving images on the device at Initialize
+        (SmoothingRecursiveGaussianImageFilter, physical sigma per axis or one
+        value; None or 0 = no smoothing), as the registration method prepares
+        each level.  The unsmoothed uploads stay resident."""
+        def norm(s):
+            if s is None:
+                return None
+            a = np.asarray(s, dtype=np.float64).reshape(-1)
+            return None if np.all(a == 0) else a
+        self._fixed_sigma = norm(fixed_sigma)
+        self._moving_sigma = norm(moving_sigma)
         self._initialized = False
 
     def SetUseMovingImageGradientFilter(self, on: bool):
@@ -148,18 +168,26 @@ class MattesMutualInformationImageToImageMetricv4:
             raise N.MetricError(N.MMI_ERR_ARGUMENT, "MovingTransform is not present")
         L, ctx = self._L, self._ctx
         N.check(L.mmi_set_storage(ctx, self._storage))
-        for setter, (px, g) in ((L.mmi_set_fixed_image, self._fixed), (L.mmi_set_moving_image, self._moving)):
-            cg = g.to_c()
-            N.check(setter(ctx, ctypes.byref(cg), N.MMI_F64 if px.dtype == np.float64 else N.MMI_F32,
-                           px.ctypes.data))
+        if self._images_dirty:
+            for setter, (px, g) in ((L.mmi_set_fixed_image, self._fixed), (L.mmi_set_moving_image, self._moving)):
+                cg = g.to_c()
+                N.check(setter(ctx, ctypes.byref(cg), N.MMI_F64 if px.dtype == np.float64 else N.MMI_F32,
+                               px.ctypes.data))
+            self._images_dirty = False
+            self._gradient_dirty = True
         N.check(L.mmi_set_gradient_source(
             ctx, N.MMI_GRADIENT_IMAGE if self._use_gradient_filter else N.MMI_GRADIENT_CENTRAL_DIFFERENCE))
-        if self._use_gradient_filter and self._gradient is not None:
+        if self._use_gradient_filter and self._gradient is not None and self._gradient_dirty:
             N.check(L.mmi_set_moving_gradient(ctx, _dptr(self._gradient)))
+            self._gradient_dirty = False
         N.check(L.mmi_set_number_of_histogram_bins(ctx, self._bins))
-        if self._virtual is not None:
-            cv = self._virtual.to_c()
-            N.check(L.mmi_set_virtual_domain(ctx, ctypes.byref(cv)))
+        d = self._fixed[1].dim
+        fs = None if self._fixed_sigma is None else np.ascontiguousarray(np.broadcast_to(self._fixed_sigma, (d,)))
+        ms = None if self._moving_sigma is None else np.ascontiguousarray(np.broadcast_to(self._moving_sigma, (d,)))
+        N.check(L.mmi_set_smoothing_sigmas(ctx, _dptr(fs) if fs is not None else None,
+                                           _dptr(ms) if ms is not None else None))
+        cv = (self._virtual or self._fixed[1]).to_c()
+        N.check(L.mmi_set_virtual_domain(ctx, ctypes.byref(cv)))
         if self._fixed_tf is not None:
             self._push_transform(L.mmi_set_fixed_transform, self._fixed_tf)
         self._push_transform(L.mmi_set_moving_transform, self._moving_tf)

@@ -905,8 +905,12 @@ int orc_gradient_recursive_gaussian(const orc_geometry* g, const void* pixels, i
   return 0;
 }
 
-int orc_smooth_recursive_gaussian(const orc_geometry* g, const void* pixels, int32_t f64, double sigma, void* out,
-                                  int32_t threads) {
+// SmoothingRecursiveGaussianImageFilter (itkSmoothingRecursiveGaussianImageFilter.hxx:30-70,
+// 119-131, 199-260): the first filter runs along the last axis, then one
+// filter per axis 0 .. dim-2, each zero order with that axis' sigma; the
+// mini-pipeline is FP64 and the cast to the output pixel type comes last.
+int orc_smooth_recursive_gaussian(const orc_geometry* g, const void* pixels, int32_t f64, const double* sigma,
+                                  void* out, int32_t threads) {
   const int D = g->dim;
   int64_t n = 1;
   for (int j = 0; j < D; ++j) {
@@ -915,9 +919,10 @@ int orc_smooth_recursive_gaussian(const orc_geometry* g, const void* pixels, int
   }
   std::vector<double> vol(n);
   for (int64_t i = 0; i < n; ++i) vol[i] = f64 ? ((const double*)pixels)[i] : (double)((const float*)pixels)[i];
-  for (int axis = 0; axis < D; ++axis) {
+  for (int step = 0; step < D; ++step) {
+    const int axis = step == 0 ? D - 1 : step - 1;
     RecCoeffs c;
-    if (rec_setup(sigma, axis_spacing(g, axis), 0, false, &c)) return 1;
+    if (rec_setup(sigma[axis], axis_spacing(g, axis), 0, false, &c)) return 1;
     filter_axis(g, vol.data(), axis, c, threads);
   }
   for (int64_t i = 0; i < n; ++i) {

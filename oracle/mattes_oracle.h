@@ -98,9 +98,11 @@ void orc_transform_point(const orc_transform* tf, int32_t dim, const double* in,
 int orc_gradient_recursive_gaussian(const orc_geometry* g, const void* pixels, int32_t pixel_f64, double sigma,
                                     double* out, int32_t threads);
 
-/* SmoothingRecursiveGaussianImageFilter with one sigma for all axes
- * (normalise across scale off); out is float or double like the input. */
-int orc_smooth_recursive_gaussian(const orc_geometry* g, const void* pixels, int32_t pixel_f64, double sigma,
+/* SmoothingRecursiveGaussianImageFilter: zero-order recursive Gaussian with
+ * sigma[axis] (physical units) along the last axis, then axes 0 .. dim-2;
+ * normalise across scale off; FP64 internally, out is float or double like
+ * the input. */
+int orc_smooth_recursive_gaussian(const orc_geometry* g, const void* pixels, int32_t pixel_f64, const double* sigma,
                                   void* out, int32_t threads);
 
 #ifdef __cplusplus

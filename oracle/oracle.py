@@ -104,7 +104,7 @@ def lib():
                                                       ctypes.c_double, _dp, ctypes.c_int32]
         L.orc_smooth_recursive_gaussian.restype = ctypes.c_int
         L.orc_smooth_recursive_gaussian.argtypes = [ctypes.POINTER(CGeometry), ctypes.c_void_p, ctypes.c_int32,
-                                                    ctypes.c_double, ctypes.c_void_p, ctypes.c_int32]
+                                                    _dp, ctypes.c_void_p, ctypes.c_int32]
         _lib = L
     return _lib
 
@@ -236,12 +236,14 @@ def gradient_image(img: np.ndarray, geom: ImageGeometry, sigma: float = None, th
     return out
 
 
-def smooth(img: np.ndarray, geom: ImageGeometry, sigma: float, threads: int = 4) -> np.ndarray:
+def smooth(img: np.ndarray, geom: ImageGeometry, sigma, threads: int = 4) -> np.ndarray:
+    """SmoothingRecursiveGaussianImageFilter; sigma is physical, one value or one per axis."""
     img = np.ascontiguousarray(img)
     f64 = 1 if img.dtype == np.float64 else 0
     out = np.zeros_like(img)
     g = geom.to_c()
-    if lib().orc_smooth_recursive_gaussian(ctypes.byref(g), img.ctypes.data, f64, sigma, out.ctypes.data, threads):
+    s = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma, dtype=np.float64), (geom.dim,)))
+    if lib().orc_smooth_recursive_gaussian(ctypes.byref(g), img.ctypes.data, f64, _ptr(s), out.ctypes.data, threads):
         raise OracleError("smoothing failed")
     return out
 
