@@ -32,8 +32,12 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_SAMPLE_3D = {"f32": 20, "bf16": 10, "f16": 10}
 # HBM bytes per evaluation (FETCH_SIZE + WRITE_SIZE summed over the evaluation's
 # kernels, scaled by the counters' reading for a copy of known size), from the
-# rocprofv3 --pmc runs summarised in profiles/.  None where not yet profiled.
-TRAFFIC_BYTES_PER_EVAL = {}
+# rocprofv3 --pmc runs summarised in profiles/mattes_affine_256cube_fast_path.md
+# and profiles/mattes_bspline_192cube_mesh16.md.  None where not profiled.
+TRAFFIC_BYTES_PER_EVAL = {
+    ("mattes_mi_dense_value_and_derivative_affine_256cube", "f32"): 588.2e6,
+    ("mattes_mi_dense_value_and_derivative_bspline_192cube_mesh16", "f32"): 751.4e6,
+}
 
 
 def parse():
