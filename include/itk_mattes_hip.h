@@ -156,6 +156,11 @@ int mmi_get_histogram_parameters(mmi_context* ctx, double* fixed_min, double* fi
  * the last mmi_evaluate, measured with HIP events on the context stream. */
 int mmi_get_last_timing(mmi_context* ctx, float* total_ms, float* pass1_ms, float* pass2_ms);
 
+/* Diagnostics (no counterpart in the reference): which kernels the last
+ * mmi_evaluate ran: 0 the exact FP64 path, 1 the certified fast path, 2 the
+ * fast path redone on the exact path after an exception-list overflow. */
+int mmi_get_last_path(mmi_context* ctx, int32_t* path);
+
 #ifdef __cplusplus
 }
 #endif

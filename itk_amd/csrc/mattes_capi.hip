@@ -276,6 +276,9 @@ struct mmi_context {
   DevBuf hist, fmarg, nvalid, pdf, fm, mm, pratio, pratio_f, rowsum, result, partials, deriv, sample_fbin, sample_mbin,
       scratch;
   DevBuf exc_count, exc_list, exc_overflow;  // fast-path exception lists
+  DevBuf ftab;                               // fast path: per-axis fixed cells (coarse levels)
+  bool ftab_valid = false;
+  mmi_geometry ftab_key_v{}, ftab_key_f{};
   // separable B-spline derivative: per-sample factors, contraction stages, axis tables
   DevBuf bs_g, bs_t1, bs_t2;
   DevBuf bs_start[3], bs_w[3], bs_inside[3], bs_lo[3], bs_hi[3];
@@ -286,6 +289,7 @@ struct mmi_context {
   int grid = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float last_total_ms = 0, last_pass1_ms = 0, last_pass2_ms = 0;
+  int32_t last_path = 0;  // 0 exact, 1 fast, 2 fast redone exactly
 };
 
 namespace {
@@ -414,10 +418,62 @@ bool fixed_direct_offsets(const mmi_geometry& v, const mmi_geometry& f, int64_t*
   return true;
 }
 
+// Per-axis fixed-image cells of the virtual samples (axis-aligned virtual and
+// fixed grids, identity fixed transform): for each virtual index along an axis,
+// the reference's continuous index (computed with its expressions: the other
+// axes' terms are exact zeros), the inside-buffer test of that axis and the
+// interpolation cell of LinearInterpolateImageFunction.  Cached on the device
+// per geometry pair.
+bool fixed_axis_tables(mmi_context* ctx, const mmi_geometry& v, const mmi_geometry& f, FastArgs* F) {
+  const int D = v.dim;
+  if (!is_diagonal(v.index_to_physical, D) || !is_diagonal(f.physical_to_index, D)) return false;
+  const bool same = ctx->ftab_valid && std::memcmp(&ctx->ftab_key_v, &v, sizeof(v)) == 0 &&
+                    std::memcmp(&ctx->ftab_key_f, &f, sizeof(f)) == 0;
+  int64_t off[4] = {0, 0, 0, 0};
+  for (int a = 0; a < D; ++a) off[a + 1] = off[a] + v.size[a];
+  if (!same) {
+    ctx->ftab_valid = false;
+    std::vector<int4> tab((size_t)off[D]);
+    int64_t stride = 1;
+    for (int a = 0; a < D; ++a) {
+      const double s = v.index_to_physical[a * D + a];
+      const double q = f.physical_to_index[a * D + a];
+      const int64_t fend = f.start[a] + f.size[a] - 1;
+      for (int64_t i = 0; i < v.size[a]; ++i) {
+        const double p = v.origin[a] + s * (double)(v.start[a] + i);
+        const double ci = q * (p - f.origin[a]);
+        int4 e = make_int4(0, 0, 0, 0);
+        if (ci >= (double)f.start[a] - 0.5 && ci < (double)fend + 0.5) {
+          int64_t b = (int64_t)itk_floor_host(ci);
+          if (b < f.start[a]) b = f.start[a];
+          const double d = ci - (double)b;
+          const bool active = d > 0.0 && b + 1 <= fend;
+          const float df = (float)d;
+          int32_t bits;
+          std::memcpy(&bits, &df, 4);
+          e = make_int4((int32_t)((b - f.start[a]) * stride), active ? (int32_t)stride : 0, active ? bits : 0, 1);
+        }
+        tab[(size_t)(off[a] + i)] = e;
+      }
+      stride *= f.size[a];
+    }
+    if (ctx->ftab.ensure(sizeof(int4) * tab.size()) != hipSuccess) return false;
+    if (hipMemcpyAsync(ctx->ftab.p, tab.data(), sizeof(int4) * tab.size(), hipMemcpyHostToDevice, ctx->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+      return false;
+    ctx->ftab_key_v = v;
+    ctx->ftab_key_f = f;
+    ctx->ftab_valid = true;
+  }
+  for (int a = 0; a < 3; ++a) F->ftab[a] = a < D ? ctx->ftab.as<int4>() + off[a] : nullptr;
+  return true;
+}
+
 // Parameters and error bounds of the certified fast path (FastArgs in
 // mattes_kernels.h; buffers are filled in by the caller).  Returns false when
 // the path does not apply.
-bool setup_fast(const mmi_context* ctx, FastArgs* F) {
+bool setup_fast(mmi_context* ctx, FastArgs* F) {
   std::memset(F, 0, sizeof(*F));
   const int D = ctx->fixed_g.dim;
   const int B = ctx->bins;
@@ -490,8 +546,14 @@ bool setup_fast(const mmi_context* ctx, FastArgs* F) {
   const double u = std::ldexp(1.0, -22);
   const double bt_m = 2.0 * ((ev + Mm * u) / ctx->moving_bin_size + u * (Mm / ctx->moving_bin_size +
                                                                           std::fabs(ctx->moving_norm_min) + B));
-  const double bt_f = 2.0 * (Mf * u / ctx->fixed_bin_size + u * (Mf / ctx->fixed_bin_size +
-                                                                std::fabs(ctx->fixed_norm_min) + B));
+  // fixed samples on pixel centres (direct reads), else per-axis cell tables
+  // and an FP32 interpolation with the same error bound as the moving value's
+  int64_t off[3] = {0, 0, 0};
+  const bool direct = fixed_direct_offsets(v, ctx->fixed_g, off);
+  if (!direct && !fixed_axis_tables(ctx, v, ctx->fixed_g, F)) return false;
+  const double ef = direct ? 0.0 : Mf * std::ldexp(1.0, -19);
+  const double bt_f = 2.0 * ((ef + Mf * u) / ctx->fixed_bin_size + u * (Mf / ctx->fixed_bin_size +
+                                                                       std::fabs(ctx->fixed_norm_min) + B));
   if (bt_m > 0.05 || bt_f > 0.05) return false;
   F->ev = (float)ev;
   F->bt_m = (float)bt_m;
@@ -502,8 +564,7 @@ bool setup_fast(const mmi_context* ctx, FastArgs* F) {
   F->fnm = (float)ctx->fixed_norm_min;
   F->mmin = (float)ctx->moving_min;
   F->mmax = (float)ctx->moving_max;
-  int64_t off[3] = {0, 0, 0};
-  if (!fixed_direct_offsets(v, ctx->fixed_g, off)) return false;
+  F->fixed_interp = direct ? 0 : 1;
   const mmi_geometry& fg = ctx->fixed_g;
   for (int k = 0; k < 3; ++k) {
     F->foff[k] = k < D ? (int32_t)(off[k] - fg.start[k]) : 0;
@@ -731,7 +792,7 @@ int mmi_context_destroy(mmi_context* ctx) {
                     &ctx->coeffs, &ctx->hist, &ctx->fmarg,
                     &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->result,
                     &ctx->partials, &ctx->deriv, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
-                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_overflow, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2};
+                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_overflow, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->ftab};
   for (DevBuf* b : bufs) b->release();
   for (int j = 0; j < 3; ++j) {
     ctx->bs_start[j].release();
@@ -1079,6 +1140,7 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
 
   FastArgs fa;
   bool fast = setup_fast(ctx, &fa);
+  ctx->last_path = fast ? 1 : 0;
   if (fast) {
     MMI_HIP(ctx->exc_count.ensure(sizeof(uint32_t) * ctx->grid));
     MMI_HIP(ctx->exc_list.ensure(sizeof(uint32_t) * ctx->grid * kExceptionCapacity));
@@ -1199,6 +1261,7 @@ retry:
   if (overflow) {
     // more uncertain samples in one workgroup than its list holds: redo exactly
     fast = false;
+    ctx->last_path = 2;
     goto retry;
   }
   float t1 = 0, t2 = 0;
@@ -1248,6 +1311,12 @@ int mmi_get_histogram_parameters(mmi_context* ctx, double* fixed_min, double* fi
   if (moving_max) *moving_max = ctx->moving_max;
   if (fixed_bin_size) *fixed_bin_size = ctx->fixed_bin_size;
   if (moving_bin_size) *moving_bin_size = ctx->moving_bin_size;
+  return MMI_OK;
+}
+
+int mmi_get_last_path(mmi_context* ctx, int32_t* path) {
+  if (!ctx || !path) return fail(MMI_ERR_ARGUMENT, "NULL argument");
+  *path = ctx->last_path;
   return MMI_OK;
 }
 

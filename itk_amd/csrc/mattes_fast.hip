@@ -127,9 +127,37 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
   const float mterm = fmaf(v, f.inv_mbs, -f.mnm);
   mbin = fast_bin(mterm, f.bt_m, f.bins, &uncertain);
   arg0 = (float)(mbin - 1) - mterm;
-  // fixed value: the pixel the fixed sample sits on (verified on the host)
-  const int32_t fl = (idx[0] + f.foff[0]) + f.fsy * (idx[1] + f.foff[1]) + (D == 3 ? f.fsz * (idx[2] + f.foff[2]) : 0);
-  const float fv = P::ld(static_cast<const typename P::pixel*>(f.fixed_px), fl);
+  const typename P::pixel* fimg = static_cast<const typename P::pixel*>(f.fixed_px);
+  float fv;
+  if (f.fixed_interp) {
+    // fixed cell from the per-axis tables (the reference's continuous index
+    // along each axis, computed on the host); FP32 multilinear value
+    const int4 tx = f.ftab[0][idx[0] - f.vstart[0]];
+    const int4 ty = f.ftab[1][idx[1] - f.vstart[1]];
+    const int4 tz = D == 3 ? f.ftab[2][idx[2] - f.vstart[2]] : make_int4(0, 0, 0, 1);
+    out = out | ((tx.w & ty.w & tz.w) == 0);
+    const int32_t fb = tx.x + ty.x + tz.x;
+    const float u0 = __int_as_float(tx.z), u1 = __int_as_float(ty.z);
+    const float f000 = P::ld(fimg, fb), f100 = P::ld(fimg, fb + tx.y);
+    const float f010 = P::ld(fimg, fb + ty.y), f110 = P::ld(fimg, fb + ty.y + tx.y);
+    const float fx00 = fmaf(f100 - f000, u0, f000);
+    const float fx10 = fmaf(f110 - f010, u0, f010);
+    fv = fmaf(fx10 - fx00, u1, fx00);
+    if (D == 3) {
+      const float u2 = __int_as_float(tz.z);
+      const float f001 = P::ld(fimg, fb + tz.y), f101 = P::ld(fimg, fb + tz.y + tx.y);
+      const float f011 = P::ld(fimg, fb + tz.y + ty.y), f111 = P::ld(fimg, fb + tz.y + ty.y + tx.y);
+      const float fx01 = fmaf(f101 - f001, u0, f001);
+      const float fx11 = fmaf(f111 - f011, u0, f011);
+      const float fxx1 = fmaf(fx11 - fx01, u1, fx01);
+      fv = fmaf(fxx1 - fv, u2, fv);
+    }
+  } else {
+    // the pixel the fixed sample sits on (verified on the host)
+    const int32_t fl =
+        (idx[0] + f.foff[0]) + f.fsy * (idx[1] + f.foff[1]) + (D == 3 ? f.fsz * (idx[2] + f.foff[2]) : 0);
+    fv = P::ld(fimg, fl);
+  }
   const float fterm = fmaf(fv, f.inv_fbs, -f.fnm);
   fbin = fast_bin(fterm, f.bt_f, f.bins, &uncertain);
   return out ? 0 : (uncertain ? 2 : 1);

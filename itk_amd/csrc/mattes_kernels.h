@@ -37,7 +37,12 @@ struct FastArgs {
   FastDiv vdx, vdy;
   const void* fixed_px;  // storage type given by `storage` (MMI_F32 / MMI_BF16 / MMI_F16)
   int32_t fsy, fsz;
-  int32_t foff[3];  // fixed pixel index = virtual index + foff (buffer-relative)
+  int32_t foff[3];  // fixed_interp == 0: fixed pixel index = virtual index + foff (buffer-relative)
+  int32_t fixed_interp;
+  // fixed_interp == 1: per-axis fixed cells of the virtual indices, one int4 per
+  // index: (lower-corner element offset, upper-corner delta or 0, FP32 bits of
+  // the interpolation weight or 0, inside-buffer flag)
+  const int4* ftab[3];
   int32_t storage;
   const void* moving_px;
   const void* texels;

@@ -273,6 +273,12 @@ class MattesMutualInformationImageToImageMetricv4:
         keys = ("fixed_min", "fixed_max", "moving_min", "moving_max", "fixed_bin_size", "moving_bin_size")
         return {k: v.value for k, v in zip(keys, vals)}
 
+    def GetLastEvaluationPath(self) -> str:
+        """"exact", "fast" or "fast_overflow_exact" for the last evaluation."""
+        v = ctypes.c_int32()
+        N.check(self._L.mmi_get_last_path(self._ctx, ctypes.byref(v)))
+        return ("exact", "fast", "fast_overflow_exact")[v.value]
+
     def GetLastTiming(self) -> dict:
         t = [ctypes.c_float() for _ in range(3)]
         N.check(self._L.mmi_get_last_timing(self._ctx, *[ctypes.byref(v) for v in t]))

@@ -207,6 +207,7 @@ def test_fast_path_matches_exact_path(oracle_mod, shape, seed):
         m.SetMovingTransform(tf)
         m.Initialize()
         v, dv = m.GetValueAndDerivative()
+        assert m.GetLastEvaluationPath() == ("exact" if exact else "fast")
         out.append((v, dv, m.GetNumberOfValidPoints()) + m.GetSampleBins())
         m.close()
     (v0, d0, n0, f0, m0), (v1, d1, n1, f1, m1) = out
@@ -248,3 +249,25 @@ def test_device_smoothing_matches_oracle(oracle_mod):
     m.SetSmoothingSigmas(None, None)
     m.Initialize()
     _compare(oracle_mod, m, fixed, moving, geom, tf, gradient=oracle_mod.gradient_image(moving, geom))
+
+
+@pytest.mark.parametrize("factors", [2, 4, [2, 3, 1]])
+def test_fast_path_on_shrunk_virtual_domain(oracle_mod, factors):
+    """Coarse pyramid levels: virtual samples fall between fixed pixels, so the
+    fast path interpolates the fixed image from per-axis cell tables.  Bins and
+    counts equal the exact path's and the oracle's."""
+    fixed, moving, geom = blob_problem((64, 60, 56), 61)
+    grad = oracle_mod.gradient_image(moving, geom)
+    virt = geom.shrink(factors)
+    tf = affine_for(geom, affine_params(3, np.random.default_rng(62), scale=2.0))
+    m_fast = _gpu_metric(fixed, moving, geom, tf, gradient=grad, virtual=virt)
+    m_exact = _gpu_metric(fixed, moving, geom, tf, gradient=grad, virtual=virt, exact=True)
+    v0, d0 = m_fast.GetValueAndDerivative()
+    assert m_fast.GetLastEvaluationPath() == "fast"
+    v1, d1 = m_exact.GetValueAndDerivative()
+    assert m_exact.GetLastEvaluationPath() == "exact"
+    assert m_fast.GetNumberOfValidPoints() == m_exact.GetNumberOfValidPoints()
+    for a, b in zip(m_fast.GetSampleBins(), m_exact.GetSampleBins()):
+        assert np.array_equal(a, b)
+    assert v0 == pytest.approx(v1, rel=1e-7)
+    _compare(oracle_mod, m_fast, fixed, moving, geom, tf, gradient=grad, virtual=virt)
