@@ -161,6 +161,29 @@ int mmi_get_last_timing(mmi_context* ctx, float* total_ms, float* pass1_ms, floa
  * fast path redone on the exact path after an exception-list overflow. */
 int mmi_get_last_path(mmi_context* ctx, int32_t* path);
 
+/* ---- Batched evaluation (serving many registrations at once) ----------------
+ * No counterpart in the reference (its metric evaluates one registration per
+ * call).  A batch evaluates several initialised contexts in one pass: those on
+ * the certified fast path (affine, identity fixed transform, gradient image;
+ * one dimension and storage per batch) share one launch of each kernel, which
+ * the batch captures as a HIP graph and replays while the set of contexts and
+ * their shapes stay the same; the others are evaluated one by one meanwhile.
+ * A context may appear once per batch; the moving parameters are the ones
+ * last set on each context.  values[i], n_valid[i], derivatives[i] (the
+ * context's parameter count, when want_derivative) and status[i] (MMI_OK or
+ * the code mmi_evaluate would return) receive context i's results; the call
+ * itself fails only on argument or device errors. */
+typedef struct mmi_batch mmi_batch;
+
+int mmi_batch_create(int device, mmi_batch** out);
+int mmi_batch_destroy(mmi_batch* batch);
+/* HIP graph capture and replay of the fused launch (default on). */
+int mmi_batch_set_graphs(mmi_batch* batch, int32_t enabled);
+int mmi_batch_evaluate(mmi_batch* batch, mmi_context* const* ctxs, int32_t n, int32_t want_derivative,
+                       double* values, double* const* derivatives, int64_t* n_valid, int32_t* status);
+/* Device time of the last fused launch and how many contexts it held. */
+int mmi_batch_get_last_timing(mmi_batch* batch, float* device_ms, int32_t* n_fused);
+
 #ifdef __cplusplus
 }
 #endif

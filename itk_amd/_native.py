@@ -51,6 +51,14 @@ SIGNATURES = {
     "mmi_evaluate": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp, ctypes.POINTER(ctypes.c_int64), _dp, _dp]),
     "mmi_get_sample_bins": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int16), ctypes.POINTER(ctypes.c_int16)]),
     "mmi_get_histogram_parameters": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "mmi_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "mmi_batch_destroy": (ctypes.c_int, [_vp]),
+    "mmi_batch_set_graphs": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mmi_batch_evaluate": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_int32, ctypes.c_int32, _dp,
+                                          ctypes.POINTER(_dp), ctypes.POINTER(ctypes.c_int64),
+                                          ctypes.POINTER(ctypes.c_int32)]),
+    "mmi_batch_get_last_timing": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float),
+                                                 ctypes.POINTER(ctypes.c_int32)]),
     "mmi_get_last_path": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "mmi_get_last_timing": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                            ctypes.POINTER(ctypes.c_float)]),

@@ -28,22 +28,25 @@ namespace mmi {
 
 constexpr int kXcd = 8;
 
+// Samples of local workgroup lb of lg (one evaluation's share of a launch).
 template <int D>
-__device__ __forceinline__ void run_range(const FastArgs& f, uint32_t& begin, uint32_t& end, uint32_t& step) {
+__device__ __forceinline__ void run_range(const FastArgs& f, uint32_t lb, uint32_t lg, uint32_t& begin, uint32_t& end,
+                                          uint32_t& step) {
   if (f.xcd_chunks) {
-    // workgroup b runs on XCD b % 8: give each XCD one contiguous share
-    const uint32_t xcd = blockIdx.x % kXcd;
-    const uint32_t lb = blockIdx.x / kXcd;
-    const uint32_t bpx = gridDim.x / kXcd;
+    // workgroup b runs on XCD b % 8 (lb and b agree mod 8: shares start at
+    // multiples of 8): give each XCD one contiguous share
+    const uint32_t xcd = lb % kXcd;
+    const uint32_t lbx = lb / kXcd;
+    const uint32_t bpx = lg / kXcd;
     const uint32_t chunk = (f.n + kXcd - 1) / kXcd;
     const uint32_t c0 = xcd * chunk;
-    begin = c0 + lb * blockDim.x + threadIdx.x;
+    begin = c0 + lbx * blockDim.x + threadIdx.x;
     end = min(f.n, c0 + chunk);
     step = bpx * blockDim.x;
   } else {
-    begin = blockIdx.x * blockDim.x + threadIdx.x;
+    begin = lb * blockDim.x + threadIdx.x;
     end = f.n;
-    step = gridDim.x * blockDim.x;
+    step = lg * blockDim.x;
   }
 }
 
@@ -164,7 +167,7 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
 }
 
 template <int D, typename P>
-__global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f) {
+__device__ __forceinline__ void pass1_fast_body(const FastArgs& f, uint32_t lb, uint32_t lg) {
   extern __shared__ unsigned long long lds_u64[];
   const int B = f.bins;
   const int BB = B * B;
@@ -176,7 +179,7 @@ __global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f) {
   __syncthreads();
   unsigned int nvalid = 0;
   uint32_t s, end, step;
-  run_range<D>(f, s, end, step);
+  run_range<D>(f, lb, lg, s, end, step);
   for (; s < end; s += step) {
     int32_t idx[3];
     fast_index<D>(f, s, idx);
@@ -186,7 +189,7 @@ __global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f) {
     const int c = fast_classify<D, P>(f, idx, fbin, mbin, arg0, cell);
     if (c == 2) {
       const unsigned int slot = atomicAdd(&counters[1], 1u);
-      if (slot < (unsigned int)f.exc_cap) f.exc_list[blockIdx.x * f.exc_cap + slot] = s;
+      if (slot < (unsigned int)f.exc_cap) f.exc_list[lb * f.exc_cap + slot] = s;
       continue;
     }
     if (f.sample_fbin) {
@@ -219,23 +222,23 @@ __global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f) {
   if (threadIdx.x == 0) {
     if (counters[0]) atomicAdd(f.nvalid, (unsigned long long)counters[0]);
     const unsigned int ne = counters[1];
-    f.exc_count[blockIdx.x] = ne;
+    f.exc_count[lb] = ne;
     if (ne > (unsigned int)f.exc_cap) atomicOr(f.exc_overflow, 1u);
   }
 }
 
 // Exact FP64 evaluation of the listed samples of one pass-1 workgroup.
 template <int D, typename P>
-__global__ void __launch_bounds__(256) exceptions_pass1_kernel(EvalArgs a, FastArgs f) {
+__device__ __forceinline__ void exceptions_pass1_body(const EvalArgs& a, const FastArgs& f, uint32_t lb) {
   extern __shared__ unsigned int lds_u32[];
-  const uint32_t ne = min(f.exc_count[blockIdx.x], (uint32_t)f.exc_cap);
+  const uint32_t ne = min(f.exc_count[lb], (uint32_t)f.exc_cap);
   if (ne == 0) return;  // uniform over the workgroup
   const int B = a.bins;
   unsigned int* fmarg = lds_u32;  // [B] fixed marginal counts, [B] valid count
   for (int i = threadIdx.x; i <= B; i += blockDim.x) fmarg[i] = 0u;
   __syncthreads();
   for (uint32_t t = threadIdx.x; t < ne; t += blockDim.x) {
-    const uint32_t s = f.exc_list[blockIdx.x * f.exc_cap + t];
+    const uint32_t s = f.exc_list[lb * f.exc_cap + t];
     int32_t idx[3];
     sample_index<D>(a.virt, s, idx);
     SampleOut<D> o;
@@ -276,7 +279,7 @@ __device__ __forceinline__ void add_moments(double* acc, float w, float3 g, cons
 }
 
 template <int D, typename P>
-__global__ void __launch_bounds__(256) pass2_fast_kernel(FastArgs f) {
+__device__ __forceinline__ void pass2_fast_body(const FastArgs& f, uint32_t lb, uint32_t lg) {
   extern __shared__ float lds_f32[];
   const int B = f.bins;
   const int BB = B * B;
@@ -289,7 +292,7 @@ __global__ void __launch_bounds__(256) pass2_fast_kernel(FastArgs f) {
 #pragma unroll
   for (int i = 0; i < PA; ++i) acc[i] = 0.0;
   uint32_t s, end, step;
-  run_range<D>(f, s, end, step);
+  run_range<D>(f, lb, lg, s, end, step);
   for (; s < end; s += step) {
     int32_t idx[3];
     fast_index<D>(f, s, idx);
@@ -301,22 +304,23 @@ __global__ void __launch_bounds__(256) pass2_fast_kernel(FastArgs f) {
     if (w == 0.0f) continue;
     add_moments<D>(acc, w, interp_gradient3<D, P>(static_cast<const typename P::texel*>(f.texels), cell), idx);
   }
-  block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)blockIdx.x * PA);
+  block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)lb * PA);
 }
 
 // Exception samples of one pass-1 workgroup, in sample order (bitonic sort of
 // the list in LDS), summed in a fixed tree: deterministic.
 template <int D, typename P>
-__global__ void __launch_bounds__(256) exceptions_pass2_kernel(EvalArgs a, FastArgs f) {
+__device__ __forceinline__ void exceptions_pass2_body(const EvalArgs& a, const FastArgs& f, uint32_t lb,
+                                                      uint32_t lg) {
   __shared__ uint32_t keys[256];
   __shared__ double scratch[4 * 12];
   constexpr int PA = D * D + D;
-  const uint32_t ne = min(f.exc_count[blockIdx.x], (uint32_t)f.exc_cap);
+  const uint32_t ne = min(f.exc_count[lb], (uint32_t)f.exc_cap);
   double acc[PA];
 #pragma unroll
   for (int i = 0; i < PA; ++i) acc[i] = 0.0;
   if (ne > 0) {
-    keys[threadIdx.x] = threadIdx.x < ne ? f.exc_list[blockIdx.x * f.exc_cap + threadIdx.x] : 0xFFFFFFFFu;
+    keys[threadIdx.x] = threadIdx.x < ne ? f.exc_list[lb * f.exc_cap + threadIdx.x] : 0xFFFFFFFFu;
     __syncthreads();
     for (uint32_t k = 2; k <= 256; k <<= 1) {
       for (uint32_t j = k >> 1; j > 0; j >>= 1) {
@@ -345,7 +349,7 @@ __global__ void __launch_bounds__(256) exceptions_pass2_kernel(EvalArgs a, FastA
       }
     }
   }
-  block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)(gridDim.x + blockIdx.x) * PA);
+  block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)(lg + lb) * PA);
 }
 
 // Fixed-order reduction of 2 * nblocks rows of index-space moments, then the
@@ -354,9 +358,8 @@ __global__ void __launch_bounds__(256) exceptions_pass2_kernel(EvalArgs a, FastA
 //   d[D*D + b]   = nFactor * M1[b]
 // with V the virtual index-to-physical matrix and c0 = virtual origin - centre.
 template <int D>
-__global__ void __launch_bounds__(256) fast_derivative_kernel(const double* __restrict__ partials, int rows,
-                                                              FastMap map, const DevResult* __restrict__ result,
-                                                              double* __restrict__ deriv) {
+__device__ __forceinline__ void fast_derivative_body(const double* __restrict__ partials, int rows, const FastMap& map,
+                                                     const DevResult* __restrict__ result, double* __restrict__ deriv) {
   constexpr int PA = D * D + D;
   __shared__ double scratch[4 * PA];
   __shared__ double mom[PA];
@@ -381,6 +384,112 @@ __global__ void __launch_bounds__(256) fast_derivative_kernel(const double* __re
       deriv[D * D + b] = m1 * nf;
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Kernels: one evaluation per launch, or a batch of evaluations per launch
+// (workgroup ranges per evaluation in `start`, multiples of 8 so that the
+// XCD of a workgroup and of its local index agree).
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int batch_item(const int32_t* __restrict__ start, int n, uint32_t block) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((uint32_t)start[mid] <= block) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int D, typename P>
+__global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f) {
+  pass1_fast_body<D, P>(f, blockIdx.x, gridDim.x);
+}
+
+template <int D, typename P>
+__global__ void __launch_bounds__(256) pass1_fast_batch(const FastArgs* __restrict__ fa,
+                                                        const int32_t* __restrict__ start, int n) {
+  const int r = batch_item(start, n, blockIdx.x);
+  pass1_fast_body<D, P>(fa[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+}
+
+template <int D, typename P>
+__global__ void __launch_bounds__(256) exceptions_pass1_kernel(EvalArgs a, FastArgs f) {
+  exceptions_pass1_body<D, P>(a, f, blockIdx.x);
+}
+
+template <int D, typename P>
+__global__ void __launch_bounds__(256) exceptions_pass1_batch(const EvalArgs* __restrict__ ea,
+                                                              const FastArgs* __restrict__ fa,
+                                                              const int32_t* __restrict__ start, int n) {
+  const int r = batch_item(start, n, blockIdx.x);
+  exceptions_pass1_body<D, P>(ea[r], fa[r], blockIdx.x - start[r]);
+}
+
+template <int D, typename P>
+__global__ void __launch_bounds__(256) pass2_fast_kernel(FastArgs f) {
+  pass2_fast_body<D, P>(f, blockIdx.x, gridDim.x);
+}
+
+template <int D, typename P>
+__global__ void __launch_bounds__(256) pass2_fast_batch(const FastArgs* __restrict__ fa,
+                                                        const int32_t* __restrict__ start, int n) {
+  const int r = batch_item(start, n, blockIdx.x);
+  pass2_fast_body<D, P>(fa[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+}
+
+template <int D, typename P>
+__global__ void __launch_bounds__(256) exceptions_pass2_kernel(EvalArgs a, FastArgs f) {
+  exceptions_pass2_body<D, P>(a, f, blockIdx.x, gridDim.x);
+}
+
+template <int D, typename P>
+__global__ void __launch_bounds__(256) exceptions_pass2_batch(const EvalArgs* __restrict__ ea,
+                                                              const FastArgs* __restrict__ fa,
+                                                              const int32_t* __restrict__ start, int n) {
+  const int r = batch_item(start, n, blockIdx.x);
+  exceptions_pass2_body<D, P>(ea[r], fa[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) fast_derivative_kernel(const double* __restrict__ partials, int rows,
+                                                              FastMap map, const DevResult* __restrict__ result,
+                                                              double* __restrict__ deriv) {
+  fast_derivative_body<D>(partials, rows, map, result, deriv);
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) fast_derivative_batch(const BatchDeriv* __restrict__ items) {
+  const BatchDeriv& it = items[blockIdx.x];
+  fast_derivative_body<D>(it.partials, it.rows, it.map, it.result, it.deriv);
+}
+
+// Zeroes one evaluation's histogram, marginal, count and overflow flag per
+// workgroup (a kernel rather than memsets, so the sequence can be captured).
+__global__ void __launch_bounds__(256) batch_clear(const FastArgs* __restrict__ fa) {
+  const FastArgs& f = fa[blockIdx.x];
+  const int BB = f.bins * f.bins;
+  for (int i = threadIdx.x; i < BB; i += blockDim.x) f.hist[i] = 0ull;
+  for (int i = threadIdx.x; i < f.bins; i += blockDim.x) f.fmarg[i] = 0ull;
+  if (threadIdx.x == 0) {
+    *f.nvalid = 0ull;
+    *f.exc_overflow = 0u;
+  }
+}
+
+// Value, count, status (+16 on exception-list overflow) and derivative of
+// every evaluation of the batch into one buffer, for one copy to the host.
+__global__ void batch_gather(const BatchOut* __restrict__ items, double* __restrict__ out, int stride) {
+  const BatchOut& it = items[blockIdx.x];
+  double* o = out + (int64_t)blockIdx.x * stride;
+  if (threadIdx.x == 0) {
+    o[0] = it.result->value;
+    o[1] = (double)it.result->n_valid;
+    o[2] = (double)(it.result->status + (*it.overflow ? 16 : 0));
+  }
+  if (it.deriv)
+    for (int i = threadIdx.x; i < it.P; i += blockDim.x) o[3 + i] = it.deriv[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -435,6 +544,28 @@ hipError_t launch_fast_derivative(const double* partials, int rows, const FastMa
     hipLaunchKernelGGL(fast_derivative_kernel<2>, dim3(1), dim3(256), 0, st, partials, rows, map, r, deriv);
   else
     hipLaunchKernelGGL(fast_derivative_kernel<3>, dim3(1), dim3(256), 0, st, partials, rows, map, r, deriv);
+  return hipGetLastError();
+}
+
+hipError_t launch_fast_batch(const BatchLaunch& b, hipStream_t st) {
+  const int B = b.bins_max;
+  hipLaunchKernelGGL(batch_clear, dim3(b.n), dim3(256), 0, st, b.fa);
+  const size_t lds1 = (size_t)B * B * 8 + (size_t)(B + 2) * 4;
+  MMI_DISPATCH_FAST(pass1_fast_batch, b.storage, b.dim, b.total_blocks, lds1, st, b.fa, b.start, b.n);
+  MMI_DISPATCH_FAST(exceptions_pass1_batch, b.storage, b.dim, b.total_blocks, (size_t)(B + 1) * 4, st, b.ea, b.fa,
+                    b.start, b.n);
+  hipError_t e = launch_finalize_batch(b.fz, b.n, B, st);
+  if (e != hipSuccess) return e;
+  if (b.want_derivative) {
+    const size_t lds2 = ((size_t)B * B + 128) * 4;
+    MMI_DISPATCH_FAST(pass2_fast_batch, b.storage, b.dim, b.total_blocks, lds2, st, b.fa, b.start, b.n);
+    MMI_DISPATCH_FAST(exceptions_pass2_batch, b.storage, b.dim, b.total_blocks, 0, st, b.ea, b.fa, b.start, b.n);
+    if (b.dim == 2)
+      hipLaunchKernelGGL(fast_derivative_batch<2>, dim3(b.n), dim3(256), 0, st, b.dv);
+    else
+      hipLaunchKernelGGL(fast_derivative_batch<3>, dim3(b.n), dim3(256), 0, st, b.dv);
+  }
+  hipLaunchKernelGGL(batch_gather, dim3(b.n), dim3(64), 0, st, b.items, b.out, b.out_stride);
   return hipGetLastError();
 }
 

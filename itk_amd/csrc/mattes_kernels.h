@@ -140,6 +140,45 @@ struct FinalizeArgs {
   double moving_bin_size;
 };
 
+// Batched evaluation (several contexts per launch).
+struct BatchDeriv {
+  const double* partials;
+  int32_t rows;
+  int32_t pad_;
+  FastMap map;
+  const DevResult* result;
+  double* deriv;
+};
+
+struct BatchOut {
+  const DevResult* result;
+  const double* deriv;  // NULL for value-only
+  const uint32_t* overflow;
+  int32_t P;
+  int32_t pad_;
+};
+
+struct BatchLaunch {
+  int32_t n;  // evaluations
+  int32_t dim, storage, want_derivative;
+  int32_t total_blocks;  // start[n]
+  int32_t out_stride;
+  int32_t bins_max;
+  int32_t pad_;
+  const int32_t* start;  // [n + 1] first workgroup of each evaluation
+  const FastArgs* fa;    // [n]
+  const EvalArgs* ea;    // [n]
+  const FinalizeArgs* fz;
+  const BatchDeriv* dv;
+  const BatchOut* items;
+  double* out;  // [n][out_stride]: value, n_valid, status, derivative
+};
+
+// The finalize workgroup stages the PDF and marginals in LDS when they fit.
+inline bool finalize_uses_lds(int bins) {
+  return ((size_t)bins * bins + 3 * (size_t)bins) * sizeof(double) <= 48 * 1024;
+}
+
 struct RecCoeffs {
   double N0, N1, N2, N3, D1, D2, D3, D4, M1, M2, M3, M4, BN1, BN2, BN3, BN4, BM1, BM2, BM3, BM4;
 };
@@ -150,6 +189,11 @@ struct Direction {
 
 hipError_t launch_pass1(const EvalArgs& a, int grid, hipStream_t st);
 hipError_t launch_finalize(const FinalizeArgs& f, hipStream_t st);
+// n finalize workgroups, item lds flags already set with finalize_uses_lds.
+hipError_t launch_finalize_batch(const FinalizeArgs* fz, int n, int bins_max, hipStream_t st);
+// The whole fast-path sequence for a batch (clear, pass 1, exceptions,
+// finalize, pass 2, exceptions, derivative, gather), in launch order.
+hipError_t launch_fast_batch(const BatchLaunch& b, hipStream_t st);
 hipError_t launch_pass2(const EvalArgs& a, int grid, hipStream_t st);
 hipError_t launch_fast_pass1(const FastArgs& f, int dim, int grid, hipStream_t st);
 hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st);

@@ -110,7 +110,7 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
   return x;
 }
 
-__global__ void __launch_bounds__(kFinalizeThreads) finalize_kernel(FinalizeArgs f) {
+__device__ __forceinline__ void finalize_body(const FinalizeArgs& f) {
   extern __shared__ double lds_d[];
   __shared__ unsigned long long redu[kFinalizeThreads / 64];
   __shared__ double reds[kFinalizeThreads / 64];
@@ -186,6 +186,12 @@ __global__ void __launch_bounds__(kFinalizeThreads) finalize_kernel(FinalizeArgs
     f.result->value = -1.0 * sum;
     f.result->n_factor = -1.0 / (f.moving_bin_size * (double)*f.nvalid);
   }
+}
+
+__global__ void __launch_bounds__(kFinalizeThreads) finalize_kernel(FinalizeArgs f) { finalize_body(f); }
+
+__global__ void __launch_bounds__(kFinalizeThreads) finalize_batch(const FinalizeArgs* __restrict__ fz) {
+  finalize_body(fz[blockIdx.x]);
 }
 
 // ---------------------------------------------------------------------------
@@ -345,9 +351,16 @@ hipError_t launch_pass2(const EvalArgs& a, int grid, hipStream_t st) {
 
 hipError_t launch_finalize(const FinalizeArgs& f, hipStream_t st) {
   FinalizeArgs g = f;
-  const size_t lds = ((size_t)f.bins * f.bins + 3 * (size_t)f.bins) * sizeof(double);
-  g.lds = lds <= 48 * 1024 ? 1 : 0;
-  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kFinalizeThreads), g.lds ? lds : 0, st, g);
+  g.lds = finalize_uses_lds(f.bins) ? 1 : 0;
+  const size_t lds = g.lds ? ((size_t)f.bins * f.bins + 3 * (size_t)f.bins) * sizeof(double) : 0;
+  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kFinalizeThreads), lds, st, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_batch(const FinalizeArgs* fz, int n, int bins_max, hipStream_t st) {
+  const size_t lds =
+      finalize_uses_lds(bins_max) ? ((size_t)bins_max * bins_max + 3 * (size_t)bins_max) * sizeof(double) : 0;
+  hipLaunchKernelGGL(finalize_batch, dim3(n), dim3(kFinalizeThreads), lds, st, fz);
   return hipGetLastError();
 }
 

@@ -218,11 +218,15 @@ class MattesMutualInformationImageToImageMetricv4:
     def UpdateTransformParameters(self, derivative, factor: float = 1.0):
         self._moving_tf.update_transform_parameters(derivative, factor)
 
+    def _push_parameters(self) -> np.ndarray:
+        p = np.ascontiguousarray(self._moving_tf.get_parameters(), dtype=np.float64)
+        N.check(self._L.mmi_set_moving_parameters(self._ctx, _dptr(p), p.size))
+        return p
+
     def _evaluate(self, want_derivative: bool, want_pdf: bool = False):
         if not self._initialized:
             raise N.MetricError(N.MMI_ERR_ARGUMENT, "Initialize() must be called first")
-        p = np.ascontiguousarray(self._moving_tf.get_parameters(), dtype=np.float64)
-        N.check(self._L.mmi_set_moving_parameters(self._ctx, _dptr(p), p.size))
+        p = self._push_parameters()
         value = ctypes.c_double()
         nvalid = ctypes.c_int64()
         deriv = np.zeros(p.size) if want_derivative else None

@@ -1,0 +1,104 @@
+"""Batched evaluation (mmi_batch): every context's results equal its own
+evaluation bit for bit — fused fast-path contexts keep their own workgroup
+partition, the others run the single-context code — across graph replays with
+eager work in between, mixed with contexts off the fast path and with a
+degenerate member."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from _cases import affine_for, affine_params, blob_problem, bspline_for
+from itk_amd._native import MMI_ERR_DEGENERATE, MetricError
+from itk_amd.batch import BatchEvaluator
+from itk_amd.metric import MattesMutualInformationImageToImageMetricv4 as Metric
+
+pytestmark = pytest.mark.gpu
+
+
+def _metric(fixed, moving, geom, tf, **kw):
+    m = Metric()
+    m.SetFixedImage(fixed, geom)
+    m.SetMovingImage(moving, geom)
+    if kw.get("virtual") is not None:
+        m.SetVirtualDomain(kw["virtual"])
+    if kw.get("cd"):
+        m.SetUseMovingImageGradientFilter(False)
+    m.SetMovingTransform(tf)
+    m.Initialize()
+    return m
+
+
+def _problems():
+    out = []
+    for k, shape in enumerate([(48, 44, 40), (64, 60, 56), (40, 36, 34), (56, 50, 44)]):
+        fixed, moving, geom = blob_problem(shape, 70 + k)
+        tf = affine_for(geom, affine_params(3, np.random.default_rng(80 + k), scale=2.0))
+        out.append((fixed, moving, geom, tf))
+    return out
+
+
+def _alone(m):
+    v, d = m.GetValueAndDerivative()
+    return v, d, m.GetNumberOfValidPoints(), m.GetLastEvaluationPath()
+
+
+def test_fused_batch_equals_single_evaluations():
+    probs = _problems()
+    ms = [_metric(*p) for p in probs]
+    ms.append(_metric(*probs[1][:3], probs[1][3], virtual=probs[1][2].shrink(2)))  # coarse level, fast path
+    be = BatchEvaluator()
+    rng = np.random.default_rng(90)
+    for rep in range(3):  # replays of one captured graph, with eager work between them
+        for (_, _, geom, tf), m in zip(probs + [probs[1]], ms):
+            tf.set_parameters(affine_params(3, rng, scale=2.0))
+        got = be.evaluate(ms)
+        t = be.last_timing()
+        assert t["fused"] == len(ms)
+        for m, (v, d, nv, err) in zip(ms, got):
+            assert err is None
+            v1, d1, nv1, path = _alone(m)
+            assert path == "fast"
+            assert nv == nv1
+            assert v == v1
+            assert np.array_equal(d, d1)
+    be.close()
+
+
+def test_mixed_batch_and_degenerate_member():
+    probs = _problems()
+    fixed, moving, geom, tf = probs[0]
+    members = [
+        _metric(*probs[0]),                                    # fused
+        _metric(*probs[2][:3], probs[2][3], cd=True),          # central differences: single
+        _metric(fixed, moving, geom, bspline_for(geom, (3, 3, 3), seed=91, sigma=0.6)),  # B-spline: single
+        _metric(*probs[3]),                                    # fused
+    ]
+    far = affine_for(geom, affine_params(3, np.random.default_rng(92)))
+    p = far.get_parameters()
+    p[-3:] += 1e4
+    far.set_parameters(p)
+    members.append(_metric(fixed, moving, geom, far))          # every sample outside: degenerate
+    be = BatchEvaluator()
+    got = be.evaluate(members)
+    assert be.last_timing()["fused"] == 3
+    for m, (v, d, nv, err) in zip(members[:4], got[:4]):
+        assert err is None
+        v1, d1, nv1, _ = _alone(m)
+        assert (v, nv) == (v1, nv1)
+        assert np.array_equal(d, d1)
+    assert got[4][3] is not None and got[4][3].code == MMI_ERR_DEGENERATE
+    with pytest.raises(MetricError):
+        members[4].GetValueAndDerivative()
+    be.close()
+
+
+def test_batch_value_only_and_without_graphs():
+    probs = _problems()
+    ms = [_metric(*p) for p in probs[:3]]
+    be = BatchEvaluator(graphs=False)
+    got = be.evaluate(ms, want_derivative=False)
+    for m, (v, d, nv, err) in zip(ms, got):
+        assert err is None and d is None
+        assert v == m.GetValue()
+    be.close()
