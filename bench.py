@@ -11,6 +11,12 @@ parameters (as an optimizer iteration does) and returns value and the
 --workload bspline measures the 192^3 cubic B-spline case instead (mesh 16^3,
 19^3 x 3 = 20,577 parameters, moving image warped by an 8^3-mesh B-spline).
 
+--serving measures whole registrations instead: a Poisson stream of requests
+(sizes uniform over 128^3, 160x160x96, 192^3, 256x256x128, 256^3; 70% affine
+over a 3-level pyramid, 30% cubic B-spline with an 8^3 or 16^3 mesh), served
+by the continuous-batching server, against the same requests run one at a time
+on the plain metric.
+
 Prints one JSON line.  --gpus above 1 is rejected: this build measures one GPU.
 """
 from __future__ import annotations
@@ -53,7 +59,112 @@ def parse():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serving", action="store_true", help="registration serving workload")
+    ap.add_argument("--requests", type=int, default=32, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=40.0, help="serving: mean arrivals per second")
+    ap.add_argument("--inflight", type=int, default=32, help="serving: registrations in flight at most")
     return ap.parse_args()
+
+
+SERVING_SIZES = [(128, 128, 128), (160, 160, 96), (192, 192, 192), (256, 256, 128), (256, 256, 256)]
+
+
+def _serving_spec(rid: int, seed: int):
+    """Kind, size, mesh and image seed of request rid (splitmix64 stream of the seed)."""
+    from itk_amd import synth
+
+    u = synth.uniform(seed + rid, 23, 4)
+    size = SERVING_SIZES[min(4, int(u[0] * 5))]
+    kind = "affine" if u[1] < 0.7 else "bspline"
+    mesh = 8 if u[2] < 0.5 else 16
+    return kind, size, mesh, seed * 1000 + rid
+
+
+def _synthesize(args):
+    rid, seed = args
+    from itk_amd import synth
+
+    kind, size, mesh, img_seed = _serving_spec(rid, seed)
+    if kind == "affine":
+        p = synth.make_affine_problem(size, img_seed)
+    else:
+        p = synth.make_bspline_problem(size, img_seed, mesh=8, sigma_vox=2.0)
+    return rid, kind, size, mesh, p.fixed, p.moving
+
+
+def serving(args):
+    """Registrations/s and latency of the continuous-batching server on a
+    Poisson stream, against the same requests one at a time."""
+    import concurrent.futures as cf
+
+    from itk_amd.geometry import ImageGeometry
+    from itk_amd.serving import RegistrationRequest, RegistrationServer, latency_summary, run_sequential
+
+    t0 = time.perf_counter()
+    workers = min(16, os.cpu_count() or 1)
+    with cf.ProcessPoolExecutor(max_workers=workers) as ex:
+        made = list(ex.map(_synthesize, [(i, args.seed) for i in range(args.requests)]))
+    print("# synthesized %d requests in %.1f s" % (len(made), time.perf_counter() - t0), file=sys.stderr, flush=True)
+    from itk_amd import synth
+
+    gaps = -np.log(1.0 - synth.uniform(args.seed, 29, args.requests)) / args.rate
+    arrivals = np.cumsum(gaps) - gaps[0]
+
+    def requests(poisson: bool):
+        out = []
+        for (rid, kind, size, mesh, fixed, moving), t in zip(made, arrivals):
+            out.append(RegistrationRequest(rid, kind, fixed, moving, ImageGeometry(size=size),
+                                           arrival=float(t) if poisson else 0.0, mesh=mesh,
+                                           iterations=100 if kind == "affine" else 50))
+        return out
+
+    seq, busy = run_sequential(requests(False))
+    print("# sequential: %d registrations in %.2f s" % (len(seq), busy), file=sys.stderr, flush=True)
+    server = RegistrationServer(max_inflight=args.inflight)
+    sat = server.run(requests(False))
+    sat_sum = latency_summary(sat)
+    sat_stats = {"batches": server.batches, "evaluations": server.batched_evaluations,
+                 "largest_batch": server.max_batch, "peak_in_flight": server.peak_inflight}
+    print("# saturated batched: %s %s" % (sat_sum, sat_stats), file=sys.stderr, flush=True)
+    server = RegistrationServer(max_inflight=args.inflight)
+    poi = server.run(requests(True))
+    poi_sum = latency_summary(poi)
+    print("# poisson batched: %s" % poi_sum, file=sys.stderr, flush=True)
+    seq_poisson, _ = run_sequential(requests(True))
+    seq_poi_sum = latency_summary(seq_poisson)
+    same = all(a.error is None and b.error is None and np.array_equal(a.parameters, b.parameters)
+               for a, b in zip(seq, sat))
+    seq_rate = len(seq) / busy
+    kinds = {"affine": sum(1 for m in made if m[1] == "affine"), "bspline": sum(1 for m in made if m[1] == "bspline")}
+    line = {
+        "metric": "MI value+grad evals/s & HBM GB/s; batched registrations/s + p99 latency",
+        "value": sat_sum["registrations_per_s"],
+        "unit": "registrations/s",
+        "n_gpus": 1,
+        "steps": args.requests,
+        "warmup": 0,
+        "ms_per_step": 1e3 / sat_sum["registrations_per_s"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 images, f64 coordinates/PDF",
+        "data": "synthetic (seeded blob volumes per request id, see itk_amd/synth.py)",
+        "config": {"workload": "registration_serving_mixed_sizes", "requests": args.requests, "kinds": kinds,
+                   "arrival_rate_per_s": args.rate, "max_in_flight": args.inflight,
+                   "affine": "3 levels (shrink 4,2,1; sigma 2,1,0 voxels), RegularStep GD, <=100 its/level",
+                   "bspline": "mesh 8^3 or 16^3, L-BFGS-B, <=50 iterations", "parallelism": "single GPU"},
+        "serving": {
+            "sequential_registrations_per_s": seq_rate,
+            "batched_registrations_per_s": sat_sum["registrations_per_s"],
+            "speedup_vs_sequential": sat_sum["registrations_per_s"] / seq_rate,
+            "batched_saturated": dict(sat_sum, **sat_stats),
+            "batched_poisson": poi_sum,
+            "sequential_poisson": seq_poi_sum,
+            "results_identical_alone_and_batched": bool(same),
+        },
+    }
+    print(json.dumps(line), flush=True)
+    return 0
 
 
 class Workload:
@@ -128,6 +239,8 @@ def main():
     if args.gpus != 1:
         print("bench.py measures one GPU only (this build has no multi-GPU path); run with --gpus 1", file=sys.stderr)
         return 2
+    if args.serving:
+        return serving(args)
     from itk_amd.metric import MattesMutualInformationImageToImageMetricv4
 
     wl = Workload(args)

@@ -280,7 +280,7 @@ struct mmi_context {
   bool ftab_valid = false;
   mmi_geometry ftab_key_v{}, ftab_key_f{};
   // separable B-spline derivative: per-sample factors, contraction stages, axis tables
-  DevBuf bs_g, bs_t1, bs_t2;
+  DevBuf bs_g, bs_t1, bs_t2, bs_rows;
   DevBuf bs_start[3], bs_w[3], bs_inside[3], bs_lo[3], bs_hi[3];
   bool bs_tables = false;
   DevGeom bs_key_virt{}, bs_key_grid{};
@@ -470,6 +470,61 @@ bool fixed_axis_tables(mmi_context* ctx, const mmi_geometry& v, const mmi_geomet
   return true;
 }
 
+int bspline_separable(mmi_context* ctx, const DevGeom& virt, int D, AxisTable* tabs, bool* usable);
+
+// B-spline part of the fast path: the per-axis support tables (shared with the
+// separable derivative), the moving image's map, and the bound on the moving
+// continuous index.  The displacement is evaluated as sum_k wx[k] * row[k]
+// over coefficient lines reduced over the y and z supports instead of the
+// reference's 64-term sum in node order; both are FP64 sums of non-negative
+// weights (summing to 1) times coefficients, so they differ by at most
+// ~100 ulp of max|coefficient|.  That, the rounding of vp + U and of the
+// index map, is covered with a wide margin by
+// e_ci = 1024 eps (rowsum|P2I| (D max|C| + max|mp| + max|o_m|) + max|ci|).
+bool setup_fast_bspline(mmi_context* ctx, FastArgs* F) {
+  const int D = ctx->fixed_g.dim;
+  const mmi_geometry& v = ctx->virtual_g;
+  const mmi_geometry& m = ctx->moving_g;
+  if (ctx->dmtf.grid.dim != D || !is_diagonal(v.index_to_physical, D)) return false;
+  AxisTable tabs[3];
+  std::memset(tabs, 0, sizeof(tabs));
+  bool usable = false;
+  if (bspline_separable(ctx, to_dev(v), D, tabs, &usable) != MMI_OK || !usable) return false;
+  for (int a = 0; a < 3; ++a) {
+    F->bs_start[a] = a < D ? tabs[a].start : nullptr;
+    F->bs_w[a] = a < D ? tabs[a].w : nullptr;
+    F->bs_inside[a] = a < D ? tabs[a].inside : nullptr;
+  }
+  F->bs_gx = ctx->dmtf.grid.size[0];
+  for (int k = 0; k < D; ++k) {
+    F->vorg[k] = v.origin[k];
+    F->vsp[k] = v.index_to_physical[k * D + k];
+    F->morg[k] = m.origin[k];
+  }
+  for (int i = 0; i < D * D; ++i) F->mp2i[i] = m.physical_to_index[i];
+  const int64_t Ny = v.size[1], Nz = D == 3 ? v.size[2] : 1;
+  if (ctx->bs_rows.ensure(sizeof(double) * Nz * Ny * D * F->bs_gx) != hipSuccess) return false;
+  F->bs_rows = ctx->bs_rows.as<double>();
+  double maxC = 0.0;
+  for (double c : ctx->mtf.params) maxC = std::max(maxC, std::fabs(c));
+  double maxVP = 0.0, maxO = 0.0, rowP2I = 0.0;
+  for (int corner = 0; corner < (1 << D); ++corner)
+    for (int k = 0; k < D; ++k) {
+      const int64_t i = v.start[k] + (((corner >> k) & 1) ? v.size[k] - 1 : 0);
+      maxVP = std::max(maxVP, std::fabs(v.origin[k]) + std::fabs(F->vsp[k] * (double)i));
+    }
+  for (int k = 0; k < D; ++k) {
+    maxO = std::max(maxO, std::fabs(m.origin[k]));
+    double r = 0.0;
+    for (int j = 0; j < D; ++j) r += std::fabs(m.physical_to_index[k * D + j]);
+    rowP2I = std::max(rowP2I, r);
+  }
+  const double maxMP = maxVP + D * maxC;
+  const double maxCI = rowP2I * (maxMP + maxO);
+  F->e_ci = 1024.0 * std::ldexp(1.0, -52) * (rowP2I * (D * maxC + maxMP + maxO) + maxCI) + 1e-300;
+  return true;
+}
+
 // Parameters and error bounds of the certified fast path (FastArgs in
 // mattes_kernels.h; buffers are filled in by the caller).  Returns false when
 // the path does not apply.
@@ -479,12 +534,16 @@ bool setup_fast(mmi_context* ctx, FastArgs* F) {
   const int B = ctx->bins;
   if (num_pixels(ctx->virtual_g) >= (1ll << 31)) return false;
   if (ctx->flags & 2u) return false;  // exact path requested
-  if (ctx->mtf.kind != MMI_TF_AFFINE || ctx->ftf.kind != MMI_TF_IDENTITY) return false;
+  const bool bs = ctx->mtf.kind == MMI_TF_BSPLINE;
+  if ((ctx->mtf.kind != MMI_TF_AFFINE && !bs) || ctx->ftf.kind != MMI_TF_IDENTITY) return false;
   if (ctx->gradient_source != MMI_GRADIENT_IMAGE) return false;
   if ((size_t)B * B * 8 + (B + 1) * 4 > 64 * 1024) return false;
   const mmi_geometry& v = ctx->virtual_g;
   const mmi_geometry& m = ctx->moving_g;
   const DevTransform& t = ctx->dmtf;
+  F->mk = bs ? 2 : 1;
+  if (bs && !setup_fast_bspline(ctx, F)) return false;
+  if (!bs) {
   // ci = P2I_m (M (o_v + I2P_v idx) + off - o_m) = A idx + b, in long double
   long double MI[9] = {0}, A[9] = {0}, bb[3] = {0}, w[3] = {0};
   for (int i = 0; i < D; ++i)
@@ -538,6 +597,7 @@ bool setup_fast(mmi_context* ctx, FastArgs* F) {
     S = std::max(S, mag * (1.0 + rowA));
   }
   F->e_ci = 64.0 * S * std::ldexp(1.0, -52) + 1e-300;
+  }
   if (F->e_ci > 1e-6) return false;
   // intensity bounds
   const double Mm = std::max(std::fabs(ctx->moving_min), std::fabs(ctx->moving_max));
@@ -792,7 +852,7 @@ int mmi_context_destroy(mmi_context* ctx) {
                     &ctx->coeffs, &ctx->hist, &ctx->fmarg,
                     &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->result,
                     &ctx->partials, &ctx->deriv, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
-                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_overflow, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->ftab};
+                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_overflow, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->ftab};
   for (DevBuf* b : bufs) b->release();
   for (int j = 0; j < 3; ++j) {
     ctx->bs_start[j].release();
@@ -1094,6 +1154,29 @@ int mmi_set_moving_parameters(mmi_context* ctx, const double* params, int64_t n_
 
 namespace {
 
+// The separable B-spline derivative from the per-sample factors in bs_g:
+// contraction along x (LDS-staged rows), then y, then z with the nFactor scale.
+int bspline_contract(mmi_context* ctx, const DevGeom& virt, const AxisTable* tabs, const DevResult* result) {
+  const int D = virt.dim;
+  hipStream_t st = ctx->stream;
+  const int64_t Nx = virt.size[0], Ny = virt.size[1], Nz = D == 3 ? virt.size[2] : 1;
+  const int Gx = ctx->dmtf.grid.size[0], Gy = ctx->dmtf.grid.size[1], Gz = D == 3 ? ctx->dmtf.grid.size[2] : 1;
+  MMI_HIP(ctx->bs_t1.ensure(sizeof(double) * D * Nz * Ny * Gx));
+  MMI_HIP(launch_separable_contract(ctx->bs_g.as<float>(), nullptr, ctx->bs_t1.as<double>(), D * Nz * Ny, (int)Nx, 1,
+                                    Gx, tabs[0], nullptr, st));
+  if (D == 3) {
+    MMI_HIP(ctx->bs_t2.ensure(sizeof(double) * D * Nz * Gy * Gx));
+    MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t1.as<double>(), ctx->bs_t2.as<double>(), D * Nz, (int)Ny, Gx,
+                                      Gy, tabs[1], nullptr, st));
+    MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t2.as<double>(), ctx->deriv.as<double>(), D, (int)Nz,
+                                      (int64_t)Gy * Gx, Gz, tabs[2], result, st));
+  } else {
+    MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t1.as<double>(), ctx->deriv.as<double>(), D, (int)Ny, Gx, Gy,
+                                      tabs[1], result, st));
+  }
+  return MMI_OK;
+}
+
 // Validation, buffers and kernel arguments of one evaluation; `fast` tells
 // whether the certified fast path applies (then `fa` is complete, including
 // the derivative partials when want_derivative is set).
@@ -1223,6 +1306,17 @@ retry:
   MMI_HIP(hipMemsetAsync(ctx->nvalid.p, 0, sizeof(unsigned long long), st));
   if (fast) MMI_HIP(hipMemsetAsync(ctx->exc_overflow.p, 0, sizeof(uint32_t), st));
   MMI_HIP(hipEventRecord(ctx->ev[0], st));
+  if (fast && fa.mk == 2) {
+    AxisTable rt[3];
+    for (int k = 0; k < 3; ++k) {
+      rt[k].start = fa.bs_start[k];
+      rt[k].w = fa.bs_w[k];
+      rt[k].inside = fa.bs_inside[k];
+      rt[k].lo = rt[k].hi = nullptr;
+    }
+    MMI_HIP(launch_bspline_rows(ctx->dmtf.coeffs, ctx->dmtf.pdim, ctx->dmtf.grid.size[0], ctx->dmtf.grid.size[1], D,
+                                (int)a.virt.size[1], D == 3 ? (int)a.virt.size[2] : 1, rt, ctx->bs_rows.as<double>(), st));
+  }
   if (fast) {
     MMI_HIP(launch_fast_pass1(fa, D, ctx->grid, st));
     MMI_HIP(launch_fast_exceptions1(a, fa, ctx->grid, st));
@@ -1237,12 +1331,20 @@ retry:
   AxisTable tabs[3];
   std::memset(tabs, 0, sizeof(tabs));
   bool separable = false;
-  if (want_derivative && !fast && ctx->mtf.kind == MMI_TF_BSPLINE)
+  if (want_derivative && ctx->mtf.kind == MMI_TF_BSPLINE)
     if (int rc = bspline_separable(ctx, a.virt, D, tabs, &separable)) return rc;
   if (want_derivative) {
     MMI_HIP(ctx->deriv.ensure(sizeof(double) * P));
     MMI_HIP(hipEventRecord(ctx->ev[2], st));
-    if (fast) {
+    if (fast && fa.mk == 2) {
+      // B-spline fast path: per-sample factors, then the separable contraction
+      MMI_HIP(ctx->bs_g.ensure(sizeof(float) * D * n));
+      fa.bspline_g = ctx->bs_g.as<float>();
+      MMI_HIP(launch_fast_pass2(fa, D, ctx->grid, st));
+      MMI_HIP(launch_fast_exceptions2(a, fa, ctx->grid, st));
+      if (int rc = bspline_contract(ctx, a.virt, tabs, f.result)) return rc;
+      MMI_HIP(hipEventRecord(ctx->ev[3], st));
+    } else if (fast) {
       MMI_HIP(launch_fast_pass2(fa, D, ctx->grid, st));
       MMI_HIP(launch_fast_exceptions2(a, fa, ctx->grid, st));
       MMI_HIP(launch_fast_derivative(fa.partials, 2 * ctx->grid, fast_map(ctx), D, f.result, ctx->deriv.as<double>(),
@@ -1262,21 +1364,7 @@ retry:
       MMI_HIP(ctx->bs_g.ensure(sizeof(float) * D * n));
       a.bspline_g = ctx->bs_g.as<float>();
       MMI_HIP(launch_pass2(a, ctx->grid, st));
-      const int64_t Nx = a.virt.size[0], Ny = a.virt.size[1], Nz = D == 3 ? a.virt.size[2] : 1;
-      const int Gx = ctx->dmtf.grid.size[0], Gy = ctx->dmtf.grid.size[1], Gz = D == 3 ? ctx->dmtf.grid.size[2] : 1;
-      MMI_HIP(ctx->bs_t1.ensure(sizeof(double) * D * Nz * Ny * Gx));
-      MMI_HIP(launch_separable_contract(a.bspline_g, nullptr, ctx->bs_t1.as<double>(), D * Nz * Ny, (int)Nx, 1, Gx,
-                                        tabs[0], nullptr, st));
-      if (D == 3) {
-        MMI_HIP(ctx->bs_t2.ensure(sizeof(double) * D * Nz * Gy * Gx));
-        MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t1.as<double>(), ctx->bs_t2.as<double>(), D * Nz, (int)Ny,
-                                          Gx, Gy, tabs[1], nullptr, st));
-        MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t2.as<double>(), ctx->deriv.as<double>(), D, (int)Nz,
-                                          (int64_t)Gy * Gx, Gz, tabs[2], f.result, st));
-      } else {
-        MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t1.as<double>(), ctx->deriv.as<double>(), D, (int)Ny, Gx,
-                                          Gy, tabs[1], f.result, st));
-      }
+      if (int rc = bspline_contract(ctx, a.virt, tabs, f.result)) return rc;
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     } else {
       MMI_HIP(hipMemsetAsync(ctx->deriv.p, 0, sizeof(double) * P, st));
@@ -1446,7 +1534,7 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
     bool fast = false;
     if (int rc = prepare_eval(ctxs[i], want_derivative, a, f, fast)) return rc;
     const int64_t P = (int64_t)ctxs[i]->mtf.params.size();
-    if (fast && P <= 12 && (storage < 0 || (a.virt.dim == dim && a.storage == storage))) {
+    if (fast && f.mk == 1 && P <= 12 && (storage < 0 || (a.virt.dim == dim && a.storage == storage))) {
       dim = a.virt.dim;
       storage = a.storage;
       fused.push_back(i);

@@ -76,18 +76,62 @@ __device__ __forceinline__ int fast_bin(float term, float bound, int B, bool* un
 // 0: not counted, 1: counted (bins, arg0, cell set), 2: uncertain.  Straight
 // line code: the cell is clamped into the buffer so every load is in bounds,
 // and the decisions are combined at the end.
-template <int D, typename P>
-__device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* idx, int& fbin, int& mbin, float& arg0,
-                                             Cell& cell) {
-  double ci[3];
-  const double x = (double)idx[0], y = (double)idx[1], z = (double)idx[2];
+// Moving continuous index of a virtual sample.  MK 1: one FP64 affine map of
+// the index.  MK 2: the virtual point, the B-spline displacement from the
+// per-row coefficient lines (sum over the four x nodes) and the moving
+// physical-to-index map; *bsin tells whether the sample is inside the
+// transform's valid region (else the displacement is zero).
+template <int D, int MK>
+__device__ __forceinline__ void fast_cindex(const FastArgs& f, const int32_t* idx, double* ci, bool* bsin) {
+  if constexpr (MK == 2) {
+    double vp[3], mp[3];
 #pragma unroll
-  for (int k = 0; k < D; ++k) {
-    double c = f.b[k];
-    if (D == 3) c = fma(f.A[k * 3 + 2], z, c);
-    c = fma(f.A[k * D + 1], y, c);
-    ci[k] = fma(f.A[k * D + 0], x, c);
+    for (int k = 0; k < D; ++k) {
+      double s = f.vorg[k];
+#pragma unroll
+      for (int j = 0; j < D; ++j) s += (j == k ? f.vsp[k] : 0.0) * (double)idx[j];
+      vp[k] = s;
+    }
+    const int xl = idx[0] - f.vstart[0], yl = idx[1] - f.vstart[1], zl = D == 3 ? idx[2] - f.vstart[2] : 0;
+    const bool in = f.bs_inside[0][xl] && f.bs_inside[1][yl] && (D < 3 || f.bs_inside[2][zl]);
+    *bsin = in;
+    if (in) {
+      const double* w = f.bs_w[0] + 4 * xl;
+      const double* row = f.bs_rows + ((int64_t)(zl * f.vsize1 + yl) * D) * f.bs_gx + f.bs_start[0][xl];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const double* r = row + d * f.bs_gx;
+        mp[d] = (w[0] * r[0] + w[1] * r[1] + w[2] * r[2] + w[3] * r[3]) + vp[d];
+      }
+    } else {
+#pragma unroll
+      for (int d = 0; d < D; ++d) mp[d] = vp[d];
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < D; ++j) s += f.mp2i[k * D + j] * (mp[j] - f.morg[j]);
+      ci[k] = s;
+    }
+  } else {
+    *bsin = true;
+    const double x = (double)idx[0], y = (double)idx[1], z = (double)idx[2];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      double c = f.b[k];
+      if (D == 3) c = fma(f.A[k * 3 + 2], z, c);
+      c = fma(f.A[k * D + 1], y, c);
+      ci[k] = fma(f.A[k * D + 0], x, c);
+    }
   }
+}
+
+template <int D, typename P, int MK>
+__device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* idx, int& fbin, int& mbin, float& arg0,
+                                             Cell& cell, bool* bsin) {
+  double ci[3];
+  fast_cindex<D, MK>(f, idx, ci, bsin);
   bool out = false, uncertain = false;
   int32_t bb[3] = {0, 0, 0};
   bool act[3] = {false, false, false};
@@ -166,7 +210,7 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
   return out ? 0 : (uncertain ? 2 : 1);
 }
 
-template <int D, typename P>
+template <int D, typename P, int MK>
 __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, uint32_t lb, uint32_t lg) {
   extern __shared__ unsigned long long lds_u64[];
   const int B = f.bins;
@@ -186,7 +230,8 @@ __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, uint32_t lb, 
     int fbin = 0, mbin = 0;
     float arg0 = 0.0f;
     Cell cell;
-    const int c = fast_classify<D, P>(f, idx, fbin, mbin, arg0, cell);
+    bool bsin;
+    const int c = fast_classify<D, P, MK>(f, idx, fbin, mbin, arg0, cell, &bsin);
     if (c == 2) {
       const unsigned int slot = atomicAdd(&counters[1], 1u);
       if (slot < (unsigned int)f.exc_cap) f.exc_list[lb * f.exc_cap + slot] = s;
@@ -228,7 +273,7 @@ __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, uint32_t lb, 
 }
 
 // Exact FP64 evaluation of the listed samples of one pass-1 workgroup.
-template <int D, typename P>
+template <int D, typename P, int MK>
 __device__ __forceinline__ void exceptions_pass1_body(const EvalArgs& a, const FastArgs& f, uint32_t lb) {
   extern __shared__ unsigned int lds_u32[];
   const uint32_t ne = min(f.exc_count[lb], (uint32_t)f.exc_cap);
@@ -242,7 +287,7 @@ __device__ __forceinline__ void exceptions_pass1_body(const EvalArgs& a, const F
     int32_t idx[3];
     sample_index<D>(a.virt, s, idx);
     SampleOut<D> o;
-    const bool ok = eval_sample<D, 1, P>(a, idx, o);
+    const bool ok = eval_sample<D, MK, P>(a, idx, o);
     if (a.sample_fbin) {
       a.sample_fbin[s] = ok ? (int16_t)o.fbin : (int16_t)-1;
       a.sample_mbin[s] = ok ? (int16_t)o.mbin : (int16_t)-1;
@@ -278,7 +323,7 @@ __device__ __forceinline__ void add_moments(double* acc, float w, float3 g, cons
   }
 }
 
-template <int D, typename P>
+template <int D, typename P, int MK>
 __device__ __forceinline__ void pass2_fast_body(const FastArgs& f, uint32_t lb, uint32_t lg) {
   extern __shared__ float lds_f32[];
   const int B = f.bins;
@@ -299,23 +344,66 @@ __device__ __forceinline__ void pass2_fast_body(const FastArgs& f, uint32_t lb, 
     int fbin = 0, mbin = 0;
     float arg0 = 0.0f;
     Cell cell;
-    if (fast_classify<D, P>(f, idx, fbin, mbin, arg0, cell) != 1) continue;
-    const float w = parzen_deriv_weight_fast(arg0, pr + fbin * B + (mbin - 1));
-    if (w == 0.0f) continue;
-    add_moments<D>(acc, w, interp_gradient3<D, P>(static_cast<const typename P::texel*>(f.texels), cell), idx);
+    bool bsin;
+    const int c = fast_classify<D, P, MK>(f, idx, fbin, mbin, arg0, cell, &bsin);
+    if constexpr (MK == 2) {
+      // per-sample factors of the separable B-spline derivative, for every
+      // sample (uncertain ones are rewritten by the exception kernel)
+      float gw[3] = {0.0f, 0.0f, 0.0f};
+      if (c == 1 && bsin) {
+        const float w = parzen_deriv_weight_fast(arg0, pr + fbin * B + (mbin - 1));
+        if (w != 0.0f) {
+          const float3 g = interp_gradient3<D, P>(static_cast<const typename P::texel*>(f.texels), cell);
+          gw[0] = (float)((double)w * (double)g.x);
+          gw[1] = (float)((double)w * (double)g.y);
+          gw[2] = (float)((double)w * (double)g.z);
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) f.bspline_g[(int64_t)d * f.n + s] = gw[d];
+    } else {
+      if (c != 1) continue;
+      const float w = parzen_deriv_weight_fast(arg0, pr + fbin * B + (mbin - 1));
+      if (w == 0.0f) continue;
+      add_moments<D>(acc, w, interp_gradient3<D, P>(static_cast<const typename P::texel*>(f.texels), cell), idx);
+    }
   }
-  block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)lb * PA);
+  if constexpr (MK == 1) block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)lb * PA);
 }
 
 // Exception samples of one pass-1 workgroup, in sample order (bitonic sort of
 // the list in LDS), summed in a fixed tree: deterministic.
-template <int D, typename P>
+template <int D, typename P, int MK>
 __device__ __forceinline__ void exceptions_pass2_body(const EvalArgs& a, const FastArgs& f, uint32_t lb,
                                                       uint32_t lg) {
   __shared__ uint32_t keys[256];
   __shared__ double scratch[4 * 12];
   constexpr int PA = D * D + D;
   const uint32_t ne = min(f.exc_count[lb], (uint32_t)f.exc_cap);
+  if constexpr (MK == 2) {
+    // B-spline: each listed sample rewrites its own per-sample factors (no
+    // sum here, so no ordering is needed)
+    for (uint32_t t = threadIdx.x; t < ne; t += blockDim.x) {
+      const uint32_t s = f.exc_list[lb * f.exc_cap + t];
+      int32_t idx[3];
+      sample_index<D>(a.virt, s, idx);
+      SampleOut<D> o;
+      float gw[3] = {0.0f, 0.0f, 0.0f};
+      if (eval_sample<D, 2, P>(a, idx, o) && o.sup.inside) {
+        const float w = parzen_deriv_weight((float)((double)(o.mbin - 1) - o.mterm),
+                                            f.pratio_f + o.fbin * a.bins + (o.mbin - 1));
+        if (w != 0.0f) {
+          const float3 g = interp_gradient3<D, P>(static_cast<const typename P::texel*>(a.texels), o.cell);
+          gw[0] = (float)((double)w * (double)g.x);
+          gw[1] = (float)((double)w * (double)g.y);
+          gw[2] = (float)((double)w * (double)g.z);
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) f.bspline_g[(int64_t)d * f.n + s] = gw[d];
+    }
+    return;
+  }
   double acc[PA];
 #pragma unroll
   for (int i = 0; i < PA; ++i) acc[i] = 0.0;
@@ -402,21 +490,21 @@ __device__ __forceinline__ int batch_item(const int32_t* __restrict__ start, int
   return lo;
 }
 
-template <int D, typename P>
+template <int D, typename P, int MK>
 __global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f) {
-  pass1_fast_body<D, P>(f, blockIdx.x, gridDim.x);
+  pass1_fast_body<D, P, MK>(f, blockIdx.x, gridDim.x);
 }
 
 template <int D, typename P>
 __global__ void __launch_bounds__(256) pass1_fast_batch(const FastArgs* __restrict__ fa,
                                                         const int32_t* __restrict__ start, int n) {
   const int r = batch_item(start, n, blockIdx.x);
-  pass1_fast_body<D, P>(fa[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+  pass1_fast_body<D, P, 1>(fa[r], blockIdx.x - start[r], start[r + 1] - start[r]);
 }
 
-template <int D, typename P>
+template <int D, typename P, int MK>
 __global__ void __launch_bounds__(256) exceptions_pass1_kernel(EvalArgs a, FastArgs f) {
-  exceptions_pass1_body<D, P>(a, f, blockIdx.x);
+  exceptions_pass1_body<D, P, MK>(a, f, blockIdx.x);
 }
 
 template <int D, typename P>
@@ -424,24 +512,24 @@ __global__ void __launch_bounds__(256) exceptions_pass1_batch(const EvalArgs* __
                                                               const FastArgs* __restrict__ fa,
                                                               const int32_t* __restrict__ start, int n) {
   const int r = batch_item(start, n, blockIdx.x);
-  exceptions_pass1_body<D, P>(ea[r], fa[r], blockIdx.x - start[r]);
+  exceptions_pass1_body<D, P, 1>(ea[r], fa[r], blockIdx.x - start[r]);
 }
 
-template <int D, typename P>
+template <int D, typename P, int MK>
 __global__ void __launch_bounds__(256) pass2_fast_kernel(FastArgs f) {
-  pass2_fast_body<D, P>(f, blockIdx.x, gridDim.x);
+  pass2_fast_body<D, P, MK>(f, blockIdx.x, gridDim.x);
 }
 
 template <int D, typename P>
 __global__ void __launch_bounds__(256) pass2_fast_batch(const FastArgs* __restrict__ fa,
                                                         const int32_t* __restrict__ start, int n) {
   const int r = batch_item(start, n, blockIdx.x);
-  pass2_fast_body<D, P>(fa[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+  pass2_fast_body<D, P, 1>(fa[r], blockIdx.x - start[r], start[r + 1] - start[r]);
 }
 
-template <int D, typename P>
+template <int D, typename P, int MK>
 __global__ void __launch_bounds__(256) exceptions_pass2_kernel(EvalArgs a, FastArgs f) {
-  exceptions_pass2_body<D, P>(a, f, blockIdx.x, gridDim.x);
+  exceptions_pass2_body<D, P, MK>(a, f, blockIdx.x, gridDim.x);
 }
 
 template <int D, typename P>
@@ -449,7 +537,7 @@ __global__ void __launch_bounds__(256) exceptions_pass2_batch(const EvalArgs* __
                                                               const FastArgs* __restrict__ fa,
                                                               const int32_t* __restrict__ start, int n) {
   const int r = batch_item(start, n, blockIdx.x);
-  exceptions_pass2_body<D, P>(ea[r], fa[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+  exceptions_pass2_body<D, P, 1>(ea[r], fa[r], blockIdx.x - start[r], start[r + 1] - start[r]);
 }
 
 template <int D>
@@ -516,25 +604,65 @@ __global__ void batch_gather(const BatchOut* __restrict__ items, double* __restr
     }                                                                                               \
   } while (0)
 
+// Single-evaluation kernels also dispatch on the moving transform kind.
+#define MMI_DISPATCH_FAST_MK(KERNEL, mk, storage, dim, grid, lds, st, ...)                          \
+  do {                                                                                              \
+    if ((mk) == 2) {                                                                                \
+      if ((storage) == kStorageBF16) {                                                              \
+        if ((dim) == 2)                                                                             \
+          hipLaunchKernelGGL((KERNEL<2, PxBF16, 2>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);  \
+        else                                                                                        \
+          hipLaunchKernelGGL((KERNEL<3, PxBF16, 2>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);  \
+      } else if ((storage) == kStorageF16) {                                                        \
+        if ((dim) == 2)                                                                             \
+          hipLaunchKernelGGL((KERNEL<2, PxF16, 2>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);   \
+        else                                                                                        \
+          hipLaunchKernelGGL((KERNEL<3, PxF16, 2>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);   \
+      } else {                                                                                      \
+        if ((dim) == 2)                                                                             \
+          hipLaunchKernelGGL((KERNEL<2, PxF32, 2>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);   \
+        else                                                                                        \
+          hipLaunchKernelGGL((KERNEL<3, PxF32, 2>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);   \
+      }                                                                                             \
+    } else {                                                                                        \
+      if ((storage) == kStorageBF16) {                                                              \
+        if ((dim) == 2)                                                                             \
+          hipLaunchKernelGGL((KERNEL<2, PxBF16, 1>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);  \
+        else                                                                                        \
+          hipLaunchKernelGGL((KERNEL<3, PxBF16, 1>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);  \
+      } else if ((storage) == kStorageF16) {                                                        \
+        if ((dim) == 2)                                                                             \
+          hipLaunchKernelGGL((KERNEL<2, PxF16, 1>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);   \
+        else                                                                                        \
+          hipLaunchKernelGGL((KERNEL<3, PxF16, 1>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);   \
+      } else {                                                                                      \
+        if ((dim) == 2)                                                                             \
+          hipLaunchKernelGGL((KERNEL<2, PxF32, 1>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);   \
+        else                                                                                        \
+          hipLaunchKernelGGL((KERNEL<3, PxF32, 1>), dim3(grid), dim3(256), lds, st, __VA_ARGS__);   \
+      }                                                                                             \
+    }                                                                                               \
+  } while (0)
+
 hipError_t launch_fast_pass1(const FastArgs& f, int dim, int grid, hipStream_t st) {
   const size_t lds = (size_t)f.bins * f.bins * 8 + (size_t)(f.bins + 2) * 4;
-  MMI_DISPATCH_FAST(pass1_fast_kernel, f.storage, dim, grid, lds, st, f);
+  MMI_DISPATCH_FAST_MK(pass1_fast_kernel, f.mk, f.storage, dim, grid, lds, st, f);
   return hipGetLastError();
 }
 
 hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st) {
-  MMI_DISPATCH_FAST(exceptions_pass1_kernel, f.storage, a.virt.dim, grid, (a.bins + 1) * 4, st, a, f);
+  MMI_DISPATCH_FAST_MK(exceptions_pass1_kernel, f.mk, f.storage, a.virt.dim, grid, (a.bins + 1) * 4, st, a, f);
   return hipGetLastError();
 }
 
 hipError_t launch_fast_pass2(const FastArgs& f, int dim, int grid, hipStream_t st) {
   const size_t lds = ((size_t)f.bins * f.bins + 128) * 4;
-  MMI_DISPATCH_FAST(pass2_fast_kernel, f.storage, dim, grid, lds, st, f);
+  MMI_DISPATCH_FAST_MK(pass2_fast_kernel, f.mk, f.storage, dim, grid, lds, st, f);
   return hipGetLastError();
 }
 
 hipError_t launch_fast_exceptions2(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st) {
-  MMI_DISPATCH_FAST(exceptions_pass2_kernel, f.storage, a.virt.dim, grid, 0, st, a, f);
+  MMI_DISPATCH_FAST_MK(exceptions_pass2_kernel, f.mk, f.storage, a.virt.dim, grid, 0, st, a, f);
   return hipGetLastError();
 }
 

@@ -44,6 +44,7 @@ struct FastArgs {
   // the interpolation weight or 0, inside-buffer flag)
   const int4* ftab[3];
   int32_t storage;
+  int32_t mk;  // moving transform: 1 affine (ci = A idx + b), 2 cubic B-spline
   const void* moving_px;
   const void* texels;
   int32_t mstart[3], mend[3];
@@ -70,6 +71,20 @@ struct FastArgs {
   uint32_t* exc_count;
   uint32_t* exc_list;
   uint32_t* exc_overflow;
+  // Cubic B-spline moving transform on a separable geometry (kernels built
+  // with MK == 2): per-axis support tables of the virtual indices, the
+  // coefficient lines reduced over y and z for every virtual row, and the
+  // moving image's physical-to-index map.  ci = P2I_m (vp + U - o_m) with
+  // U_d = sum_k wx[x][k] * rows[(z, y, d)][sx + k].
+  const int32_t* bs_start[3];
+  const double* bs_w[3];
+  const uint8_t* bs_inside[3];
+  const double* bs_rows;  // [vsize2][vsize1][D][Gx]
+  int32_t bs_gx;
+  int32_t pad_bs_;
+  double vorg[3], vsp[3];     // virtual origin and (diagonal) spacing
+  double mp2i[9], morg[3];    // moving physical_to_index and origin
+  float* bspline_g;           // [D][n] per-sample factors of the separable derivative
 };
 
 // Map from index-space derivative moments to the affine parameters.
@@ -204,6 +219,11 @@ hipError_t launch_fast_derivative(const double* partials, int rows, const FastMa
 hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, const DevResult* r, double* deriv,
                                     hipStream_t st);
 hipError_t launch_derivative_scale(double* deriv, int64_t P, const DevResult* r, hipStream_t st);
+// Coefficient lines reduced over the y and z supports, per virtual row:
+// rows[((z * Ny + y) * D + d) * Gx + a] = sum_c wz[z][c] sum_b wy[y][b] C_d[sz+c][sy+b][a]
+// (0 for rows outside the valid region).
+hipError_t launch_bspline_rows(const double* coeffs, int32_t pdim, int Gx, int Gy, int D, int Ny, int Nz,
+                               const AxisTable* tabs, double* rows, hipStream_t st);
 hipError_t launch_bspline_axis_table(const DevGeom& virt, const DevGeom& grid, int axis, int32_t* start, double* w,
                                      uint8_t* inside, hipStream_t st);
 // out[o][g][i] = sum_l w[l][g - start[l]] * in[o][l][i] over the virtual

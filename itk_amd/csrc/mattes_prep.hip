@@ -217,6 +217,32 @@ __global__ void texels_to_storage_kernel(const float4* __restrict__ in, uint2* _
   out[i] = u;
 }
 
+// Coefficient lines of the B-spline fast path, one thread per (row, d, a).
+__global__ void bspline_rows_kernel(const double* __restrict__ coeffs, int32_t pdim, int Gx, int Gy, int D, int Ny,
+                                    int64_t total, AxisTable ty, AxisTable tz, double* __restrict__ rows) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int a = (int)(e % Gx);
+  int64_t r = e / Gx;
+  const int d = (int)(r % D);
+  r /= D;
+  const int y = (int)(r % Ny);
+  const int z = (int)(r / Ny);
+  double acc = 0.0;
+  if (ty.inside[y] && (D < 3 || tz.inside[z])) {
+    const int sy = ty.start[y];
+    const int sz = D == 3 ? tz.start[z] : 0;
+    const double* c = coeffs + (int64_t)d * pdim;
+    for (int k = 0; k < (D == 3 ? 4 : 1); ++k) {
+      double inner = 0.0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) inner += ty.w[y * 4 + b] * c[a + (int64_t)Gx * (sy + b + (int64_t)Gy * (sz + k))];
+      acc += (D == 3 ? tz.w[z * 4 + k] : 1.0) * inner;
+    }
+  }
+  rows[e] = acc;
+}
+
 // Per-axis B-spline support of the virtual samples for a separable geometry.
 // The grid continuous index along `axis` is computed from the full virtual
 // point with the reference's expressions; the other axes' terms are exact
@@ -383,6 +409,15 @@ hipError_t launch_pack_texels(const float* moving, const double* grad, int64_t n
                               float4* tex, hipStream_t st) {
   hipLaunchKernelGGL(pack_texels_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, moving, grad, n, D, dir,
                      tex);
+  return hipGetLastError();
+}
+
+hipError_t launch_bspline_rows(const double* coeffs, int32_t pdim, int Gx, int Gy, int D, int Ny, int Nz,
+                               const AxisTable* tabs, double* rows, hipStream_t st) {
+  const int64_t total = (int64_t)Nz * Ny * D * Gx;
+  const AxisTable tz = D == 3 ? tabs[2] : tabs[1];
+  hipLaunchKernelGGL(bspline_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, coeffs, pdim, Gx,
+                     Gy, D, Ny, total, tabs[1], tz, rows);
   return hipGetLastError();
 }
 

@@ -271,3 +271,24 @@ def test_fast_path_on_shrunk_virtual_domain(oracle_mod, factors):
         assert np.array_equal(a, b)
     assert v0 == pytest.approx(v1, rel=1e-7)
     _compare(oracle_mod, m_fast, fixed, moving, geom, tf, gradient=grad, virtual=virt)
+
+
+@pytest.mark.parametrize("shape,mesh", [((48, 44, 40), (4, 3, 5)), ((120, 100), (6, 5))])
+def test_bspline_fast_path_matches_exact_path(oracle_mod, shape, mesh):
+    """Certified B-spline fast path (displacement from per-row coefficient
+    lines, FP32 interpolation, exception lists) against the all-FP64 path:
+    identical bins and counts; and against the oracle."""
+    fixed, moving, geom = blob_problem(shape, 63)
+    grad = oracle_mod.gradient_image(moving, geom)
+    tf = bspline_for(geom, mesh, seed=64, sigma=1.2)
+    m_fast = _gpu_metric(fixed, moving, geom, tf, gradient=grad)
+    m_exact = _gpu_metric(fixed, moving, geom, tf, gradient=grad, exact=True)
+    v0, d0 = m_fast.GetValueAndDerivative()
+    assert m_fast.GetLastEvaluationPath() == "fast"
+    v1, d1 = m_exact.GetValueAndDerivative()
+    assert m_fast.GetNumberOfValidPoints() == m_exact.GetNumberOfValidPoints()
+    for a, b in zip(m_fast.GetSampleBins(), m_exact.GetSampleBins()):
+        assert np.array_equal(a, b)
+    assert v0 == pytest.approx(v1, rel=1e-7)
+    assert np.linalg.norm(d0 - d1) <= 1e-5 * np.linalg.norm(d1)
+    _compare(oracle_mod, m_fast, fixed, moving, geom, tf, gradient=grad)
