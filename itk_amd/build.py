@@ -21,6 +21,7 @@ HIPCC_FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-shared",
+    "-pthread",
     # the FP64 coordinate path must round like the reference's x86-64 build
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/itk_mattes_hip.h"
@@ -277,6 +278,9 @@ struct mmi_context {
       scratch;
   DevBuf exc_count, exc_list, exc_overflow;  // fast-path exception lists
   DevBuf ftab;                               // fast path: per-axis fixed cells (coarse levels)
+  // grow-only work space of the preparation filters (no device frees, which
+  // would synchronise the whole device, while other contexts run)
+  DevBuf work_a, work_b, work_grad;
   bool ftab_valid = false;
   mmi_geometry ftab_key_v{}, ftab_key_f{};
   // separable B-spline derivative: per-sample factors, contraction stages, axis tables
@@ -288,11 +292,29 @@ struct mmi_context {
   int blocks_per_cu = 4;                     // evaluation grid = CUs x blocks_per_cu workgroups of 256
   int grid = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // pinned host staging: B-spline coefficients up, result and derivative down
+  void* h_coeffs = nullptr;
+  size_t h_coeffs_bytes = 0;
+  void* h_out = nullptr;
+  size_t h_out_bytes = 0;
   float last_total_ms = 0, last_pass1_ms = 0, last_pass2_ms = 0;
   int32_t last_path = 0;  // 0 exact, 1 fast, 2 fast redone exactly
 };
 
 namespace {
+
+int ensure_host_pinned(void** p, size_t* have, size_t need) {
+  if (*p && *have >= need) return MMI_OK;
+  if (*p) {
+    MMI_HIP(hipDeviceSynchronize());  // no copy from the old buffer in flight (rare: growth only)
+    MMI_HIP(hipHostFree(*p));
+    *p = nullptr;
+    *have = 0;
+  }
+  MMI_HIP(hipHostMalloc(p, need ? need : 16, hipHostMallocDefault));
+  *have = need;
+  return MMI_OK;
+}
 
 int build_transform(mmi_context* ctx, int D, HostTransform& h, DevTransform& d, bool moving) {
   std::memset(&d, 0, sizeof(d));
@@ -346,7 +368,12 @@ int build_transform(mmi_context* ctx, int D, HostTransform& h, DevTransform& d, 
     d.grid = to_dev(g);
     d.pdim = (int32_t)nodes;
     MMI_HIP(ctx->coeffs.ensure(sizeof(double) * h.params.size()));
-    MMI_HIP(hipMemcpyAsync(ctx->coeffs.p, h.params.data(), sizeof(double) * h.params.size(), hipMemcpyHostToDevice,
+    // through pinned memory: an asynchronous copy, ordered before the next
+    // evaluation on the context stream
+    if (int rc = ensure_host_pinned(&ctx->h_coeffs, &ctx->h_coeffs_bytes, sizeof(double) * h.params.size()))
+      return rc;
+    std::memcpy(ctx->h_coeffs, h.params.data(), sizeof(double) * h.params.size());
+    MMI_HIP(hipMemcpyAsync(ctx->coeffs.p, ctx->h_coeffs, sizeof(double) * h.params.size(), hipMemcpyHostToDevice,
                            ctx->stream));
     d.coeffs = ctx->coeffs.as<double>();
     return MMI_OK;
@@ -682,7 +709,8 @@ int smooth_image(mmi_context* ctx, const float* in, const mmi_geometry& g, const
                   "The number of pixels along dimension " + std::to_string(i) +
                       " is less than 4. This filter requires a minimum of four pixels along the dimension to be "
                       "processed.");
-  DevBuf a, b;
+  DevBuf& a = ctx->work_a;
+  DevBuf& b = ctx->work_b;
   MMI_HIP(a.ensure(sizeof(double) * n));
   MMI_HIP(b.ensure(sizeof(double) * n));
   MMI_HIP(out.ensure(sizeof(float) * n));
@@ -702,8 +730,6 @@ int smooth_image(mmi_context* ctx, const float* in, const mmi_geometry& g, const
   }
   MMI_HIP(launch_double_to_float(src, out.as<float>(), n, ctx->stream));
   MMI_HIP(hipStreamSynchronize(ctx->stream));
-  a.release();
-  b.release();
   return MMI_OK;
 }
 
@@ -852,7 +878,8 @@ int mmi_context_destroy(mmi_context* ctx) {
                     &ctx->coeffs, &ctx->hist, &ctx->fmarg,
                     &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->result,
                     &ctx->partials, &ctx->deriv, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
-                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_overflow, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->ftab};
+                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_overflow, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->ftab, &ctx->work_a,
+                    &ctx->work_b, &ctx->work_grad};
   for (DevBuf* b : bufs) b->release();
   for (int j = 0; j < 3; ++j) {
     ctx->bs_start[j].release();
@@ -863,6 +890,8 @@ int mmi_context_destroy(mmi_context* ctx) {
   }
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
+  if (ctx->h_coeffs) (void)hipHostFree(ctx->h_coeffs);
+  if (ctx->h_out) (void)hipHostFree(ctx->h_out);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MMI_OK;
@@ -906,7 +935,7 @@ int mmi_set_moving_gradient(mmi_context* ctx, const double* gradient) {
   MMI_HIP(hipSetDevice(ctx->device));
   const int D = ctx->moving_g.dim;
   const int64_t n = num_pixels(ctx->moving_g);
-  DevBuf tmp;
+  DevBuf& tmp = ctx->work_grad;
   MMI_HIP(tmp.ensure(sizeof(double) * n * D));
   MMI_HIP(hipMemcpyAsync(tmp.p, gradient, sizeof(double) * n * D, hipMemcpyHostToDevice, ctx->stream));
   MMI_HIP(ctx->texels.ensure(sizeof(float4) * n));
@@ -915,7 +944,6 @@ int mmi_set_moving_gradient(mmi_context* ctx, const double* gradient) {
   MMI_HIP(launch_pack_texels(ctx->moving_px.as<float>(), tmp.as<double>(), n, D, ident, ctx->texels.as<float4>(),
                              ctx->stream));
   MMI_HIP(hipStreamSynchronize(ctx->stream));
-  tmp.release();
   ctx->have_texels = true;
   ctx->user_texels = true;
   return refresh_storage_texels(ctx);
@@ -938,7 +966,9 @@ int mmi_compute_moving_gradient(mmi_context* ctx) {
   // sigma = maximum spacing, normalised across scale (I2IM:487-503).
   double sigma = 0.0;
   for (int i = 0; i < D; ++i) sigma = std::max(sigma, g.spacing[i]);
-  DevBuf a, b, grad;
+  DevBuf& a = ctx->work_a;
+  DevBuf& b = ctx->work_b;
+  DevBuf& grad = ctx->work_grad;
   MMI_HIP(a.ensure(sizeof(double) * n));
   MMI_HIP(b.ensure(sizeof(double) * n));
   MMI_HIP(grad.ensure(sizeof(double) * n * D));
@@ -970,9 +1000,6 @@ int mmi_compute_moving_gradient(mmi_context* ctx) {
   MMI_HIP(ctx->texels.ensure(sizeof(float4) * n));
   MMI_HIP(launch_pack_texels(level_moving(ctx), grad.as<double>(), n, D, dir, ctx->texels.as<float4>(), ctx->stream));
   MMI_HIP(hipStreamSynchronize(ctx->stream));
-  a.release();
-  b.release();
-  grad.release();
   ctx->have_texels = true;
   ctx->user_texels = false;
   for (int i = 0; i < 3; ++i) ctx->texel_sigma[i] = ctx->moving_smoothed ? ctx->moving_sigma[i] : 0.0;
@@ -1374,16 +1401,25 @@ retry:
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     }
   }
-  DevResult r;
-  uint32_t overflow = 0;
-  if (fast) MMI_HIP(hipMemcpyAsync(&overflow, ctx->exc_overflow.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  MMI_HIP(hipMemcpyAsync(&r, ctx->result.p, sizeof(DevResult), hipMemcpyDeviceToHost, st));
+  // result, overflow flag and derivative through one pinned buffer
+  if (int rc = ensure_host_pinned(&ctx->h_out, &ctx->h_out_bytes, sizeof(DevResult) + 16 + sizeof(double) * P))
+    return rc;
+  char* hout = static_cast<char*>(ctx->h_out);
+  uint32_t* h_overflow = reinterpret_cast<uint32_t*>(hout + sizeof(DevResult));
+  double* h_deriv = reinterpret_cast<double*>(hout + sizeof(DevResult) + 16);
+  *h_overflow = 0;
+  if (fast) MMI_HIP(hipMemcpyAsync(h_overflow, ctx->exc_overflow.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  MMI_HIP(hipMemcpyAsync(hout, ctx->result.p, sizeof(DevResult), hipMemcpyDeviceToHost, st));
   if (want_derivative && derivative)
-    MMI_HIP(hipMemcpyAsync(derivative, ctx->deriv.p, sizeof(double) * P, hipMemcpyDeviceToHost, st));
+    MMI_HIP(hipMemcpyAsync(h_deriv, ctx->deriv.p, sizeof(double) * P, hipMemcpyDeviceToHost, st));
   if (joint_pdf) MMI_HIP(hipMemcpyAsync(joint_pdf, ctx->pdf.p, sizeof(double) * B * B, hipMemcpyDeviceToHost, st));
   if (fixed_marginal)
     MMI_HIP(hipMemcpyAsync(fixed_marginal, ctx->fm.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
   MMI_HIP(hipStreamSynchronize(st));
+  DevResult r;
+  std::memcpy(&r, hout, sizeof(DevResult));
+  const uint32_t overflow = *h_overflow;
+  if (want_derivative && derivative) std::memcpy(derivative, h_deriv, sizeof(double) * P);
   if (overflow) {
     // more uncertain samples in one workgroup than its list holds: redo exactly
     fast = false;
@@ -1546,6 +1582,9 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
   }
   const int nf = (int)fused.size();
   b->last_fused = nf;
+  // uploads made on the contexts' own streams (tables, parameters) complete
+  // before the batch stream reads them
+  for (int k = 0; k < nf; ++k) MMI_HIP(hipStreamSynchronize(ctxs[fused[k]]->stream));
   const int stride = 16;  // value, count, status, up to 12 derivative entries (+ pad)
   if (nf > 0) {
     const size_t o_start = 0;
@@ -1643,14 +1682,23 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
     }
     MMI_HIP(hipEventRecord(b->ev1, b->stream));
   }
-  // the other contexts, one by one, while the fused launch runs
-  for (int i : single) {
+  // the other contexts, each on its own stream from its own host thread, while
+  // the fused launch runs
+  auto run_single = [&](int i) {
     double v = 0.0;
     int64_t nv = 0;
     status[i] = mmi_evaluate(ctxs[i], want_derivative, &v, want_derivative ? derivatives[i] : nullptr, &nv, nullptr,
                              nullptr);
     values[i] = v;
     n_valid[i] = nv;
+  };
+  if (single.size() == 1) {
+    run_single(single[0]);
+  } else if (!single.empty()) {
+    std::vector<std::thread> workers;
+    workers.reserve(single.size());
+    for (int i : single) workers.emplace_back(run_single, i);
+    for (auto& w : workers) w.join();
   }
   if (nf > 0) {
     MMI_HIP(hipStreamSynchronize(b->stream));

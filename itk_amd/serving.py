@@ -117,10 +117,10 @@ class _BatchingMetric:
 
 
 class _Job:
-    def __init__(self, server, req: RegistrationRequest, device: int, storage: str):
+    def __init__(self, server, lane, req: RegistrationRequest, metric):
         self.req = req
-        self.metric = MattesMutualInformationImageToImageMetricv4(device=device)
-        self.metric.SetImageStorage(storage)
+        self.lane = lane
+        self.metric = metric
         self.proxy = _BatchingMetric(self.metric, server, self)
         self.server = server
         self.admitted = 0.0
@@ -140,118 +140,164 @@ class _Job:
         self.server._finish(self, params, err)
 
 
+class _Lane:
+    """One batching loop with its own stream: it gathers the evaluations its
+    registrations wait for and runs them as one batch.  With two lanes, one
+    lane's registrations take their host-side optimizer steps while the
+    other's batch runs on the device."""
+
+    def __init__(self, server, device: int, graphs: bool):
+        self.server = server
+        self.batch = BatchEvaluator(device, graphs=graphs)
+        self.cv = threading.Condition(server.lock)
+        self.pending = []
+        self.first = None
+        self.running = set()
+        self.thread = threading.Thread(target=self._loop, daemon=True)
+
+    def _loop(self):
+        srv = self.server
+        while True:
+            with srv.lock:
+                while True:
+                    if srv.stopping and not self.running and not self.pending:
+                        self.batch.close()
+                        return
+                    if self.pending and (len(self.pending) >= len(self.running) or
+                                         time.perf_counter() - self.first >= srv.max_wait):
+                        break
+                    self.cv.wait(srv.max_wait if self.pending else 0.05)
+                items, self.pending, self.first = self.pending, [], None
+            self._run_batch(items)
+
+    def _run_batch(self, items):
+        srv = self.server
+        for want in (True, False):
+            group = [it for it in items if it[1] == want]
+            if not group:
+                continue
+            out = self.batch.evaluate([job.metric for job, _, _ in group], want_derivative=want)
+            with srv.lock:
+                srv.batches += 1
+                srv.batched_evaluations += len(group)
+                srv.max_batch = max(srv.max_batch, len(group))
+            for (job, _, slot), (v, d, nv, err) in zip(group, out):
+                slot["value"], slot["deriv"], slot["err"] = v, d, err
+                slot["event"].set()
+
+
 class RegistrationServer:
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 64 << 30, max_inflight: int = 32,
-                 max_wait_s: float = 0.002, graphs: bool = True, storage: str = "f32"):
+                 max_wait_s: float = 0.0005, graphs: bool = True, storage: str = "f32", lanes: int = 2):
         self.device = device
         self.budget = int(hbm_budget_bytes)
         self.max_inflight = int(max_inflight)
         self.max_wait = float(max_wait_s)
         self.storage = storage
-        self.batch = BatchEvaluator(device, graphs=graphs)
-        self.cv = threading.Condition()
-        self.pending = []      # [job, want_derivative, slot]
-        self.running = set()
+        self.graphs = graphs
+        self.n_lanes = max(1, int(lanes))
+        self.lock = threading.Lock()
+        self.main_cv = threading.Condition(self.lock)
         self.results = []
         self.batches = 0
         self.batched_evaluations = 0
         self.max_batch = 0
         self.peak_inflight = 0
+        self.stopping = False
+        self.pool = []  # idle metrics: contexts whose device buffers only grow
 
     # -- called from worker threads ----------------------------------------------
     def _evaluate(self, job, want_derivative: bool):
-        slot = {"done": False}
-        with self.cv:
-            self.pending.append((job, want_derivative, slot))
-            self.cv.notify_all()
-            while not slot["done"]:
-                self.cv.wait()
+        slot = {"event": threading.Event()}
+        lane = job.lane
+        with self.lock:
+            if not lane.pending:
+                lane.first = time.perf_counter()
+            lane.pending.append((job, want_derivative, slot))
+            if len(lane.pending) >= len(lane.running):
+                lane.cv.notify()
+        slot["event"].wait()
         job.evaluations += 1
         if slot["err"] is not None:
             raise slot["err"]
         return slot["value"], slot["deriv"]
 
     def _finish(self, job, params, err):
-        with self.cv:
+        with self.lock:
+            self.pool.append(job.metric)  # device buffers kept for the next registration
             job.result = RegistrationResult(job.req.rid, job.req.kind, params, job.req.arrival, job.admitted,
                                             self._now(), job.evaluations, err)
             self.results.append(job.result)
-            self.running.discard(job)
+            job.lane.running.discard(job)
+            self.running -= 1
             self.used -= job.req.device_bytes()
-            self.cv.notify_all()
-        job.metric.close()
+            job.lane.cv.notify()
+            self.main_cv.notify()
 
-    # -- scheduler ---------------------------------------------------------------
     def _now(self) -> float:
         return time.perf_counter() - self.t0
-
-    def _run_batch(self, items):
-        for want in (True, False):
-            group = [it for it in items if it[1] == want]
-            if not group:
-                continue
-            out = self.batch.evaluate([job.metric for job, _, _ in group], want_derivative=want)
-            self.batches += 1
-            self.batched_evaluations += len(group)
-            self.max_batch = max(self.max_batch, len(group))
-            with self.cv:
-                for (job, _, slot), (v, d, nv, err) in zip(group, out):
-                    slot.update(value=v, deriv=d, err=err, done=True)
-                self.cv.notify_all()
 
     def run(self, requests):
         """Serve the requests (arrival times relative to now); returns the results."""
         queue = sorted(requests, key=lambda r: (r.arrival, r.rid))
         self.t0 = time.perf_counter()
         self.used = 0
+        self.running = 0
+        self.stopping = False
+        lanes = [_Lane(self, self.device, self.graphs) for _ in range(self.n_lanes)]
         total = len(queue)
-        while True:
-            with self.cv:
+        for ln in lanes:
+            ln.thread.start()
+        with self.lock:
+            while len(self.results) < total:
                 now = self._now()
-                # admission: arrived, within budget and slots, in arrival order
-                while queue and queue[0].arrival <= now and len(self.running) < self.max_inflight and (
-                        self.used + queue[0].device_bytes() <= self.budget or not self.running):
+                # admission: arrived, within budget and slots, in arrival order;
+                # each registration joins the lane with fewer running ones
+                while queue and queue[0].arrival <= now and self.running < self.max_inflight and (
+                        self.used + queue[0].device_bytes() <= self.budget or self.running == 0):
                     req = queue.pop(0)
-                    job = _Job(self, req, self.device, self.storage)
+                    lane = min(lanes, key=lambda ln: len(ln.running))
+                    if self.pool:
+                        metric = self.pool.pop()
+                    else:
+                        metric = MattesMutualInformationImageToImageMetricv4(device=self.device)
+                        metric.SetImageStorage(self.storage)
+                    job = _Job(self, lane, req, metric)
                     job.admitted = now
                     self.used += req.device_bytes()
-                    self.running.add(job)
-                    self.peak_inflight = max(self.peak_inflight, len(self.running))
+                    self.running += 1
+                    lane.running.add(job)
+                    self.peak_inflight = max(self.peak_inflight, self.running)
                     job.thread.start()
-                if len(self.results) == total:
-                    break
-                if not self.running:
-                    # idle until the next arrival
-                    self.cv.wait(max(1e-4, queue[0].arrival - self._now()))
-                    continue
-                # gather: every running registration waiting, or the first
-                # waiter has waited max_wait, or a new request has arrived
-                t_first = None
-                while len(self.pending) < len(self.running):
-                    if self.pending and t_first is None:
-                        t_first = time.perf_counter()
-                    if t_first is not None and time.perf_counter() - t_first >= self.max_wait:
-                        break
-                    if queue and queue[0].arrival <= self._now() and len(self.running) < self.max_inflight:
-                        break
-                    timeout = self.max_wait if self.pending else 0.01
-                    if queue:
-                        timeout = min(timeout, max(1e-4, queue[0].arrival - self._now()))
-                    self.cv.wait(timeout)
-                    if len(self.results) == total:
-                        break
-                items = self.pending
-                self.pending = []
-            if items:
-                self._run_batch(items)
-        self.batch.close()
+                timeout = 0.05
+                if queue:
+                    timeout = min(timeout, max(1e-4, queue[0].arrival - self._now()))
+                self.main_cv.wait(timeout)
+            self.stopping = True
+            for ln in lanes:
+                ln.cv.notify()
+        for ln in lanes:
+            ln.thread.join()
+        for m in self.pool:
+            m.close()
+        self.pool = []
         return sorted(self.results, key=lambda r: r.rid)
 
 
 def run_sequential(requests, device: int = 0, storage: str = "f32"):
-    """The same requests one at a time on the plain metric, in arrival order
-    (each starts at max(arrival, previous finish))."""
+    """The same requests one at a time on the plain metric (one metric reused,
+    as a one-at-a-time service would), in arrival order: each starts at
+    max(arrival, previous finish)."""
     queue = sorted(requests, key=lambda r: (r.arrival, r.rid))
+    m = MattesMutualInformationImageToImageMetricv4(device=device)
+    m.SetImageStorage(storage)
+    calls = [0]
+    orig = m.GetValueAndDerivative
+
+    def counted():
+        calls[0] += 1
+        return orig()
+    m.GetValueAndDerivative = counted
     t0 = time.perf_counter()
     results = []
     busy = 0.0
@@ -260,27 +306,19 @@ def run_sequential(requests, device: int = 0, storage: str = "f32"):
         if now < req.arrival:
             time.sleep(req.arrival - now)
         start = time.perf_counter() - t0
-        m = MattesMutualInformationImageToImageMetricv4(device=device)
-        m.SetImageStorage(storage)
+        calls[0] = 0
         err = None
         tf = None
-        calls = [0]
-        orig = m.GetValueAndDerivative
-
-        def counted():
-            calls[0] += 1
-            return orig()
-        m.GetValueAndDerivative = counted
         try:
             reg, tf, _ = build_registration(req, m)
             reg.Update()
         except Exception as e:
             err = e
-        m.close()
         end = time.perf_counter() - t0
         busy += end - start
         results.append(RegistrationResult(req.rid, req.kind, tf.get_parameters() if tf is not None else None,
                                           req.arrival, start, end, calls[0], err))
+    m.close()
     return sorted(results, key=lambda r: r.rid), busy
 
 
