@@ -120,6 +120,8 @@ def serving(args):
 
     seq, busy = run_sequential(requests(False))
     print("# sequential: %d registrations in %.2f s" % (len(seq), busy), file=sys.stderr, flush=True)
+    _, busy_native = run_sequential(requests(False), native_loop=True)
+    print("# sequential, C optimizer loop: %.2f s" % busy_native, file=sys.stderr, flush=True)
     server = RegistrationServer(max_inflight=args.inflight)
     sat = server.run(requests(False))
     sat_sum = latency_summary(sat)
@@ -155,6 +157,7 @@ def serving(args):
                    "bspline": "mesh 8^3 or 16^3, L-BFGS-B, <=50 iterations", "parallelism": "single GPU"},
         "serving": {
             "sequential_registrations_per_s": seq_rate,
+            "sequential_c_loop_registrations_per_s": len(seq) / busy_native,
             "batched_registrations_per_s": sat_sum["registrations_per_s"],
             "speedup_vs_sequential": sat_sum["registrations_per_s"] / seq_rate,
             "batched_saturated": dict(sat_sum, **sat_stats),
@@ -240,6 +243,9 @@ def main():
         print("bench.py measures one GPU only (this build has no multi-GPU path); run with --gpus 1", file=sys.stderr)
         return 2
     if args.serving:
+        # many registrations' streams run at once: give the process more
+        # hardware queues than HIP's default 4 (set before the runtime starts)
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
         return serving(args)
     from itk_amd.metric import MattesMutualInformationImageToImageMetricv4
 

@@ -184,6 +184,58 @@ int mmi_batch_evaluate(mmi_batch* batch, mmi_context* const* ctxs, int32_t n, in
 /* Device time of the last fused launch and how many contexts it held. */
 int mmi_batch_get_last_timing(mmi_batch* batch, float* device_ms, int32_t* n_fused);
 
+/* RegularStepGradientDescentOptimizerv4 over a batch
+ * (Modules/Numerics/Optimizersv4/include/itkRegularStepGradientDescentOptimizerv4.hxx:25-171,
+ * itkGradientDescentOptimizerv4.hxx:72-165): every iteration evaluates all
+ * still-running contexts in one batched pass, then takes each one's
+ * AdvanceOneStep on the host, in the reference's arithmetic (Kahan sums, the
+ * relaxation on a direction change, the previous gradient re-multiplied by
+ * 1/scales each step), so each context's iterates equal a one-at-a-time run
+ * bit for bit.  For transforms of at most 12 parameters (affine).  The
+ * caller owns one state per context, started with mmi_regular_step_start;
+ * each call runs at most max_steps iterations, so contexts can join and
+ * leave between calls.  The final parameters stay in each context
+ * (mmi_get_moving_parameters). */
+typedef struct mmi_regular_step_options {
+  double learning_rate;
+  double minimum_step_length;
+  double relaxation_factor;
+  double gradient_magnitude_tolerance;
+  int32_t number_of_iterations;
+  int32_t pad_;
+} mmi_regular_step_options;
+
+enum mmi_stop_condition {
+  MMI_STOP_RUNNING = 0,
+  MMI_STOP_MAXIMUM_NUMBER_OF_ITERATIONS = 1,
+  MMI_STOP_GRADIENT_MAGNITUDE_TOLERANCE = 2,
+  MMI_STOP_STEP_TOO_SMALL = 3,
+  MMI_STOP_METRIC_ERROR = 4
+};
+
+typedef struct mmi_regular_step_state {
+  double gradient[12];
+  double previous_gradient[12];
+  double scales[12];
+  double relaxation;  /* CurrentLearningRateRelaxation */
+  double value;       /* last metric value */
+  double gradient_magnitude;
+  int32_t n_scales;   /* 0: identity scales */
+  int32_t scales_identity;
+  int32_t iteration;  /* CurrentIteration */
+  int32_t stop;       /* enum mmi_stop_condition */
+  int32_t status;     /* metric status when stop == MMI_STOP_METRIC_ERROR */
+  int32_t pad_;
+} mmi_regular_step_state;
+
+/* StartOptimization: zero gradients, relaxation 1, iteration 0; scales (one
+ * per parameter, > eps) or NULL for identity. */
+int mmi_regular_step_start(mmi_regular_step_state* state, const double* scales, int32_t n_scales);
+int mmi_batch_regular_step(mmi_batch* batch, mmi_context* const* ctxs, int32_t n,
+                           const mmi_regular_step_options* options, mmi_regular_step_state* states, int32_t max_steps);
+/* The moving transform parameters a context holds (P doubles). */
+int mmi_get_moving_parameters(mmi_context* ctx, double* params, int64_t n_params);
+
 #ifdef __cplusplus
 }
 #endif

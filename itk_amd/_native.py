@@ -59,12 +59,36 @@ SIGNATURES = {
                                           ctypes.POINTER(ctypes.c_int32)]),
     "mmi_batch_get_last_timing": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float),
                                                  ctypes.POINTER(ctypes.c_int32)]),
+    "mmi_regular_step_start": (ctypes.c_int, [_vp, _dp, ctypes.c_int32]),
+    "mmi_batch_regular_step": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_int32, _vp, _vp, ctypes.c_int32]),
+    "mmi_get_moving_parameters": (ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
     "mmi_get_last_path": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "mmi_get_last_timing": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                            ctypes.POINTER(ctypes.c_float)]),
 }
 
 _lib = None
+
+
+class RegularStepOptions(ctypes.Structure):
+    _fields_ = [("learning_rate", ctypes.c_double), ("minimum_step_length", ctypes.c_double),
+                ("relaxation_factor", ctypes.c_double), ("gradient_magnitude_tolerance", ctypes.c_double),
+                ("number_of_iterations", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+class RegularStepState(ctypes.Structure):
+    _fields_ = [("gradient", ctypes.c_double * 12), ("previous_gradient", ctypes.c_double * 12),
+                ("scales", ctypes.c_double * 12), ("relaxation", ctypes.c_double), ("value", ctypes.c_double),
+                ("gradient_magnitude", ctypes.c_double), ("n_scales", ctypes.c_int32),
+                ("scales_identity", ctypes.c_int32), ("iteration", ctypes.c_int32), ("stop", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+MMI_STOP_RUNNING = 0
+MMI_STOP_MAXIMUM_NUMBER_OF_ITERATIONS = 1
+MMI_STOP_GRADIENT_MAGNITUDE_TOLERANCE = 2
+MMI_STOP_STEP_TOO_SMALL = 3
+MMI_STOP_METRIC_ERROR = 4
 
 
 class MetricError(RuntimeError):

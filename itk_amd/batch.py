@@ -7,6 +7,7 @@ evaluating each metric on its own.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 
 import numpy as np
@@ -69,8 +70,46 @@ class BatchEvaluator:
             out.append((float(values[i]), derivs[i], int(nvalid[i]), err))
         return out
 
+    def regular_step(self, metrics, options, states, max_steps: int):
+        """Up to max_steps RegularStep iterations for every metric whose state
+        is still running (mmi_batch_regular_step); the metrics' transforms
+        receive the new parameters."""
+        n = len(metrics)
+        if n == 0:
+            return
+        ctxs = (ctypes.c_void_p * n)()
+        for i, m in enumerate(metrics):
+            if not m._initialized:
+                raise N.MetricError(N.MMI_ERR_ARGUMENT, "Initialize() must be called first")
+            m._push_parameters()
+            ctxs[i] = m._ctx
+        arr = (N.RegularStepState * n)(*states)
+        N.check(self._L.mmi_batch_regular_step(self._b, ctxs, n, ctypes.byref(options), arr, int(max_steps)))
+        for i, m in enumerate(metrics):
+            states[i] = arr[i]
+            p = np.zeros(m.GetNumberOfParameters())
+            N.check(self._L.mmi_get_moving_parameters(m._ctx, p.ctypes.data_as(N._dp), p.size))
+            m._moving_tf.set_parameters(p)
+
     def last_timing(self) -> dict:
         ms = ctypes.c_float()
         nf = ctypes.c_int32()
         N.check(self._L.mmi_batch_get_last_timing(self._b, ctypes.byref(ms), ctypes.byref(nf)))
         return {"device_ms": ms.value, "fused": nf.value}
+
+
+_shared = {}
+
+
+def shared_evaluator(device: int = 0) -> BatchEvaluator:
+    """One evaluator per device for single registrations' native loops."""
+    if device not in _shared:
+        _shared[device] = BatchEvaluator(device)
+    return _shared[device]
+
+
+@atexit.register
+def _close_shared():
+    for ev in _shared.values():
+        ev.close()
+    _shared.clear()

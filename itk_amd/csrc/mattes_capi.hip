@@ -286,6 +286,7 @@ struct mmi_context {
   // separable B-spline derivative: per-sample factors, contraction stages, axis tables
   DevBuf bs_g, bs_t1, bs_t2, bs_rows;
   DevBuf bs_start[3], bs_w[3], bs_inside[3], bs_lo[3], bs_hi[3];
+  int32_t bs_increasing[3] = {0, 0, 0};
   bool bs_tables = false;
   DevGeom bs_key_virt{}, bs_key_grid{};
   int xcd_chunks = 1;                        // fast path: one contiguous sample share per XCD
@@ -805,6 +806,13 @@ int bspline_axis_tables(mmi_context* ctx, const DevGeom& virt, const DevGeom& gr
       for (int g = 0; g < nodes; ++g)
         for (int i = lo[g]; i <= hi[g]; ++i)
           if (!inside[i] || g < start[i] || g > start[i] + 3) return MMI_OK;  // not usable: per-sample path
+      int32_t inc = 1, prev = std::numeric_limits<int32_t>::min();
+      for (int i = 0; i < len; ++i) {
+        if (!inside[i]) continue;
+        if (start[i] < prev) inc = 0;
+        prev = start[i];
+      }
+      ctx->bs_increasing[j] = inc;
       MMI_HIP(hipMemcpyAsync(ctx->bs_lo[j].p, lo.data(), sizeof(int32_t) * nodes, hipMemcpyHostToDevice, ctx->stream));
       MMI_HIP(hipMemcpyAsync(ctx->bs_hi[j].p, hi.data(), sizeof(int32_t) * nodes, hipMemcpyHostToDevice, ctx->stream));
       MMI_HIP(hipStreamSynchronize(ctx->stream));
@@ -820,6 +828,7 @@ int bspline_axis_tables(mmi_context* ctx, const DevGeom& virt, const DevGeom& gr
     tabs[j].inside = ctx->bs_inside[j].as<uint8_t>();
     tabs[j].lo = ctx->bs_lo[j].as<int32_t>();
     tabs[j].hi = ctx->bs_hi[j].as<int32_t>();
+    tabs[j].increasing = ctx->bs_increasing[j];
   }
   return MMI_OK;
 }
@@ -1729,6 +1738,156 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
         std::memcpy(derivatives[i], o + 3, sizeof(double) * c->mtf.params.size());
     }
   }
+  return MMI_OK;
+}
+
+int mmi_regular_step_start(mmi_regular_step_state* st, const double* scales, int32_t n_scales) {
+  if (!st) return fail(MMI_ERR_ARGUMENT, "state is NULL");
+  if (n_scales < 0 || n_scales > 12 || (n_scales > 0 && !scales)) return fail(MMI_ERR_ARGUMENT, "bad scales");
+  std::memset(st, 0, sizeof(*st));
+  st->relaxation = 1.0;
+  st->n_scales = n_scales;
+  st->scales_identity = 1;
+  for (int i = 0; i < n_scales; ++i) {
+    if (!(scales[i] > std::numeric_limits<double>::epsilon()))
+      return fail(MMI_ERR_ARGUMENT, "m_Scales values must be > epsilon.");
+    st->scales[i] = scales[i];
+    // identity within 0.01 (itkObjectToObjectOptimizerBase.cxx:113-133)
+    if (std::fabs(1.0 - scales[i]) > 0.01) st->scales_identity = 0;
+  }
+  return MMI_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// CompensatedSummation (itkCompensatedSummation.hxx:38-48).
+struct Kahan {
+  double sum = 0.0, c = 0.0;
+  void add(double x) {
+    const double y = x - c;
+    const double t = sum + y;
+    c = (t - sum) - y;
+    sum = t;
+  }
+};
+
+// RegularStepGradientDescentOptimizerv4::AdvanceOneStep; returns the stop
+// condition it met (MMI_STOP_RUNNING when it updated the parameters).
+int regular_step_advance(mmi_regular_step_state& s, const mmi_regular_step_options& o, std::vector<double>& params) {
+  const int P = (int)params.size();
+  if (!s.scales_identity && P > 0) {
+    for (int j = 0; j < P; ++j) {
+      const double f = 1.0 / s.scales[j % s.n_scales];
+      s.gradient[j] = s.gradient[j] * f;
+      s.previous_gradient[j] = s.previous_gradient[j] * f;
+    }
+  }
+  Kahan g2;
+  for (int j = 0; j < P; ++j) {
+    const double w = s.gradient[j];
+    g2.add(w * w);
+  }
+  const double magnitude = std::sqrt(g2.sum);
+  s.gradient_magnitude = magnitude;
+  if (magnitude < o.gradient_magnitude_tolerance) return MMI_STOP_GRADIENT_MAGNITUDE_TOLERANCE;
+  Kahan sp;
+  for (int j = 0; j < P; ++j) {
+    const double a = s.gradient[j], b = s.previous_gradient[j];
+    sp.add(a * b);
+  }
+  if (sp.sum < 0) s.relaxation *= o.relaxation_factor;
+  const double step = s.relaxation * o.learning_rate;
+  if (step < o.minimum_step_length) return MMI_STOP_STEP_TOO_SMALL;
+  for (int j = 0; j < P; ++j) s.gradient[j] = s.gradient[j] * s.relaxation * o.learning_rate;
+  const double factor = 1.0 / magnitude;
+  // Transform::UpdateTransformParameters (itkTransform.hxx:71-115)
+  for (int j = 0; j < P; ++j) params[j] = factor == 1.0 ? params[j] + s.gradient[j] : params[j] + s.gradient[j] * factor;
+  return MMI_STOP_RUNNING;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mmi_batch_regular_step(mmi_batch* b, mmi_context* const* ctxs, int32_t n, const mmi_regular_step_options* o,
+                           mmi_regular_step_state* states, int32_t max_steps) {
+  if (!b || !o || (n > 0 && (!ctxs || !states))) return fail(MMI_ERR_ARGUMENT, "NULL argument");
+  if (o->relaxation_factor < 0.0)
+    return fail(MMI_ERR_ARGUMENT, "Relaxation factor must be positive. Current value is " +
+                                      std::to_string(o->relaxation_factor));
+  if (o->relaxation_factor >= 1.0)
+    return fail(MMI_ERR_ARGUMENT, "Relaxation factor must be less than 1.0. Current value is " +
+                                      std::to_string(o->relaxation_factor));
+  if (o->gradient_magnitude_tolerance < 0.0)
+    return fail(MMI_ERR_ARGUMENT, "Gradient magnitude tolerance must be greater or equal 0.0.");
+  for (int i = 0; i < n; ++i) {
+    if (!ctxs[i]) return fail(MMI_ERR_ARGUMENT, "context is NULL");
+    const size_t P = ctxs[i]->mtf.params.size();
+    if (P > 12) return fail(MMI_ERR_ARGUMENT, "mmi_batch_regular_step handles transforms of at most 12 parameters");
+    if (states[i].n_scales != 0 && (size_t)states[i].n_scales != P)
+      return fail(MMI_ERR_ARGUMENT, "The size of Scales differs from the number of parameters");
+  }
+  std::vector<mmi_context*> act;
+  std::vector<int> idx;
+  std::vector<double> values;
+  std::vector<double*> derivs;
+  std::vector<int64_t> nvalid;
+  std::vector<int32_t> status;
+  for (int step = 0; step < max_steps; ++step) {
+    act.clear();
+    idx.clear();
+    for (int i = 0; i < n; ++i) {
+      mmi_regular_step_state& s = states[i];
+      if (s.stop != MMI_STOP_RUNNING) continue;
+      if (s.iteration >= o->number_of_iterations) {  // checked before evaluating (GDv4:82-89)
+        s.stop = MMI_STOP_MAXIMUM_NUMBER_OF_ITERATIONS;
+        continue;
+      }
+      act.push_back(ctxs[i]);
+      idx.push_back(i);
+    }
+    if (act.empty()) break;
+    const int m = (int)act.size();
+    values.assign(m, 0.0);
+    nvalid.assign(m, 0);
+    status.assign(m, 0);
+    derivs.assign(m, nullptr);
+    for (int k = 0; k < m; ++k) {
+      mmi_regular_step_state& s = states[idx[k]];
+      std::memcpy(s.previous_gradient, s.gradient, sizeof(s.gradient));  // swap; the evaluation overwrites
+      derivs[k] = s.gradient;
+    }
+    if (int rc = mmi_batch_evaluate(b, act.data(), m, 1, values.data(), derivs.data(), nvalid.data(), status.data()))
+      return rc;
+    for (int k = 0; k < m; ++k) {
+      mmi_regular_step_state& s = states[idx[k]];
+      mmi_context* c = act[k];
+      if (status[k] != MMI_OK) {
+        s.stop = MMI_STOP_METRIC_ERROR;
+        s.status = status[k];
+        continue;
+      }
+      s.value = values[k];
+      std::vector<double> params = c->mtf.params;
+      const int stop = regular_step_advance(s, *o, params);
+      s.iteration += 1;
+      if (stop != MMI_STOP_RUNNING) {
+        s.stop = stop;
+        continue;
+      }
+      if (int rc = mmi_set_moving_parameters(c, params.data(), (int64_t)params.size())) return rc;
+    }
+  }
+  return MMI_OK;
+}
+
+int mmi_get_moving_parameters(mmi_context* ctx, double* params, int64_t n_params) {
+  if (!ctx || !params) return fail(MMI_ERR_ARGUMENT, "NULL argument");
+  if ((int64_t)ctx->mtf.params.size() != n_params)
+    return fail(MMI_ERR_ARGUMENT, "parameter count differs from the moving transform's");
+  std::memcpy(params, ctx->mtf.params.data(), sizeof(double) * n_params);
   return MMI_OK;
 }
 

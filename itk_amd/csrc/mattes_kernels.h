@@ -137,6 +137,8 @@ struct AxisTable {
   const uint8_t* inside; // [len]
   const int32_t* lo;     // [nodes]
   const int32_t* hi;     // [nodes]
+  int32_t increasing;    // support start nondecreasing along the axis
+  int32_t pad_;
 };
 
 struct FinalizeArgs {

@@ -98,18 +98,28 @@ __global__ void recursive_gaussian_kernel(const double* __restrict__ in, double*
     o3 = q0;
   }
   double xm1 = X(3), xm2 = X(2), xm3 = X(1);
-  for (int64_t i = 4; i < ln; ++i) {
-    const double xi = X(i);
-    double q = xi * c.N0 + xm1 * c.N1 + xm2 * c.N2 + xm3 * c.N3;
-    q -= o0 * c.D1 + o1 * c.D2 + o2 * c.D3 + o3 * c.D4;
-    y[i * stride] = q;
-    o3 = o2;
-    o2 = o1;
-    o1 = o0;
-    o0 = q;
-    xm3 = xm2;
-    xm2 = xm1;
-    xm1 = xi;
+  // the recursion is sequential along the line; its inputs are loaded a chunk
+  // ahead so the loads of one chunk are in flight together
+  constexpr int kChunk = 8;
+  for (int64_t i0 = 4; i0 < ln; i0 += kChunk) {
+    double xs[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) xs[k] = i0 + k < ln ? X(i0 + k) : 0.0;
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) {
+      if (i0 + k >= ln) break;
+      const double xi = xs[k];
+      double q = xi * c.N0 + xm1 * c.N1 + xm2 * c.N2 + xm3 * c.N3;
+      q -= o0 * c.D1 + o1 * c.D2 + o2 * c.D3 + o3 * c.D4;
+      y[(i0 + k) * stride] = q;
+      o3 = o2;
+      o2 = o1;
+      o1 = o0;
+      o0 = q;
+      xm3 = xm2;
+      xm2 = xm1;
+      xm1 = xi;
+    }
   }
   // anti-causal
   const double v2 = X(ln - 1);
@@ -134,19 +144,32 @@ __global__ void recursive_gaussian_kernel(const double* __restrict__ in, double*
     s3 = r1;
   }
   double xp1 = X(ln - 3), xp2 = X(ln - 2), xp3 = X(ln - 1);  // data[i+1..i+3] for i = ln-4
-  for (int64_t i = ln - 4; i > 0; --i) {
-    const double xi = X(i);
-    // scratch[i-1] = data[i]*M1 + data[i+1]*M2 + data[i+2]*M3 + data[i+3]*M4 - (...)
-    double q = xi * c.M1 + xp1 * c.M2 + xp2 * c.M3 + xp3 * c.M4;
-    q -= s0 * c.D1 + s1 * c.D2 + s2 * c.D3 + s3 * c.D4;
-    y[(i - 1) * stride] += q;
-    s3 = s2;
-    s2 = s1;
-    s1 = s0;
-    s0 = q;
-    xp3 = xp2;
-    xp2 = xp1;
-    xp1 = xi;
+  for (int64_t i0 = ln - 4; i0 > 0; i0 -= kChunk) {
+    // i runs i0, i0-1, ... > 0; it reads data[i] and updates out[i-1]
+    double xs[kChunk], ys[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) {
+      const int64_t i = i0 - k;
+      xs[k] = i > 0 ? X(i) : 0.0;
+      ys[k] = i > 0 ? y[(i - 1) * stride] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) {
+      const int64_t i = i0 - k;
+      if (i <= 0) break;
+      const double xi = xs[k];
+      // scratch[i-1] = data[i]*M1 + data[i+1]*M2 + data[i+2]*M3 + data[i+3]*M4 - (...)
+      double q = xi * c.M1 + xp1 * c.M2 + xp2 * c.M3 + xp3 * c.M4;
+      q -= s0 * c.D1 + s1 * c.D2 + s2 * c.D3 + s3 * c.D4;
+      y[(i - 1) * stride] = ys[k] + q;
+      s3 = s2;
+      s2 = s1;
+      s1 = s0;
+      s0 = q;
+      xp3 = xp2;
+      xp2 = xp1;
+      xp1 = xi;
+    }
   }
 }
 
@@ -296,45 +319,50 @@ __global__ void __launch_bounds__(256) separable_contract_kernel(const T* __rest
   out[e] = acc;
 }
 
-// The first (x) axis of the contraction.  Persistent workgroups of four
-// wavefronts take four rows at a time; the axis table and the rows are staged
-// in LDS (coalesced loads), then each output node sums its range from LDS in
-// index order.
-constexpr int kRowThreads = 256;
-constexpr int kRowsPerStep = kRowThreads / 64;
-
-__global__ void __launch_bounds__(kRowThreads) separable_rows_kernel(const float* __restrict__ in,
-                                                                     double* __restrict__ out, int64_t rows, int L,
-                                                                     int G, AxisTable t) {
-  extern __shared__ double lds_rows[];
-  double* w = lds_rows;                                       // [L][4]
-  int32_t* start = reinterpret_cast<int32_t*>(w + 4 * L);     // [L]
-  int32_t* lo = start + L;                                    // [G]
-  int32_t* hi = lo + G;                                       // [G]
-  float* row = reinterpret_cast<float*>(hi + G);              // [kRowsPerStep][L]
-  for (int i = threadIdx.x; i < 4 * L; i += kRowThreads) w[i] = t.w[i];
-  for (int i = threadIdx.x; i < L; i += kRowThreads) start[i] = t.start[i];
-  for (int i = threadIdx.x; i < G; i += kRowThreads) {
-    lo[i] = t.lo[i];
-    hi[i] = t.hi[i];
-  }
-  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int64_t r0 = (int64_t)blockIdx.x * kRowsPerStep; r0 < rows; r0 += (int64_t)gridDim.x * kRowsPerStep) {
-    __syncthreads();  // previous rows consumed (and the tables staged, first time)
-    const int64_t nr = min((int64_t)kRowsPerStep, rows - r0);
-    for (int i = threadIdx.x; i < nr * L; i += kRowThreads) row[i] = in[r0 * L + i];
-    __syncthreads();
-    const int64_t r = r0 + wid;
-    if (wid < nr) {
-      const float* x = row + wid * L;
-      for (int g = lane; g < G; g += 64) {
-        double acc = 0.0;
-        const int h = hi[g];
-        for (int l = lo[g]; l <= h; ++l) acc += w[l * 4 + (g - start[l])] * (double)x[l];
-        out[r * G + g] = acc;
-      }
+// The first (x) axis of the contraction, one thread per row: the row's
+// samples stream in index order through a window of the four nodes the
+// current support covers; a node is written once the window has moved past
+// it.  Every node receives its terms in increasing index order starting from
+// zero, as in separable_contract_kernel, so both give the same bits.  Needs
+// a nondecreasing support start along the axis.
+__global__ void __launch_bounds__(256) separable_rows_kernel(const float* __restrict__ in, double* __restrict__ out,
+                                                             int64_t rows, int L, int G, AxisTable t) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const float* x = in + r * L;
+  double* o = out + r * G;
+  int base = -1, written = 0;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (int l = 0; l < L; ++l) {
+    if (!t.inside[l]) continue;
+    const int s = t.start[l];
+    if (base < 0) base = s;
+    while (base < s) {
+      for (; written < base; ++written) o[written] = 0.0;
+      o[base] = a0;
+      written = base + 1;
+      a0 = a1;
+      a1 = a2;
+      a2 = a3;
+      a3 = 0.0;
+      ++base;
     }
+    const double v = (double)x[l];
+    const double* w = t.w + 4 * l;
+    a0 += w[0] * v;
+    a1 += w[1] * v;
+    a2 += w[2] * v;
+    a3 += w[3] * v;
   }
+  if (base >= 0) {
+    for (; written < base; ++written) o[written] = 0.0;
+    o[base] = a0;
+    o[base + 1] = a1;
+    o[base + 2] = a2;
+    o[base + 3] = a3;
+    written = base + 4;
+  }
+  for (; written < G; ++written) o[written] = 0.0;
 }
 
 hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, const DevResult* r, double* deriv,
@@ -434,12 +462,9 @@ hipError_t launch_separable_contract(const float* in_f, const double* in_d, doub
   const int64_t total = outer * G * inner;
   if (total <= 0) return hipSuccess;
   const dim3 grid((unsigned)((total + 255) / 256));
-  const size_t row_lds = (size_t)L * (4 * sizeof(double) + sizeof(int32_t) + kRowsPerStep * sizeof(float)) +
-                         (size_t)G * 2 * sizeof(int32_t);
-  if (in_f && inner == 1 && !scale && row_lds <= 48 * 1024) {
-    const int64_t steps = (outer + kRowsPerStep - 1) / kRowsPerStep;
-    const unsigned blocks = (unsigned)std::min<int64_t>(steps, 2048);
-    hipLaunchKernelGGL(separable_rows_kernel, dim3(blocks), dim3(kRowThreads), row_lds, st, in_f, out, outer, L, G, t);
+  if (in_f && inner == 1 && !scale && t.increasing) {
+    hipLaunchKernelGGL(separable_rows_kernel, dim3((unsigned)((outer + 255) / 256)), dim3(256), 0, st, in_f, out,
+                       outer, L, G, t);
   } else if (in_f)
     hipLaunchKernelGGL(separable_contract_kernel<float>, grid, dim3(256), 0, st, in_f, out, total, L, inner, G, t,
                        scale);

@@ -15,6 +15,7 @@ exactly the reference's split between optimizer and metric.
 """
 from __future__ import annotations
 
+import ctypes
 import math
 from enum import Enum
 
@@ -124,6 +125,12 @@ class RegularStepGradientDescentOptimizerv4(_OptimizerBase):
         self.current_relaxation = 0.0
         self.gradient = None
         self.previous_gradient = None
+        # The iterations run in the C ABI's batched loop (bit-identical to the
+        # loop below) when the metric is the device metric of a transform of at
+        # most 12 parameters and nothing needs a per-iteration callback.
+        # native_runner(metric, options, state) may hand them to a serving lane.
+        self.use_native_loop = True
+        self.native_runner = None
 
     # itk setters
     def SetLearningRate(self, v: float):
@@ -154,8 +161,64 @@ class RegularStepGradientDescentOptimizerv4(_OptimizerBase):
         if self.return_best:
             self.best_parameters = self.metric.GetParameters()
             self.best_value = float(np.finfo(np.float64).max)
-        if not do_only_initialization:
+        if do_only_initialization:
+            return
+        if self._native_ok():
+            self._run_native()
+        else:
             self.ResumeOptimization()
+
+    def _native_ok(self) -> bool:
+        if not self.use_native_loop or self.observers or self.return_best:
+            return False
+        if self.metric.GetNumberOfParameters() > 12:
+            return False
+        if self.native_runner is not None:
+            return True
+        from .metric import MattesMutualInformationImageToImageMetricv4
+        return type(self.metric) is MattesMutualInformationImageToImageMetricv4
+
+    def _run_native(self):
+        from . import _native as N
+        from .batch import shared_evaluator
+
+        opts = N.RegularStepOptions(self.learning_rate, self.minimum_step_length, self.relaxation_factor,
+                                    self.gradient_magnitude_tolerance, self.number_of_iterations, 0)
+        st = N.RegularStepState()
+        sc = np.ascontiguousarray(self.scales, dtype=np.float64)
+        L = N.load()
+        N.check(L.mmi_regular_step_start(ctypes.byref(st), sc.ctypes.data_as(N._dp), sc.size))
+        if self.native_runner is not None:
+            st = self.native_runner(self.metric, opts, st)
+        else:
+            states = [st]
+            shared_evaluator().regular_step([self.metric], opts, states, self.number_of_iterations + 1)
+            st = states[0]
+        P = self.metric.GetNumberOfParameters()
+        self.current_iteration = int(st.iteration)
+        self.current_value = float(st.value)
+        self.gradient = np.array(st.gradient[:P])
+        self.previous_gradient = np.array(st.previous_gradient[:P])
+        self.current_relaxation = float(st.relaxation)
+        self.stop_description = "RegularStepGradientDescentOptimizerv4: "
+        if st.stop == N.MMI_STOP_METRIC_ERROR:
+            self.stop_condition = StopCondition.COSTFUNCTION_ERROR
+            self.stop_description += "Metric error during optimization"
+            raise N.MetricError(int(st.status), "metric evaluation failed during optimization (status %d)"
+                                % st.status)
+        if st.stop == N.MMI_STOP_MAXIMUM_NUMBER_OF_ITERATIONS:
+            self.stop_condition = StopCondition.MAXIMUM_NUMBER_OF_ITERATIONS
+            self.stop_description += "Maximum number of iterations (%d) exceeded." % self.number_of_iterations
+        elif st.stop == N.MMI_STOP_GRADIENT_MAGNITUDE_TOLERANCE:
+            self.stop_condition = StopCondition.GRADIENT_MAGNITUDE_TOLERANCE
+            self.stop_description += ("Gradient magnitude tolerance met after %d iterations. Gradient magnitude (%g) "
+                                      "is less than gradient magnitude tolerance (%g)."
+                                      % (st.iteration - 1, st.gradient_magnitude, self.gradient_magnitude_tolerance))
+        elif st.stop == N.MMI_STOP_STEP_TOO_SMALL:
+            self.stop_condition = StopCondition.STEP_TOO_SMALL
+            self.stop_description += ("Step too small after %d iterations. Current step (%g) is less than minimum "
+                                      "step (%g)." % (st.iteration - 1, st.relaxation * self.learning_rate,
+                                                      self.minimum_step_length))
 
     def StopOptimization(self):
         if self.return_best:

@@ -51,7 +51,7 @@ class RegistrationRequest:
         return int(self.geometry.num_pixels) * 72 + (64 << 20)
 
 
-def build_registration(req: RegistrationRequest, metric):
+def build_registration(req: RegistrationRequest, metric, native_loop: bool = True):
     """The registration each request runs: affine with RegularStep gradient
     descent over the pyramid, or a cubic B-spline with L-BFGS-B on one level."""
     geom = req.geometry
@@ -64,6 +64,7 @@ def build_registration(req: RegistrationRequest, metric):
         opt = RegularStepGradientDescentOptimizerv4(
             learning_rate=req.learning_rate, minimum_step_length=1e-3, relaxation_factor=0.5,
             number_of_iterations=req.iterations, scales_estimator=RegistrationParameterScalesFromPhysicalShift())
+        opt.use_native_loop = native_loop
         reg.SetShrinkFactorsPerLevel(req.shrink)
         reg.SetSmoothingSigmasPerLevel(req.sigmas)
         reg.SetSmoothingSigmasAreSpecifiedInPhysicalUnits(False)
@@ -132,7 +133,9 @@ class _Job:
         err = None
         tf = None
         try:
-            reg, tf, _ = build_registration(self.req, self.proxy)
+            reg, tf, opt = build_registration(self.req, self.proxy)
+            if isinstance(opt, RegularStepGradientDescentOptimizerv4):
+                opt.native_runner = lambda m, o, st: self.server._optimize(self, o, st)
             reg.Update()
         except Exception as e:  # reported in the result
             err = e
@@ -152,6 +155,7 @@ class _Lane:
         self.cv = threading.Condition(server.lock)
         self.pending = []
         self.first = None
+        self.optimizing = []  # [job, options, state, event]: RegularStep levels run in the C loop
         self.running = set()
         self.thread = threading.Thread(target=self._loop, daemon=True)
 
@@ -163,12 +167,43 @@ class _Lane:
                     if srv.stopping and not self.running and not self.pending:
                         self.batch.close()
                         return
-                    if self.pending and (len(self.pending) >= len(self.running) or
-                                         time.perf_counter() - self.first >= srv.max_wait):
+                    waiting = len(self.pending) + len(self.optimizing)
+                    if waiting and (waiting >= len(self.running) or
+                                    time.perf_counter() - self.first >= srv.max_wait):
                         break
-                    self.cv.wait(srv.max_wait if self.pending else 0.05)
-                items, self.pending, self.first = self.pending, [], None
-            self._run_batch(items)
+                    self.cv.wait(srv.max_wait if waiting else 0.05)
+                items, self.pending = self.pending, []
+                self.first = time.perf_counter() if self.optimizing else None
+                opt = list(self.optimizing)
+            if opt:
+                self._run_optimizing(opt)
+            if items:
+                self._run_batch(items)
+
+    def _run_optimizing(self, jobs):
+        """A few RegularStep iterations of every registration optimizing in this
+        lane, grouped by optimizer settings, in the C loop."""
+        srv = self.server
+        groups = {}
+        for it in jobs:
+            o = it[1]
+            key = (o.learning_rate, o.minimum_step_length, o.relaxation_factor, o.gradient_magnitude_tolerance,
+                   o.number_of_iterations)
+            groups.setdefault(key, []).append(it)
+        for group in groups.values():
+            states = [it[2] for it in group]
+            self.batch.regular_step([it[0].metric for it in group], group[0][1], states, srv.chunk_steps)
+            done = []
+            with srv.lock:
+                srv.batches += srv.chunk_steps
+                srv.max_batch = max(srv.max_batch, len(group))
+                for it, st in zip(group, states):
+                    it[2] = st
+                    if st.stop != 0:
+                        self.optimizing.remove(it)
+                        done.append(it)
+            for it in done:
+                it[3].set()
 
     def _run_batch(self, items):
         srv = self.server
@@ -188,7 +223,8 @@ class _Lane:
 
 class RegistrationServer:
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 64 << 30, max_inflight: int = 32,
-                 max_wait_s: float = 0.0005, graphs: bool = True, storage: str = "f32", lanes: int = 2):
+                 max_wait_s: float = 0.0005, graphs: bool = True, storage: str = "f32", lanes: int = 2,
+                 chunk_steps: int = 5):
         self.device = device
         self.budget = int(hbm_budget_bytes)
         self.max_inflight = int(max_inflight)
@@ -196,6 +232,7 @@ class RegistrationServer:
         self.storage = storage
         self.graphs = graphs
         self.n_lanes = max(1, int(lanes))
+        self.chunk_steps = max(1, int(chunk_steps))
         self.lock = threading.Lock()
         self.main_cv = threading.Condition(self.lock)
         self.results = []
@@ -221,6 +258,19 @@ class RegistrationServer:
         if slot["err"] is not None:
             raise slot["err"]
         return slot["value"], slot["deriv"]
+
+    def _optimize(self, job, options, state):
+        """Hand one RegularStep level to the job's lane; returns the final state."""
+        it = [job, options, state, threading.Event()]
+        lane = job.lane
+        with self.lock:
+            if not lane.pending and not lane.optimizing:
+                lane.first = time.perf_counter()
+            lane.optimizing.append(it)
+            lane.cv.notify()
+        it[3].wait()
+        job.evaluations += int(it[2].iteration)  # one evaluation per iteration
+        return it[2]
 
     def _finish(self, job, params, err):
         with self.lock:
@@ -284,10 +334,12 @@ class RegistrationServer:
         return sorted(self.results, key=lambda r: r.rid)
 
 
-def run_sequential(requests, device: int = 0, storage: str = "f32"):
+def run_sequential(requests, device: int = 0, storage: str = "f32", native_loop: bool = False):
     """The same requests one at a time on the plain metric (one metric reused,
     as a one-at-a-time service would), in arrival order: each starts at
-    max(arrival, previous finish)."""
+    max(arrival, previous finish).  By default the optimizer calls the metric
+    once per iteration from Python, as with the reference's optimizers;
+    native_loop runs affine levels in the C loop instead."""
     queue = sorted(requests, key=lambda r: (r.arrival, r.rid))
     m = MattesMutualInformationImageToImageMetricv4(device=device)
     m.SetImageStorage(storage)
@@ -310,7 +362,7 @@ def run_sequential(requests, device: int = 0, storage: str = "f32"):
         err = None
         tf = None
         try:
-            reg, tf, _ = build_registration(req, m)
+            reg, tf, _ = build_registration(req, m, native_loop=native_loop)
             reg.Update()
         except Exception as e:
             err = e

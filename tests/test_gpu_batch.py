@@ -102,3 +102,66 @@ def test_batch_value_only_and_without_graphs():
         assert err is None and d is None
         assert v == m.GetValue()
     be.close()
+
+
+def _rsgd(native: bool, metric, iterations=40):
+    from itk_amd.optimizers import RegularStepGradientDescentOptimizerv4
+    from itk_amd.scales import RegistrationParameterScalesFromPhysicalShift
+
+    est = RegistrationParameterScalesFromPhysicalShift()
+    est.SetTransform(metric.GetMovingTransform())
+    est.SetVirtualDomain(metric._virtual or metric._fixed[1])
+    opt = RegularStepGradientDescentOptimizerv4(learning_rate=2.0, minimum_step_length=1e-3,
+                                                number_of_iterations=iterations, scales=est.EstimateScales())
+    opt.use_native_loop = native
+    opt.SetMetric(metric)
+    return opt
+
+
+def test_native_regular_step_matches_python_loop():
+    """The C ABI's batched RegularStep loop gives the Python optimizer's
+    iterates bit for bit, one registration at a time and several at once."""
+    from itk_amd import _native as N
+    from itk_amd import synth
+
+    probs = []
+    for k, shape in enumerate([(48, 44, 40), (40, 36, 34), (56, 50, 44)]):
+        p = synth.make_affine_problem(shape, 120 + k, n_blobs=12)
+        probs.append(p)
+    ref = []
+    for p in probs:
+        tf = affine_for(p.geometry)
+        m = _metric(p.fixed, p.moving, p.geometry, tf)
+        opt = _rsgd(False, m)
+        opt.StartOptimization()
+        ref.append((tf.get_parameters(), opt.GetCurrentIteration(), opt.GetCurrentMetricValue(),
+                    opt.GetStopCondition()))
+    # one at a time, native
+    for p, (params, its, val, stop) in zip(probs, ref):
+        tf = affine_for(p.geometry)
+        m = _metric(p.fixed, p.moving, p.geometry, tf)
+        opt = _rsgd(True, m)
+        opt.StartOptimization()
+        assert opt.GetCurrentIteration() == its and opt.GetStopCondition() == stop
+        assert opt.GetCurrentMetricValue() == val
+        assert np.array_equal(tf.get_parameters(), params)
+    # all together, in chunks of 7 iterations
+    be = BatchEvaluator()
+    ms, states = [], []
+    for p in probs:
+        tf = affine_for(p.geometry)
+        m = _metric(p.fixed, p.moving, p.geometry, tf)
+        opt = _rsgd(True, m)
+        opt._prepare_scales()
+        st = N.RegularStepState()
+        sc = np.ascontiguousarray(opt.scales)
+        N.check(N.load().mmi_regular_step_start(__import__("ctypes").byref(st), sc.ctypes.data_as(N._dp), sc.size))
+        ms.append(m)
+        states.append(st)
+    opts = N.RegularStepOptions(2.0, 1e-3, 0.5, 1e-4, 40, 0)
+    while any(s.stop == N.MMI_STOP_RUNNING for s in states):
+        be.regular_step(ms, opts, states, 7)
+    for m, st, (params, its, val, _) in zip(ms, states, ref):
+        assert st.iteration == its and st.value == val
+        assert np.array_equal(m.GetMovingTransform().get_parameters(), params)
+    be.close()
