@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--requests", type=int, default=32, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=40.0, help="serving: mean arrivals per second")
     ap.add_argument("--inflight", type=int, default=32, help="serving: registrations in flight at most")
+    ap.add_argument("--serving-requests", type=int, default=16,
+                    help="requests of the serving summary added to the metric line (0: none)")
     return ap.parse_args()
 
 
@@ -92,19 +94,28 @@ def _synthesize(args):
     return rid, kind, size, mesh, p.fixed, p.moving
 
 
-def serving(args):
-    """Registrations/s and latency of the continuous-batching server on a
-    Poisson stream, against the same requests one at a time."""
+def synthesize_requests(args):
+    """The serving requests' images, made by a pool of worker processes (run
+    before this process starts the GPU runtime)."""
     import concurrent.futures as cf
-
-    from itk_amd.geometry import ImageGeometry
-    from itk_amd.serving import RegistrationRequest, RegistrationServer, latency_summary, run_sequential
 
     t0 = time.perf_counter()
     workers = min(16, os.cpu_count() or 1)
     with cf.ProcessPoolExecutor(max_workers=workers) as ex:
         made = list(ex.map(_synthesize, [(i, args.seed) for i in range(args.requests)]))
     print("# synthesized %d requests in %.1f s" % (len(made), time.perf_counter() - t0), file=sys.stderr, flush=True)
+    return made
+
+
+def serving(args, quiet: bool = False, made=None):
+    """Registrations/s and latency of the continuous-batching server on a
+    Poisson stream, against the same requests one at a time.  Prints the JSON
+    line, or (quiet) returns its serving summary."""
+    from itk_amd.geometry import ImageGeometry
+    from itk_amd.serving import RegistrationRequest, RegistrationServer, latency_summary, run_sequential
+
+    if made is None:
+        made = synthesize_requests(args)
     from itk_amd import synth
 
     gaps = -np.log(1.0 - synth.uniform(args.seed, 29, args.requests)) / args.rate
@@ -166,6 +177,8 @@ def serving(args):
             "results_identical_alone_and_batched": bool(same),
         },
     }
+    if quiet:
+        return dict(line["serving"], requests=args.requests, kinds=kinds, arrival_rate_per_s=args.rate)
     print(json.dumps(line), flush=True)
     return 0
 
@@ -243,10 +256,12 @@ def main():
         print("bench.py measures one GPU only (this build has no multi-GPU path); run with --gpus 1", file=sys.stderr)
         return 2
     if args.serving:
-        # many registrations' streams run at once: give the process more
-        # hardware queues than HIP's default 4 (set before the runtime starts)
-        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
         return serving(args)
+    made = None
+    if args.serving_requests > 0:
+        sargs = argparse.Namespace(**vars(args))
+        sargs.requests = args.serving_requests
+        made = synthesize_requests(sargs)
     from itk_amd.metric import MattesMutualInformationImageToImageMetricv4
 
     wl = Workload(args)
@@ -310,6 +325,10 @@ def main():
     m.close()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
+    if made is not None:
+        # the metric's second half: registrations of mixed sizes served by the
+        # continuous-batching server (a short run of the --serving workload)
+        line["serving"] = serving(sargs, quiet=True, made=made)
     print(json.dumps(line), flush=True)
     return 0
 
