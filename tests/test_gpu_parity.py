@@ -251,6 +251,31 @@ def test_device_smoothing_matches_oracle(oracle_mod):
     _compare(oracle_mod, m, fixed, moving, geom, tf, gradient=oracle_mod.gradient_image(moving, geom))
 
 
+def test_user_gradient_with_smoothing(oracle_mod):
+    """A gradient image given by the caller stays as given while the moving
+    image is smoothed for a level; the fast derivative pass reads the moving
+    value from the texels, whose value channel follows the smoothed image:
+    bins, count, value and derivative as the oracle's, on both levels."""
+    fixed, moving, geom = blob_problem((48, 44, 40), 43)
+    grad = oracle_mod.gradient_image(moving, geom)
+    tf = affine_for(geom, affine_params(3, np.random.default_rng(44)))
+    m = Metric(record_sample_bins=True)
+    m.SetFixedImage(fixed, geom)
+    m.SetMovingImage(moving, geom)
+    m.SetMovingImageGradient(grad)
+    m.SetSmoothingSigmas([1.0] * 3, [1.5] * 3)
+    m.SetMovingTransform(tf)
+    m.Initialize()
+    fs = oracle_mod.smooth(fixed, geom, [1.0] * 3)
+    ms = oracle_mod.smooth(moving, geom, [1.5] * 3)
+    _compare(oracle_mod, m, fs, ms, geom, tf, gradient=grad)
+    assert m.GetLastEvaluationPath() == "fast"
+    m.SetSmoothingSigmas(None, None)
+    m.Initialize()
+    _compare(oracle_mod, m, fixed, moving, geom, tf, gradient=grad)
+    assert m.GetLastEvaluationPath() == "fast"
+
+
 @pytest.mark.parametrize("factors", [2, 4, [2, 3, 1]])
 def test_fast_path_on_shrunk_virtual_domain(oracle_mod, factors):
     """Coarse pyramid levels: virtual samples fall between fixed pixels, so the
