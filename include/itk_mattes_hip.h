@@ -148,6 +148,15 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
  * counted), in virtual buffer order; for parity tests.  Either may be NULL. */
 int mmi_get_sample_bins(mmi_context* ctx, int16_t* fixed_bin, int16_t* moving_bin);
 
+/* The moving texels the evaluation interpolates, FP32 (value, gradient x, y,
+ * z; 2-D: z = 0) per moving pixel in buffer order: the value channel is the
+ * level's moving image (smoothed when the level smooths it), the gradient the
+ * output of GradientRecursiveGaussianImageFilter
+ * (itkGradientRecursiveGaussianImageFilter.hxx:214-238) or the caller's
+ * gradient, in physical space.  Diagnostic (parity tests of the device
+ * filters); texels4 holds 4 floats per pixel. */
+int mmi_get_moving_texels(mmi_context* ctx, float* texels4);
+
 /* Bin geometry chosen by mmi_initialize. */
 int mmi_get_histogram_parameters(mmi_context* ctx, double* fixed_min, double* fixed_max, double* moving_min,
                                  double* moving_max, double* fixed_bin_size, double* moving_bin_size);

@@ -50,6 +50,7 @@ SIGNATURES = {
     "mmi_set_moving_parameters": (ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
     "mmi_evaluate": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp, ctypes.POINTER(ctypes.c_int64), _dp, _dp]),
     "mmi_get_sample_bins": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int16), ctypes.POINTER(ctypes.c_int16)]),
+    "mmi_get_moving_texels": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "mmi_get_histogram_parameters": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "mmi_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     "mmi_batch_destroy": (ctypes.c_int, [_vp]),

@@ -715,9 +715,10 @@ int smooth_image(mmi_context* ctx, const float* in, const mmi_geometry& g, const
   MMI_HIP(a.ensure(sizeof(double) * n));
   MMI_HIP(b.ensure(sizeof(double) * n));
   MMI_HIP(out.ensure(sizeof(float) * n));
-  MMI_HIP(launch_float_to_double(in, a.as<double>(), n, ctx->stream));
-  double* src = a.as<double>();
-  double* dst = b.as<double>();
+  // float image in, FP64 between the passes, float image out (the casts are
+  // those of the reference's pipeline, done inside the first and last pass)
+  const void* src = in;
+  double* bufs[2] = {a.as<double>(), b.as<double>()};
   for (int step = 0; step < D; ++step) {
     const int axis = step == 0 ? D - 1 : step - 1;
     RecCoeffs c;
@@ -726,10 +727,16 @@ int smooth_image(mmi_context* ctx, const float* in, const mmi_geometry& g, const
     int64_t stride = 1;
     for (int j = 0; j < axis; ++j) stride *= g.size[j];
     const int64_t ln = g.size[axis];
-    MMI_HIP(launch_recursive_gaussian(src, dst, ln, stride, n / ln, c, ctx->stream));
-    std::swap(src, dst);
+    RecSink sink{};
+    sink.mode = kSinkDouble;
+    if (step == D - 1) {
+      sink.mode = kSinkFloat;
+      sink.outf = out.as<float>();
+    }
+    double* dst = bufs[step & 1];
+    MMI_HIP(launch_recursive_gaussian(src, step == 0, dst, ln, stride, n / ln, c, sink, ctx->stream));
+    src = dst;
   }
-  MMI_HIP(launch_double_to_float(src, out.as<float>(), n, ctx->stream));
   MMI_HIP(hipStreamSynchronize(ctx->stream));
   return MMI_OK;
 }
@@ -981,10 +988,10 @@ int mmi_compute_moving_gradient(mmi_context* ctx) {
   MMI_HIP(a.ensure(sizeof(double) * n));
   MMI_HIP(b.ensure(sizeof(double) * n));
   MMI_HIP(grad.ensure(sizeof(double) * n * D));
+  double* bufs[2] = {a.as<double>(), b.as<double>()};
   for (int dim = 0; dim < D; ++dim) {
-    MMI_HIP(launch_float_to_double(level_moving(ctx), a.as<double>(), n, ctx->stream));
-    double* src = a.as<double>();
-    double* dst = b.as<double>();
+    // the float image widened inside the first pass; FP64 between passes
+    const void* src = level_moving(ctx);
     // derivative along dim, then zero order along the other axes in
     // increasing order (GradRG:214-238)
     int order[3] = {dim, -1, -1};
@@ -999,10 +1006,20 @@ int mmi_compute_moving_gradient(mmi_context* ctx) {
       int64_t stride = 1;
       for (int j = 0; j < axis; ++j) stride *= g.size[j];
       const int64_t ln = g.size[axis];
-      MMI_HIP(launch_recursive_gaussian(src, dst, ln, stride, n / ln, c, ctx->stream));
-      std::swap(src, dst);
+      // the last pass writes gradient component dim (value / spacing)
+      RecSink sink{};
+      sink.mode = kSinkDouble;
+      if (step == D - 1) {
+        sink.mode = kSinkGradient;
+        sink.comp = dim;
+        sink.dim = D;
+        sink.spacing = g.spacing[dim];
+        sink.grad = grad.as<double>();
+      }
+      double* dst = bufs[step & 1];
+      MMI_HIP(launch_recursive_gaussian(src, step == 0, dst, ln, stride, n / ln, c, sink, ctx->stream));
+      src = dst;
     }
-    MMI_HIP(launch_gradient_component(src, n, g.spacing[dim], dim, grad.as<double>(), D, ctx->stream));
   }
   Direction dir{};
   for (int i = 0; i < D * D; ++i) dir.m[i] = g.direction[i];
@@ -1908,6 +1925,17 @@ int mmi_get_sample_bins(mmi_context* ctx, int16_t* fixed_bin, int16_t* moving_bi
     MMI_HIP(hipMemcpyAsync(fixed_bin, ctx->sample_fbin.p, sizeof(int16_t) * n, hipMemcpyDeviceToHost, ctx->stream));
   if (moving_bin)
     MMI_HIP(hipMemcpyAsync(moving_bin, ctx->sample_mbin.p, sizeof(int16_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+  MMI_HIP(hipStreamSynchronize(ctx->stream));
+  return MMI_OK;
+}
+
+int mmi_get_moving_texels(mmi_context* ctx, float* texels4) {
+  if (!ctx || !texels4) return fail(MMI_ERR_ARGUMENT, "NULL argument");
+  if (!ctx->initialized) return fail(MMI_ERR_ARGUMENT, "not initialized");
+  if (!ctx->have_texels) return fail(MMI_ERR_ARGUMENT, "no moving gradient image (central differences)");
+  MMI_HIP(hipSetDevice(ctx->device));
+  MMI_HIP(hipMemcpyAsync(texels4, ctx->texels.p, sizeof(float4) * num_pixels(ctx->moving_g), hipMemcpyDeviceToHost,
+                         ctx->stream));
   MMI_HIP(hipStreamSynchronize(ctx->stream));
   return MMI_OK;
 }

@@ -200,6 +200,19 @@ struct RecCoeffs {
   double N0, N1, N2, N3, D1, D2, D3, D4, M1, M2, M3, M4, BN1, BN2, BN3, BN4, BM1, BM2, BM3, BM4;
 };
 
+// Destination of a recursive-Gaussian pass's final values.
+constexpr int32_t kSinkDouble = 0;    // the pass's double output volume
+constexpr int32_t kSinkGradient = 1;  // grad[e * dim + comp] = value / spacing
+constexpr int32_t kSinkFloat = 2;     // outf[e] = (float)value
+struct RecSink {
+  int32_t mode;
+  int32_t comp, dim;
+  int32_t pad_;
+  double spacing;
+  double* grad;
+  float* outf;
+};
+
 struct Direction {
   double m[9];
 };
@@ -236,12 +249,8 @@ hipError_t launch_minmax(const void* px, int storage, int64_t n, float* out, int
 // FP32 image -> 16-bit storage (round to nearest even), and FP32 texels -> 16-bit texels.
 hipError_t launch_to_storage(const float* in, void* out, int storage, int64_t n, hipStream_t st);
 hipError_t launch_texels_to_storage(const float4* in, void* out, int storage, int64_t n, hipStream_t st);
-hipError_t launch_recursive_gaussian(const double* in, double* out, int64_t ln, int64_t stride, int64_t nlines,
-                                     const RecCoeffs& c, hipStream_t st);
-hipError_t launch_float_to_double(const float* in, double* out, int64_t n, hipStream_t st);
-hipError_t launch_double_to_float(const double* in, float* out, int64_t n, hipStream_t st);
-hipError_t launch_gradient_component(const double* vol, int64_t n, double spacing, int comp, double* grad, int D,
-                                     hipStream_t st);
+hipError_t launch_recursive_gaussian(const void* in, bool in_float, double* out, int64_t ln, int64_t stride,
+                                     int64_t nlines, const RecCoeffs& c, const RecSink& sink, hipStream_t st);
 hipError_t launch_pack_texels(const float* moving, const double* grad, int64_t n, int D, const Direction& dir,
                               float4* tex, hipStream_t st);
 

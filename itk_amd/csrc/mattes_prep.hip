@@ -61,19 +61,34 @@ __global__ void minmax_kernel(const typename P::pixel* __restrict__ px, int64_t 
   }
 }
 
-// Recursive Gaussian along one axis of a double volume, one line per thread
+// Recursive Gaussian along one axis of a volume, one line per thread
 // (RecursiveSeparableImageFilter::FilterDataArray,
 // Modules/Filtering/ImageFilterBase/include/itkRecursiveSeparableImageFilter.hxx:62-143):
 // the causal pass is written to `out`, the anti-causal pass is added in a
-// second sweep; both keep their 4-tap histories in registers.
-__global__ void recursive_gaussian_kernel(const double* __restrict__ in, double* __restrict__ out, int64_t ln,
-                                          int64_t stride, int64_t nlines, RecCoeffs c) {
+// second sweep; both keep their 4-tap histories in registers.  The input is
+// the float image (first pass of a chain: its values widened exactly) or a
+// double volume; the final values go where `sink` says (the double volume,
+// value / spacing into one component of D-interleaved gradient vectors, or
+// a float volume), so the chain needs no separate conversion kernels.
+template <typename TIn>
+__global__ void recursive_gaussian_kernel(const TIn* __restrict__ in, double* __restrict__ out, int64_t ln,
+                                          int64_t stride, int64_t nlines, RecCoeffs c, RecSink sink) {
   const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= nlines) return;
   const int64_t lo = l % stride, hi = l / stride;
-  const double* x = in + lo + hi * stride * ln;
-  double* y = out + lo + hi * stride * ln;
-  auto X = [&](int64_t i) { return x[i * stride]; };
+  const int64_t base = lo + hi * stride * ln;
+  const TIn* x = in + base;
+  double* y = out + base;
+  auto X = [&](int64_t i) { return (double)x[i * stride]; };
+  auto FIN = [&](int64_t i, double v) {
+    const int64_t e = base + i * stride;
+    if (sink.mode == kSinkGradient)
+      sink.grad[e * sink.dim + sink.comp] = v / sink.spacing;
+    else if (sink.mode == kSinkFloat)
+      sink.outf[e] = (float)v;
+    else
+      out[e] = v;
+  };
   // causal
   const double v1 = X(0);
   double o0, o1, o2, o3;  // outs[i-1..i-4]
@@ -134,10 +149,10 @@ __global__ void recursive_gaussian_kernel(const double* __restrict__ in, double*
     r2 -= r1 * c.D1 + v2 * c.BM2 + v2 * c.BM3 + v2 * c.BM4;
     r3 -= r2 * c.D1 + r1 * c.D2 + v2 * c.BM3 + v2 * c.BM4;
     r4 -= r3 * c.D1 + r2 * c.D2 + r1 * c.D3 + v2 * c.BM4;
-    y[(ln - 1) * stride] += r1;
-    y[(ln - 2) * stride] += r2;
-    y[(ln - 3) * stride] += r3;
-    y[(ln - 4) * stride] += r4;
+    FIN(ln - 1, y[(ln - 1) * stride] + r1);
+    FIN(ln - 2, y[(ln - 2) * stride] + r2);
+    FIN(ln - 3, y[(ln - 3) * stride] + r3);
+    FIN(ln - 4, y[(ln - 4) * stride] + r4);
     s0 = r4;
     s1 = r3;
     s2 = r2;
@@ -161,7 +176,7 @@ __global__ void recursive_gaussian_kernel(const double* __restrict__ in, double*
       // scratch[i-1] = data[i]*M1 + data[i+1]*M2 + data[i+2]*M3 + data[i+3]*M4 - (...)
       double q = xi * c.M1 + xp1 * c.M2 + xp2 * c.M3 + xp3 * c.M4;
       q -= s0 * c.D1 + s1 * c.D2 + s2 * c.D3 + s3 * c.D4;
-      y[(i - 1) * stride] = ys[k] + q;
+      FIN(i - 1, ys[k] + q);
       s3 = s2;
       s2 = s1;
       s1 = s0;
@@ -171,24 +186,6 @@ __global__ void recursive_gaussian_kernel(const double* __restrict__ in, double*
       xp1 = xi;
     }
   }
-}
-
-__global__ void float_to_double_kernel(const float* __restrict__ in, double* __restrict__ out, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (double)in[i];
-}
-
-// Cast of the smoothing mini-pipeline's FP64 result to the FP32 pixel type.
-__global__ void double_to_float_kernel(const double* __restrict__ in, float* __restrict__ out, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (float)in[i];
-}
-
-// gradient component `comp` = vol / spacing into the texel's slot comp+1.
-__global__ void gradient_component_kernel(const double* __restrict__ vol, int64_t n, double spacing, int comp,
-                                          double* __restrict__ grad, int D) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) grad[i * D + comp] = vol[i] / spacing;
 }
 
 // TransformLocalVectorToPhysicalVector (itkImageBase.h:636-653) + packing into
@@ -365,6 +362,78 @@ __global__ void __launch_bounds__(256) separable_rows_kernel(const float* __rest
   for (; written < G; ++written) o[written] = 0.0;
 }
 
+// The first (x) axis when its table and a block of rows fit in LDS: a
+// workgroup stages the axis table (support starts, weights, node ranges) once
+// and then `nr` rows at a time with coalesced loads (rows padded by one float
+// against bank conflicts); each output node of the block is one thread summing
+// w[l][g - start[l]] * x[l] over l in [lo[g], hi[g]] in increasing l, the
+// order of separable_contract_kernel, so both give the same bits.  The
+// outputs of a block are contiguous, so the stores coalesce.
+constexpr int kRowsLdsBytes = 64 * 1024;
+
+// LDS of separable_rows_lds_kernel: weights, starts and node ranges, the
+// output block, the padded row tile.
+__host__ __device__ inline size_t rows_lds_bytes(int L, int G, int nr) {
+  return sizeof(double) * ((size_t)4 * L + (L + 2 * G + 1) / 2 + (size_t)nr * G) + sizeof(float) * (size_t)nr * (L + 1);
+}
+
+__global__ void __launch_bounds__(256) separable_rows_lds_kernel(const float* __restrict__ in,
+                                                                 double* __restrict__ out, int64_t rows, int L, int G,
+                                                                 int nr, AxisTable t) {
+  extern __shared__ double lds_d[];
+  double* wt = lds_d;                                      // [L][4]
+  int32_t* st = reinterpret_cast<int32_t*>(wt + 4 * L);   // [L]
+  int32_t* lo = st + L;                                    // [G]
+  int32_t* hi = lo + G;                                    // [G]
+  double* ob = lds_d + 4 * L + (L + 2 * G + 1) / 2;         // [nr][G] outputs of a block
+  float* tile = reinterpret_cast<float*>(ob + nr * G);     // [nr][L + 1]
+  for (int i = threadIdx.x; i < 4 * L; i += 256) wt[i] = t.w[i];
+  for (int i = threadIdx.x; i < L; i += 256) st[i] = t.start[i];
+  for (int i = threadIdx.x; i < G; i += 256) {
+    lo[i] = t.lo[i];
+    hi[i] = t.hi[i];
+  }
+  const int ld = L + 1;
+  for (int64_t r0 = (int64_t)blockIdx.x * nr; r0 < rows; r0 += (int64_t)gridDim.x * nr) {
+    const int n = (int)min((int64_t)nr, rows - r0);
+    __syncthreads();  // the tables are staged; the previous block is done with the tile
+    const float* src = in + r0 * L;
+    const int cnt = n * L;
+    // eight loads in flight per thread; element i of the block goes to row
+    // i / L of the padded tile, i.e. to tile[i + i / L]
+    for (int i0 = 0; i0 < cnt; i0 += 8 * 256) {
+      float v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int i = i0 + k * 256 + (int)threadIdx.x;
+        v[k] = i < cnt ? src[i] : 0.0f;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int i = i0 + k * 256 + (int)threadIdx.x;
+        if (i < cnt) tile[i + i / L] = v[k];
+      }
+    }
+    __syncthreads();
+    // rows fastest: a wavefront shares one or two nodes, so its table reads
+    // are broadcasts and its sample reads fall on distinct banks
+    const int outs = n * G;
+    for (int e = threadIdx.x; e < outs; e += 256) {
+      const int g = e / n;
+      const int row = e - g * n;
+      const float* x = tile + row * ld;
+      double acc = 0.0;
+      const int h = hi[g];
+#pragma unroll 4
+      for (int l = lo[g]; l <= h; ++l) acc += wt[l * 4 + (g - st[l])] * (double)x[l];
+      ob[row * G + g] = acc;
+    }
+    __syncthreads();
+    double* dst = out + r0 * G;
+    for (int e = threadIdx.x; e < outs; e += 256) dst[e] = ob[e];
+  }
+}
+
 hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, const DevResult* r, double* deriv,
                                     hipStream_t st) {
   hipLaunchKernelGGL(derivative_reduce_kernel, dim3(P), dim3(256), 0, st, partials, nblocks, P, r, deriv);
@@ -409,27 +478,15 @@ hipError_t launch_texels_to_storage(const float4* in, void* out, int storage, in
   return hipGetLastError();
 }
 
-hipError_t launch_recursive_gaussian(const double* in, double* out, int64_t ln, int64_t stride, int64_t nlines,
-                                     const RecCoeffs& c, hipStream_t st) {
-  hipLaunchKernelGGL(recursive_gaussian_kernel, dim3((unsigned)((nlines + 127) / 128)), dim3(128), 0, st, in, out,
-                     ln, stride, nlines, c);
-  return hipGetLastError();
-}
-
-hipError_t launch_float_to_double(const float* in, double* out, int64_t n, hipStream_t st) {
-  hipLaunchKernelGGL(float_to_double_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, out, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_double_to_float(const double* in, float* out, int64_t n, hipStream_t st) {
-  hipLaunchKernelGGL(double_to_float_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, out, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_gradient_component(const double* vol, int64_t n, double spacing, int comp, double* grad, int D,
-                                     hipStream_t st) {
-  hipLaunchKernelGGL(gradient_component_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vol, n, spacing,
-                     comp, grad, D);
+hipError_t launch_recursive_gaussian(const void* in, bool in_float, double* out, int64_t ln, int64_t stride,
+                                     int64_t nlines, const RecCoeffs& c, const RecSink& sink, hipStream_t st) {
+  const dim3 grid((unsigned)((nlines + 127) / 128));
+  if (in_float)
+    hipLaunchKernelGGL(recursive_gaussian_kernel<float>, grid, dim3(128), 0, st, static_cast<const float*>(in), out,
+                       ln, stride, nlines, c, sink);
+  else
+    hipLaunchKernelGGL(recursive_gaussian_kernel<double>, grid, dim3(128), 0, st, static_cast<const double*>(in),
+                       out, ln, stride, nlines, c, sink);
   return hipGetLastError();
 }
 
@@ -462,7 +519,13 @@ hipError_t launch_separable_contract(const float* in_f, const double* in_d, doub
   const int64_t total = outer * G * inner;
   if (total <= 0) return hipSuccess;
   const dim3 grid((unsigned)((total + 255) / 256));
-  if (in_f && inner == 1 && !scale && t.increasing) {
+  int nr = (int)std::min<int64_t>(32, outer);
+  while (nr > 8 && rows_lds_bytes(L, G, nr) > (size_t)kRowsLdsBytes) nr /= 2;
+  if (in_f && inner == 1 && !scale && rows_lds_bytes(L, G, nr) <= (size_t)kRowsLdsBytes) {
+    const int64_t blocks = std::min<int64_t>((outer + nr - 1) / nr, 2048);
+    hipLaunchKernelGGL(separable_rows_lds_kernel, dim3((unsigned)blocks), dim3(256), rows_lds_bytes(L, G, nr), st,
+                       in_f, out, outer, L, G, nr, t);
+  } else if (in_f && inner == 1 && !scale && t.increasing) {
     hipLaunchKernelGGL(separable_rows_kernel, dim3((unsigned)((outer + 255) / 256)), dim3(256), 0, st, in_f, out,
                        outer, L, G, t);
   } else if (in_f)

@@ -271,6 +271,14 @@ class MattesMutualInformationImageToImageMetricv4:
                                             mb.ctypes.data_as(ctypes.POINTER(ctypes.c_int16))))
         return fb, mb
 
+    def GetMovingTexels(self) -> np.ndarray:
+        """The device's FP32 moving texels, shape moving.shape + (4,): value
+        channel (the level's moving image) and physical gradient."""
+        shape = self._moving[0].shape
+        out = np.zeros(tuple(shape) + (4,), dtype=np.float32)
+        N.check(self._L.mmi_get_moving_texels(self._ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
     def GetHistogramParameters(self) -> dict:
         vals = [ctypes.c_double() for _ in range(6)]
         N.check(self._L.mmi_get_histogram_parameters(self._ctx, *[ctypes.byref(v) for v in vals]))

@@ -251,6 +251,37 @@ def test_device_smoothing_matches_oracle(oracle_mod):
     _compare(oracle_mod, m, fixed, moving, geom, tf, gradient=oracle_mod.gradient_image(moving, geom))
 
 
+@pytest.mark.parametrize("shape,spacing,sigma", [
+    ((40, 36, 30), (0.9, 1.2, 1.6), (1.2, 0.8, 1.5)),
+    ((70, 33, 21), (1.0, 1.0, 1.0), (2.0, 2.0, 2.0)),  # lines of 70 along x: tiles 32 + 32 + 6
+    ((37, 50), (1.1, 0.7), (1.5, 0.9)),  # 2-D: the last pass of both filters runs along x
+])
+def test_device_filters_bitwise(oracle_mod, shape, spacing, sigma):
+    """Device per-level smoothing and gradient filter against the oracle's, bit
+    for bit, through the FP32 texels: the value channel is the smoothed moving
+    image, the gradient channels the filter's FP64 output rounded to FP32."""
+    fixed, moving, _ = blob_problem(shape, 47)
+    geom = ImageGeometry(size=shape, spacing=spacing, origin=(1.0,) * len(shape))
+    tf = affine_for(geom, affine_params(len(shape), np.random.default_rng(48)))
+    for sig in (None, sigma):
+        m = Metric()
+        m.SetFixedImage(fixed, geom)
+        m.SetMovingImage(moving, geom)
+        if sig is not None:
+            m.SetSmoothingSigmas(sig, sig)
+        m.SetMovingTransform(tf)
+        m.Initialize()
+        t = m.GetMovingTexels()
+        ms = moving if sig is None else oracle_mod.smooth(moving, geom, sig)
+        assert ms.dtype == np.float32
+        grad = oracle_mod.gradient_image(ms, geom)
+        assert np.array_equal(t[..., 0].view(np.uint32), ms.view(np.uint32))
+        g32 = grad.astype(np.float32)
+        for k in range(len(shape)):
+            assert np.array_equal(t[..., 1 + k].view(np.uint32), g32[..., k].view(np.uint32)), k
+        m.close()
+
+
 def test_user_gradient_with_smoothing(oracle_mod):
     """A gradient image given by the caller stays as given while the moving
     image is smoothed for a level; the fast derivative pass reads the moving
