@@ -37,12 +37,12 @@ HBM_PEAK_GBS = 8000.0
 # images per sample: 4 + 4 + 12 bytes in FP32 storage, half in BF16/FP16.
 BYTES_PER_SAMPLE_3D = {"f32": 20, "bf16": 10, "f16": 10}
 # HBM bytes per evaluation (FETCH_SIZE + WRITE_SIZE summed over the evaluation's
-# kernels, scaled by the counters' reading for a copy of known size), from the
+# kernels, scaled by the counters' reading for a kernel of known traffic), from the
 # rocprofv3 --pmc runs summarised in profiles/mattes_affine_256cube_fast_path.md
 # and profiles/mattes_bspline_192cube_mesh16.md.  None where not profiled.
 TRAFFIC_BYTES_PER_EVAL = {
-    ("mattes_mi_dense_value_and_derivative_affine_256cube", "f32"): 588.2e6,
-    ("mattes_mi_dense_value_and_derivative_bspline_192cube_mesh16", "f32"): 751.4e6,
+    ("mattes_mi_dense_value_and_derivative_affine_256cube", "f32"): 587.1e6,
+    ("mattes_mi_dense_value_and_derivative_bspline_192cube_mesh16", "f32"): 541.8e6,
 }
 
 
@@ -62,7 +62,7 @@ def parse():
     ap.add_argument("--serving", action="store_true", help="registration serving workload")
     ap.add_argument("--requests", type=int, default=32, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=40.0, help="serving: mean arrivals per second")
-    ap.add_argument("--inflight", type=int, default=32, help="serving: registrations in flight at most")
+    ap.add_argument("--inflight", type=int, default=2, help="serving: registrations in flight at most")
     ap.add_argument("--serving-requests", type=int, default=16,
                     help="requests of the serving summary added to the metric line (0: none)")
     return ap.parse_args()
@@ -107,15 +107,35 @@ def synthesize_requests(args):
     return made
 
 
-def serving(args, quiet: bool = False, made=None):
+def serving_summary_child(args, timeout_s: float = 300.0) -> dict:
+    """The serving summary of the metric line, from `bench.py --serving` run as
+    a child process with a time limit (started before this process touches the
+    GPU), so a serving problem cannot take the metric line with it."""
+    import subprocess
+
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--serving", "--requests", str(args.serving_requests),
+           "--rate", str(args.rate), "--inflight", str(args.inflight), "--seed", str(args.seed)]
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, timeout=timeout_s, text=True)
+    except subprocess.TimeoutExpired:
+        return {"error": "serving run exceeded %.0f s" % timeout_s}
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": "serving run failed (exit %d)" % r.returncode}
+    d = json.loads(lines[-1])
+    c = d["config"]
+    return dict(d["serving"], requests=c["requests"], kinds=c["kinds"], arrival_rate_per_s=c["arrival_rate_per_s"],
+                max_in_flight=c["max_in_flight"])
+
+
+def serving(args):
     """Registrations/s and latency of the continuous-batching server on a
-    Poisson stream, against the same requests one at a time.  Prints the JSON
-    line, or (quiet) returns its serving summary."""
+    Poisson stream, against the same requests one at a time; prints the JSON
+    line."""
     from itk_amd.geometry import ImageGeometry
     from itk_amd.serving import RegistrationRequest, RegistrationServer, latency_summary, run_sequential
 
-    if made is None:
-        made = synthesize_requests(args)
+    made = synthesize_requests(args)
     from itk_amd import synth
 
     gaps = -np.log(1.0 - synth.uniform(args.seed, 29, args.requests)) / args.rate
@@ -177,8 +197,6 @@ def serving(args, quiet: bool = False, made=None):
             "results_identical_alone_and_batched": bool(same),
         },
     }
-    if quiet:
-        return dict(line["serving"], requests=args.requests, kinds=kinds, arrival_rate_per_s=args.rate)
     print(json.dumps(line), flush=True)
     return 0
 
@@ -257,11 +275,7 @@ def main():
         return 2
     if args.serving:
         return serving(args)
-    made = None
-    if args.serving_requests > 0:
-        sargs = argparse.Namespace(**vars(args))
-        sargs.requests = args.serving_requests
-        made = synthesize_requests(sargs)
+    served = serving_summary_child(args) if args.serving_requests > 0 else None
     from itk_amd.metric import MattesMutualInformationImageToImageMetricv4
 
     wl = Workload(args)
@@ -325,10 +339,10 @@ def main():
     m.close()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
-    if made is not None:
+    if served is not None:
         # the metric's second half: registrations of mixed sizes served by the
         # continuous-batching server (a short run of the --serving workload)
-        line["serving"] = serving(sargs, quiet=True, made=made)
+        line["serving"] = served
     print(json.dumps(line), flush=True)
     return 0
 

@@ -561,6 +561,10 @@ bool setup_fast(mmi_context* ctx, FastArgs* F) {
   const int D = ctx->fixed_g.dim;
   const int B = ctx->bins;
   if (num_pixels(ctx->virtual_g) >= (1ll << 31)) return false;
+  // the fast kernels address the images with 32-bit byte offsets
+  if (num_pixels(ctx->moving_g) * (int64_t)sizeof(float4) >= (1ll << 32) ||
+      num_pixels(ctx->fixed_g) * (int64_t)sizeof(float) >= (1ll << 32))
+    return false;
   if (ctx->flags & 2u) return false;  // exact path requested
   const bool bs = ctx->mtf.kind == MMI_TF_BSPLINE;
   if ((ctx->mtf.kind != MMI_TF_AFFINE && !bs) || ctx->ftf.kind != MMI_TF_IDENTITY) return false;
@@ -1630,10 +1634,12 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
     int32_t* start = reinterpret_cast<int32_t*>(hs + o_start);
     int bins_max = 0;
     int total = 0;
+    size_t lds1 = 0;
     for (int k = 0; k < nf; ++k) {
       start[k] = total;
       total += ctxs[fused[k]]->grid;  // multiples of 8
       bins_max = std::max(bins_max, ctxs[fused[k]]->bins);
+      lds1 = std::max(lds1, pass1_lds_bytes(ctxs[fused[k]]->bins));
     }
     start[nf] = total;
     const int32_t lds = finalize_uses_lds(bins_max) ? 1 : 0;
@@ -1670,6 +1676,7 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
     L.total_blocks = total;
     L.out_stride = stride;
     L.bins_max = bins_max;
+    L.lds1 = (int32_t)lds1;
     L.start = reinterpret_cast<const int32_t*>(ds + o_start);
     L.fa = reinterpret_cast<const FastArgs*>(ds + o_fa);
     L.ea = reinterpret_cast<const EvalArgs*>(ds + o_ea);
@@ -1679,7 +1686,8 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
     L.out = b->d_out.as<double>();
     MMI_HIP(hipEventRecord(b->ev0, b->stream));
     if (b->use_graph) {
-      const std::vector<int64_t> sig = {nf, dim, storage, L.want_derivative, total, stride, bins_max, (int64_t)bytes,
+      const std::vector<int64_t> sig = {nf, dim, storage, L.want_derivative, total, stride, bins_max, L.lds1,
+                                        (int64_t)bytes,
                                         (int64_t)(intptr_t)b->d_stage.p, (int64_t)(intptr_t)b->d_out.p,
                                         (int64_t)(intptr_t)b->h_stage, (int64_t)(intptr_t)b->h_out};
       if (!b->exec || sig != b->sig) {

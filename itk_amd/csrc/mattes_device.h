@@ -190,35 +190,49 @@ __device__ __forceinline__ double lerp_exact(double a, double b, double t) { ret
 // Texels hold (moving value, dI/dx, dI/dy, dI/dz): float4 for FP32 storage,
 // four 16-bit halves packed in a uint2 for BF16 and FP16 storage (x = value |
 // dI/dx << 16, y = dI/dy | dI/dz << 16).
+// Element i of a buffer below 4 GiB through a 32-bit byte offset from the
+// wavefront-uniform base: the load takes the base in scalar registers and the
+// offset in one vector register, with no 64-bit address arithmetic per load
+// (the fast path's images; the host checks their size).
+template <typename T>
+__device__ __forceinline__ T at32(const T* p, uint32_t i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + i * (uint32_t)sizeof(T));
+}
+
 struct PxF32 {
   using pixel = float;
   using texel = float4;
   static __device__ __forceinline__ float ld(const pixel* p, int32_t i) { return p[i]; }
   static __device__ __forceinline__ float4 ldt(const texel* t, int32_t i) { return t[i]; }
+  static __device__ __forceinline__ float ld32(const pixel* p, uint32_t i) { return at32(p, i); }
+  static __device__ __forceinline__ float4 ldt32(const texel* t, uint32_t i) { return at32(t, i); }
 };
 
 struct PxBF16 {
   using pixel = uint16_t;
   using texel = uint2;
-  static __device__ __forceinline__ float ld(const pixel* p, int32_t i) {
-    return __uint_as_float((uint32_t)p[i] << 16);
-  }
-  static __device__ __forceinline__ float4 ldt(const texel* t, int32_t i) {
-    const uint2 u = t[i];
+  static __device__ __forceinline__ float cv(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+  static __device__ __forceinline__ float4 cvt(uint2 u) {
     return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xFFFF0000u), __uint_as_float(u.y << 16),
                        __uint_as_float(u.y & 0xFFFF0000u));
   }
+  static __device__ __forceinline__ float ld(const pixel* p, int32_t i) { return cv(p[i]); }
+  static __device__ __forceinline__ float4 ldt(const texel* t, int32_t i) { return cvt(t[i]); }
+  static __device__ __forceinline__ float ld32(const pixel* p, uint32_t i) { return cv(at32(p, i)); }
+  static __device__ __forceinline__ float4 ldt32(const texel* t, uint32_t i) { return cvt(at32(t, i)); }
 };
 
 struct PxF16 {
   using pixel = uint16_t;
   using texel = uint2;
   static __device__ __forceinline__ float h2f(uint32_t h) { return (float)__builtin_bit_cast(_Float16, (uint16_t)h); }
-  static __device__ __forceinline__ float ld(const pixel* p, int32_t i) { return h2f(p[i]); }
-  static __device__ __forceinline__ float4 ldt(const texel* t, int32_t i) {
-    const uint2 u = t[i];
+  static __device__ __forceinline__ float4 cvt(uint2 u) {
     return make_float4(h2f(u.x & 0xFFFFu), h2f(u.x >> 16), h2f(u.y & 0xFFFFu), h2f(u.y >> 16));
   }
+  static __device__ __forceinline__ float ld(const pixel* p, int32_t i) { return h2f(p[i]); }
+  static __device__ __forceinline__ float4 ldt(const texel* t, int32_t i) { return cvt(t[i]); }
+  static __device__ __forceinline__ float ld32(const pixel* p, uint32_t i) { return h2f(at32(p, i)); }
+  static __device__ __forceinline__ float4 ldt32(const texel* t, uint32_t i) { return cvt(at32(t, i)); }
 };
 
 // Exact FP64 multilinear value of a float image (the reference's RealType is
@@ -245,22 +259,24 @@ __device__ __forceinline__ double interp_exact(const typename P::pixel* __restri
 
 // Gradient at a moving cell: only the gradient components of the texels are
 // interpolated here (FP32), with the same cell as the value.
-template <int D, typename P>
+// U32: 32-bit byte-offset loads (fast path; texel buffer below 4 GiB).
+template <int D, typename P, bool U32 = false>
 __device__ __forceinline__ float3 interp_gradient3(const typename P::texel* __restrict__ tex, const Cell& c) {
   const float t0 = (float)c.d[0], t1 = (float)c.d[1], t2 = (float)c.d[2];
-  const float4 a000 = P::ldt(tex, c.base);
-  const float4 a100 = P::ldt(tex, c.base + c.ox);
-  const float4 a010 = P::ldt(tex, c.base + c.oy);
-  const float4 a110 = P::ldt(tex, c.base + c.oy + c.ox);
+  auto L = [&](int32_t i) { return U32 ? P::ldt32(tex, (uint32_t)i) : P::ldt(tex, i); };
+  const float4 a000 = L(c.base);
+  const float4 a100 = L(c.base + c.ox);
+  const float4 a010 = L(c.base + c.oy);
+  const float4 a110 = L(c.base + c.oy + c.ox);
   const float gx0 = fmaf(a100.y - a000.y, t0, a000.y), gx1 = fmaf(a110.y - a010.y, t0, a010.y);
   const float gy0 = fmaf(a100.z - a000.z, t0, a000.z), gy1 = fmaf(a110.z - a010.z, t0, a010.z);
   const float gz0 = fmaf(a100.w - a000.w, t0, a000.w), gz1 = fmaf(a110.w - a010.w, t0, a010.w);
   float3 r = make_float3(fmaf(gx1 - gx0, t1, gx0), fmaf(gy1 - gy0, t1, gy0), fmaf(gz1 - gz0, t1, gz0));
   if (D == 3) {
-    const float4 a001 = P::ldt(tex, c.base + c.oz);
-    const float4 a101 = P::ldt(tex, c.base + c.oz + c.ox);
-    const float4 a011 = P::ldt(tex, c.base + c.oz + c.oy);
-    const float4 a111 = P::ldt(tex, c.base + c.oz + c.oy + c.ox);
+    const float4 a001 = L(c.base + c.oz);
+    const float4 a101 = L(c.base + c.oz + c.ox);
+    const float4 a011 = L(c.base + c.oz + c.oy);
+    const float4 a111 = L(c.base + c.oz + c.oy + c.ox);
     const float hx0 = fmaf(a101.y - a001.y, t0, a001.y), hx1 = fmaf(a111.y - a011.y, t0, a011.y);
     const float hy0 = fmaf(a101.z - a001.z, t0, a001.z), hy1 = fmaf(a111.z - a011.z, t0, a011.z);
     const float hz0 = fmaf(a101.w - a001.w, t0, a001.w), hz1 = fmaf(a111.w - a011.w, t0, a011.w);

@@ -142,8 +142,11 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
     out = out | (ci[k] < lo - f.e_ci) | (ci[k] >= hi + f.e_ci);
     uncertain = uncertain | (ci[k] < lo + f.e_ci) | (ci[k] >= hi - f.e_ci);
     // moving cell (the reference's branch rules as duplicated corners)
+    // floor, not ITK's Math::Floor: they differ only within an ulp below an
+    // integer, where either cell holds ci and the interpolant is continuous
+    // (its slope bound is part of ev)
     const double cc = fmin(fmax(ci[k], lo), hi);
-    int32_t fl = itk_floor(cc);
+    int32_t fl = (int32_t)floor(cc);
     fl = fl < f.mstart[k] ? f.mstart[k] : (fl > f.mend[k] ? f.mend[k] : fl);
     bb[k] = fl;
     cell.d[k] = ci[k] - (double)fl;
@@ -155,15 +158,17 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
   cell.oz = (D == 3 && act[2]) ? f.msz : 0;
   const float t0 = (float)cell.d[0], t1 = (float)cell.d[1], t2 = (float)cell.d[2];
   const typename P::pixel* img = static_cast<const typename P::pixel*>(f.moving_px);
-  const float v000 = P::ld(img, cell.base), v100 = P::ld(img, cell.base + cell.ox);
-  const float v010 = P::ld(img, cell.base + cell.oy), v110 = P::ld(img, cell.base + cell.oy + cell.ox);
+  // every cell offset is non-negative and below the buffer size
+  const uint32_t cb = (uint32_t)cell.base, cx = (uint32_t)cell.ox, cy = (uint32_t)cell.oy, cz = (uint32_t)cell.oz;
+  const float v000 = P::ld32(img, cb), v100 = P::ld32(img, cb + cx);
+  const float v010 = P::ld32(img, cb + cy), v110 = P::ld32(img, cb + cy + cx);
   const float vx00 = fmaf(v100 - v000, t0, v000);
   const float vx10 = fmaf(v110 - v010, t0, v010);
   float v = fmaf(vx10 - vx00, t1, vx00);
   if (D == 3) {
-    const float v001 = P::ld(img, cell.base + cell.oz), v101 = P::ld(img, cell.base + cell.oz + cell.ox);
-    const float v011 = P::ld(img, cell.base + cell.oz + cell.oy);
-    const float v111 = P::ld(img, cell.base + cell.oz + cell.oy + cell.ox);
+    const float v001 = P::ld32(img, cb + cz), v101 = P::ld32(img, cb + cz + cx);
+    const float v011 = P::ld32(img, cb + cz + cy);
+    const float v111 = P::ld32(img, cb + cz + cy + cx);
     const float vx01 = fmaf(v101 - v001, t0, v001);
     const float vx11 = fmaf(v111 - v011, t0, v011);
     const float vxx1 = fmaf(vx11 - vx01, t1, vx01);
@@ -183,17 +188,18 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
     const int4 ty = f.ftab[1][idx[1] - f.vstart[1]];
     const int4 tz = D == 3 ? f.ftab[2][idx[2] - f.vstart[2]] : make_int4(0, 0, 0, 1);
     out = out | ((tx.w & ty.w & tz.w) == 0);
-    const int32_t fb = tx.x + ty.x + tz.x;
+    const uint32_t fb = (uint32_t)(tx.x + ty.x + tz.x);
+    const uint32_t fx = (uint32_t)tx.y, fy = (uint32_t)ty.y, fz = (uint32_t)tz.y;
     const float u0 = __int_as_float(tx.z), u1 = __int_as_float(ty.z);
-    const float f000 = P::ld(fimg, fb), f100 = P::ld(fimg, fb + tx.y);
-    const float f010 = P::ld(fimg, fb + ty.y), f110 = P::ld(fimg, fb + ty.y + tx.y);
+    const float f000 = P::ld32(fimg, fb), f100 = P::ld32(fimg, fb + fx);
+    const float f010 = P::ld32(fimg, fb + fy), f110 = P::ld32(fimg, fb + fy + fx);
     const float fx00 = fmaf(f100 - f000, u0, f000);
     const float fx10 = fmaf(f110 - f010, u0, f010);
     fv = fmaf(fx10 - fx00, u1, fx00);
     if (D == 3) {
       const float u2 = __int_as_float(tz.z);
-      const float f001 = P::ld(fimg, fb + tz.y), f101 = P::ld(fimg, fb + tz.y + tx.y);
-      const float f011 = P::ld(fimg, fb + tz.y + ty.y), f111 = P::ld(fimg, fb + tz.y + ty.y + tx.y);
+      const float f001 = P::ld32(fimg, fb + fz), f101 = P::ld32(fimg, fb + fz + fx);
+      const float f011 = P::ld32(fimg, fb + fz + fy), f111 = P::ld32(fimg, fb + fz + fy + fx);
       const float fx01 = fmaf(f101 - f001, u0, f001);
       const float fx11 = fmaf(f111 - f011, u0, f011);
       const float fxx1 = fmaf(fx11 - fx01, u1, fx01);
@@ -203,7 +209,7 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
     // the pixel the fixed sample sits on (verified on the host)
     const int32_t fl =
         (idx[0] + f.foff[0]) + f.fsy * (idx[1] + f.foff[1]) + (D == 3 ? f.fsz * (idx[2] + f.foff[2]) : 0);
-    fv = P::ld(fimg, fl);
+    fv = P::ld32(fimg, (uint32_t)fl);
   }
   const float fterm = fmaf(fv, f.inv_fbs, -f.fnm);
   fbin = fast_bin(fterm, f.bt_f, f.bins, &uncertain);
@@ -215,11 +221,15 @@ __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, uint32_t lb, 
   extern __shared__ unsigned long long lds_u64[];
   const int B = f.bins;
   const int BB = B * B;
-  unsigned long long* hist = lds_u64;
-  unsigned int* fmarg = reinterpret_cast<unsigned int*>(lds_u64 + BB);
-  unsigned int* counters = fmarg + B;  // [0] valid count, [1] exception count
-  for (int i = threadIdx.x; i < BB; i += blockDim.x) hist[i] = 0ull;
-  for (int i = threadIdx.x; i < B + 2; i += blockDim.x) fmarg[i] = 0u;
+  const int copies = hist_copies(B);
+  unsigned long long* hist0 = lds_u64;                                      // [copies][B][B]
+  unsigned int* fmarg0 = reinterpret_cast<unsigned int*>(lds_u64 + copies * BB);  // [copies][B]
+  unsigned int* counters = fmarg0 + copies * B;  // [0] valid count, [1] exception count
+  const int mine = copies == 2 ? (int)(threadIdx.x & 1u) : 0;
+  unsigned long long* hist = hist0 + mine * BB;
+  unsigned int* fmarg = fmarg0 + mine * B;
+  for (int i = threadIdx.x; i < copies * BB; i += blockDim.x) hist0[i] = 0ull;
+  for (int i = threadIdx.x; i < copies * B + 2; i += blockDim.x) fmarg0[i] = 0u;
   __syncthreads();
   unsigned int nvalid = 0;
   uint32_t s, end, step;
@@ -246,10 +256,11 @@ __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, uint32_t lb, 
     atomicAdd(&fmarg[fbin], 1u);
     unsigned int q[4];
     parzen_fixed_fast(arg0, q);
+    // mbin is clamped to [2, B-3], so the 4 bins stay in the row; a zero
+    // weight (clamped windows only) adds nothing
     unsigned long long* row = hist + fbin * B + (mbin - 1);
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (q[k]) atomicAdd(&row[k], (unsigned long long)q[k]);
+    for (int k = 0; k < 4; ++k) atomicAdd(&row[k], (unsigned long long)q[k]);
   }
   unsigned int v = nvalid;
 #pragma unroll
@@ -257,11 +268,11 @@ __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, uint32_t lb, 
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(&counters[0], v);
   __syncthreads();
   for (int i = threadIdx.x; i < BB; i += blockDim.x) {
-    const unsigned long long h = hist[i];
+    const unsigned long long h = hist0[i] + (copies == 2 ? hist0[BB + i] : 0ull);
     if (h) atomicAdd(&f.hist[i], h);
   }
   for (int i = threadIdx.x; i < B; i += blockDim.x) {
-    const unsigned int m = fmarg[i];
+    const unsigned int m = fmarg0[i] + (copies == 2 ? fmarg0[B + i] : 0u);
     if (m) atomicAdd(&f.fmarg[i], (unsigned long long)m);
   }
   if (threadIdx.x == 0) {
@@ -353,7 +364,7 @@ __device__ __forceinline__ void pass2_fast_body(const FastArgs& f, uint32_t lb, 
       if (c == 1 && bsin) {
         const float w = parzen_deriv_weight_fast(arg0, pr + fbin * B + (mbin - 1));
         if (w != 0.0f) {
-          const float3 g = interp_gradient3<D, P>(static_cast<const typename P::texel*>(f.texels), cell);
+          const float3 g = interp_gradient3<D, P, true>(static_cast<const typename P::texel*>(f.texels), cell);
           gw[0] = (float)((double)w * (double)g.x);
           gw[1] = (float)((double)w * (double)g.y);
           gw[2] = (float)((double)w * (double)g.z);
@@ -365,7 +376,7 @@ __device__ __forceinline__ void pass2_fast_body(const FastArgs& f, uint32_t lb, 
       if (c != 1) continue;
       const float w = parzen_deriv_weight_fast(arg0, pr + fbin * B + (mbin - 1));
       if (w == 0.0f) continue;
-      add_moments<D>(acc, w, interp_gradient3<D, P>(static_cast<const typename P::texel*>(f.texels), cell), idx);
+      add_moments<D>(acc, w, interp_gradient3<D, P, true>(static_cast<const typename P::texel*>(f.texels), cell), idx);
     }
   }
   if constexpr (MK == 1) block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)lb * PA);
@@ -645,7 +656,7 @@ __global__ void batch_gather(const BatchOut* __restrict__ items, double* __restr
   } while (0)
 
 hipError_t launch_fast_pass1(const FastArgs& f, int dim, int grid, hipStream_t st) {
-  const size_t lds = (size_t)f.bins * f.bins * 8 + (size_t)(f.bins + 2) * 4;
+  const size_t lds = pass1_lds_bytes(f.bins);
   MMI_DISPATCH_FAST_MK(pass1_fast_kernel, f.mk, f.storage, dim, grid, lds, st, f);
   return hipGetLastError();
 }
@@ -678,7 +689,7 @@ hipError_t launch_fast_derivative(const double* partials, int rows, const FastMa
 hipError_t launch_fast_batch(const BatchLaunch& b, hipStream_t st) {
   const int B = b.bins_max;
   hipLaunchKernelGGL(batch_clear, dim3(b.n), dim3(256), 0, st, b.fa);
-  const size_t lds1 = (size_t)B * B * 8 + (size_t)(B + 2) * 4;
+  const size_t lds1 = (size_t)b.lds1;
   MMI_DISPATCH_FAST(pass1_fast_batch, b.storage, b.dim, b.total_blocks, lds1, st, b.fa, b.start, b.n);
   MMI_DISPATCH_FAST(exceptions_pass1_batch, b.storage, b.dim, b.total_blocks, (size_t)(B + 1) * 4, st, b.ea, b.fa,
                     b.start, b.n);

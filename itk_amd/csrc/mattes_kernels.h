@@ -181,7 +181,7 @@ struct BatchLaunch {
   int32_t total_blocks;  // start[n]
   int32_t out_stride;
   int32_t bins_max;
-  int32_t pad_;
+  int32_t lds1;  // pass-1 LDS bytes: the largest pass1_lds_bytes of the members
   const int32_t* start;  // [n + 1] first workgroup of each evaluation
   const FastArgs* fa;    // [n]
   const EvalArgs* ea;    // [n]
@@ -190,6 +190,18 @@ struct BatchLaunch {
   const BatchOut* items;
   double* out;  // [n][out_stride]: value, n_valid, status, derivative
 };
+
+// Copies of the pass-1 joint histogram (and fixed marginal) in LDS: two when
+// they fit in 64 KiB, updated by even and odd lanes.  Neighbouring samples
+// (adjacent lanes) usually fall in the same bins, so this halves the
+// same-address serialisation of the 64-bit LDS atomics.
+__host__ __device__ inline int hist_copies(int bins) {
+  return 2 * (size_t)bins * bins * 8 + (2 * (size_t)bins + 2) * 4 <= 64 * 1024 ? 2 : 1;
+}
+__host__ __device__ inline size_t pass1_lds_bytes(int bins) {
+  const size_t c = (size_t)hist_copies(bins);
+  return c * bins * bins * 8 + (c * bins + 2) * 4;
+}
 
 // The finalize workgroup stages the PDF and marginals in LDS when they fit.
 inline bool finalize_uses_lds(int bins) {

@@ -222,10 +222,13 @@ class _Lane:
 
 
 class RegistrationServer:
-    def __init__(self, device: int = 0, hbm_budget_bytes: int = 64 << 30, max_inflight: int = 32,
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 64 << 30, max_inflight: int = 2,
                  max_wait_s: float = 0.0005, graphs: bool = True, storage: str = "f32", lanes: int = 2,
-                 chunk_steps: int = 5):
+                 chunk_steps: int = 5, wait_limit_s: float = 600.0):
         self.device = device
+        # a registration whose evaluation is not served within this time fails
+        # (reported in its result) instead of waiting forever
+        self.wait_limit = float(wait_limit_s)
         self.budget = int(hbm_budget_bytes)
         self.max_inflight = int(max_inflight)
         self.max_wait = float(max_wait_s)
@@ -253,7 +256,8 @@ class RegistrationServer:
             lane.pending.append((job, want_derivative, slot))
             if len(lane.pending) >= len(lane.running):
                 lane.cv.notify()
-        slot["event"].wait()
+        if not slot["event"].wait(self.wait_limit):
+            raise RuntimeError("batched evaluation not served within %.0f s" % self.wait_limit)
         job.evaluations += 1
         if slot["err"] is not None:
             raise slot["err"]
@@ -268,7 +272,8 @@ class RegistrationServer:
                 lane.first = time.perf_counter()
             lane.optimizing.append(it)
             lane.cv.notify()
-        it[3].wait()
+        if not it[3].wait(self.wait_limit):
+            raise RuntimeError("batched optimizer level not served within %.0f s" % self.wait_limit)
         job.evaluations += int(it[2].iteration)  # one evaluation per iteration
         return it[2]
 
