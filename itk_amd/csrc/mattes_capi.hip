@@ -20,6 +20,7 @@ using namespace mmi;
 // Uncertain samples one fast-path workgroup may defer to the exact path (one
 // per thread of the sorting workgroup in exceptions_pass2_kernel).
 constexpr int kExceptionCapacity = 256;
+static_assert(2 * 4 * kExceptionCapacity <= kExcSlots, "exception hash table too small");
 
 namespace {
 
@@ -274,9 +275,12 @@ struct mmi_context {
   DevTransform dftf{}, dmtf{};
   DevBuf coeffs;
 
-  DevBuf hist, fmarg, nvalid, pdf, fm, mm, pratio, pratio_f, rowsum, result, partials, deriv, sample_fbin, sample_mbin,
-      scratch;
-  DevBuf exc_count, exc_list, exc_overflow;  // fast-path exception lists
+  DevBuf hist, fmarg, nvalid, pdf, fm, mm, pratio, pratio_f, rowsum, partials, sample_fbin, sample_mbin, scratch;
+  // the evaluation's outputs in one block (one copy to the host): the result,
+  // the fast path's exception-overflow flag, the derivative (dev_result,
+  // dev_overflow, dev_deriv)
+  DevBuf out;
+  DevBuf exc_count, exc_list;  // fast-path exception lists
   DevBuf ftab;                               // fast path: per-axis fixed cells (coarse levels)
   // grow-only work space of the preparation filters (no device frees, which
   // would synchronise the whole device, while other contexts run)
@@ -303,6 +307,18 @@ struct mmi_context {
 };
 
 namespace {
+
+// The output block: DevResult, the exception-overflow flag (16 bytes), then
+// P derivative values.
+static_assert(sizeof(DevResult) % 8 == 0, "derivative alignment");
+size_t out_bytes(int64_t P) { return sizeof(DevResult) + 16 + sizeof(double) * (size_t)P; }
+DevResult* dev_result(const mmi_context* ctx) { return ctx->out.as<DevResult>(); }
+uint32_t* dev_overflow(const mmi_context* ctx) {
+  return reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->out.p) + sizeof(DevResult));
+}
+double* dev_deriv(const mmi_context* ctx) {
+  return reinterpret_cast<double*>(static_cast<char*>(ctx->out.p) + sizeof(DevResult) + 16);
+}
 
 int ensure_host_pinned(void** p, size_t* have, size_t need) {
   if (*p && *have >= need) return MMI_OK;
@@ -896,9 +912,9 @@ int mmi_context_destroy(mmi_context* ctx) {
   DevBuf* bufs[] = {&ctx->fixed_px, &ctx->moving_px, &ctx->texels, &ctx->fixed_lp, &ctx->moving_lp, &ctx->texels_lp,
                     &ctx->fixed_work, &ctx->moving_work,
                     &ctx->coeffs, &ctx->hist, &ctx->fmarg,
-                    &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->result,
-                    &ctx->partials, &ctx->deriv, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
-                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_overflow, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->ftab, &ctx->work_a,
+                    &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->out,
+                    &ctx->partials, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
+                    &ctx->exc_count, &ctx->exc_list, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->ftab, &ctx->work_a,
                     &ctx->work_b, &ctx->work_grad};
   for (DevBuf* b : bufs) b->release();
   for (int j = 0; j < 3; ++j) {
@@ -1165,7 +1181,7 @@ int mmi_initialize(mmi_context* ctx) {
   MMI_HIP(ctx->fm.ensure(sizeof(double) * B));
   MMI_HIP(ctx->mm.ensure(sizeof(double) * B));
   MMI_HIP(ctx->rowsum.ensure(sizeof(double) * B));
-  MMI_HIP(ctx->result.ensure(sizeof(DevResult)));
+  MMI_HIP(ctx->out.ensure(out_bytes((int64_t)ctx->mtf.params.size())));
   const int64_t n = num_pixels(ctx->virtual_g);
   // a multiple of 8: the fast kernels split the samples into one share per XCD
   const int64_t want = std::min<int64_t>((n + 255) / 256, (int64_t)ctx->cu_count * ctx->blocks_per_cu);
@@ -1225,10 +1241,10 @@ int bspline_contract(mmi_context* ctx, const DevGeom& virt, const AxisTable* tab
     MMI_HIP(ctx->bs_t2.ensure(sizeof(double) * D * Nz * Gy * Gx));
     MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t1.as<double>(), ctx->bs_t2.as<double>(), D * Nz, (int)Ny, Gx,
                                       Gy, tabs[1], nullptr, st));
-    MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t2.as<double>(), ctx->deriv.as<double>(), D, (int)Nz,
+    MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t2.as<double>(), dev_deriv(ctx), D, (int)Nz,
                                       (int64_t)Gy * Gx, Gz, tabs[2], result, st));
   } else {
-    MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t1.as<double>(), ctx->deriv.as<double>(), D, (int)Ny, Gx, Gy,
+    MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t1.as<double>(), dev_deriv(ctx), D, (int)Ny, Gx, Gy,
                                       tabs[1], result, st));
   }
   return MMI_OK;
@@ -1248,6 +1264,7 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
   const int B = ctx->bins;
   const int64_t n = num_pixels(ctx->virtual_g);
   const int64_t P = (int64_t)ctx->mtf.params.size();
+  MMI_HIP(ctx->out.ensure(out_bytes(P)));
   std::memset(&a, 0, sizeof(a));
   a.virt = to_dev(ctx->virtual_g);
   a.fixed = to_dev(ctx->fixed_g);
@@ -1286,7 +1303,6 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
   if (fast) {
     MMI_HIP(ctx->exc_count.ensure(sizeof(uint32_t) * ctx->grid));
     MMI_HIP(ctx->exc_list.ensure(sizeof(uint32_t) * ctx->grid * kExceptionCapacity));
-    MMI_HIP(ctx->exc_overflow.ensure(sizeof(uint32_t)));
     fa.xcd_chunks = ctx->xcd_chunks;
     fa.exc_cap = kExceptionCapacity;
     fa.fixed_px = a.fixed_px;
@@ -1301,10 +1317,9 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
     fa.pratio_f = a.pratio_f;
     fa.exc_count = ctx->exc_count.as<uint32_t>();
     fa.exc_list = ctx->exc_list.as<uint32_t>();
-    fa.exc_overflow = ctx->exc_overflow.as<uint32_t>();
+    fa.exc_overflow = dev_overflow(ctx);
   }
   if (fast && want_derivative) {
-    MMI_HIP(ctx->deriv.ensure(sizeof(double) * P));
     MMI_HIP(ctx->partials.ensure(sizeof(double) * P * 2 * ctx->grid));
     fa.partials = ctx->partials.as<double>();
   }
@@ -1326,7 +1341,7 @@ FinalizeArgs finalize_args(mmi_context* ctx) {
   f.pratio = ctx->pratio.as<double>();
   f.pratio_f = ctx->pratio_f.as<float>();
   f.rowsum = ctx->rowsum.as<double>();
-  f.result = ctx->result.as<DevResult>();
+  f.result = dev_result(ctx);
   f.moving_bin_size = ctx->moving_bin_size;
   return f;
 }
@@ -1358,10 +1373,9 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
   const int64_t P = (int64_t)ctx->mtf.params.size();
   hipStream_t st = ctx->stream;
 retry:
-  MMI_HIP(hipMemsetAsync(ctx->hist.p, 0, sizeof(unsigned long long) * B * B, st));
-  MMI_HIP(hipMemsetAsync(ctx->fmarg.p, 0, sizeof(unsigned long long) * B, st));
-  MMI_HIP(hipMemsetAsync(ctx->nvalid.p, 0, sizeof(unsigned long long), st));
-  if (fast) MMI_HIP(hipMemsetAsync(ctx->exc_overflow.p, 0, sizeof(uint32_t), st));
+  // histogram, marginal, count and overflow flag cleared by one small kernel
+  MMI_HIP(launch_clear_eval(ctx->hist.as<unsigned long long>(), B * B, ctx->fmarg.as<unsigned long long>(), B,
+                            ctx->nvalid.as<unsigned long long>(), dev_overflow(ctx), st));
   MMI_HIP(hipEventRecord(ctx->ev[0], st));
   if (fast && fa.mk == 2) {
     AxisTable rt[3];
@@ -1391,7 +1405,6 @@ retry:
   if (want_derivative && ctx->mtf.kind == MMI_TF_BSPLINE)
     if (int rc = bspline_separable(ctx, a.virt, D, tabs, &separable)) return rc;
   if (want_derivative) {
-    MMI_HIP(ctx->deriv.ensure(sizeof(double) * P));
     MMI_HIP(hipEventRecord(ctx->ev[2], st));
     if (fast && fa.mk == 2) {
       // B-spline fast path: per-sample factors, then the separable contraction
@@ -1404,14 +1417,13 @@ retry:
     } else if (fast) {
       MMI_HIP(launch_fast_pass2(fa, D, ctx->grid, st));
       MMI_HIP(launch_fast_exceptions2(a, fa, ctx->grid, st));
-      MMI_HIP(launch_fast_derivative(fa.partials, 2 * ctx->grid, fast_map(ctx), D, f.result, ctx->deriv.as<double>(),
-                                     st));
+      MMI_HIP(launch_fast_derivative(fa.partials, 2 * ctx->grid, fast_map(ctx), D, f.result, dev_deriv(ctx), st));
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     } else if (ctx->mtf.kind == MMI_TF_AFFINE) {
       MMI_HIP(ctx->partials.ensure(sizeof(double) * P * ctx->grid));
       a.partials = ctx->partials.as<double>();
       MMI_HIP(launch_pass2(a, ctx->grid, st));
-      MMI_HIP(launch_derivative_reduce(a.partials, ctx->grid, (int)P, f.result, ctx->deriv.as<double>(), st));
+      MMI_HIP(launch_derivative_reduce(a.partials, ctx->grid, (int)P, f.result, dev_deriv(ctx), st));
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     } else if (separable) {
       // Separable geometry: the B-spline Jacobian weight of node (a, b, c) at
@@ -1424,24 +1436,21 @@ retry:
       if (int rc = bspline_contract(ctx, a.virt, tabs, f.result)) return rc;
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     } else {
-      MMI_HIP(hipMemsetAsync(ctx->deriv.p, 0, sizeof(double) * P, st));
-      a.deriv_accum = ctx->deriv.as<double>();
+      MMI_HIP(hipMemsetAsync(dev_deriv(ctx), 0, sizeof(double) * P, st));
+      a.deriv_accum = dev_deriv(ctx);
       MMI_HIP(launch_pass2(a, ctx->grid, st));
-      MMI_HIP(launch_derivative_scale(ctx->deriv.as<double>(), P, f.result, st));
+      MMI_HIP(launch_derivative_scale(dev_deriv(ctx), P, f.result, st));
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     }
   }
-  // result, overflow flag and derivative through one pinned buffer
-  if (int rc = ensure_host_pinned(&ctx->h_out, &ctx->h_out_bytes, sizeof(DevResult) + 16 + sizeof(double) * P))
-    return rc;
+  // result, overflow flag and derivative: one copy of the output block
+  // through pinned memory
+  const size_t ob = out_bytes(want_derivative && derivative ? P : 0);
+  if (int rc = ensure_host_pinned(&ctx->h_out, &ctx->h_out_bytes, out_bytes(P))) return rc;
   char* hout = static_cast<char*>(ctx->h_out);
-  uint32_t* h_overflow = reinterpret_cast<uint32_t*>(hout + sizeof(DevResult));
-  double* h_deriv = reinterpret_cast<double*>(hout + sizeof(DevResult) + 16);
-  *h_overflow = 0;
-  if (fast) MMI_HIP(hipMemcpyAsync(h_overflow, ctx->exc_overflow.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  MMI_HIP(hipMemcpyAsync(hout, ctx->result.p, sizeof(DevResult), hipMemcpyDeviceToHost, st));
-  if (want_derivative && derivative)
-    MMI_HIP(hipMemcpyAsync(h_deriv, ctx->deriv.p, sizeof(double) * P, hipMemcpyDeviceToHost, st));
+  const uint32_t* h_overflow = reinterpret_cast<const uint32_t*>(hout + sizeof(DevResult));
+  const double* h_deriv = reinterpret_cast<const double*>(hout + sizeof(DevResult) + 16);
+  MMI_HIP(hipMemcpyAsync(hout, ctx->out.p, ob, hipMemcpyDeviceToHost, st));
   if (joint_pdf) MMI_HIP(hipMemcpyAsync(joint_pdf, ctx->pdf.p, sizeof(double) * B * B, hipMemcpyDeviceToHost, st));
   if (fixed_marginal)
     MMI_HIP(hipMemcpyAsync(fixed_marginal, ctx->fm.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
@@ -1656,13 +1665,13 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
       d.partials = fa[k].partials;
       d.rows = 2 * c->grid;
       d.map = fast_map(c);
-      d.result = c->result.as<DevResult>();
-      d.deriv = c->deriv.as<double>();
+      d.result = dev_result(c);
+      d.deriv = dev_deriv(c);
       std::memcpy(hs + o_dv + sizeof(BatchDeriv) * k, &d, sizeof(d));
       BatchOut it;
       std::memset(&it, 0, sizeof(it));
-      it.result = c->result.as<DevResult>();
-      it.deriv = want_derivative ? c->deriv.as<double>() : nullptr;
+      it.result = dev_result(c);
+      it.deriv = want_derivative ? dev_deriv(c) : nullptr;
       it.overflow = fa[k].exc_overflow;
       it.P = (int32_t)P;
       std::memcpy(hs + o_it + sizeof(BatchOut) * k, &it, sizeof(it));

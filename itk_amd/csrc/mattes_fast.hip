@@ -283,14 +283,24 @@ __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, uint32_t lb, 
   }
 }
 
-// Exact FP64 evaluation of the listed samples of one pass-1 workgroup.
+// Exact FP64 evaluation of the listed samples of one pass-1 workgroup.  Their
+// Parzen weights are summed per histogram bin in an LDS hash table (open
+// addressing; integer sums, so the order does not matter), then each touched
+// bin takes one global atomic.
 template <int D, typename P, int MK>
 __device__ __forceinline__ void exceptions_pass1_body(const EvalArgs& a, const FastArgs& f, uint32_t lb) {
-  extern __shared__ unsigned int lds_u32[];
+  extern __shared__ unsigned long long lds_u64e[];
   const uint32_t ne = min(f.exc_count[lb], (uint32_t)f.exc_cap);
   if (ne == 0) return;  // uniform over the workgroup
   const int B = a.bins;
-  unsigned int* fmarg = lds_u32;  // [B] fixed marginal counts, [B] valid count
+  unsigned long long* sums = lds_u64e;                                          // [kExcSlots]
+  unsigned int* keys = reinterpret_cast<unsigned int*>(sums + kExcSlots);       // [kExcSlots]
+  unsigned int* fmarg = keys + kExcSlots;  // [B] fixed marginal counts, [B] valid count
+  constexpr unsigned int kEmpty = 0xFFFFFFFFu;
+  for (int i = threadIdx.x; i < kExcSlots; i += blockDim.x) {
+    keys[i] = kEmpty;
+    sums[i] = 0ull;
+  }
   for (int i = threadIdx.x; i <= B; i += blockDim.x) fmarg[i] = 0u;
   __syncthreads();
   for (uint32_t t = threadIdx.x; t < ne; t += blockDim.x) {
@@ -308,12 +318,23 @@ __device__ __forceinline__ void exceptions_pass1_body(const EvalArgs& a, const F
     atomicAdd(&fmarg[o.fbin], 1u);
     unsigned int q[4];
     parzen_fixed((float)((double)(o.mbin - 1) - o.mterm), q);
-    unsigned long long* row = a.hist + o.fbin * a.bins + (o.mbin - 1);
+    const unsigned int base = (unsigned int)(o.fbin * B + (o.mbin - 1));
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (q[k]) atomicAdd(&row[k], (unsigned long long)q[k]);
+    for (int k = 0; k < 4; ++k) {
+      if (!q[k]) continue;
+      const unsigned int bin = base + k;
+      unsigned int h = (bin * 2654435761u) >> (32 - 11);  // kExcSlots = 2^11
+      for (;;) {
+        const unsigned int prev = atomicCAS(&keys[h], kEmpty, bin);
+        if (prev == kEmpty || prev == bin) break;
+        h = (h + 1) & (kExcSlots - 1);
+      }
+      atomicAdd(&sums[h], (unsigned long long)q[k]);
+    }
   }
   __syncthreads();
+  for (int i = threadIdx.x; i < kExcSlots; i += blockDim.x)
+    if (keys[i] != kEmpty && sums[i]) atomicAdd(&a.hist[keys[i]], sums[i]);
   // one global atomic per non-zero count of the workgroup
   for (int i = threadIdx.x; i < B; i += blockDim.x)
     if (fmarg[i]) atomicAdd(&a.fmarg[i], (unsigned long long)fmarg[i]);
@@ -662,7 +683,8 @@ hipError_t launch_fast_pass1(const FastArgs& f, int dim, int grid, hipStream_t s
 }
 
 hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st) {
-  MMI_DISPATCH_FAST_MK(exceptions_pass1_kernel, f.mk, f.storage, a.virt.dim, grid, (a.bins + 1) * 4, st, a, f);
+  MMI_DISPATCH_FAST_MK(exceptions_pass1_kernel, f.mk, f.storage, a.virt.dim, grid, exceptions1_lds_bytes(a.bins), st,
+                       a, f);
   return hipGetLastError();
 }
 
@@ -691,7 +713,7 @@ hipError_t launch_fast_batch(const BatchLaunch& b, hipStream_t st) {
   hipLaunchKernelGGL(batch_clear, dim3(b.n), dim3(256), 0, st, b.fa);
   const size_t lds1 = (size_t)b.lds1;
   MMI_DISPATCH_FAST(pass1_fast_batch, b.storage, b.dim, b.total_blocks, lds1, st, b.fa, b.start, b.n);
-  MMI_DISPATCH_FAST(exceptions_pass1_batch, b.storage, b.dim, b.total_blocks, (size_t)(B + 1) * 4, st, b.ea, b.fa,
+  MMI_DISPATCH_FAST(exceptions_pass1_batch, b.storage, b.dim, b.total_blocks, exceptions1_lds_bytes(B), st, b.ea, b.fa,
                     b.start, b.n);
   hipError_t e = launch_finalize_batch(b.fz, b.n, B, st);
   if (e != hipSuccess) return e;

@@ -203,6 +203,15 @@ __host__ __device__ inline size_t pass1_lds_bytes(int bins) {
   return c * bins * bins * 8 + (c * bins + 2) * 4;
 }
 
+// Exceptions of pass 1: a workgroup sums its samples' Parzen weights per
+// joint-histogram bin in an LDS hash table before one global atomic per bin,
+// so hot bins do not serialise the L2 with per-sample 64-bit atomics.  The
+// table holds at least twice the 4 bins of every listed sample.
+constexpr int kExcSlots = 2048;
+__host__ __device__ inline size_t exceptions1_lds_bytes(int bins) {
+  return (((size_t)bins + 1) * 4 + 7) / 8 * 8 + (size_t)kExcSlots * (8 + 4);
+}
+
 // The finalize workgroup stages the PDF and marginals in LDS when they fit.
 inline bool finalize_uses_lds(int bins) {
   return ((size_t)bins * bins + 3 * (size_t)bins) * sizeof(double) <= 48 * 1024;
@@ -263,6 +272,8 @@ hipError_t launch_to_storage(const float* in, void* out, int storage, int64_t n,
 hipError_t launch_texels_to_storage(const float4* in, void* out, int storage, int64_t n, hipStream_t st);
 hipError_t launch_recursive_gaussian(const void* in, bool in_float, double* out, int64_t ln, int64_t stride,
                                      int64_t nlines, const RecCoeffs& c, const RecSink& sink, hipStream_t st);
+hipError_t launch_clear_eval(unsigned long long* hist, int n_hist, unsigned long long* fmarg, int n_fmarg,
+                             unsigned long long* nvalid, uint32_t* overflow, hipStream_t st);
 hipError_t launch_pack_texels(const float* moving, const double* grad, int64_t n, int D, const Direction& dir,
                               float4* tex, hipStream_t st);
 
