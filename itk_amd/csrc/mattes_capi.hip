@@ -17,11 +17,6 @@
 
 using namespace mmi;
 
-// Uncertain samples one fast-path workgroup may defer to the exact path (one
-// per thread of the sorting workgroup in exceptions_pass2_kernel).
-constexpr int kExceptionCapacity = 256;
-static_assert(2 * 4 * kExceptionCapacity <= kExcSlots, "exception hash table too small");
-
 namespace {
 
 thread_local std::string g_last_error = "";
@@ -1302,9 +1297,9 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
   ctx->last_path = fast ? 1 : 0;
   if (fast) {
     MMI_HIP(ctx->exc_count.ensure(sizeof(uint32_t) * ctx->grid));
-    MMI_HIP(ctx->exc_list.ensure(sizeof(uint32_t) * ctx->grid * kExceptionCapacity));
+    MMI_HIP(ctx->exc_list.ensure(sizeof(uint32_t) * ctx->grid * kExcCap));
     fa.xcd_chunks = ctx->xcd_chunks;
-    fa.exc_cap = kExceptionCapacity;
+    fa.exc_cap = kExcCap;
     fa.fixed_px = a.fixed_px;
     fa.moving_px = a.moving_px;
     fa.texels = a.texels;
@@ -1389,8 +1384,8 @@ retry:
                                 (int)a.virt.size[1], D == 3 ? (int)a.virt.size[2] : 1, rt, ctx->bs_rows.as<double>(), st));
   }
   if (fast) {
-    MMI_HIP(launch_fast_pass1(fa, D, ctx->grid, st));
-    MMI_HIP(launch_fast_exceptions1(a, fa, ctx->grid, st));
+    MMI_HIP(launch_fast_pass1(fa, a, D, ctx->grid, st));
+    if (fa.mk == 2) MMI_HIP(launch_fast_exceptions1(a, fa, ctx->grid, st));
   } else {
     MMI_HIP(launch_pass1(a, ctx->grid, st));
   }
@@ -1410,13 +1405,12 @@ retry:
       // B-spline fast path: per-sample factors, then the separable contraction
       MMI_HIP(ctx->bs_g.ensure(sizeof(float) * D * n));
       fa.bspline_g = ctx->bs_g.as<float>();
-      MMI_HIP(launch_fast_pass2(fa, D, ctx->grid, st));
+      MMI_HIP(launch_fast_pass2(fa, a, D, ctx->grid, st));
       MMI_HIP(launch_fast_exceptions2(a, fa, ctx->grid, st));
       if (int rc = bspline_contract(ctx, a.virt, tabs, f.result)) return rc;
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     } else if (fast) {
-      MMI_HIP(launch_fast_pass2(fa, D, ctx->grid, st));
-      MMI_HIP(launch_fast_exceptions2(a, fa, ctx->grid, st));
+      MMI_HIP(launch_fast_pass2(fa, a, D, ctx->grid, st));
       MMI_HIP(launch_fast_derivative(fa.partials, 2 * ctx->grid, fast_map(ctx), D, f.result, dev_deriv(ctx), st));
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     } else if (ctx->mtf.kind == MMI_TF_AFFINE) {

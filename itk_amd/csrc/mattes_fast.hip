@@ -10,15 +10,15 @@
 // and is evaluated afterwards on the exact FP64 path (mattes_eval.h), so the
 // bins and the valid count are bit-identical to the reference's.
 //
-// Kernels: pass1_fast (histogram of certain samples) -> exceptions_pass1
-// (exact samples, integer atomics) -> finalize (mattes_kernels.hip) ->
-// pass2_fast (derivative moments of certain samples) -> exceptions_pass2
-// (exact samples, sorted per list so sums are in a fixed order) ->
-// fast_derivative (fixed-order reduction, index-space moments mapped to the
-// affine parameters).
+// Kernels: pass1_fast (histogram of certain samples; each workgroup then
+// evaluates its own listed samples exactly into the same LDS histogram) ->
+// finalize (mattes_kernels.hip) -> pass2_fast (derivative moments of certain
+// samples) -> exceptions_pass2 (exact samples, sorted per list so sums are in
+// a fixed order) -> fast_derivative (fixed-order reduction, index-space
+// moments mapped to the affine parameters).
 //
-// The hot kernels take a compact argument block so that the uniform values
-// they use fit the scalar register file; the exception kernels take the full
+// The hot loops read a compact argument block so that the uniform values they
+// use fit the scalar register file; the exact evaluations take the full
 // EvalArgs.
 #include "mattes_eval.h"
 
@@ -216,13 +216,43 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
   return out ? 0 : (uncertain ? 2 : 1);
 }
 
+// The uncertain samples a pass-1 workgroup listed, on the exact FP64 path,
+// into the workgroup's LDS histogram (copy 0), fixed marginal and valid count,
+// at the end of pass 1 (affine: the exact path's registers do not raise the
+// kernel's count; the samples' data is still in L2).
 template <int D, typename P, int MK>
-__device__ __forceinline__ void pass1_fast_body(const FastArgs& f, uint32_t lb, uint32_t lg) {
+__device__ __forceinline__ void pass1_exceptions(const EvalArgs& a, const uint32_t* list, uint32_t ne,
+                                                             unsigned long long* hist, unsigned int* fmarg,
+                                                             unsigned int* nvalid) {
+  const int B = a.bins;
+  for (uint32_t t = threadIdx.x; t < ne; t += blockDim.x) {
+    const uint32_t s = list[t];
+    int32_t idx[3];
+    sample_index<D>(a.virt, s, idx);
+    SampleOut<D> o;
+    const bool ok = eval_sample<D, MK, P>(a, idx, o);
+    if (a.sample_fbin) {
+      a.sample_fbin[s] = ok ? (int16_t)o.fbin : (int16_t)-1;
+      a.sample_mbin[s] = ok ? (int16_t)o.mbin : (int16_t)-1;
+    }
+    if (!ok) continue;
+    atomicAdd(nvalid, 1u);
+    atomicAdd(&fmarg[o.fbin], 1u);
+    unsigned int q[4];
+    parzen_fixed((float)((double)(o.mbin - 1) - o.mterm), q);
+    unsigned long long* row = hist + o.fbin * B + (o.mbin - 1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) atomicAdd(&row[k], (unsigned long long)q[k]);
+  }
+}
+
+template <int D, typename P, int MK>
+__device__ __forceinline__ void pass1_fast_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
   extern __shared__ unsigned long long lds_u64[];
   const int B = f.bins;
   const int BB = B * B;
   const int copies = hist_copies(B);
-  unsigned long long* hist0 = lds_u64;                                      // [copies][B][B]
+  unsigned long long* hist0 = lds_u64;                                            // [copies][B][B]
   unsigned int* fmarg0 = reinterpret_cast<unsigned int*>(lds_u64 + copies * BB);  // [copies][B]
   unsigned int* counters = fmarg0 + copies * B;  // [0] valid count, [1] exception count
   const int mine = copies == 2 ? (int)(threadIdx.x & 1u) : 0;
@@ -243,8 +273,9 @@ __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, uint32_t lb, 
     bool bsin;
     const int c = fast_classify<D, P, MK>(f, idx, fbin, mbin, arg0, cell, &bsin);
     if (c == 2) {
+      // listed for the exact path (here and in pass 2)
       const unsigned int slot = atomicAdd(&counters[1], 1u);
-      if (slot < (unsigned int)f.exc_cap) f.exc_list[lb * f.exc_cap + slot] = s;
+      if (slot < (unsigned int)kExcCap) f.exc_list[lb * kExcCap + slot] = s;
       continue;
     }
     if (f.sample_fbin) {
@@ -267,6 +298,14 @@ __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, uint32_t lb, 
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(&counters[0], v);
   __syncthreads();
+  const unsigned int ne = counters[1];
+  if constexpr (MK == 1) {
+    // the list this workgroup wrote (visible to it after the barrier)
+    if (ne)
+      pass1_exceptions<D, P, MK>(a, f.exc_list + lb * kExcCap, min(ne, (unsigned int)kExcCap), hist0, fmarg0,
+                                 &counters[0]);
+    __syncthreads();
+  }
   for (int i = threadIdx.x; i < BB; i += blockDim.x) {
     const unsigned long long h = hist0[i] + (copies == 2 ? hist0[BB + i] : 0ull);
     if (h) atomicAdd(&f.hist[i], h);
@@ -277,20 +316,21 @@ __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, uint32_t lb, 
   }
   if (threadIdx.x == 0) {
     if (counters[0]) atomicAdd(f.nvalid, (unsigned long long)counters[0]);
-    const unsigned int ne = counters[1];
     f.exc_count[lb] = ne;
-    if (ne > (unsigned int)f.exc_cap) atomicOr(f.exc_overflow, 1u);
+    if (ne > (unsigned int)kExcCap) atomicOr(f.exc_overflow, 1u);
   }
 }
 
-// Exact FP64 evaluation of the listed samples of one pass-1 workgroup.  Their
+// Exact FP64 evaluation of the listed samples of one pass-1 workgroup, as a
+// kernel of its own (B-spline: its exact path would raise pass 1's register
+// count if it ran in pass 1's tail).  Their
 // Parzen weights are summed per histogram bin in an LDS hash table (open
 // addressing; integer sums, so the order does not matter), then each touched
 // bin takes one global atomic.
 template <int D, typename P, int MK>
 __device__ __forceinline__ void exceptions_pass1_body(const EvalArgs& a, const FastArgs& f, uint32_t lb) {
   extern __shared__ unsigned long long lds_u64e[];
-  const uint32_t ne = min(f.exc_count[lb], (uint32_t)f.exc_cap);
+  const uint32_t ne = min(f.exc_count[lb], (uint32_t)kExcCap);
   if (ne == 0) return;  // uniform over the workgroup
   const int B = a.bins;
   unsigned long long* sums = lds_u64e;                                          // [kExcSlots]
@@ -304,7 +344,7 @@ __device__ __forceinline__ void exceptions_pass1_body(const EvalArgs& a, const F
   for (int i = threadIdx.x; i <= B; i += blockDim.x) fmarg[i] = 0u;
   __syncthreads();
   for (uint32_t t = threadIdx.x; t < ne; t += blockDim.x) {
-    const uint32_t s = f.exc_list[lb * f.exc_cap + t];
+    const uint32_t s = f.exc_list[lb * kExcCap + t];
     int32_t idx[3];
     sample_index<D>(a.virt, s, idx);
     SampleOut<D> o;
@@ -353,54 +393,6 @@ __device__ __forceinline__ void add_moments(double* acc, float w, float3 g, cons
     for (int j = 0; j < D; ++j) acc[b * D + j] = fma(gw[b], (double)idx[j], acc[b * D + j]);
     acc[D * D + b] += gw[b];
   }
-}
-
-template <int D, typename P, int MK>
-__device__ __forceinline__ void pass2_fast_body(const FastArgs& f, uint32_t lb, uint32_t lg) {
-  extern __shared__ float lds_f32[];
-  const int B = f.bins;
-  const int BB = B * B;
-  double* scratch = reinterpret_cast<double*>(lds_f32);  // 4 wavefronts x 12 doubles (128 floats)
-  float* pr = lds_f32 + 128;
-  for (int i = threadIdx.x; i < BB; i += blockDim.x) pr[i] = f.pratio_f[i];
-  __syncthreads();
-  constexpr int PA = D * D + D;
-  double acc[PA];
-#pragma unroll
-  for (int i = 0; i < PA; ++i) acc[i] = 0.0;
-  uint32_t s, end, step;
-  run_range<D>(f, lb, lg, s, end, step);
-  for (; s < end; s += step) {
-    int32_t idx[3];
-    fast_index<D>(f, s, idx);
-    int fbin = 0, mbin = 0;
-    float arg0 = 0.0f;
-    Cell cell;
-    bool bsin;
-    const int c = fast_classify<D, P, MK>(f, idx, fbin, mbin, arg0, cell, &bsin);
-    if constexpr (MK == 2) {
-      // per-sample factors of the separable B-spline derivative, for every
-      // sample (uncertain ones are rewritten by the exception kernel)
-      float gw[3] = {0.0f, 0.0f, 0.0f};
-      if (c == 1 && bsin) {
-        const float w = parzen_deriv_weight_fast(arg0, pr + fbin * B + (mbin - 1));
-        if (w != 0.0f) {
-          const float3 g = interp_gradient3<D, P, true>(static_cast<const typename P::texel*>(f.texels), cell);
-          gw[0] = (float)((double)w * (double)g.x);
-          gw[1] = (float)((double)w * (double)g.y);
-          gw[2] = (float)((double)w * (double)g.z);
-        }
-      }
-#pragma unroll
-      for (int d = 0; d < D; ++d) f.bspline_g[(int64_t)d * f.n + s] = gw[d];
-    } else {
-      if (c != 1) continue;
-      const float w = parzen_deriv_weight_fast(arg0, pr + fbin * B + (mbin - 1));
-      if (w == 0.0f) continue;
-      add_moments<D>(acc, w, interp_gradient3<D, P, true>(static_cast<const typename P::texel*>(f.texels), cell), idx);
-    }
-  }
-  if constexpr (MK == 1) block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)lb * PA);
 }
 
 // Exception samples of one pass-1 workgroup, in sample order (bitonic sort of
@@ -472,6 +464,58 @@ __device__ __forceinline__ void exceptions_pass2_body(const EvalArgs& a, const F
   block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)(lg + lb) * PA);
 }
 
+template <int D, typename P, int MK>
+__device__ __forceinline__ void pass2_fast_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
+  extern __shared__ float lds_f32[];
+  const int B = f.bins;
+  const int BB = B * B;
+  double* scratch = reinterpret_cast<double*>(lds_f32);  // 4 wavefronts x 12 doubles (128 floats)
+  float* pr = lds_f32 + 128;
+  for (int i = threadIdx.x; i < BB; i += blockDim.x) pr[i] = f.pratio_f[i];
+  __syncthreads();
+  constexpr int PA = D * D + D;
+  double acc[PA];
+#pragma unroll
+  for (int i = 0; i < PA; ++i) acc[i] = 0.0;
+  uint32_t s, end, step;
+  run_range<D>(f, lb, lg, s, end, step);
+  for (; s < end; s += step) {
+    int32_t idx[3];
+    fast_index<D>(f, s, idx);
+    int fbin = 0, mbin = 0;
+    float arg0 = 0.0f;
+    Cell cell;
+    bool bsin;
+    const int c = fast_classify<D, P, MK>(f, idx, fbin, mbin, arg0, cell, &bsin);
+    if constexpr (MK == 2) {
+      // per-sample factors of the separable B-spline derivative, for every
+      // sample (uncertain ones are rewritten by the exception kernel)
+      float gw[3] = {0.0f, 0.0f, 0.0f};
+      if (c == 1 && bsin) {
+        const float w = parzen_deriv_weight_fast(arg0, pr + fbin * B + (mbin - 1));
+        if (w != 0.0f) {
+          const float3 g = interp_gradient3<D, P, true>(static_cast<const typename P::texel*>(f.texels), cell);
+          gw[0] = (float)((double)w * (double)g.x);
+          gw[1] = (float)((double)w * (double)g.y);
+          gw[2] = (float)((double)w * (double)g.z);
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) f.bspline_g[(int64_t)d * f.n + s] = gw[d];
+    } else {
+      if (c != 1) continue;
+      const float w = parzen_deriv_weight_fast(arg0, pr + fbin * B + (mbin - 1));
+      if (w == 0.0f) continue;
+      add_moments<D>(acc, w, interp_gradient3<D, P, true>(static_cast<const typename P::texel*>(f.texels), cell), idx);
+    }
+  }
+  if constexpr (MK == 1) {
+    block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)lb * PA);
+    // this workgroup's listed samples, exactly, into their own partial row
+    exceptions_pass2_body<D, P, 1>(a, f, lb, lg);
+  }
+}
+
 // Fixed-order reduction of 2 * nblocks rows of index-space moments, then the
 // map to the affine derivative:
 //   d[b*D + d]   = nFactor * (sum_j V[d][j] M[b][j] + c0[d] M1[b])
@@ -523,15 +567,16 @@ __device__ __forceinline__ int batch_item(const int32_t* __restrict__ start, int
 }
 
 template <int D, typename P, int MK>
-__global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f) {
-  pass1_fast_body<D, P, MK>(f, blockIdx.x, gridDim.x);
+__global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f, EvalArgs a) {
+  pass1_fast_body<D, P, MK>(f, a, blockIdx.x, gridDim.x);
 }
 
 template <int D, typename P>
 __global__ void __launch_bounds__(256) pass1_fast_batch(const FastArgs* __restrict__ fa,
+                                                        const EvalArgs* __restrict__ ea,
                                                         const int32_t* __restrict__ start, int n) {
   const int r = batch_item(start, n, blockIdx.x);
-  pass1_fast_body<D, P, 1>(fa[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+  pass1_fast_body<D, P, 1>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
 }
 
 template <int D, typename P, int MK>
@@ -539,37 +584,22 @@ __global__ void __launch_bounds__(256) exceptions_pass1_kernel(EvalArgs a, FastA
   exceptions_pass1_body<D, P, MK>(a, f, blockIdx.x);
 }
 
-template <int D, typename P>
-__global__ void __launch_bounds__(256) exceptions_pass1_batch(const EvalArgs* __restrict__ ea,
-                                                              const FastArgs* __restrict__ fa,
-                                                              const int32_t* __restrict__ start, int n) {
-  const int r = batch_item(start, n, blockIdx.x);
-  exceptions_pass1_body<D, P, 1>(ea[r], fa[r], blockIdx.x - start[r]);
-}
-
 template <int D, typename P, int MK>
-__global__ void __launch_bounds__(256) pass2_fast_kernel(FastArgs f) {
-  pass2_fast_body<D, P, MK>(f, blockIdx.x, gridDim.x);
+__global__ void __launch_bounds__(256) pass2_fast_kernel(FastArgs f, EvalArgs a) {
+  pass2_fast_body<D, P, MK>(f, a, blockIdx.x, gridDim.x);
 }
 
 template <int D, typename P>
 __global__ void __launch_bounds__(256) pass2_fast_batch(const FastArgs* __restrict__ fa,
+                                                        const EvalArgs* __restrict__ ea,
                                                         const int32_t* __restrict__ start, int n) {
   const int r = batch_item(start, n, blockIdx.x);
-  pass2_fast_body<D, P, 1>(fa[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+  pass2_fast_body<D, P, 1>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
 }
 
 template <int D, typename P, int MK>
 __global__ void __launch_bounds__(256) exceptions_pass2_kernel(EvalArgs a, FastArgs f) {
   exceptions_pass2_body<D, P, MK>(a, f, blockIdx.x, gridDim.x);
-}
-
-template <int D, typename P>
-__global__ void __launch_bounds__(256) exceptions_pass2_batch(const EvalArgs* __restrict__ ea,
-                                                              const FastArgs* __restrict__ fa,
-                                                              const int32_t* __restrict__ start, int n) {
-  const int r = batch_item(start, n, blockIdx.x);
-  exceptions_pass2_body<D, P, 1>(ea[r], fa[r], blockIdx.x - start[r], start[r + 1] - start[r]);
 }
 
 template <int D>
@@ -676,9 +706,9 @@ __global__ void batch_gather(const BatchOut* __restrict__ items, double* __restr
     }                                                                                               \
   } while (0)
 
-hipError_t launch_fast_pass1(const FastArgs& f, int dim, int grid, hipStream_t st) {
+hipError_t launch_fast_pass1(const FastArgs& f, const EvalArgs& a, int dim, int grid, hipStream_t st) {
   const size_t lds = pass1_lds_bytes(f.bins);
-  MMI_DISPATCH_FAST_MK(pass1_fast_kernel, f.mk, f.storage, dim, grid, lds, st, f);
+  MMI_DISPATCH_FAST_MK(pass1_fast_kernel, f.mk, f.storage, dim, grid, lds, st, f, a);
   return hipGetLastError();
 }
 
@@ -688,9 +718,9 @@ hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int gri
   return hipGetLastError();
 }
 
-hipError_t launch_fast_pass2(const FastArgs& f, int dim, int grid, hipStream_t st) {
+hipError_t launch_fast_pass2(const FastArgs& f, const EvalArgs& a, int dim, int grid, hipStream_t st) {
   const size_t lds = ((size_t)f.bins * f.bins + 128) * 4;
-  MMI_DISPATCH_FAST_MK(pass2_fast_kernel, f.mk, f.storage, dim, grid, lds, st, f);
+  MMI_DISPATCH_FAST_MK(pass2_fast_kernel, f.mk, f.storage, dim, grid, lds, st, f, a);
   return hipGetLastError();
 }
 
@@ -712,15 +742,12 @@ hipError_t launch_fast_batch(const BatchLaunch& b, hipStream_t st) {
   const int B = b.bins_max;
   hipLaunchKernelGGL(batch_clear, dim3(b.n), dim3(256), 0, st, b.fa);
   const size_t lds1 = (size_t)b.lds1;
-  MMI_DISPATCH_FAST(pass1_fast_batch, b.storage, b.dim, b.total_blocks, lds1, st, b.fa, b.start, b.n);
-  MMI_DISPATCH_FAST(exceptions_pass1_batch, b.storage, b.dim, b.total_blocks, exceptions1_lds_bytes(B), st, b.ea, b.fa,
-                    b.start, b.n);
+  MMI_DISPATCH_FAST(pass1_fast_batch, b.storage, b.dim, b.total_blocks, lds1, st, b.fa, b.ea, b.start, b.n);
   hipError_t e = launch_finalize_batch(b.fz, b.n, B, st);
   if (e != hipSuccess) return e;
   if (b.want_derivative) {
     const size_t lds2 = ((size_t)B * B + 128) * 4;
-    MMI_DISPATCH_FAST(pass2_fast_batch, b.storage, b.dim, b.total_blocks, lds2, st, b.fa, b.start, b.n);
-    MMI_DISPATCH_FAST(exceptions_pass2_batch, b.storage, b.dim, b.total_blocks, 0, st, b.ea, b.fa, b.start, b.n);
+    MMI_DISPATCH_FAST(pass2_fast_batch, b.storage, b.dim, b.total_blocks, lds2, st, b.fa, b.ea, b.start, b.n);
     if (b.dim == 2)
       hipLaunchKernelGGL(fast_derivative_batch<2>, dim3(b.n), dim3(256), 0, st, b.dv);
     else

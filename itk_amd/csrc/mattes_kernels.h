@@ -191,26 +191,31 @@ struct BatchLaunch {
   double* out;  // [n][out_stride]: value, n_valid, status, derivative
 };
 
-// Copies of the pass-1 joint histogram (and fixed marginal) in LDS: two when
-// they fit in 64 KiB, updated by even and odd lanes.  Neighbouring samples
-// (adjacent lanes) usually fall in the same bins, so this halves the
-// same-address serialisation of the 64-bit LDS atomics.
-__host__ __device__ inline int hist_copies(int bins) {
-  return 2 * (size_t)bins * bins * 8 + (2 * (size_t)bins + 2) * 4 <= 64 * 1024 ? 2 : 1;
-}
-__host__ __device__ inline size_t pass1_lds_bytes(int bins) {
-  const size_t c = (size_t)hist_copies(bins);
-  return c * bins * bins * 8 + (c * bins + 2) * 4;
-}
-
-// Exceptions of pass 1: a workgroup sums its samples' Parzen weights per
-// joint-histogram bin in an LDS hash table before one global atomic per bin,
-// so hot bins do not serialise the L2 with per-sample 64-bit atomics.  The
-// table holds at least twice the 4 bins of every listed sample.
+// B-spline exceptions of pass 1 (a kernel of their own): a workgroup sums its
+// samples' Parzen weights per joint-histogram bin in an LDS hash table before
+// one global atomic per bin, so hot bins do not serialise the L2.  The table
+// holds at least twice the 4 bins of every listed sample.
 constexpr int kExcSlots = 2048;
 __host__ __device__ inline size_t exceptions1_lds_bytes(int bins) {
   return (((size_t)bins + 1) * 4 + 7) / 8 * 8 + (size_t)kExcSlots * (8 + 4);
 }
+
+// Uncertain samples a fast-path workgroup can list (more: the evaluation is
+// redone on the exact path).
+constexpr int kExcCap = 256;
+static_assert(2 * 4 * kExcCap <= kExcSlots, "exception hash table too small");
+
+// Copies of the pass-1 joint histogram (and fixed marginal) in LDS: two when
+// they fit in 64 KiB, updated by even and odd lanes.  Neighbouring samples (adjacent lanes) usually fall in the same
+// bins, so this halves the same-address serialisation of the 64-bit LDS
+// atomics.
+__host__ __device__ inline size_t pass1_lds_bytes_with(int bins, int copies) {
+  return (size_t)copies * bins * bins * 8 + ((size_t)copies * bins + 2) * 4;
+}
+__host__ __device__ inline int hist_copies(int bins) {
+  return pass1_lds_bytes_with(bins, 2) <= 64 * 1024 ? 2 : 1;
+}
+__host__ __device__ inline size_t pass1_lds_bytes(int bins) { return pass1_lds_bytes_with(bins, hist_copies(bins)); }
 
 // The finalize workgroup stages the PDF and marginals in LDS when they fit.
 inline bool finalize_uses_lds(int bins) {
@@ -246,9 +251,15 @@ hipError_t launch_finalize_batch(const FinalizeArgs* fz, int n, int bins_max, hi
 // finalize, pass 2, exceptions, derivative, gather), in launch order.
 hipError_t launch_fast_batch(const BatchLaunch& b, hipStream_t st);
 hipError_t launch_pass2(const EvalArgs& a, int grid, hipStream_t st);
-hipError_t launch_fast_pass1(const FastArgs& f, int dim, int grid, hipStream_t st);
+// Pass 1 of the fast path; each workgroup ends by evaluating its own uncertain
+// samples on the exact path (the EvalArgs) into its LDS histogram.
+hipError_t launch_fast_pass1(const FastArgs& f, const EvalArgs& a, int dim, int grid, hipStream_t st);
+// B-spline fast path: pass 1's listed samples on the exact path.
 hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st);
-hipError_t launch_fast_pass2(const FastArgs& f, int dim, int grid, hipStream_t st);
+// Pass 2 of the fast path; affine workgroups end with their listed samples on
+// the exact path (their own partial row).
+hipError_t launch_fast_pass2(const FastArgs& f, const EvalArgs& a, int dim, int grid, hipStream_t st);
+// B-spline fast path: pass 2's listed samples on the exact path.
 hipError_t launch_fast_exceptions2(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st);
 hipError_t launch_fast_derivative(const double* partials, int rows, const FastMap& map, int dim,
                                   const DevResult* r, double* deriv, hipStream_t st);
