@@ -61,6 +61,7 @@ const char* mmi_last_error(void);
 /* Context flags. */
 #define MMI_FLAG_RECORD_SAMPLE_BINS 1u /* keep per-sample Parzen bins for mmi_get_sample_bins */
 #define MMI_FLAG_EXACT_PATH_ONLY 2u    /* evaluate every sample on the FP64 path (no certified FP32 fast path) */
+#define MMI_FLAG_TIME_PASSES 4u        /* time pass 1 and pass 2 apart (two more events per evaluation) */
 
 /* Create a context on a HIP device with the flags above (0 for defaults). */
 int mmi_context_create(int device, uint32_t flags, mmi_context** out);
@@ -161,8 +162,10 @@ int mmi_get_moving_texels(mmi_context* ctx, float* texels4);
 int mmi_get_histogram_parameters(mmi_context* ctx, double* fixed_min, double* fixed_max, double* moving_min,
                                  double* moving_max, double* fixed_bin_size, double* moving_bin_size);
 
-/* Timing support: average device time (ms) of the main evaluation kernels of
- * the last mmi_evaluate, measured with HIP events on the context stream. */
+/* Timing support: device time (ms) of the last mmi_evaluate, from HIP events
+ * on the context stream around its kernels; pass1_ms (clear, pass 1 with its
+ * exceptions) and pass2_ms (finalize excluded: pass 2 through the derivative)
+ * only with MMI_FLAG_TIME_PASSES, else -1. */
 int mmi_get_last_timing(mmi_context* ctx, float* total_ms, float* pass1_ms, float* pass2_ms);
 
 /* Diagnostics (no counterpart in the reference): which kernels the last

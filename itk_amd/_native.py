@@ -25,6 +25,7 @@ MMI_TF_IDENTITY, MMI_TF_AFFINE, MMI_TF_BSPLINE = 0, 1, 2
 MMI_GRADIENT_CENTRAL_DIFFERENCE, MMI_GRADIENT_IMAGE = 0, 1
 MMI_FLAG_RECORD_SAMPLE_BINS = 1
 MMI_FLAG_EXACT_PATH_ONLY = 2
+MMI_FLAG_TIME_PASSES = 4
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p

@@ -299,6 +299,7 @@ struct mmi_context {
   size_t h_out_bytes = 0;
   float last_total_ms = 0, last_pass1_ms = 0, last_pass2_ms = 0;
   int32_t last_path = 0;  // 0 exact, 1 fast, 2 fast redone exactly
+  bool acc_clean = false;  // pass-1 accumulators known to be zero (finalize zeroes them)
 };
 
 namespace {
@@ -1186,6 +1187,7 @@ int mmi_initialize(mmi_context* ctx) {
     MMI_HIP(ctx->sample_mbin.ensure(sizeof(int16_t) * n));
   }
   ctx->initialized = true;
+  ctx->acc_clean = false;  // the accumulators may have been reallocated
   return MMI_OK;
 }
 
@@ -1337,6 +1339,7 @@ FinalizeArgs finalize_args(mmi_context* ctx) {
   f.pratio_f = ctx->pratio_f.as<float>();
   f.rowsum = ctx->rowsum.as<double>();
   f.result = dev_result(ctx);
+  f.overflow = dev_overflow(ctx);
   f.moving_bin_size = ctx->moving_bin_size;
   return f;
 }
@@ -1367,11 +1370,18 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
   const int64_t n = num_pixels(ctx->virtual_g);
   const int64_t P = (int64_t)ctx->mtf.params.size();
   hipStream_t st = ctx->stream;
+  // An event record costs the queue a few microseconds (a timestamp with a
+  // release), so pass 1 and pass 2 are timed apart only on request.
+  const bool time_passes = (ctx->flags & 4u) != 0;
 retry:
-  // histogram, marginal, count and overflow flag cleared by one small kernel
-  MMI_HIP(launch_clear_eval(ctx->hist.as<unsigned long long>(), B * B, ctx->fmarg.as<unsigned long long>(), B,
-                            ctx->nvalid.as<unsigned long long>(), dev_overflow(ctx), st));
   MMI_HIP(hipEventRecord(ctx->ev[0], st));
+  // pass 1's accumulators (histogram, marginal, count, overflow word) are
+  // zero between evaluations: finalize leaves them so; a clear kernel runs
+  // only after initialisation or an interrupted evaluation
+  if (!ctx->acc_clean)
+    MMI_HIP(launch_clear_eval(ctx->hist.as<unsigned long long>(), B * B, ctx->fmarg.as<unsigned long long>(), B,
+                              ctx->nvalid.as<unsigned long long>(), dev_overflow(ctx), st));
+  ctx->acc_clean = false;
   if (fast && fa.mk == 2) {
     AxisTable rt[3];
     for (int k = 0; k < 3; ++k) {
@@ -1389,7 +1399,7 @@ retry:
   } else {
     MMI_HIP(launch_pass1(a, ctx->grid, st));
   }
-  MMI_HIP(hipEventRecord(ctx->ev[1], st));
+  if (time_passes || !want_derivative) MMI_HIP(hipEventRecord(ctx->ev[1], st));
 
   const FinalizeArgs f = finalize_args(ctx);
   MMI_HIP(launch_finalize(f, st));
@@ -1400,7 +1410,7 @@ retry:
   if (want_derivative && ctx->mtf.kind == MMI_TF_BSPLINE)
     if (int rc = bspline_separable(ctx, a.virt, D, tabs, &separable)) return rc;
   if (want_derivative) {
-    MMI_HIP(hipEventRecord(ctx->ev[2], st));
+    if (time_passes) MMI_HIP(hipEventRecord(ctx->ev[2], st));
     if (fast && fa.mk == 2) {
       // B-spline fast path: per-sample factors, then the separable contraction
       MMI_HIP(ctx->bs_g.ensure(sizeof(float) * D * n));
@@ -1442,16 +1452,16 @@ retry:
   const size_t ob = out_bytes(want_derivative && derivative ? P : 0);
   if (int rc = ensure_host_pinned(&ctx->h_out, &ctx->h_out_bytes, out_bytes(P))) return rc;
   char* hout = static_cast<char*>(ctx->h_out);
-  const uint32_t* h_overflow = reinterpret_cast<const uint32_t*>(hout + sizeof(DevResult));
   const double* h_deriv = reinterpret_cast<const double*>(hout + sizeof(DevResult) + 16);
   MMI_HIP(hipMemcpyAsync(hout, ctx->out.p, ob, hipMemcpyDeviceToHost, st));
   if (joint_pdf) MMI_HIP(hipMemcpyAsync(joint_pdf, ctx->pdf.p, sizeof(double) * B * B, hipMemcpyDeviceToHost, st));
   if (fixed_marginal)
     MMI_HIP(hipMemcpyAsync(fixed_marginal, ctx->fm.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
   MMI_HIP(hipStreamSynchronize(st));
+  ctx->acc_clean = true;  // finalize ran
   DevResult r;
   std::memcpy(&r, hout, sizeof(DevResult));
-  const uint32_t overflow = *h_overflow;
+  const bool overflow = r.exc_overflow != 0;
   if (want_derivative && derivative) std::memcpy(derivative, h_deriv, sizeof(double) * P);
   if (overflow) {
     // more uncertain samples in one workgroup than its list holds: redo exactly
@@ -1459,13 +1469,16 @@ retry:
     ctx->last_path = 2;
     goto retry;
   }
-  float t1 = 0, t2 = 0;
-  MMI_HIP(hipEventElapsedTime(&t1, ctx->ev[0], ctx->ev[1]));
-  ctx->last_pass1_ms = t1;
-  ctx->last_pass2_ms = 0;
-  if (want_derivative) {
-    MMI_HIP(hipEventElapsedTime(&t2, ctx->ev[2], ctx->ev[3]));
-    ctx->last_pass2_ms = t2;
+  ctx->last_pass1_ms = ctx->last_pass2_ms = -1.0f;
+  if (time_passes) {
+    float t1 = 0, t2 = 0;
+    MMI_HIP(hipEventElapsedTime(&t1, ctx->ev[0], ctx->ev[1]));
+    ctx->last_pass1_ms = t1;
+    ctx->last_pass2_ms = 0;
+    if (want_derivative) {
+      MMI_HIP(hipEventElapsedTime(&t2, ctx->ev[2], ctx->ev[3]));
+      ctx->last_pass2_ms = t2;
+    }
   }
   float tt = 0;
   MMI_HIP(hipEventElapsedTime(&tt, ctx->ev[0], want_derivative ? ctx->ev[3] : ctx->ev[1]));
@@ -1617,7 +1630,10 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
   b->last_fused = nf;
   // uploads made on the contexts' own streams (tables, parameters) complete
   // before the batch stream reads them
-  for (int k = 0; k < nf; ++k) MMI_HIP(hipStreamSynchronize(ctxs[fused[k]]->stream));
+  for (int k = 0; k < nf; ++k) {
+    MMI_HIP(hipStreamSynchronize(ctxs[fused[k]]->stream));
+    ctxs[fused[k]]->acc_clean = false;  // the batch clears and uses them on its own stream
+  }
   const int stride = 16;  // value, count, status, up to 12 derivative entries (+ pad)
   if (nf > 0) {
     const size_t o_start = 0;
@@ -1666,7 +1682,6 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
       std::memset(&it, 0, sizeof(it));
       it.result = dev_result(c);
       it.deriv = want_derivative ? dev_deriv(c) : nullptr;
-      it.overflow = fa[k].exc_overflow;
       it.P = (int32_t)P;
       std::memcpy(hs + o_it + sizeof(BatchOut) * k, &it, sizeof(it));
     }

@@ -636,7 +636,7 @@ __global__ void batch_gather(const BatchOut* __restrict__ items, double* __restr
   if (threadIdx.x == 0) {
     o[0] = it.result->value;
     o[1] = (double)it.result->n_valid;
-    o[2] = (double)(it.result->status + (*it.overflow ? 16 : 0));
+    o[2] = (double)(it.result->status + (it.result->exc_overflow ? 16 : 0));
   }
   if (it.deriv)
     for (int i = threadIdx.x; i < it.P; i += blockDim.x) o[3 + i] = it.deriv[i];

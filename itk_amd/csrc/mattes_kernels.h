@@ -16,7 +16,7 @@ constexpr int kStorageF16 = 3;
 struct DevResult {
   unsigned long long n_valid;
   int32_t status;  // 0 ok, 1 no valid sample, 2 joint PDF summed to zero
-  int32_t pad_;
+  int32_t exc_overflow;  // fast path: a pass-1 workgroup listed more samples than it holds
   double value;
   double joint_pdf_sum;
   double n_factor;  // -1 / (moving bin size * NValid)
@@ -144,9 +144,11 @@ struct AxisTable {
 struct FinalizeArgs {
   int32_t bins;
   int32_t lds;  // set by launch_finalize: PDF and marginals staged in LDS
-  const unsigned long long* hist;
-  const unsigned long long* fmarg;
-  const unsigned long long* nvalid;
+  // pass 1's accumulators: read, then left zeroed for the next evaluation
+  unsigned long long* hist;
+  unsigned long long* fmarg;
+  unsigned long long* nvalid;
+  uint32_t* overflow;  // exception-list overflow word: moved into the result, zeroed
   double* pdf;     // [B*B] normalised joint PDF
   double* fm;      // [B]
   double* mm;      // [B]
@@ -170,7 +172,6 @@ struct BatchDeriv {
 struct BatchOut {
   const DevResult* result;
   const double* deriv;  // NULL for value-only
-  const uint32_t* overflow;
   int32_t P;
   int32_t pad_;
 };

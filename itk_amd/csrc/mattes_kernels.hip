@@ -115,6 +115,7 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& f) {
   __shared__ unsigned long long redu[kFinalizeThreads / 64];
   __shared__ double reds[kFinalizeThreads / 64];
   __shared__ double s_norm;
+  __shared__ unsigned long long s_nvalid;
   const int B = f.bins;
   const int BB = B * B;
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6, nw = blockDim.x >> 6;
@@ -128,10 +129,21 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& f) {
   part = wave_sum(part);
   if (lane == 0) redu[wid] = part;
   __syncthreads();
+  // pass 1's accumulators are left zeroed for the next evaluation once read
+  auto release = [&]() {
+    for (int i = t; i < BB; i += blockDim.x) f.hist[i] = 0ull;
+    for (int i = t; i < B; i += blockDim.x) f.fmarg[i] = 0ull;
+    if (t == 0) {
+      *f.nvalid = 0ull;
+      *f.overflow = 0u;
+    }
+  };
   if (t == 0) {
     unsigned long long total = 0;
     for (int w = 0; w < nw; ++w) total += redu[w];
     const unsigned long long nvalid = *f.nvalid;
+    s_nvalid = nvalid;
+    f.result->exc_overflow = *f.overflow ? 1 : 0;
     const double jointPDFSum = (double)total / kFixedPointScale;
     f.result->n_valid = nvalid;
     f.result->joint_pdf_sum = jointPDFSum;
@@ -140,7 +152,10 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& f) {
   }
   __syncthreads();
   const double nf = s_norm;
-  if (nf == 0.0) return;
+  if (nf == 0.0) {
+    release();
+    return;
+  }
   const double closeToZero = 2.220446049250313e-16;
   for (int i = t; i < BB; i += blockDim.x) {
     const double v = ((double)f.hist[i] / kFixedPointScale) * nf;
@@ -154,6 +169,7 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& f) {
     lrow[i] = fmv > closeToZero ? log(fmv) : 0.0;  // log of the fixed marginal, once per row
   }
   __syncthreads();
+  release();
   // moving marginal: column sums in the reference's row order (Mattes:405-417)
   for (int m = t; m < B; m += blockDim.x) {
     double acc = 0.0;
@@ -184,7 +200,7 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& f) {
     double sum = 0.0;
     for (int w = 0; w < nw; ++w) sum += reds[w];
     f.result->value = -1.0 * sum;
-    f.result->n_factor = -1.0 / (f.moving_bin_size * (double)*f.nvalid);
+    f.result->n_factor = -1.0 / (f.moving_bin_size * (double)s_nvalid);
   }
 }
 

@@ -39,11 +39,12 @@ def transform_fixed_parameters(tf) -> np.ndarray:
 class MattesMutualInformationImageToImageMetricv4:
     """Dense-sampling Mattes MI metric evaluated on one HIP device."""
 
-    def __init__(self, device: int = 0, record_sample_bins: bool = False, exact_path_only: bool = False):
+    def __init__(self, device: int = 0, record_sample_bins: bool = False, exact_path_only: bool = False,
+                 time_passes: bool = False):
         self._L = N.load()
         self._ctx = ctypes.c_void_p()
         flags = (N.MMI_FLAG_RECORD_SAMPLE_BINS if record_sample_bins else 0) | (
-            N.MMI_FLAG_EXACT_PATH_ONLY if exact_path_only else 0)
+            N.MMI_FLAG_EXACT_PATH_ONLY if exact_path_only else 0) | (N.MMI_FLAG_TIME_PASSES if time_passes else 0)
         N.check(self._L.mmi_context_create(device, flags, ctypes.byref(self._ctx)))
         self._bins = 50
         self._fixed = self._moving = None
@@ -294,4 +295,5 @@ class MattesMutualInformationImageToImageMetricv4:
     def GetLastTiming(self) -> dict:
         t = [ctypes.c_float() for _ in range(3)]
         N.check(self._L.mmi_get_last_timing(self._ctx, *[ctypes.byref(v) for v in t]))
+        # pass1_ms / pass2_ms are -1 unless the metric was made with time_passes
         return {"total_ms": t[0].value, "pass1_ms": t[1].value, "pass2_ms": t[2].value}
