@@ -48,8 +48,10 @@ SIGNATURES = {
     "mmi_initialize": (ctypes.c_int, [_vp]),
     "mmi_set_fixed_transform": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, ctypes.c_int32, _dp, ctypes.c_int64]),
     "mmi_set_moving_transform": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, ctypes.c_int32, _dp, ctypes.c_int64]),
-    "mmi_set_moving_parameters": (ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
-    "mmi_evaluate": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp, ctypes.POINTER(ctypes.c_int64), _dp, _dp]),
+    # per-evaluation calls take plain addresses (ints): building typed ctypes
+    # pointers costs microseconds per call
+    "mmi_set_moving_parameters": (ctypes.c_int, [_vp, _vp, ctypes.c_int64]),
+    "mmi_evaluate": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp]),
     "mmi_get_sample_bins": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int16), ctypes.POINTER(ctypes.c_int16)]),
     "mmi_get_moving_texels": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "mmi_get_histogram_parameters": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),

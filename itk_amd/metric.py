@@ -47,6 +47,8 @@ class MattesMutualInformationImageToImageMetricv4:
             N.MMI_FLAG_EXACT_PATH_ONLY if exact_path_only else 0) | (N.MMI_FLAG_TIME_PASSES if time_passes else 0)
         N.check(self._L.mmi_context_create(device, flags, ctypes.byref(self._ctx)))
         self._bins = 50
+        self._out = np.zeros(2)  # [value (double), valid count (int64 bits)]
+        self._out_addr = self._out.ctypes.data
         self._fixed = self._moving = None
         self._virtual = None
         self._moving_tf = None
@@ -221,26 +223,27 @@ class MattesMutualInformationImageToImageMetricv4:
 
     def _push_parameters(self) -> np.ndarray:
         p = np.ascontiguousarray(self._moving_tf.get_parameters(), dtype=np.float64)
-        N.check(self._L.mmi_set_moving_parameters(self._ctx, _dptr(p), p.size))
+        N.check(self._L.mmi_set_moving_parameters(self._ctx, p.ctypes.data, p.size))
         return p
 
     def _evaluate(self, want_derivative: bool, want_pdf: bool = False):
         if not self._initialized:
             raise N.MetricError(N.MMI_ERR_ARGUMENT, "Initialize() must be called first")
         p = self._push_parameters()
-        value = ctypes.c_double()
-        nvalid = ctypes.c_int64()
+        out = self._out  # value and valid count, reused across calls
+        out[:] = 0.0
         deriv = np.zeros(p.size) if want_derivative else None
         pdf = np.zeros(self._bins * self._bins) if want_pdf else None
-        rc = self._L.mmi_evaluate(self._ctx, 1 if want_derivative else 0, ctypes.byref(value),
-                                  _dptr(deriv) if deriv is not None else None, ctypes.byref(nvalid),
-                                  _dptr(pdf) if pdf is not None else None, None)
-        self._n_valid = nvalid.value
+        rc = self._L.mmi_evaluate(self._ctx, 1 if want_derivative else 0, self._out_addr,
+                                  deriv.ctypes.data if deriv is not None else None, self._out_addr + 8,
+                                  pdf.ctypes.data if pdf is not None else None, None)
+        value = float(out[0])
+        self._n_valid = int(out[1:2].view(np.int64)[0])
         N.check(rc)
-        self._value = value.value
+        self._value = value
         if pdf is not None:
             self._last_pdf = pdf.reshape(self._bins, self._bins)
-        return value.value, deriv
+        return value, deriv
 
     def GetValue(self) -> float:
         return self._evaluate(False)[0]
