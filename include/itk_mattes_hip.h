@@ -112,14 +112,45 @@ int mmi_set_smoothing_sigmas(mmi_context* ctx, const double* fixed_sigma, const 
  * If never called, the fixed image's geometry is used. */
 int mmi_set_virtual_domain(mmi_context* ctx, const mmi_geometry* g);
 
+/* Sparse sampling over a virtual-space point set
+ * (ImageToImageMetricv4::SetVirtualSampledPointSet with
+ * SetUseSampledPointSet(true) and SetUseVirtualSampledPointSet(true),
+ * itkImageToImageMetricv4.h:100-101,484; the sparse threader iterates the
+ * points in order, itkImageToImageMetricv4GetValueAndDerivativeThreader.hxx:44-63).
+ * points: n x dim doubles (x, y[, z]) in point order; n = 0 returns to dense
+ * sampling over the virtual domain.  The fixed intensity range is then taken
+ * over the fixed pixels the points round to after the fixed transform, and the
+ * fixed mask does not enter it (itkMattesMutualInformationImageToImageMetricv4.hxx:102-152).
+ * Sample i of mmi_get_sample_bins is point i.  Takes effect at the next
+ * mmi_initialize. */
+int mmi_set_virtual_points(mmi_context* ctx, int32_t dim, const double* points, int64_t n);
+
+/* Fixed and moving image masks (ImageToImageMetricv4::SetFixedImageMask /
+ * SetMovingImageMask, itkImageToImageMetricv4.h:420-438) as
+ * ImageMaskSpatialObject with its default settings: a point is inside when its
+ * rounded index (RoundHalfIntegerUp) lies in the mask's buffer and the mask
+ * pixel there is non-zero (itkImageMaskSpatialObject.hxx:36-48).  The fixed
+ * mask is tested at the fixed-space point, the moving mask at the mapped
+ * moving point (itkImageToImageMetricv4.hxx:298-309,351-362); without a point
+ * set, the fixed (moving) intensity range is taken over the voxels whose
+ * physical points lie inside the fixed (moving) mask
+ * (itkMattesMutualInformationImageToImageMetricv4.hxx:155-220).  g: the mask
+ * image's geometry, pixels: one byte per voxel in x-fastest order; g == NULL
+ * removes the mask.  Takes effect at the next mmi_initialize. */
+int mmi_set_fixed_mask(mmi_context* ctx, const mmi_geometry* g, const uint8_t* pixels);
+int mmi_set_moving_mask(mmi_context* ctx, const mmi_geometry* g, const uint8_t* pixels);
+
 /* MattesMutualInformationImageToImageMetricv4::SetNumberOfHistogramBins
  * (clamped to >= 5, itkMattesMutualInformationImageToImageMetricv4.h:146). */
 int mmi_set_number_of_histogram_bins(mmi_context* ctx, int32_t bins);
 
 /* MattesMutualInformationImageToImageMetricv4::Initialize
  * (itkMattesMutualInformationImageToImageMetricv4.hxx:70-327): intensity
- * ranges over the whole buffers, degenerate-range errors with the
- * reference's messages, bin size and normalised minimum. */
+ * ranges (whole buffers, masked voxels or sampled points, as set above),
+ * degenerate-range errors with the reference's messages, bin size and
+ * normalised minimum.  MMI_ERR_DEGENERATE when the point set or a mask leaves
+ * no pixel to take a range over (the reference goes on with an inverted
+ * range). */
 int mmi_initialize(mmi_context* ctx);
 
 /* Fixed transform (identity by default) and moving transform.

@@ -300,7 +300,22 @@ struct mmi_context {
   float last_total_ms = 0, last_pass1_ms = 0, last_pass2_ms = 0;
   int32_t last_path = 0;  // 0 exact, 1 fast, 2 fast redone exactly
   bool acc_clean = false;  // pass-1 accumulators known to be zero (finalize zeroes them)
+  // sparse sampling: virtual-space points (n_points > 0) instead of the voxels
+  int64_t n_points = 0;
+  int32_t points_dim = 0;
+  DevBuf points;  // n_points x points_dim doubles
+  // image masks [0] fixed, [1] moving: pixels and the DevMask the kernels read
+  bool have_mask[2] = {false, false};
+  mmi_geometry mask_g[2] = {};
+  DevBuf mask_px[2], mask_desc[2];
 };
+
+namespace {
+// Samples of one evaluation: the listed points, or the virtual voxels.
+int64_t sample_count(const mmi_context* ctx) {
+  return ctx->n_points > 0 ? ctx->n_points : num_pixels(ctx->virtual_g);
+}
+}  // namespace
 
 namespace {
 
@@ -573,6 +588,8 @@ bool setup_fast(mmi_context* ctx, FastArgs* F) {
   const int D = ctx->fixed_g.dim;
   const int B = ctx->bins;
   if (num_pixels(ctx->virtual_g) >= (1ll << 31)) return false;
+  // point sets and masks run on the exact path
+  if (ctx->n_points > 0 || ctx->have_mask[0] || ctx->have_mask[1]) return false;
   // the fast kernels address the images with 32-bit byte offsets
   if (num_pixels(ctx->moving_g) * (int64_t)sizeof(float4) >= (1ll << 32) ||
       num_pixels(ctx->fixed_g) * (int64_t)sizeof(float) >= (1ll << 32))
@@ -785,6 +802,25 @@ int device_minmax(mmi_context* ctx, const void* px, int storage, int64_t n, doub
   return MMI_OK;
 }
 
+int device_minmax_select(mmi_context* ctx, const void* px, const mmi_geometry& g, const DevMask* mask,
+                         const double* points, int64_t n, double* lo, double* hi) {
+  const int blocks = (int)std::min<int64_t>(1024, (n + 255) / 256);
+  MMI_HIP(ctx->scratch.ensure(sizeof(float) * 2 * blocks));
+  MMI_HIP(launch_minmax_select(px, ctx->active_storage, to_dev(g), mask, points, ctx->dftf, n, ctx->scratch.as<float>(),
+                               blocks, ctx->stream));
+  std::vector<float> h(2 * blocks);
+  MMI_HIP(hipMemcpyAsync(h.data(), ctx->scratch.p, sizeof(float) * 2 * blocks, hipMemcpyDeviceToHost, ctx->stream));
+  MMI_HIP(hipStreamSynchronize(ctx->stream));
+  float a = std::numeric_limits<float>::max(), b = -std::numeric_limits<float>::max();
+  for (int i = 0; i < blocks; ++i) {
+    a = std::min(a, h[2 * i]);
+    b = std::max(b, h[2 * i + 1]);
+  }
+  *lo = a;
+  *hi = b;
+  return MMI_OK;
+}
+
 // Axis tables of the separable B-spline derivative for this virtual domain and
 // grid, rebuilt when either changes (they do not depend on the coefficients).
 int bspline_axis_tables(mmi_context* ctx, const DevGeom& virt, const DevGeom& grid, int D, AxisTable* tabs,
@@ -861,6 +897,7 @@ int bspline_axis_tables(mmi_context* ctx, const DevGeom& virt, const DevGeom& gr
 int bspline_separable(mmi_context* ctx, const DevGeom& virt, int D, AxisTable* tabs, bool* usable) {
   *usable = false;
   if (ctx->mtf.kind != MMI_TF_BSPLINE || ctx->dmtf.grid.dim != D) return MMI_OK;
+  if (ctx->n_points > 0) return MMI_OK;  // samples are not the virtual grid's voxels
   if (!is_diagonal(ctx->virtual_g.index_to_physical, D) || !is_diagonal(ctx->dmtf.grid.p2i, D)) return MMI_OK;
   return bspline_axis_tables(ctx, virt, ctx->dmtf.grid, D, tabs, usable);
 }
@@ -908,7 +945,8 @@ int mmi_context_destroy(mmi_context* ctx) {
   DevBuf* bufs[] = {&ctx->fixed_px, &ctx->moving_px, &ctx->texels, &ctx->fixed_lp, &ctx->moving_lp, &ctx->texels_lp,
                     &ctx->fixed_work, &ctx->moving_work,
                     &ctx->coeffs, &ctx->hist, &ctx->fmarg,
-                    &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->out,
+                    &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->out, &ctx->points, &ctx->mask_px[0], &ctx->mask_px[1],
+                    &ctx->mask_desc[0], &ctx->mask_desc[1],
                     &ctx->partials, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
                     &ctx->exc_count, &ctx->exc_list, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->ftab, &ctx->work_a,
                     &ctx->work_b, &ctx->work_grad};
@@ -1064,6 +1102,54 @@ int mmi_set_virtual_domain(mmi_context* ctx, const mmi_geometry* g) {
   return MMI_OK;
 }
 
+int mmi_set_virtual_points(mmi_context* ctx, int32_t dim, const double* points, int64_t n) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (n < 0 || (n > 0 && !points)) return fail(MMI_ERR_ARGUMENT, "points is NULL or n < 0");
+  if (n > 0 && dim != 2 && dim != 3) return fail(MMI_ERR_ARGUMENT, "points must be 2-D or 3-D");
+  if (n >= (1ll << 31)) return fail(MMI_ERR_ARGUMENT, "at most 2^31 - 1 points");
+  MMI_HIP(hipSetDevice(ctx->device));
+  if (n > 0) {
+    MMI_HIP(ctx->points.ensure(sizeof(double) * dim * n));
+    MMI_HIP(hipMemcpy(ctx->points.p, points, sizeof(double) * dim * n, hipMemcpyHostToDevice));
+  }
+  ctx->n_points = n;
+  ctx->points_dim = n > 0 ? dim : 0;
+  ctx->initialized = false;
+  return MMI_OK;
+}
+
+static int set_mask(mmi_context* ctx, int which, const mmi_geometry* g, const uint8_t* pixels) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  ctx->initialized = false;
+  if (!g) {  // no geometry: remove the mask
+    ctx->have_mask[which] = false;
+    return MMI_OK;
+  }
+  if (!pixels) return fail(MMI_ERR_ARGUMENT, "mask pixels are NULL");
+  if (int rc = check_geometry(g)) return rc;
+  const int64_t n = num_pixels(*g);
+  if (n >= (1ll << 31)) return fail(MMI_ERR_ARGUMENT, "masks of 2^31 or more pixels are not supported");
+  MMI_HIP(hipSetDevice(ctx->device));
+  MMI_HIP(ctx->mask_px[which].ensure(n));
+  MMI_HIP(hipMemcpy(ctx->mask_px[which].p, pixels, n, hipMemcpyHostToDevice));
+  DevMask m;
+  m.g = to_dev(*g);
+  m.px = ctx->mask_px[which].as<uint8_t>();
+  MMI_HIP(ctx->mask_desc[which].ensure(sizeof(DevMask)));
+  MMI_HIP(hipMemcpy(ctx->mask_desc[which].p, &m, sizeof(DevMask), hipMemcpyHostToDevice));
+  ctx->mask_g[which] = *g;
+  ctx->have_mask[which] = true;
+  return MMI_OK;
+}
+
+int mmi_set_fixed_mask(mmi_context* ctx, const mmi_geometry* g, const uint8_t* pixels) {
+  return set_mask(ctx, 0, g, pixels);
+}
+
+int mmi_set_moving_mask(mmi_context* ctx, const mmi_geometry* g, const uint8_t* pixels) {
+  return set_mask(ctx, 1, g, pixels);
+}
+
 int mmi_set_smoothing_sigmas(mmi_context* ctx, const double* fixed_sigma, const double* moving_sigma) {
   if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
   if (!ctx->have_fixed || !ctx->have_moving) return fail(MMI_ERR_ARGUMENT, "set the fixed and moving images first");
@@ -1096,6 +1182,11 @@ int mmi_initialize(mmi_context* ctx) {
   if (!ctx->have_fixed) return fail(MMI_ERR_ARGUMENT, "FixedImage is not present");
   if (!ctx->have_moving) return fail(MMI_ERR_ARGUMENT, "MovingImage is not present");
   if (ctx->fixed_g.dim != ctx->moving_g.dim) return fail(MMI_ERR_ARGUMENT, "fixed and moving dimensions differ");
+  if (ctx->n_points > 0 && ctx->points_dim != ctx->fixed_g.dim)
+    return fail(MMI_ERR_ARGUMENT, "point set and image dimensions differ");
+  for (int k = 0; k < 2; ++k)
+    if (ctx->have_mask[k] && ctx->mask_g[k].dim != ctx->fixed_g.dim)
+      return fail(MMI_ERR_ARGUMENT, "mask and image dimensions differ");
   MMI_HIP(hipSetDevice(ctx->device));
   // Per-level smoothing of the fixed and moving images
   // (ImageRegistrationMethodv4::InitializeRegistrationAtEachLevel,
@@ -1133,12 +1224,31 @@ int mmi_initialize(mmi_context* ctx) {
     }
     ctx->active_storage = ctx->storage;
   }
-  if (int rc = device_minmax(ctx, fixed_pixels(ctx), ctx->active_storage, num_pixels(ctx->fixed_g), &ctx->fixed_min,
-                             &ctx->fixed_max))
+  // Mattes:105-220: the fixed range over the sampled points when sparse
+  // sampling is on (masks then do not matter), else over the masked voxels;
+  // the moving range over its masked voxels
+  if (ctx->n_points > 0) {
+    if (int rc = device_minmax_select(ctx, fixed_pixels(ctx), ctx->fixed_g, nullptr, ctx->points.as<double>(),
+                                      ctx->n_points, &ctx->fixed_min, &ctx->fixed_max))
+      return rc;
+  } else if (ctx->have_mask[0]) {
+    if (int rc = device_minmax_select(ctx, fixed_pixels(ctx), ctx->fixed_g, ctx->mask_desc[0].as<DevMask>(), nullptr,
+                                      num_pixels(ctx->fixed_g), &ctx->fixed_min, &ctx->fixed_max))
+      return rc;
+  } else if (int rc = device_minmax(ctx, fixed_pixels(ctx), ctx->active_storage, num_pixels(ctx->fixed_g),
+                                    &ctx->fixed_min, &ctx->fixed_max)) {
     return rc;
-  if (int rc = device_minmax(ctx, moving_pixels(ctx), ctx->active_storage, num_pixels(ctx->moving_g),
-                             &ctx->moving_min, &ctx->moving_max))
+  }
+  if (ctx->have_mask[1]) {
+    if (int rc = device_minmax_select(ctx, moving_pixels(ctx), ctx->moving_g, ctx->mask_desc[1].as<DevMask>(), nullptr,
+                                      num_pixels(ctx->moving_g), &ctx->moving_min, &ctx->moving_max))
+      return rc;
+  } else if (int rc = device_minmax(ctx, moving_pixels(ctx), ctx->active_storage, num_pixels(ctx->moving_g),
+                                    &ctx->moving_min, &ctx->moving_max)) {
     return rc;
+  }
+  if (ctx->fixed_min > ctx->fixed_max || ctx->moving_min > ctx->moving_max)
+    return fail(MMI_ERR_DEGENERATE, "no fixed or moving voxel (sampled point) inside the analysis region");
   if (!std::isfinite(ctx->fixed_min) || !std::isfinite(ctx->fixed_max) || !std::isfinite(ctx->moving_min) ||
       !std::isfinite(ctx->moving_max))
     return fail(MMI_ERR_ARGUMENT, "image intensities are not finite in the storage format (FP16 holds |v| <= 65504)");
@@ -1178,7 +1288,7 @@ int mmi_initialize(mmi_context* ctx) {
   MMI_HIP(ctx->mm.ensure(sizeof(double) * B));
   MMI_HIP(ctx->rowsum.ensure(sizeof(double) * B));
   MMI_HIP(ctx->out.ensure(out_bytes((int64_t)ctx->mtf.params.size())));
-  const int64_t n = num_pixels(ctx->virtual_g);
+  const int64_t n = sample_count(ctx);
   // a multiple of 8: the fast kernels split the samples into one share per XCD
   const int64_t want = std::min<int64_t>((n + 255) / 256, (int64_t)ctx->cu_count * ctx->blocks_per_cu);
   ctx->grid = (int)std::max<int64_t>(8, (want + 7) / 8 * 8);
@@ -1259,7 +1369,7 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
   MMI_HIP(hipSetDevice(ctx->device));
   const int D = ctx->fixed_g.dim;
   const int B = ctx->bins;
-  const int64_t n = num_pixels(ctx->virtual_g);
+  const int64_t n = sample_count(ctx);
   const int64_t P = (int64_t)ctx->mtf.params.size();
   MMI_HIP(ctx->out.ensure(out_bytes(P)));
   std::memset(&a, 0, sizeof(a));
@@ -1273,6 +1383,9 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
   a.texels = texel_pixels(ctx);
   a.storage = ctx->active_storage;
   a.n = n;
+  a.points = ctx->n_points > 0 ? ctx->points.as<double>() : nullptr;
+  a.fixed_mask = ctx->have_mask[0] ? ctx->mask_desc[0].as<DevMask>() : nullptr;
+  a.moving_mask = ctx->have_mask[1] ? ctx->mask_desc[1].as<DevMask>() : nullptr;
   a.bins = B;
   a.hist_in_lds = (size_t)B * B * 8 + (B + 1) * 4 <= 64 * 1024 ? 1 : 0;
   a.gradient_source = ctx->gradient_source;
@@ -1367,7 +1480,7 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
   if (int rc = prepare_eval(ctx, want_derivative, a, fa, fast)) return rc;
   const int D = ctx->fixed_g.dim;
   const int B = ctx->bins;
-  const int64_t n = num_pixels(ctx->virtual_g);
+  const int64_t n = sample_count(ctx);
   const int64_t P = (int64_t)ctx->mtf.params.size();
   hipStream_t st = ctx->stream;
   // An event record costs the queue a few microseconds (a timestamp with a
@@ -1945,7 +2058,7 @@ int mmi_get_sample_bins(mmi_context* ctx, int16_t* fixed_bin, int16_t* moving_bi
   if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
   if (!(ctx->flags & 1u)) return fail(MMI_ERR_ARGUMENT, "create the context with flag 1 to record sample bins");
   if (!ctx->initialized) return fail(MMI_ERR_ARGUMENT, "not initialized");
-  const int64_t n = num_pixels(ctx->virtual_g);
+  const int64_t n = sample_count(ctx);
   MMI_HIP(hipSetDevice(ctx->device));
   if (fixed_bin)
     MMI_HIP(hipMemcpyAsync(fixed_bin, ctx->sample_fbin.p, sizeof(int16_t) * n, hipMemcpyDeviceToHost, ctx->stream));

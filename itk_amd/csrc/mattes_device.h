@@ -91,6 +91,13 @@ __device__ __forceinline__ int32_t itk_floor(double x) {
   return ((int32_t)__builtin_rint(2.0 * x - 0.5)) >> 1;
 }
 
+// Math::RoundHalfIntegerUp<int64>(double) = RoundHalfIntegerToEven(2x + 0.5)
+// >> 1 (itkMathDetail.h:232-235, the form of x86-64 builds).  Callers pass
+// |x| < 2^30.
+__device__ __forceinline__ int32_t itk_round(double x) {
+  return ((int32_t)__builtin_rint(2.0 * x + 0.5)) >> 1;
+}
+
 // BSplineKernelFunction<3> (Modules/Core/Common/include/itkBSplineKernelFunction.h:153-175).
 __device__ __forceinline__ double bspline3(double u) {
   const double a = fabs(u);
@@ -140,6 +147,41 @@ __device__ __forceinline__ void point_to_cindex(const DevGeom& g, const double* 
     for (int c = 0; c < D; ++c) s += g.p2i[r * D + c] * v[c];
     ci[r] = s;
   }
+}
+
+// An image mask as ImageMaskSpatialObject::IsInsideInObjectSpace sees a
+// point (Modules/Core/SpatialObjects/include/itkImageMaskSpatialObject.hxx:36-48,
+// identity object-to-world transform): the rounded index
+// (ImageBase::TransformPhysicalPointToIndex, itkImageBase.h:465-479) inside
+// the mask's buffer and the mask pixel there not 0.
+struct DevMask {
+  DevGeom g;
+  const uint8_t* px;
+};
+
+// ImageBase::TransformPhysicalPointToIndex(point, index) (itkImageBase.h:465-499):
+// whether the rounded index is in the buffer, and its buffer offset.
+template <int D>
+__device__ __forceinline__ bool rounded_index_inside(const DevGeom& g, const double* p, int32_t* off) {
+  double ci[3];
+  point_to_cindex<D>(g, p, ci);
+  int32_t k[3] = {0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    // indices outside [start - 1, end + 1] round outside the buffer (and stay
+    // in range for the rounding); NaN fails the test
+    if (!(ci[i] > (double)g.start[i] - 1.0 && ci[i] < (double)g.end[i] + 1.0)) return false;
+    k[i] = itk_round(ci[i]);
+    if (k[i] < g.start[i] || k[i] > g.end[i]) return false;
+  }
+  *off = (k[0] - g.start[0]) + g.sy * (k[1] - g.start[1]) + (D == 3 ? g.sz * (k[2] - g.start[2]) : 0);
+  return true;
+}
+
+template <int D>
+__device__ __forceinline__ bool mask_inside(const DevMask& m, const double* p) {
+  int32_t off = 0;
+  return rounded_index_inside<D>(m.g, p, &off) && m.px[off] != 0;
 }
 
 // ImageFunction::IsInsideBuffer(ContinuousIndex) (itkImageFunction.h:159-170).

@@ -29,6 +29,21 @@ __device__ __forceinline__ void fixed_point(const DevTransform& t, const double*
   }
 }
 
+// Virtual point of sample s: a listed point (sparse threader,
+// itkImageToImageMetricv4GetValueAndDerivativeThreader.hxx:44-63) or the
+// physical point of voxel s (dense threader, same file :26-42).
+template <int D>
+__device__ __forceinline__ void virtual_point(const EvalArgs& a, uint32_t s, double* vp) {
+  if (a.points) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) vp[k] = a.points[(int64_t)s * D + k];
+  } else {
+    int32_t idx[3];
+    sample_index<D>(a.virt, s, idx);
+    index_to_point<D>(a.virt, idx, vp);
+  }
+}
+
 // ProcessVirtualPoint + Mattes ProcessPoint up to the bins
 // (itkImageToImageMetricv4GetValueAndDerivativeThreaderBase.hxx:174-273,
 // itkImageToImageMetricv4.hxx:289-373,
@@ -36,16 +51,21 @@ __device__ __forceinline__ void fixed_point(const DevTransform& t, const double*
 // exact FP64.  MK is the moving transform kind (1 affine, 2 B-spline); P the
 // image storage format.
 template <int D, int MK, typename P>
-__device__ __forceinline__ bool eval_sample(const EvalArgs& a, const int32_t* idx, SampleOut<D>& o) {
-  index_to_point<D>(a.virt, idx, o.vp);
+__device__ __forceinline__ bool eval_sample(const EvalArgs& a, const double* vp, SampleOut<D>& o) {
+#pragma unroll
+  for (int k = 0; k < D; ++k) o.vp[k] = vp[k];
   double fp[3], ci[3];
   fixed_point<D>(a.ftf, o.vp, fp);
+  // the fixed mask at the mapped fixed point (itkImageToImageMetricv4.hxx:298-309)
+  if (a.fixed_mask && !mask_inside<D>(*a.fixed_mask, fp)) return false;
   point_to_cindex<D>(a.fixed, fp, ci);
   if (!inside_buffer<D>(a.fixed, ci)) return false;
   const Cell fc = make_cell<D>(a.fixed, ci);
   const double fval = (double)(float)interp_exact<D, P>(static_cast<const typename P::pixel*>(a.fixed_px), fc);
 
   transform_point<D, MK>(a.mtf, o.vp, o.mp, &o.sup);
+  // the moving mask at the mapped moving point (itkImageToImageMetricv4.hxx:351-362)
+  if (a.moving_mask && !mask_inside<D>(*a.moving_mask, o.mp)) return false;
   point_to_cindex<D>(a.moving, o.mp, ci);
   if (!inside_buffer<D>(a.moving, ci)) return false;
   o.cell = make_cell<D>(a.moving, ci);

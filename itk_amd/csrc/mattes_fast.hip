@@ -227,10 +227,10 @@ __device__ __forceinline__ void pass1_exceptions(const EvalArgs& a, const uint32
   const int B = a.bins;
   for (uint32_t t = threadIdx.x; t < ne; t += blockDim.x) {
     const uint32_t s = list[t];
-    int32_t idx[3];
-    sample_index<D>(a.virt, s, idx);
+    double vp[3];
+    virtual_point<D>(a, s, vp);
     SampleOut<D> o;
-    const bool ok = eval_sample<D, MK, P>(a, idx, o);
+    const bool ok = eval_sample<D, MK, P>(a, vp, o);
     if (a.sample_fbin) {
       a.sample_fbin[s] = ok ? (int16_t)o.fbin : (int16_t)-1;
       a.sample_mbin[s] = ok ? (int16_t)o.mbin : (int16_t)-1;
@@ -345,10 +345,10 @@ __device__ __forceinline__ void exceptions_pass1_body(const EvalArgs& a, const F
   __syncthreads();
   for (uint32_t t = threadIdx.x; t < ne; t += blockDim.x) {
     const uint32_t s = f.exc_list[lb * kExcCap + t];
-    int32_t idx[3];
-    sample_index<D>(a.virt, s, idx);
+    double vp[3];
+    virtual_point<D>(a, s, vp);
     SampleOut<D> o;
-    const bool ok = eval_sample<D, MK, P>(a, idx, o);
+    const bool ok = eval_sample<D, MK, P>(a, vp, o);
     if (a.sample_fbin) {
       a.sample_fbin[s] = ok ? (int16_t)o.fbin : (int16_t)-1;
       a.sample_mbin[s] = ok ? (int16_t)o.mbin : (int16_t)-1;
@@ -409,11 +409,11 @@ __device__ __forceinline__ void exceptions_pass2_body(const EvalArgs& a, const F
     // sum here, so no ordering is needed)
     for (uint32_t t = threadIdx.x; t < ne; t += blockDim.x) {
       const uint32_t s = f.exc_list[lb * f.exc_cap + t];
-      int32_t idx[3];
-      sample_index<D>(a.virt, s, idx);
+      double vp[3];
+      virtual_point<D>(a, s, vp);
       SampleOut<D> o;
       float gw[3] = {0.0f, 0.0f, 0.0f};
-      if (eval_sample<D, 2, P>(a, idx, o) && o.sup.inside) {
+      if (eval_sample<D, 2, P>(a, vp, o) && o.sup.inside) {
         const float w = parzen_deriv_weight((float)((double)(o.mbin - 1) - o.mterm),
                                             f.pratio_f + o.fbin * a.bins + (o.mbin - 1));
         if (w != 0.0f) {
@@ -449,11 +449,14 @@ __device__ __forceinline__ void exceptions_pass2_body(const EvalArgs& a, const F
       }
     }
     if (threadIdx.x < ne) {
+      // the fast path samples the virtual voxels: the moments need the index
       const uint32_t s = keys[threadIdx.x];
       int32_t idx[3];
       sample_index<D>(a.virt, s, idx);
+      double vp[3];
+      index_to_point<D>(a.virt, idx, vp);
       SampleOut<D> o;
-      if (eval_sample<D, 1, P>(a, idx, o)) {
+      if (eval_sample<D, 1, P>(a, vp, o)) {
         const float w = parzen_deriv_weight((float)((double)(o.mbin - 1) - o.mterm),
                                             f.pratio_f + o.fbin * a.bins + (o.mbin - 1));
         if (w != 0.0f)

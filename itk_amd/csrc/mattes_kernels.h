@@ -103,6 +103,11 @@ struct EvalArgs {
   const void* moving_px;
   const void* texels;
   int64_t n;  // virtual samples
+  // sparse sampling: n virtual-space points (D doubles each) instead of the
+  // virtual voxels; NULL for dense sampling
+  const double* points;
+  const DevMask* fixed_mask;   // NULL: no mask
+  const DevMask* moving_mask;  // NULL: no mask
   int32_t storage;        // MMI_F32, MMI_BF16 or MMI_F16
   int32_t pad2_;
   int32_t bins;
@@ -286,6 +291,11 @@ hipError_t launch_recursive_gaussian(const void* in, bool in_float, double* out,
                                      int64_t nlines, const RecCoeffs& c, const RecSink& sink, hipStream_t st);
 hipError_t launch_clear_eval(unsigned long long* hist, int n_hist, unsigned long long* fmarg, int n_fmarg,
                              unsigned long long* nvalid, uint32_t* overflow, hipStream_t st);
+// Per-workgroup min/max over the masked voxels of an image (mask) or over the
+// pixels the listed virtual points round to after the fixed transform (points).
+hipError_t launch_minmax_select(const void* px, int storage, const DevGeom& img, const DevMask* mask,
+                                const double* points, const DevTransform& ftf, int64_t n, float* out, int blocks,
+                                hipStream_t st);
 hipError_t launch_pack_texels(const float* moving, const double* grad, int64_t n, int D, const Direction& dir,
                               float4* tex, hipStream_t st);
 

@@ -54,10 +54,10 @@ __global__ void __launch_bounds__(256) pass1_kernel(EvalArgs a) {
   const uint32_t n = (uint32_t)a.n;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
-    int32_t idx[3];
-    sample_index<D>(a.virt, s, idx);
+    double vp[3];
+    virtual_point<D>(a, s, vp);
     SampleOut<D> o;
-    const bool ok = eval_sample<D, MK, P>(a, idx, o);
+    const bool ok = eval_sample<D, MK, P>(a, vp, o);
     if (a.sample_fbin) {
       a.sample_fbin[s] = ok ? (int16_t)o.fbin : (int16_t)-1;
       a.sample_mbin[s] = ok ? (int16_t)o.mbin : (int16_t)-1;
@@ -277,10 +277,10 @@ __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
   const uint32_t n = (uint32_t)a.n;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += stride) {
-    int32_t idx[3];
-    sample_index<D>(a.virt, s, idx);
+    double vp[3];
+    virtual_point<D>(a, s, vp);
     SampleOut<D> o;
-    const bool ok = eval_sample<D, MK, P>(a, idx, o);
+    const bool ok = eval_sample<D, MK, P>(a, vp, o);
     float w = 0.0f;
     if (ok) w = parzen_deriv_weight((float)((double)(o.mbin - 1) - o.mterm), pr + o.fbin * B + (o.mbin - 1));
     double gd[3] = {0.0, 0.0, 0.0};

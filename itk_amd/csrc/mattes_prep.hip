@@ -2,7 +2,7 @@
 // v4 evaluation: intensity range, recursive-Gaussian gradient image, texel
 // packing, derivative reduction.  Compiled with -ffp-contract=off like the
 // evaluation kernels, so the gradient image rounds like the reference filter.
-#include "mattes_kernels.h"
+#include "mattes_eval.h"
 
 #include <hip/hip_runtime.h>
 
@@ -432,6 +432,77 @@ __global__ void __launch_bounds__(256) separable_rows_lds_kernel(const float* __
     double* dst = out + r0 * G;
     for (int e = threadIdx.x; e < outs; e += 256) dst[e] = ob[e];
   }
+}
+
+// Intensity range over part of an image (MattesMI::Initialize,
+// itkMattesMutualInformationImageToImageMetricv4.hxx:105-220): the voxels
+// whose physical points lie inside the mask, or (sparse sampling) the pixels
+// the listed virtual points round to after the fixed transform.  Per
+// workgroup min and max as minmax_kernel writes them.
+template <int D, typename P>
+__global__ void __launch_bounds__(256) minmax_select_kernel(const typename P::pixel* __restrict__ px, DevGeom img,
+                                                            const DevMask* __restrict__ mask,
+                                                            const double* __restrict__ points, DevTransform ftf,
+                                                            int64_t n, float* __restrict__ out) {
+  __shared__ float smin[256], smax[256];
+  float lo = 3.402823466e38f, hi = -3.402823466e38f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t off = 0;
+    double p[3] = {0.0, 0.0, 0.0};
+    if (points) {
+      double vp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < D; ++k) vp[k] = points[i * D + k];
+      fixed_point<D>(ftf, vp, p);
+      if (!rounded_index_inside<D>(img, p, &off)) continue;
+    } else {
+      int32_t idx[3];
+      sample_index<D>(img, (uint32_t)i, idx);
+      index_to_point<D>(img, idx, p);
+      if (!mask_inside<D>(*mask, p)) continue;
+      off = (int32_t)i;
+    }
+    const float v = P::ld(px, off);
+    lo = v < lo ? v : lo;
+    hi = v > hi ? v : hi;
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + w]);
+      smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = smin[0];
+    out[2 * blockIdx.x + 1] = smax[0];
+  }
+}
+
+hipError_t launch_minmax_select(const void* px, int storage, const DevGeom& img, const DevMask* mask,
+                                const double* points, const DevTransform& ftf, int64_t n, float* out, int blocks,
+                                hipStream_t st) {
+#define MMI_MINMAX_SELECT(PX)                                                                                    \
+  do {                                                                                                           \
+    const auto* p = static_cast<const typename PX::pixel*>(px);                                                  \
+    if (img.dim == 2)                                                                                            \
+      hipLaunchKernelGGL((minmax_select_kernel<2, PX>), dim3(blocks), dim3(256), 0, st, p, img, mask, points, ftf, \
+                         n, out);                                                                                \
+    else                                                                                                         \
+      hipLaunchKernelGGL((minmax_select_kernel<3, PX>), dim3(blocks), dim3(256), 0, st, p, img, mask, points, ftf, \
+                         n, out);                                                                                \
+  } while (0)
+  if (storage == kStorageBF16)
+    MMI_MINMAX_SELECT(PxBF16);
+  else if (storage == kStorageF16)
+    MMI_MINMAX_SELECT(PxF16);
+  else
+    MMI_MINMAX_SELECT(PxF32);
+#undef MMI_MINMAX_SELECT
+  return hipGetLastError();
 }
 
 hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, const DevResult* r, double* deriv,

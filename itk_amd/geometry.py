@@ -127,6 +127,28 @@ class ImageGeometry:
             p[i] = s
         return p
 
+    def physical_points_to_index(self, points) -> tuple:
+        """ImageBase::TransformPhysicalPointToIndex (itkImageBase.h:465-479)
+        for an (n, dim) array: per row, sum += M[i][j] * (p[j] - origin[j])
+        from zero, then Math::RoundHalfIntegerUp as built on x86-64
+        (round-to-even of 2x + 0.5, halved; itkMathDetail.h:382-385).
+        Returns (index (n, dim) int64, inside (n,) bool); an index too large
+        for int64 counts as outside."""
+        pts = np.asarray(points, dtype=np.float64).reshape(-1, self.dim)
+        d = self.dim
+        idx = np.zeros((pts.shape[0], d), dtype=np.int64)
+        inside = np.ones(pts.shape[0], dtype=bool)
+        with np.errstate(invalid="ignore"):
+            for i in range(d):
+                s = np.zeros(pts.shape[0])
+                for j in range(d):
+                    s = s + self.physical_to_index[i, j] * (pts[:, j] - self.origin[j])
+                ok = np.abs(s) < 1.0e18
+                r = np.where(ok, np.rint(2.0 * np.where(ok, s, 0.0) + 0.5), 0.0).astype(np.int64) >> 1
+                idx[:, i] = r
+                inside &= ok & (r >= self.start[i]) & (r < self.start[i] + self.size[i])
+        return idx, inside
+
     def center_point(self) -> np.ndarray:
         """Physical centre of the buffer (the usual affine centre)."""
         return self.index_to_point([(s - 1) / 2.0 for s in self.size])

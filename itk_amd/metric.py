@@ -37,7 +37,8 @@ def transform_fixed_parameters(tf) -> np.ndarray:
 
 
 class MattesMutualInformationImageToImageMetricv4:
-    """Dense-sampling Mattes MI metric evaluated on one HIP device."""
+    """Mattes MI metric evaluated on one HIP device: dense over the virtual
+    domain or sparse over a point set, with optional image masks."""
 
     def __init__(self, device: int = 0, record_sample_bins: bool = False, exact_path_only: bool = False,
                  time_passes: bool = False):
@@ -60,6 +61,13 @@ class MattesMutualInformationImageToImageMetricv4:
         self._last_pdf = None
         self._storage = N.MMI_F32
         self._fixed_sigma = self._moving_sigma = None
+        self._masks = [None, None]  # ImageMaskSpatialObject for fixed, moving
+        self._use_sampled_points = False
+        self._use_virtual_points = False
+        self._virtual_points = None
+        self._fixed_points = None
+        self._sample_points = None  # the virtual points evaluated (sparse sampling)
+        self._skipped_fixed_points = 0
         self._images_dirty = True    # host images not yet on the device
         self._gradient_dirty = True  # user gradient not yet on the device
         self._initialized = False
@@ -141,6 +149,93 @@ class MattesMutualInformationImageToImageMetricv4:
         self._virtual = geometry
         self._initialized = False
 
+    # -- masks and sparse sampling (itkImageToImageMetricv4.h:87-101,420-484) --
+    def SetFixedImageMask(self, mask):
+        """An ImageMaskSpatialObject (itk_amd.masks), or None for no mask."""
+        self._masks[0] = mask
+        self._initialized = False
+
+    def GetFixedImageMask(self):
+        return self._masks[0]
+
+    def SetMovingImageMask(self, mask):
+        self._masks[1] = mask
+        self._initialized = False
+
+    def GetMovingImageMask(self):
+        return self._masks[1]
+
+    def SetFixedSampledPointSet(self, points):
+        """Fixed-space points (n x dim); used when UseSampledPointSet is on and
+        UseVirtualSampledPointSet is off."""
+        self._fixed_points = None if points is None else np.array(points, dtype=np.float64)
+        self._initialized = False
+
+    def SetVirtualSampledPointSet(self, points):
+        self._virtual_points = None if points is None else np.array(points, dtype=np.float64)
+        self._initialized = False
+
+    def SetUseSampledPointSet(self, on: bool):
+        self._use_sampled_points = bool(on)
+        self._initialized = False
+
+    def GetUseSampledPointSet(self) -> bool:
+        return self._use_sampled_points
+
+    def SetUseVirtualSampledPointSet(self, on: bool):
+        self._use_virtual_points = bool(on)
+        self._initialized = False
+
+    def GetUseVirtualSampledPointSet(self) -> bool:
+        return self._use_virtual_points
+
+    def GetVirtualSampledPointSet(self):
+        return self._sample_points
+
+    def GetNumberOfSkippedFixedSampledPoints(self) -> int:
+        return self._skipped_fixed_points
+
+    def GetNumberOfDomainPoints(self) -> int:
+        """Samples per evaluation: the point set's size or the virtual voxels."""
+        if self._sample_points is not None:
+            return int(self._sample_points.shape[0])
+        return (self._virtual or self._fixed[1]).num_pixels
+
+    def _sampled_virtual_points(self):
+        """The virtual point set of ImageToImageMetricv4::Initialize
+        (itkImageToImageMetricv4.hxx:124-129): the user's virtual points, or
+        the fixed points mapped by the inverse fixed transform and kept when
+        their rounded virtual index is inside the virtual domain
+        (MapFixedSampledPointSetToVirtual, itkImageToImageMetricv4.hxx:567-611).
+        Fixed points are taken with the identity fixed transform only."""
+        d = self._fixed[1].dim
+        self._skipped_fixed_points = 0
+        if self._use_virtual_points:
+            if self._virtual_points is None:
+                raise N.MetricError(N.MMI_ERR_ARGUMENT, "VirtualSampledPointSet is not present")
+            return np.ascontiguousarray(self._virtual_points.reshape(-1, d))
+        if self._fixed_points is None:
+            raise N.MetricError(N.MMI_ERR_ARGUMENT, "Fixed Sample point set is empty.")
+        ftf = self._fixed_tf
+        if ftf is not None and not (ftf.kind == TF_AFFINE and np.array_equal(ftf.matrix, np.eye(d))
+                                    and not np.any(ftf.offset)):
+            raise NotImplementedError("fixed sampled point sets need the identity fixed transform")
+        pts = self._fixed_points.reshape(-1, d)
+        virt = self._virtual or self._fixed[1]
+        _, keep = virt.physical_points_to_index(pts)
+        # the reference takes the fixed range over every fixed point, the
+        # device over the kept ones: they agree unless a dropped point lies
+        # inside the fixed image
+        _, in_fixed = self._fixed[1].physical_points_to_index(pts[~keep])
+        if in_fixed.any():
+            raise NotImplementedError("fixed sampled points outside the virtual domain but inside the fixed image")
+        self._skipped_fixed_points = int((~keep).sum())
+        if not keep.any():
+            raise N.MetricError(N.MMI_ERR_ARGUMENT,
+                                "The virtual sampled point set has zero points because no fixed sampled points were "
+                                "within the virtual domain after mapping. There are no points to evaluate.")
+        return np.ascontiguousarray(pts[keep])
+
     def SetFixedTransform(self, tf):
         self._fixed_tf = tf
         self._initialized = False
@@ -191,6 +286,19 @@ class MattesMutualInformationImageToImageMetricv4:
                                            _dptr(ms) if ms is not None else None))
         cv = (self._virtual or self._fixed[1]).to_c()
         N.check(L.mmi_set_virtual_domain(ctx, ctypes.byref(cv)))
+        for setter, mask in ((L.mmi_set_fixed_mask, self._masks[0]), (L.mmi_set_moving_mask, self._masks[1])):
+            if mask is None:
+                N.check(setter(ctx, None, None))
+            else:
+                flags = np.ascontiguousarray(mask.inside_flags())
+                mg = mask.geometry.to_c()
+                N.check(setter(ctx, ctypes.byref(mg), flags.ctypes.data))
+        if self._use_sampled_points:
+            self._sample_points = self._sampled_virtual_points()
+            N.check(L.mmi_set_virtual_points(ctx, d, _dptr(self._sample_points), self._sample_points.shape[0]))
+        else:
+            self._sample_points = None
+            N.check(L.mmi_set_virtual_points(ctx, d, None, 0))
         if self._fixed_tf is not None:
             self._push_transform(L.mmi_set_fixed_transform, self._fixed_tf)
         self._push_transform(L.mmi_set_moving_transform, self._moving_tf)
@@ -267,8 +375,7 @@ class MattesMutualInformationImageToImageMetricv4:
 
     # -- diagnostics --------------------------------------------------------
     def GetSampleBins(self):
-        g = self._virtual or self._fixed[1]
-        n = g.num_pixels
+        n = self.GetNumberOfDomainPoints()
         fb = np.zeros(n, dtype=np.int16)
         mb = np.zeros(n, dtype=np.int16)
         N.check(self._L.mmi_get_sample_bins(self._ctx, fb.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)),

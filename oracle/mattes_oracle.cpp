@@ -23,6 +23,8 @@
 //   RecSep   = Modules/Filtering/ImageFilterBase/include/itkRecursiveSeparableImageFilter.hxx
 //   GradRG   = Modules/Filtering/ImageGradient/include/itkGradientRecursiveGaussianImageFilter.hxx
 //   Kahan    = Modules/Core/Common/include/itkCompensatedSummation.hxx
+//   Thr      = Modules/Registration/Metricsv4/include/itkImageToImageMetricv4GetValueAndDerivativeThreader.hxx
+//   MaskSO   = Modules/Core/SpatialObjects/include/itkImageMaskSpatialObject.hxx
 //
 // Parity pins (tests/test_oracle.py): the reference test's known metric
 // values and its finite-difference derivative check
@@ -123,6 +125,48 @@ inline bool inside_buffer(const orc_geometry* g, const double* ci) {
   }
   return true;
 }
+
+// Math::RoundHalfIntegerUp<int64>(double) as built on x86-64 (SSE2 rounding of
+// 2x + 0.5 to even, then a halving shift; itkMathDetail.h:382-385).
+inline int64_t itk_round(double x) {
+  return (int64_t)std::nearbyint(2.0 * x + 0.5) >> 1;
+}
+
+// ImageBase::TransformPhysicalPointToIndex (ImgBase:465-479: per row,
+// sum += M[i][j] * (p[j] - origin[j]) from zero, then RoundHalfIntegerUp) and
+// ImageRegion::IsInside(Index) on the image's region.  Indices whose doubled
+// value does not fit int64 lie outside every region.
+inline bool physical_point_to_index(const orc_geometry* g, const double* p, int64_t* idx) {
+  const int D = g->dim;
+  bool inside = true;
+  for (int i = 0; i < D; ++i) {
+    double sum = 0.0;
+    for (int j = 0; j < D; ++j) sum += g->physical_to_index[i * D + j] * (p[j] - g->origin[j]);
+    if (!(std::fabs(sum) < 1.0e18)) return false;
+    idx[i] = itk_round(sum);
+    inside = inside && idx[i] >= g->start[i] && idx[i] < g->start[i] + g->size[i];
+  }
+  return inside;
+}
+
+inline int64_t buffer_offset(const orc_geometry* g, const int64_t* idx) {
+  const int D = g->dim;
+  const int64_t z = D == 3 ? idx[2] - g->start[2] : 0;
+  return (idx[0] - g->start[0]) + g->size[0] * ((idx[1] - g->start[1]) + g->size[1] * z);
+}
+
+// ImageMaskSpatialObject::IsInsideInObjectSpace with UseMaskValue off
+// (MaskSO:36-48); the object-to-world transform is the
+// identity, so world and object points coincide (itkSpatialObject.hxx:186-190).
+struct Mask {
+  const orc_geometry* g = nullptr;
+  const uint8_t* px = nullptr;
+  bool inside(const double* p) const {
+    int64_t idx[3] = {0, 0, 0};
+    if (!physical_point_to_index(g, p, idx)) return false;
+    return px[buffer_offset(g, idx)] != 0;
+  }
+};
 
 // LinearInterpolateImageFunction::EvaluateOptimized, 2-D (LinInterp:157-224) and
 // 3-D (LinInterp:226-471).  The reference branches on distance <= 0 and on
@@ -380,19 +424,21 @@ struct Problem {
   double fBinSize, fNormMin, mBinSize, mNormMin;
   int64_t P;
   orc_result* res;
+  Mask fmask, mmask;
 };
 
 // One virtual sample: ThrBase:174-273 (ProcessVirtualPoint), I2IM:289-373
 // (TransformAndEvaluateFixed/MovingPoint), MattesT:215-387 (ProcessPoint).
-void process_sample(const Problem& pb, const int64_t* vidx, int64_t lin, ThreadState& ts) {
+// Masks: TransformAndEvaluateFixedPoint / MovingPoint test the mask at the
+// mapped point before the buffer (I2IM:298-309, 351-362).
+void process_sample(const Problem& pb, const double* vp, int64_t lin, ThreadState& ts) {
   const int D = pb.virt->dim;
   const int B = pb.opt->bins;
-  double vp[3];
-  index_to_point(pb.virt, vidx, vp);
 
   // Fixed side.
   double fp[3], ci[3];
   tf_point(pb.ftf, D, vp, fp, nullptr);
+  if (pb.fmask.g && !pb.fmask.inside(fp)) return;
   point_to_cindex(pb.fixed.g, fp, ci);
   if (!inside_buffer(pb.fixed.g, ci)) return;
   auto ffetch = [&](int64_t x, int64_t y, int64_t z) { return pb.fixed.at(x, y, z); };
@@ -402,6 +448,7 @@ void process_sample(const Problem& pb, const int64_t* vidx, int64_t lin, ThreadS
   double mp[3];
   BSplineEval be;
   tf_point(pb.mtf, D, vp, mp, &be);
+  if (pb.mmask.g && !pb.mmask.inside(mp)) return;
   point_to_cindex(pb.moving.g, mp, ci);
   if (!inside_buffer(pb.moving.g, ci)) return;
   auto mfetch = [&](int64_t x, int64_t y, int64_t z) { return pb.moving.at(x, y, z); };
@@ -525,10 +572,72 @@ void orc_transform_point(const orc_transform* tf, int32_t dim, const double* in,
   tf_point(tf, dim, in, out, nullptr);
 }
 
+int64_t orc_itk_round(double x) { return itk_round(x); }
+
+int orc_physical_point_to_index(const orc_geometry* g, const double* point, int64_t* idx) {
+  return physical_point_to_index(g, point, idx) ? 1 : 0;
+}
+
+// MattesMI::Initialize (Mattes:97-220): running min / max from the pixel
+// type's max and NonpositiveMin over the selected pixels.
+int64_t orc_intensity_range(const orc_geometry* g, const void* pixels, int32_t pixel_f64,
+                            const orc_geometry* mask_geom, const uint8_t* mask, const double* points,
+                            int64_t n_points, const orc_transform* fixed_tf, double* lo, double* hi) {
+  const int D = g->dim;
+  const Image img{g, pixels, pixel_f64};
+  double mn = pixel_f64 ? std::numeric_limits<double>::max() : (double)std::numeric_limits<float>::max();
+  double mx = -mn;
+  int64_t taken = 0;
+  auto take = [&](const int64_t* idx) {
+    const double v = img.at(idx[0], D > 1 ? idx[1] : 0, D > 2 ? idx[2] : 0);
+    mn = mn < v ? mn : v;
+    mx = mx > v ? mx : v;
+    ++taken;
+  };
+  if (points) {
+    // Mattes:106-133: virtual point -> fixed transform -> rounded index
+    for (int64_t i = 0; i < n_points; ++i) {
+      double fp[3] = {0, 0, 0};
+      int64_t idx[3] = {0, 0, 0};
+      tf_point(fixed_tf, D, points + i * D, fp, nullptr);
+      if (physical_point_to_index(g, fp, idx)) take(idx);
+    }
+  } else {
+    Mask m{mask_geom, mask};
+    const int64_t sz = D == 3 ? g->size[2] : 1;
+    int64_t idx[3] = {0, 0, 0};
+    for (int64_t z = 0; z < sz; ++z)
+      for (int64_t y = 0; y < g->size[1]; ++y)
+        for (int64_t x = 0; x < g->size[0]; ++x) {
+          idx[0] = g->start[0] + x;
+          idx[1] = g->start[1] + y;
+          idx[2] = D == 3 ? g->start[2] + z : 0;
+          if (m.g) {  // Mattes:159-172, 195-207
+            double p[3];
+            index_to_point(g, idx, p);
+            if (!m.inside(p)) continue;
+          }
+          take(idx);
+        }
+  }
+  *lo = mn;
+  *hi = mx;
+  return taken;
+}
+
 int orc_mattes_evaluate(const orc_geometry* virt, const orc_geometry* fixed_geom, const void* fixed_pixels,
                         const orc_geometry* moving_geom, const void* moving_pixels, const double* moving_gradient,
                         const orc_transform* fixed_tf, const orc_transform* moving_tf, const orc_options* opt,
                         orc_result* res, char* err, int errlen) {
+  return orc_mattes_evaluate_ex(virt, fixed_geom, fixed_pixels, moving_geom, moving_pixels, moving_gradient, fixed_tf,
+                                moving_tf, opt, nullptr, res, err, errlen);
+}
+
+int orc_mattes_evaluate_ex(const orc_geometry* virt, const orc_geometry* fixed_geom, const void* fixed_pixels,
+                           const orc_geometry* moving_geom, const void* moving_pixels,
+                           const double* moving_gradient, const orc_transform* fixed_tf,
+                           const orc_transform* moving_tf, const orc_options* opt, const orc_sampling* smp,
+                           orc_result* res, char* err, int errlen) {
   const int D = virt->dim;
   const int B = opt->bins;
   if (D != 2 && D != 3) {
@@ -548,6 +657,11 @@ int orc_mattes_evaluate(const orc_geometry* virt, const orc_geometry* fixed_geom
   pb.mtf = moving_tf;
   pb.opt = opt;
   pb.res = res;
+  const bool sparse = smp && smp->points;
+  if (smp) {
+    pb.fmask = Mask{smp->fixed_mask_geom, smp->fixed_mask};
+    pb.mmask = Mask{smp->moving_mask_geom, smp->moving_mask};
+  }
   if (orc_bin_parameters(opt->fixed_min, opt->fixed_max, B, &pb.fBinSize, &pb.fNormMin, err, errlen)) return 1;
   if (orc_bin_parameters(opt->moving_min, opt->moving_max, B, &pb.mBinSize, &pb.mNormMin, err, errlen)) return 1;
   pb.P = tf_num_params(moving_tf, D);
@@ -559,6 +673,7 @@ int orc_mattes_evaluate(const orc_geometry* virt, const orc_geometry* fixed_geom
 
   int64_t N = 1;
   for (int j = 0; j < D; ++j) N *= virt->size[j];
+  if (sparse) N = smp->n_points;
   if (res->fixed_bin)
     for (int64_t i = 0; i < N; ++i) res->fixed_bin[i] = -1;
   if (res->moving_bin)
@@ -566,8 +681,9 @@ int orc_mattes_evaluate(const orc_geometry* virt, const orc_geometry* fixed_geom
   if (res->moving_value)
     for (int64_t i = 0; i < N; ++i) res->moving_value[i] = std::numeric_limits<double>::quiet_NaN();
 
-  // Slabs along the slowest axis (ThreadedImageRegionPartitioner).
-  const int64_t slow = virt->size[D - 1];
+  // Slabs along the slowest axis (ThreadedImageRegionPartitioner), or
+  // contiguous runs of the point set (ThreadedIndexedContainerPartitioner).
+  const int64_t slow = sparse ? N : virt->size[D - 1];
   int T = std::max(1, opt->threads);
   if ((int64_t)T > slow) T = (int)slow;
   const double dpdfBytes = deriv ? (double)B * B * (double)pb.P * 8.0 : 0.0;
@@ -581,13 +697,23 @@ int orc_mattes_evaluate(const orc_geometry* virt, const orc_geometry* fixed_geom
   auto work = [&](int t) {
     const int64_t z0 = slow * t / T, z1 = slow * (t + 1) / T;
     int64_t idx[3];
+    double vp[3];
     const int64_t sx = virt->size[0], sy = D == 3 ? virt->size[1] : 1;
+    if (sparse) {
+      // sparse threader (Thr:44-63): the listed point, in list order
+      for (int64_t i = z0; i < z1; ++i) {
+        for (int k = 0; k < D; ++k) vp[k] = smp->points[i * D + k];
+        process_sample(pb, vp, i, ts[t]);
+      }
+      return;
+    }
     for (int64_t s = z0; s < z1; ++s) {
       if (D == 2) {
         for (int64_t x = 0; x < sx; ++x) {
           idx[0] = virt->start[0] + x;
           idx[1] = virt->start[1] + s;
-          process_sample(pb, idx, x + sx * s, ts[t]);
+          index_to_point(virt, idx, vp);
+          process_sample(pb, vp, x + sx * s, ts[t]);
         }
       } else {
         for (int64_t y = 0; y < sy; ++y)
@@ -595,7 +721,8 @@ int orc_mattes_evaluate(const orc_geometry* virt, const orc_geometry* fixed_geom
             idx[0] = virt->start[0] + x;
             idx[1] = virt->start[1] + y;
             idx[2] = virt->start[2] + s;
-            process_sample(pb, idx, x + sx * (y + sy * s), ts[t]);
+            index_to_point(virt, idx, vp);
+            process_sample(pb, vp, x + sx * (y + sy * s), ts[t]);
           }
       }
     }

@@ -67,6 +67,19 @@ typedef struct {
   double* moving_value;    /* [N virtual], interpolated moving value (NaN when outside); or NULL */
 } orc_result;
 
+/* Sample selection beyond the dense virtual domain.  points: n_points x dim
+ * virtual-space points (the sparse threader's point set), or NULL for every
+ * virtual voxel.  Masks: ImageMaskSpatialObject with default settings over a
+ * uint8 image of the given geometry, or NULL geometry for none. */
+typedef struct {
+  const double* points;
+  int64_t n_points;
+  const orc_geometry* fixed_mask_geom;
+  const uint8_t* fixed_mask;
+  const orc_geometry* moving_mask_geom;
+  const uint8_t* moving_mask;
+} orc_sampling;
+
 /* Mattes MI value (and derivative) over every voxel of the virtual domain.
  * Returns 0 on success, 1 with the reference's exception text in err. */
 int orc_mattes_evaluate(const orc_geometry* virt, const orc_geometry* fixed_geom, const void* fixed_pixels,
@@ -74,6 +87,30 @@ int orc_mattes_evaluate(const orc_geometry* virt, const orc_geometry* fixed_geom
                         const double* moving_gradient, const orc_transform* fixed_tf,
                         const orc_transform* moving_tf, const orc_options* opt, orc_result* res, char* err,
                         int errlen);
+
+/* orc_mattes_evaluate over the samples and masks of smp (NULL: dense, no
+ * masks).  Result arrays indexed by sample have n_points entries when points
+ * are given. */
+int orc_mattes_evaluate_ex(const orc_geometry* virt, const orc_geometry* fixed_geom, const void* fixed_pixels,
+                           const orc_geometry* moving_geom, const void* moving_pixels,
+                           const double* moving_gradient, const orc_transform* fixed_tf,
+                           const orc_transform* moving_tf, const orc_options* opt, const orc_sampling* smp,
+                           orc_result* res, char* err, int errlen);
+
+/* Intensity range of MattesMutualInformationImageToImageMetricv4::Initialize
+ * over one image: with points, the pixels the points (mapped by fixed_tf)
+ * round to; else with a mask, the voxels whose physical points the mask holds;
+ * else every voxel.  Returns the number of pixels taken; lo/hi keep the
+ * reference's start values (max, -max of the pixel type) when it is 0. */
+int64_t orc_intensity_range(const orc_geometry* g, const void* pixels, int32_t pixel_f64,
+                            const orc_geometry* mask_geom, const uint8_t* mask, const double* points,
+                            int64_t n_points, const orc_transform* fixed_tf, double* lo, double* hi);
+
+/* Math::RoundHalfIntegerUp<int64>(double) and the rounded-index test of
+ * ImageBase::TransformPhysicalPointToIndex: 1 with idx set when the rounded
+ * index is inside the image. */
+int64_t orc_itk_round(double x);
+int orc_physical_point_to_index(const orc_geometry* g, const double* point, int64_t* idx);
 
 /* Bin geometry of MattesMutualInformationImageToImageMetricv4::Initialize. */
 int orc_bin_parameters(double true_min, double true_max, int32_t bins, double* bin_size, double* normalized_min,

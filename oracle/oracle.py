@@ -65,6 +65,17 @@ class CResult(ctypes.Structure):
     ]
 
 
+class CSampling(ctypes.Structure):
+    _fields_ = [
+        ("points", _dp),
+        ("n_points", ctypes.c_int64),
+        ("fixed_mask_geom", ctypes.POINTER(CGeometry)),
+        ("fixed_mask", ctypes.c_void_p),
+        ("moving_mask_geom", ctypes.POINTER(CGeometry)),
+        ("moving_mask", ctypes.c_void_p),
+    ]
+
+
 def build(force: bool = False) -> str:
     """Compile the oracle with its Makefile (g++ only)."""
     if force or not os.path.exists(_LIB_PATH):
@@ -84,6 +95,22 @@ def lib():
             ctypes.POINTER(CTransform), ctypes.POINTER(CTransform), ctypes.POINTER(COptions),
             ctypes.POINTER(CResult), ctypes.c_char_p, ctypes.c_int,
         ]
+        L.orc_mattes_evaluate_ex.restype = ctypes.c_int
+        L.orc_mattes_evaluate_ex.argtypes = [
+            ctypes.POINTER(CGeometry), ctypes.POINTER(CGeometry), ctypes.c_void_p,
+            ctypes.POINTER(CGeometry), ctypes.c_void_p, _dp,
+            ctypes.POINTER(CTransform), ctypes.POINTER(CTransform), ctypes.POINTER(COptions),
+            ctypes.POINTER(CSampling), ctypes.POINTER(CResult), ctypes.c_char_p, ctypes.c_int,
+        ]
+        L.orc_intensity_range.restype = ctypes.c_int64
+        L.orc_intensity_range.argtypes = [ctypes.POINTER(CGeometry), ctypes.c_void_p, ctypes.c_int32,
+                                          ctypes.POINTER(CGeometry), ctypes.c_void_p, _dp, ctypes.c_int64,
+                                          ctypes.POINTER(CTransform), _dp, _dp]
+        L.orc_itk_round.restype = ctypes.c_int64
+        L.orc_itk_round.argtypes = [ctypes.c_double]
+        L.orc_physical_point_to_index.restype = ctypes.c_int
+        L.orc_physical_point_to_index.argtypes = [ctypes.POINTER(CGeometry), _dp,
+                                                  ctypes.POINTER(ctypes.c_int64)]
         L.orc_bin_parameters.restype = ctypes.c_int
         L.orc_bin_parameters.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int32, _dp, _dp,
                                          ctypes.c_char_p, ctypes.c_int]
@@ -147,6 +174,47 @@ def image_range(img: np.ndarray) -> tuple:
     return float(img.min()), float(img.max())
 
 
+def _mask_c(mask):
+    """(uint8 pixels, ImageGeometry) or None -> (geometry struct, pixels)."""
+    if mask is None:
+        return None, None
+    px, geom = mask
+    px = np.ascontiguousarray(px, dtype=np.uint8)
+    return geom.to_c(), px
+
+
+def intensity_range(img: np.ndarray, geom: ImageGeometry, *, mask=None, points=None, fixed_tf=None) -> tuple:
+    """MattesMI::Initialize's range over the pixels that ``points`` (virtual
+    space, through ``fixed_tf``) round to, else over the voxels inside
+    ``mask``, else over every voxel; (lo, hi, count)."""
+    img = np.ascontiguousarray(img)
+    f64 = 1 if img.dtype == np.float64 else 0
+    g = geom.to_c()
+    mg, mpx = _mask_c(mask)
+    tf, keep = make_transform(fixed_tf)
+    pts = None if points is None else np.ascontiguousarray(points, dtype=np.float64)
+    lo, hi = ctypes.c_double(), ctypes.c_double()
+    n = lib().orc_intensity_range(ctypes.byref(g), img.ctypes.data, f64,
+                                  ctypes.byref(mg) if mg is not None else None,
+                                  mpx.ctypes.data if mpx is not None else None, _ptr(pts),
+                                  0 if pts is None else pts.shape[0], ctypes.byref(tf), ctypes.byref(lo),
+                                  ctypes.byref(hi))
+    return lo.value, hi.value, int(n)
+
+
+def itk_round(x: float) -> int:
+    return int(lib().orc_itk_round(x))
+
+
+def physical_point_to_index(geom: ImageGeometry, point):
+    """(inside, index) of ImageBase::TransformPhysicalPointToIndex."""
+    g = geom.to_c()
+    p = (ctypes.c_double * 3)(*list(point) + [0.0] * (3 - len(point)))
+    idx = (ctypes.c_int64 * 3)()
+    ok = lib().orc_physical_point_to_index(ctypes.byref(g), p, idx)
+    return bool(ok), tuple(idx[: geom.dim])
+
+
 def bin_parameters(tmin, tmax, bins):
     bs, nm = ctypes.c_double(), ctypes.c_double()
     err = ctypes.create_string_buffer(512)
@@ -158,13 +226,16 @@ def bin_parameters(tmin, tmax, bins):
 def mattes(fixed: np.ndarray, fixed_geom: ImageGeometry, moving: np.ndarray, moving_geom: ImageGeometry,
            moving_tf, *, bins: int = 50, virtual_geom: ImageGeometry = None, fixed_tf=None,
            moving_gradient: np.ndarray = None, derivative: bool = True, threads: int = 1,
-           want_samples: bool = False, ranges=None) -> dict:
-    """Dense Mattes MI (value, derivative) exactly as the reference computes it.
+           want_samples: bool = False, ranges=None, points=None, fixed_mask=None, moving_mask=None) -> dict:
+    """Mattes MI (value, derivative) exactly as the reference computes it.
 
     ``fixed``/``moving`` are float32 or float64 numpy arrays in buffer order
     (numpy shape = reversed ITK size).  ``moving_gradient`` (float64, shape +
     (D,)) selects the gradient-image source; ``None`` selects central
-    differences.
+    differences.  ``points`` (n x D, virtual space) replaces the dense virtual
+    domain by the point set; ``fixed_mask`` / ``moving_mask`` are (uint8
+    pixels, ImageGeometry).  Without ``ranges`` the intensity ranges are taken
+    as MattesMI::Initialize takes them for these settings.
     """
     L = lib()
     if fixed.dtype != moving.dtype or fixed.dtype not in (np.float32, np.float64):
@@ -182,11 +253,30 @@ def mattes(fixed: np.ndarray, fixed_geom: ImageGeometry, moving: np.ndarray, mov
     opt.gradient_source = 1 if moving_gradient is not None else 0
     opt.compute_derivative = 1 if derivative else 0
     opt.threads = threads
+    pts = None if points is None else np.ascontiguousarray(points, dtype=np.float64).reshape(-1, fixed_geom.dim)
     if ranges is None:
-        ranges = (image_range(fixed), image_range(moving))
+        if pts is None and fixed_mask is None and moving_mask is None:
+            ranges = (image_range(fixed), image_range(moving))
+        else:
+            fr = intensity_range(fixed, fixed_geom, mask=None if pts is not None else fixed_mask, points=pts,
+                                 fixed_tf=fixed_tf)
+            mr = intensity_range(moving, moving_geom, mask=moving_mask)
+            ranges = (fr[:2], mr[:2])
     (opt.fixed_min, opt.fixed_max), (opt.moving_min, opt.moving_max) = ranges
     P = moving_tf.number_of_parameters
-    n = virtual_geom.num_pixels
+    n = virtual_geom.num_pixels if pts is None else pts.shape[0]
+    smp = CSampling()
+    fmg, fmpx = _mask_c(fixed_mask)
+    mmg, mmpx = _mask_c(moving_mask)
+    if pts is not None:
+        smp.points = _ptr(pts)
+        smp.n_points = pts.shape[0]
+    if fmg is not None:
+        smp.fixed_mask_geom = ctypes.pointer(fmg)
+        smp.fixed_mask = fmpx.ctypes.data
+    if mmg is not None:
+        smp.moving_mask_geom = ctypes.pointer(mmg)
+        smp.moving_mask = mmpx.ctypes.data
     out = {
         "derivative": np.zeros(P) if derivative else None,
         "joint_pdf": np.zeros(bins * bins),
@@ -210,9 +300,9 @@ def mattes(fixed: np.ndarray, fixed_geom: ImageGeometry, moving: np.ndarray, mov
     if moving_gradient is not None:
         grad = np.ascontiguousarray(moving_gradient, dtype=np.float64)
     err = ctypes.create_string_buffer(1024)
-    rc = L.orc_mattes_evaluate(ctypes.byref(vg), ctypes.byref(fg), fixed.ctypes.data, ctypes.byref(mg),
-                               moving.ctypes.data, _ptr(grad), ctypes.byref(ftf), ctypes.byref(mtf),
-                               ctypes.byref(opt), ctypes.byref(res), err, 1024)
+    rc = L.orc_mattes_evaluate_ex(ctypes.byref(vg), ctypes.byref(fg), fixed.ctypes.data, ctypes.byref(mg),
+                                  moving.ctypes.data, _ptr(grad), ctypes.byref(ftf), ctypes.byref(mtf),
+                                  ctypes.byref(opt), ctypes.byref(smp), ctypes.byref(res), err, 1024)
     if rc != 0:
         raise OracleError(err.value.decode())
     out["value"] = res.value

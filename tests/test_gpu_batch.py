@@ -93,6 +93,30 @@ def test_mixed_batch_and_degenerate_member():
     be.close()
 
 
+def test_batch_mixed_histogram_sizes():
+    """Members with 32, 50 and 80 bins in one fused launch: pass 1's LDS holds
+    two histogram copies for the smaller ones and one for 80 bins, and the
+    launch is sized for the largest; every member equals its own evaluation."""
+    probs = _problems()
+    ms = []
+    for (fixed, moving, geom, tf), bins in zip(probs[:3], (32, 80, 50)):
+        m = _metric(fixed, moving, geom, tf)
+        m.SetNumberOfHistogramBins(bins)
+        m.Initialize()
+        ms.append(m)
+    be = BatchEvaluator()
+    for rep in range(2):
+        got = be.evaluate(ms)
+        assert be.last_timing()["fused"] == 3
+        for m, (v, d, nv, err) in zip(ms, got):
+            assert err is None
+            v1, d1, nv1, path = _alone(m)
+            assert path == "fast"
+            assert (v, nv) == (v1, nv1)
+            assert np.array_equal(d, d1)
+    be.close()
+
+
 def test_batch_value_only_and_without_graphs():
     probs = _problems()
     ms = [_metric(*p) for p in probs[:3]]

@@ -82,7 +82,9 @@ def test_anisotropic_geometry_and_virtual_domain(oracle_mod):
     _compare(oracle_mod, m, fixed, moving, geom, tf, gradient=grad, virtual=virt)
 
 
-@pytest.mark.parametrize("bins", [5, 32, 100])
+# 5 and 32: two LDS histogram copies in pass 1; 80: one copy; 100: the
+# histogram exceeds LDS, exact path
+@pytest.mark.parametrize("bins", [5, 32, 80, 100])
 def test_histogram_sizes(oracle_mod, bins):
     fixed, moving, geom = blob_problem((24, 22, 20), 12)
     grad = oracle_mod.gradient_image(moving, geom)
