@@ -140,6 +140,29 @@ int mmi_set_virtual_points(mmi_context* ctx, int32_t dim, const double* points, 
 int mmi_set_fixed_mask(mmi_context* ctx, const mmi_geometry* g, const uint8_t* pixels);
 int mmi_set_moving_mask(mmi_context* ctx, const mmi_geometry* g, const uint8_t* pixels);
 
+/* Metric sampling strategies (ImageRegistrationMethodv4Enums::MetricSamplingStrategy,
+ * itkImageRegistrationMethodv4.h:51-56). */
+#define MMI_SAMPLING_NONE 0
+#define MMI_SAMPLING_REGULAR 1
+#define MMI_SAMPLING_RANDOM 2
+
+/* ImageRegistrationMethodv4::SetMetricSamplePoints for one level
+ * (itkImageRegistrationMethodv4.hxx:867-1024), on the host, no device work:
+ * the virtual-space sample points for mmi_set_virtual_points.  REGULAR keeps
+ * the first voxel of the virtual region and then every ceil(1/percentage)-th
+ * in buffer order; RANDOM draws (unsigned long)((float)N * percentage) voxels
+ * with replacement (ImageRandomConstIteratorWithIndex).  Each point is moved
+ * by GetNormalVariate() * spacing / 3 per axis and kept only when the fixed
+ * mask (mask_geom / mask as in mmi_set_fixed_mask, or NULL) holds it.
+ * seed: in/out, the method's m_CurrentRandomSeed; the perturbation stream is
+ * seeded with *seed and, for RANDOM, the voxel stream with *seed + 1, and
+ * *seed advances by the seeds used (1 or 2).  points: capacity x dim doubles;
+ * with points == NULL, *n_points receives the capacity needed (the count
+ * before the mask) and nothing else happens. */
+int mmi_metric_sample_points(const mmi_geometry* virt, int32_t strategy, double percentage, uint32_t* seed,
+                             const mmi_geometry* mask_geom, const uint8_t* mask, double* points, int64_t capacity,
+                             int64_t* n_points);
+
 /* MattesMutualInformationImageToImageMetricv4::SetNumberOfHistogramBins
  * (clamped to >= 5, itkMattesMutualInformationImageToImageMetricv4.h:146). */
 int mmi_set_number_of_histogram_bins(mmi_context* ctx, int32_t bins);

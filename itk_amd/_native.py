@@ -26,6 +26,7 @@ MMI_GRADIENT_CENTRAL_DIFFERENCE, MMI_GRADIENT_IMAGE = 0, 1
 MMI_FLAG_RECORD_SAMPLE_BINS = 1
 MMI_FLAG_EXACT_PATH_ONLY = 2
 MMI_FLAG_TIME_PASSES = 4
+MMI_SAMPLING_NONE, MMI_SAMPLING_REGULAR, MMI_SAMPLING_RANDOM = 0, 1, 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -45,6 +46,9 @@ SIGNATURES = {
     "mmi_set_smoothing_sigmas": (ctypes.c_int, [_vp, _dp, _dp]),
     "mmi_set_virtual_domain": (ctypes.c_int, [_vp, _gp]),
     "mmi_set_virtual_points": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, ctypes.c_int64]),
+    "mmi_metric_sample_points": (ctypes.c_int, [_gp, ctypes.c_int32, ctypes.c_double,
+                                                ctypes.POINTER(ctypes.c_uint32), _gp, _vp, _dp, ctypes.c_int64,
+                                                ctypes.POINTER(ctypes.c_int64)]),
     "mmi_set_fixed_mask": (ctypes.c_int, [_vp, _gp, _vp]),
     "mmi_set_moving_mask": (ctypes.c_int, [_vp, _gp, _vp]),
     "mmi_set_number_of_histogram_bins": (ctypes.c_int, [_vp, ctypes.c_int32]),

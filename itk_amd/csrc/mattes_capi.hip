@@ -26,6 +26,14 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+}  // namespace
+
+// The last-error string of the calling thread, for the other translation units
+// of the library (host-side sampling).
+int mmi::set_last_error(int code, const char* msg) { return fail(code, msg); }
+
+namespace {
+
 #define MMI_HIP(call)                                                                                  \
   do {                                                                                                 \
     hipError_t e_ = (call);                                                                            \

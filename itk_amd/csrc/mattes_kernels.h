@@ -299,4 +299,7 @@ hipError_t launch_minmax_select(const void* px, int storage, const DevGeom& img,
 hipError_t launch_pack_texels(const float* moving, const double* grad, int64_t n, int D, const Direction& dir,
                               float4* tex, hipStream_t st);
 
+// Sets the calling thread's mmi_last_error text and returns code.
+int set_last_error(int code, const char* msg);
+
 }  // namespace mmi

@@ -295,6 +295,8 @@ class MattesMutualInformationImageToImageMetricv4:
                 N.check(setter(ctx, ctypes.byref(mg), flags.ctypes.data))
         if self._use_sampled_points:
             self._sample_points = self._sampled_virtual_points()
+            if self._sample_points.shape[0] == 0:
+                raise N.MetricError(N.MMI_ERR_DEGENERATE, "the sampled point set has no points")
             N.check(L.mmi_set_virtual_points(ctx, d, _dptr(self._sample_points), self._sample_points.shape[0]))
         else:
             self._sample_points = None

@@ -9,6 +9,12 @@ each image's spacing when SmoothingSigmasAreSpecifiedInPhysicalUnits is off),
 the metric is re-initialised and the optimizer runs from the transform's
 current parameters.  Images stay resident on the device across levels: only
 the smoothing, gradient image and intensity ranges are recomputed there.
+
+With a REGULAR or RANDOM metric sampling strategy each level draws its sample
+points in the level's virtual domain (SetMetricSamplePoints,
+itkImageRegistrationMethodv4.hxx:699-702,867-1024) and hands them to the
+metric as its virtual sampled point set; the seed restarts from the method's
+random seed at every Update (:791-794).
 """
 from __future__ import annotations
 
@@ -16,6 +22,7 @@ import time
 
 import numpy as np
 
+from . import sampling
 from .geometry import ImageGeometry
 
 
@@ -32,6 +39,15 @@ class ImageRegistrationMethodv4:
         self.sigmas_in_physical_units = True
         self.level_log = []
         self.observers = []
+        self.fixed_mask = None
+        self.moving_mask = None
+        self.sampling_strategy = sampling.NONE
+        self.sampling_percentages = [1.0] * len(self.shrink_factors)
+        # the reference seeds from the clock by default (GetNextSeed); so does
+        # this method, with its own clock hash
+        self.reseed = False
+        self.random_seed = int(time.time_ns() & 0x7FFFFFFF)
+        self.current_random_seed = self.random_seed
 
     # -- inputs ---------------------------------------------------------------
     def SetFixedImage(self, pixels: np.ndarray, geometry: ImageGeometry):
@@ -57,6 +73,63 @@ class ImageRegistrationMethodv4:
         n = int(n)
         self.shrink_factors = (list(self.shrink_factors) + [1] * n)[:n]
         self.smoothing_sigmas = (list(self.smoothing_sigmas) + [0.0] * n)[:n]
+        # the reference resets the sampling percentages to 1 when the level
+        # count changes (itkImageRegistrationMethodv4.hxx:851-863)
+        if len(self.sampling_percentages) != n:
+            self.sampling_percentages = [1.0] * n
+
+    def SetFixedImageMask(self, mask):
+        """ImageMaskSpatialObject (itk_amd.masks) given to the metric at every
+        level, and used to drop sample points."""
+        self.fixed_mask = mask
+
+    def SetMovingImageMask(self, mask):
+        self.moving_mask = mask
+
+    def SetMetricSamplingStrategy(self, strategy: int):
+        """sampling.NONE, sampling.REGULAR or sampling.RANDOM."""
+        if strategy not in (sampling.NONE, sampling.REGULAR, sampling.RANDOM):
+            raise ValueError("Invalid sampling strategy requested.")
+        self.sampling_strategy = strategy
+
+    def GetMetricSamplingStrategy(self) -> int:
+        return self.sampling_strategy
+
+    def SetMetricSamplingPercentage(self, p: float):
+        self.SetMetricSamplingPercentagePerLevel([p] * self.GetNumberOfLevels())
+
+    def SetMetricSamplingPercentagePerLevel(self, ps):
+        ps = [float(p) for p in ps]
+        for p in ps:
+            if p <= 0.0 or p > 1.0:
+                raise ValueError("sampling percentage outside of (0,1]")
+        self.sampling_percentages = ps
+
+    def GetMetricSamplingPercentagePerLevel(self):
+        return list(self.sampling_percentages)
+
+    def MetricSamplingReinitializeSeed(self, seed: int = None):
+        """No argument: a fresh clock seed for every level's streams; with a
+        seed: reproducible streams from that seed (itkImageRegistrationMethodv4.hxx:1225-1250)."""
+        if seed is None:
+            self.reseed = True
+        else:
+            self.reseed = False
+            self.random_seed = int(seed)
+            self.current_random_seed = int(seed)
+
+    def _sample_points(self, level: int, virtual: ImageGeometry):
+        if self.reseed:
+            seed = int(time.time_ns() & 0xFFFFFFFF)
+        else:
+            seed = self.current_random_seed
+        if level >= len(self.sampling_percentages):
+            raise ValueError("one sampling percentage per level is required")
+        pts, nxt = sampling.metric_sample_points(virtual, self.sampling_strategy, self.sampling_percentages[level],
+                                                 seed, fixed_mask=self.fixed_mask)
+        if not self.reseed:
+            self.current_random_seed += nxt - (seed & 0xFFFFFFFF)
+        return pts
 
     def GetNumberOfLevels(self) -> int:
         return len(self.shrink_factors)
@@ -102,6 +175,10 @@ class ImageRegistrationMethodv4:
         m.SetMovingImage(moving_px, moving_geom)
         m.SetMovingTransform(self.transform)
         self.level_log = []
+        self.current_random_seed = self.random_seed  # the same streams on every Update
+        if self.fixed_mask is not None or self.moving_mask is not None or hasattr(m, "SetFixedImageMask"):
+            m.SetFixedImageMask(self.fixed_mask)
+            m.SetMovingImageMask(self.moving_mask)
         for level, (factor, sigma) in enumerate(zip(self.shrink_factors, self.smoothing_sigmas)):
             for fn in self.observers:
                 fn(self, level)
@@ -110,6 +187,11 @@ class ImageRegistrationMethodv4:
             virtual = base.shrink(factors)
             m.SetVirtualDomain(virtual)
             m.SetSmoothingSigmas(self._level_sigmas(sigma, fixed_geom), self._level_sigmas(sigma, moving_geom))
+            if self.sampling_strategy != sampling.NONE:
+                pts = self._sample_points(level, virtual)
+                m.SetVirtualSampledPointSet(pts)
+                m.SetUseSampledPointSet(True)
+                m.SetUseVirtualSampledPointSet(True)
             m.Initialize()
             est = getattr(self.optimizer, "scales_estimator", None)
             if est is not None:
@@ -119,6 +201,7 @@ class ImageRegistrationMethodv4:
             self.optimizer.StartOptimization()
             self.level_log.append({
                 "level": level, "shrink": factors, "sigma": sigma, "virtual_size": list(virtual.size),
+                "samples": m.GetNumberOfDomainPoints() if hasattr(m, "GetNumberOfDomainPoints") else None,
                 "iterations": self.optimizer.GetCurrentIteration(),
                 "value": self.optimizer.GetCurrentMetricValue(),
                 "stop": self.optimizer.GetStopConditionDescription(),

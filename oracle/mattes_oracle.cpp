@@ -25,6 +25,9 @@
 //   Kahan    = Modules/Core/Common/include/itkCompensatedSummation.hxx
 //   Thr      = Modules/Registration/Metricsv4/include/itkImageToImageMetricv4GetValueAndDerivativeThreader.hxx
 //   MaskSO   = Modules/Core/SpatialObjects/include/itkImageMaskSpatialObject.hxx
+//   RegMethod= Modules/Registration/RegistrationMethodsv4/include/itkImageRegistrationMethodv4.hxx
+//   MTGen    = Modules/Core/Common/src/itkMersenneTwisterRandomVariateGenerator.cxx (and include/...h)
+//   RandIt   = Modules/Core/Common/include/itkImageRandomConstIteratorWithIndex.hxx (and .h)
 //
 // Parity pins (tests/test_oracle.py): the reference test's known metric
 // values and its finite-difference derivative check
@@ -623,6 +626,136 @@ int64_t orc_intensity_range(const orc_geometry* g, const void* pixels, int32_t p
   *lo = mn;
   *hi = mx;
   return taken;
+}
+
+}  // extern "C"
+
+namespace {
+
+// MersenneTwisterRandomVariateGenerator (MTGen.cxx): the state is seeded and
+// reloaded at once (InitializeWithoutMutexLocking, :175-193), reload() walks
+// the state with the two loops of :195-222 and twist() of :53-57, and
+// GetIntegerVariate tempers (:226-240).
+struct MtGen {
+  uint32_t state[624];
+  uint32_t* next = nullptr;
+  int left = 0;
+
+  static uint32_t twist(uint32_t m, uint32_t s0, uint32_t s1) {
+    const uint32_t mix = (s0 & 0x80000000u) | (s1 & 0x7fffffffu);
+    return m ^ (mix >> 1) ^ (uint32_t)(-(int32_t)(s1 & 1u) & 0x9908b0df);
+  }
+  void reload() {
+    uint32_t* p = state;
+    for (int i = 624 - 397; i--; ++p) *p = twist(p[397], p[0], p[1]);
+    for (int i = 397; --i; ++p) *p = twist(p[397 - 624], p[0], p[1]);
+    *p = twist(p[397 - 624], p[0], state[0]);
+    left = 624;
+    next = state;
+  }
+  void seed(uint32_t s) {
+    state[0] = s;
+    for (uint32_t i = 1; i < 624; ++i) state[i] = 1812433253u * (state[i - 1] ^ (state[i - 1] >> 30)) + i;
+    reload();
+  }
+  uint32_t integer() {
+    if (left == 0) reload();
+    --left;
+    uint32_t s1 = *next++;
+    s1 ^= (s1 >> 11);
+    s1 ^= (s1 << 7) & 0x9d2c5680u;
+    s1 ^= (s1 << 15) & 0xefc60000u;
+    return s1 ^ (s1 >> 18);
+  }
+  // MTGen.h:203-226
+  double open_range() { return ((double)integer() + 0.5) / (double)(1ull << 32); }
+  double open_upper_range() { return (double)integer() / (double)(1ull << 32); }
+  // MTGen.h:364-375 (mean 0, variance 1)
+  double normal() {
+    const double mean = 0.0, variance = 1.0;
+    const double r = std::sqrt(-2.0 * std::log(1.0 - open_range()) * variance);
+    const double phi = 2.0 * 3.14159265358979323846 * open_upper_range();
+    return mean + r * std::cos(phi);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+void orc_mt_integers(uint32_t seed, uint32_t* out, int64_t n) {
+  MtGen g;
+  g.seed(seed);
+  for (int64_t i = 0; i < n; ++i) out[i] = g.integer();
+}
+
+void orc_mt_normals(uint32_t seed, double* out, int64_t n) {
+  MtGen g;
+  g.seed(seed);
+  for (int64_t i = 0; i < n; ++i) out[i] = g.normal();
+}
+
+// RegMethod:867-1024 for a single image metric.
+int64_t orc_metric_sample_points(const orc_geometry* virt, int32_t strategy, double percentage, uint32_t* seed,
+                                 const orc_geometry* mask_geom, const uint8_t* mask, double* out) {
+  if (!virt || !seed || (strategy != 1 && strategy != 2) || !(percentage > 0.0 && percentage <= 1.0)) return -1;
+  const int D = virt->dim;
+  int64_t npix = 1;
+  for (int j = 0; j < D; ++j) npix *= virt->size[j];
+  double oneThird[3];
+  for (int d = 0; d < D; ++d) oneThird[d] = virt->spacing[d] / 3.0;
+  MtGen randomizer;
+  randomizer.seed((*seed)++);
+  Mask fmask{mask_geom, mask};
+  int64_t index = 0;
+  auto add = [&](const int64_t* idx) {
+    double point[3];
+    index_to_point(virt, idx, point);
+    for (int d = 0; d < D; ++d) point[d] += randomizer.normal() * oneThird[d];
+    if (!fmask.g || fmask.inside(point)) {
+      for (int d = 0; d < D; ++d) out[index * D + d] = point[d];
+      ++index;
+    }
+  };
+  if (strategy == 1) {
+    const auto sampleCount = (unsigned long)std::ceil(1.0 / percentage);
+    unsigned long count = sampleCount;
+    int64_t idx[3] = {0, 0, 0};
+    const int64_t sz = D == 3 ? virt->size[2] : 1;
+    for (int64_t z = 0; z < sz; ++z)
+      for (int64_t y = 0; y < virt->size[1]; ++y)
+        for (int64_t x = 0; x < virt->size[0]; ++x) {
+          if (count == sampleCount) {
+            count = 0;
+            idx[0] = virt->start[0] + x;
+            idx[1] = virt->start[1] + y;
+            idx[2] = D == 3 ? virt->start[2] + z : 0;
+            add(idx);
+          }
+          ++count;
+        }
+  } else {
+    const auto sampleCount = (unsigned long)((float)npix * percentage);
+    MtGen iter;  // ImageRandomConstIteratorWithIndex's generator, reseeded
+    iter.seed((uint32_t)(int32_t)(*seed)++);
+    // GoToBegin / operator++ each jump (RandIt.hxx:47-68, RandIt.h:161-197)
+    auto jump = [&](int64_t* idx) {
+      auto position = (int64_t)(iter.open_range() * ((double)npix - 0.5));
+      for (int d = 0; d < D; ++d) {
+        const int64_t residual = position % virt->size[d];
+        idx[d] = residual + virt->start[d];
+        position -= residual;
+        position /= virt->size[d];
+      }
+    };
+    int64_t idx[3] = {0, 0, 0};
+    jump(idx);
+    for (unsigned long done = 0; done < sampleCount; ++done) {
+      add(idx);
+      jump(idx);
+    }
+  }
+  return index;
 }
 
 int orc_mattes_evaluate(const orc_geometry* virt, const orc_geometry* fixed_geom, const void* fixed_pixels,

@@ -112,6 +112,19 @@ int64_t orc_intensity_range(const orc_geometry* g, const void* pixels, int32_t p
 int64_t orc_itk_round(double x);
 int orc_physical_point_to_index(const orc_geometry* g, const double* point, int64_t* idx);
 
+/* MersenneTwisterRandomVariateGenerator seeded with seed: n integer variates,
+ * or n normal variates (mean 0, variance 1). */
+void orc_mt_integers(uint32_t seed, uint32_t* out, int64_t n);
+void orc_mt_normals(uint32_t seed, double* out, int64_t n);
+
+/* ImageRegistrationMethodv4::SetMetricSamplePoints for one level: strategy 1
+ * REGULAR, 2 RANDOM; seed in/out (m_CurrentRandomSeed); fixed mask optional;
+ * out has room for the bound (REGULAR ceil(N / ceil(1/p)), RANDOM
+ * (unsigned long)((float)N * p)) points of dim doubles.  Returns the number of
+ * points kept, -1 on a bad argument. */
+int64_t orc_metric_sample_points(const orc_geometry* virt, int32_t strategy, double percentage, uint32_t* seed,
+                                 const orc_geometry* mask_geom, const uint8_t* mask, double* out);
+
 /* Bin geometry of MattesMutualInformationImageToImageMetricv4::Initialize. */
 int orc_bin_parameters(double true_min, double true_max, int32_t bins, double* bin_size, double* normalized_min,
                        char* err, int errlen);

@@ -6,6 +6,7 @@ this module; product code never does.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import subprocess
 
@@ -106,6 +107,14 @@ def lib():
         L.orc_intensity_range.argtypes = [ctypes.POINTER(CGeometry), ctypes.c_void_p, ctypes.c_int32,
                                           ctypes.POINTER(CGeometry), ctypes.c_void_p, _dp, ctypes.c_int64,
                                           ctypes.POINTER(CTransform), _dp, _dp]
+        L.orc_mt_integers.restype = None
+        L.orc_mt_integers.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int64]
+        L.orc_mt_normals.restype = None
+        L.orc_mt_normals.argtypes = [ctypes.c_uint32, _dp, ctypes.c_int64]
+        L.orc_metric_sample_points.restype = ctypes.c_int64
+        L.orc_metric_sample_points.argtypes = [ctypes.POINTER(CGeometry), ctypes.c_int32, ctypes.c_double,
+                                               ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(CGeometry),
+                                               ctypes.c_void_p, _dp]
         L.orc_itk_round.restype = ctypes.c_int64
         L.orc_itk_round.argtypes = [ctypes.c_double]
         L.orc_physical_point_to_index.restype = ctypes.c_int
@@ -200,6 +209,41 @@ def intensity_range(img: np.ndarray, geom: ImageGeometry, *, mask=None, points=N
                                   0 if pts is None else pts.shape[0], ctypes.byref(tf), ctypes.byref(lo),
                                   ctypes.byref(hi))
     return lo.value, hi.value, int(n)
+
+
+def mt_integers(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, dtype=np.uint32)
+    lib().orc_mt_integers(seed & 0xFFFFFFFF, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), n)
+    return out
+
+
+def mt_normals(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n)
+    lib().orc_mt_normals(seed & 0xFFFFFFFF, _ptr(out), n)
+    return out
+
+
+SAMPLING_REGULAR, SAMPLING_RANDOM = 1, 2
+
+
+def metric_sample_points(virtual_geom: ImageGeometry, strategy: int, percentage: float, seed: int, mask=None):
+    """(points (n, dim), next seed) of SetMetricSamplePoints for one level."""
+    n = virtual_geom.num_pixels
+    if strategy == SAMPLING_REGULAR:
+        every = int(math.ceil(1.0 / percentage))
+        bound = (n + every - 1) // every
+    else:
+        bound = int(float(np.float32(n)) * percentage)
+    out = np.zeros((max(bound, 1), virtual_geom.dim))
+    g = virtual_geom.to_c()
+    mg, mpx = _mask_c(mask)
+    s = ctypes.c_uint32(seed & 0xFFFFFFFF)
+    k = lib().orc_metric_sample_points(ctypes.byref(g), strategy, percentage, ctypes.byref(s),
+                                       ctypes.byref(mg) if mg is not None else None,
+                                       mpx.ctypes.data if mpx is not None else None, _ptr(out))
+    if k < 0:
+        raise OracleError("bad sampling arguments")
+    return out[:k].copy(), s.value
 
 
 def itk_round(x: float) -> int:

@@ -20,6 +20,9 @@ class OracleMattesMetric:
         self.tf = None
         self.fixed_sigma = self.moving_sigma = None
         self.value = float(np.finfo(np.float64).max)
+        self.points = None
+        self.use_points = False
+        self.masks = [None, None]
 
     def SetFixedImage(self, px, geom):
         self.fixed = (np.asarray(px, dtype=np.float32), geom)
@@ -36,6 +39,26 @@ class OracleMattesMetric:
     def SetNumberOfHistogramBins(self, b):
         self.bins = max(5, int(b))
 
+    def SetVirtualSampledPointSet(self, pts):
+        self.points = None if pts is None else np.asarray(pts, dtype=np.float64)
+
+    def SetUseSampledPointSet(self, on):
+        self.use_points = bool(on)
+
+    def SetUseVirtualSampledPointSet(self, on):
+        if not on:
+            raise NotImplementedError("fixed-space point sets")
+
+    def SetFixedImageMask(self, mask):
+        self.masks[0] = mask
+
+    def SetMovingImageMask(self, mask):
+        self.masks[1] = mask
+
+    @staticmethod
+    def _mask(so):
+        return None if so is None else (so.inside_flags(), so.geometry)
+
     def SetSmoothingSigmas(self, fixed_sigma=None, moving_sigma=None):
         self.fixed_sigma, self.moving_sigma = fixed_sigma, moving_sigma
 
@@ -51,7 +74,9 @@ class OracleMattesMetric:
     def GetValueAndDerivative(self):
         (_, fg), (_, mg) = self.fixed, self.moving
         r = self.o.mattes(self.level_fixed, fg, self.level_moving, mg, self.tf, bins=self.bins,
-                          virtual_geom=self.virtual, moving_gradient=self.gradient, threads=self.threads)
+                          virtual_geom=self.virtual, moving_gradient=self.gradient, threads=self.threads,
+                          points=self.points if self.use_points else None,
+                          fixed_mask=self._mask(self.masks[0]), moving_mask=self._mask(self.masks[1]))
         self.value = r["value"]
         return r["value"], r["derivative"]
 

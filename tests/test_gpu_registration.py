@@ -35,7 +35,7 @@ def truth_transform(problem) -> AffineTransform:
     return tf
 
 
-def run_affine(problem, metric, iterations, shrink, sigmas, lr=2.0):
+def run_affine(problem, metric, iterations, shrink, sigmas, lr=2.0, sampling=None):
     geom = problem.geometry
     tf = affine_for(geom)
     opt = RegularStepGradientDescentOptimizerv4(learning_rate=lr, minimum_step_length=1e-3,
@@ -50,6 +50,12 @@ def run_affine(problem, metric, iterations, shrink, sigmas, lr=2.0):
     reg.SetShrinkFactorsPerLevel(shrink)
     reg.SetSmoothingSigmasPerLevel(sigmas)
     reg.SetSmoothingSigmasAreSpecifiedInPhysicalUnits(False)
+    if sampling is not None:
+        strategy, pct, seed, mask = sampling
+        reg.SetMetricSamplingStrategy(strategy)
+        reg.SetMetricSamplingPercentage(pct)
+        reg.MetricSamplingReinitializeSeed(seed)
+        reg.SetFixedImageMask(mask)
     reg.Update()
     return tf, reg.level_log
 
@@ -69,6 +75,40 @@ def test_affine_registration_matches_oracle(oracle_mod, shape, iterations):
     print("mean displacement vs oracle run %.3e voxel, vs generating transform %.3f voxel" % (d_oracle, d_truth))
     assert d_oracle <= 0.1
     assert d_truth <= 1.5
+
+
+@pytest.mark.parametrize("shape,strategy,pct,masked", [((128, 112), "random", 0.25, True),
+                                                        ((48, 44, 40), "regular", 0.2, False)])
+def test_sampled_registration_matches_oracle(oracle_mod, shape, strategy, pct, masked):
+    """REGULAR / RANDOM metric sampling (with a fixed mask in 2-D): the same
+    point sets per level on both metrics, final transforms within 0.1 voxel."""
+    from itk_amd import sampling
+    from itk_amd.masks import ImageMaskSpatialObject
+
+    p = synth.make_affine_problem(shape, 53)
+    mask = None
+    if masked:
+        mpx = np.zeros(p.geometry.shape, dtype=np.uint8)
+        sl = tuple(slice(s // 8, s - s // 8) for s in p.geometry.shape)
+        mpx[sl] = 1
+        mask = ImageMaskSpatialObject(mpx, p.geometry)
+    code = sampling.RANDOM if strategy == "random" else sampling.REGULAR
+    shrink, sigmas = [2, 1], [1.0, 0.0]
+    logs, tfs = [], []
+    for metric in (Metric(), OracleMattesMetric(oracle_mod)):
+        tf, log = run_affine(p, metric, 40, shrink, sigmas, sampling=(code, pct, 20240, mask))
+        tfs.append(tf)
+        logs.append(log)
+    for a, b in zip(*logs):
+        print("level %d: %s samples, gpu %d its value %.6f | oracle %d its value %.6f"
+              % (a["level"], a["samples"], a["iterations"], a["value"], b["iterations"], b["value"]))
+        assert a["samples"] > 0
+    stride = 1 if len(shape) == 2 else 4
+    d_oracle = mean_displacement(tfs[0], tfs[1], p.geometry, stride)
+    d_truth = mean_displacement(tfs[0], truth_transform(p), p.geometry, stride)
+    print("mean displacement vs oracle run %.3e voxel, vs generating transform %.3f voxel" % (d_oracle, d_truth))
+    assert d_oracle <= 0.1
+    assert d_truth <= 2.0
 
 
 def test_bspline_registration_lbfgsb_matches_oracle(oracle_mod):
