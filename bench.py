@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--separate-value-gathers", action="store_true",
+                    help="pass 2 gathers moving values apart from the texels (comparison runs)")
     ap.add_argument("--serving", action="store_true", help="registration serving workload")
     ap.add_argument("--requests", type=int, default=32, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=40.0, help="serving: mean arrivals per second")
@@ -281,7 +283,7 @@ def main():
     wl = Workload(args)
     problem, tf = wl.problem, wl.tf
     geom = problem.geometry
-    m = MattesMutualInformationImageToImageMetricv4(device=0)
+    m = MattesMutualInformationImageToImageMetricv4(device=0, texel_values=not args.separate_value_gathers)
     m.SetImageStorage(args.storage)
     m.SetFixedImage(problem.fixed, geom)
     m.SetMovingImage(problem.moving, geom)

@@ -62,6 +62,8 @@ const char* mmi_last_error(void);
 #define MMI_FLAG_RECORD_SAMPLE_BINS 1u /* keep per-sample Parzen bins for mmi_get_sample_bins */
 #define MMI_FLAG_EXACT_PATH_ONLY 2u    /* evaluate every sample on the FP64 path (no certified FP32 fast path) */
 #define MMI_FLAG_TIME_PASSES 4u        /* time pass 1 and pass 2 apart (two more events per evaluation) */
+#define MMI_FLAG_SEPARATE_VALUE_GATHERS 8u /* B-spline fast path: pass 2 gathers moving values from the value
+                                             image as well as the texels (same results; for comparison runs) */
 
 /* Create a context on a HIP device with the flags above (0 for defaults). */
 int mmi_context_create(int device, uint32_t flags, mmi_context** out);
@@ -221,6 +223,11 @@ int mmi_get_histogram_parameters(mmi_context* ctx, double* fixed_min, double* fi
  * exceptions) and pass2_ms (finalize excluded: pass 2 through the derivative)
  * only with MMI_FLAG_TIME_PASSES, else -1. */
 int mmi_get_last_timing(mmi_context* ctx, float* total_ms, float* pass1_ms, float* pass2_ms);
+
+/* Diagnostics: the samples of the last fast-path evaluation that its
+ * certification sent to the exact FP64 path (-1 when the last evaluation was
+ * not on the fast path), and the samples per evaluation. */
+int mmi_get_exception_count(mmi_context* ctx, int64_t* n_exact, int64_t* n_samples);
 
 /* Diagnostics (no counterpart in the reference): which kernels the last
  * mmi_evaluate ran: 0 the exact FP64 path, 1 the certified fast path, 2 the

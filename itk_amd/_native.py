@@ -26,6 +26,7 @@ MMI_GRADIENT_CENTRAL_DIFFERENCE, MMI_GRADIENT_IMAGE = 0, 1
 MMI_FLAG_RECORD_SAMPLE_BINS = 1
 MMI_FLAG_EXACT_PATH_ONLY = 2
 MMI_FLAG_TIME_PASSES = 4
+MMI_FLAG_SEPARATE_VALUE_GATHERS = 8
 MMI_SAMPLING_NONE, MMI_SAMPLING_REGULAR, MMI_SAMPLING_RANDOM = 0, 1, 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -74,6 +75,7 @@ SIGNATURES = {
     "mmi_batch_regular_step": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_int32, _vp, _vp, ctypes.c_int32]),
     "mmi_get_moving_parameters": (ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
     "mmi_get_last_path": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
+    "mmi_get_exception_count": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "mmi_get_last_timing": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                            ctypes.POINTER(ctypes.c_float)]),
 }

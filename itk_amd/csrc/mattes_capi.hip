@@ -260,6 +260,7 @@ struct mmi_context {
 
   bool have_fixed = false, have_moving = false, have_virtual = false, have_texels = false;
   bool user_texels = false;  // gradient given by mmi_set_moving_gradient (not recomputed per level)
+  bool texel_values_level = false;  // the texels' value channel is the level's moving image
   mmi_geometry fixed_g{}, moving_g{}, virtual_g{};
   DevBuf fixed_px, moving_px, texels;  // FP32 uploads and FP32 gradient texels
   DevBuf fixed_lp, moving_lp, texels_lp;  // 16-bit storage copies (MMI_BF16 / MMI_F16)
@@ -284,6 +285,7 @@ struct mmi_context {
   // dev_overflow, dev_deriv)
   DevBuf out;
   DevBuf exc_count, exc_list;  // fast-path exception lists
+  DevBuf exc_res;              // B-spline: exact results of the listed samples
   DevBuf ftab;                               // fast path: per-axis fixed cells (coarse levels)
   // grow-only work space of the preparation filters (no device frees, which
   // would synchronise the whole device, while other contexts run)
@@ -956,7 +958,7 @@ int mmi_context_destroy(mmi_context* ctx) {
                     &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->out, &ctx->points, &ctx->mask_px[0], &ctx->mask_px[1],
                     &ctx->mask_desc[0], &ctx->mask_desc[1],
                     &ctx->partials, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
-                    &ctx->exc_count, &ctx->exc_list, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->ftab, &ctx->work_a,
+                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_res, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->ftab, &ctx->work_a,
                     &ctx->work_b, &ctx->work_grad};
   for (DevBuf* b : bufs) b->release();
   for (int j = 0; j < 3; ++j) {
@@ -1024,6 +1026,7 @@ int mmi_set_moving_gradient(mmi_context* ctx, const double* gradient) {
   MMI_HIP(hipStreamSynchronize(ctx->stream));
   ctx->have_texels = true;
   ctx->user_texels = true;
+  ctx->texel_values_level = false;  // packed from the unsmoothed upload; mmi_initialize decides
   return refresh_storage_texels(ctx);
 }
 
@@ -1090,6 +1093,7 @@ int mmi_compute_moving_gradient(mmi_context* ctx) {
   MMI_HIP(hipStreamSynchronize(ctx->stream));
   ctx->have_texels = true;
   ctx->user_texels = false;
+  ctx->texel_values_level = true;
   for (int i = 0; i < 3; ++i) ctx->texel_sigma[i] = ctx->moving_smoothed ? ctx->moving_sigma[i] : 0.0;
   return refresh_storage_texels(ctx);
 }
@@ -1217,6 +1221,8 @@ int mmi_initialize(mmi_context* ctx) {
   if (ctx->gradient_source == MMI_GRADIENT_IMAGE && stale) {
     if (int rc = mmi_compute_moving_gradient(ctx)) return rc;
   }
+  // user texels were packed from the unsmoothed upload
+  if (ctx->user_texels) ctx->texel_values_level = !ctx->moving_smoothed;
   // 16-bit storage: every later read of the images (intensity range, bins,
   // interpolation, gradient) sees the rounded copies.
   ctx->active_storage = MMI_F32;
@@ -1421,6 +1427,10 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
   if (fast) {
     MMI_HIP(ctx->exc_count.ensure(sizeof(uint32_t) * ctx->grid));
     MMI_HIP(ctx->exc_list.ensure(sizeof(uint32_t) * ctx->grid * kExcCap));
+    if (ctx->mtf.kind == MMI_TF_BSPLINE) {
+      MMI_HIP(ctx->exc_res.ensure(sizeof(ExcResult) * ctx->grid * kExcCap));
+      fa.exc_res = ctx->exc_res.as<ExcResult>();
+    }
     fa.xcd_chunks = ctx->xcd_chunks;
     fa.exc_cap = kExcCap;
     fa.fixed_px = a.fixed_px;
@@ -1514,6 +1524,9 @@ retry:
     MMI_HIP(launch_bspline_rows(ctx->dmtf.coeffs, ctx->dmtf.pdim, ctx->dmtf.grid.size[0], ctx->dmtf.grid.size[1], D,
                                 (int)a.virt.size[1], D == 3 ? (int)a.virt.size[2] : 1, rt, ctx->bs_rows.as<double>(), st));
   }
+  // B-spline pass 2 takes the moving values from the texels it gathers for
+  // the gradient when their value channel is the level's moving image
+  fa.tex_values = (ctx->texel_values_level && !(ctx->flags & MMI_FLAG_SEPARATE_VALUE_GATHERS)) ? 1 : 0;
   if (fast) {
     MMI_HIP(launch_fast_pass1(fa, a, D, ctx->grid, st));
     if (fa.mk == 2) MMI_HIP(launch_fast_exceptions1(a, fa, ctx->grid, st));
@@ -1536,8 +1549,7 @@ retry:
       // B-spline fast path: per-sample factors, then the separable contraction
       MMI_HIP(ctx->bs_g.ensure(sizeof(float) * D * n));
       fa.bspline_g = ctx->bs_g.as<float>();
-      MMI_HIP(launch_fast_pass2(fa, a, D, ctx->grid, st));
-      MMI_HIP(launch_fast_exceptions2(a, fa, ctx->grid, st));
+      MMI_HIP(launch_fast_pass2(fa, a, D, ctx->grid, st));  // its tail writes the listed samples' factors
       if (int rc = bspline_contract(ctx, a.virt, tabs, f.result)) return rc;
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     } else if (fast) {
@@ -2097,6 +2109,20 @@ int mmi_get_histogram_parameters(mmi_context* ctx, double* fixed_min, double* fi
   if (moving_max) *moving_max = ctx->moving_max;
   if (fixed_bin_size) *fixed_bin_size = ctx->fixed_bin_size;
   if (moving_bin_size) *moving_bin_size = ctx->moving_bin_size;
+  return MMI_OK;
+}
+
+int mmi_get_exception_count(mmi_context* ctx, int64_t* n_exact, int64_t* n_samples) {
+  if (!ctx || !n_exact || !n_samples) return fail(MMI_ERR_ARGUMENT, "NULL argument");
+  *n_samples = ctx->initialized ? sample_count(ctx) : 0;
+  *n_exact = -1;
+  if (ctx->last_path != 1 || !ctx->exc_count.p) return MMI_OK;
+  MMI_HIP(hipSetDevice(ctx->device));
+  std::vector<uint32_t> h((size_t)ctx->grid);
+  MMI_HIP(hipMemcpy(h.data(), ctx->exc_count.p, sizeof(uint32_t) * h.size(), hipMemcpyDeviceToHost));
+  int64_t total = 0;
+  for (uint32_t c : h) total += c;
+  *n_exact = total;
   return MMI_OK;
 }
 

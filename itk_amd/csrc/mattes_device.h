@@ -439,13 +439,26 @@ __device__ __forceinline__ void transform_point(const DevTransform& t, const dou
       for (int i = 0; i < D; ++i) out[i] = in[i];
       return;
     }
+    // the support's nodes in the reference's order (x fastest), summed in
+    // that order; the four x nodes of a support row are adjacent in memory and
+    // their loads are issued together, several rows ahead of the sums
     double acc[3] = {0.0, 0.0, 0.0};
-    constexpr int NW = D == 3 ? 64 : 16;
-    for (int k = 0; k < NW; ++k) {
-      const double w = bspline_weight<D>(*sup, k);
-      const int32_t node = bspline_node<D>(t.grid, *sup, k);
+    constexpr int NR = D == 3 ? 16 : 4;
+    const int32_t gx = t.grid.size[0], gxy = t.grid.size[0] * t.grid.size[1];
+#pragma unroll 4
+    for (int r = 0; r < NR; ++r) {
+      const int32_t row = sup->start[0] + gx * (sup->start[1] + (r & 3)) + (D == 3 ? gxy * (sup->start[2] + (r >> 2)) : 0);
+      double c[3][4];
 #pragma unroll
-      for (int j = 0; j < D; ++j) acc[j] += w * t.coeffs[j * t.pdim + node];
+      for (int j = 0; j < D; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c[j][i] = t.coeffs[j * t.pdim + row + i];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const double w = bspline_weight<D>(*sup, r * 4 + i);
+#pragma unroll
+        for (int j = 0; j < D; ++j) acc[j] += w * c[j][i];
+      }
     }
 #pragma unroll
     for (int j = 0; j < D; ++j) out[j] = acc[j] + in[j];

@@ -10,12 +10,16 @@
 // and is evaluated afterwards on the exact FP64 path (mattes_eval.h), so the
 // bins and the valid count are bit-identical to the reference's.
 //
-// Kernels: pass1_fast (histogram of certain samples; each workgroup then
-// evaluates its own listed samples exactly into the same LDS histogram) ->
-// finalize (mattes_kernels.hip) -> pass2_fast (derivative moments of certain
-// samples) -> exceptions_pass2 (exact samples, sorted per list so sums are in
-// a fixed order) -> fast_derivative (fixed-order reduction, index-space
-// moments mapped to the affine parameters).
+// Kernels, affine: pass1_fast (histogram of certain samples; each workgroup
+// then evaluates its own listed samples exactly into the same LDS histogram)
+// -> finalize (mattes_kernels.hip) -> pass2_fast (derivative moments of
+// certain samples, then the workgroup's listed samples exactly, sorted so the
+// sums are in a fixed order) -> fast_derivative (fixed-order reduction,
+// index-space moments mapped to the affine parameters).  B-spline:
+// bspline_rows -> pass1_fast -> exceptions_pass1 (listed samples exactly,
+// eight lists per workgroup, their results kept per list slot) -> finalize
+// -> pass2 (per-sample factors; its tail writes the listed samples' factors
+// from the kept results) -> separable contraction (mattes_prep.hip).
 //
 // The hot loops read a compact argument block so that the uniform values they
 // use fit the scalar register file; the exact evaluations take the full
@@ -127,9 +131,13 @@ __device__ __forceinline__ void fast_cindex(const FastArgs& f, const int32_t* id
   }
 }
 
-template <int D, typename P, int MK>
+// TEX: gather the moving texels (value and gradient channels) instead of the
+// moving values, and return the interpolated gradient and the moving Parzen
+// term as well; the value is interpolated with the same operations either way.
+template <int D, typename P, int MK, bool TEX = false>
 __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* idx, int& fbin, int& mbin, float& arg0,
-                                             Cell& cell, bool* bsin) {
+                                             Cell& cell, bool* bsin, float3* grad = nullptr,
+                                             float* mterm_out = nullptr) {
   double ci[3];
   fast_cindex<D, MK>(f, idx, ci, bsin);
   bool out = false, uncertain = false;
@@ -157,28 +165,60 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
   cell.oy = act[1] ? f.msy : 0;
   cell.oz = (D == 3 && act[2]) ? f.msz : 0;
   const float t0 = (float)cell.d[0], t1 = (float)cell.d[1], t2 = (float)cell.d[2];
-  const typename P::pixel* img = static_cast<const typename P::pixel*>(f.moving_px);
   // every cell offset is non-negative and below the buffer size
   const uint32_t cb = (uint32_t)cell.base, cx = (uint32_t)cell.ox, cy = (uint32_t)cell.oy, cz = (uint32_t)cell.oz;
-  const float v000 = P::ld32(img, cb), v100 = P::ld32(img, cb + cx);
-  const float v010 = P::ld32(img, cb + cy), v110 = P::ld32(img, cb + cy + cx);
-  const float vx00 = fmaf(v100 - v000, t0, v000);
-  const float vx10 = fmaf(v110 - v010, t0, v010);
-  float v = fmaf(vx10 - vx00, t1, vx00);
-  if (D == 3) {
-    const float v001 = P::ld32(img, cb + cz), v101 = P::ld32(img, cb + cz + cx);
-    const float v011 = P::ld32(img, cb + cz + cy);
-    const float v111 = P::ld32(img, cb + cz + cy + cx);
-    const float vx01 = fmaf(v101 - v001, t0, v001);
-    const float vx11 = fmaf(v111 - v011, t0, v011);
-    const float vxx1 = fmaf(vx11 - vx01, t1, vx01);
-    v = fmaf(vxx1 - v, t2, v);
+  float v;
+  if constexpr (TEX) {
+    // value as below, gradient as interp_gradient3, from one texel load per corner
+    const typename P::texel* tex = static_cast<const typename P::texel*>(f.texels);
+    const float4 a000 = P::ldt32(tex, cb), a100 = P::ldt32(tex, cb + cx);
+    const float4 a010 = P::ldt32(tex, cb + cy), a110 = P::ldt32(tex, cb + cy + cx);
+    const float vx00 = fmaf(a100.x - a000.x, t0, a000.x);
+    const float vx10 = fmaf(a110.x - a010.x, t0, a010.x);
+    v = fmaf(vx10 - vx00, t1, vx00);
+    const float gx0 = fmaf(a100.y - a000.y, t0, a000.y), gx1 = fmaf(a110.y - a010.y, t0, a010.y);
+    const float gy0 = fmaf(a100.z - a000.z, t0, a000.z), gy1 = fmaf(a110.z - a010.z, t0, a010.z);
+    const float gz0 = fmaf(a100.w - a000.w, t0, a000.w), gz1 = fmaf(a110.w - a010.w, t0, a010.w);
+    float3 g = make_float3(fmaf(gx1 - gx0, t1, gx0), fmaf(gy1 - gy0, t1, gy0), fmaf(gz1 - gz0, t1, gz0));
+    if (D == 3) {
+      const float4 a001 = P::ldt32(tex, cb + cz), a101 = P::ldt32(tex, cb + cz + cx);
+      const float4 a011 = P::ldt32(tex, cb + cz + cy), a111 = P::ldt32(tex, cb + cz + cy + cx);
+      const float vx01 = fmaf(a101.x - a001.x, t0, a001.x);
+      const float vx11 = fmaf(a111.x - a011.x, t0, a011.x);
+      const float vxx1 = fmaf(vx11 - vx01, t1, vx01);
+      v = fmaf(vxx1 - v, t2, v);
+      const float hx0 = fmaf(a101.y - a001.y, t0, a001.y), hx1 = fmaf(a111.y - a011.y, t0, a011.y);
+      const float hy0 = fmaf(a101.z - a001.z, t0, a001.z), hy1 = fmaf(a111.z - a011.z, t0, a011.z);
+      const float hz0 = fmaf(a101.w - a001.w, t0, a001.w), hz1 = fmaf(a111.w - a011.w, t0, a011.w);
+      const float s0 = fmaf(hx1 - hx0, t1, hx0), s1 = fmaf(hy1 - hy0, t1, hy0), s2 = fmaf(hz1 - hz0, t1, hz0);
+      g.x = fmaf(s0 - g.x, t2, g.x);
+      g.y = fmaf(s1 - g.y, t2, g.y);
+      g.z = fmaf(s2 - g.z, t2, g.z);
+    }
+    *grad = g;
+  } else {
+    const typename P::pixel* img = static_cast<const typename P::pixel*>(f.moving_px);
+    const float v000 = P::ld32(img, cb), v100 = P::ld32(img, cb + cx);
+    const float v010 = P::ld32(img, cb + cy), v110 = P::ld32(img, cb + cy + cx);
+    const float vx00 = fmaf(v100 - v000, t0, v000);
+    const float vx10 = fmaf(v110 - v010, t0, v010);
+    v = fmaf(vx10 - vx00, t1, vx00);
+    if (D == 3) {
+      const float v001 = P::ld32(img, cb + cz), v101 = P::ld32(img, cb + cz + cx);
+      const float v011 = P::ld32(img, cb + cz + cy);
+      const float v111 = P::ld32(img, cb + cz + cy + cx);
+      const float vx01 = fmaf(v101 - v001, t0, v001);
+      const float vx11 = fmaf(v111 - v011, t0, v011);
+      const float vxx1 = fmaf(vx11 - vx01, t1, vx01);
+      v = fmaf(vxx1 - v, t2, v);
+    }
   }
   out = out | (v < f.mmin - f.ev) | (v > f.mmax + f.ev);
   uncertain = uncertain | (v < f.mmin + f.ev) | (v > f.mmax - f.ev);
   const float mterm = fmaf(v, f.inv_mbs, -f.mnm);
   mbin = fast_bin(mterm, f.bt_m, f.bins, &uncertain);
   arg0 = (float)(mbin - 1) - mterm;
+  if constexpr (TEX) *mterm_out = mterm;
   const typename P::pixel* fimg = static_cast<const typename P::pixel*>(f.fixed_px);
   float fv;
   if (f.fixed_interp) {
@@ -321,30 +361,55 @@ __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, const EvalArg
   }
 }
 
-// Exact FP64 evaluation of the listed samples of one pass-1 workgroup, as a
-// kernel of its own (B-spline: its exact path would raise pass 1's register
-// count if it ran in pass 1's tail).  Their
-// Parzen weights are summed per histogram bin in an LDS hash table (open
-// addressing; integer sums, so the order does not matter), then each touched
-// bin takes one global atomic.
+// Exceptions of kExcListsPerGroup consecutive pass-1 lists per workgroup:
+// the lists' counts and prefix in LDS, then one listed sample per thread.
+constexpr int kExcListsPerGroup = 8;
+
+struct ExcLists {
+  uint32_t first;                      // first list of the workgroup
+  uint32_t pre[kExcListsPerGroup + 1];  // prefix of the clamped list lengths
+};
+
+__device__ __forceinline__ void exc_lists_setup(const FastArgs& f, uint32_t lb, ExcLists& e) {
+  if (threadIdx.x == 0) {
+    e.first = lb * kExcListsPerGroup;
+    uint32_t acc = 0;
+    for (int l = 0; l < kExcListsPerGroup; ++l) {
+      e.pre[l] = acc;
+      acc += min(f.exc_count[e.first + l], (uint32_t)kExcCap);
+    }
+    e.pre[kExcListsPerGroup] = acc;
+  }
+  __syncthreads();
+}
+
+// List slot (list * kExcCap + position) of the workgroup's t-th listed sample.
+__device__ __forceinline__ uint32_t exc_lists_slot(const ExcLists& e, uint32_t t) {
+  int l = 0;
+  while (t >= e.pre[l + 1]) ++l;
+  return (e.first + l) * kExcCap + (t - e.pre[l]);
+}
+
+__device__ __forceinline__ uint32_t exc_lists_sample(const FastArgs& f, const ExcLists& e, uint32_t t) {
+  return f.exc_list[exc_lists_slot(e, t)];
+}
+
+// Exact FP64 evaluation of the listed samples of pass 1, as a kernel of its
+// own (B-spline: its exact path would raise pass 1's register count if it ran
+// in pass 1's tail).  Histogram bins take global atomics (integer sums, so the
+// order does not matter); the fixed marginal and the count are summed in LDS
+// first.
 template <int D, typename P, int MK>
 __device__ __forceinline__ void exceptions_pass1_body(const EvalArgs& a, const FastArgs& f, uint32_t lb) {
-  extern __shared__ unsigned long long lds_u64e[];
-  const uint32_t ne = min(f.exc_count[lb], (uint32_t)kExcCap);
-  if (ne == 0) return;  // uniform over the workgroup
+  __shared__ ExcLists e;
+  extern __shared__ unsigned int fmarg_e[];  // [B] fixed marginal counts, [B] valid count
   const int B = a.bins;
-  unsigned long long* sums = lds_u64e;                                          // [kExcSlots]
-  unsigned int* keys = reinterpret_cast<unsigned int*>(sums + kExcSlots);       // [kExcSlots]
-  unsigned int* fmarg = keys + kExcSlots;  // [B] fixed marginal counts, [B] valid count
-  constexpr unsigned int kEmpty = 0xFFFFFFFFu;
-  for (int i = threadIdx.x; i < kExcSlots; i += blockDim.x) {
-    keys[i] = kEmpty;
-    sums[i] = 0ull;
-  }
-  for (int i = threadIdx.x; i <= B; i += blockDim.x) fmarg[i] = 0u;
-  __syncthreads();
+  for (int i = threadIdx.x; i <= B; i += blockDim.x) fmarg_e[i] = 0u;
+  exc_lists_setup(f, lb, e);
+  const uint32_t ne = e.pre[kExcListsPerGroup];
+  if (ne == 0) return;  // uniform over the workgroup
   for (uint32_t t = threadIdx.x; t < ne; t += blockDim.x) {
-    const uint32_t s = f.exc_list[lb * kExcCap + t];
+    const uint32_t s = exc_lists_sample(f, e, t);
     double vp[3];
     virtual_point<D>(a, s, vp);
     SampleOut<D> o;
@@ -353,32 +418,38 @@ __device__ __forceinline__ void exceptions_pass1_body(const EvalArgs& a, const F
       a.sample_fbin[s] = ok ? (int16_t)o.fbin : (int16_t)-1;
       a.sample_mbin[s] = ok ? (int16_t)o.mbin : (int16_t)-1;
     }
+    if constexpr (MK == 2) {
+      // what pass 2 needs of this sample, at its list slot
+      ExcResult r;
+      r.fbin = -1;
+      r.mbin = 0;
+      r.arg = 0.0f;
+      r.g[0] = r.g[1] = r.g[2] = 0.0f;
+      if (ok && o.sup.inside) {
+        r.fbin = o.fbin;
+        r.mbin = o.mbin;
+        r.arg = (float)((double)(o.mbin - 1) - o.mterm);
+        const float3 g = interp_gradient3<D, P>(static_cast<const typename P::texel*>(a.texels), o.cell);
+        r.g[0] = g.x;
+        r.g[1] = g.y;
+        r.g[2] = g.z;
+      }
+      f.exc_res[exc_lists_slot(e, t)] = r;
+    }
     if (!ok) continue;
-    atomicAdd(&fmarg[B], 1u);
-    atomicAdd(&fmarg[o.fbin], 1u);
+    atomicAdd(&fmarg_e[B], 1u);
+    atomicAdd(&fmarg_e[o.fbin], 1u);
     unsigned int q[4];
     parzen_fixed((float)((double)(o.mbin - 1) - o.mterm), q);
-    const unsigned int base = (unsigned int)(o.fbin * B + (o.mbin - 1));
+    unsigned long long* row = a.hist + o.fbin * B + (o.mbin - 1);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (!q[k]) continue;
-      const unsigned int bin = base + k;
-      unsigned int h = (bin * 2654435761u) >> (32 - 11);  // kExcSlots = 2^11
-      for (;;) {
-        const unsigned int prev = atomicCAS(&keys[h], kEmpty, bin);
-        if (prev == kEmpty || prev == bin) break;
-        h = (h + 1) & (kExcSlots - 1);
-      }
-      atomicAdd(&sums[h], (unsigned long long)q[k]);
-    }
+    for (int k = 0; k < 4; ++k)
+      if (q[k]) atomicAdd(&row[k], (unsigned long long)q[k]);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kExcSlots; i += blockDim.x)
-    if (keys[i] != kEmpty && sums[i]) atomicAdd(&a.hist[keys[i]], sums[i]);
-  // one global atomic per non-zero count of the workgroup
   for (int i = threadIdx.x; i < B; i += blockDim.x)
-    if (fmarg[i]) atomicAdd(&a.fmarg[i], (unsigned long long)fmarg[i]);
-  if (threadIdx.x == 0 && fmarg[B]) atomicAdd(a.nvalid, (unsigned long long)fmarg[B]);
+    if (fmarg_e[i]) atomicAdd(&a.fmarg[i], (unsigned long long)fmarg_e[i]);
+  if (threadIdx.x == 0 && fmarg_e[B]) atomicAdd(a.nvalid, (unsigned long long)fmarg_e[B]);
 }
 
 // Derivative moments in index space, per sample:
@@ -404,30 +475,6 @@ __device__ __forceinline__ void exceptions_pass2_body(const EvalArgs& a, const F
   __shared__ double scratch[4 * 12];
   constexpr int PA = D * D + D;
   const uint32_t ne = min(f.exc_count[lb], (uint32_t)f.exc_cap);
-  if constexpr (MK == 2) {
-    // B-spline: each listed sample rewrites its own per-sample factors (no
-    // sum here, so no ordering is needed)
-    for (uint32_t t = threadIdx.x; t < ne; t += blockDim.x) {
-      const uint32_t s = f.exc_list[lb * f.exc_cap + t];
-      double vp[3];
-      virtual_point<D>(a, s, vp);
-      SampleOut<D> o;
-      float gw[3] = {0.0f, 0.0f, 0.0f};
-      if (eval_sample<D, 2, P>(a, vp, o) && o.sup.inside) {
-        const float w = parzen_deriv_weight((float)((double)(o.mbin - 1) - o.mterm),
-                                            f.pratio_f + o.fbin * a.bins + (o.mbin - 1));
-        if (w != 0.0f) {
-          const float3 g = interp_gradient3<D, P>(static_cast<const typename P::texel*>(a.texels), o.cell);
-          gw[0] = (float)((double)w * (double)g.x);
-          gw[1] = (float)((double)w * (double)g.y);
-          gw[2] = (float)((double)w * (double)g.z);
-        }
-      }
-#pragma unroll
-      for (int d = 0; d < D; ++d) f.bspline_g[(int64_t)d * f.n + s] = gw[d];
-    }
-    return;
-  }
   double acc[PA];
 #pragma unroll
   for (int i = 0; i < PA; ++i) acc[i] = 0.0;
@@ -467,7 +514,10 @@ __device__ __forceinline__ void exceptions_pass2_body(const EvalArgs& a, const F
   block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)(lg + lb) * PA);
 }
 
-template <int D, typename P, int MK>
+// TEX: the moving values come from the texels' value channel, gathered with
+// the gradient channels (one load per corner instead of two); the same
+// values, so the same classification and results.
+template <int D, typename P, int MK, bool TEX = false>
 __device__ __forceinline__ void pass2_fast_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
   extern __shared__ float lds_f32[];
   const int B = f.bins;
@@ -489,7 +539,13 @@ __device__ __forceinline__ void pass2_fast_body(const FastArgs& f, const EvalArg
     float arg0 = 0.0f;
     Cell cell;
     bool bsin;
-    const int c = fast_classify<D, P, MK>(f, idx, fbin, mbin, arg0, cell, &bsin);
+    float3 gt = make_float3(0.0f, 0.0f, 0.0f);
+    float mterm_unused;
+    const int c = TEX ? fast_classify<D, P, MK, true>(f, idx, fbin, mbin, arg0, cell, &bsin, &gt, &mterm_unused)
+                      : fast_classify<D, P, MK>(f, idx, fbin, mbin, arg0, cell, &bsin);
+    auto gradient = [&]() {
+      return TEX ? gt : interp_gradient3<D, P, true>(static_cast<const typename P::texel*>(f.texels), cell);
+    };
     if constexpr (MK == 2) {
       // per-sample factors of the separable B-spline derivative, for every
       // sample (uncertain ones are rewritten by the exception kernel)
@@ -497,25 +553,48 @@ __device__ __forceinline__ void pass2_fast_body(const FastArgs& f, const EvalArg
       if (c == 1 && bsin) {
         const float w = parzen_deriv_weight_fast(arg0, pr + fbin * B + (mbin - 1));
         if (w != 0.0f) {
-          const float3 g = interp_gradient3<D, P, true>(static_cast<const typename P::texel*>(f.texels), cell);
+          const float3 g = gradient();
           gw[0] = (float)((double)w * (double)g.x);
           gw[1] = (float)((double)w * (double)g.y);
           gw[2] = (float)((double)w * (double)g.z);
         }
       }
+      // listed samples are written by the tail below from pass 1's results
+      if (c != 2) {
 #pragma unroll
-      for (int d = 0; d < D; ++d) f.bspline_g[(int64_t)d * f.n + s] = gw[d];
+        for (int d = 0; d < D; ++d) f.bspline_g[(int64_t)d * f.n + s] = gw[d];
+      }
     } else {
       if (c != 1) continue;
       const float w = parzen_deriv_weight_fast(arg0, pr + fbin * B + (mbin - 1));
       if (w == 0.0f) continue;
-      add_moments<D>(acc, w, interp_gradient3<D, P, true>(static_cast<const typename P::texel*>(f.texels), cell), idx);
+      add_moments<D>(acc, w, gradient(), idx);
     }
   }
   if constexpr (MK == 1) {
     block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)lb * PA);
     // this workgroup's listed samples, exactly, into their own partial row
     exceptions_pass2_body<D, P, 1>(a, f, lb, lg);
+  } else {
+    // this workgroup's listed samples (the same partition as pass 1): their
+    // per-sample factors from the exact results pass 1's exception kernel
+    // stored, with the exact path's weight and product
+    const uint32_t ne = min(f.exc_count[lb], (uint32_t)kExcCap);
+    for (uint32_t t = threadIdx.x; t < ne; t += blockDim.x) {
+      const uint32_t slot = lb * kExcCap + t;
+      const uint32_t s = f.exc_list[slot];
+      const ExcResult r = f.exc_res[slot];
+      float gw[3] = {0.0f, 0.0f, 0.0f};
+      if (r.fbin >= 0) {
+        const float w = parzen_deriv_weight(r.arg, f.pratio_f + r.fbin * B + (r.mbin - 1));
+        if (w != 0.0f) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) gw[d] = (float)((double)w * (double)r.g[d]);
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) f.bspline_g[(int64_t)d * f.n + s] = gw[d];
+    }
   }
 }
 
@@ -592,17 +671,20 @@ __global__ void __launch_bounds__(256) pass2_fast_kernel(FastArgs f, EvalArgs a)
   pass2_fast_body<D, P, MK>(f, a, blockIdx.x, gridDim.x);
 }
 
+// B-spline pass 2 with the moving values from the texels (measured faster for
+// the B-spline kernel; the affine kernel's registers would rise past 128 and
+// cost a wave per SIMD, which measured slower)
+template <int D, typename P>
+__global__ void __launch_bounds__(256) pass2_tex_kernel(FastArgs f, EvalArgs a) {
+  pass2_fast_body<D, P, 2, true>(f, a, blockIdx.x, gridDim.x);
+}
+
 template <int D, typename P>
 __global__ void __launch_bounds__(256) pass2_fast_batch(const FastArgs* __restrict__ fa,
                                                         const EvalArgs* __restrict__ ea,
                                                         const int32_t* __restrict__ start, int n) {
   const int r = batch_item(start, n, blockIdx.x);
   pass2_fast_body<D, P, 1>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
-}
-
-template <int D, typename P, int MK>
-__global__ void __launch_bounds__(256) exceptions_pass2_kernel(EvalArgs a, FastArgs f) {
-  exceptions_pass2_body<D, P, MK>(a, f, blockIdx.x, gridDim.x);
 }
 
 template <int D>
@@ -716,19 +798,18 @@ hipError_t launch_fast_pass1(const FastArgs& f, const EvalArgs& a, int dim, int 
 }
 
 hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st) {
-  MMI_DISPATCH_FAST_MK(exceptions_pass1_kernel, f.mk, f.storage, a.virt.dim, grid, exceptions1_lds_bytes(a.bins), st,
-                       a, f);
+  // grid (the pass-1 workgroups) is a multiple of 8 = kExcListsPerGroup
+  MMI_DISPATCH_FAST_MK(exceptions_pass1_kernel, f.mk, f.storage, a.virt.dim, grid / kExcListsPerGroup,
+                       ((size_t)a.bins + 1) * 4, st, a, f);
   return hipGetLastError();
 }
 
 hipError_t launch_fast_pass2(const FastArgs& f, const EvalArgs& a, int dim, int grid, hipStream_t st) {
   const size_t lds = ((size_t)f.bins * f.bins + 128) * 4;
-  MMI_DISPATCH_FAST_MK(pass2_fast_kernel, f.mk, f.storage, dim, grid, lds, st, f, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_fast_exceptions2(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st) {
-  MMI_DISPATCH_FAST_MK(exceptions_pass2_kernel, f.mk, f.storage, a.virt.dim, grid, 0, st, a, f);
+  if (f.tex_values && f.mk == 2)
+    MMI_DISPATCH_FAST(pass2_tex_kernel, f.storage, dim, grid, lds, st, f, a);
+  else
+    MMI_DISPATCH_FAST_MK(pass2_fast_kernel, f.mk, f.storage, dim, grid, lds, st, f, a);
   return hipGetLastError();
 }
 

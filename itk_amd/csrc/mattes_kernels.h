@@ -85,6 +85,23 @@ struct FastArgs {
   double vorg[3], vsp[3];     // virtual origin and (diagonal) spacing
   double mp2i[9], morg[3];    // moving physical_to_index and origin
   float* bspline_g;           // [D][n] per-sample factors of the separable derivative
+  // 1: the B-spline pass 2 reads the moving values from the texels' value
+  // channel (it equals the level's moving image), one texel load per corner
+  int32_t tex_values;
+  int32_t pad_tex_;
+  // B-spline: the exact results of the listed samples, one per list slot,
+  // written by the pass-1 exception kernel for pass 2's tail
+  struct ExcResult* exc_res;
+};
+
+// Exact result of one listed B-spline sample: bins, the FP32 Parzen
+// argument, the gradient at its cell; fbin < 0 when the sample is not counted
+// or outside the transform's valid region.
+struct ExcResult {
+  int32_t fbin, mbin;
+  float arg;
+  float g[3];
+  int32_t pad_[2];
 };
 
 // Map from index-space derivative moments to the affine parameters.
@@ -266,7 +283,6 @@ hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int gri
 // the exact path (their own partial row).
 hipError_t launch_fast_pass2(const FastArgs& f, const EvalArgs& a, int dim, int grid, hipStream_t st);
 // B-spline fast path: pass 2's listed samples on the exact path.
-hipError_t launch_fast_exceptions2(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st);
 hipError_t launch_fast_derivative(const double* partials, int rows, const FastMap& map, int dim,
                                   const DevResult* r, double* deriv, hipStream_t st);
 hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, const DevResult* r, double* deriv,

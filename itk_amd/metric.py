@@ -41,11 +41,12 @@ class MattesMutualInformationImageToImageMetricv4:
     domain or sparse over a point set, with optional image masks."""
 
     def __init__(self, device: int = 0, record_sample_bins: bool = False, exact_path_only: bool = False,
-                 time_passes: bool = False):
+                 time_passes: bool = False, texel_values: bool = True):
         self._L = N.load()
         self._ctx = ctypes.c_void_p()
         flags = (N.MMI_FLAG_RECORD_SAMPLE_BINS if record_sample_bins else 0) | (
-            N.MMI_FLAG_EXACT_PATH_ONLY if exact_path_only else 0) | (N.MMI_FLAG_TIME_PASSES if time_passes else 0)
+            N.MMI_FLAG_EXACT_PATH_ONLY if exact_path_only else 0) | (N.MMI_FLAG_TIME_PASSES if time_passes else 0) | (
+            0 if texel_values else N.MMI_FLAG_SEPARATE_VALUE_GATHERS)
         N.check(self._L.mmi_context_create(device, flags, ctypes.byref(self._ctx)))
         self._bins = 50
         self._out = np.zeros(2)  # [value (double), valid count (int64 bits)]
@@ -397,6 +398,13 @@ class MattesMutualInformationImageToImageMetricv4:
         N.check(self._L.mmi_get_histogram_parameters(self._ctx, *[ctypes.byref(v) for v in vals]))
         keys = ("fixed_min", "fixed_max", "moving_min", "moving_max", "fixed_bin_size", "moving_bin_size")
         return {k: v.value for k, v in zip(keys, vals)}
+
+    def GetExceptionCount(self) -> tuple:
+        """(samples the last fast-path evaluation sent to the exact path, or -1
+        off the fast path; samples per evaluation)."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        N.check(self._L.mmi_get_exception_count(self._ctx, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def GetLastEvaluationPath(self) -> str:
         """"exact", "fast" or "fast_overflow_exact" for the last evaluation."""

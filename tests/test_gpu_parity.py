@@ -350,3 +350,44 @@ def test_bspline_fast_path_matches_exact_path(oracle_mod, shape, mesh):
     assert v0 == pytest.approx(v1, rel=1e-7)
     assert np.linalg.norm(d0 - d1) <= 1e-5 * np.linalg.norm(d1)
     _compare(oracle_mod, m_fast, fixed, moving, geom, tf, gradient=grad)
+
+
+@pytest.mark.parametrize("case", ["3d_f32", "3d_bf16", "2d_f32", "3d_smoothed_user_gradient", "3d_80_bins",
+                                  "3d_bspline"])
+def test_texel_value_gathers_match_separate_gathers(oracle_mod, case):
+    """Pass 2 taking the moving values from the texels' value channel gives
+    the bits of the pass 2 that gathers them from the value image."""
+    shape = (60, 52, 44) if case.startswith("3d") else (200, 180)
+    fixed, moving, geom = blob_problem(shape, 140)
+    d = geom.dim
+    bs = case == "3d_bspline"
+    tf = bspline_for(geom, (4, 4, 3), seed=141, sigma=0.7) if bs else \
+        affine_for(geom, affine_params(d, np.random.default_rng(141), scale=2.0))
+    p0 = tf.get_parameters()
+    out = []
+    for tv in (True, False):
+        m = Metric(texel_values=tv)
+        if case == "3d_bf16":
+            m.SetImageStorage("bf16")
+        m.SetFixedImage(fixed, geom)
+        m.SetMovingImage(moving, geom)
+        if case == "3d_smoothed_user_gradient":
+            m.SetMovingImageGradient(oracle_mod.gradient_image(moving, geom))
+            m.SetSmoothingSigmas(1.0, 1.0)
+        if case == "3d_80_bins":
+            m.SetNumberOfHistogramBins(80)
+        m.SetMovingTransform(tf)
+        m.Initialize()
+        vals = []
+        for k in range(3):
+            if bs:
+                tf.set_parameters(p0 + np.random.default_rng(142 + k).normal(0, 0.05, p0.size))
+            else:
+                tf.set_parameters(affine_params(d, np.random.default_rng(142 + k), scale=2.0))
+            v, dv = m.GetValueAndDerivative()
+            assert m.GetLastEvaluationPath() == "fast"
+            vals.append((v, dv.copy(), m.GetNumberOfValidPoints()))
+        out.append(vals)
+    for (v0, d0, n0), (v1, d1, n1) in zip(*out):
+        assert n0 == n1 and v0 == v1
+        assert np.array_equal(d0, d1)
