@@ -293,7 +293,7 @@ struct mmi_context {
   bool ftab_valid = false;
   mmi_geometry ftab_key_v{}, ftab_key_f{};
   // separable B-spline derivative: per-sample factors, contraction stages, axis tables
-  DevBuf bs_g, bs_t1, bs_t2, bs_rows;
+  DevBuf bs_g, bs_t1, bs_t2, bs_rows, bs_rows_t1;
   DevBuf bs_start[3], bs_w[3], bs_inside[3], bs_lo[3], bs_hi[3];
   int32_t bs_increasing[3] = {0, 0, 0};
   bool bs_tables = false;
@@ -569,6 +569,8 @@ bool setup_fast_bspline(mmi_context* ctx, FastArgs* F) {
   for (int i = 0; i < D * D; ++i) F->mp2i[i] = m.physical_to_index[i];
   const int64_t Ny = v.size[1], Nz = D == 3 ? v.size[2] : 1;
   if (ctx->bs_rows.ensure(sizeof(double) * Nz * Ny * D * F->bs_gx) != hipSuccess) return false;
+  if (ctx->bs_rows_t1.ensure(sizeof(double) * (D == 3 ? ctx->dmtf.grid.size[2] : 1) * Ny * D * F->bs_gx) != hipSuccess)
+    return false;
   F->bs_rows = ctx->bs_rows.as<double>();
   double maxC = 0.0;
   for (double c : ctx->mtf.params) maxC = std::max(maxC, std::fabs(c));
@@ -958,7 +960,7 @@ int mmi_context_destroy(mmi_context* ctx) {
                     &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->out, &ctx->points, &ctx->mask_px[0], &ctx->mask_px[1],
                     &ctx->mask_desc[0], &ctx->mask_desc[1],
                     &ctx->partials, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
-                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_res, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->ftab, &ctx->work_a,
+                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_res, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->bs_rows_t1, &ctx->ftab, &ctx->work_a,
                     &ctx->work_b, &ctx->work_grad};
   for (DevBuf* b : bufs) b->release();
   for (int j = 0; j < 3; ++j) {
@@ -1521,8 +1523,10 @@ retry:
       rt[k].inside = fa.bs_inside[k];
       rt[k].lo = rt[k].hi = nullptr;
     }
-    MMI_HIP(launch_bspline_rows(ctx->dmtf.coeffs, ctx->dmtf.pdim, ctx->dmtf.grid.size[0], ctx->dmtf.grid.size[1], D,
-                                (int)a.virt.size[1], D == 3 ? (int)a.virt.size[2] : 1, rt, ctx->bs_rows.as<double>(), st));
+    MMI_HIP(launch_bspline_rows(ctx->dmtf.coeffs, ctx->dmtf.pdim, ctx->dmtf.grid.size[0], ctx->dmtf.grid.size[1],
+                                D == 3 ? ctx->dmtf.grid.size[2] : 1, D, (int)a.virt.size[1],
+                                D == 3 ? (int)a.virt.size[2] : 1, rt, ctx->bs_rows_t1.as<double>(),
+                                ctx->bs_rows.as<double>(), st));
   }
   // B-spline pass 2 takes the moving values from the texels it gathers for
   // the gradient when their value channel is the level's moving image

@@ -290,9 +290,10 @@ hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, 
 hipError_t launch_derivative_scale(double* deriv, int64_t P, const DevResult* r, hipStream_t st);
 // Coefficient lines reduced over the y and z supports, per virtual row:
 // rows[((z * Ny + y) * D + d) * Gx + a] = sum_c wz[z][c] sum_b wy[y][b] C_d[sz+c][sy+b][a]
-// (0 for rows outside the valid region).
-hipError_t launch_bspline_rows(const double* coeffs, int32_t pdim, int Gx, int Gy, int D, int Ny, int Nz,
-                               const AxisTable* tabs, double* rows, hipStream_t st);
+// (0 for rows outside the valid region); t1: D * Gz * Ny * Gx doubles of
+// scratch (3-D).
+hipError_t launch_bspline_rows(const double* coeffs, int32_t pdim, int Gx, int Gy, int Gz, int D, int Ny, int Nz,
+                               const AxisTable* tabs, double* t1, double* rows, hipStream_t st);
 hipError_t launch_bspline_axis_table(const DevGeom& virt, const DevGeom& grid, int axis, int32_t* start, double* w,
                                      uint8_t* inside, hipStream_t st);
 // out[o][g][i] = sum_l w[l][g - start[l]] * in[o][l][i] over the virtual

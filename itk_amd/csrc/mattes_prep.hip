@@ -237,28 +237,54 @@ __global__ void texels_to_storage_kernel(const float4* __restrict__ in, uint2* _
   out[i] = u;
 }
 
-// Coefficient lines of the B-spline fast path, one thread per (row, d, a).
-__global__ void bspline_rows_kernel(const double* __restrict__ coeffs, int32_t pdim, int Gx, int Gy, int D, int Ny,
-                                    int64_t total, AxisTable ty, AxisTable tz, double* __restrict__ rows) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Coefficient lines of the B-spline fast path in two contractions, each
+// summing in the order the per-sample displacement would (over the y support
+// inside, then over the z support):
+//   t1[gz][y][d][a]  = sum_b wy[y][b] coef[d][gz][sy(y) + b][a]
+//   rows[z][y][d][a] = sum_c wz[z][c] t1[sz(z) + c][y][d][a]
+// In 2-D the first one writes the rows themselves (Gz = 1).  Zero where the
+// row's support is outside the transform's valid region.
+__global__ void __launch_bounds__(256) bspline_rows_y_kernel(const double* __restrict__ coeffs, int32_t pdim, int Gx,
+                                                             int Gy, int Ny, int per, uint32_t total, AxisTable ty,
+                                                             double* __restrict__ out) {
+  const uint32_t e = blockIdx.x * 256u + threadIdx.x;
   if (e >= total) return;
-  const int a = (int)(e % Gx);
-  int64_t r = e / Gx;
-  const int d = (int)(r % D);
-  r /= D;
-  const int y = (int)(r % Ny);
-  const int z = (int)(r / Ny);
-  double acc = 0.0;
-  if (ty.inside[y] && (D < 3 || tz.inside[z])) {
-    const int sy = ty.start[y];
-    const int sz = D == 3 ? tz.start[z] : 0;
-    const double* c = coeffs + (int64_t)d * pdim;
-    for (int k = 0; k < (D == 3 ? 4 : 1); ++k) {
-      double inner = 0.0;
+  const uint32_t r = e / (uint32_t)per;
+  const int ea = (int)(e - r * (uint32_t)per);
+  const int y = (int)(r % (uint32_t)Ny);
+  const int gz = (int)(r / (uint32_t)Ny);
+  const int d = ea / Gx;
+  const int a = ea - d * Gx;
+  double inner = 0.0;
+  if (ty.inside[y]) {
+    const double* c = coeffs + (int64_t)d * pdim + a + (int64_t)Gx * (ty.start[y] + (int64_t)Gy * gz);
+    double v[4];
 #pragma unroll
-      for (int b = 0; b < 4; ++b) inner += ty.w[y * 4 + b] * c[a + (int64_t)Gx * (sy + b + (int64_t)Gy * (sz + k))];
-      acc += (D == 3 ? tz.w[z * 4 + k] : 1.0) * inner;
-    }
+    for (int b = 0; b < 4; ++b) v[b] = c[Gx * b];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) inner += ty.w[y * 4 + b] * v[b];
+  }
+  out[e] = inner;
+}
+
+__global__ void __launch_bounds__(256) bspline_rows_z_kernel(const double* __restrict__ t1, int Ny, int per,
+                                                             uint32_t total, AxisTable ty, AxisTable tz,
+                                                             double* __restrict__ rows) {
+  const uint32_t e = blockIdx.x * 256u + threadIdx.x;
+  if (e >= total) return;
+  const uint32_t r = e / (uint32_t)per;
+  const int ea = (int)(e - r * (uint32_t)per);
+  const int y = (int)(r % (uint32_t)Ny);
+  const int z = (int)(r / (uint32_t)Ny);
+  double acc = 0.0;
+  if (ty.inside[y] && tz.inside[z]) {
+    const int64_t plane = (int64_t)Ny * per;
+    const double* t = t1 + ((int64_t)tz.start[z] * Ny + y) * per + ea;
+    double v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = t[c * plane];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc += tz.w[z * 4 + c] * v[c];
   }
   rows[e] = acc;
 }
@@ -588,12 +614,21 @@ hipError_t launch_clear_eval(unsigned long long* hist, int n_hist, unsigned long
   return hipGetLastError();
 }
 
-hipError_t launch_bspline_rows(const double* coeffs, int32_t pdim, int Gx, int Gy, int D, int Ny, int Nz,
-                               const AxisTable* tabs, double* rows, hipStream_t st) {
-  const int64_t total = (int64_t)Nz * Ny * D * Gx;
-  const AxisTable tz = D == 3 ? tabs[2] : tabs[1];
-  hipLaunchKernelGGL(bspline_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, coeffs, pdim, Gx,
-                     Gy, D, Ny, total, tabs[1], tz, rows);
+hipError_t launch_bspline_rows(const double* coeffs, int32_t pdim, int Gx, int Gy, int Gz, int D, int Ny, int Nz,
+                               const AxisTable* tabs, double* t1, double* rows, hipStream_t st) {
+  const int per = D * Gx;
+  if (D == 2) {
+    const uint32_t total = (uint32_t)((int64_t)Ny * per);
+    hipLaunchKernelGGL(bspline_rows_y_kernel, dim3((total + 255) / 256), dim3(256), 0, st, coeffs, pdim, Gx, Gy, Ny,
+                       per, total, tabs[1], rows);
+    return hipGetLastError();
+  }
+  const uint32_t t1n = (uint32_t)((int64_t)Gz * Ny * per);
+  hipLaunchKernelGGL(bspline_rows_y_kernel, dim3((t1n + 255) / 256), dim3(256), 0, st, coeffs, pdim, Gx, Gy, Ny, per,
+                     t1n, tabs[1], t1);
+  const uint32_t total = (uint32_t)((int64_t)Nz * Ny * per);
+  hipLaunchKernelGGL(bspline_rows_z_kernel, dim3((total + 255) / 256), dim3(256), 0, st, t1, Ny, per, total, tabs[1],
+                     tabs[2], rows);
   return hipGetLastError();
 }
 
