@@ -41,8 +41,8 @@ BYTES_PER_SAMPLE_3D = {"f32": 20, "bf16": 10, "f16": 10}
 # rocprofv3 --pmc runs summarised in profiles/mattes_affine_256cube_fast_path.md
 # and profiles/mattes_bspline_192cube_mesh16.md.  None where not profiled.
 TRAFFIC_BYTES_PER_EVAL = {
-    ("mattes_mi_dense_value_and_derivative_affine_256cube", "f32"): 583.8e6,
-    ("mattes_mi_dense_value_and_derivative_bspline_192cube_mesh16", "f32"): 506.9e6,
+    ("mattes_mi_dense_value_and_derivative_affine_256cube", "f32"): 462.8e6,
+    ("mattes_mi_dense_value_and_derivative_bspline_192cube_mesh16", "f32"): 491.6e6,
 }
 
 
@@ -59,8 +59,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--separate-value-gathers", action="store_true",
-                    help="pass 2 gathers moving values apart from the texels (comparison runs)")
+    ap.add_argument("--pass2-reclassify", action="store_true",
+                    help="pass 2 classifies every sample again (comparison runs)")
     ap.add_argument("--serving", action="store_true", help="registration serving workload")
     ap.add_argument("--requests", type=int, default=32, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=40.0, help="serving: mean arrivals per second")
@@ -283,7 +283,7 @@ def main():
     wl = Workload(args)
     problem, tf = wl.problem, wl.tf
     geom = problem.geometry
-    m = MattesMutualInformationImageToImageMetricv4(device=0, texel_values=not args.separate_value_gathers)
+    m = MattesMutualInformationImageToImageMetricv4(device=0, pass2_reclassify=args.pass2_reclassify)
     m.SetImageStorage(args.storage)
     m.SetFixedImage(problem.fixed, geom)
     m.SetMovingImage(problem.moving, geom)

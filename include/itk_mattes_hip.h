@@ -62,8 +62,8 @@ const char* mmi_last_error(void);
 #define MMI_FLAG_RECORD_SAMPLE_BINS 1u /* keep per-sample Parzen bins for mmi_get_sample_bins */
 #define MMI_FLAG_EXACT_PATH_ONLY 2u    /* evaluate every sample on the FP64 path (no certified FP32 fast path) */
 #define MMI_FLAG_TIME_PASSES 4u        /* time pass 1 and pass 2 apart (two more events per evaluation) */
-#define MMI_FLAG_SEPARATE_VALUE_GATHERS 8u /* B-spline fast path: pass 2 gathers moving values from the value
-                                             image as well as the texels (same results; for comparison runs) */
+#define MMI_FLAG_PASS2_RECLASSIFY 8u   /* fast path: pass 2 classifies every sample again instead of reading
+                                          the byte per sample pass 1 leaves (same results; comparison runs) */
 
 /* Create a context on a HIP device with the flags above (0 for defaults). */
 int mmi_context_create(int device, uint32_t flags, mmi_context** out);

@@ -26,7 +26,7 @@ MMI_GRADIENT_CENTRAL_DIFFERENCE, MMI_GRADIENT_IMAGE = 0, 1
 MMI_FLAG_RECORD_SAMPLE_BINS = 1
 MMI_FLAG_EXACT_PATH_ONLY = 2
 MMI_FLAG_TIME_PASSES = 4
-MMI_FLAG_SEPARATE_VALUE_GATHERS = 8
+MMI_FLAG_PASS2_RECLASSIFY = 8
 MMI_SAMPLING_NONE, MMI_SAMPLING_REGULAR, MMI_SAMPLING_RANDOM = 0, 1, 2
 
 _dp = ctypes.POINTER(ctypes.c_double)

@@ -286,6 +286,7 @@ struct mmi_context {
   DevBuf out;
   DevBuf exc_count, exc_list;  // fast-path exception lists
   DevBuf exc_res;              // B-spline: exact results of the listed samples
+  DevBuf cls;                  // pass 1's byte per sample for pass 2
   DevBuf ftab;                               // fast path: per-axis fixed cells (coarse levels)
   // grow-only work space of the preparation filters (no device frees, which
   // would synchronise the whole device, while other contexts run)
@@ -960,7 +961,7 @@ int mmi_context_destroy(mmi_context* ctx) {
                     &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->out, &ctx->points, &ctx->mask_px[0], &ctx->mask_px[1],
                     &ctx->mask_desc[0], &ctx->mask_desc[1],
                     &ctx->partials, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
-                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_res, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->bs_rows_t1, &ctx->ftab, &ctx->work_a,
+                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_res, &ctx->cls, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->bs_rows_t1, &ctx->ftab, &ctx->work_a,
                     &ctx->work_b, &ctx->work_grad};
   for (DevBuf* b : bufs) b->release();
   for (int j = 0; j < 3; ++j) {
@@ -1528,9 +1529,13 @@ retry:
                                 D == 3 ? (int)a.virt.size[2] : 1, rt, ctx->bs_rows_t1.as<double>(),
                                 ctx->bs_rows.as<double>(), st));
   }
-  // B-spline pass 2 takes the moving values from the texels it gathers for
-  // the gradient when their value channel is the level's moving image
-  fa.tex_values = (ctx->texel_values_level && !(ctx->flags & MMI_FLAG_SEPARATE_VALUE_GATHERS)) ? 1 : 0;
+  // pass 1 leaves a byte per sample (fixed bin or skip) for pass 2, which
+  // then also takes the moving values from the texels it gathers for the
+  // gradient: their value channel must be the level's moving image
+  if (fast && want_derivative && B <= 253 && !(ctx->flags & MMI_FLAG_PASS2_RECLASSIFY) && ctx->texel_values_level) {
+    MMI_HIP(ctx->cls.ensure((size_t)n));
+    fa.cls = ctx->cls.as<uint8_t>();
+  }
   if (fast) {
     MMI_HIP(launch_fast_pass1(fa, a, D, ctx->grid, st));
     if (fa.mk == 2) MMI_HIP(launch_fast_exceptions1(a, fa, ctx->grid, st));

@@ -312,6 +312,7 @@ __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, const EvalArg
     Cell cell;
     bool bsin;
     const int c = fast_classify<D, P, MK>(f, idx, fbin, mbin, arg0, cell, &bsin);
+    if (f.cls) f.cls[s] = (uint8_t)(c == 1 ? (uint32_t)fbin : (c == 2 ? kClsListed : kClsSkip));
     if (c == 2) {
       // listed for the exact path (here and in pass 2)
       const unsigned int slot = atomicAdd(&counters[1], 1u);
@@ -517,7 +518,11 @@ __device__ __forceinline__ void exceptions_pass2_body(const EvalArgs& a, const F
 // TEX: the moving values come from the texels' value channel, gathered with
 // the gradient channels (one load per corner instead of two); the same
 // values, so the same classification and results.
-template <int D, typename P, int MK, bool TEX = false>
+// CLS: pass 1's byte per sample says which samples count and their fixed
+// bins, so pass 2 skips the others before any gather and drops its bound
+// tests and fixed-image reads (the classifier's unused results are not
+// computed); the moving side is evaluated as in pass 1, so the bits agree.
+template <int D, typename P, int MK, bool TEX = false, bool CLS = false>
 __device__ __forceinline__ void pass2_fast_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
   extern __shared__ float lds_f32[];
   const int B = f.bins;
@@ -533,6 +538,19 @@ __device__ __forceinline__ void pass2_fast_body(const FastArgs& f, const EvalArg
   uint32_t s, end, step;
   run_range<D>(f, lb, lg, s, end, step);
   for (; s < end; s += step) {
+    uint32_t cb = 0;
+    if constexpr (CLS) {
+      cb = f.cls[s];
+      if (cb >= kClsListed) {
+        if constexpr (MK == 2) {
+          if (cb == kClsSkip) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) f.bspline_g[(int64_t)d * f.n + s] = 0.0f;
+          }
+        }
+        continue;
+      }
+    }
     int32_t idx[3];
     fast_index<D>(f, s, idx);
     int fbin = 0, mbin = 0;
@@ -541,8 +559,12 @@ __device__ __forceinline__ void pass2_fast_body(const FastArgs& f, const EvalArg
     bool bsin;
     float3 gt = make_float3(0.0f, 0.0f, 0.0f);
     float mterm_unused;
-    const int c = TEX ? fast_classify<D, P, MK, true>(f, idx, fbin, mbin, arg0, cell, &bsin, &gt, &mterm_unused)
-                      : fast_classify<D, P, MK>(f, idx, fbin, mbin, arg0, cell, &bsin);
+    int c = TEX ? fast_classify<D, P, MK, true>(f, idx, fbin, mbin, arg0, cell, &bsin, &gt, &mterm_unused)
+                : fast_classify<D, P, MK>(f, idx, fbin, mbin, arg0, cell, &bsin);
+    if constexpr (CLS) {
+      c = 1;
+      fbin = (int)cb;
+    }
     auto gradient = [&]() {
       return TEX ? gt : interp_gradient3<D, P, true>(static_cast<const typename P::texel*>(f.texels), cell);
     };
@@ -671,12 +693,12 @@ __global__ void __launch_bounds__(256) pass2_fast_kernel(FastArgs f, EvalArgs a)
   pass2_fast_body<D, P, MK>(f, a, blockIdx.x, gridDim.x);
 }
 
-// B-spline pass 2 with the moving values from the texels (measured faster for
-// the B-spline kernel; the affine kernel's registers would rise past 128 and
-// cost a wave per SIMD, which measured slower)
-template <int D, typename P>
-__global__ void __launch_bounds__(256) pass2_tex_kernel(FastArgs f, EvalArgs a) {
-  pass2_fast_body<D, P, 2, true>(f, a, blockIdx.x, gridDim.x);
+// Pass 2 after a pass 1 that wrote its byte per sample, with the moving values
+// interpolated from the texels it gathers for the gradient (one load per
+// corner; the byte per sample keeps the registers under 128 with it).
+template <int D, typename P, int MK>
+__global__ void __launch_bounds__(256) pass2_cls_kernel(FastArgs f, EvalArgs a) {
+  pass2_fast_body<D, P, MK, true, true>(f, a, blockIdx.x, gridDim.x);
 }
 
 template <int D, typename P>
@@ -806,8 +828,8 @@ hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int gri
 
 hipError_t launch_fast_pass2(const FastArgs& f, const EvalArgs& a, int dim, int grid, hipStream_t st) {
   const size_t lds = ((size_t)f.bins * f.bins + 128) * 4;
-  if (f.tex_values && f.mk == 2)
-    MMI_DISPATCH_FAST(pass2_tex_kernel, f.storage, dim, grid, lds, st, f, a);
+  if (f.cls)
+    MMI_DISPATCH_FAST_MK(pass2_cls_kernel, f.mk, f.storage, dim, grid, lds, st, f, a);
   else
     MMI_DISPATCH_FAST_MK(pass2_fast_kernel, f.mk, f.storage, dim, grid, lds, st, f, a);
   return hipGetLastError();

@@ -85,14 +85,17 @@ struct FastArgs {
   double vorg[3], vsp[3];     // virtual origin and (diagonal) spacing
   double mp2i[9], morg[3];    // moving physical_to_index and origin
   float* bspline_g;           // [D][n] per-sample factors of the separable derivative
-  // 1: the B-spline pass 2 reads the moving values from the texels' value
-  // channel (it equals the level's moving image), one texel load per corner
-  int32_t tex_values;
-  int32_t pad_tex_;
   // B-spline: the exact results of the listed samples, one per list slot,
   // written by the pass-1 exception kernel for pass 2's tail
   struct ExcResult* exc_res;
+  // One byte per sample from pass 1 for pass 2 (NULL: pass 2 classifies
+  // again): the fixed bin of a counted certain sample, kClsListed for a
+  // sample on the exact path, kClsSkip for one not counted.
+  uint8_t* cls;
 };
+
+constexpr uint32_t kClsListed = 0xFEu;
+constexpr uint32_t kClsSkip = 0xFFu;
 
 // Exact result of one listed B-spline sample: bins, the FP32 Parzen
 // argument, the gradient at its cell; fbin < 0 when the sample is not counted

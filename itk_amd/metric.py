@@ -41,12 +41,12 @@ class MattesMutualInformationImageToImageMetricv4:
     domain or sparse over a point set, with optional image masks."""
 
     def __init__(self, device: int = 0, record_sample_bins: bool = False, exact_path_only: bool = False,
-                 time_passes: bool = False, texel_values: bool = True):
+                 time_passes: bool = False, pass2_reclassify: bool = False):
         self._L = N.load()
         self._ctx = ctypes.c_void_p()
         flags = (N.MMI_FLAG_RECORD_SAMPLE_BINS if record_sample_bins else 0) | (
             N.MMI_FLAG_EXACT_PATH_ONLY if exact_path_only else 0) | (N.MMI_FLAG_TIME_PASSES if time_passes else 0) | (
-            0 if texel_values else N.MMI_FLAG_SEPARATE_VALUE_GATHERS)
+            N.MMI_FLAG_PASS2_RECLASSIFY if pass2_reclassify else 0)
         N.check(self._L.mmi_context_create(device, flags, ctypes.byref(self._ctx)))
         self._bins = 50
         self._out = np.zeros(2)  # [value (double), valid count (int64 bits)]

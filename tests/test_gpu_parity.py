@@ -354,9 +354,10 @@ def test_bspline_fast_path_matches_exact_path(oracle_mod, shape, mesh):
 
 @pytest.mark.parametrize("case", ["3d_f32", "3d_bf16", "2d_f32", "3d_smoothed_user_gradient", "3d_80_bins",
                                   "3d_bspline"])
-def test_texel_value_gathers_match_separate_gathers(oracle_mod, case):
-    """Pass 2 taking the moving values from the texels' value channel gives
-    the bits of the pass 2 that gathers them from the value image."""
+def test_pass2_from_pass1_classes_matches_reclassifying(oracle_mod, case):
+    """Pass 2 reading pass 1's byte per sample and the moving values from the
+    texels gives the bits of the pass 2 that classifies every sample again
+    (with user texels under smoothing both run the reclassifying pass 2)."""
     shape = (60, 52, 44) if case.startswith("3d") else (200, 180)
     fixed, moving, geom = blob_problem(shape, 140)
     d = geom.dim
@@ -365,8 +366,8 @@ def test_texel_value_gathers_match_separate_gathers(oracle_mod, case):
         affine_for(geom, affine_params(d, np.random.default_rng(141), scale=2.0))
     p0 = tf.get_parameters()
     out = []
-    for tv in (True, False):
-        m = Metric(texel_values=tv)
+    for again in (False, True):
+        m = Metric(pass2_reclassify=again)
         if case == "3d_bf16":
             m.SetImageStorage("bf16")
         m.SetFixedImage(fixed, geom)
