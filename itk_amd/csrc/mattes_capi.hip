@@ -314,7 +314,8 @@ struct mmi_context {
   // sparse sampling: virtual-space points (n_points > 0) instead of the voxels
   int64_t n_points = 0;
   int32_t points_dim = 0;
-  DevBuf points;  // n_points x points_dim doubles
+  DevBuf points;     // n_points x points_dim doubles, in spatial (Morton) order
+  DevBuf point_ids;  // the caller's index of each stored point
   // image masks [0] fixed, [1] moving: pixels and the DevMask the kernels read
   bool have_mask[2] = {false, false};
   mmi_geometry mask_g[2] = {};
@@ -322,6 +323,53 @@ struct mmi_context {
 };
 
 namespace {
+// Morton order of points quantised to 2^10 (3-D) or 2^15 (2-D) cells of their
+// bounding box; LSD radix sort, stable (equal keys keep the caller's order).
+void spatial_order(const double* p, int dim, int64_t n, std::vector<uint32_t>* order) {
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  bool any = false;
+  for (int64_t k = 0; k < n; ++k) {
+    bool fin = true;
+    for (int d = 0; d < dim; ++d) fin = fin && std::isfinite(p[k * dim + d]);
+    if (!fin) continue;
+    for (int d = 0; d < dim; ++d) {
+      const double v = p[k * dim + d];
+      lo[d] = any ? std::min(lo[d], v) : v;
+      hi[d] = any ? std::max(hi[d], v) : v;
+    }
+    any = true;
+  }
+  const int bits = dim == 3 ? 10 : 15;
+  const double cells = (double)((1 << bits) - 1);
+  std::vector<uint32_t> key((size_t)n);
+  for (int64_t k = 0; k < n; ++k) {
+    uint32_t code = 0;
+    for (int d = 0; d < dim; ++d) {
+      const double v = p[k * dim + d];
+      double q = (hi[d] > lo[d] && std::isfinite(v)) ? (v - lo[d]) / (hi[d] - lo[d]) * cells : 0.0;
+      q = std::min(std::max(q, 0.0), cells);
+      const uint32_t c = (uint32_t)q;
+      for (int b = 0; b < bits; ++b) code |= ((c >> b) & 1u) << (b * dim + d);
+    }
+    key[(size_t)k] = code;
+  }
+  std::vector<uint32_t> idx((size_t)n), tmp((size_t)n), ktmp((size_t)n);
+  for (int64_t k = 0; k < n; ++k) idx[(size_t)k] = (uint32_t)k;
+  for (int shift = 0; shift < 32; shift += 8) {
+    size_t count[257] = {0};
+    for (int64_t k = 0; k < n; ++k) ++count[((key[(size_t)k] >> shift) & 255u) + 1];
+    for (int b = 0; b < 256; ++b) count[b + 1] += count[b];
+    for (int64_t k = 0; k < n; ++k) {
+      const size_t dst = count[(key[(size_t)k] >> shift) & 255u]++;
+      tmp[dst] = idx[(size_t)k];
+      ktmp[dst] = key[(size_t)k];
+    }
+    idx.swap(tmp);
+    key.swap(ktmp);
+  }
+  order->swap(idx);
+}
+
 // Samples of one evaluation: the listed points, or the virtual voxels.
 int64_t sample_count(const mmi_context* ctx) {
   return ctx->n_points > 0 ? ctx->n_points : num_pixels(ctx->virtual_g);
@@ -958,7 +1006,7 @@ int mmi_context_destroy(mmi_context* ctx) {
   DevBuf* bufs[] = {&ctx->fixed_px, &ctx->moving_px, &ctx->texels, &ctx->fixed_lp, &ctx->moving_lp, &ctx->texels_lp,
                     &ctx->fixed_work, &ctx->moving_work,
                     &ctx->coeffs, &ctx->hist, &ctx->fmarg,
-                    &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->out, &ctx->points, &ctx->mask_px[0], &ctx->mask_px[1],
+                    &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->out, &ctx->points, &ctx->point_ids, &ctx->mask_px[0], &ctx->mask_px[1],
                     &ctx->mask_desc[0], &ctx->mask_desc[1],
                     &ctx->partials, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
                     &ctx->exc_count, &ctx->exc_list, &ctx->exc_res, &ctx->cls, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->bs_rows_t1, &ctx->ftab, &ctx->work_a,
@@ -1124,8 +1172,18 @@ int mmi_set_virtual_points(mmi_context* ctx, int32_t dim, const double* points, 
   if (n >= (1ll << 31)) return fail(MMI_ERR_ARGUMENT, "at most 2^31 - 1 points");
   MMI_HIP(hipSetDevice(ctx->device));
   if (n > 0) {
+    // stored in Morton order of their bounding-box cells, so that neighbouring
+    // threads read neighbouring image memory; the histogram is an integer sum
+    // (order-free) and the bins are reported at the caller's indices
+    std::vector<uint32_t> order;
+    spatial_order(points, dim, n, &order);
+    std::vector<double> sorted((size_t)n * dim);
+    for (int64_t k = 0; k < n; ++k)
+      for (int d = 0; d < dim; ++d) sorted[(size_t)k * dim + d] = points[(size_t)order[k] * dim + d];
     MMI_HIP(ctx->points.ensure(sizeof(double) * dim * n));
-    MMI_HIP(hipMemcpy(ctx->points.p, points, sizeof(double) * dim * n, hipMemcpyHostToDevice));
+    MMI_HIP(ctx->point_ids.ensure(sizeof(uint32_t) * n));
+    MMI_HIP(hipMemcpy(ctx->points.p, sorted.data(), sizeof(double) * dim * n, hipMemcpyHostToDevice));
+    MMI_HIP(hipMemcpy(ctx->point_ids.p, order.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
   }
   ctx->n_points = n;
   ctx->points_dim = n > 0 ? dim : 0;
@@ -1401,6 +1459,7 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
   a.storage = ctx->active_storage;
   a.n = n;
   a.points = ctx->n_points > 0 ? ctx->points.as<double>() : nullptr;
+  a.point_ids = ctx->n_points > 0 ? ctx->point_ids.as<uint32_t>() : nullptr;
   a.fixed_mask = ctx->have_mask[0] ? ctx->mask_desc[0].as<DevMask>() : nullptr;
   a.moving_mask = ctx->have_mask[1] ? ctx->mask_desc[1].as<DevMask>() : nullptr;
   a.bins = B;

@@ -126,6 +126,7 @@ struct EvalArgs {
   // sparse sampling: n virtual-space points (D doubles each) instead of the
   // virtual voxels; NULL for dense sampling
   const double* points;
+  const uint32_t* point_ids;   // the caller's index of stored point k (points are kept in spatial order)
   const DevMask* fixed_mask;   // NULL: no mask
   const DevMask* moving_mask;  // NULL: no mask
   int32_t storage;        // MMI_F32, MMI_BF16 or MMI_F16

@@ -59,8 +59,9 @@ __global__ void __launch_bounds__(256) pass1_kernel(EvalArgs a) {
     SampleOut<D> o;
     const bool ok = eval_sample<D, MK, P>(a, vp, o);
     if (a.sample_fbin) {
-      a.sample_fbin[s] = ok ? (int16_t)o.fbin : (int16_t)-1;
-      a.sample_mbin[s] = ok ? (int16_t)o.mbin : (int16_t)-1;
+      const uint32_t so = a.point_ids ? a.point_ids[s] : s;
+      a.sample_fbin[so] = ok ? (int16_t)o.fbin : (int16_t)-1;
+      a.sample_mbin[so] = ok ? (int16_t)o.mbin : (int16_t)-1;
     }
     const unsigned long long vm = __ballot(ok);
     if ((threadIdx.x & 63) == 0 && vm) atomicAdd(nvalid, (unsigned int)__popcll(vm));

@@ -30,7 +30,9 @@ def _requests():
 def test_results_same_alone_and_in_the_mix():
     reqs = _requests()
     alone, _ = run_sequential(reqs)
-    server = RegistrationServer(max_wait_s=0.002, max_inflight=8)
+    # a batching window long enough that concurrent registrations meet in a
+    # lane however fast a single evaluation is
+    server = RegistrationServer(max_wait_s=0.05, max_inflight=8)
     mixed = server.run(reqs)
     print("batches %d, evaluations %d, largest batch %d, peak in flight %d"
           % (server.batches, server.batched_evaluations, server.max_batch, server.peak_inflight))
