@@ -644,45 +644,30 @@ bool setup_fast_bspline(mmi_context* ctx, FastArgs* F) {
 // Parameters and error bounds of the certified fast path (FastArgs in
 // mattes_kernels.h; buffers are filled in by the caller).  Returns false when
 // the path does not apply.
-bool setup_fast(mmi_context* ctx, FastArgs* F) {
-  std::memset(F, 0, sizeof(*F));
-  const int D = ctx->fixed_g.dim;
-  const int B = ctx->bins;
-  if (num_pixels(ctx->virtual_g) >= (1ll << 31)) return false;
-  // point sets and masks run on the exact path
-  if (ctx->n_points > 0 || ctx->have_mask[0] || ctx->have_mask[1]) return false;
-  // the fast kernels address the images with 32-bit byte offsets
-  if (num_pixels(ctx->moving_g) * (int64_t)sizeof(float4) >= (1ll << 32) ||
-      num_pixels(ctx->fixed_g) * (int64_t)sizeof(float) >= (1ll << 32))
-    return false;
-  if (ctx->flags & 2u) return false;  // exact path requested
-  const bool bs = ctx->mtf.kind == MMI_TF_BSPLINE;
-  if ((ctx->mtf.kind != MMI_TF_AFFINE && !bs) || ctx->ftf.kind != MMI_TF_IDENTITY) return false;
-  if (ctx->gradient_source != MMI_GRADIENT_IMAGE) return false;
-  if ((size_t)B * B * 8 + (B + 1) * 4 > 64 * 1024) return false;
-  const mmi_geometry& v = ctx->virtual_g;
-  const mmi_geometry& m = ctx->moving_g;
-  const DevTransform& t = ctx->dmtf;
-  F->mk = bs ? 2 : 1;
-  if (bs && !setup_fast_bspline(ctx, F)) return false;
-  if (!bs) {
-  // ci = P2I_m (M (o_v + I2P_v idx) + off - o_m) = A idx + b, in long double
+// Continuous index in geometry g of the point T(o_v + I2P_v idx), T(x) = M x +
+// off, as one affine map of the virtual index: ci = A idx + b (folded in long
+// double), with a bound e on |FP64 A idx + b - the reference's chain of
+// roundings| (index -> point -> transform -> continuous index): 64 ulp of the
+// largest intermediate magnitude over the domain's corners.
+void affine_index_map(const mmi_geometry& v, const double* M, const double* off, const mmi_geometry& g, double* Aout,
+                      double* bout, double* e) {
+  const int D = v.dim;
   long double MI[9] = {0}, A[9] = {0}, bb[3] = {0}, w[3] = {0};
   for (int i = 0; i < D; ++i)
     for (int j = 0; j < D; ++j)
-      for (int k = 0; k < D; ++k) MI[i * D + j] += (long double)t.m[i * D + k] * v.index_to_physical[k * D + j];
+      for (int k = 0; k < D; ++k) MI[i * D + j] += (long double)M[i * D + k] * v.index_to_physical[k * D + j];
   for (int i = 0; i < D; ++i)
     for (int j = 0; j < D; ++j)
-      for (int k = 0; k < D; ++k) A[i * D + j] += (long double)m.physical_to_index[i * D + k] * MI[k * D + j];
+      for (int k = 0; k < D; ++k) A[i * D + j] += (long double)g.physical_to_index[i * D + k] * MI[k * D + j];
   for (int i = 0; i < D; ++i) {
-    long double s = (long double)t.off[i] - m.origin[i];
-    for (int k = 0; k < D; ++k) s += (long double)t.m[i * D + k] * v.origin[k];
+    long double s = (long double)off[i] - g.origin[i];
+    for (int k = 0; k < D; ++k) s += (long double)M[i * D + k] * v.origin[k];
     w[i] = s;
   }
   for (int i = 0; i < D; ++i)
-    for (int k = 0; k < D; ++k) bb[i] += (long double)m.physical_to_index[i * D + k] * w[k];
-  for (int i = 0; i < D * D; ++i) F->A[i] = (double)A[i];
-  for (int i = 0; i < D; ++i) F->b[i] = (double)bb[i];
+    for (int k = 0; k < D; ++k) bb[i] += (long double)g.physical_to_index[i * D + k] * w[k];
+  for (int i = 0; i < D * D; ++i) Aout[i] = (double)A[i];
+  for (int i = 0; i < D; ++i) bout[i] = (double)bb[i];
   // magnitude of every intermediate of the reference's chain over the domain
   double S = 0.0;
   for (int corner = 0; corner < (1 << D); ++corner) {
@@ -698,27 +683,76 @@ bool setup_fast(mmi_context* ctx, FastArgs* F) {
       mag = std::max(mag, s);
     }
     for (int i = 0; i < D; ++i) {
-      double s = std::fabs(t.off[i]);
-      mp[i] = t.off[i];
+      double s = std::fabs(off[i]);
+      mp[i] = off[i];
       for (int j = 0; j < D; ++j) {
-        mp[i] += t.m[i * D + j] * vp[j];
-        s += std::fabs(t.m[i * D + j] * vp[j]);
+        mp[i] += M[i * D + j] * vp[j];
+        s += std::fabs(M[i * D + j] * vp[j]);
       }
       mag = std::max(mag, s);
     }
     for (int i = 0; i < D; ++i) {
       double s = 0.0;
       for (int j = 0; j < D; ++j) {
-        ci[i] += m.physical_to_index[i * D + j] * (mp[j] - m.origin[j]);
-        s += std::fabs(m.physical_to_index[i * D + j]) * (std::fabs(mp[j]) + std::fabs(m.origin[j]));
+        ci[i] += g.physical_to_index[i * D + j] * (mp[j] - g.origin[j]);
+        s += std::fabs(g.physical_to_index[i * D + j]) * (std::fabs(mp[j]) + std::fabs(g.origin[j]));
       }
       mag = std::max(mag, s);
     }
     double rowA = 0.0;
-    for (int i = 0; i < D * D; ++i) rowA = std::max(rowA, std::fabs(F->A[i]));
+    for (int i = 0; i < D * D; ++i) rowA = std::max(rowA, std::fabs(Aout[i]));
     S = std::max(S, mag * (1.0 + rowA));
   }
-  F->e_ci = 64.0 * S * std::ldexp(1.0, -52) + 1e-300;
+  *e = 64.0 * S * std::ldexp(1.0, -52) + 1e-300;
+}
+
+bool setup_fast(mmi_context* ctx, FastArgs* F) {
+  std::memset(F, 0, sizeof(*F));
+  const int D = ctx->fixed_g.dim;
+  const int B = ctx->bins;
+  if (num_pixels(ctx->virtual_g) >= (1ll << 31)) return false;
+  // point sets run on the exact path (masks: affine only, below)
+  if (ctx->n_points > 0) return false;
+  // the fast kernels address the images with 32-bit byte offsets
+  if (num_pixels(ctx->moving_g) * (int64_t)sizeof(float4) >= (1ll << 32) ||
+      num_pixels(ctx->fixed_g) * (int64_t)sizeof(float) >= (1ll << 32))
+    return false;
+  if (ctx->flags & 2u) return false;  // exact path requested
+  const bool bs = ctx->mtf.kind == MMI_TF_BSPLINE;
+  if ((ctx->mtf.kind != MMI_TF_AFFINE && !bs) || ctx->ftf.kind != MMI_TF_IDENTITY) return false;
+  if (ctx->gradient_source != MMI_GRADIENT_IMAGE) return false;
+  if ((size_t)B * B * 8 + (B + 1) * 4 > 64 * 1024) return false;
+  const mmi_geometry& v = ctx->virtual_g;
+  const mmi_geometry& m = ctx->moving_g;
+  const DevTransform& t = ctx->dmtf;
+  F->mk = bs ? 2 : 1;
+  if (bs && !setup_fast_bspline(ctx, F)) return false;
+  if (!bs) affine_index_map(v, t.m, t.off, m, F->A, F->b, &F->e_ci);
+  // image masks (affine): the masks' continuous indices are affine maps of the
+  // virtual index as well, certified like the moving image's
+  const mmi_geometry* mg[2] = {ctx->have_mask[0] ? &ctx->mask_g[0] : nullptr,
+                               ctx->have_mask[1] ? &ctx->mask_g[1] : nullptr};
+  FastMask* fm[2] = {&F->fmask, &F->mmask};
+  for (int k = 0; k < 2; ++k) {
+    fm[k]->px = nullptr;
+    if (!mg[k]) continue;
+    if (bs) return false;
+    double ident[9] = {0}, zero[3] = {0, 0, 0};
+    for (int i = 0; i < D; ++i) ident[i * D + i] = 1.0;
+    // fixed mask at the fixed point (identity fixed transform), moving mask at
+    // the mapped moving point
+    if (k == 0)
+      affine_index_map(v, ident, zero, *mg[k], fm[k]->A, fm[k]->b, &fm[k]->e);
+    else
+      affine_index_map(v, t.m, t.off, *mg[k], fm[k]->A, fm[k]->b, &fm[k]->e);
+    if (fm[k]->e > 1e-6) return false;
+    for (int i = 0; i < 3; ++i) {
+      fm[k]->start[i] = i < D ? (int32_t)mg[k]->start[i] : 0;
+      fm[k]->end[i] = i < D ? (int32_t)(mg[k]->start[i] + mg[k]->size[i] - 1) : 0;
+    }
+    fm[k]->sy = (int32_t)mg[k]->size[0];
+    fm[k]->sz = (int32_t)(mg[k]->size[0] * (D > 1 ? mg[k]->size[1] : 1));
+    fm[k]->px = ctx->mask_px[k].as<uint8_t>();
   }
   if (F->e_ci > 1e-6) return false;
   // intensity bounds
@@ -1817,7 +1851,9 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
     bool fast = false;
     if (int rc = prepare_eval(ctxs[i], want_derivative, a, f, fast)) return rc;
     const int64_t P = (int64_t)ctxs[i]->mtf.params.size();
-    if (fast && f.mk == 1 && P <= 12 && (storage < 0 || (a.virt.dim == dim && a.storage == storage))) {
+    // (masked contexts run their own masked kernels)
+    const bool masked = f.fmask.px || f.mmask.px;
+    if (fast && f.mk == 1 && !masked && P <= 12 && (storage < 0 || (a.virt.dim == dim && a.storage == storage))) {
       dim = a.virt.dim;
       storage = a.storage;
       fused.push_back(i);

@@ -131,10 +131,39 @@ __device__ __forceinline__ void fast_cindex(const FastArgs& f, const int32_t* id
   }
 }
 
+// ImageMaskSpatialObject test of a fast-path sample: the mask's continuous
+// index from its affine map, rounded half up; certain when it lies more than
+// e from every half-integer (then the reference's rounding gives the same
+// index).  Sets *out when the mask certainly excludes the sample, *uncertain
+// when the rounding is not certain.
+template <int D>
+__device__ __forceinline__ void fast_mask(const FastMask& m, double x, double y, double z, bool* out,
+                                          bool* uncertain) {
+  if (!m.px) return;  // uniform
+  bool in = true, unc = false;
+  int32_t off = 0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    double c = m.b[k];
+    if (D == 3) c = fma(m.A[k * 3 + 2], z, c);
+    c = fma(m.A[k * D + 1], y, c);
+    c = fma(m.A[k * D + 0], x, c);
+    const double r = floor(c + 0.5);
+    const double fr = c - (r - 0.5);
+    unc = unc | !((fr > m.e) & (fr < 1.0 - m.e));
+    const int32_t ri = (int32_t)fmin(fmax(r, -1.0e9), 1.0e9);
+    in = in & (ri >= m.start[k]) & (ri <= m.end[k]);
+    off += (ri - m.start[k]) * (k == 0 ? 1 : (k == 1 ? m.sy : m.sz));
+  }
+  if (in) in = m.px[off] != 0;
+  *uncertain = *uncertain | unc;
+  *out = *out | (!unc & !in);
+}
+
 // TEX: gather the moving texels (value and gradient channels) instead of the
 // moving values, and return the interpolated gradient and the moving Parzen
 // term as well; the value is interpolated with the same operations either way.
-template <int D, typename P, int MK, bool TEX = false>
+template <int D, typename P, int MK, bool TEX = false, bool MASK = false>
 __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* idx, int& fbin, int& mbin, float& arg0,
                                              Cell& cell, bool* bsin, float3* grad = nullptr,
                                              float* mterm_out = nullptr) {
@@ -159,6 +188,11 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
     bb[k] = fl;
     cell.d[k] = ci[k] - (double)fl;
     act[k] = (cell.d[k] > 0.0) & (fl + 1 <= f.mend[k]);
+  }
+  if constexpr (MASK) {
+    const double x = (double)idx[0], y = (double)idx[1], z = (double)idx[2];
+    fast_mask<D>(f.fmask, x, y, z, &out, &uncertain);
+    fast_mask<D>(f.mmask, x, y, z, &out, &uncertain);
   }
   cell.base = (bb[0] - f.mstart[0]) + f.msy * (bb[1] - f.mstart[1]) + (D == 3 ? f.msz * (bb[2] - f.mstart[2]) : 0);
   cell.ox = act[0] ? 1 : 0;
@@ -286,7 +320,9 @@ __device__ __forceinline__ void pass1_exceptions(const EvalArgs& a, const uint32
   }
 }
 
-template <int D, typename P, int MK>
+// MASK: the affine path with image masks (kernels of their own, so that the
+// unmasked kernels keep their registers).
+template <int D, typename P, int MK, bool MASK = false>
 __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
   extern __shared__ unsigned long long lds_u64[];
   const int B = f.bins;
@@ -311,7 +347,7 @@ __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, const EvalArg
     float arg0 = 0.0f;
     Cell cell;
     bool bsin;
-    const int c = fast_classify<D, P, MK>(f, idx, fbin, mbin, arg0, cell, &bsin);
+    const int c = fast_classify<D, P, MK, false, MASK>(f, idx, fbin, mbin, arg0, cell, &bsin);
     if (f.cls) f.cls[s] = (uint8_t)(c == 1 ? (uint32_t)fbin : (c == 2 ? kClsListed : kClsSkip));
     if (c == 2) {
       // listed for the exact path (here and in pass 2)
@@ -522,7 +558,7 @@ __device__ __forceinline__ void exceptions_pass2_body(const EvalArgs& a, const F
 // bins, so pass 2 skips the others before any gather and drops its bound
 // tests and fixed-image reads (the classifier's unused results are not
 // computed); the moving side is evaluated as in pass 1, so the bits agree.
-template <int D, typename P, int MK, bool TEX = false, bool CLS = false>
+template <int D, typename P, int MK, bool TEX = false, bool CLS = false, bool MASK = false>
 __device__ __forceinline__ void pass2_fast_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
   extern __shared__ float lds_f32[];
   const int B = f.bins;
@@ -559,8 +595,8 @@ __device__ __forceinline__ void pass2_fast_body(const FastArgs& f, const EvalArg
     bool bsin;
     float3 gt = make_float3(0.0f, 0.0f, 0.0f);
     float mterm_unused;
-    int c = TEX ? fast_classify<D, P, MK, true>(f, idx, fbin, mbin, arg0, cell, &bsin, &gt, &mterm_unused)
-                : fast_classify<D, P, MK>(f, idx, fbin, mbin, arg0, cell, &bsin);
+    int c = TEX ? fast_classify<D, P, MK, true, MASK>(f, idx, fbin, mbin, arg0, cell, &bsin, &gt, &mterm_unused)
+                : fast_classify<D, P, MK, false, MASK>(f, idx, fbin, mbin, arg0, cell, &bsin);
     if constexpr (CLS) {
       c = 1;
       fbin = (int)cb;
@@ -693,6 +729,18 @@ __global__ void __launch_bounds__(256) pass2_fast_kernel(FastArgs f, EvalArgs a)
   pass2_fast_body<D, P, MK>(f, a, blockIdx.x, gridDim.x);
 }
 
+// Affine with image masks: pass 1, and the pass 2 that classifies again (the
+// one after pass 1's byte per sample needs no mask test).
+template <int D, typename P>
+__global__ void __launch_bounds__(256) pass1_mask_kernel(FastArgs f, EvalArgs a) {
+  pass1_fast_body<D, P, 1, true>(f, a, blockIdx.x, gridDim.x);
+}
+
+template <int D, typename P>
+__global__ void __launch_bounds__(256) pass2_mask_kernel(FastArgs f, EvalArgs a) {
+  pass2_fast_body<D, P, 1, false, false, true>(f, a, blockIdx.x, gridDim.x);
+}
+
 // Pass 2 after a pass 1 that wrote its byte per sample, with the moving values
 // interpolated from the texels it gathers for the gradient (one load per
 // corner; the byte per sample keeps the registers under 128 with it).
@@ -815,7 +863,12 @@ __global__ void batch_gather(const BatchOut* __restrict__ items, double* __restr
 
 hipError_t launch_fast_pass1(const FastArgs& f, const EvalArgs& a, int dim, int grid, hipStream_t st) {
   const size_t lds = pass1_lds_bytes(f.bins);
-  MMI_DISPATCH_FAST_MK(pass1_fast_kernel, f.mk, f.storage, dim, grid, lds, st, f, a);
+  if (f.fmask.px || f.mmask.px) {
+    if (f.mk != 1) return hipErrorInvalidValue;
+    MMI_DISPATCH_FAST(pass1_mask_kernel, f.storage, dim, grid, lds, st, f, a);
+  } else {
+    MMI_DISPATCH_FAST_MK(pass1_fast_kernel, f.mk, f.storage, dim, grid, lds, st, f, a);
+  }
   return hipGetLastError();
 }
 
@@ -830,6 +883,8 @@ hipError_t launch_fast_pass2(const FastArgs& f, const EvalArgs& a, int dim, int 
   const size_t lds = ((size_t)f.bins * f.bins + 128) * 4;
   if (f.cls)
     MMI_DISPATCH_FAST_MK(pass2_cls_kernel, f.mk, f.storage, dim, grid, lds, st, f, a);
+  else if (f.fmask.px || f.mmask.px)
+    MMI_DISPATCH_FAST(pass2_mask_kernel, f.storage, dim, grid, lds, st, f, a);
   else
     MMI_DISPATCH_FAST_MK(pass2_fast_kernel, f.mk, f.storage, dim, grid, lds, st, f, a);
   return hipGetLastError();

@@ -29,6 +29,18 @@ struct DevResult {
 // Parzen terms run in FP32 with the error bounds below, and any sample whose
 // inside test, range test or bin index could differ from the reference's
 // within those bounds is listed and redone on the exact FP64 path.
+// An image mask on the fast path: its continuous index is A idx + b of the
+// virtual index with |error| <= e against the reference's chain; px NULL: no
+// mask.
+struct FastMask {
+  const uint8_t* px;
+  double A[9];
+  double b[3];
+  double e;
+  int32_t start[3], end[3];
+  int32_t sy, sz;
+};
+
 struct FastArgs {
   uint32_t n;  // virtual samples
   int32_t bins;
@@ -92,6 +104,7 @@ struct FastArgs {
   // again): the fixed bin of a counted certain sample, kClsListed for a
   // sample on the exact path, kClsSkip for one not counted.
   uint8_t* cls;
+  FastMask fmask, mmask;  // fixed mask at the fixed point, moving mask at the mapped moving point
 };
 
 constexpr uint32_t kClsListed = 0xFEu;
