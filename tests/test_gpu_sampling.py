@@ -242,3 +242,28 @@ def test_masked_member_in_batch():
         assert (v, nv) == (v1, m.GetNumberOfValidPoints())
         assert np.array_equal(d, d1)
     be.close()
+
+
+def test_mask_on_half_integer_indices(oracle_mod):
+    """A mask on a grid of twice the spacing along x with the image's origin:
+    every odd x index maps exactly onto a half-integer mask index, where ITK
+    rounds up.  The fast path cannot certify those samples (they go to the
+    exact path; so many that the lists overflow and the evaluation is redone
+    exactly), and the result still equals the oracle's."""
+    fixed, moving, geom = blob_problem((40, 32, 28), 79)
+    grad = oracle_mod.gradient_image(moving, geom)
+    tf = affine_for(geom, affine_params(3, np.random.default_rng(80)))
+    mg = ImageGeometry(size=(21, 32, 28), origin=geom.origin,
+                       spacing=(2.0 * geom.spacing[0], geom.spacing[1], geom.spacing[2]))
+    rng = np.random.default_rng(81)
+    mpx = (rng.uniform(size=mg.shape) < 0.6).astype(np.uint8)
+    m = _metric(fixed, moving, geom, tf, grad=grad, fmask=(mpx, mg))
+    v, d = m.GetValueAndDerivative()
+    assert m.GetLastEvaluationPath() in ("fast", "fast_overflow_exact")
+    r = oracle_mod.mattes(fixed, geom, moving, geom, tf, moving_gradient=grad, want_samples=True, threads=8,
+                          fixed_mask=(mpx, mg))
+    assert m.GetNumberOfValidPoints() == r["n_valid"]
+    fb, mb = m.GetSampleBins()
+    assert np.array_equal(fb, r["fixed_bin"]) and np.array_equal(mb, r["moving_bin"])
+    assert abs(v - r["value"]) <= VALUE_RTOL * abs(r["value"])
+    assert np.linalg.norm(d - r["derivative"]) <= DERIV_RTOL * np.linalg.norm(r["derivative"])
