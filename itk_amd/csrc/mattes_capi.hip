@@ -720,7 +720,9 @@ bool setup_fast(mmi_context* ctx, FastArgs* F) {
   if (ctx->flags & 2u) return false;  // exact path requested
   const bool bs = ctx->mtf.kind == MMI_TF_BSPLINE;
   if ((ctx->mtf.kind != MMI_TF_AFFINE && !bs) || ctx->ftf.kind != MMI_TF_IDENTITY) return false;
-  if (ctx->gradient_source != MMI_GRADIENT_IMAGE) return false;
+  // central differences (affine): the fast pass 1 (the histogram needs no
+  // gradient), then the exact pass 2
+  if (ctx->gradient_source != MMI_GRADIENT_IMAGE && bs) return false;
   if ((size_t)B * B * 8 + (B + 1) * 4 > 64 * 1024) return false;
   const mmi_geometry& v = ctx->virtual_g;
   const mmi_geometry& m = ctx->moving_g;
@@ -1647,14 +1649,17 @@ retry:
     if (int rc = bspline_separable(ctx, a.virt, D, tabs, &separable)) return rc;
   if (want_derivative) {
     if (time_passes) MMI_HIP(hipEventRecord(ctx->ev[2], st));
-    if (fast && fa.mk == 2) {
+    // the fast pass 2 interpolates the gradient image; with central
+    // differences pass 2 runs on the exact path after the fast pass 1
+    const bool fast2 = fast && ctx->gradient_source == MMI_GRADIENT_IMAGE;
+    if (fast2 && fa.mk == 2) {
       // B-spline fast path: per-sample factors, then the separable contraction
       MMI_HIP(ctx->bs_g.ensure(sizeof(float) * D * n));
       fa.bspline_g = ctx->bs_g.as<float>();
       MMI_HIP(launch_fast_pass2(fa, a, D, ctx->grid, st));  // its tail writes the listed samples' factors
       if (int rc = bspline_contract(ctx, a.virt, tabs, f.result)) return rc;
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
-    } else if (fast) {
+    } else if (fast2) {
       MMI_HIP(launch_fast_pass2(fa, a, D, ctx->grid, st));
       MMI_HIP(launch_fast_derivative(fa.partials, 2 * ctx->grid, fast_map(ctx), D, f.result, dev_deriv(ctx), st));
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
@@ -1853,7 +1858,9 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
     const int64_t P = (int64_t)ctxs[i]->mtf.params.size();
     // (masked contexts run their own masked kernels)
     const bool masked = f.fmask.px || f.mmask.px;
-    if (fast && f.mk == 1 && !masked && P <= 12 && (storage < 0 || (a.virt.dim == dim && a.storage == storage))) {
+    const bool image_gradient = ctxs[i]->gradient_source == MMI_GRADIENT_IMAGE;
+    if (fast && f.mk == 1 && !masked && image_gradient && P <= 12 &&
+        (storage < 0 || (a.virt.dim == dim && a.storage == storage))) {
       dim = a.virt.dim;
       storage = a.storage;
       fused.push_back(i);

@@ -18,11 +18,13 @@ def _requests():
     shapes = [(40, 36, 32), (48, 44, 40), (32, 32, 28), (44, 40, 36)]
     for i, shape in enumerate(shapes):
         p = synth.make_affine_problem(shape, 100 + i, n_blobs=12)
-        reqs.append(RegistrationRequest(i, "affine", p.fixed, p.moving, p.geometry, arrival=0.01 * i,
+        # arrivals close together: each registration takes only milliseconds,
+        # and the test needs them to overlap
+        reqs.append(RegistrationRequest(i, "affine", p.fixed, p.moving, p.geometry, arrival=0.0005 * i,
                                         shrink=[2, 1], sigmas=[1.0, 0.0], iterations=25))
     for j, shape in enumerate([(36, 34, 30), (32, 30, 28)]):
         p = synth.make_bspline_problem(shape, 110 + j, mesh=4, sigma_vox=1.0, n_blobs=12)
-        reqs.append(RegistrationRequest(10 + j, "bspline", p.fixed, p.moving, p.geometry, arrival=0.005 + 0.01 * j,
+        reqs.append(RegistrationRequest(10 + j, "bspline", p.fixed, p.moving, p.geometry, arrival=0.00025 + 0.0005 * j,
                                         mesh=3, iterations=8))
     return reqs
 
