@@ -190,6 +190,41 @@ class MattesMutualInformationImageToImageMetricv4:
     def GetUseVirtualSampledPointSet(self) -> bool:
         return self._use_virtual_points
 
+    def UseSampledPointSetOn(self):
+        self.SetUseSampledPointSet(True)
+
+    def UseSampledPointSetOff(self):
+        self.SetUseSampledPointSet(False)
+
+    def UseVirtualSampledPointSetOn(self):
+        self.SetUseVirtualSampledPointSet(True)
+
+    def UseVirtualSampledPointSetOff(self):
+        self.SetUseVirtualSampledPointSet(False)
+
+    # -- settings of the reference class with no effect on the device -------
+    def SetMaximumNumberOfWorkUnits(self, n: int):
+        """Kept for interface compatibility: the evaluation's partition is the
+        device's (the results do not depend on it)."""
+        self._work_units = int(n)
+
+    def GetMaximumNumberOfWorkUnits(self) -> int:
+        return getattr(self, "_work_units", 0)
+
+    def SetUseFixedImageGradientFilter(self, on: bool):
+        """Mattes MI uses no fixed-image gradient; kept for compatibility."""
+        self._use_fixed_gradient_filter = bool(on)
+
+    def GetUseFixedImageGradientFilter(self) -> bool:
+        return getattr(self, "_use_fixed_gradient_filter", True)
+
+    def GetMetricCategory(self) -> str:
+        """ObjectToObjectMetricBaseTemplateEnums::MetricCategory (itkImageToImageMetricv4.h:662-666)."""
+        return "IMAGE_METRIC"
+
+    def GetFixedTransform(self):
+        return self._fixed_tf
+
     def GetVirtualSampledPointSet(self):
         return self._sample_points
 

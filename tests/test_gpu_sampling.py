@@ -267,3 +267,26 @@ def test_mask_on_half_integer_indices(oracle_mod):
     assert np.array_equal(fb, r["fixed_bin"]) and np.array_equal(mb, r["moving_bin"])
     assert abs(v - r["value"]) <= VALUE_RTOL * abs(r["value"])
     assert np.linalg.norm(d - r["derivative"]) <= DERIV_RTOL * np.linalg.norm(r["derivative"])
+
+
+def test_interface_switches():
+    """The reference's On/Off switches and the settings with no device effect."""
+    fixed, moving, geom = blob_problem((24, 20, 16), 90)
+    tf = affine_for(geom)
+    m = Metric()
+    m.SetFixedImage(fixed, geom)
+    m.SetMovingImage(moving, geom)
+    m.SetMovingTransform(tf)
+    m.SetMaximumNumberOfWorkUnits(3)
+    m.SetUseFixedImageGradientFilter(False)
+    assert m.GetMaximumNumberOfWorkUnits() == 3 and not m.GetUseFixedImageGradientFilter()
+    assert m.GetMetricCategory() == "IMAGE_METRIC" and m.GetFixedTransform() is None
+    m.SetVirtualSampledPointSet(_random_points(geom, 500, 91)[:-2])
+    m.UseSampledPointSetOn()
+    m.UseVirtualSampledPointSetOn()
+    m.Initialize()
+    assert m.GetNumberOfDomainPoints() == 500
+    m.UseSampledPointSetOff()
+    m.Initialize()
+    assert m.GetNumberOfDomainPoints() == geom.num_pixels
+    m.GetValueAndDerivative()
