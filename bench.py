@@ -270,6 +270,39 @@ def cpu_baseline(wl: Workload, seconds: float):
                       "evaluations (oracle, %d threads, slab partition)" % (n, what, el, threads)}
 
 
+def pass_breakdown(args, wl, n):
+    """Pass 1 and pass 2 timed apart (HIP events between them, a context with
+    pass timing; outside the timed region): pass 1 reads a fixed and a moving
+    value per sample, pass 2 one moving texel (value and gradient)."""
+    from itk_amd.metric import MattesMutualInformationImageToImageMetricv4
+
+    geom = wl.problem.geometry
+    m = MattesMutualInformationImageToImageMetricv4(device=0, time_passes=True)
+    m.SetImageStorage(args.storage)
+    m.SetFixedImage(wl.problem.fixed, geom)
+    m.SetMovingImage(wl.problem.moving, geom)
+    m.SetNumberOfHistogramBins(50)
+    m.SetMovingTransform(wl.tf)
+    m.Initialize()
+    p1 = p2 = 0.0
+    reps = 10
+    for i in range(reps + 2):
+        wl.tf.set_parameters(wl.params[i % len(wl.params)])
+        m.GetValueAndDerivative()
+        if i >= 2:
+            t = m.GetLastTiming()
+            p1 += t["pass1_ms"] / reps
+            p2 += t["pass2_ms"] / reps
+    m.close()
+    px = 4 if args.storage == "f32" else 2
+    b1, b2 = n * 2 * px, n * 4 * px
+    out = {"pass1_ms": p1, "pass1_algorithmic_bytes": b1, "pass1_frac": b1 / (p1 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "pass2_ms": p2, "pass2_algorithmic_bytes": b2, "pass2_frac": b2 / (p2 * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if wl.kind != "affine":
+        out["note"] = "pass 1 includes the coefficient lines and the exception kernel, pass 2 the contraction"
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -336,6 +369,7 @@ def main():
         "last_value": v,
     }
     m.close()
+    line["roofline"]["passes"] = pass_breakdown(args, wl, n)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     if served is not None:
