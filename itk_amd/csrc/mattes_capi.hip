@@ -1905,6 +1905,20 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
       lds1 = std::max(lds1, pass1_lds_bytes(ctxs[fused[k]]->bins));
     }
     start[nf] = total;
+    // pass 1's byte per sample when every member can use it (same bits either way)
+    bool cls = want_derivative != 0;
+    for (int k = 0; k < nf && cls; ++k) {
+      mmi_context* c = ctxs[fused[k]];
+      cls = c->bins <= 253 && c->texel_values_level && !(c->flags & MMI_FLAG_PASS2_RECLASSIFY);
+    }
+    for (int k = 0; k < nf; ++k) {
+      fa[k].cls = nullptr;
+      if (cls) {
+        mmi_context* c = ctxs[fused[k]];
+        MMI_HIP(c->cls.ensure((size_t)sample_count(c)));
+        fa[k].cls = c->cls.as<uint8_t>();
+      }
+    }
     const int32_t lds = finalize_uses_lds(bins_max) ? 1 : 0;
     for (int k = 0; k < nf; ++k) {
       mmi_context* c = ctxs[fused[k]];
@@ -1939,6 +1953,7 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
     L.out_stride = stride;
     L.bins_max = bins_max;
     L.lds1 = (int32_t)lds1;
+    L.cls = cls ? 1 : 0;
     L.start = reinterpret_cast<const int32_t*>(ds + o_start);
     L.fa = reinterpret_cast<const FastArgs*>(ds + o_fa);
     L.ea = reinterpret_cast<const EvalArgs*>(ds + o_ea);
@@ -1948,7 +1963,7 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
     L.out = b->d_out.as<double>();
     MMI_HIP(hipEventRecord(b->ev0, b->stream));
     if (b->use_graph) {
-      const std::vector<int64_t> sig = {nf, dim, storage, L.want_derivative, total, stride, bins_max, L.lds1,
+      const std::vector<int64_t> sig = {nf, dim, storage, L.want_derivative, total, stride, bins_max, L.lds1, L.cls,
                                         (int64_t)bytes,
                                         (int64_t)(intptr_t)b->d_stage.p, (int64_t)(intptr_t)b->d_out.p,
                                         (int64_t)(intptr_t)b->h_stage, (int64_t)(intptr_t)b->h_out};

@@ -719,6 +719,14 @@ __global__ void __launch_bounds__(256) pass1_fast_batch(const FastArgs* __restri
   pass1_fast_body<D, P, 1>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
 }
 
+template <int D, typename P>
+__global__ void __launch_bounds__(256) pass2_cls_batch(const FastArgs* __restrict__ fa,
+                                                       const EvalArgs* __restrict__ ea,
+                                                       const int32_t* __restrict__ start, int n) {
+  const int r = batch_item(start, n, blockIdx.x);
+  pass2_fast_body<D, P, 1, true, true>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+}
+
 template <int D, typename P, int MK>
 __global__ void __launch_bounds__(256) exceptions_pass1_kernel(EvalArgs a, FastArgs f) {
   exceptions_pass1_body<D, P, MK>(a, f, blockIdx.x);
@@ -908,7 +916,10 @@ hipError_t launch_fast_batch(const BatchLaunch& b, hipStream_t st) {
   if (e != hipSuccess) return e;
   if (b.want_derivative) {
     const size_t lds2 = ((size_t)B * B + 128) * 4;
-    MMI_DISPATCH_FAST(pass2_fast_batch, b.storage, b.dim, b.total_blocks, lds2, st, b.fa, b.ea, b.start, b.n);
+    if (b.cls)
+      MMI_DISPATCH_FAST(pass2_cls_batch, b.storage, b.dim, b.total_blocks, lds2, st, b.fa, b.ea, b.start, b.n);
+    else
+      MMI_DISPATCH_FAST(pass2_fast_batch, b.storage, b.dim, b.total_blocks, lds2, st, b.fa, b.ea, b.start, b.n);
     if (b.dim == 2)
       hipLaunchKernelGGL(fast_derivative_batch<2>, dim3(b.n), dim3(256), 0, st, b.dv);
     else

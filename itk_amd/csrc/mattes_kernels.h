@@ -222,6 +222,8 @@ struct BatchLaunch {
   int32_t out_stride;
   int32_t bins_max;
   int32_t lds1;  // pass-1 LDS bytes: the largest pass1_lds_bytes of the members
+  int32_t cls;   // 1: every member has pass 1's byte per sample (pass 2 reads it)
+  int32_t pad_cls_;
   const int32_t* start;  // [n + 1] first workgroup of each evaluation
   const FastArgs* fa;    // [n]
   const EvalArgs* ea;    // [n]
