@@ -222,7 +222,7 @@ class _Lane:
 
 
 class RegistrationServer:
-    def __init__(self, device: int = 0, hbm_budget_bytes: int = 64 << 30, max_inflight: int = 2,
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 64 << 30, max_inflight: int = 1,
                  max_wait_s: float = 0.0005, graphs: bool = True, storage: str = "f32", lanes: int = 2,
                  chunk_steps: int = 5, wait_limit_s: float = 600.0):
         self.device = device

@@ -49,7 +49,7 @@ def test_results_same_alone_and_in_the_mix():
 def test_admission_respects_hbm_budget():
     reqs = _requests()[:3]
     budget = max(r.device_bytes() for r in reqs) + 1
-    server = RegistrationServer(hbm_budget_bytes=budget)
+    server = RegistrationServer(hbm_budget_bytes=budget, max_inflight=8)  # the budget, not the cap, limits
     out = server.run(reqs)
     assert server.peak_inflight == 1
     alone, _ = run_sequential(reqs)
