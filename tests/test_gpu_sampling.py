@@ -290,3 +290,35 @@ def test_interface_switches():
     m.Initialize()
     assert m.GetNumberOfDomainPoints() == geom.num_pixels
     m.GetValueAndDerivative()
+
+
+@pytest.mark.parametrize("case", ["2d_f32", "3d_bf16"])
+def test_masks_fast_path_2d_and_16bit(oracle_mod, case):
+    """Both masks on the fast path in 2-D, and with BF16 storage (against the
+    oracle on the host-rounded images)."""
+    shape = (150, 130) if case == "2d_f32" else (40, 36, 30)
+    fixed, moving, geom = blob_problem(shape, 92)
+    d = geom.dim
+    tf = affine_for(geom, affine_params(d, np.random.default_rng(93), scale=2.0))
+    fmask, mmask = _coarse_mask(geom, 94), _coarse_mask(geom, 95, factor=3)
+    if case == "3d_bf16":
+        rnd = lambda a: (np.asarray(a, np.float32).view(np.uint32) + 0x7FFF +
+                         ((np.asarray(a, np.float32).view(np.uint32) >> 16) & 1)).astype(np.uint32) \
+            & np.uint32(0xFFFF0000)
+        r32 = lambda a: rnd(a).view(np.float32)
+        grad = oracle_mod.gradient_image(moving, geom)
+        grad = r32(grad.astype(np.float32)).astype(np.float64)
+        m = Metric(record_sample_bins=True)
+        m.SetImageStorage("bf16")
+        m.SetFixedImage(fixed, geom)
+        m.SetMovingImage(moving, geom)
+        m.SetMovingImageGradient(grad)
+        m.SetMovingTransform(tf)
+        m.SetFixedImageMask(ImageMaskSpatialObject(*fmask))
+        m.SetMovingImageMask(ImageMaskSpatialObject(*mmask))
+        m.Initialize()
+        _check(oracle_mod, m, r32(fixed), r32(moving), geom, tf, grad=grad, fmask=fmask, mmask=mmask)
+    else:
+        grad = oracle_mod.gradient_image(moving, geom)
+        m = _metric(fixed, moving, geom, tf, grad=grad, fmask=fmask, mmask=mmask)
+        _check(oracle_mod, m, fixed, moving, geom, tf, grad=grad, fmask=fmask, mmask=mmask)
