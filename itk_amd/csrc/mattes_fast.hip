@@ -88,14 +88,11 @@ __device__ __forceinline__ int fast_bin(float term, float bound, int B, bool* un
 template <int D, int MK>
 __device__ __forceinline__ void fast_cindex(const FastArgs& f, const int32_t* idx, double* ci, bool* bsin) {
   if constexpr (MK == 2) {
+    // diagonal virtual geometry: the off-diagonal terms of the reference's
+    // sum are exact zeros, so one product per axis gives the same point
     double vp[3], mp[3];
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-      double s = f.vorg[k];
-#pragma unroll
-      for (int j = 0; j < D; ++j) s += (j == k ? f.vsp[k] : 0.0) * (double)idx[j];
-      vp[k] = s;
-    }
+    for (int k = 0; k < D; ++k) vp[k] = f.vorg[k] + f.vsp[k] * (double)idx[k];
     const int xl = idx[0] - f.vstart[0], yl = idx[1] - f.vstart[1], zl = D == 3 ? idx[2] - f.vstart[2] : 0;
     const bool in = f.bs_inside[0][xl] && f.bs_inside[1][yl] && (D < 3 || f.bs_inside[2][zl]);
     *bsin = in;
