@@ -198,3 +198,20 @@ def test_mask_spatial_object_mask_value():
     assert m.IsInsideInWorldSpace([0.9, 1.49])      # rounds to (0, 1): pixel 2
     assert m.IsInsideInWorldSpace([-1.0, 1.0])      # x index -0.5 rounds up to 0: (0, 1), pixel 2
     assert not m.IsInsideInWorldSpace([-1.2, 1.0])  # x index -0.6 rounds to -1: outside
+
+
+def test_masks_thread_invariance_and_value_only(oracle_mod):
+    """With masks, the slab partition changes only the derivative's rounding,
+    and the value-only evaluation gives the same value and count."""
+    fixed, moving, geom = blob_problem((20, 18, 16), 51)
+    tf = affine_for(geom, affine_params(3, np.random.default_rng(52), scale=2.0))
+    rng = np.random.default_rng(53)
+    fm = ((rng.uniform(size=geom.shape) < 0.7).astype(np.uint8), geom)
+    mm = ((rng.uniform(size=geom.shape) < 0.8).astype(np.uint8), geom)
+    a = oracle_mod.mattes(fixed, geom, moving, geom, tf, fixed_mask=fm, moving_mask=mm, threads=1)
+    b = oracle_mod.mattes(fixed, geom, moving, geom, tf, fixed_mask=fm, moving_mask=mm, threads=5)
+    c = oracle_mod.mattes(fixed, geom, moving, geom, tf, fixed_mask=fm, moving_mask=mm, derivative=False)
+    assert a["n_valid"] == b["n_valid"] == c["n_valid"]
+    assert b["value"] == pytest.approx(a["value"], rel=1e-13)
+    assert c["value"] == a["value"]
+    assert np.allclose(a["derivative"], b["derivative"], rtol=1e-11, atol=1e-14)
