@@ -41,8 +41,8 @@ BYTES_PER_SAMPLE_3D = {"f32": 20, "bf16": 10, "f16": 10}
 # rocprofv3 --pmc runs summarised in profiles/mattes_affine_256cube_fast_path.md
 # and profiles/mattes_bspline_192cube_mesh16.md.  None where not profiled.
 TRAFFIC_BYTES_PER_EVAL = {
-    ("mattes_mi_dense_value_and_derivative_affine_256cube", "f32"): 462.8e6,
-    ("mattes_mi_dense_value_and_derivative_bspline_192cube_mesh16", "f32"): 491.6e6,
+    ("mattes_mi_dense_value_and_derivative_affine_256cube", "f32"): 461.4e6,
+    ("mattes_mi_dense_value_and_derivative_bspline_192cube_mesh16", "f32"): 491.1e6,
 }
 
 
