@@ -78,6 +78,21 @@ def test_reference_illegal_file_raises(tmp_path):
         T.ReadTransform(str(p))
 
 
+def test_float_typed_file_reads_as_double(tmp_path):
+    # a reader of double transforms takes a _float_ class name as its double
+    # counterpart and keeps every digit of the text
+    # (Modules/IO/TransformBase/include/itkTransformIOBase.h:203, CorrectTransformPrecisionType)
+    p = tmp_path / "f.tfm"
+    p.write_text("#Insight Transform File V1.0\n#Transform 0\nTransform: AffineTransform_float_2_2\n"
+                 "Parameters: 1.1000000000000001 0.2 -0.3 0.9 4.25 -1.0000000000000002\n"
+                 "FixedParameters: 0.5 -0.5\n")
+    (tf,) = T.ReadTransform(str(p))
+    assert tf.kind == T.TF_AFFINE and tf.dim == 2
+    assert np.array_equal(tf.get_parameters(), [1.1000000000000001, 0.2, -0.3, 0.9, 4.25, -1.0000000000000002])
+    assert np.array_equal(tf.center, [0.5, -0.5])
+    assert T.type_string(tf) == "AffineTransform_double_2_2"
+
+
 def test_round_trips(tmp_path):
     geom = ImageGeometry(size=(30, 26, 20), spacing=(0.9, 1.2, 1.7), origin=(-3.0, 4.5, 0.25))
     aff = affine_for(geom, affine_params(3, np.random.default_rng(2), scale=3.0))
