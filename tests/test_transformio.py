@@ -97,8 +97,9 @@ def test_round_trips(tmp_path):
     geom = ImageGeometry(size=(30, 26, 20), spacing=(0.9, 1.2, 1.7), origin=(-3.0, 4.5, 0.25))
     aff = affine_for(geom, affine_params(3, np.random.default_rng(2), scale=3.0))
     aff2 = affine_for(ImageGeometry(size=(40, 30)), affine_params(2, np.random.default_rng(3)))
+    bsp2 = bspline_for(ImageGeometry(size=(33, 21), spacing=(0.7, 1.3), origin=(2.0, -1.5)), (5, 4), seed=7)
     bsp = bspline_for(geom, (4, 3, 5), seed=4, sigma=1.3)
-    for tf in (aff, aff2, bsp):
+    for tf in (aff, aff2, bsp2, bsp):
         p = str(tmp_path / "t.tfm")
         T.WriteTransform(p, tf)
         (back,) = T.ReadTransform(p)
