@@ -201,6 +201,14 @@ int mmi_set_moving_parameters(mmi_context* ctx, const double* params, int64_t n_
 int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, double* derivative, int64_t* n_valid,
                  double* joint_pdf, double* fixed_marginal);
 
+/* Launch configuration of the evaluation kernels (no counterpart in the
+ * reference; tuning and comparison runs): workgroups of 256 per compute unit
+ * (the grid holds at most one workgroup per 256 samples) and the rows per walk
+ * of the affine row-walk kernels (0: the sample-strided kernels).  A negative
+ * value restores the library's choice.  Takes effect at the next
+ * mmi_initialize. */
+int mmi_set_launch_config(mmi_context* ctx, int32_t blocks_per_cu, int32_t rows_walk);
+
 /* Per-sample Parzen bins of the last evaluation (int16, -1 = sample not
  * counted), in virtual buffer order; for parity tests.  Either may be NULL. */
 int mmi_get_sample_bins(mmi_context* ctx, int16_t* fixed_bin, int16_t* moving_bin);

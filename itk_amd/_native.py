@@ -53,6 +53,7 @@ SIGNATURES = {
     "mmi_set_fixed_mask": (ctypes.c_int, [_vp, _gp, _vp]),
     "mmi_set_moving_mask": (ctypes.c_int, [_vp, _gp, _vp]),
     "mmi_set_number_of_histogram_bins": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mmi_set_launch_config": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
     "mmi_initialize": (ctypes.c_int, [_vp]),
     "mmi_set_fixed_transform": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, ctypes.c_int32, _dp, ctypes.c_int64]),
     "mmi_set_moving_transform": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, ctypes.c_int32, _dp, ctypes.c_int64]),

@@ -301,6 +301,8 @@ struct mmi_context {
   DevGeom bs_key_virt{}, bs_key_grid{};
   int xcd_chunks = 1;                        // fast path: one contiguous sample share per XCD
   int blocks_per_cu = 4;                     // evaluation grid = CUs x blocks_per_cu workgroups of 256
+  int rows_walk = 16;                        // affine fast path: rows per walk (0: sample-strided kernels)
+  int default_blocks_per_cu = 4, default_rows_walk = 16;  // the library's choice (environment overrides)
   int grid = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // pinned host staging: B-spline coefficients up, result and derivative down
@@ -706,6 +708,59 @@ void affine_index_map(const mmi_geometry& v, const double* M, const double* off,
   *e = 64.0 * S * std::ldexp(1.0, -52) + 1e-300;
 }
 
+// Row walks of the affine fast path (FastRows): items of 64 x by `walk` rows,
+// the moving continuous index stepped in 32.32 fixed point.  Widens F->e_ci by
+// the walk's error: the FP64 start is within e_ci of the reference's chain and
+// within 3 roundings of S (< e_ci / 8, e_ci being 64 ulp of S) of the exact
+// map, its conversion truncates below 2^-32, and each step's rounded increment
+// is within 2^-33 of the map's y column.
+void setup_rows(const mmi_context* ctx, FastArgs* F) {
+  FastRows& R = F->rows;
+  std::memset(&R, 0, sizeof(R));
+  if (ctx->rows_walk <= 0) return;
+  const int D = ctx->fixed_g.dim;
+  const mmi_geometry& v = ctx->virtual_g;
+  const mmi_geometry& m = ctx->moving_g;
+  // cell offsets through 24-bit products; index steps far below 2^31
+  const int64_t msy = m.size[0], msz = D == 3 ? m.size[0] * m.size[1] : 0;
+  if (msy >= (1 << 24) || msz >= (1 << 24)) return;
+  for (int k = 0; k < D; ++k)
+    if (m.size[k] >= (1 << 24) || std::fabs(m.start[k]) >= (1 << 24)) return;
+  for (int i = 0; i < D * D; ++i)
+    if (!(std::fabs(F->A[i]) < 1048576.0)) return;
+  const int64_t nx = v.size[0], ny = v.size[1], nz = D == 3 ? v.size[2] : 1;
+  const int64_t nxseg = (nx + 63) / 64;
+  // walks long enough to amortise the FP64 start, short enough that every
+  // wavefront of the grid gets an item
+  const int64_t waves = (int64_t)ctx->grid * 4;
+  const int64_t walk = std::max<int64_t>(1, std::min<int64_t>({(int64_t)ctx->rows_walk, ny, nxseg * ny * nz / waves}));
+  const int64_t nyblk = (ny + walk - 1) / walk;
+  const int64_t items = nxseg * nyblk * nz;
+  if (items >= (1ll << 31)) return;
+  R.walk = (int32_t)walk;
+  R.nx = (int32_t)nx;
+  R.ny = (int32_t)ny;
+  R.items = (uint32_t)items;
+  R.dxseg = make_fastdiv((uint32_t)nxseg);
+  R.dyblk = make_fastdiv((uint32_t)nyblk);
+  const double e = 1.125 * F->e_ci + (double)(walk + 2) * std::ldexp(1.0, -32);
+  const long double two32 = 4294967296.0L;
+  for (int k = 0; k < D; ++k) {
+    R.qy[k] = (long long)std::llround((long double)F->A[k * D + 1] * two32);
+    const long double lo = (long double)m.start[k] - 0.5L, hi = (long double)(m.start[k] + m.size[k] - 1) + 0.5L;
+    const long long lo_out = (long long)std::floor((lo - e) * two32) - 1;
+    const long long hi_out = (long long)std::ceil((hi + e) * two32) + 1;
+    const long long lo_in = (long long)std::ceil((lo + e) * two32) + 1;
+    const long long hi_in = (long long)std::floor((hi - e) * two32) - 1;
+    R.lo_out[k] = lo_out;
+    R.span_out[k] = (unsigned long long)(hi_out - lo_out);
+    R.lo_in[k] = lo_in;
+    R.span_in[k] = hi_in > lo_in ? (unsigned long long)(hi_in - lo_in) : 0ull;
+  }
+  F->e_ci = e;
+  R.on = 1;
+}
+
 bool setup_fast(mmi_context* ctx, FastArgs* F) {
   std::memset(F, 0, sizeof(*F));
   const int D = ctx->fixed_g.dim;
@@ -757,6 +812,13 @@ bool setup_fast(mmi_context* ctx, FastArgs* F) {
     fm[k]->px = ctx->mask_px[k].as<uint8_t>();
   }
   if (F->e_ci > 1e-6) return false;
+  // fixed samples on pixel centres (direct reads), else per-axis cell tables
+  // and an FP32 interpolation with the same error bound as the moving value's
+  int64_t off[3] = {0, 0, 0};
+  const bool direct = fixed_direct_offsets(v, ctx->fixed_g, off);
+  if (!direct && !fixed_axis_tables(ctx, v, ctx->fixed_g, F)) return false;
+  // row walks: their fixed-point steps widen e_ci (before the bounds below)
+  if (!bs && direct && !mg[0] && !mg[1]) setup_rows(ctx, F);
   // intensity bounds
   const double Mm = std::max(std::fabs(ctx->moving_min), std::fabs(ctx->moving_max));
   const double Mf = std::max(std::fabs(ctx->fixed_min), std::fabs(ctx->fixed_max));
@@ -764,11 +826,6 @@ bool setup_fast(mmi_context* ctx, FastArgs* F) {
   const double u = std::ldexp(1.0, -22);
   const double bt_m = 2.0 * ((ev + Mm * u) / ctx->moving_bin_size + u * (Mm / ctx->moving_bin_size +
                                                                           std::fabs(ctx->moving_norm_min) + B));
-  // fixed samples on pixel centres (direct reads), else per-axis cell tables
-  // and an FP32 interpolation with the same error bound as the moving value's
-  int64_t off[3] = {0, 0, 0};
-  const bool direct = fixed_direct_offsets(v, ctx->fixed_g, off);
-  if (!direct && !fixed_axis_tables(ctx, v, ctx->fixed_g, F)) return false;
   const double ef = direct ? 0.0 : Mf * std::ldexp(1.0, -19);
   const double bt_f = 2.0 * ((ef + Mf * u) / ctx->fixed_bin_size + u * (Mf / ctx->fixed_bin_size +
                                                                        std::fabs(ctx->fixed_norm_min) + B));
@@ -1017,6 +1074,9 @@ int mmi_context_create(int device, uint32_t flags, mmi_context** out) {
   ctx->flags = flags;
   if (const char* e = std::getenv("MMI_XCD_CHUNKS")) ctx->xcd_chunks = std::atoi(e) != 0;
   if (const char* e = std::getenv("MMI_BLOCKS_PER_CU")) ctx->blocks_per_cu = std::max(1, std::min(64, std::atoi(e)));
+  if (const char* e = std::getenv("MMI_ROWS_WALK")) ctx->rows_walk = std::max(0, std::min(256, std::atoi(e)));
+  ctx->default_blocks_per_cu = ctx->blocks_per_cu;
+  ctx->default_rows_walk = ctx->rows_walk;
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete ctx;
@@ -1275,6 +1335,15 @@ int mmi_set_smoothing_sigmas(mmi_context* ctx, const double* fixed_sigma, const 
     if (positive != 0 && positive != D)
       return fail(MMI_ERR_ARGUMENT, "smoothing sigmas must be all zero or all positive");
   }
+  ctx->initialized = false;
+  return MMI_OK;
+}
+
+int mmi_set_launch_config(mmi_context* ctx, int32_t blocks_per_cu, int32_t rows_walk) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (blocks_per_cu > 64 || rows_walk > 256) return fail(MMI_ERR_ARGUMENT, "launch configuration out of range");
+  ctx->blocks_per_cu = blocks_per_cu > 0 ? blocks_per_cu : ctx->default_blocks_per_cu;
+  ctx->rows_walk = rows_walk >= 0 ? rows_walk : ctx->default_rows_walk;
   ctx->initialized = false;
   return MMI_OK;
 }
@@ -1631,6 +1700,10 @@ retry:
     MMI_HIP(ctx->cls.ensure((size_t)n));
     fa.cls = ctx->cls.as<uint8_t>();
   }
+  // a row-walk pass 1 needs a pass 2 that follows its byte per sample (or the
+  // exact pass 2 of central differences, which classifies on its own)
+  if (fast && fa.rows.on && want_derivative && !fa.cls && ctx->gradient_source == MMI_GRADIENT_IMAGE)
+    fa.rows.on = 0;
   if (fast) {
     MMI_HIP(launch_fast_pass1(fa, a, D, ctx->grid, st));
     if (fa.mk == 2) MMI_HIP(launch_fast_exceptions1(a, fa, ctx->grid, st));
@@ -1859,7 +1932,12 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
     // (masked contexts run their own masked kernels)
     const bool masked = f.fmask.px || f.mmask.px;
     const bool image_gradient = ctxs[i]->gradient_source == MMI_GRADIENT_IMAGE;
-    if (fast && f.mk == 1 && !masked && image_gradient && P <= 12 &&
+    // with the derivative, fused members take pass 1's byte per sample (as
+    // they do alone); the others are evaluated on their own
+    mmi_context* c = ctxs[i];
+    const bool cls_ok = !want_derivative || (c->bins <= 253 && c->texel_values_level &&
+                                             !(c->flags & MMI_FLAG_PASS2_RECLASSIFY));
+    if (fast && f.mk == 1 && !masked && image_gradient && P <= 12 && cls_ok &&
         (storage < 0 || (a.virt.dim == dim && a.storage == storage))) {
       dim = a.virt.dim;
       storage = a.storage;
@@ -1905,12 +1983,8 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
       lds1 = std::max(lds1, pass1_lds_bytes(ctxs[fused[k]]->bins));
     }
     start[nf] = total;
-    // pass 1's byte per sample when every member can use it (same bits either way)
-    bool cls = want_derivative != 0;
-    for (int k = 0; k < nf && cls; ++k) {
-      mmi_context* c = ctxs[fused[k]];
-      cls = c->bins <= 253 && c->texel_values_level && !(c->flags & MMI_FLAG_PASS2_RECLASSIFY);
-    }
+    // pass 1's byte per sample: every fused member can use it (see above)
+    const bool cls = want_derivative != 0;
     for (int k = 0; k < nf; ++k) {
       fa[k].cls = nullptr;
       if (cls) {

@@ -73,7 +73,7 @@ __device__ __forceinline__ void fast_index(const FastArgs& f, uint32_t s, int32_
 // an unclamped bin edge (|term - n| <= bound for an edge n in [3, B-3]).
 __device__ __forceinline__ int fast_bin(float term, float bound, int B, bool* uncertain) {
   const float n = rintf(term);
-  *uncertain = *uncertain || ((fabsf(term - n) <= bound) & (n >= 3.0f) & (n <= (float)(B - 3)));
+  *uncertain = *uncertain | ((fabsf(term - n) <= bound) & (n >= 3.0f) & (n <= (float)(B - 3)));
   return (int)fminf(fmaxf(term, 2.0f), (float)(B - 3));
 }
 
@@ -157,47 +157,15 @@ __device__ __forceinline__ void fast_mask(const FastMask& m, double x, double y,
   *out = *out | (!unc & !in);
 }
 
-// TEX: gather the moving texels (value and gradient channels) instead of the
-// moving values, and return the interpolated gradient and the moving Parzen
-// term as well; the value is interpolated with the same operations either way.
-template <int D, typename P, int MK, bool TEX = false, bool MASK = false>
-__device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* idx, int& fbin, int& mbin, float& arg0,
-                                             Cell& cell, bool* bsin, float3* grad = nullptr,
-                                             float* mterm_out = nullptr) {
-  double ci[3];
-  fast_cindex<D, MK>(f, idx, ci, bsin);
-  bool out = false, uncertain = false;
-  int32_t bb[3] = {0, 0, 0};
-  bool act[3] = {false, false, false};
-  cell.d[2] = 0.0;
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    const double lo = (double)f.cilo[k], hi = (double)f.cihi[k];
-    out = out | (ci[k] < lo - f.e_ci) | (ci[k] >= hi + f.e_ci);
-    uncertain = uncertain | (ci[k] < lo + f.e_ci) | (ci[k] >= hi - f.e_ci);
-    // moving cell (the reference's branch rules as duplicated corners)
-    // floor, not ITK's Math::Floor: they differ only within an ulp below an
-    // integer, where either cell holds ci and the interpolant is continuous
-    // (its slope bound is part of ev)
-    const double cc = fmin(fmax(ci[k], lo), hi);
-    int32_t fl = (int32_t)floor(cc);
-    fl = fl < f.mstart[k] ? f.mstart[k] : (fl > f.mend[k] ? f.mend[k] : fl);
-    bb[k] = fl;
-    cell.d[k] = ci[k] - (double)fl;
-    act[k] = (cell.d[k] > 0.0) & (fl + 1 <= f.mend[k]);
-  }
-  if constexpr (MASK) {
-    const double x = (double)idx[0], y = (double)idx[1], z = (double)idx[2];
-    fast_mask<D>(f.fmask, x, y, z, &out, &uncertain);
-    fast_mask<D>(f.mmask, x, y, z, &out, &uncertain);
-  }
-  cell.base = (bb[0] - f.mstart[0]) + f.msy * (bb[1] - f.mstart[1]) + (D == 3 ? f.msz * (bb[2] - f.mstart[2]) : 0);
-  cell.ox = act[0] ? 1 : 0;
-  cell.oy = act[1] ? f.msy : 0;
-  cell.oz = (D == 3 && act[2]) ? f.msz : 0;
-  const float t0 = (float)cell.d[0], t1 = (float)cell.d[1], t2 = (float)cell.d[2];
-  // every cell offset is non-negative and below the buffer size
-  const uint32_t cb = (uint32_t)cell.base, cx = (uint32_t)cell.ox, cy = (uint32_t)cell.oy, cz = (uint32_t)cell.oz;
+// FP32 multilinear value of the moving image at a cell (corner element offsets
+// cb + {0, cx} + {0, cy} + {0, cz}, weights t0..t2; every offset non-negative
+// and inside the buffer).  TEX: gather the moving texels (value and gradient
+// channels) instead of the moving values and return the interpolated gradient
+// as well; the value is interpolated with the same operations either way, and
+// every fast-path kernel interpolates through this function.
+template <int D, typename P, bool TEX>
+__device__ __forceinline__ float fast_interp(const FastArgs& f, uint32_t cb, uint32_t cx, uint32_t cy, uint32_t cz,
+                                             float t0, float t1, float t2, float3* grad) {
   float v;
   if constexpr (TEX) {
     // value as below, gradient as interp_gradient3, from one texel load per corner
@@ -244,12 +212,68 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
       v = fmaf(vxx1 - v, t2, v);
     }
   }
+  return v;
+}
+
+// The intensity part of a fast-path classification, after the coordinate
+// part's out / uncertain flags: the moving value's range test, the moving and
+// fixed Parzen bins with their certainty.  0: not counted, 1: counted, 2:
+// uncertain.
+__device__ __forceinline__ int fast_bins(const FastArgs& f, float v, float fv, bool out, bool uncertain, int& fbin,
+                                         int& mbin, float& arg0, float* mterm_out) {
   out = out | (v < f.mmin - f.ev) | (v > f.mmax + f.ev);
   uncertain = uncertain | (v < f.mmin + f.ev) | (v > f.mmax - f.ev);
   const float mterm = fmaf(v, f.inv_mbs, -f.mnm);
   mbin = fast_bin(mterm, f.bt_m, f.bins, &uncertain);
   arg0 = (float)(mbin - 1) - mterm;
-  if constexpr (TEX) *mterm_out = mterm;
+  if (mterm_out) *mterm_out = mterm;
+  const float fterm = fmaf(fv, f.inv_fbs, -f.fnm);
+  fbin = fast_bin(fterm, f.bt_f, f.bins, &uncertain);
+  return out ? 0 : (uncertain ? 2 : 1);
+}
+
+// TEX: gather the moving texels (value and gradient channels) instead of the
+// moving values, and return the interpolated gradient and the moving Parzen
+// term as well; the value is interpolated with the same operations either way.
+template <int D, typename P, int MK, bool TEX = false, bool MASK = false>
+__device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* idx, int& fbin, int& mbin, float& arg0,
+                                             Cell& cell, bool* bsin, float3* grad = nullptr,
+                                             float* mterm_out = nullptr) {
+  double ci[3];
+  fast_cindex<D, MK>(f, idx, ci, bsin);
+  bool out = false, uncertain = false;
+  int32_t bb[3] = {0, 0, 0};
+  bool act[3] = {false, false, false};
+  cell.d[2] = 0.0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const double lo = (double)f.cilo[k], hi = (double)f.cihi[k];
+    out = out | (ci[k] < lo - f.e_ci) | (ci[k] >= hi + f.e_ci);
+    uncertain = uncertain | (ci[k] < lo + f.e_ci) | (ci[k] >= hi - f.e_ci);
+    // moving cell (the reference's branch rules as duplicated corners)
+    // floor, not ITK's Math::Floor: they differ only within an ulp below an
+    // integer, where either cell holds ci and the interpolant is continuous
+    // (its slope bound is part of ev)
+    const double cc = fmin(fmax(ci[k], lo), hi);
+    int32_t fl = (int32_t)floor(cc);
+    fl = fl < f.mstart[k] ? f.mstart[k] : (fl > f.mend[k] ? f.mend[k] : fl);
+    bb[k] = fl;
+    cell.d[k] = ci[k] - (double)fl;
+    act[k] = (cell.d[k] > 0.0) & (fl + 1 <= f.mend[k]);
+  }
+  if constexpr (MASK) {
+    const double x = (double)idx[0], y = (double)idx[1], z = (double)idx[2];
+    fast_mask<D>(f.fmask, x, y, z, &out, &uncertain);
+    fast_mask<D>(f.mmask, x, y, z, &out, &uncertain);
+  }
+  cell.base = (bb[0] - f.mstart[0]) + f.msy * (bb[1] - f.mstart[1]) + (D == 3 ? f.msz * (bb[2] - f.mstart[2]) : 0);
+  cell.ox = act[0] ? 1 : 0;
+  cell.oy = act[1] ? f.msy : 0;
+  cell.oz = (D == 3 && act[2]) ? f.msz : 0;
+  const float t0 = (float)cell.d[0], t1 = (float)cell.d[1], t2 = (float)cell.d[2];
+  // every cell offset is non-negative and below the buffer size
+  const uint32_t cb = (uint32_t)cell.base, cx = (uint32_t)cell.ox, cy = (uint32_t)cell.oy, cz = (uint32_t)cell.oz;
+  const float v = fast_interp<D, P, TEX>(f, cb, cx, cy, cz, t0, t1, t2, grad);
   const typename P::pixel* fimg = static_cast<const typename P::pixel*>(f.fixed_px);
   float fv;
   if (f.fixed_interp) {
@@ -282,9 +306,7 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
         (idx[0] + f.foff[0]) + f.fsy * (idx[1] + f.foff[1]) + (D == 3 ? f.fsz * (idx[2] + f.foff[2]) : 0);
     fv = P::ld32(fimg, (uint32_t)fl);
   }
-  const float fterm = fmaf(fv, f.inv_fbs, -f.fnm);
-  fbin = fast_bin(fterm, f.bt_f, f.bins, &uncertain);
-  return out ? 0 : (uncertain ? 2 : 1);
+  return fast_bins(f, v, fv, out, uncertain, fbin, mbin, arg0, TEX ? mterm_out : nullptr);
 }
 
 // The uncertain samples a pass-1 workgroup listed, on the exact FP64 path,
@@ -317,23 +339,103 @@ __device__ __forceinline__ void pass1_exceptions(const EvalArgs& a, const uint32
   }
 }
 
+// A pass-1 workgroup's accumulators in LDS: `copies` joint histograms and
+// fixed marginals (even and odd lanes take their own copy), the valid count and
+// the exception count.
+struct P1Lds {
+  unsigned long long* hist0;  // [copies][B][B]
+  unsigned int* fmarg0;       // [copies][B]
+  unsigned int* counters;     // [0] valid count, [1] exception count
+  unsigned long long* hist;   // this lane's copy
+  unsigned int* fmarg;
+  int copies;
+};
+
+__device__ __forceinline__ P1Lds pass1_begin(int B) {
+  extern __shared__ unsigned long long lds_u64[];
+  const int BB = B * B;
+  P1Lds L;
+  L.copies = hist_copies(B);
+  L.hist0 = lds_u64;
+  L.fmarg0 = reinterpret_cast<unsigned int*>(lds_u64 + L.copies * BB);
+  L.counters = L.fmarg0 + L.copies * B;
+  const int mine = L.copies == 2 ? (int)(threadIdx.x & 1u) : 0;
+  L.hist = L.hist0 + mine * BB;
+  L.fmarg = L.fmarg0 + mine * B;
+  for (int i = threadIdx.x; i < L.copies * BB; i += blockDim.x) L.hist0[i] = 0ull;
+  for (int i = threadIdx.x; i < L.copies * B + 2; i += blockDim.x) L.fmarg0[i] = 0u;
+  __syncthreads();
+  return L;
+}
+
+// One classified sample s of pass 1 (c: 0 not counted, 1 counted, 2
+// uncertain): its byte for pass 2, its list slot when uncertain, its recorded
+// bins, and the Parzen weights of a counted sample in the LDS histogram.
+__device__ __forceinline__ void pass1_add(const FastArgs& f, const P1Lds& L, uint32_t lb, uint32_t s, int c, int fbin,
+                                          int mbin, float arg0, unsigned int& nvalid) {
+  if (f.cls) f.cls[s] = (uint8_t)(c == 1 ? (uint32_t)fbin : (c == 2 ? kClsListed : kClsSkip));
+  if (c == 2) {
+    // listed for the exact path (here and in pass 2)
+    const unsigned int slot = atomicAdd(&L.counters[1], 1u);
+    if (slot < (unsigned int)kExcCap) f.exc_list[lb * kExcCap + slot] = s;
+    return;
+  }
+  if (f.sample_fbin) {
+    f.sample_fbin[s] = c == 1 ? (int16_t)fbin : (int16_t)-1;
+    f.sample_mbin[s] = c == 1 ? (int16_t)mbin : (int16_t)-1;
+  }
+  if (c == 0) return;
+  ++nvalid;
+  atomicAdd(&L.fmarg[fbin], 1u);
+  unsigned int q[4];
+  parzen_fixed_fast(arg0, q);
+  // mbin is clamped to [2, B-3], so the 4 bins stay in the row; a zero
+  // weight (clamped windows only) adds nothing
+  unsigned long long* row = L.hist + __umul24((uint32_t)fbin, (uint32_t)f.bins) + (mbin - 1);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) atomicAdd(&row[k], (unsigned long long)q[k]);
+}
+
+// End of pass 1: the valid count, the workgroup's listed samples on the exact
+// path (affine), the LDS accumulators into the evaluation's.
+template <int D, typename P, int MK>
+__device__ __forceinline__ void pass1_end(const FastArgs& f, const EvalArgs& a, const P1Lds& L, uint32_t lb,
+                                          unsigned int nvalid) {
+  const int B = f.bins;
+  const int BB = B * B;
+  unsigned int v = nvalid;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(&L.counters[0], v);
+  __syncthreads();
+  const unsigned int ne = L.counters[1];
+  if constexpr (MK == 1) {
+    // the list this workgroup wrote (visible to it after the barrier)
+    if (ne)
+      pass1_exceptions<D, P, MK>(a, f.exc_list + lb * kExcCap, min(ne, (unsigned int)kExcCap), L.hist0, L.fmarg0,
+                                 &L.counters[0]);
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < BB; i += blockDim.x) {
+    const unsigned long long h = L.hist0[i] + (L.copies == 2 ? L.hist0[BB + i] : 0ull);
+    if (h) atomicAdd(&f.hist[i], h);
+  }
+  for (int i = threadIdx.x; i < B; i += blockDim.x) {
+    const unsigned int m = L.fmarg0[i] + (L.copies == 2 ? L.fmarg0[B + i] : 0u);
+    if (m) atomicAdd(&f.fmarg[i], (unsigned long long)m);
+  }
+  if (threadIdx.x == 0) {
+    if (L.counters[0]) atomicAdd(f.nvalid, (unsigned long long)L.counters[0]);
+    f.exc_count[lb] = ne;
+    if (ne > (unsigned int)kExcCap) atomicOr(f.exc_overflow, 1u);
+  }
+}
+
 // MASK: the affine path with image masks (kernels of their own, so that the
 // unmasked kernels keep their registers).
 template <int D, typename P, int MK, bool MASK = false>
 __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
-  extern __shared__ unsigned long long lds_u64[];
-  const int B = f.bins;
-  const int BB = B * B;
-  const int copies = hist_copies(B);
-  unsigned long long* hist0 = lds_u64;                                            // [copies][B][B]
-  unsigned int* fmarg0 = reinterpret_cast<unsigned int*>(lds_u64 + copies * BB);  // [copies][B]
-  unsigned int* counters = fmarg0 + copies * B;  // [0] valid count, [1] exception count
-  const int mine = copies == 2 ? (int)(threadIdx.x & 1u) : 0;
-  unsigned long long* hist = hist0 + mine * BB;
-  unsigned int* fmarg = fmarg0 + mine * B;
-  for (int i = threadIdx.x; i < copies * BB; i += blockDim.x) hist0[i] = 0ull;
-  for (int i = threadIdx.x; i < copies * B + 2; i += blockDim.x) fmarg0[i] = 0u;
-  __syncthreads();
+  const P1Lds L = pass1_begin(f.bins);
   unsigned int nvalid = 0;
   uint32_t s, end, step;
   run_range<D>(f, lb, lg, s, end, step);
@@ -345,54 +447,146 @@ __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, const EvalArg
     Cell cell;
     bool bsin;
     const int c = fast_classify<D, P, MK, false, MASK>(f, idx, fbin, mbin, arg0, cell, &bsin);
-    if (f.cls) f.cls[s] = (uint8_t)(c == 1 ? (uint32_t)fbin : (c == 2 ? kClsListed : kClsSkip));
-    if (c == 2) {
-      // listed for the exact path (here and in pass 2)
-      const unsigned int slot = atomicAdd(&counters[1], 1u);
-      if (slot < (unsigned int)kExcCap) f.exc_list[lb * kExcCap + slot] = s;
-      continue;
-    }
-    if (f.sample_fbin) {
-      f.sample_fbin[s] = c == 1 ? (int16_t)fbin : (int16_t)-1;
-      f.sample_mbin[s] = c == 1 ? (int16_t)mbin : (int16_t)-1;
-    }
-    if (c == 0) continue;
-    ++nvalid;
-    atomicAdd(&fmarg[fbin], 1u);
-    unsigned int q[4];
-    parzen_fixed_fast(arg0, q);
-    // mbin is clamped to [2, B-3], so the 4 bins stay in the row; a zero
-    // weight (clamped windows only) adds nothing
-    unsigned long long* row = hist + fbin * B + (mbin - 1);
+    pass1_add(f, L, lb, s, c, fbin, mbin, arg0, nvalid);
+  }
+  pass1_end<D, P, MK>(f, a, L, lb, nvalid);
+}
+
+// ---------------------------------------------------------------------------
+// Row walks (affine moving transform, dense sampling, no masks, fixed samples
+// on fixed pixel centres).  A wavefront takes one item at a time: 64
+// consecutive x of `walk` consecutive rows.  Each lane maps its first sample
+// with the FP64 affine map of fast_cindex, converts the continuous index to
+// 32.32 fixed point and steps it down the column by the rounded y column of
+// the map; the host's bound e_ci covers the map, the conversion and the steps.
+// Per sample, the cell, the interpolation weights and the buffer tests are a
+// few integer operations on the fixed-point index.
+// ---------------------------------------------------------------------------
+
+// The wavefront's items: XCD-contiguous shares (as run_range), the
+// workgroup's wavefronts on consecutive items.  Wavefront-uniform.
+__device__ __forceinline__ void rows_range(const FastArgs& f, uint32_t lb, uint32_t lg, uint32_t& it, uint32_t& end,
+                                           uint32_t& step) {
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t nw = blockDim.x >> 6;
+  const uint32_t items = f.rows.items;
+  if (f.xcd_chunks) {
+    const uint32_t xcd = lb % kXcd, lbx = lb / kXcd, bpx = lg / kXcd;
+    const uint32_t chunk = (items + kXcd - 1) / kXcd;
+    const uint32_t c0 = xcd * chunk;
+    it = c0 + lbx * nw + wave;
+    end = min(items, c0 + chunk);
+    step = bpx * nw;
+  } else {
+    it = lb * nw + wave;
+    end = items;
+    step = lg * nw;
+  }
+}
+
+template <int D>
+struct RowWalk {
+  long long q[3];   // moving continuous index, 32.32 fixed point (floor in the high word)
+  uint32_t s;       // sample index (virtual buffer order)
+  uint32_t fp;      // fixed pixel element offset
+  int32_t x, y, z;  // virtual index of the lane's sample (y advances)
+  int32_t rows;     // samples of the walk (wavefront-uniform)
+  bool active;      // x inside the virtual row
+};
+
+// c in 32.32 fixed point, truncated (error below 2^-32); |c| clamped to 2^30
+// (such samples are outside every buffer the fast path takes).
+__device__ __forceinline__ long long to_fixed32(double c) {
+  c = fmin(fmax(c, -1073741824.0), 1073741824.0);
+  const double fl = floor(c);
+  const double fr = fmin((c - fl) * 4294967296.0, 4294967295.0);
+  return (long long)(((unsigned long long)(uint32_t)(int32_t)fl << 32) | (unsigned long long)(uint32_t)fr);
+}
+
+template <int D>
+__device__ __forceinline__ void rows_start(const FastArgs& f, uint32_t item, RowWalk<D>& w) {
+  const FastRows& R = f.rows;
+  const uint32_t r = fdiv(item, R.dxseg);
+  const uint32_t xs = item - r * R.dxseg.d;
+  const uint32_t zl = D == 3 ? fdiv(r, R.dyblk) : 0u;
+  const uint32_t yb = r - zl * R.dyblk.d;
+  const int32_t xl = (int32_t)(xs * 64u + (threadIdx.x & 63u));
+  const int32_t y0 = (int32_t)(yb * (uint32_t)R.walk);
+  w.active = xl < R.nx;
+  w.rows = min(R.walk, R.ny - y0);
+  w.x = f.vstart[0] + xl;
+  w.y = f.vstart[1] + y0;
+  w.z = D == 3 ? f.vstart[2] + (int32_t)zl : 0;
+  const int32_t idx[3] = {w.x, w.y, w.z};
+  double ci[3];
+  bool bsin;
+  fast_cindex<D, 1>(f, idx, ci, &bsin);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) atomicAdd(&row[k], (unsigned long long)q[k]);
-  }
-  unsigned int v = nvalid;
+  for (int k = 0; k < D; ++k) w.q[k] = to_fixed32(ci[k]);
+  w.s = (uint32_t)xl + (uint32_t)R.nx * ((uint32_t)y0 + (uint32_t)R.ny * zl);
+  w.fp = (uint32_t)((w.x + f.foff[0]) + f.fsy * (w.y + f.foff[1]) + (D == 3 ? f.fsz * (w.z + f.foff[2]) : 0));
+}
+
+template <int D>
+__device__ __forceinline__ void rows_step(const FastArgs& f, RowWalk<D>& w) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(&counters[0], v);
-  __syncthreads();
-  const unsigned int ne = counters[1];
-  if constexpr (MK == 1) {
-    // the list this workgroup wrote (visible to it after the barrier)
-    if (ne)
-      pass1_exceptions<D, P, MK>(a, f.exc_list + lb * kExcCap, min(ne, (unsigned int)kExcCap), hist0, fmarg0,
-                                 &counters[0]);
-    __syncthreads();
+  for (int k = 0; k < D; ++k) w.q[k] += f.rows.qy[k];
+  w.s += (uint32_t)f.rows.nx;
+  w.fp += (uint32_t)f.fsy;
+  ++w.y;
+}
+
+// fast_classify of a row-walk sample: buffer tests against the fixed-point
+// thresholds, the cell from the high word (clamped into the buffer, the upper
+// corner duplicated outside [start, end - 1] as in the reference's branch
+// rules), the weights from the low word; interpolation and bins as
+// fast_classify.
+template <int D, typename P, bool TEX = false>
+__device__ __forceinline__ int rows_classify(const FastArgs& f, const RowWalk<D>& w, int& fbin, int& mbin,
+                                             float& arg0, float3* grad = nullptr) {
+  const FastRows& R = f.rows;
+  bool out = false, uncertain = false;
+  int32_t fl[3] = {0, 0, 0};
+  uint32_t off[3] = {0u, 0u, 0u};
+  float t[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const long long q = w.q[k];
+    out = out | ((unsigned long long)(q - R.lo_out[k]) >= R.span_out[k]);
+    uncertain = uncertain | ((unsigned long long)(q - R.lo_in[k]) >= R.span_in[k]);
+    const int32_t I = (int32_t)(q >> 32);
+    fl[k] = min(max(I, f.mstart[k]), f.mend[k]);
+    const bool act = (uint32_t)(I - f.mstart[k]) < (uint32_t)(f.mend[k] - f.mstart[k]);
+    const uint32_t stride = k == 0 ? 1u : (k == 1 ? (uint32_t)f.msy : (uint32_t)f.msz);
+    off[k] = act ? stride : 0u;
+    t[k] = (float)(uint32_t)q * 2.3283064365386963e-10f;  // 2^-32
   }
-  for (int i = threadIdx.x; i < BB; i += blockDim.x) {
-    const unsigned long long h = hist0[i] + (copies == 2 ? hist0[BB + i] : 0ull);
-    if (h) atomicAdd(&f.hist[i], h);
+  // 24-bit products: the host checks the moving extents and strides
+  uint32_t cb = (uint32_t)(fl[0] - f.mstart[0]) + __umul24((uint32_t)f.msy, (uint32_t)(fl[1] - f.mstart[1]));
+  if (D == 3) cb += __umul24((uint32_t)f.msz, (uint32_t)(fl[2] - f.mstart[2]));
+  const float v = fast_interp<D, P, TEX>(f, cb, off[0], off[1], off[2], t[0], t[1], t[2], grad);
+  const float fv = P::ld32(static_cast<const typename P::pixel*>(f.fixed_px), w.fp);
+  return fast_bins(f, v, fv, out, uncertain, fbin, mbin, arg0, nullptr);
+}
+
+template <int D, typename P>
+__device__ __forceinline__ void pass1_rows_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
+  const P1Lds L = pass1_begin(f.bins);
+  unsigned int nvalid = 0;
+  uint32_t it, iend, istep;
+  rows_range(f, lb, lg, it, iend, istep);
+  for (; it < iend; it += istep) {
+    RowWalk<D> w;
+    rows_start<D>(f, it, w);
+    for (int j = 0; j < w.rows; ++j, rows_step<D>(f, w)) {
+      if (!w.active) continue;
+      int fbin = 0, mbin = 0;
+      float arg0 = 0.0f;
+      const int c = rows_classify<D, P>(f, w, fbin, mbin, arg0);
+      pass1_add(f, L, lb, w.s, c, fbin, mbin, arg0, nvalid);
+    }
   }
-  for (int i = threadIdx.x; i < B; i += blockDim.x) {
-    const unsigned int m = fmarg0[i] + (copies == 2 ? fmarg0[B + i] : 0u);
-    if (m) atomicAdd(&f.fmarg[i], (unsigned long long)m);
-  }
-  if (threadIdx.x == 0) {
-    if (counters[0]) atomicAdd(f.nvalid, (unsigned long long)counters[0]);
-    f.exc_count[lb] = ne;
-    if (ne > (unsigned int)kExcCap) atomicOr(f.exc_overflow, 1u);
-  }
+  pass1_end<D, P, 1>(f, a, L, lb, nvalid);
 }
 
 // Exceptions of kExcListsPerGroup consecutive pass-1 lists per workgroup:
@@ -653,6 +847,63 @@ __device__ __forceinline__ void pass2_fast_body(const FastArgs& f, const EvalArg
   }
 }
 
+// Pass 2 over row walks (after a row-walk pass 1 that wrote its byte per
+// sample, same items per workgroup): per counted sample the moving value and
+// gradient from the texels, the Parzen derivative weight w, and the
+// index-space moments.  Along a walk x and z are fixed and y advances, so a
+// lane sums S_b = sum w g_b and Y_b = sum y w g_b in FP32 and adds
+// M[b][x] += x S_b, M[b][y] += Y_b, M[b][z] += z S_b, M1[b] += S_b in FP64 at
+// the end of the walk.
+template <int D, typename P>
+__device__ __forceinline__ void pass2_rows_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
+  extern __shared__ float lds_f32[];
+  const int B = f.bins;
+  const int BB = B * B;
+  double* scratch = reinterpret_cast<double*>(lds_f32);  // 4 wavefronts x 12 doubles (128 floats)
+  float* pr = lds_f32 + 128;
+  for (int i = threadIdx.x; i < BB; i += blockDim.x) pr[i] = f.pratio_f[i];
+  __syncthreads();
+  constexpr int PA = D * D + D;
+  double acc[PA];
+#pragma unroll
+  for (int i = 0; i < PA; ++i) acc[i] = 0.0;
+  uint32_t it, iend, istep;
+  rows_range(f, lb, lg, it, iend, istep);
+  for (; it < iend; it += istep) {
+    RowWalk<D> w;
+    rows_start<D>(f, it, w);
+    float sg[3] = {0.0f, 0.0f, 0.0f}, sy[3] = {0.0f, 0.0f, 0.0f};
+    float yf = (float)w.y;
+    for (int j = 0; j < w.rows; ++j, rows_step<D>(f, w), yf += 1.0f) {
+      if (!w.active) continue;
+      const uint32_t cb = f.cls[w.s];
+      if (cb >= kClsListed) continue;
+      int fbin, mbin;
+      float arg0;
+      float3 g;
+      rows_classify<D, P, true>(f, w, fbin, mbin, arg0, &g);
+      const float wt = parzen_deriv_weight_fast(arg0, pr + (int)cb * B + (mbin - 1));
+      const float gw[3] = {wt * g.x, wt * g.y, wt * g.z};
+#pragma unroll
+      for (int b = 0; b < D; ++b) {
+        sg[b] += gw[b];
+        sy[b] = fmaf(yf, gw[b], sy[b]);
+      }
+    }
+    const double x = (double)w.x, z = (double)w.z;
+#pragma unroll
+    for (int b = 0; b < D; ++b) {
+      acc[b * D + 0] = fma((double)sg[b], x, acc[b * D + 0]);
+      acc[b * D + 1] += (double)sy[b];
+      if (D == 3) acc[b * D + 2] = fma((double)sg[b], z, acc[b * D + 2]);
+      acc[D * D + b] += (double)sg[b];
+    }
+  }
+  block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)lb * PA);
+  // this workgroup's listed samples, exactly, into their own partial row
+  exceptions_pass2_body<D, P, 1>(a, f, lb, lg);
+}
+
 // Fixed-order reduction of 2 * nblocks rows of index-space moments, then the
 // map to the affine derivative:
 //   d[b*D + d]   = nFactor * (sum_j V[d][j] M[b][j] + c0[d] M1[b])
@@ -708,12 +959,18 @@ __global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f, EvalArgs a)
   pass1_fast_body<D, P, MK>(f, a, blockIdx.x, gridDim.x);
 }
 
+// Batched kernels: each member runs the body it runs alone (row walks or
+// the sample-strided loop; a workgroup-uniform branch), so its results are the
+// bits of its own evaluation.
 template <int D, typename P>
 __global__ void __launch_bounds__(256) pass1_fast_batch(const FastArgs* __restrict__ fa,
                                                         const EvalArgs* __restrict__ ea,
                                                         const int32_t* __restrict__ start, int n) {
   const int r = batch_item(start, n, blockIdx.x);
-  pass1_fast_body<D, P, 1>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+  if (fa[r].rows.on)
+    pass1_rows_body<D, P>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+  else
+    pass1_fast_body<D, P, 1>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
 }
 
 template <int D, typename P>
@@ -721,7 +978,20 @@ __global__ void __launch_bounds__(256) pass2_cls_batch(const FastArgs* __restric
                                                        const EvalArgs* __restrict__ ea,
                                                        const int32_t* __restrict__ start, int n) {
   const int r = batch_item(start, n, blockIdx.x);
-  pass2_fast_body<D, P, 1, true, true>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+  if (fa[r].rows.on)
+    pass2_rows_body<D, P>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+  else
+    pass2_fast_body<D, P, 1, true, true>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+}
+
+template <int D, typename P>
+__global__ void __launch_bounds__(256) pass1_rows_kernel(FastArgs f, EvalArgs a) {
+  pass1_rows_body<D, P>(f, a, blockIdx.x, gridDim.x);
+}
+
+template <int D, typename P>
+__global__ void __launch_bounds__(256) pass2_rows_kernel(FastArgs f, EvalArgs a) {
+  pass2_rows_body<D, P>(f, a, blockIdx.x, gridDim.x);
 }
 
 template <int D, typename P, int MK>
@@ -871,6 +1141,8 @@ hipError_t launch_fast_pass1(const FastArgs& f, const EvalArgs& a, int dim, int 
   if (f.fmask.px || f.mmask.px) {
     if (f.mk != 1) return hipErrorInvalidValue;
     MMI_DISPATCH_FAST(pass1_mask_kernel, f.storage, dim, grid, lds, st, f, a);
+  } else if (f.rows.on) {
+    MMI_DISPATCH_FAST(pass1_rows_kernel, f.storage, dim, grid, lds, st, f, a);
   } else {
     MMI_DISPATCH_FAST_MK(pass1_fast_kernel, f.mk, f.storage, dim, grid, lds, st, f, a);
   }
@@ -886,7 +1158,11 @@ hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int gri
 
 hipError_t launch_fast_pass2(const FastArgs& f, const EvalArgs& a, int dim, int grid, hipStream_t st) {
   const size_t lds = ((size_t)f.bins * f.bins + 128) * 4;
-  if (f.cls)
+  // a row-walk pass 1 is followed by a pass 2 that reads its byte per sample
+  if (f.rows.on && !f.cls) return hipErrorInvalidValue;
+  if (f.rows.on)
+    MMI_DISPATCH_FAST(pass2_rows_kernel, f.storage, dim, grid, lds, st, f, a);
+  else if (f.cls)
     MMI_DISPATCH_FAST_MK(pass2_cls_kernel, f.mk, f.storage, dim, grid, lds, st, f, a);
   else if (f.fmask.px || f.mmask.px)
     MMI_DISPATCH_FAST(pass2_mask_kernel, f.storage, dim, grid, lds, st, f, a);

@@ -41,6 +41,27 @@ struct FastMask {
   int32_t sy, sz;
 };
 
+// Row walks of the affine fast path (dense sampling, no masks, fixed samples on
+// fixed pixel centres): a wavefront takes 64 consecutive x of `walk`
+// consecutive rows (one item); each lane maps its first sample with the FP64
+// affine map and steps the moving continuous index down its column in 32.32
+// fixed point (|error| <= e_ci including the steps).  Items are numbered x
+// segment fastest, then walk, then z.
+struct FastRows {
+  int32_t on;
+  int32_t walk;   // rows per walk
+  int32_t nx, ny; // virtual size along x and y
+  uint32_t items; // nxseg * nyblk * nz
+  int32_t pad_;
+  FastDiv dxseg;  // division by the x segments per row
+  FastDiv dyblk;  // division by the walks per column
+  long long qy[3];                 // A[k][1] * 2^32, rounded
+  long long lo_out[3];             // below: certainly outside the buffer
+  unsigned long long span_out[3];  // lo_out + span_out and above: certainly outside
+  long long lo_in[3];              // [lo_in, lo_in + span_in): certainly inside
+  unsigned long long span_in[3];
+};
+
 struct FastArgs {
   uint32_t n;  // virtual samples
   int32_t bins;
@@ -105,6 +126,7 @@ struct FastArgs {
   // sample on the exact path, kClsSkip for one not counted.
   uint8_t* cls;
   FastMask fmask, mmask;  // fixed mask at the fixed point, moving mask at the mapped moving point
+  FastRows rows;          // rows.on: the row-walk kernels (pass 2 then needs cls)
 };
 
 constexpr uint32_t kClsListed = 0xFEu;
@@ -233,19 +255,11 @@ struct BatchLaunch {
   double* out;  // [n][out_stride]: value, n_valid, status, derivative
 };
 
-// B-spline exceptions of pass 1 (a kernel of their own): a workgroup sums its
-// samples' Parzen weights per joint-histogram bin in an LDS hash table before
-// one global atomic per bin, so hot bins do not serialise the L2.  The table
-// holds at least twice the 4 bins of every listed sample.
-constexpr int kExcSlots = 2048;
-__host__ __device__ inline size_t exceptions1_lds_bytes(int bins) {
-  return (((size_t)bins + 1) * 4 + 7) / 8 * 8 + (size_t)kExcSlots * (8 + 4);
-}
-
 // Uncertain samples a fast-path workgroup can list (more: the evaluation is
-// redone on the exact path).
+// redone on the exact path).  The B-spline exception kernel sums the fixed
+// marginal and the valid count of its listed samples in (B + 1) * 4 bytes of
+// LDS and adds their histogram weights with global 64-bit atomics.
 constexpr int kExcCap = 256;
-static_assert(2 * 4 * kExcCap <= kExcSlots, "exception hash table too small");
 
 // Copies of the pass-1 joint histogram (and fixed marginal) in LDS: two when
 // they fit in 64 KiB, updated by even and odd lanes.  Neighbouring samples (adjacent lanes) usually fall in the same

@@ -49,6 +49,7 @@ class MattesMutualInformationImageToImageMetricv4:
             N.MMI_FLAG_PASS2_RECLASSIFY if pass2_reclassify else 0)
         N.check(self._L.mmi_context_create(device, flags, ctypes.byref(self._ctx)))
         self._bins = 50
+        self._launch = (-1, -1)  # library's launch configuration
         self._out = np.zeros(2)  # [value (double), valid count (int64 bits)]
         self._out_addr = self._out.ctypes.data
         self._fixed = self._moving = None
@@ -145,6 +146,13 @@ class MattesMutualInformationImageToImageMetricv4:
 
     def GetNumberOfHistogramBins(self) -> int:
         return self._bins
+
+    def SetLaunchConfiguration(self, blocks_per_cu: int = -1, rows_walk: int = -1):
+        """Kernel launch configuration (no counterpart in ITK): workgroups per
+        compute unit and rows per walk of the affine row-walk kernels (0: the
+        sample-strided kernels); -1 keeps the library's choice."""
+        self._launch = (int(blocks_per_cu), int(rows_walk))
+        self._initialized = False
 
     def SetVirtualDomain(self, geometry: ImageGeometry):
         self._virtual = geometry
@@ -315,6 +323,7 @@ class MattesMutualInformationImageToImageMetricv4:
             N.check(L.mmi_set_moving_gradient(ctx, _dptr(self._gradient)))
             self._gradient_dirty = False
         N.check(L.mmi_set_number_of_histogram_bins(ctx, self._bins))
+        N.check(L.mmi_set_launch_config(ctx, *self._launch))
         d = self._fixed[1].dim
         fs = None if self._fixed_sigma is None else np.ascontiguousarray(np.broadcast_to(self._fixed_sigma, (d,)))
         ms = None if self._moving_sigma is None else np.ascontiguousarray(np.broadcast_to(self._moving_sigma, (d,)))
