@@ -279,7 +279,7 @@ struct mmi_context {
   DevTransform dftf{}, dmtf{};
   DevBuf coeffs;
 
-  DevBuf hist, fmarg, nvalid, pdf, fm, mm, pratio, pratio_f, rowsum, partials, sample_fbin, sample_mbin, scratch;
+  DevBuf hist, nvalid, pdf, fm, mm, pratio, pratio_f, rowsum, partials, sample_fbin, sample_mbin, scratch;
   // the evaluation's outputs in one block (one copy to the host): the result,
   // the fast path's exception-overflow flag, the derivative (dev_result,
   // dev_overflow, dev_deriv)
@@ -303,6 +303,8 @@ struct mmi_context {
   int blocks_per_cu = 4;                     // evaluation grid = CUs x blocks_per_cu workgroups of 256
   int rows_walk = 16;                        // affine fast path: rows per walk (0: sample-strided kernels)
   int default_blocks_per_cu = 4, default_rows_walk = 16;  // the library's choice (environment overrides)
+  int hist_copies = 0;                       // pass 1's LDS histogram copies (0: by size)
+  int max_lds_per_block = 64 * 1024;         // the device's LDS bytes per workgroup
   int grid = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // pinned host staging: B-spline coefficients up, result and derivative down
@@ -1075,6 +1077,7 @@ int mmi_context_create(int device, uint32_t flags, mmi_context** out) {
   if (const char* e = std::getenv("MMI_XCD_CHUNKS")) ctx->xcd_chunks = std::atoi(e) != 0;
   if (const char* e = std::getenv("MMI_BLOCKS_PER_CU")) ctx->blocks_per_cu = std::max(1, std::min(64, std::atoi(e)));
   if (const char* e = std::getenv("MMI_ROWS_WALK")) ctx->rows_walk = std::max(0, std::min(256, std::atoi(e)));
+  if (const char* e = std::getenv("MMI_HIST_COPIES")) ctx->hist_copies = std::max(0, std::min(4, std::atoi(e)));
   ctx->default_blocks_per_cu = ctx->blocks_per_cu;
   ctx->default_rows_walk = ctx->rows_walk;
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -1085,6 +1088,9 @@ int mmi_context_create(int device, uint32_t flags, mmi_context** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     ctx->cu_count = prop.multiProcessorCount;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+    ctx->max_lds_per_block = lds;
   for (auto& ev : ctx->ev) {
     if (hipEventCreate(&ev) != hipSuccess) {
       mmi_context_destroy(ctx);
@@ -1101,7 +1107,7 @@ int mmi_context_destroy(mmi_context* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   DevBuf* bufs[] = {&ctx->fixed_px, &ctx->moving_px, &ctx->texels, &ctx->fixed_lp, &ctx->moving_lp, &ctx->texels_lp,
                     &ctx->fixed_work, &ctx->moving_work,
-                    &ctx->coeffs, &ctx->hist, &ctx->fmarg,
+                    &ctx->coeffs, &ctx->hist,
                     &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->out, &ctx->points, &ctx->point_ids, &ctx->mask_px[0], &ctx->mask_px[1],
                     &ctx->mask_desc[0], &ctx->mask_desc[1],
                     &ctx->partials, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
@@ -1459,7 +1465,6 @@ int mmi_initialize(mmi_context* ctx) {
   if (!ctx->have_virtual) ctx->virtual_g = ctx->fixed_g;
   const int B = ctx->bins;
   MMI_HIP(ctx->hist.ensure(sizeof(unsigned long long) * B * B));
-  MMI_HIP(ctx->fmarg.ensure(sizeof(unsigned long long) * B));
   MMI_HIP(ctx->nvalid.ensure(sizeof(unsigned long long)));
   MMI_HIP(ctx->pdf.ensure(sizeof(double) * B * B));
   MMI_HIP(ctx->pratio.ensure(sizeof(double) * B * B));
@@ -1577,7 +1582,6 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
   a.fixed_bin_size = ctx->fixed_bin_size;
   a.fixed_norm_min = ctx->fixed_norm_min;
   a.hist = ctx->hist.as<unsigned long long>();
-  a.fmarg = ctx->fmarg.as<unsigned long long>();
   a.nvalid = ctx->nvalid.as<unsigned long long>();
   if (ctx->flags & 1u) {
     a.sample_fbin = ctx->sample_fbin.as<int16_t>();
@@ -1600,12 +1604,15 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
     }
     fa.xcd_chunks = ctx->xcd_chunks;
     fa.exc_cap = kExcCap;
+    fa.hist_copies = (ctx->hist_copies == 1 || ctx->hist_copies == 2 || ctx->hist_copies == 4) &&
+                             pass1_lds_bytes_with(B, ctx->hist_copies) <= (size_t)ctx->max_lds_per_block
+                         ? ctx->hist_copies
+                         : hist_copies(B);
     fa.fixed_px = a.fixed_px;
     fa.moving_px = a.moving_px;
     fa.texels = a.texels;
     fa.storage = a.storage;
     fa.hist = a.hist;
-    fa.fmarg = a.fmarg;
     fa.nvalid = a.nvalid;
     fa.sample_fbin = a.sample_fbin;
     fa.sample_mbin = a.sample_mbin;
@@ -1628,7 +1635,6 @@ FinalizeArgs finalize_args(mmi_context* ctx) {
   std::memset(&f, 0, sizeof(f));
   f.bins = ctx->bins;
   f.hist = ctx->hist.as<unsigned long long>();
-  f.fmarg = ctx->fmarg.as<unsigned long long>();
   f.nvalid = ctx->nvalid.as<unsigned long long>();
   f.pdf = ctx->pdf.as<double>();
   f.fm = ctx->fm.as<double>();
@@ -1677,8 +1683,8 @@ retry:
   // zero between evaluations: finalize leaves them so; a clear kernel runs
   // only after initialisation or an interrupted evaluation
   if (!ctx->acc_clean)
-    MMI_HIP(launch_clear_eval(ctx->hist.as<unsigned long long>(), B * B, ctx->fmarg.as<unsigned long long>(), B,
-                              ctx->nvalid.as<unsigned long long>(), dev_overflow(ctx), st));
+    MMI_HIP(launch_clear_eval(ctx->hist.as<unsigned long long>(), B * B, ctx->nvalid.as<unsigned long long>(),
+                              dev_overflow(ctx), st));
   ctx->acc_clean = false;
   if (fast && fa.mk == 2) {
     AxisTable rt[3];
@@ -1980,7 +1986,7 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
       start[k] = total;
       total += ctxs[fused[k]]->grid;  // multiples of 8
       bins_max = std::max(bins_max, ctxs[fused[k]]->bins);
-      lds1 = std::max(lds1, pass1_lds_bytes(ctxs[fused[k]]->bins));
+      lds1 = std::max(lds1, pass1_lds_bytes_with(ctxs[fused[k]]->bins, fa[k].hist_copies));
     }
     start[nf] = total;
     // pass 1's byte per sample: every fused member can use it (see above)

@@ -310,13 +310,12 @@ __device__ __forceinline__ int fast_classify(const FastArgs& f, const int32_t* i
 }
 
 // The uncertain samples a pass-1 workgroup listed, on the exact FP64 path,
-// into the workgroup's LDS histogram (copy 0), fixed marginal and valid count,
+// into the workgroup's LDS histogram (copy 0) and valid count,
 // at the end of pass 1 (affine: the exact path's registers do not raise the
 // kernel's count; the samples' data is still in L2).
 template <int D, typename P, int MK>
 __device__ __forceinline__ void pass1_exceptions(const EvalArgs& a, const uint32_t* list, uint32_t ne,
-                                                             unsigned long long* hist, unsigned int* fmarg,
-                                                             unsigned int* nvalid) {
+                                                             unsigned long long* hist, unsigned int* nvalid) {
   const int B = a.bins;
   for (uint32_t t = threadIdx.x; t < ne; t += blockDim.x) {
     const uint32_t s = list[t];
@@ -330,7 +329,6 @@ __device__ __forceinline__ void pass1_exceptions(const EvalArgs& a, const uint32
     }
     if (!ok) continue;
     atomicAdd(nvalid, 1u);
-    atomicAdd(&fmarg[o.fbin], 1u);
     unsigned int q[4];
     parzen_fixed((float)((double)(o.mbin - 1) - o.mterm), q);
     unsigned long long* row = hist + o.fbin * B + (o.mbin - 1);
@@ -339,31 +337,28 @@ __device__ __forceinline__ void pass1_exceptions(const EvalArgs& a, const uint32
   }
 }
 
-// A pass-1 workgroup's accumulators in LDS: `copies` joint histograms and
-// fixed marginals (even and odd lanes take their own copy), the valid count and
-// the exception count.
+// A pass-1 workgroup's accumulators in LDS: `copies` joint histograms (1, 2 or
+// 4; lane i takes copy i mod copies), the valid count and the exception count.
+// The fixed marginal is not accumulated: each counted sample's Parzen weights
+// sum to 2^32, and finalize takes the counts from the histogram's rows.
 struct P1Lds {
   unsigned long long* hist0;  // [copies][B][B]
-  unsigned int* fmarg0;       // [copies][B]
   unsigned int* counters;     // [0] valid count, [1] exception count
   unsigned long long* hist;   // this lane's copy
-  unsigned int* fmarg;
   int copies;
 };
 
-__device__ __forceinline__ P1Lds pass1_begin(int B) {
+__device__ __forceinline__ P1Lds pass1_begin(int B, int copies) {
   extern __shared__ unsigned long long lds_u64[];
   const int BB = B * B;
   P1Lds L;
-  L.copies = hist_copies(B);
+  L.copies = copies;
   L.hist0 = lds_u64;
-  L.fmarg0 = reinterpret_cast<unsigned int*>(lds_u64 + L.copies * BB);
-  L.counters = L.fmarg0 + L.copies * B;
-  const int mine = L.copies == 2 ? (int)(threadIdx.x & 1u) : 0;
+  L.counters = reinterpret_cast<unsigned int*>(lds_u64 + L.copies * BB);
+  const int mine = (int)(threadIdx.x & (uint32_t)(L.copies - 1));
   L.hist = L.hist0 + mine * BB;
-  L.fmarg = L.fmarg0 + mine * B;
   for (int i = threadIdx.x; i < L.copies * BB; i += blockDim.x) L.hist0[i] = 0ull;
-  for (int i = threadIdx.x; i < L.copies * B + 2; i += blockDim.x) L.fmarg0[i] = 0u;
+  if (threadIdx.x < 2) L.counters[threadIdx.x] = 0u;
   __syncthreads();
   return L;
 }
@@ -386,7 +381,6 @@ __device__ __forceinline__ void pass1_add(const FastArgs& f, const P1Lds& L, uin
   }
   if (c == 0) return;
   ++nvalid;
-  atomicAdd(&L.fmarg[fbin], 1u);
   unsigned int q[4];
   parzen_fixed_fast(arg0, q);
   // mbin is clamped to [2, B-3], so the 4 bins stay in the row; a zero
@@ -412,17 +406,14 @@ __device__ __forceinline__ void pass1_end(const FastArgs& f, const EvalArgs& a, 
   if constexpr (MK == 1) {
     // the list this workgroup wrote (visible to it after the barrier)
     if (ne)
-      pass1_exceptions<D, P, MK>(a, f.exc_list + lb * kExcCap, min(ne, (unsigned int)kExcCap), L.hist0, L.fmarg0,
+      pass1_exceptions<D, P, MK>(a, f.exc_list + lb * kExcCap, min(ne, (unsigned int)kExcCap), L.hist0,
                                  &L.counters[0]);
     __syncthreads();
   }
   for (int i = threadIdx.x; i < BB; i += blockDim.x) {
-    const unsigned long long h = L.hist0[i] + (L.copies == 2 ? L.hist0[BB + i] : 0ull);
+    unsigned long long h = L.hist0[i];
+    for (int c = 1; c < L.copies; ++c) h += L.hist0[c * BB + i];
     if (h) atomicAdd(&f.hist[i], h);
-  }
-  for (int i = threadIdx.x; i < B; i += blockDim.x) {
-    const unsigned int m = L.fmarg0[i] + (L.copies == 2 ? L.fmarg0[B + i] : 0u);
-    if (m) atomicAdd(&f.fmarg[i], (unsigned long long)m);
   }
   if (threadIdx.x == 0) {
     if (L.counters[0]) atomicAdd(f.nvalid, (unsigned long long)L.counters[0]);
@@ -435,7 +426,7 @@ __device__ __forceinline__ void pass1_end(const FastArgs& f, const EvalArgs& a, 
 // unmasked kernels keep their registers).
 template <int D, typename P, int MK, bool MASK = false>
 __device__ __forceinline__ void pass1_fast_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
-  const P1Lds L = pass1_begin(f.bins);
+  const P1Lds L = pass1_begin(f.bins, f.hist_copies);
   unsigned int nvalid = 0;
   uint32_t s, end, step;
   run_range<D>(f, lb, lg, s, end, step);
@@ -565,13 +556,14 @@ __device__ __forceinline__ int rows_classify(const FastArgs& f, const RowWalk<D>
   uint32_t cb = (uint32_t)(fl[0] - f.mstart[0]) + __umul24((uint32_t)f.msy, (uint32_t)(fl[1] - f.mstart[1]));
   if (D == 3) cb += __umul24((uint32_t)f.msz, (uint32_t)(fl[2] - f.mstart[2]));
   const float v = fast_interp<D, P, TEX>(f, cb, off[0], off[1], off[2], t[0], t[1], t[2], grad);
-  const float fv = P::ld32(static_cast<const typename P::pixel*>(f.fixed_px), w.fp);
+  // pass 2 (TEX) takes the fixed bin from pass 1's byte: no fixed read
+  const float fv = TEX ? 0.0f : P::ld32(static_cast<const typename P::pixel*>(f.fixed_px), w.fp);
   return fast_bins(f, v, fv, out, uncertain, fbin, mbin, arg0, nullptr);
 }
 
 template <int D, typename P>
 __device__ __forceinline__ void pass1_rows_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
-  const P1Lds L = pass1_begin(f.bins);
+  const P1Lds L = pass1_begin(f.bins, f.hist_copies);
   unsigned int nvalid = 0;
   uint32_t it, iend, istep;
   rows_range(f, lb, lg, it, iend, istep);
@@ -625,14 +617,13 @@ __device__ __forceinline__ uint32_t exc_lists_sample(const FastArgs& f, const Ex
 // Exact FP64 evaluation of the listed samples of pass 1, as a kernel of its
 // own (B-spline: its exact path would raise pass 1's register count if it ran
 // in pass 1's tail).  Histogram bins take global atomics (integer sums, so the
-// order does not matter); the fixed marginal and the count are summed in LDS
-// first.
+// order does not matter); the valid count is summed in LDS first.
 template <int D, typename P, int MK>
 __device__ __forceinline__ void exceptions_pass1_body(const EvalArgs& a, const FastArgs& f, uint32_t lb) {
   __shared__ ExcLists e;
-  extern __shared__ unsigned int fmarg_e[];  // [B] fixed marginal counts, [B] valid count
+  __shared__ unsigned int n_ok;
   const int B = a.bins;
-  for (int i = threadIdx.x; i <= B; i += blockDim.x) fmarg_e[i] = 0u;
+  if (threadIdx.x == 0) n_ok = 0u;
   exc_lists_setup(f, lb, e);
   const uint32_t ne = e.pre[kExcListsPerGroup];
   if (ne == 0) return;  // uniform over the workgroup
@@ -665,8 +656,7 @@ __device__ __forceinline__ void exceptions_pass1_body(const EvalArgs& a, const F
       f.exc_res[exc_lists_slot(e, t)] = r;
     }
     if (!ok) continue;
-    atomicAdd(&fmarg_e[B], 1u);
-    atomicAdd(&fmarg_e[o.fbin], 1u);
+    atomicAdd(&n_ok, 1u);
     unsigned int q[4];
     parzen_fixed((float)((double)(o.mbin - 1) - o.mterm), q);
     unsigned long long* row = a.hist + o.fbin * B + (o.mbin - 1);
@@ -675,9 +665,7 @@ __device__ __forceinline__ void exceptions_pass1_body(const EvalArgs& a, const F
       if (q[k]) atomicAdd(&row[k], (unsigned long long)q[k]);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < B; i += blockDim.x)
-    if (fmarg_e[i]) atomicAdd(&a.fmarg[i], (unsigned long long)fmarg_e[i]);
-  if (threadIdx.x == 0 && fmarg_e[B]) atomicAdd(a.nvalid, (unsigned long long)fmarg_e[B]);
+  if (threadIdx.x == 0 && n_ok) atomicAdd(a.nvalid, (unsigned long long)n_ok);
 }
 
 // Derivative moments in index space, per sample:
@@ -1051,7 +1039,6 @@ __global__ void __launch_bounds__(256) batch_clear(const FastArgs* __restrict__ 
   const FastArgs& f = fa[blockIdx.x];
   const int BB = f.bins * f.bins;
   for (int i = threadIdx.x; i < BB; i += blockDim.x) f.hist[i] = 0ull;
-  for (int i = threadIdx.x; i < f.bins; i += blockDim.x) f.fmarg[i] = 0ull;
   if (threadIdx.x == 0) {
     *f.nvalid = 0ull;
     *f.exc_overflow = 0u;
@@ -1137,7 +1124,7 @@ __global__ void batch_gather(const BatchOut* __restrict__ items, double* __restr
   } while (0)
 
 hipError_t launch_fast_pass1(const FastArgs& f, const EvalArgs& a, int dim, int grid, hipStream_t st) {
-  const size_t lds = pass1_lds_bytes(f.bins);
+  const size_t lds = pass1_lds_bytes_with(f.bins, f.hist_copies);
   if (f.fmask.px || f.mmask.px) {
     if (f.mk != 1) return hipErrorInvalidValue;
     MMI_DISPATCH_FAST(pass1_mask_kernel, f.storage, dim, grid, lds, st, f, a);
@@ -1151,8 +1138,7 @@ hipError_t launch_fast_pass1(const FastArgs& f, const EvalArgs& a, int dim, int 
 
 hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st) {
   // grid (the pass-1 workgroups) is a multiple of 8 = kExcListsPerGroup
-  MMI_DISPATCH_FAST_MK(exceptions_pass1_kernel, f.mk, f.storage, a.virt.dim, grid / kExcListsPerGroup,
-                       ((size_t)a.bins + 1) * 4, st, a, f);
+  MMI_DISPATCH_FAST_MK(exceptions_pass1_kernel, f.mk, f.storage, a.virt.dim, grid / kExcListsPerGroup, 0, st, a, f);
   return hipGetLastError();
 }
 

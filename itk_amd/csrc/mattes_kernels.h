@@ -94,8 +94,9 @@ struct FastArgs {
   float bt_f;        // bound on |fterm_fast - fterm_reference|
   int32_t xcd_chunks;
   int32_t exc_cap;
+  int32_t hist_copies;  // pass 1's LDS histogram copies (1 or 2; see hist_copies)
+  int32_t pad_hc_;
   unsigned long long* hist;
-  unsigned long long* fmarg;
   unsigned long long* nvalid;
   int16_t* sample_fbin;
   int16_t* sample_mbin;
@@ -174,7 +175,6 @@ struct EvalArgs {
   double moving_bin_size, moving_norm_min;
   double fixed_bin_size, fixed_norm_min;
   unsigned long long* hist;   // [B*B] fixed point 2^-32
-  unsigned long long* fmarg;  // [B]
   unsigned long long* nvalid;
   int16_t* sample_fbin;  // optional [n]
   int16_t* sample_mbin;  // optional [n]
@@ -207,7 +207,6 @@ struct FinalizeArgs {
   int32_t lds;  // set by launch_finalize: PDF and marginals staged in LDS
   // pass 1's accumulators: read, then left zeroed for the next evaluation
   unsigned long long* hist;
-  unsigned long long* fmarg;
   unsigned long long* nvalid;
   uint32_t* overflow;  // exception-list overflow word: moved into the result, zeroed
   double* pdf;     // [B*B] normalised joint PDF
@@ -261,12 +260,12 @@ struct BatchLaunch {
 // LDS and adds their histogram weights with global 64-bit atomics.
 constexpr int kExcCap = 256;
 
-// Copies of the pass-1 joint histogram (and fixed marginal) in LDS: two when
-// they fit in 64 KiB, updated by even and odd lanes.  Neighbouring samples (adjacent lanes) usually fall in the same
-// bins, so this halves the same-address serialisation of the 64-bit LDS
-// atomics.
+// Copies of the pass-1 joint histogram in LDS (plus two counters): two when
+// they fit in 64 KiB, updated by even and odd lanes.  Neighbouring samples
+// (adjacent lanes) usually fall in the same bins, so this halves the
+// same-address serialisation of the 64-bit LDS atomics.
 __host__ __device__ inline size_t pass1_lds_bytes_with(int bins, int copies) {
-  return (size_t)copies * bins * bins * 8 + ((size_t)copies * bins + 2) * 4;
+  return (size_t)copies * bins * bins * 8 + 2 * 4;
 }
 __host__ __device__ inline int hist_copies(int bins) {
   return pass1_lds_bytes_with(bins, 2) <= 64 * 1024 ? 2 : 1;
@@ -339,8 +338,8 @@ hipError_t launch_to_storage(const float* in, void* out, int storage, int64_t n,
 hipError_t launch_texels_to_storage(const float4* in, void* out, int storage, int64_t n, hipStream_t st);
 hipError_t launch_recursive_gaussian(const void* in, bool in_float, double* out, int64_t ln, int64_t stride,
                                      int64_t nlines, const RecCoeffs& c, const RecSink& sink, hipStream_t st);
-hipError_t launch_clear_eval(unsigned long long* hist, int n_hist, unsigned long long* fmarg, int n_fmarg,
-                             unsigned long long* nvalid, uint32_t* overflow, hipStream_t st);
+hipError_t launch_clear_eval(unsigned long long* hist, int n_hist, unsigned long long* nvalid, uint32_t* overflow,
+                             hipStream_t st);
 // Per-workgroup min/max over the masked voxels of an image (mask) or over the
 // pixels the listed virtual points round to after the fixed transform (points).
 hipError_t launch_minmax_select(const void* px, int storage, const DevGeom& img, const DevMask* mask,

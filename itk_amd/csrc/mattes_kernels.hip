@@ -43,12 +43,11 @@ __global__ void __launch_bounds__(256) pass1_kernel(EvalArgs a) {
   const int B = a.bins;
   const int BB = B * B;
   unsigned long long* hist = a.hist_in_lds ? lds_u64 : a.hist;
-  unsigned int* fmarg = reinterpret_cast<unsigned int*>(lds_u64 + (a.hist_in_lds ? BB : 0));
-  unsigned int* nvalid = fmarg + B;
+  unsigned int* nvalid = reinterpret_cast<unsigned int*>(lds_u64 + (a.hist_in_lds ? BB : 0));
   if (a.hist_in_lds) {
     for (int i = threadIdx.x; i < BB; i += blockDim.x) hist[i] = 0ull;
   }
-  for (int i = threadIdx.x; i < B + 1; i += blockDim.x) fmarg[i] = 0u;
+  if (threadIdx.x == 0) *nvalid = 0u;
   __syncthreads();
 
   const uint32_t n = (uint32_t)a.n;
@@ -66,7 +65,6 @@ __global__ void __launch_bounds__(256) pass1_kernel(EvalArgs a) {
     const unsigned long long vm = __ballot(ok);
     if ((threadIdx.x & 63) == 0 && vm) atomicAdd(nvalid, (unsigned int)__popcll(vm));
     if (!ok) continue;
-    atomicAdd(&fmarg[o.fbin], 1u);
     unsigned int q[4];
     parzen_fixed((float)((double)(o.mbin - 1) - o.mterm), q);
     unsigned long long* row = hist + o.fbin * B + (o.mbin - 1);
@@ -80,10 +78,6 @@ __global__ void __launch_bounds__(256) pass1_kernel(EvalArgs a) {
       const unsigned long long v = hist[i];
       if (v) atomicAdd(&a.hist[i], v);
     }
-  }
-  for (int i = threadIdx.x; i < B; i += blockDim.x) {
-    const unsigned int v = fmarg[i];
-    if (v) atomicAdd(&a.fmarg[i], (unsigned long long)v);
   }
   if (threadIdx.x == 0 && *nvalid) atomicAdd(a.nvalid, (unsigned long long)*nvalid);
 }
@@ -133,7 +127,6 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& f) {
   // pass 1's accumulators are left zeroed for the next evaluation once read
   auto release = [&]() {
     for (int i = t; i < BB; i += blockDim.x) f.hist[i] = 0ull;
-    for (int i = t; i < B; i += blockDim.x) f.fmarg[i] = 0ull;
     if (t == 0) {
       *f.nvalid = 0ull;
       *f.overflow = 0u;
@@ -163,8 +156,14 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& f) {
     pdf[i] = v;
     if (f.lds) f.pdf[i] = v;
   }
+  // fixed marginal (the reference counts each valid sample once in its fixed
+  // bin): the Parzen weights of a counted sample sum to 2^32 within a few
+  // units, so a row's total rounded to a multiple of 2^32 is the row's count
+  // exactly (below 2^30 samples)
   for (int i = t; i < B; i += blockDim.x) {
-    const double fmv = (double)f.fmarg[i] * nf;
+    unsigned long long rs = 0ull;
+    for (int m = 0; m < B; ++m) rs += f.hist[i * B + m];
+    const double fmv = (double)((rs + (1ull << 31)) >> 32) * nf;
     fm[i] = fmv;
     if (f.lds) f.fm[i] = fmv;
     lrow[i] = fmv > closeToZero ? log(fmv) : 0.0;  // log of the fixed marginal, once per row
@@ -355,7 +354,7 @@ __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
   } while (0)
 
 hipError_t launch_pass1(const EvalArgs& a, int grid, hipStream_t st) {
-  const size_t lds = (a.hist_in_lds ? (size_t)a.bins * a.bins * 8 : 0) + (size_t)(a.bins + 1) * 4;
+  const size_t lds = (a.hist_in_lds ? (size_t)a.bins * a.bins * 8 : 0) + 8;
   MMI_DISPATCH_EXACT(pass1_kernel, a, grid, lds, st);
   return hipGetLastError();
 }

@@ -594,23 +594,20 @@ hipError_t launch_pack_texels(const float* moving, const double* grad, int64_t n
   return hipGetLastError();
 }
 
-// The accumulators an evaluation starts from: joint histogram, fixed
-// marginal, valid count, exception-overflow flag (one launch instead of four
-// memsets).
+// The accumulators an evaluation starts from: joint histogram, valid count,
+// exception-overflow flag (one launch instead of three memsets).
 __global__ void clear_eval_kernel(unsigned long long* __restrict__ hist, int n_hist,
-                                  unsigned long long* __restrict__ fmarg, int n_fmarg,
                                   unsigned long long* __restrict__ nvalid, uint32_t* __restrict__ overflow) {
   for (int i = threadIdx.x; i < n_hist; i += blockDim.x) hist[i] = 0ull;
-  for (int i = threadIdx.x; i < n_fmarg; i += blockDim.x) fmarg[i] = 0ull;
   if (threadIdx.x == 0) {
     *nvalid = 0ull;
     *overflow = 0u;
   }
 }
 
-hipError_t launch_clear_eval(unsigned long long* hist, int n_hist, unsigned long long* fmarg, int n_fmarg,
-                             unsigned long long* nvalid, uint32_t* overflow, hipStream_t st) {
-  hipLaunchKernelGGL(clear_eval_kernel, dim3(1), dim3(1024), 0, st, hist, n_hist, fmarg, n_fmarg, nvalid, overflow);
+hipError_t launch_clear_eval(unsigned long long* hist, int n_hist, unsigned long long* nvalid, uint32_t* overflow,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(clear_eval_kernel, dim3(1), dim3(1024), 0, st, hist, n_hist, nvalid, overflow);
   return hipGetLastError();
 }
 
