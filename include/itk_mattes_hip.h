@@ -209,6 +209,25 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
  * mmi_initialize. */
 int mmi_set_launch_config(mmi_context* ctx, int32_t blocks_per_cu, int32_t rows_walk);
 
+/* mmi_evaluate without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * buffer the evaluation writes besides value, derivative and n_valid lives in
+ * one static device array per GPU (about 19 MiB, resolved at
+ * mmi_context_create), so the call makes no hipMalloc, hipFree or pinned-host
+ * allocation.  Limits: an initialised context on the certified affine fast
+ * path (affine moving transform, identity fixed transform, dense sampling of
+ * virtual voxels on fixed pixel centres, no masks or point set, the moving
+ * gradient image for the derivative, no MMI_FLAG_RECORD_SAMPLE_BINS or
+ * MMI_FLAG_EXACT_PATH_ONLY), at most 2^24 virtual samples, 128 histogram
+ * bins, 12 parameters and 2048 workgroups per pass; anything else returns
+ * MMI_ERR_ARGUMENT.  The array is shared by every context on the GPU and
+ * keeps the last call's contents: calls that overlap on one GPU (several host
+ * threads, or contexts whose streams run concurrently) race.  Results equal
+ * mmi_evaluate's on the same context. */
+int mmi_evaluate_static(mmi_context* ctx, int32_t want_derivative, double* value, double* derivative,
+                        int64_t* n_valid);
+
 /* Per-sample Parzen bins of the last evaluation (int16, -1 = sample not
  * counted), in virtual buffer order; for parity tests.  Either may be NULL. */
 int mmi_get_sample_bins(mmi_context* ctx, int16_t* fixed_bin, int16_t* moving_bin);

@@ -61,6 +61,7 @@ SIGNATURES = {
     # pointers costs microseconds per call
     "mmi_set_moving_parameters": (ctypes.c_int, [_vp, _vp, ctypes.c_int64]),
     "mmi_evaluate": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp]),
+    "mmi_evaluate_static": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _vp, _vp]),
     "mmi_get_sample_bins": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int16), ctypes.POINTER(ctypes.c_int16)]),
     "mmi_get_moving_texels": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "mmi_get_histogram_parameters": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),

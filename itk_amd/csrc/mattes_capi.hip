@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -279,7 +280,7 @@ struct mmi_context {
   DevTransform dftf{}, dmtf{};
   DevBuf coeffs;
 
-  DevBuf hist, nvalid, pdf, fm, mm, pratio, pratio_f, rowsum, partials, sample_fbin, sample_mbin, scratch;
+  DevBuf hist, nvalid, pdf, fm, mm, pratio, pratio_f, rowsum, partials, sample_fbin, sample_mbin, minmax_out;
   // the evaluation's outputs in one block (one copy to the host): the result,
   // the fast path's exception-overflow flag, the derivative (dev_result,
   // dev_overflow, dev_deriv)
@@ -387,11 +388,118 @@ namespace {
 static_assert(sizeof(DevResult) % 8 == 0, "derivative alignment");
 size_t out_bytes(int64_t P) { return sizeof(DevResult) + 16 + sizeof(double) * (size_t)P; }
 DevResult* dev_result(const mmi_context* ctx) { return ctx->out.as<DevResult>(); }
-uint32_t* dev_overflow(const mmi_context* ctx) {
-  return reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->out.p) + sizeof(DevResult));
-}
 double* dev_deriv(const mmi_context* ctx) {
   return reinterpret_cast<double*>(static_cast<char*>(ctx->out.p) + sizeof(DevResult) + 16);
+}
+
+// Device buffers an evaluation writes: the context's own (grown on first use)
+// or the static scratch of mmi_evaluate_static.
+struct EvalBufs {
+  unsigned long long* hist;  // [B * B]
+  unsigned long long* nvalid;
+  char* out;  // out_bytes(P): DevResult, exception-overflow flag, derivative
+  double* pdf;
+  double* fm;
+  double* mm;
+  double* pratio;
+  float* pratio_f;
+  double* rowsum;
+  double* partials;     // [2 * grid * P]
+  uint32_t* exc_count;  // [grid]
+  uint32_t* exc_list;   // [grid * kExcCap]
+  uint8_t* cls;         // [n] pass 1's byte per sample; NULL: the context's, grown when used
+  DevResult* result() const { return reinterpret_cast<DevResult*>(out); }
+  uint32_t* overflow() const { return reinterpret_cast<uint32_t*>(out + sizeof(DevResult)); }
+  double* deriv() const { return reinterpret_cast<double*>(out + sizeof(DevResult) + 16); }
+};
+
+// ---- Device scratch of mmi_evaluate_static ---------------------------------
+// One file-scope device array per GPU holds every buffer the evaluation
+// writes besides the caller's outputs: the joint histogram and counters, the
+// finalize tables, the result block, the derivative partials, the exception
+// lists and pass 1's byte per sample.  Sized for the limits below
+// (kStaticScratchBytes, about 19 MiB).
+constexpr int kStaticMaxBins = 128;
+constexpr int64_t kStaticMaxSamples = 1ll << 24;  // 256^3 virtual samples
+constexpr int kStaticMaxGrid = 2048;               // workgroups per pass
+constexpr int kStaticMaxParams = 12;               // 3-D affine
+constexpr size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t kStaticB2 = (size_t)kStaticMaxBins * kStaticMaxBins;
+constexpr size_t kOffHist = 0;
+constexpr size_t kOffNvalid = kOffHist + al256(kStaticB2 * 8);
+constexpr size_t kOffOut = kOffNvalid + 256;
+constexpr size_t kOffPdf = kOffOut + al256(sizeof(DevResult) + 16 + 8 * kStaticMaxParams);
+constexpr size_t kOffFm = kOffPdf + al256(kStaticB2 * 8);
+constexpr size_t kOffMm = kOffFm + al256(kStaticMaxBins * 8);
+constexpr size_t kOffRowsum = kOffMm + al256(kStaticMaxBins * 8);
+constexpr size_t kOffPratio = kOffRowsum + al256(kStaticMaxBins * 8);
+constexpr size_t kOffPratioF = kOffPratio + al256(kStaticB2 * 8);
+constexpr size_t kOffPartials = kOffPratioF + al256(kStaticB2 * 4);
+constexpr size_t kOffExcCount = kOffPartials + al256((size_t)2 * kStaticMaxGrid * kStaticMaxParams * 8);
+constexpr size_t kOffExcList = kOffExcCount + al256((size_t)kStaticMaxGrid * 4);
+constexpr size_t kOffCls = kOffExcList + al256((size_t)kStaticMaxGrid * kExcCap * 4);
+constexpr size_t kStaticScratchBytes = kOffCls + al256((size_t)kStaticMaxSamples);
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+
+// In host code this name is only a handle for hipGetSymbolAddress: kernels
+// and copies take the per-device address resolved below.
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` on each GPU
+
+// Resolves `scratch` on `device` (current) once; later calls only read.
+int resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(MMI_ERR_ARGUMENT, "device index above 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return MMI_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(MMI_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+  return MMI_OK;
+}
+
+EvalBufs scratch_bufs(unsigned char* base) {
+  EvalBufs b;
+  b.hist = reinterpret_cast<unsigned long long*>(base + kOffHist);
+  b.nvalid = reinterpret_cast<unsigned long long*>(base + kOffNvalid);
+  b.out = reinterpret_cast<char*>(base + kOffOut);
+  b.pdf = reinterpret_cast<double*>(base + kOffPdf);
+  b.fm = reinterpret_cast<double*>(base + kOffFm);
+  b.mm = reinterpret_cast<double*>(base + kOffMm);
+  b.pratio = reinterpret_cast<double*>(base + kOffPratio);
+  b.pratio_f = reinterpret_cast<float*>(base + kOffPratioF);
+  b.rowsum = reinterpret_cast<double*>(base + kOffRowsum);
+  b.partials = reinterpret_cast<double*>(base + kOffPartials);
+  b.exc_count = reinterpret_cast<uint32_t*>(base + kOffExcCount);
+  b.exc_list = reinterpret_cast<uint32_t*>(base + kOffExcList);
+  b.cls = reinterpret_cast<uint8_t*>(base + kOffCls);
+  return b;
+}
+
+// The context's buffers as they are (no growth).
+EvalBufs ctx_bufs(mmi_context* ctx) {
+  EvalBufs b;
+  b.hist = ctx->hist.as<unsigned long long>();
+  b.nvalid = ctx->nvalid.as<unsigned long long>();
+  b.out = static_cast<char*>(ctx->out.p);
+  b.pdf = ctx->pdf.as<double>();
+  b.fm = ctx->fm.as<double>();
+  b.mm = ctx->mm.as<double>();
+  b.pratio = ctx->pratio.as<double>();
+  b.pratio_f = ctx->pratio_f.as<float>();
+  b.rowsum = ctx->rowsum.as<double>();
+  b.partials = ctx->partials.as<double>();
+  b.exc_count = ctx->exc_count.as<uint32_t>();
+  b.exc_list = ctx->exc_list.as<uint32_t>();
+  b.cls = nullptr;
+  return b;
 }
 
 int ensure_host_pinned(void** p, size_t* have, size_t need) {
@@ -943,10 +1051,10 @@ int refresh_storage_texels(mmi_context* ctx) {
 
 int device_minmax(mmi_context* ctx, const void* px, int storage, int64_t n, double* lo, double* hi) {
   const int blocks = (int)std::min<int64_t>(1024, (n + 255) / 256);
-  MMI_HIP(ctx->scratch.ensure(sizeof(float) * 2 * blocks));
-  MMI_HIP(launch_minmax(px, storage, n, ctx->scratch.as<float>(), blocks, ctx->stream));
+  MMI_HIP(ctx->minmax_out.ensure(sizeof(float) * 2 * blocks));
+  MMI_HIP(launch_minmax(px, storage, n, ctx->minmax_out.as<float>(), blocks, ctx->stream));
   std::vector<float> h(2 * blocks);
-  MMI_HIP(hipMemcpyAsync(h.data(), ctx->scratch.p, sizeof(float) * 2 * blocks, hipMemcpyDeviceToHost, ctx->stream));
+  MMI_HIP(hipMemcpyAsync(h.data(), ctx->minmax_out.p, sizeof(float) * 2 * blocks, hipMemcpyDeviceToHost, ctx->stream));
   MMI_HIP(hipStreamSynchronize(ctx->stream));
   float a = std::numeric_limits<float>::max(), b = -std::numeric_limits<float>::max();
   for (int i = 0; i < blocks; ++i) {
@@ -961,11 +1069,11 @@ int device_minmax(mmi_context* ctx, const void* px, int storage, int64_t n, doub
 int device_minmax_select(mmi_context* ctx, const void* px, const mmi_geometry& g, const DevMask* mask,
                          const double* points, int64_t n, double* lo, double* hi) {
   const int blocks = (int)std::min<int64_t>(1024, (n + 255) / 256);
-  MMI_HIP(ctx->scratch.ensure(sizeof(float) * 2 * blocks));
-  MMI_HIP(launch_minmax_select(px, ctx->active_storage, to_dev(g), mask, points, ctx->dftf, n, ctx->scratch.as<float>(),
+  MMI_HIP(ctx->minmax_out.ensure(sizeof(float) * 2 * blocks));
+  MMI_HIP(launch_minmax_select(px, ctx->active_storage, to_dev(g), mask, points, ctx->dftf, n, ctx->minmax_out.as<float>(),
                                blocks, ctx->stream));
   std::vector<float> h(2 * blocks);
-  MMI_HIP(hipMemcpyAsync(h.data(), ctx->scratch.p, sizeof(float) * 2 * blocks, hipMemcpyDeviceToHost, ctx->stream));
+  MMI_HIP(hipMemcpyAsync(h.data(), ctx->minmax_out.p, sizeof(float) * 2 * blocks, hipMemcpyDeviceToHost, ctx->stream));
   MMI_HIP(hipStreamSynchronize(ctx->stream));
   float a = std::numeric_limits<float>::max(), b = -std::numeric_limits<float>::max();
   for (int i = 0; i < blocks; ++i) {
@@ -1071,6 +1179,9 @@ int mmi_context_create(int device, uint32_t flags, mmi_context** out) {
   MMI_HIP(hipGetDeviceCount(&count));
   if (device < 0 || device >= count) return fail(MMI_ERR_ARGUMENT, "no such HIP device");
   MMI_HIP(hipSetDevice(device));
+  // the static scratch's address on this device, once per device (the
+  // library's set-up; mmi_evaluate_static only reads it)
+  if (int rc = resolve_scratch(device)) return rc;
   auto* ctx = new mmi_context();
   ctx->device = device;
   ctx->flags = flags;
@@ -1110,7 +1221,7 @@ int mmi_context_destroy(mmi_context* ctx) {
                     &ctx->coeffs, &ctx->hist,
                     &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->out, &ctx->points, &ctx->point_ids, &ctx->mask_px[0], &ctx->mask_px[1],
                     &ctx->mask_desc[0], &ctx->mask_desc[1],
-                    &ctx->partials, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->scratch,
+                    &ctx->partials, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->minmax_out,
                     &ctx->exc_count, &ctx->exc_list, &ctx->exc_res, &ctx->cls, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->bs_rows_t1, &ctx->ftab, &ctx->work_a,
                     &ctx->work_b, &ctx->work_grad};
   for (DevBuf* b : bufs) b->release();
@@ -1521,7 +1632,8 @@ namespace {
 
 // The separable B-spline derivative from the per-sample factors in bs_g:
 // contraction along x (LDS-staged rows), then y, then z with the nFactor scale.
-int bspline_contract(mmi_context* ctx, const DevGeom& virt, const AxisTable* tabs, const DevResult* result) {
+int bspline_contract(mmi_context* ctx, const DevGeom& virt, const AxisTable* tabs, const DevResult* result,
+                     double* deriv) {
   const int D = virt.dim;
   hipStream_t st = ctx->stream;
   const int64_t Nx = virt.size[0], Ny = virt.size[1], Nz = D == 3 ? virt.size[2] : 1;
@@ -1533,10 +1645,10 @@ int bspline_contract(mmi_context* ctx, const DevGeom& virt, const AxisTable* tab
     MMI_HIP(ctx->bs_t2.ensure(sizeof(double) * D * Nz * Gy * Gx));
     MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t1.as<double>(), ctx->bs_t2.as<double>(), D * Nz, (int)Ny, Gx,
                                       Gy, tabs[1], nullptr, st));
-    MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t2.as<double>(), dev_deriv(ctx), D, (int)Nz,
+    MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t2.as<double>(), deriv, D, (int)Nz,
                                       (int64_t)Gy * Gx, Gz, tabs[2], result, st));
   } else {
-    MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t1.as<double>(), dev_deriv(ctx), D, (int)Ny, Gx, Gy,
+    MMI_HIP(launch_separable_contract(nullptr, ctx->bs_t1.as<double>(), deriv, D, (int)Ny, Gx, Gy,
                                       tabs[1], result, st));
   }
   return MMI_OK;
@@ -1544,19 +1656,22 @@ int bspline_contract(mmi_context* ctx, const DevGeom& virt, const AxisTable* tab
 
 // Validation, buffers and kernel arguments of one evaluation; `fast` tells
 // whether the certified fast path applies (then `fa` is complete, including
-// the derivative partials when want_derivative is set).
-int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArgs& fa, bool& fast) {
+// the derivative partials when want_derivative is set).  sb: the buffers to
+// write (mmi_evaluate_static's scratch), or NULL for the context's own,
+// grown to this evaluation's sizes; `b` receives the ones used.
+int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArgs& fa, bool& fast,
+                 const EvalBufs* sb, EvalBufs& b) {
   if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
   if (!ctx->initialized) return fail(MMI_ERR_ARGUMENT, "mmi_initialize must precede mmi_evaluate");
   if (ctx->mtf.kind == MMI_TF_IDENTITY && want_derivative)
     return fail(MMI_ERR_ARGUMENT, "MovingTransform is not present");
   if (ctx->virtual_g.dim != ctx->fixed_g.dim) return fail(MMI_ERR_ARGUMENT, "virtual domain dimension differs");
   MMI_HIP(hipSetDevice(ctx->device));
-  const int D = ctx->fixed_g.dim;
   const int B = ctx->bins;
   const int64_t n = sample_count(ctx);
   const int64_t P = (int64_t)ctx->mtf.params.size();
-  MMI_HIP(ctx->out.ensure(out_bytes(P)));
+  if (!sb) MMI_HIP(ctx->out.ensure(out_bytes(P)));
+  b = sb ? *sb : ctx_bufs(ctx);
   std::memset(&a, 0, sizeof(a));
   a.virt = to_dev(ctx->virtual_g);
   a.fixed = to_dev(ctx->fixed_g);
@@ -1581,14 +1696,14 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
   a.moving_norm_min = ctx->moving_norm_min;
   a.fixed_bin_size = ctx->fixed_bin_size;
   a.fixed_norm_min = ctx->fixed_norm_min;
-  a.hist = ctx->hist.as<unsigned long long>();
-  a.nvalid = ctx->nvalid.as<unsigned long long>();
+  a.hist = b.hist;
+  a.nvalid = b.nvalid;
   if (ctx->flags & 1u) {
     a.sample_fbin = ctx->sample_fbin.as<int16_t>();
     a.sample_mbin = ctx->sample_mbin.as<int16_t>();
   }
-  a.pratio = ctx->pratio.as<double>();
-  a.pratio_f = ctx->pratio_f.as<float>();
+  a.pratio = b.pratio;
+  a.pratio_f = b.pratio_f;
   a.pratio_in_lds = (size_t)B * B * 4 + 512 <= 64 * 1024 ? 1 : 0;
   if (want_derivative && ctx->gradient_source == MMI_GRADIENT_IMAGE && !ctx->have_texels)
     return fail(MMI_ERR_ARGUMENT, "moving gradient image missing");
@@ -1596,8 +1711,12 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
   fast = setup_fast(ctx, &fa);
   ctx->last_path = fast ? 1 : 0;
   if (fast) {
-    MMI_HIP(ctx->exc_count.ensure(sizeof(uint32_t) * ctx->grid));
-    MMI_HIP(ctx->exc_list.ensure(sizeof(uint32_t) * ctx->grid * kExcCap));
+    if (!sb) {
+      MMI_HIP(ctx->exc_count.ensure(sizeof(uint32_t) * ctx->grid));
+      MMI_HIP(ctx->exc_list.ensure(sizeof(uint32_t) * ctx->grid * kExcCap));
+      b.exc_count = ctx->exc_count.as<uint32_t>();
+      b.exc_list = ctx->exc_list.as<uint32_t>();
+    }
     if (ctx->mtf.kind == MMI_TF_BSPLINE) {
       MMI_HIP(ctx->exc_res.ensure(sizeof(ExcResult) * ctx->grid * kExcCap));
       fa.exc_res = ctx->exc_res.as<ExcResult>();
@@ -1617,33 +1736,34 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
     fa.sample_fbin = a.sample_fbin;
     fa.sample_mbin = a.sample_mbin;
     fa.pratio_f = a.pratio_f;
-    fa.exc_count = ctx->exc_count.as<uint32_t>();
-    fa.exc_list = ctx->exc_list.as<uint32_t>();
-    fa.exc_overflow = dev_overflow(ctx);
+    fa.exc_count = b.exc_count;
+    fa.exc_list = b.exc_list;
+    fa.exc_overflow = b.overflow();
   }
   if (fast && want_derivative) {
-    MMI_HIP(ctx->partials.ensure(sizeof(double) * P * 2 * ctx->grid));
-    fa.partials = ctx->partials.as<double>();
+    if (!sb) {
+      MMI_HIP(ctx->partials.ensure(sizeof(double) * P * 2 * ctx->grid));
+      b.partials = ctx->partials.as<double>();
+    }
+    fa.partials = b.partials;
   }
-  (void)n;
-  (void)D;
   return MMI_OK;
 }
 
-FinalizeArgs finalize_args(mmi_context* ctx) {
+FinalizeArgs finalize_args(const mmi_context* ctx, const EvalBufs& b) {
   FinalizeArgs f;
   std::memset(&f, 0, sizeof(f));
   f.bins = ctx->bins;
-  f.hist = ctx->hist.as<unsigned long long>();
-  f.nvalid = ctx->nvalid.as<unsigned long long>();
-  f.pdf = ctx->pdf.as<double>();
-  f.fm = ctx->fm.as<double>();
-  f.mm = ctx->mm.as<double>();
-  f.pratio = ctx->pratio.as<double>();
-  f.pratio_f = ctx->pratio_f.as<float>();
-  f.rowsum = ctx->rowsum.as<double>();
-  f.result = dev_result(ctx);
-  f.overflow = dev_overflow(ctx);
+  f.hist = b.hist;
+  f.nvalid = b.nvalid;
+  f.pdf = b.pdf;
+  f.fm = b.fm;
+  f.mm = b.mm;
+  f.pratio = b.pratio;
+  f.pratio_f = b.pratio_f;
+  f.rowsum = b.rowsum;
+  f.result = b.result();
+  f.overflow = b.overflow();
   f.moving_bin_size = ctx->moving_bin_size;
   return f;
 }
@@ -1658,17 +1778,20 @@ FastMap fast_map(const mmi_context* ctx) {
   return map;
 }
 
-}  // namespace
-
-extern "C" {
-
-int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, double* derivative, int64_t* n_valid,
-                 double* joint_pdf, double* fixed_marginal) {
+// GetValueAndDerivative / GetValue into the buffers of `sb` (NULL: the
+// context's own).  With sb the context must be on the affine fast path (the
+// caller checks): the branches that grow B-spline buffers are not reached,
+// the results come back through the caller's host memory instead of the
+// context's pinned staging, and the accumulators are cleared first (other
+// contexts on the device share the scratch).
+int evaluate_impl(mmi_context* ctx, int32_t want_derivative, double* value, double* derivative, int64_t* n_valid,
+                  double* joint_pdf, double* fixed_marginal, const EvalBufs* sb) {
   if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
   EvalArgs a;
   FastArgs fa;
+  EvalBufs b;
   bool fast = false;
-  if (int rc = prepare_eval(ctx, want_derivative, a, fa, fast)) return rc;
+  if (int rc = prepare_eval(ctx, want_derivative, a, fa, fast, sb, b)) return rc;
   const int D = ctx->fixed_g.dim;
   const int B = ctx->bins;
   const int64_t n = sample_count(ctx);
@@ -1679,12 +1802,11 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
   const bool time_passes = (ctx->flags & 4u) != 0;
 retry:
   MMI_HIP(hipEventRecord(ctx->ev[0], st));
-  // pass 1's accumulators (histogram, marginal, count, overflow word) are
-  // zero between evaluations: finalize leaves them so; a clear kernel runs
-  // only after initialisation or an interrupted evaluation
-  if (!ctx->acc_clean)
-    MMI_HIP(launch_clear_eval(ctx->hist.as<unsigned long long>(), B * B, ctx->nvalid.as<unsigned long long>(),
-                              dev_overflow(ctx), st));
+  // pass 1's accumulators (histogram, count, overflow word) are zero between
+  // evaluations of a context: finalize leaves them so; a clear kernel runs
+  // only after initialisation, an interrupted evaluation or in the scratch
+  if (sb || !ctx->acc_clean)
+    MMI_HIP(launch_clear_eval(b.hist, B * B, b.nvalid, b.overflow(), st));
   ctx->acc_clean = false;
   if (fast && fa.mk == 2) {
     AxisTable rt[3];
@@ -1703,8 +1825,12 @@ retry:
   // then also takes the moving values from the texels it gathers for the
   // gradient: their value channel must be the level's moving image
   if (fast && want_derivative && B <= 253 && !(ctx->flags & MMI_FLAG_PASS2_RECLASSIFY) && ctx->texel_values_level) {
-    MMI_HIP(ctx->cls.ensure((size_t)n));
-    fa.cls = ctx->cls.as<uint8_t>();
+    if (b.cls) {
+      fa.cls = b.cls;
+    } else {
+      MMI_HIP(ctx->cls.ensure((size_t)n));
+      fa.cls = ctx->cls.as<uint8_t>();
+    }
   }
   // a row-walk pass 1 needs a pass 2 that follows its byte per sample (or the
   // exact pass 2 of central differences, which classifies on its own)
@@ -1718,7 +1844,7 @@ retry:
   }
   if (time_passes || !want_derivative) MMI_HIP(hipEventRecord(ctx->ev[1], st));
 
-  const FinalizeArgs f = finalize_args(ctx);
+  const FinalizeArgs f = finalize_args(ctx, b);
   MMI_HIP(launch_finalize(f, st));
 
   AxisTable tabs[3];
@@ -1736,17 +1862,20 @@ retry:
       MMI_HIP(ctx->bs_g.ensure(sizeof(float) * D * n));
       fa.bspline_g = ctx->bs_g.as<float>();
       MMI_HIP(launch_fast_pass2(fa, a, D, ctx->grid, st));  // its tail writes the listed samples' factors
-      if (int rc = bspline_contract(ctx, a.virt, tabs, f.result)) return rc;
+      if (int rc = bspline_contract(ctx, a.virt, tabs, f.result, b.deriv())) return rc;
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     } else if (fast2) {
       MMI_HIP(launch_fast_pass2(fa, a, D, ctx->grid, st));
-      MMI_HIP(launch_fast_derivative(fa.partials, 2 * ctx->grid, fast_map(ctx), D, f.result, dev_deriv(ctx), st));
+      MMI_HIP(launch_fast_derivative(fa.partials, 2 * ctx->grid, fast_map(ctx), D, f.result, b.deriv(), st));
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     } else if (ctx->mtf.kind == MMI_TF_AFFINE) {
-      MMI_HIP(ctx->partials.ensure(sizeof(double) * P * ctx->grid));
-      a.partials = ctx->partials.as<double>();
+      if (!sb) {
+        MMI_HIP(ctx->partials.ensure(sizeof(double) * P * ctx->grid));
+        b.partials = ctx->partials.as<double>();
+      }
+      a.partials = b.partials;
       MMI_HIP(launch_pass2(a, ctx->grid, st));
-      MMI_HIP(launch_derivative_reduce(a.partials, ctx->grid, (int)P, f.result, dev_deriv(ctx), st));
+      MMI_HIP(launch_derivative_reduce(a.partials, ctx->grid, (int)P, f.result, b.deriv(), st));
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     } else if (separable) {
       // Separable geometry: the B-spline Jacobian weight of node (a, b, c) at
@@ -1756,32 +1885,41 @@ retry:
       MMI_HIP(ctx->bs_g.ensure(sizeof(float) * D * n));
       a.bspline_g = ctx->bs_g.as<float>();
       MMI_HIP(launch_pass2(a, ctx->grid, st));
-      if (int rc = bspline_contract(ctx, a.virt, tabs, f.result)) return rc;
+      if (int rc = bspline_contract(ctx, a.virt, tabs, f.result, b.deriv())) return rc;
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     } else {
-      MMI_HIP(hipMemsetAsync(dev_deriv(ctx), 0, sizeof(double) * P, st));
-      a.deriv_accum = dev_deriv(ctx);
+      MMI_HIP(hipMemsetAsync(b.deriv(), 0, sizeof(double) * P, st));
+      a.deriv_accum = b.deriv();
       MMI_HIP(launch_pass2(a, ctx->grid, st));
-      MMI_HIP(launch_derivative_scale(dev_deriv(ctx), P, f.result, st));
+      MMI_HIP(launch_derivative_scale(b.deriv(), P, f.result, st));
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     }
   }
   // result, overflow flag and derivative: one copy of the output block
-  // through pinned memory
+  // (through the context's pinned staging; the static variant copies into a
+  // block on its stack and the caller's arrays)
   const size_t ob = out_bytes(want_derivative && derivative ? P : 0);
-  if (int rc = ensure_host_pinned(&ctx->h_out, &ctx->h_out_bytes, out_bytes(P))) return rc;
-  char* hout = static_cast<char*>(ctx->h_out);
-  const double* h_deriv = reinterpret_cast<const double*>(hout + sizeof(DevResult) + 16);
-  MMI_HIP(hipMemcpyAsync(hout, ctx->out.p, ob, hipMemcpyDeviceToHost, st));
-  if (joint_pdf) MMI_HIP(hipMemcpyAsync(joint_pdf, ctx->pdf.p, sizeof(double) * B * B, hipMemcpyDeviceToHost, st));
-  if (fixed_marginal)
-    MMI_HIP(hipMemcpyAsync(fixed_marginal, ctx->fm.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
-  MMI_HIP(hipStreamSynchronize(st));
-  ctx->acc_clean = true;  // finalize ran
   DevResult r;
-  std::memcpy(&r, hout, sizeof(DevResult));
+  if (sb) {
+    MMI_HIP(hipMemcpyAsync(&r, b.out, sizeof(DevResult), hipMemcpyDeviceToHost, st));
+    if (want_derivative && derivative)
+      MMI_HIP(hipMemcpyAsync(derivative, b.deriv(), sizeof(double) * P, hipMemcpyDeviceToHost, st));
+  } else {
+    if (int rc = ensure_host_pinned(&ctx->h_out, &ctx->h_out_bytes, out_bytes(P))) return rc;
+    MMI_HIP(hipMemcpyAsync(ctx->h_out, b.out, ob, hipMemcpyDeviceToHost, st));
+  }
+  if (joint_pdf) MMI_HIP(hipMemcpyAsync(joint_pdf, b.pdf, sizeof(double) * B * B, hipMemcpyDeviceToHost, st));
+  if (fixed_marginal)
+    MMI_HIP(hipMemcpyAsync(fixed_marginal, b.fm, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+  MMI_HIP(hipStreamSynchronize(st));
+  ctx->acc_clean = !sb;  // finalize ran (it zeroes the accumulators it read)
+  if (!sb) {
+    const char* hout = static_cast<const char*>(ctx->h_out);
+    std::memcpy(&r, hout, sizeof(DevResult));
+    if (want_derivative && derivative)
+      std::memcpy(derivative, hout + sizeof(DevResult) + 16, sizeof(double) * P);
+  }
   const bool overflow = r.exc_overflow != 0;
-  if (want_derivative && derivative) std::memcpy(derivative, h_deriv, sizeof(double) * P);
   if (overflow) {
     // more uncertain samples in one workgroup than its list holds: redo exactly
     fast = false;
@@ -1810,8 +1948,51 @@ retry:
                 "image centers by translation.");
   if (r.status == 2) return fail(MMI_ERR_DEGENERATE, "Joint PDF summed to zero");
   if (value) *value = r.value;
-  (void)D;
   return MMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, double* derivative, int64_t* n_valid,
+                 double* joint_pdf, double* fixed_marginal) {
+  return evaluate_impl(ctx, want_derivative, value, derivative, n_valid, joint_pdf, fixed_marginal, nullptr);
+}
+
+int mmi_evaluate_static(mmi_context* ctx, int32_t want_derivative, double* value, double* derivative,
+                        int64_t* n_valid) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (!ctx->initialized) return fail(MMI_ERR_ARGUMENT, "mmi_initialize must precede mmi_evaluate_static");
+  unsigned char* base = g_scratch_ptr[ctx->device];
+  if (!base) return fail(MMI_ERR_DEVICE, "static scratch not resolved on this device");
+  const int64_t P = (int64_t)ctx->mtf.params.size();
+  if (ctx->bins > kStaticMaxBins)
+    return fail(MMI_ERR_ARGUMENT, "mmi_evaluate_static: more than 128 histogram bins");
+  if (sample_count(ctx) > kStaticMaxSamples)
+    return fail(MMI_ERR_ARGUMENT, "mmi_evaluate_static: more than 2^24 virtual samples");
+  if (ctx->grid > kStaticMaxGrid)
+    return fail(MMI_ERR_ARGUMENT, "mmi_evaluate_static: more than 2048 workgroups per pass");
+  if (P > kStaticMaxParams) return fail(MMI_ERR_ARGUMENT, "mmi_evaluate_static: more than 12 parameters");
+  // the affine fast path over the virtual voxels: no allocation on any branch
+  // it can take (the exact-path redo after a list overflow included)
+  const bool ok = ctx->mtf.kind == MMI_TF_AFFINE && ctx->ftf.kind == MMI_TF_IDENTITY && ctx->n_points == 0 &&
+                  !ctx->have_mask[0] && !ctx->have_mask[1] && !(ctx->flags & (MMI_FLAG_EXACT_PATH_ONLY |
+                                                                              MMI_FLAG_RECORD_SAMPLE_BINS)) &&
+                  (!want_derivative || ctx->gradient_source == MMI_GRADIENT_IMAGE);
+  if (!ok)
+    return fail(MMI_ERR_ARGUMENT,
+                "mmi_evaluate_static takes an affine moving transform, identity fixed transform, dense sampling "
+                "without masks and the moving gradient image");
+  // fixed samples on fixed pixel centres first (other geometries would build
+  // per-axis cell tables in the context)
+  int64_t off[3];
+  FastArgs fa;
+  if (!fixed_direct_offsets(ctx->virtual_g, ctx->fixed_g, off) || !setup_fast(ctx, &fa))
+    return fail(MMI_ERR_ARGUMENT,
+                "mmi_evaluate_static takes virtual samples on fixed pixel centres within the fast path's bounds");
+  const EvalBufs sb = scratch_bufs(base);
+  return evaluate_impl(ctx, want_derivative, value, derivative, n_valid, nullptr, nullptr, &sb);
 }
 
 }  // extern "C"
@@ -1933,7 +2114,8 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
     EvalArgs a;
     FastArgs f;
     bool fast = false;
-    if (int rc = prepare_eval(ctxs[i], want_derivative, a, f, fast)) return rc;
+    EvalBufs eb;
+    if (int rc = prepare_eval(ctxs[i], want_derivative, a, f, fast, nullptr, eb)) return rc;
     const int64_t P = (int64_t)ctxs[i]->mtf.params.size();
     // (masked contexts run their own masked kernels)
     const bool masked = f.fmask.px || f.mmask.px;
@@ -2005,7 +2187,7 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
       const int64_t P = (int64_t)c->mtf.params.size();
       std::memcpy(hs + o_fa + sizeof(FastArgs) * k, &fa[k], sizeof(FastArgs));
       std::memcpy(hs + o_ea + sizeof(EvalArgs) * k, &ea[k], sizeof(EvalArgs));
-      FinalizeArgs z = finalize_args(c);
+      FinalizeArgs z = finalize_args(c, ctx_bufs(c));
       z.lds = lds;
       std::memcpy(hs + o_fz + sizeof(FinalizeArgs) * k, &z, sizeof(z));
       BatchDeriv d;
