@@ -64,7 +64,7 @@ def parse():
     ap.add_argument("--serving", action="store_true", help="registration serving workload")
     ap.add_argument("--requests", type=int, default=32, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=40.0, help="serving: mean arrivals per second")
-    ap.add_argument("--inflight", type=int, default=1, help="serving: registrations in flight at most")
+    ap.add_argument("--inflight", type=int, default=3, help="serving: registrations in flight at most")
     ap.add_argument("--serving-requests", type=int, default=16,
                     help="requests of the serving summary added to the metric line (0: none)")
     return ap.parse_args()

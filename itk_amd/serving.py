@@ -97,7 +97,8 @@ class RegistrationResult:
 
 
 class _BatchingMetric:
-    """The device metric with its evaluations routed through the scheduler."""
+    """The device metric with its evaluations routed through the scheduler
+    (a job without a lane evaluates on its own context and stream)."""
 
     def __init__(self, metric, server, job):
         self._m = metric
@@ -107,14 +108,20 @@ class _BatchingMetric:
     def __getattr__(self, name):
         return getattr(self._m, name)
 
+    def _eval(self, want_derivative: bool):
+        if self._job.lane is None:
+            self._job.evaluations += 1
+            return self._m.GetValueAndDerivative() if want_derivative else (self._m.GetValue(), None)
+        return self._server._evaluate(self._job, want_derivative)
+
     def GetValueAndDerivative(self):
-        return self._server._evaluate(self._job, True)
+        return self._eval(True)
 
     def GetValue(self):
-        return self._server._evaluate(self._job, False)[0]
+        return self._eval(False)[0]
 
     def GetDerivative(self):
-        return self._server._evaluate(self._job, True)[1]
+        return self._eval(True)[1]
 
 
 class _Job:
@@ -222,8 +229,8 @@ class _Lane:
 
 
 class RegistrationServer:
-    def __init__(self, device: int = 0, hbm_budget_bytes: int = 64 << 30, max_inflight: int = 1,
-                 max_wait_s: float = 0.0005, graphs: bool = True, storage: str = "f32", lanes: int = 2,
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 64 << 30, max_inflight: int = 3,
+                 max_wait_s: float = 0.0005, graphs: bool = True, storage: str = "f32", lanes: int = 1,
                  chunk_steps: int = 5, wait_limit_s: float = 600.0):
         self.device = device
         # a registration whose evaluation is not served within this time fails
@@ -283,10 +290,11 @@ class RegistrationServer:
             job.result = RegistrationResult(job.req.rid, job.req.kind, params, job.req.arrival, job.admitted,
                                             self._now(), job.evaluations, err)
             self.results.append(job.result)
-            job.lane.running.discard(job)
+            if job.lane is not None:
+                job.lane.running.discard(job)
+                job.lane.cv.notify()
             self.running -= 1
             self.used -= job.req.device_bytes()
-            job.lane.cv.notify()
             self.main_cv.notify()
 
     def _now(self) -> float:
@@ -311,7 +319,10 @@ class RegistrationServer:
                 while queue and queue[0].arrival <= now and self.running < self.max_inflight and (
                         self.used + queue[0].device_bytes() <= self.budget or self.running == 0):
                     req = queue.pop(0)
-                    lane = min(lanes, key=lambda ln: len(ln.running))
+                    # affine levels share the lanes' fused RegularStep loops;
+                    # a B-spline registration (L-BFGS-B steps on the host between
+                    # evaluations) runs on its own stream, so no lane waits for it
+                    lane = min(lanes, key=lambda ln: len(ln.running)) if req.kind == "affine" else None
                     if self.pool:
                         metric = self.pool.pop()
                     else:
@@ -321,7 +332,8 @@ class RegistrationServer:
                     job.admitted = now
                     self.used += req.device_bytes()
                     self.running += 1
-                    lane.running.add(job)
+                    if lane is not None:
+                        lane.running.add(job)
                     self.peak_inflight = max(self.peak_inflight, self.running)
                     job.thread.start()
                 timeout = 0.05

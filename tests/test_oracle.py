@@ -320,3 +320,64 @@ def test_recursive_gaussian_approximates_gaussian(oracle_mod):
     gx = oracle_mod.gradient_image(img, geom, sigma=sig)[..., 0]
     refg = ndimage.gaussian_filter(img, sig, order=(0, 1), mode="nearest", truncate=8.0) * sig
     assert rel(gx, refg) <= 2e-2
+
+
+def test_bspline_node_layout_single_coefficient(oracle_mod):
+    """One coefficient (dimension 1, node (2, 3, 1)) set, all others zero: the
+    displacement is c * B3(u_x - 2) B3(u_y - 3) B3(u_z - 1) in dimension 1 only,
+    u the point's continuous index in the grid (itkBSplineTransform.hxx:508-590,
+    parameters dimension-major, nodes x fastest).  A permuted node layout or
+    dimension order moves the bump elsewhere."""
+    geom = ImageGeometry(size=(20, 18, 16), spacing=(1.0, 1.5, 2.0), origin=(-3.0, 1.0, 2.0))
+    tf = BSplineTransform.for_image(geom, [4, 3, 2])
+    G = tf.grid
+    node, dim, c = (2, 3, 1), 1, 0.37
+    p = np.zeros(tf.number_of_parameters)
+    p[dim * G.num_pixels + node[0] + G.size[0] * (node[1] + G.size[1] * node[2])] = c
+
+    def b3(u):
+        u = abs(u)
+        return (4 - 6 * u * u + 3 * u ** 3) / 6 if u < 1 else ((2 - u) ** 3 / 6 if u < 2 else 0.0)
+
+    tf.set_parameters(p)
+    rng = np.random.default_rng(12)
+    inside_support = 0
+    for _ in range(300):
+        x = geom.index_to_point(rng.uniform(0, np.array(geom.size) - 1))
+        u = (np.array(x) - np.array(G.origin)) / np.array(G.spacing)
+        w = b3(u[0] - node[0]) * b3(u[1] - node[1]) * b3(u[2] - node[2])
+        inside_support += w > 0
+        expect = np.zeros(3)
+        expect[dim] = c * w
+        assert np.allclose(np.array(oracle_mod.transform_point(tf, x)) - np.array(x), expect, atol=1e-13)
+    assert inside_support > 50
+
+
+def test_recursive_gaussian_borders_extend_the_edge(oracle_mod):
+    """Near the line ends the recursive filters (zero and first order, with the
+    border coefficients BN / BM of itkRecursiveGaussianImageFilter.hxx:56-300)
+    follow a Gaussian of the edge-extended line: within 1 % of scipy's
+    mode='nearest' in a 6-pixel band at both ends, where zero, reflected or
+    mirrored extension differ by 5-90 % (the first order from reflected or
+    zero extension, the zero order from zero extension)."""
+    n = 96
+    geom = ImageGeometry(size=(n, n))
+    rng = np.random.default_rng(13)
+    img = ndimage.gaussian_filter(rng.normal(size=(n, n)), 3.0)
+    sig = 2.5
+    band = (slice(20, -20), np.r_[0:6, n - 6:n])
+
+    def rel(a, b):
+        return np.linalg.norm(a[band] - b[band]) / np.linalg.norm(b[band])
+
+    gx = oracle_mod.gradient_image(img, geom, sigma=sig)[..., 0]
+    sm = oracle_mod.smooth(img, geom, sig)
+    for mode in ("nearest", "reflect", "constant"):
+        refg = ndimage.gaussian_filter(img, sig, order=(0, 1), mode=mode, truncate=8.0) * sig
+        ref0 = ndimage.gaussian_filter(img, sig, mode=mode, truncate=8.0)
+        if mode == "nearest":
+            assert rel(gx, refg) <= 1e-2 and rel(sm, ref0) <= 1e-2
+        else:
+            assert rel(gx, refg) >= 0.05
+            if mode == "constant":
+                assert rel(sm, ref0) >= 0.05
