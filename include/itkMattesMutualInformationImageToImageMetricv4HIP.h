@@ -10,8 +10,10 @@
  * valid-point count and the reference's exception messages (library statuses
  * become itk::ExceptionObject).
  *
- * What the device takes: fixed and moving images of float or double pixels
- * (double pixels must be exact in float: the device stores FP32), dense
+ * What the device takes: fixed and moving images of float, double or integer
+ * pixels (double and 32-bit integer pixels exact in float: the device stores
+ * FP32; interpolated values assigned to the pixel type as the reference does:
+ * rounded to float, kept as double, truncated for integers), dense
  * sampling of the virtual domain or a virtual point set, ImageMaskSpatialObject
  * masks, an identity or affine fixed transform, and a moving transform that is
  * an AffineTransform or a cubic BSplineTransform, alone or as the one entry of

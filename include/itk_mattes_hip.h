@@ -30,7 +30,18 @@ extern "C" {
 #define MMI_ERR_DEGENERATE -3 /* the reference would throw (constant image, no overlap, ...) */
 #define MMI_ERR_MEMORY -4     /* device memory exhausted */
 
-enum mmi_pixel_type { MMI_F32 = 0, MMI_F64 = 1, MMI_BF16 = 2, MMI_F16 = 3 };
+enum mmi_pixel_type {
+  MMI_F32 = 0,
+  MMI_F64 = 1,
+  MMI_BF16 = 2, /* device storage only (mmi_set_storage) */
+  MMI_F16 = 3,  /* device storage only (mmi_set_storage) */
+  MMI_U8 = 4,
+  MMI_S8 = 5,
+  MMI_U16 = 6,
+  MMI_S16 = 7,
+  MMI_U32 = 8,
+  MMI_S32 = 9
+};
 enum mmi_transform_kind { MMI_TF_IDENTITY = 0, MMI_TF_AFFINE = 1, MMI_TF_BSPLINE = 2 };
 enum mmi_gradient_source { MMI_GRADIENT_CENTRAL_DIFFERENCE = 0, MMI_GRADIENT_IMAGE = 1 };
 
@@ -71,8 +82,16 @@ int mmi_context_destroy(mmi_context* ctx);
 
 /* Images: ImageToImageMetricv4::SetFixedImage / SetMovingImage
  * (itkImageToImageMetricv4.h:362-390).  pixels are host memory in buffer
- * order; pixel_type MMI_F32 or MMI_F64 (converted to the device storage type
- * chosen by mmi_set_storage). */
+ * order; pixel_type is the reference image's pixel type (MMI_F32, MMI_F64 or
+ * one of the integer types MMI_U8 ... MMI_S32).  A sample's interpolated value
+ * takes the reference's assignment to that pixel type
+ * (itkImageToImageMetricv4.hxx:318,369): rounded to float for MMI_F32, kept
+ * for MMI_F64, truncated toward zero for the integer types.  The device stores
+ * FP32 (or the storage of mmi_set_storage), so MMI_F64 and 32-bit integer
+ * pixels must be exact in float (else MMI_ERR_ARGUMENT: no silent rounding);
+ * images of other pixel types than MMI_F32 take no
+ * smoothing (mmi_set_smoothing_sigmas: the reference keeps the smoothed image
+ * in the pixel type), and integer pixels evaluate on the FP64 path. */
 int mmi_set_fixed_image(mmi_context* ctx, const mmi_geometry* g, int32_t pixel_type, const void* pixels);
 int mmi_set_moving_image(mmi_context* ctx, const mmi_geometry* g, int32_t pixel_type, const void* pixels);
 

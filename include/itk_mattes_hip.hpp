@@ -46,6 +46,30 @@ template <>
 struct pixel_code<double> {
   static constexpr int32_t value = MMI_F64;
 };
+template <>
+struct pixel_code<uint8_t> {
+  static constexpr int32_t value = MMI_U8;
+};
+template <>
+struct pixel_code<int8_t> {
+  static constexpr int32_t value = MMI_S8;
+};
+template <>
+struct pixel_code<uint16_t> {
+  static constexpr int32_t value = MMI_U16;
+};
+template <>
+struct pixel_code<int16_t> {
+  static constexpr int32_t value = MMI_S16;
+};
+template <>
+struct pixel_code<uint32_t> {
+  static constexpr int32_t value = MMI_U32;
+};
+template <>
+struct pixel_code<int32_t> {
+  static constexpr int32_t value = MMI_S32;
+};
 
 // Geometry of an image in ITK's conventions from row-major D x D matrices
 // (direction, ITK's index-to-physical and physical-to-index matrices as

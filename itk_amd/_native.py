@@ -21,6 +21,7 @@ MMI_ERR_DEGENERATE = -3
 MMI_ERR_MEMORY = -4
 
 MMI_F32, MMI_F64, MMI_BF16, MMI_F16 = 0, 1, 2, 3
+MMI_U8, MMI_S8, MMI_U16, MMI_S16, MMI_U32, MMI_S32 = 4, 5, 6, 7, 8, 9
 MMI_TF_IDENTITY, MMI_TF_AFFINE, MMI_TF_BSPLINE = 0, 1, 2
 MMI_GRADIENT_CENTRAL_DIFFERENCE, MMI_GRADIENT_IMAGE = 0, 1
 MMI_FLAG_RECORD_SAMPLE_BINS = 1

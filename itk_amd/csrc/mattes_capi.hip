@@ -305,6 +305,7 @@ struct mmi_context {
   int rows_walk = 16;                        // affine fast path: rows per walk (0: sample-strided kernels)
   int default_blocks_per_cu = 4, default_rows_walk = 16;  // the library's choice (environment overrides)
   int hist_copies = 0;                       // pass 1's LDS histogram copies (0: by size)
+  int32_t fixed_kind = kPixelFloat, moving_kind = kPixelFloat;  // reference pixel kinds of the uploads
   int max_lds_per_block = 64 * 1024;         // the device's LDS bytes per workgroup
   int grid = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -580,23 +581,47 @@ int build_transform(mmi_context* ctx, int D, HostTransform& h, DevTransform& d, 
   return fail(MMI_ERR_ARGUMENT, "unknown transform kind");
 }
 
-int upload_image(mmi_context* ctx, DevBuf& buf, const mmi_geometry* g, int32_t pixel_type, const void* pixels) {
+// Host pixels of the reference's pixel type as the device's FP32 image;
+// *kind receives the pixel kind.  Double and 32-bit integer pixels must be
+// exact in float (the FP32 image then holds every pixel exactly).
+template <typename T>
+int to_float_exact(const void* pixels, int64_t n, std::vector<float>& out, const char* what) {
+  const T* src = static_cast<const T*>(pixels);
+  out.resize((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    const float f = (float)src[i];
+    if (!((T)f == src[i]))
+      return fail(MMI_ERR_ARGUMENT, std::string(what) + " pixel " + std::to_string(i) +
+                                        " is not exact in the device's FP32 storage");
+    out[(size_t)i] = f;
+  }
+  return MMI_OK;
+}
+
+int upload_image(mmi_context* ctx, DevBuf& buf, const mmi_geometry* g, int32_t pixel_type, const void* pixels,
+                 int32_t* kind) {
   if (int rc = check_geometry(g)) return rc;
   if (!pixels) return fail(MMI_ERR_ARGUMENT, "pixels is NULL");
   const int64_t n = num_pixels(*g);
-  MMI_HIP(buf.ensure(sizeof(float) * n));
-  if (pixel_type == MMI_F32) {
-    MMI_HIP(hipMemcpyAsync(buf.p, pixels, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
-    MMI_HIP(hipStreamSynchronize(ctx->stream));
-  } else if (pixel_type == MMI_F64) {
-    std::vector<float> tmp(n);
-    const double* src = static_cast<const double*>(pixels);
-    for (int64_t i = 0; i < n; ++i) tmp[i] = (float)src[i];
-    MMI_HIP(hipMemcpyAsync(buf.p, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
-    MMI_HIP(hipStreamSynchronize(ctx->stream));
-  } else {
-    return fail(MMI_ERR_ARGUMENT, "pixel_type must be MMI_F32 or MMI_F64");
+  std::vector<float> tmp;
+  int rc = MMI_OK;
+  *kind = kPixelInteger;
+  switch (pixel_type) {
+    case MMI_F32: *kind = kPixelFloat; break;
+    case MMI_F64: *kind = kPixelDouble; rc = to_float_exact<double>(pixels, n, tmp, "double"); break;
+    case MMI_U8: rc = to_float_exact<uint8_t>(pixels, n, tmp, "uint8"); break;
+    case MMI_S8: rc = to_float_exact<int8_t>(pixels, n, tmp, "int8"); break;
+    case MMI_U16: rc = to_float_exact<uint16_t>(pixels, n, tmp, "uint16"); break;
+    case MMI_S16: rc = to_float_exact<int16_t>(pixels, n, tmp, "int16"); break;
+    case MMI_U32: rc = to_float_exact<uint32_t>(pixels, n, tmp, "uint32"); break;
+    case MMI_S32: rc = to_float_exact<int32_t>(pixels, n, tmp, "int32"); break;
+    default: return fail(MMI_ERR_ARGUMENT, "pixel_type must be MMI_F32, MMI_F64 or an integer type");
   }
+  if (rc) return rc;
+  MMI_HIP(buf.ensure(sizeof(float) * n));
+  MMI_HIP(hipMemcpyAsync(buf.p, pixel_type == MMI_F32 ? pixels : tmp.data(), sizeof(float) * n,
+                         hipMemcpyHostToDevice, ctx->stream));
+  MMI_HIP(hipStreamSynchronize(ctx->stream));
   return MMI_OK;
 }
 
@@ -873,6 +898,8 @@ void setup_rows(const mmi_context* ctx, FastArgs* F) {
 
 bool setup_fast(mmi_context* ctx, FastArgs* F) {
   std::memset(F, 0, sizeof(*F));
+  // integer pixels truncate the interpolated value: the FP64 path
+  if (ctx->fixed_kind == kPixelInteger || ctx->moving_kind == kPixelInteger) return false;
   const int D = ctx->fixed_g.dim;
   const int B = ctx->bins;
   if (num_pixels(ctx->virtual_g) >= (1ll << 31)) return false;
@@ -1253,7 +1280,7 @@ int mmi_set_storage(mmi_context* ctx, int32_t pixel_type) {
 int mmi_set_fixed_image(mmi_context* ctx, const mmi_geometry* g, int32_t pixel_type, const void* pixels) {
   if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
   MMI_HIP(hipSetDevice(ctx->device));
-  if (int rc = upload_image(ctx, ctx->fixed_px, g, pixel_type, pixels)) return rc;
+  if (int rc = upload_image(ctx, ctx->fixed_px, g, pixel_type, pixels, &ctx->fixed_kind)) return rc;
   ctx->fixed_g = *g;
   ctx->have_fixed = true;
   ctx->initialized = false;
@@ -1263,7 +1290,7 @@ int mmi_set_fixed_image(mmi_context* ctx, const mmi_geometry* g, int32_t pixel_t
 int mmi_set_moving_image(mmi_context* ctx, const mmi_geometry* g, int32_t pixel_type, const void* pixels) {
   if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
   MMI_HIP(hipSetDevice(ctx->device));
-  if (int rc = upload_image(ctx, ctx->moving_px, g, pixel_type, pixels)) return rc;
+  if (int rc = upload_image(ctx, ctx->moving_px, g, pixel_type, pixels, &ctx->moving_kind)) return rc;
   ctx->moving_g = *g;
   ctx->have_moving = true;
   ctx->have_texels = false;
@@ -1487,6 +1514,12 @@ int mmi_initialize(mmi_context* ctx) {
   // (ImageRegistrationMethodv4::InitializeRegistrationAtEachLevel,
   // itkImageRegistrationMethodv4.hxx:591-637).
   const int D = ctx->fixed_g.dim;
+  // the reference keeps a smoothed image in its pixel type: only float images
+  // are smoothed here (their device copy is that type)
+  if ((ctx->fixed_sigma[0] > 0.0 && ctx->fixed_kind != kPixelFloat) ||
+      (ctx->moving_sigma[0] > 0.0 && ctx->moving_kind != kPixelFloat))
+    return fail(MMI_ERR_ARGUMENT, "smoothing takes float images only (other pixel types have no device copy of "
+                                  "their smoothed image)");
   ctx->fixed_smoothed = ctx->moving_smoothed = false;
   if (ctx->fixed_sigma[0] > 0.0) {
     if (int rc = smooth_image(ctx, ctx->fixed_px.as<float>(), ctx->fixed_g, ctx->fixed_sigma, ctx->fixed_work))
@@ -1682,6 +1715,8 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
   a.moving_px = moving_pixels(ctx);
   a.texels = texel_pixels(ctx);
   a.storage = ctx->active_storage;
+  a.fixed_kind = ctx->fixed_kind;
+  a.moving_kind = ctx->moving_kind;
   a.n = n;
   a.points = ctx->n_points > 0 ? ctx->points.as<double>() : nullptr;
   a.point_ids = ctx->n_points > 0 ? ctx->point_ids.as<uint32_t>() : nullptr;

@@ -44,6 +44,13 @@ __device__ __forceinline__ void virtual_point(const EvalArgs& a, uint32_t s, dou
   }
 }
 
+// The interpolator's double assigned to the image's pixel type
+// (itkImageToImageMetricv4.hxx:318,369): a float rounds, a double keeps it, an
+// integer type truncates toward zero.
+__device__ __forceinline__ double pixel_value(int32_t kind, double v) {
+  return kind == kPixelDouble ? v : (kind == kPixelInteger ? trunc(v) : (double)(float)v);
+}
+
 // ProcessVirtualPoint + Mattes ProcessPoint up to the bins
 // (itkImageToImageMetricv4GetValueAndDerivativeThreaderBase.hxx:174-273,
 // itkImageToImageMetricv4.hxx:289-373,
@@ -61,7 +68,7 @@ __device__ __forceinline__ bool eval_sample(const EvalArgs& a, const double* vp,
   point_to_cindex<D>(a.fixed, fp, ci);
   if (!inside_buffer<D>(a.fixed, ci)) return false;
   const Cell fc = make_cell<D>(a.fixed, ci);
-  const double fval = (double)(float)interp_exact<D, P>(static_cast<const typename P::pixel*>(a.fixed_px), fc);
+  const double fval = pixel_value(a.fixed_kind, interp_exact<D, P>(static_cast<const typename P::pixel*>(a.fixed_px), fc));
 
   transform_point<D, MK>(a.mtf, o.vp, o.mp, &o.sup);
   // the moving mask at the mapped moving point (itkImageToImageMetricv4.hxx:351-362)
@@ -69,7 +76,8 @@ __device__ __forceinline__ bool eval_sample(const EvalArgs& a, const double* vp,
   point_to_cindex<D>(a.moving, o.mp, ci);
   if (!inside_buffer<D>(a.moving, ci)) return false;
   o.cell = make_cell<D>(a.moving, ci);
-  const double mval = (double)(float)interp_exact<D, P>(static_cast<const typename P::pixel*>(a.moving_px), o.cell);
+  const double mval =
+      pixel_value(a.moving_kind, interp_exact<D, P>(static_cast<const typename P::pixel*>(a.moving_px), o.cell));
 
   if (mval < a.moving_min) return false;
   if (mval > a.moving_max) return false;

@@ -13,6 +13,12 @@ constexpr int kStorageF32 = 0;
 constexpr int kStorageBF16 = 2;
 constexpr int kStorageF16 = 3;
 
+// Pixel kinds of the reference's images: the interpolated value assigned to a
+// float (rounded), a double (kept) or an integer pixel (truncated).
+constexpr int32_t kPixelFloat = 0;
+constexpr int32_t kPixelDouble = 1;
+constexpr int32_t kPixelInteger = 2;
+
 struct DevResult {
   unsigned long long n_valid;
   int32_t status;  // 0 ok, 1 no valid sample, 2 joint PDF summed to zero
@@ -167,6 +173,9 @@ struct EvalArgs {
   const DevMask* moving_mask;  // NULL: no mask
   int32_t storage;        // MMI_F32, MMI_BF16 or MMI_F16
   int32_t pad2_;
+  // the reference images' pixel kinds (kPixelFloat / Double / Integer): how a
+  // sample's interpolated value is assigned to the pixel type
+  int32_t fixed_kind, moving_kind;
   int32_t bins;
   int32_t hist_in_lds;
   int32_t gradient_source;

@@ -25,6 +25,11 @@ def _dptr(a):
     return a.ctypes.data_as(_dp)
 
 
+# numpy pixel dtypes and the C ABI's pixel type codes
+_PIXEL_CODES = {np.float32: N.MMI_F32, np.float64: N.MMI_F64, np.uint8: N.MMI_U8, np.int8: N.MMI_S8,
+                np.uint16: N.MMI_U16, np.int16: N.MMI_S16, np.uint32: N.MMI_U32, np.int32: N.MMI_S32}
+
+
 def transform_fixed_parameters(tf) -> np.ndarray:
     if tf.kind == TF_AFFINE:
         return np.asarray(tf.center, dtype=np.float64)
@@ -293,10 +298,13 @@ class MattesMutualInformationImageToImageMetricv4:
 
     @staticmethod
     def _as_pixels(pixels, geometry):
+        """The pixels in a dtype of the C ABI: float32, float64 or an integer
+        type (integer images keep the reference's truncation of interpolated
+        values); other dtypes are taken as float32 images."""
         a = np.ascontiguousarray(pixels)
         if a.shape != geometry.shape:
             raise ValueError("pixel array shape %s does not match geometry %s" % (a.shape, geometry.shape))
-        if a.dtype not in (np.float32, np.float64):
+        if a.dtype.type not in _PIXEL_CODES:
             a = a.astype(np.float32)
         return a
 
@@ -313,8 +321,7 @@ class MattesMutualInformationImageToImageMetricv4:
         if self._images_dirty:
             for setter, (px, g) in ((L.mmi_set_fixed_image, self._fixed), (L.mmi_set_moving_image, self._moving)):
                 cg = g.to_c()
-                N.check(setter(ctx, ctypes.byref(cg), N.MMI_F64 if px.dtype == np.float64 else N.MMI_F32,
-                               px.ctypes.data))
+                N.check(setter(ctx, ctypes.byref(cg), _PIXEL_CODES[px.dtype.type], px.ctypes.data))
             self._images_dirty = False
             self._gradient_dirty = True
         N.check(L.mmi_set_gradient_source(

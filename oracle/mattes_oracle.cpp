@@ -252,7 +252,11 @@ double interp_scalar(const orc_geometry* g, const double* ci, Fetch fetch) {
   return lerp(vxx0, vxx1, d[2]);
 }
 
-inline double round_pixel(double v, int f64) { return f64 ? v : (double)(float)v; }
+// The interpolator's double assigned to the pixel type (ImgMetric:318,369):
+// float rounds, double keeps, an integral type truncates toward zero.
+inline double round_pixel(double v, int f64, int integer) {
+  return integer ? std::trunc(v) : (f64 ? v : (double)(float)v);
+}
 
 // B-spline transform (BSplTf:478-590, weights BSplW:43-94).
 struct BSplineEval {
@@ -445,7 +449,7 @@ void process_sample(const Problem& pb, const double* vp, int64_t lin, ThreadStat
   point_to_cindex(pb.fixed.g, fp, ci);
   if (!inside_buffer(pb.fixed.g, ci)) return;
   auto ffetch = [&](int64_t x, int64_t y, int64_t z) { return pb.fixed.at(x, y, z); };
-  const double fval = round_pixel(interp_scalar(pb.fixed.g, ci, ffetch), pb.fixed.f64);
+  const double fval = round_pixel(interp_scalar(pb.fixed.g, ci, ffetch), pb.fixed.f64, pb.opt->pixel_integer);
 
   // Moving side.
   double mp[3];
@@ -455,7 +459,7 @@ void process_sample(const Problem& pb, const double* vp, int64_t lin, ThreadStat
   point_to_cindex(pb.moving.g, mp, ci);
   if (!inside_buffer(pb.moving.g, ci)) return;
   auto mfetch = [&](int64_t x, int64_t y, int64_t z) { return pb.moving.at(x, y, z); };
-  const double mval = round_pixel(interp_scalar(pb.moving.g, ci, mfetch), pb.moving.f64);
+  const double mval = round_pixel(interp_scalar(pb.moving.g, ci, mfetch), pb.moving.f64, pb.opt->pixel_integer);
   if (pb.res->moving_value) pb.res->moving_value[lin] = mval;
 
   double grad[3] = {0, 0, 0};

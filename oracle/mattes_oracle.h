@@ -50,7 +50,7 @@ typedef struct {
   int32_t gradient_source;  /* 0: central differences of the moving image, 1: gradient image */
   int32_t compute_derivative;
   int32_t threads;          /* slabs along the slowest virtual axis */
-  int32_t pad_;
+  int32_t pixel_integer;    /* 1: the images' pixel type is integral (interpolated values truncated) */
   double fixed_min, fixed_max, moving_min, moving_max;
 } orc_options;
 

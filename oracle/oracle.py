@@ -39,7 +39,7 @@ class COptions(ctypes.Structure):
         ("gradient_source", ctypes.c_int32),
         ("compute_derivative", ctypes.c_int32),
         ("threads", ctypes.c_int32),
-        ("pad_", ctypes.c_int32),
+        ("pixel_integer", ctypes.c_int32),
         ("fixed_min", ctypes.c_double),
         ("fixed_max", ctypes.c_double),
         ("moving_min", ctypes.c_double),
@@ -270,7 +270,8 @@ def bin_parameters(tmin, tmax, bins):
 def mattes(fixed: np.ndarray, fixed_geom: ImageGeometry, moving: np.ndarray, moving_geom: ImageGeometry,
            moving_tf, *, bins: int = 50, virtual_geom: ImageGeometry = None, fixed_tf=None,
            moving_gradient: np.ndarray = None, derivative: bool = True, threads: int = 1,
-           want_samples: bool = False, ranges=None, points=None, fixed_mask=None, moving_mask=None) -> dict:
+           want_samples: bool = False, ranges=None, points=None, fixed_mask=None, moving_mask=None,
+           integer_pixels: bool = False) -> dict:
     """Mattes MI (value, derivative) exactly as the reference computes it.
 
     ``fixed``/``moving`` are float32 or float64 numpy arrays in buffer order
@@ -279,7 +280,9 @@ def mattes(fixed: np.ndarray, fixed_geom: ImageGeometry, moving: np.ndarray, mov
     differences.  ``points`` (n x D, virtual space) replaces the dense virtual
     domain by the point set; ``fixed_mask`` / ``moving_mask`` are (uint8
     pixels, ImageGeometry).  Without ``ranges`` the intensity ranges are taken
-    as MattesMI::Initialize takes them for these settings.
+    as MattesMI::Initialize takes them for these settings.  ``integer_pixels``:
+    the reference images' pixel type is integral (pass the pixel values as
+    float64), so each interpolated value is truncated toward zero.
     """
     L = lib()
     if fixed.dtype != moving.dtype or fixed.dtype not in (np.float32, np.float64):
@@ -297,6 +300,7 @@ def mattes(fixed: np.ndarray, fixed_geom: ImageGeometry, moving: np.ndarray, mov
     opt.gradient_source = 1 if moving_gradient is not None else 0
     opt.compute_derivative = 1 if derivative else 0
     opt.threads = threads
+    opt.pixel_integer = 1 if integer_pixels else 0
     pts = None if points is None else np.ascontiguousarray(points, dtype=np.float64).reshape(-1, fixed_geom.dim)
     if ranges is None:
         if pts is None and fixed_mask is None and moving_mask is None:
