@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/itk_mattes_hip.h"
+#include "mattes_affine.h"
 #include "mattes_kernels.h"
 
 using namespace mmi;
@@ -523,16 +524,9 @@ int build_transform(mmi_context* ctx, int D, HostTransform& h, DevTransform& d, 
   if (h.kind == MMI_TF_AFFINE) {
     if ((int)h.params.size() != D * D + D) return fail(MMI_ERR_ARGUMENT, "affine transform needs D*D+D parameters");
     if ((int)h.fixed.size() != D) return fail(MMI_ERR_ARGUMENT, "affine transform needs D fixed parameters (centre)");
-    // MatrixOffsetTransformBase::SetParameters + ComputeOffset (MOTB:518-556, 615-631)
-    for (int i = 0; i < D * D; ++i) d.m[i] = h.params[i];
-    for (int i = 0; i < D; ++i) {
-      const double t = h.params[D * D + i];
-      const double c = h.fixed[i];
-      double off = t + c;
-      for (int j = 0; j < D; ++j) off -= d.m[i * D + j] * h.fixed[j];
-      d.off[i] = off;
-      d.center[i] = c;
-    }
+    // MatrixOffsetTransformBase::SetParameters + ComputeOffset (the device
+    // optimizer loop uses the same function)
+    affine_from_parameters(D, h.params.data(), h.fixed.data(), d);
     return MMI_OK;
   }
   if (h.kind == MMI_TF_BSPLINE) {
@@ -778,79 +772,52 @@ bool setup_fast_bspline(mmi_context* ctx, FastArgs* F) {
   return true;
 }
 
-// Parameters and error bounds of the certified fast path (FastArgs in
-// mattes_kernels.h; buffers are filled in by the caller).  Returns false when
-// the path does not apply.
-// Continuous index in geometry g of the point T(o_v + I2P_v idx), T(x) = M x +
-// off, as one affine map of the virtual index: ci = A idx + b (folded in long
-// double), with a bound e on |FP64 A idx + b - the reference's chain of
-// roundings| (index -> point -> transform -> continuous index): 64 ulp of the
-// largest intermediate magnitude over the domain's corners.
-void affine_index_map(const mmi_geometry& v, const double* M, const double* off, const mmi_geometry& g, double* Aout,
-                      double* bout, double* e) {
-  const int D = v.dim;
-  long double MI[9] = {0}, A[9] = {0}, bb[3] = {0}, w[3] = {0};
-  for (int i = 0; i < D; ++i)
-    for (int j = 0; j < D; ++j)
-      for (int k = 0; k < D; ++k) MI[i * D + j] += (long double)M[i * D + k] * v.index_to_physical[k * D + j];
-  for (int i = 0; i < D; ++i)
-    for (int j = 0; j < D; ++j)
-      for (int k = 0; k < D; ++k) A[i * D + j] += (long double)g.physical_to_index[i * D + k] * MI[k * D + j];
-  for (int i = 0; i < D; ++i) {
-    long double s = (long double)off[i] - g.origin[i];
-    for (int k = 0; k < D; ++k) s += (long double)M[i * D + k] * v.origin[k];
-    w[i] = s;
+// Continuous index in geometry g of the virtual index through T(x) = M x +
+// off, with its bound (affine_index_map in mattes_affine.h).
+void affine_index_map_geom(const mmi_geometry& v, const double* M, const double* off, const mmi_geometry& g,
+                           double* A, double* b, double* e) {
+  int32_t vs[3] = {0, 0, 0}, vz[3] = {1, 1, 1};
+  for (int k = 0; k < v.dim; ++k) {
+    vs[k] = (int32_t)v.start[k];
+    vz[k] = (int32_t)v.size[k];
   }
-  for (int i = 0; i < D; ++i)
-    for (int k = 0; k < D; ++k) bb[i] += (long double)g.physical_to_index[i * D + k] * w[k];
-  for (int i = 0; i < D * D; ++i) Aout[i] = (double)A[i];
-  for (int i = 0; i < D; ++i) bout[i] = (double)bb[i];
-  // magnitude of every intermediate of the reference's chain over the domain
-  double S = 0.0;
-  for (int corner = 0; corner < (1 << D); ++corner) {
-    double idx[3], vp[3] = {0, 0, 0}, mp[3] = {0, 0, 0}, ci[3] = {0, 0, 0}, mag = 0.0;
-    for (int k = 0; k < D; ++k) idx[k] = (double)(v.start[k] + ((corner >> k) & 1 ? v.size[k] - 1 : 0));
-    for (int i = 0; i < D; ++i) {
-      double s = std::fabs(v.origin[i]);
-      vp[i] = v.origin[i];
-      for (int j = 0; j < D; ++j) {
-        vp[i] += v.index_to_physical[i * D + j] * idx[j];
-        s += std::fabs(v.index_to_physical[i * D + j] * idx[j]);
-      }
-      mag = std::max(mag, s);
-    }
-    for (int i = 0; i < D; ++i) {
-      double s = std::fabs(off[i]);
-      mp[i] = off[i];
-      for (int j = 0; j < D; ++j) {
-        mp[i] += M[i * D + j] * vp[j];
-        s += std::fabs(M[i * D + j] * vp[j]);
-      }
-      mag = std::max(mag, s);
-    }
-    for (int i = 0; i < D; ++i) {
-      double s = 0.0;
-      for (int j = 0; j < D; ++j) {
-        ci[i] += g.physical_to_index[i * D + j] * (mp[j] - g.origin[j]);
-        s += std::fabs(g.physical_to_index[i * D + j]) * (std::fabs(mp[j]) + std::fabs(g.origin[j]));
-      }
-      mag = std::max(mag, s);
-    }
-    double rowA = 0.0;
-    for (int i = 0; i < D * D; ++i) rowA = std::max(rowA, std::fabs(Aout[i]));
-    S = std::max(S, mag * (1.0 + rowA));
-  }
-  *e = 64.0 * S * std::ldexp(1.0, -52) + 1e-300;
+  affine_index_map(v.dim, vs, vz, v.origin, v.index_to_physical, M, off, g.origin, g.physical_to_index, A, b, e);
 }
 
-// Row walks of the affine fast path (FastRows): items of 64 x by `walk` rows,
-// the moving continuous index stepped in 32.32 fixed point.  Widens F->e_ci by
-// the walk's error: the FP64 start is within e_ci of the reference's chain and
-// within 3 roundings of S (< e_ci / 8, e_ci being 64 ulp of S) of the exact
-// map, its conversion truncates below 2^-32, and each step's rounded increment
-// is within 2^-33 of the map's y column.
-void setup_rows(const mmi_context* ctx, FastArgs* F) {
-  FastRows& R = F->rows;
+// The level-constant inputs of the affine fast path's map and bounds.
+AffineMapInputs affine_inputs(const mmi_context* ctx, const FastRows& rows) {
+  AffineMapInputs in;
+  std::memset(&in, 0, sizeof(in));
+  const mmi_geometry& v = ctx->virtual_g;
+  const mmi_geometry& m = ctx->moving_g;
+  const int D = v.dim;
+  in.dim = D;
+  in.rows = rows.on;
+  in.walk = rows.walk;
+  in.bins = ctx->bins;
+  for (int k = 0; k < 3; ++k) {
+    in.v_start[k] = k < D ? (int32_t)v.start[k] : 0;
+    in.v_size[k] = k < D ? (int32_t)v.size[k] : 1;
+    in.m_start[k] = k < D ? (int32_t)m.start[k] : 0;
+    in.m_size[k] = k < D ? (int32_t)m.size[k] : 1;
+    in.v_origin[k] = v.origin[k];
+    in.m_origin[k] = m.origin[k];
+    in.center[k] = k < D && (int)ctx->mtf.fixed.size() > k ? ctx->mtf.fixed[k] : 0.0;
+  }
+  for (int i = 0; i < 9; ++i) {
+    in.v_i2p[i] = v.index_to_physical[i];
+    in.m_p2i[i] = m.physical_to_index[i];
+  }
+  in.Mm = std::max(std::fabs(ctx->moving_min), std::fabs(ctx->moving_max));
+  in.moving_bin_size = ctx->moving_bin_size;
+  in.moving_norm_min = ctx->moving_norm_min;
+  return in;
+}
+
+// Row walks of the affine fast path (FastRows): items of 64 x by `walk` rows;
+// the layout here, the fixed-point steps and thresholds with the map
+// (rows_thresholds in mattes_affine.h).  R.on stays 0 when they do not apply.
+void setup_rows_layout(const mmi_context* ctx, FastRows& R) {
   std::memset(&R, 0, sizeof(R));
   if (ctx->rows_walk <= 0) return;
   const int D = ctx->fixed_g.dim;
@@ -861,8 +828,6 @@ void setup_rows(const mmi_context* ctx, FastArgs* F) {
   if (msy >= (1 << 24) || msz >= (1 << 24)) return;
   for (int k = 0; k < D; ++k)
     if (m.size[k] >= (1 << 24) || std::fabs(m.start[k]) >= (1 << 24)) return;
-  for (int i = 0; i < D * D; ++i)
-    if (!(std::fabs(F->A[i]) < 1048576.0)) return;
   const int64_t nx = v.size[0], ny = v.size[1], nz = D == 3 ? v.size[2] : 1;
   const int64_t nxseg = (nx + 63) / 64;
   // walks long enough to amortise the FP64 start, short enough that every
@@ -878,21 +843,6 @@ void setup_rows(const mmi_context* ctx, FastArgs* F) {
   R.items = (uint32_t)items;
   R.dxseg = make_fastdiv((uint32_t)nxseg);
   R.dyblk = make_fastdiv((uint32_t)nyblk);
-  const double e = 1.125 * F->e_ci + (double)(walk + 2) * std::ldexp(1.0, -32);
-  const long double two32 = 4294967296.0L;
-  for (int k = 0; k < D; ++k) {
-    R.qy[k] = (long long)std::llround((long double)F->A[k * D + 1] * two32);
-    const long double lo = (long double)m.start[k] - 0.5L, hi = (long double)(m.start[k] + m.size[k] - 1) + 0.5L;
-    const long long lo_out = (long long)std::floor((lo - e) * two32) - 1;
-    const long long hi_out = (long long)std::ceil((hi + e) * two32) + 1;
-    const long long lo_in = (long long)std::ceil((lo + e) * two32) + 1;
-    const long long hi_in = (long long)std::floor((hi - e) * two32) - 1;
-    R.lo_out[k] = lo_out;
-    R.span_out[k] = (unsigned long long)(hi_out - lo_out);
-    R.lo_in[k] = lo_in;
-    R.span_in[k] = hi_in > lo_in ? (unsigned long long)(hi_in - lo_in) : 0ull;
-  }
-  F->e_ci = e;
   R.on = 1;
 }
 
@@ -921,7 +871,6 @@ bool setup_fast(mmi_context* ctx, FastArgs* F) {
   const DevTransform& t = ctx->dmtf;
   F->mk = bs ? 2 : 1;
   if (bs && !setup_fast_bspline(ctx, F)) return false;
-  if (!bs) affine_index_map(v, t.m, t.off, m, F->A, F->b, &F->e_ci);
   // image masks (affine): the masks' continuous indices are affine maps of the
   // virtual index as well, certified like the moving image's
   const mmi_geometry* mg[2] = {ctx->have_mask[0] ? &ctx->mask_g[0] : nullptr,
@@ -936,9 +885,9 @@ bool setup_fast(mmi_context* ctx, FastArgs* F) {
     // fixed mask at the fixed point (identity fixed transform), moving mask at
     // the mapped moving point
     if (k == 0)
-      affine_index_map(v, ident, zero, *mg[k], fm[k]->A, fm[k]->b, &fm[k]->e);
+      affine_index_map_geom(v, ident, zero, *mg[k], fm[k]->A, fm[k]->b, &fm[k]->e);
     else
-      affine_index_map(v, t.m, t.off, *mg[k], fm[k]->A, fm[k]->b, &fm[k]->e);
+      affine_index_map_geom(v, t.m, t.off, *mg[k], fm[k]->A, fm[k]->b, &fm[k]->e);
     if (fm[k]->e > 1e-6) return false;
     for (int i = 0; i < 3; ++i) {
       fm[k]->start[i] = i < D ? (int32_t)mg[k]->start[i] : 0;
@@ -948,21 +897,29 @@ bool setup_fast(mmi_context* ctx, FastArgs* F) {
     fm[k]->sz = (int32_t)(mg[k]->size[0] * (D > 1 ? mg[k]->size[1] : 1));
     fm[k]->px = ctx->mask_px[k].as<uint8_t>();
   }
-  if (F->e_ci > 1e-6) return false;
+  if (bs && F->e_ci > 1e-6) return false;
   // fixed samples on pixel centres (direct reads), else per-axis cell tables
   // and an FP32 interpolation with the same error bound as the moving value's
   int64_t off[3] = {0, 0, 0};
   const bool direct = fixed_direct_offsets(v, ctx->fixed_g, off);
   if (!direct && !fixed_axis_tables(ctx, v, ctx->fixed_g, F)) return false;
-  // row walks: their fixed-point steps widen e_ci (before the bounds below)
-  if (!bs && direct && !mg[0] && !mg[1]) setup_rows(ctx, F);
+  double ev = 0.0, bt_m = 0.0;
+  if (!bs) {
+    // the affine map, row walks (their steps widen e_ci) and moving bounds, as
+    // the device optimizer loop recomputes them after each step
+    if (direct && !mg[0] && !mg[1]) setup_rows_layout(ctx, F->rows);
+    DevTransform tt;
+    std::memset(&tt, 0, sizeof(tt));
+    if (!affine_fast_update(affine_inputs(ctx, F->rows), ctx->mtf.params.data(), tt, *F)) return false;
+    ev = F->ev;
+    bt_m = F->bt_m;
+  } else {
+    moving_value_bounds(std::max(std::fabs(ctx->moving_min), std::fabs(ctx->moving_max)), ctx->moving_bin_size,
+                        ctx->moving_norm_min, B, F->e_ci, &ev, &bt_m);
+  }
   // intensity bounds
-  const double Mm = std::max(std::fabs(ctx->moving_min), std::fabs(ctx->moving_max));
   const double Mf = std::max(std::fabs(ctx->fixed_min), std::fabs(ctx->fixed_max));
-  const double ev = Mm * std::ldexp(1.0, -19) + 12.0 * Mm * F->e_ci;
   const double u = std::ldexp(1.0, -22);
-  const double bt_m = 2.0 * ((ev + Mm * u) / ctx->moving_bin_size + u * (Mm / ctx->moving_bin_size +
-                                                                          std::fabs(ctx->moving_norm_min) + B));
   const double ef = direct ? 0.0 : Mf * std::ldexp(1.0, -19);
   const double bt_f = 2.0 * ((ef + Mf * u) / ctx->fixed_bin_size + u * (Mf / ctx->fixed_bin_size +
                                                                        std::fabs(ctx->fixed_norm_min) + B));
