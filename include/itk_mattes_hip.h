@@ -307,10 +307,15 @@ int mmi_batch_get_last_timing(mmi_batch* batch, float* device_ms, int32_t* n_fus
  * (Modules/Numerics/Optimizersv4/include/itkRegularStepGradientDescentOptimizerv4.hxx:25-171,
  * itkGradientDescentOptimizerv4.hxx:72-165): every iteration evaluates all
  * still-running contexts in one batched pass, then takes each one's
- * AdvanceOneStep on the host, in the reference's arithmetic (Kahan sums, the
- * relaxation on a direction change, the previous gradient re-multiplied by
- * 1/scales each step), so each context's iterates equal a one-at-a-time run
- * bit for bit.  For transforms of at most 12 parameters (affine).  The
+ * AdvanceOneStep in the reference's arithmetic (Kahan sums, the relaxation on
+ * a direction change, the previous gradient re-multiplied by 1/scales each
+ * step), so each context's iterates equal a one-at-a-time run bit for bit.
+ * Contexts of the fused launch (see mmi_batch_evaluate) iterate on the
+ * device: the step and the fast path's new map run in a kernel after each
+ * evaluation, in chunks of iterations replayed as one HIP graph, with no host
+ * round trip per iteration; the others, and a context whose next evaluation
+ * leaves the fast path, continue in the host loop with the same arithmetic.
+ * For transforms of at most 12 parameters (affine).  The
  * caller owns one state per context, started with mmi_regular_step_start;
  * each call runs at most max_steps iterations, so contexts can join and
  * leave between calls.  The final parameters stay in each context
@@ -354,6 +359,15 @@ int mmi_batch_regular_step(mmi_batch* batch, mmi_context* const* ctxs, int32_t n
                            const mmi_regular_step_options* options, mmi_regular_step_state* states, int32_t max_steps);
 /* The moving transform parameters a context holds (P doubles). */
 int mmi_get_moving_parameters(mmi_context* ctx, double* params, int64_t n_params);
+/* Where mmi_batch_regular_step iterates fused contexts: on the device
+ * (on_device != 0, the default) in chunks of `chunk` iterations (1..64,
+ * default 8; a chunk's iterations after every context stopped are empty
+ * launches), or in the host loop.  No counterpart in the reference (its
+ * optimizer loop is host code); results are the same either way. */
+int mmi_batch_set_step_loop(mmi_batch* batch, int32_t on_device, int32_t chunk);
+/* Iterations the last mmi_batch_regular_step ran in the device loop (summed
+ * over contexts). */
+int mmi_batch_get_step_count(mmi_batch* batch, int32_t* device_steps);
 
 #ifdef __cplusplus
 }

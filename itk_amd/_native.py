@@ -69,6 +69,8 @@ SIGNATURES = {
     "mmi_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     "mmi_batch_destroy": (ctypes.c_int, [_vp]),
     "mmi_batch_set_graphs": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mmi_batch_set_step_loop": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
+    "mmi_batch_get_step_count": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "mmi_batch_evaluate": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_int32, ctypes.c_int32, _dp,
                                           ctypes.POINTER(_dp), ctypes.POINTER(ctypes.c_int64),
                                           ctypes.POINTER(ctypes.c_int32)]),

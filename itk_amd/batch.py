@@ -91,6 +91,18 @@ class BatchEvaluator:
             N.check(self._L.mmi_get_moving_parameters(m._ctx, p.ctypes.data_as(N._dp), p.size))
             m._moving_tf.set_parameters(p)
 
+    def set_step_loop(self, on_device: bool = True, chunk: int = 8):
+        """Where regular_step iterates fused metrics: on the device in
+        captured chunks of `chunk` iterations (default), or in the host loop;
+        the iterates are the same bits either way."""
+        N.check(self._L.mmi_batch_set_step_loop(self._b, 1 if on_device else 0, int(chunk)))
+
+    def device_steps(self) -> int:
+        """Iterations the last regular_step call ran in the device loop."""
+        n = ctypes.c_int32()
+        N.check(self._L.mmi_batch_get_step_count(self._b, ctypes.byref(n)))
+        return n.value
+
     def last_timing(self) -> dict:
         ms = ctypes.c_float()
         nf = ctypes.c_int32()

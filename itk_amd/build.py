@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libitk_mattes_hip.so")
-SOURCES = ["mattes_kernels.hip", "mattes_fast.hip", "mattes_prep.hip", "mattes_capi.hip", "mattes_sampling.cpp"]
+SOURCES = ["mattes_kernels.hip", "mattes_fast.hip", "mattes_prep.hip", "mattes_capi.hip", "mattes_optim.hip",
+           "mattes_sampling.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

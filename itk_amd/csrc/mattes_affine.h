@@ -27,19 +27,31 @@ struct AffineMapInputs {
 
 __host__ __device__ inline double amax(double a, double b) { return a > b ? a : b; }
 
+// The functions below are templates on the dimension D, so that the device's
+// loops unroll and its arrays stay in registers; the runtime-D wrappers serve
+// the host.  The operations and their order do not depend on the unrolling.
+
 // MatrixOffsetTransformBase::SetParameters + ComputeOffset
 // (itkMatrixOffsetTransformBase.hxx:518-556, 615-631): matrix row-major, then
 // offset = translation + centre - matrix * centre, summed in that order.
-__host__ __device__ inline void affine_from_parameters(int D, const double* params, const double* center,
-                                                       DevTransform& t) {
+template <int D>
+__host__ __device__ inline void affine_from_parameters_d(const double* params, const double* center,
+                                                         DevTransform& t) {
   t.kind = 1;
   for (int i = 0; i < D * D; ++i) t.m[i] = params[i];
   for (int i = 0; i < D; ++i) {
     double off = params[D * D + i] + center[i];
-    for (int j = 0; j < D; ++j) off -= t.m[i * D + j] * center[j];
+    for (int j = 0; j < D; ++j) off -= params[i * D + j] * center[j];
     t.off[i] = off;
     t.center[i] = center[i];
   }
+}
+
+inline void affine_from_parameters(int D, const double* params, const double* center, DevTransform& t) {
+  if (D == 2)
+    affine_from_parameters_d<2>(params, center, t);
+  else
+    affine_from_parameters_d<3>(params, center, t);
 }
 
 // Continuous index in geometry (g_origin, g_p2i) of the point T(o_v + I2P_v
@@ -47,11 +59,12 @@ __host__ __device__ inline void affine_from_parameters(int D, const double* para
 // ci = A idx + b, with a bound e on |FP64 A idx + b - the reference's chain
 // of roundings| (index -> point -> transform -> continuous index): 64 ulp of
 // the largest intermediate magnitude over the domain's corners.
-__host__ __device__ inline void affine_index_map(int D, const int32_t* v_start, const int32_t* v_size,
-                                                 const double* v_origin, const double* v_i2p, const double* M,
-                                                 const double* off, const double* g_origin, const double* g_p2i,
-                                                 double* A, double* b, double* e) {
-  double MI[9] = {0}, w[3] = {0};
+template <int D>
+__host__ __device__ inline void affine_index_map_d(const int32_t* v_start, const int32_t* v_size,
+                                                   const double* v_origin, const double* v_i2p, const double* M,
+                                                   const double* off, const double* g_origin, const double* g_p2i,
+                                                   double* A, double* b, double* e) {
+  double MI[D * D], w[D];
   for (int i = 0; i < D; ++i)
     for (int j = 0; j < D; ++j) {
       double s = 0.0;
@@ -78,7 +91,7 @@ __host__ __device__ inline void affine_index_map(int D, const int32_t* v_start, 
   for (int i = 0; i < D * D; ++i) rowA = amax(rowA, fabs(A[i]));
   double S = 0.0;
   for (int corner = 0; corner < (1 << D); ++corner) {
-    double idx[3], vp[3] = {0, 0, 0}, mp[3] = {0, 0, 0}, mag = 0.0;
+    double idx[D], vp[D], mp[D], mag = 0.0;
     for (int k = 0; k < D; ++k) idx[k] = (double)(v_start[k] + ((corner >> k) & 1 ? v_size[k] - 1 : 0));
     for (int i = 0; i < D; ++i) {
       double s = fabs(v_origin[i]);
@@ -108,6 +121,15 @@ __host__ __device__ inline void affine_index_map(int D, const int32_t* v_start, 
   *e = 64.0 * S * 2.220446049250313e-16 + 1e-300;
 }
 
+inline void affine_index_map(int D, const int32_t* v_start, const int32_t* v_size, const double* v_origin,
+                             const double* v_i2p, const double* M, const double* off, const double* g_origin,
+                             const double* g_p2i, double* A, double* b, double* e) {
+  if (D == 2)
+    affine_index_map_d<2>(v_start, v_size, v_origin, v_i2p, M, off, g_origin, g_p2i, A, b, e);
+  else
+    affine_index_map_d<3>(v_start, v_size, v_origin, v_i2p, M, off, g_origin, g_p2i, A, b, e);
+}
+
 // The row walks' steps and buffer thresholds in 32.32 fixed point for the map
 // A (FastRows; walk, sizes and divisions set by the host), and e widened by
 // the walk's error: the FP64 start is within e of the reference's chain and
@@ -117,8 +139,9 @@ __host__ __device__ inline void affine_index_map(int D, const int32_t* v_start, 
 // certainly-outside bounds are start - 1/2 - e and end + 1/2 + e rounded
 // outward, the certainly-inside ones start - 1/2 + e and end + 1/2 - e
 // rounded inward, one more unit of margin each.
-__host__ __device__ inline void rows_thresholds(int D, const int32_t* m_start, const int32_t* m_size, int32_t walk,
-                                                const double* A, double* e, FastRows& R) {
+template <int D>
+__host__ __device__ inline void rows_thresholds_d(const int32_t* m_start, const int32_t* m_size, int32_t walk,
+                                                  const double* A, double* e, FastRows& R) {
   const double ew = 1.125 * *e + (double)(walk + 2) * 2.3283064365386963e-10;
   const long long ce = (long long)ceil(ew * 4294967296.0);
   const long long half = 1ll << 31;
@@ -148,18 +171,32 @@ __host__ __device__ inline void moving_value_bounds(double Mm, double moving_bin
 // New parameters of the affine moving transform: the device transform and the
 // fast path's map, error bound, row-walk thresholds and moving bounds.  false:
 // the certified path does not hold for these parameters (the bounds exceed
-// what it takes).
-__host__ __device__ inline bool affine_fast_update(const AffineMapInputs& in, const double* params, DevTransform& t,
-                                                   FastArgs& F) {
-  const int D = in.dim;
-  affine_from_parameters(D, params, in.center, t);
+// what it takes; t and F are then partly updated).
+template <int D>
+__host__ __device__ inline bool affine_fast_update_d(const AffineMapInputs& in, const double* params,
+                                                     DevTransform& t, FastArgs& F) {
+  double M[D * D], off[D], A[D * D], bb[D];
+  for (int i = 0; i < D * D; ++i) M[i] = params[i];
+  for (int i = 0; i < D; ++i) {
+    double o = params[D * D + i] + in.center[i];
+    for (int j = 0; j < D; ++j) o -= M[i * D + j] * in.center[j];
+    off[i] = o;
+  }
   double e = 0.0;
-  affine_index_map(D, in.v_start, in.v_size, in.v_origin, in.v_i2p, t.m, t.off, in.m_origin, in.m_p2i, F.A, F.b, &e);
+  affine_index_map_d<D>(in.v_start, in.v_size, in.v_origin, in.v_i2p, M, off, in.m_origin, in.m_p2i, A, bb, &e);
+  t.kind = 1;
+  for (int i = 0; i < D * D; ++i) t.m[i] = M[i];
+  for (int i = 0; i < D; ++i) {
+    t.off[i] = off[i];
+    t.center[i] = in.center[i];
+  }
+  for (int i = 0; i < D * D; ++i) F.A[i] = A[i];
+  for (int i = 0; i < D; ++i) F.b[i] = bb[i];
   if (in.rows) {
     // the walk's fixed-point steps stay far below 2^31 voxels
     for (int i = 0; i < D * D; ++i)
-      if (!(fabs(F.A[i]) < 1048576.0)) return false;
-    rows_thresholds(D, in.m_start, in.m_size, in.walk, F.A, &e, F.rows);
+      if (!(fabs(A[i]) < 1048576.0)) return false;
+    rows_thresholds_d<D>(in.m_start, in.m_size, in.walk, A, &e, F.rows);
   }
   F.e_ci = e;
   double ev = 0.0, bt_m = 0.0;
@@ -167,6 +204,10 @@ __host__ __device__ inline bool affine_fast_update(const AffineMapInputs& in, co
   F.ev = (float)ev;
   F.bt_m = (float)bt_m;
   return e <= 1e-6 && bt_m <= 0.05;
+}
+
+inline bool affine_fast_update(const AffineMapInputs& in, const double* params, DevTransform& t, FastArgs& F) {
+  return in.dim == 2 ? affine_fast_update_d<2>(in, params, t, F) : affine_fast_update_d<3>(in, params, t, F);
 }
 
 }  // namespace mmi

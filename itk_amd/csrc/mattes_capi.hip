@@ -16,6 +16,7 @@
 #include "../../include/itk_mattes_hip.h"
 #include "mattes_affine.h"
 #include "mattes_kernels.h"
+#include "mattes_optim.h"
 
 using namespace mmi;
 
@@ -2007,6 +2008,16 @@ struct mmi_batch {
   DevBuf d_stage, d_out;
   float last_ms = 0.0f;
   int32_t last_fused = 0;
+  // the RegularStep loop on the device: chunks of `chunk` iterations, captured
+  // once per launch shape and buffer set
+  bool step_on_device = true;
+  int32_t chunk = 8;
+  hipGraph_t step_graph = nullptr;
+  hipGraphExec_t step_exec = nullptr;
+  std::vector<int64_t> step_sig;
+  int32_t last_device_steps = 0;  // evaluations the last mmi_batch_regular_step ran in the device loop
+  int32_t* h_flags = nullptr;     // pinned: the device loop's per-member halted flags
+  size_t h_flags_bytes = 0;
 };
 
 namespace {
@@ -2025,7 +2036,7 @@ int ensure_pinned(void** p, size_t* have, size_t need) {
   return MMI_OK;
 }
 
-void drop_graph(mmi_batch* b) {
+void drop_eval_graph(mmi_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
@@ -2033,11 +2044,127 @@ void drop_graph(mmi_batch* b) {
   b->sig.clear();
 }
 
+void drop_step_graph(mmi_batch* b) {
+  if (b->step_exec) (void)hipGraphExecDestroy(b->step_exec);
+  if (b->step_graph) (void)hipGraphDestroy(b->step_graph);
+  b->step_exec = nullptr;
+  b->step_graph = nullptr;
+  b->step_sig.clear();
+}
+
+void drop_graph(mmi_batch* b) {
+  drop_eval_graph(b);
+  drop_step_graph(b);
+}
+
 const char* degenerate_message(int status) {
   return status == 1 ? "All samples map outside moving image buffer. The images do not sufficiently overlap. They need "
                        "to be initialized to have more overlap before this metric will work. For instance, you can "
                        "align the image centers by translation."
                      : "Joint PDF summed to zero";
+}
+
+// Whether a context goes into a batch's fused launch (the certified affine
+// fast path, no image masks, the gradient image, at most 12 parameters and,
+// with the derivative, pass 1's byte per sample, as it has alone); its
+// arguments in a and f.
+int fused_candidate(mmi_context* c, int32_t want_derivative, EvalArgs& a, FastArgs& f, bool& ok) {
+  bool fast = false;
+  EvalBufs eb;
+  if (int rc = prepare_eval(c, want_derivative, a, f, fast, nullptr, eb)) return rc;
+  const int64_t P = (int64_t)c->mtf.params.size();
+  const bool masked = f.fmask.px || f.mmask.px;
+  const bool image_gradient = c->gradient_source == MMI_GRADIENT_IMAGE;
+  const bool cls_ok = !want_derivative ||
+                      (c->bins <= 253 && c->texel_values_level && !(c->flags & MMI_FLAG_PASS2_RECLASSIFY));
+  ok = fast && f.mk == 1 && !masked && image_gradient && P <= 12 && cls_ok;
+  return MMI_OK;
+}
+
+// Offsets of the fused launch's tables in the batch's staging buffers.
+struct StageLayout {
+  size_t o_start, o_fa, o_ea, o_fz, o_dv, o_it, o_extra, bytes;
+};
+
+// Stages the fused launch of the members ctxs[fused[k]] (arguments ea[k],
+// fa[k]) in the batch's pinned buffer, sized for `extra` more bytes at
+// S.o_extra, and fills L (no gather: L.out NULL, L.halted NULL).  The members'
+// own streams are drained first (their uploads precede the batch's reads).
+int stage_fused(mmi_batch* b, mmi_context* const* ctxs, const std::vector<int>& fused, std::vector<EvalArgs>& ea,
+                std::vector<FastArgs>& fa, int32_t want_derivative, int dim, int storage, size_t extra,
+                StageLayout& S, BatchLaunch& L) {
+  const int nf = (int)fused.size();
+  for (int k = 0; k < nf; ++k) {
+    MMI_HIP(hipStreamSynchronize(ctxs[fused[k]]->stream));
+    ctxs[fused[k]]->acc_clean = false;  // the batch clears and uses them on its own stream
+  }
+  S.o_start = 0;
+  S.o_fa = align16(sizeof(int32_t) * (nf + 1));
+  S.o_ea = S.o_fa + align16(sizeof(FastArgs) * nf);
+  S.o_fz = S.o_ea + align16(sizeof(EvalArgs) * nf);
+  S.o_dv = S.o_fz + align16(sizeof(FinalizeArgs) * nf);
+  S.o_it = S.o_dv + align16(sizeof(BatchDeriv) * nf);
+  S.o_extra = S.o_it + align16(sizeof(BatchOut) * nf);
+  S.bytes = S.o_extra + align16(extra);
+  if (int rc = ensure_pinned(&b->h_stage, &b->h_stage_bytes, S.bytes)) return rc;
+  MMI_HIP(b->d_stage.ensure(S.bytes));
+  char* hs = static_cast<char*>(b->h_stage);
+  char* ds = static_cast<char*>(b->d_stage.p);
+  int32_t* start = reinterpret_cast<int32_t*>(hs + S.o_start);
+  int bins_max = 0;
+  int total = 0;
+  size_t lds1 = 0;
+  for (int k = 0; k < nf; ++k) {
+    start[k] = total;
+    total += ctxs[fused[k]]->grid;  // multiples of 8
+    bins_max = std::max(bins_max, ctxs[fused[k]]->bins);
+    lds1 = std::max(lds1, pass1_lds_bytes_with(ctxs[fused[k]]->bins, fa[k].hist_copies));
+  }
+  start[nf] = total;
+  // pass 1's byte per sample: every fused member can use it (fused_candidate)
+  const bool cls = want_derivative != 0;
+  for (int k = 0; k < nf; ++k) {
+    fa[k].cls = nullptr;
+    if (cls) {
+      mmi_context* c = ctxs[fused[k]];
+      MMI_HIP(c->cls.ensure((size_t)sample_count(c)));
+      fa[k].cls = c->cls.as<uint8_t>();
+    }
+  }
+  const int32_t lds = finalize_uses_lds(bins_max) ? 1 : 0;
+  for (int k = 0; k < nf; ++k) {
+    mmi_context* c = ctxs[fused[k]];
+    c->last_path = 1;
+    std::memcpy(hs + S.o_fa + sizeof(FastArgs) * k, &fa[k], sizeof(FastArgs));
+    std::memcpy(hs + S.o_ea + sizeof(EvalArgs) * k, &ea[k], sizeof(EvalArgs));
+    FinalizeArgs z = finalize_args(c, ctx_bufs(c));
+    z.lds = lds;
+    std::memcpy(hs + S.o_fz + sizeof(FinalizeArgs) * k, &z, sizeof(z));
+    BatchDeriv d;
+    std::memset(&d, 0, sizeof(d));
+    d.partials = fa[k].partials;
+    d.rows = 2 * c->grid;
+    d.map = fast_map(c);
+    d.result = dev_result(c);
+    d.deriv = dev_deriv(c);
+    std::memcpy(hs + S.o_dv + sizeof(BatchDeriv) * k, &d, sizeof(d));
+  }
+  std::memset(&L, 0, sizeof(L));
+  L.n = nf;
+  L.dim = dim;
+  L.storage = storage;
+  L.want_derivative = want_derivative ? 1 : 0;
+  L.total_blocks = total;
+  L.bins_max = bins_max;
+  L.lds1 = (int32_t)lds1;
+  L.cls = cls ? 1 : 0;
+  L.start = reinterpret_cast<const int32_t*>(ds + S.o_start);
+  L.fa = reinterpret_cast<const FastArgs*>(ds + S.o_fa);
+  L.ea = reinterpret_cast<const EvalArgs*>(ds + S.o_ea);
+  L.fz = reinterpret_cast<const FinalizeArgs*>(ds + S.o_fz);
+  L.dv = reinterpret_cast<const BatchDeriv*>(ds + S.o_dv);
+  L.items = reinterpret_cast<const BatchOut*>(ds + S.o_it);
+  return MMI_OK;
 }
 
 }  // namespace
@@ -2068,6 +2195,7 @@ int mmi_batch_destroy(mmi_batch* b) {
   drop_graph(b);
   if (b->h_stage) (void)hipHostFree(b->h_stage);
   if (b->h_out) (void)hipHostFree(b->h_out);
+  if (b->h_flags) (void)hipHostFree(b->h_flags);
   b->d_stage.release();
   b->d_out.release();
   if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -2105,20 +2233,9 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
   for (int i = 0; i < n; ++i) {
     EvalArgs a;
     FastArgs f;
-    bool fast = false;
-    EvalBufs eb;
-    if (int rc = prepare_eval(ctxs[i], want_derivative, a, f, fast, nullptr, eb)) return rc;
-    const int64_t P = (int64_t)ctxs[i]->mtf.params.size();
-    // (masked contexts run their own masked kernels)
-    const bool masked = f.fmask.px || f.mmask.px;
-    const bool image_gradient = ctxs[i]->gradient_source == MMI_GRADIENT_IMAGE;
-    // with the derivative, fused members take pass 1's byte per sample (as
-    // they do alone); the others are evaluated on their own
-    mmi_context* c = ctxs[i];
-    const bool cls_ok = !want_derivative || (c->bins <= 253 && c->texel_values_level &&
-                                             !(c->flags & MMI_FLAG_PASS2_RECLASSIFY));
-    if (fast && f.mk == 1 && !masked && image_gradient && P <= 12 && cls_ok &&
-        (storage < 0 || (a.virt.dim == dim && a.storage == storage))) {
+    bool ok = false;
+    if (int rc = fused_candidate(ctxs[i], want_derivative, a, f, ok)) return rc;
+    if (ok && (storage < 0 || (a.virt.dim == dim && a.storage == storage))) {
       dim = a.virt.dim;
       storage = a.storage;
       fused.push_back(i);
@@ -2130,90 +2247,25 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
   }
   const int nf = (int)fused.size();
   b->last_fused = nf;
-  // uploads made on the contexts' own streams (tables, parameters) complete
-  // before the batch stream reads them
-  for (int k = 0; k < nf; ++k) {
-    MMI_HIP(hipStreamSynchronize(ctxs[fused[k]]->stream));
-    ctxs[fused[k]]->acc_clean = false;  // the batch clears and uses them on its own stream
-  }
   const int stride = 16;  // value, count, status, up to 12 derivative entries (+ pad)
   if (nf > 0) {
-    const size_t o_start = 0;
-    const size_t o_fa = align16(sizeof(int32_t) * (nf + 1));
-    const size_t o_ea = o_fa + align16(sizeof(FastArgs) * nf);
-    const size_t o_fz = o_ea + align16(sizeof(EvalArgs) * nf);
-    const size_t o_dv = o_fz + align16(sizeof(FinalizeArgs) * nf);
-    const size_t o_it = o_dv + align16(sizeof(BatchDeriv) * nf);
-    const size_t bytes = o_it + align16(sizeof(BatchOut) * nf);
-    if (int rc = ensure_pinned(&b->h_stage, &b->h_stage_bytes, bytes)) return rc;
+    BatchLaunch L;
+    StageLayout S;
+    if (int rc = stage_fused(b, ctxs, fused, ea, fa, want_derivative, dim, storage, 0, S, L)) return rc;
+    const size_t bytes = S.bytes;
     if (int rc = ensure_pinned(reinterpret_cast<void**>(&b->h_out), &b->h_out_bytes, sizeof(double) * stride * nf))
       return rc;
-    MMI_HIP(b->d_stage.ensure(bytes));
     MMI_HIP(b->d_out.ensure(sizeof(double) * stride * nf));
-    char* hs = static_cast<char*>(b->h_stage);
-    char* ds = static_cast<char*>(b->d_stage.p);
-    int32_t* start = reinterpret_cast<int32_t*>(hs + o_start);
-    int bins_max = 0;
-    int total = 0;
-    size_t lds1 = 0;
     for (int k = 0; k < nf; ++k) {
-      start[k] = total;
-      total += ctxs[fused[k]]->grid;  // multiples of 8
-      bins_max = std::max(bins_max, ctxs[fused[k]]->bins);
-      lds1 = std::max(lds1, pass1_lds_bytes_with(ctxs[fused[k]]->bins, fa[k].hist_copies));
-    }
-    start[nf] = total;
-    // pass 1's byte per sample: every fused member can use it (see above)
-    const bool cls = want_derivative != 0;
-    for (int k = 0; k < nf; ++k) {
-      fa[k].cls = nullptr;
-      if (cls) {
-        mmi_context* c = ctxs[fused[k]];
-        MMI_HIP(c->cls.ensure((size_t)sample_count(c)));
-        fa[k].cls = c->cls.as<uint8_t>();
-      }
-    }
-    const int32_t lds = finalize_uses_lds(bins_max) ? 1 : 0;
-    for (int k = 0; k < nf; ++k) {
-      mmi_context* c = ctxs[fused[k]];
-      const int64_t P = (int64_t)c->mtf.params.size();
-      std::memcpy(hs + o_fa + sizeof(FastArgs) * k, &fa[k], sizeof(FastArgs));
-      std::memcpy(hs + o_ea + sizeof(EvalArgs) * k, &ea[k], sizeof(EvalArgs));
-      FinalizeArgs z = finalize_args(c, ctx_bufs(c));
-      z.lds = lds;
-      std::memcpy(hs + o_fz + sizeof(FinalizeArgs) * k, &z, sizeof(z));
-      BatchDeriv d;
-      std::memset(&d, 0, sizeof(d));
-      d.partials = fa[k].partials;
-      d.rows = 2 * c->grid;
-      d.map = fast_map(c);
-      d.result = dev_result(c);
-      d.deriv = dev_deriv(c);
-      std::memcpy(hs + o_dv + sizeof(BatchDeriv) * k, &d, sizeof(d));
       BatchOut it;
       std::memset(&it, 0, sizeof(it));
-      it.result = dev_result(c);
-      it.deriv = want_derivative ? dev_deriv(c) : nullptr;
-      it.P = (int32_t)P;
-      std::memcpy(hs + o_it + sizeof(BatchOut) * k, &it, sizeof(it));
+      it.result = dev_result(ctxs[fused[k]]);
+      it.deriv = want_derivative ? dev_deriv(ctxs[fused[k]]) : nullptr;
+      it.P = (int32_t)ctxs[fused[k]]->mtf.params.size();
+      std::memcpy(static_cast<char*>(b->h_stage) + S.o_it + sizeof(BatchOut) * k, &it, sizeof(it));
     }
-    BatchLaunch L;
-    std::memset(&L, 0, sizeof(L));
-    L.n = nf;
-    L.dim = dim;
-    L.storage = storage;
-    L.want_derivative = want_derivative ? 1 : 0;
-    L.total_blocks = total;
+    const int total = L.total_blocks, bins_max = L.bins_max;
     L.out_stride = stride;
-    L.bins_max = bins_max;
-    L.lds1 = (int32_t)lds1;
-    L.cls = cls ? 1 : 0;
-    L.start = reinterpret_cast<const int32_t*>(ds + o_start);
-    L.fa = reinterpret_cast<const FastArgs*>(ds + o_fa);
-    L.ea = reinterpret_cast<const EvalArgs*>(ds + o_ea);
-    L.fz = reinterpret_cast<const FinalizeArgs*>(ds + o_fz);
-    L.dv = reinterpret_cast<const BatchDeriv*>(ds + o_dv);
-    L.items = reinterpret_cast<const BatchOut*>(ds + o_it);
     L.out = b->d_out.as<double>();
     MMI_HIP(hipEventRecord(b->ev0, b->stream));
     if (b->use_graph) {
@@ -2222,7 +2274,7 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
                                         (int64_t)(intptr_t)b->d_stage.p, (int64_t)(intptr_t)b->d_out.p,
                                         (int64_t)(intptr_t)b->h_stage, (int64_t)(intptr_t)b->h_out};
       if (!b->exec || sig != b->sig) {
-        drop_graph(b);
+        drop_eval_graph(b);
         MMI_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipMemcpyAsync(b->d_stage.p, b->h_stage, bytes, hipMemcpyHostToDevice, b->stream);
         if (e == hipSuccess) e = launch_fast_batch(L, b->stream);
@@ -2318,85 +2370,22 @@ int mmi_regular_step_start(mmi_regular_step_state* st, const double* scales, int
 
 namespace {
 
-// CompensatedSummation (itkCompensatedSummation.hxx:38-48).
-struct Kahan {
-  double sum = 0.0, c = 0.0;
-  void add(double x) {
-    const double y = x - c;
-    const double t = sum + y;
-    c = (t - sum) - y;
-    sum = t;
-  }
-};
-
-// RegularStepGradientDescentOptimizerv4::AdvanceOneStep; returns the stop
-// condition it met (MMI_STOP_RUNNING when it updated the parameters).
-int regular_step_advance(mmi_regular_step_state& s, const mmi_regular_step_options& o, std::vector<double>& params) {
-  const int P = (int)params.size();
-  if (!s.scales_identity && P > 0) {
-    for (int j = 0; j < P; ++j) {
-      const double f = 1.0 / s.scales[j % s.n_scales];
-      s.gradient[j] = s.gradient[j] * f;
-      s.previous_gradient[j] = s.previous_gradient[j] * f;
-    }
-  }
-  Kahan g2;
-  for (int j = 0; j < P; ++j) {
-    const double w = s.gradient[j];
-    g2.add(w * w);
-  }
-  const double magnitude = std::sqrt(g2.sum);
-  s.gradient_magnitude = magnitude;
-  if (magnitude < o.gradient_magnitude_tolerance) return MMI_STOP_GRADIENT_MAGNITUDE_TOLERANCE;
-  Kahan sp;
-  for (int j = 0; j < P; ++j) {
-    const double a = s.gradient[j], b = s.previous_gradient[j];
-    sp.add(a * b);
-  }
-  if (sp.sum < 0) s.relaxation *= o.relaxation_factor;
-  const double step = s.relaxation * o.learning_rate;
-  if (step < o.minimum_step_length) return MMI_STOP_STEP_TOO_SMALL;
-  for (int j = 0; j < P; ++j) s.gradient[j] = s.gradient[j] * s.relaxation * o.learning_rate;
-  const double factor = 1.0 / magnitude;
-  // Transform::UpdateTransformParameters (itkTransform.hxx:71-115)
-  for (int j = 0; j < P; ++j) params[j] = factor == 1.0 ? params[j] + s.gradient[j] : params[j] + s.gradient[j] * factor;
-  return MMI_STOP_RUNNING;
-}
-
-}  // namespace
-
-extern "C" {
-
-int mmi_batch_regular_step(mmi_batch* b, mmi_context* const* ctxs, int32_t n, const mmi_regular_step_options* o,
-                           mmi_regular_step_state* states, int32_t max_steps) {
-  if (!b || !o || (n > 0 && (!ctxs || !states))) return fail(MMI_ERR_ARGUMENT, "NULL argument");
-  if (o->relaxation_factor < 0.0)
-    return fail(MMI_ERR_ARGUMENT, "Relaxation factor must be positive. Current value is " +
-                                      std::to_string(o->relaxation_factor));
-  if (o->relaxation_factor >= 1.0)
-    return fail(MMI_ERR_ARGUMENT, "Relaxation factor must be less than 1.0. Current value is " +
-                                      std::to_string(o->relaxation_factor));
-  if (o->gradient_magnitude_tolerance < 0.0)
-    return fail(MMI_ERR_ARGUMENT, "Gradient magnitude tolerance must be greater or equal 0.0.");
-  for (int i = 0; i < n; ++i) {
-    if (!ctxs[i]) return fail(MMI_ERR_ARGUMENT, "context is NULL");
-    const size_t P = ctxs[i]->mtf.params.size();
-    if (P > 12) return fail(MMI_ERR_ARGUMENT, "mmi_batch_regular_step handles transforms of at most 12 parameters");
-    if (states[i].n_scales != 0 && (size_t)states[i].n_scales != P)
-      return fail(MMI_ERR_ARGUMENT, "The size of Scales differs from the number of parameters");
-  }
+// The host loop: up to budget[i] iterations of member i, the running members
+// evaluated together in each (mmi_batch_evaluate), then stepped on the host.
+int regular_step_host(mmi_batch* b, mmi_context* const* ctxs, int32_t n, const mmi_regular_step_options* o,
+                      mmi_regular_step_state* states, const std::vector<int32_t>& budget) {
   std::vector<mmi_context*> act;
   std::vector<int> idx;
   std::vector<double> values;
   std::vector<double*> derivs;
   std::vector<int64_t> nvalid;
   std::vector<int32_t> status;
-  for (int step = 0; step < max_steps; ++step) {
+  for (int step = 0;; ++step) {
     act.clear();
     idx.clear();
     for (int i = 0; i < n; ++i) {
       mmi_regular_step_state& s = states[i];
-      if (s.stop != MMI_STOP_RUNNING) continue;
+      if (s.stop != MMI_STOP_RUNNING || step >= budget[i]) continue;
       if (s.iteration >= o->number_of_iterations) {  // checked before evaluating (GDv4:82-89)
         s.stop = MMI_STOP_MAXIMUM_NUMBER_OF_ITERATIONS;
         continue;
@@ -2427,7 +2416,7 @@ int mmi_batch_regular_step(mmi_batch* b, mmi_context* const* ctxs, int32_t n, co
       }
       s.value = values[k];
       std::vector<double> params = c->mtf.params;
-      const int stop = regular_step_advance(s, *o, params);
+      const int stop = regular_step_advance(s, *o, params.data(), (int)params.size());
       s.iteration += 1;
       if (stop != MMI_STOP_RUNNING) {
         s.stop = stop;
@@ -2436,6 +2425,181 @@ int mmi_batch_regular_step(mmi_batch* b, mmi_context* const* ctxs, int32_t n, co
       if (int rc = mmi_set_moving_parameters(c, params.data(), (int64_t)params.size())) return rc;
     }
   }
+  return MMI_OK;
+}
+
+// The device loop, for the members that go into the fused launch with the
+// derivative (fused_candidate; one dimension and storage): chunks of
+// b->chunk iterations (fused evaluation, then regular_step_kernel's step and
+// new map) replayed until every member has stopped, used up max_steps or
+// handed back to the host.  budget[i] receives the iterations member i has
+// left for the host loop (members not taken keep theirs).
+int regular_step_device(mmi_batch* b, mmi_context* const* ctxs, int32_t n, const mmi_regular_step_options* o,
+                        mmi_regular_step_state* states, int32_t max_steps, std::vector<int32_t>& budget) {
+  b->last_device_steps = 0;
+  if (max_steps <= 0) return MMI_OK;
+  std::vector<int> fused;
+  std::vector<EvalArgs> ea;
+  std::vector<FastArgs> fa;
+  int dim = 0, storage = -1;
+  for (int i = 0; i < n; ++i) {
+    // stopped members, and those the host loop's first check stops, stay there
+    if (states[i].stop != MMI_STOP_RUNNING || states[i].iteration >= o->number_of_iterations) continue;
+    EvalArgs a;
+    FastArgs f;
+    bool ok = false;
+    if (int rc = fused_candidate(ctxs[i], 1, a, f, ok)) return rc;
+    if (!ok || (storage >= 0 && (a.virt.dim != dim || a.storage != storage))) continue;
+    dim = a.virt.dim;
+    storage = a.storage;
+    fused.push_back(i);
+    ea.push_back(a);
+    fa.push_back(f);
+  }
+  const int nf = (int)fused.size();
+  if (nf == 0) return MMI_OK;
+  const size_t o_ctl = align16(sizeof(StepItem) * nf);
+  const size_t o_halt = o_ctl + align16(sizeof(StepCtl));
+  const size_t extra = o_halt + align16(sizeof(int32_t) * nf);
+  BatchLaunch L;
+  StageLayout S;
+  if (int rc = stage_fused(b, ctxs, fused, ea, fa, 1, dim, storage, extra, S, L)) return rc;
+  char* hs = static_cast<char*>(b->h_stage) + S.o_extra;
+  char* ds = static_cast<char*>(b->d_stage.p) + S.o_extra;
+  StepItem* h_items = reinterpret_cast<StepItem*>(hs);
+  for (int k = 0; k < nf; ++k) {
+    mmi_context* c = ctxs[fused[k]];
+    StepItem it;
+    std::memset(&it, 0, sizeof(it));
+    it.s = states[fused[k]];
+    it.P = (int32_t)c->mtf.params.size();
+    for (int j = 0; j < it.P; ++j) it.params[j] = c->mtf.params[j];
+    it.in = affine_inputs(c, fa[k].rows);
+    it.fa = reinterpret_cast<FastArgs*>(static_cast<char*>(b->d_stage.p) + S.o_fa + sizeof(FastArgs) * k);
+    it.ea = reinterpret_cast<EvalArgs*>(static_cast<char*>(b->d_stage.p) + S.o_ea + sizeof(EvalArgs) * k);
+    it.result = dev_result(c);
+    it.deriv = dev_deriv(c);
+    std::memcpy(&h_items[k], &it, sizeof(it));
+  }
+  StepCtl ctl;
+  std::memset(&ctl, 0, sizeof(ctl));
+  ctl.o = *o;
+  ctl.max_steps = max_steps;
+  ctl.n = nf;
+  std::memcpy(hs + o_ctl, &ctl, sizeof(ctl));
+  std::memset(hs + o_halt, 0, sizeof(int32_t) * nf);
+  StepItem* d_items = reinterpret_cast<StepItem*>(ds);
+  const StepCtl* d_ctl = reinterpret_cast<const StepCtl*>(ds + o_ctl);
+  int32_t* d_halt = reinterpret_cast<int32_t*>(ds + o_halt);
+  L.halted = d_halt;
+  if (int rc = ensure_pinned(reinterpret_cast<void**>(&b->h_flags), &b->h_flags_bytes, sizeof(int32_t) * nf))
+    return rc;
+  const int K = b->chunk;
+  auto chunk = [&]() -> hipError_t {
+    for (int k = 0; k < K; ++k) {
+      hipError_t e = launch_fast_batch(L, b->stream);
+      if (e == hipSuccess) e = launch_regular_step(d_items, d_ctl, d_halt, nf, dim, b->stream);
+      if (e != hipSuccess) return e;
+    }
+    return hipMemcpyAsync(b->h_flags, d_halt, sizeof(int32_t) * nf, hipMemcpyDeviceToHost, b->stream);
+  };
+  MMI_HIP(hipMemcpyAsync(b->d_stage.p, b->h_stage, S.bytes, hipMemcpyHostToDevice, b->stream));
+  if (b->use_graph) {
+    const std::vector<int64_t> sig = {nf, dim, storage, L.total_blocks, L.bins_max, L.lds1, K, (int64_t)S.bytes,
+                                      (int64_t)(intptr_t)b->d_stage.p, (int64_t)(intptr_t)b->h_flags};
+    if (!b->step_exec || sig != b->step_sig) {
+      drop_step_graph(b);
+      MMI_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      const hipError_t e = chunk();
+      hipGraph_t g = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+      if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(MMI_ERR_DEVICE, std::string("HIP graph capture failed: ") +
+                                        hipGetErrorString(e != hipSuccess ? e : e2));
+      }
+      b->step_graph = g;
+      MMI_HIP(hipGraphInstantiate(&b->step_exec, b->step_graph, nullptr, nullptr, 0));
+      b->step_sig = sig;
+    }
+  }
+  MMI_HIP(hipEventRecord(b->ev0, b->stream));
+  const int chunks = (max_steps + K - 1) / K;
+  for (int c = 0; c < chunks; ++c) {
+    if (b->use_graph) {
+      MMI_HIP(hipGraphLaunch(b->step_exec, b->stream));
+    } else {
+      MMI_HIP(chunk());
+    }
+    MMI_HIP(hipStreamSynchronize(b->stream));
+    bool all = true;
+    for (int k = 0; k < nf; ++k) all = all && b->h_flags[k] != 0;
+    if (all) break;
+  }
+  MMI_HIP(hipEventRecord(b->ev1, b->stream));
+  MMI_HIP(hipMemcpyAsync(h_items, d_items, sizeof(StepItem) * nf, hipMemcpyDeviceToHost, b->stream));
+  MMI_HIP(hipStreamSynchronize(b->stream));
+  MMI_HIP(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
+  b->last_fused = nf;
+  for (int k = 0; k < nf; ++k) {
+    const int i = fused[k];
+    const StepItem& it = h_items[k];
+    states[i] = it.s;
+    b->last_device_steps += it.steps;
+    // the context takes the parameters the device reached (its host set-up
+    // computes the map the device computed)
+    if (int rc = mmi_set_moving_parameters(ctxs[i], it.params, it.P)) return rc;
+    if (it.dstatus) (void)fail(MMI_ERR_DEGENERATE, degenerate_message(it.dstatus));
+    budget[i] = it.handback != kStepOnDevice ? max_steps - it.steps : 0;
+  }
+  return MMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mmi_batch_regular_step(mmi_batch* b, mmi_context* const* ctxs, int32_t n, const mmi_regular_step_options* o,
+                           mmi_regular_step_state* states, int32_t max_steps) {
+  if (!b || !o || (n > 0 && (!ctxs || !states))) return fail(MMI_ERR_ARGUMENT, "NULL argument");
+  if (o->relaxation_factor < 0.0)
+    return fail(MMI_ERR_ARGUMENT, "Relaxation factor must be positive. Current value is " +
+                                      std::to_string(o->relaxation_factor));
+  if (o->relaxation_factor >= 1.0)
+    return fail(MMI_ERR_ARGUMENT, "Relaxation factor must be less than 1.0. Current value is " +
+                                      std::to_string(o->relaxation_factor));
+  if (o->gradient_magnitude_tolerance < 0.0)
+    return fail(MMI_ERR_ARGUMENT, "Gradient magnitude tolerance must be greater or equal 0.0.");
+  MMI_HIP(hipSetDevice(b->device));
+  for (int i = 0; i < n; ++i) {
+    if (!ctxs[i]) return fail(MMI_ERR_ARGUMENT, "context is NULL");
+    if (ctxs[i]->device != b->device) return fail(MMI_ERR_ARGUMENT, "contexts of a batch must share its device");
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == ctxs[i]) return fail(MMI_ERR_ARGUMENT, "a context appears twice in one batch");
+    const size_t P = ctxs[i]->mtf.params.size();
+    if (P > 12) return fail(MMI_ERR_ARGUMENT, "mmi_batch_regular_step handles transforms of at most 12 parameters");
+    if (states[i].n_scales != 0 && (size_t)states[i].n_scales != P)
+      return fail(MMI_ERR_ARGUMENT, "The size of Scales differs from the number of parameters");
+  }
+  std::vector<int32_t> budget(n, std::max<int32_t>(max_steps, 0));
+  b->last_device_steps = 0;
+  if (b->step_on_device)
+    if (int rc = regular_step_device(b, ctxs, n, o, states, max_steps, budget)) return rc;
+  return regular_step_host(b, ctxs, n, o, states, budget);
+}
+
+int mmi_batch_set_step_loop(mmi_batch* b, int32_t on_device, int32_t chunk) {
+  if (!b) return fail(MMI_ERR_ARGUMENT, "batch is NULL");
+  if (chunk < 1 || chunk > 64) return fail(MMI_ERR_ARGUMENT, "chunk must be in [1, 64] iterations");
+  b->step_on_device = on_device != 0;
+  if (chunk != b->chunk) drop_step_graph(b);
+  b->chunk = chunk;
+  return MMI_OK;
+}
+
+int mmi_batch_get_step_count(mmi_batch* b, int32_t* device_steps) {
+  if (!b || !device_steps) return fail(MMI_ERR_ARGUMENT, "NULL argument");
+  *device_steps = b->last_device_steps;
   return MMI_OK;
 }
 

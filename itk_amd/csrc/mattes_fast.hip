@@ -953,8 +953,10 @@ __global__ void __launch_bounds__(256) pass1_fast_kernel(FastArgs f, EvalArgs a)
 template <int D, typename P>
 __global__ void __launch_bounds__(256) pass1_fast_batch(const FastArgs* __restrict__ fa,
                                                         const EvalArgs* __restrict__ ea,
-                                                        const int32_t* __restrict__ start, int n) {
+                                                        const int32_t* __restrict__ start, int n,
+                                                        const int32_t* __restrict__ halted) {
   const int r = batch_item(start, n, blockIdx.x);
+  if (halted && halted[r]) return;
   if (fa[r].rows.on)
     pass1_rows_body<D, P>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
   else
@@ -964,8 +966,10 @@ __global__ void __launch_bounds__(256) pass1_fast_batch(const FastArgs* __restri
 template <int D, typename P>
 __global__ void __launch_bounds__(256) pass2_cls_batch(const FastArgs* __restrict__ fa,
                                                        const EvalArgs* __restrict__ ea,
-                                                       const int32_t* __restrict__ start, int n) {
+                                                       const int32_t* __restrict__ start, int n,
+                                                       const int32_t* __restrict__ halted) {
   const int r = batch_item(start, n, blockIdx.x);
+  if (halted && halted[r]) return;
   if (fa[r].rows.on)
     pass2_rows_body<D, P>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
   else
@@ -1015,8 +1019,10 @@ __global__ void __launch_bounds__(256) pass2_cls_kernel(FastArgs f, EvalArgs a) 
 template <int D, typename P>
 __global__ void __launch_bounds__(256) pass2_fast_batch(const FastArgs* __restrict__ fa,
                                                         const EvalArgs* __restrict__ ea,
-                                                        const int32_t* __restrict__ start, int n) {
+                                                        const int32_t* __restrict__ start, int n,
+                                                        const int32_t* __restrict__ halted) {
   const int r = batch_item(start, n, blockIdx.x);
+  if (halted && halted[r]) return;
   pass2_fast_body<D, P, 1>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
 }
 
@@ -1028,14 +1034,18 @@ __global__ void __launch_bounds__(256) fast_derivative_kernel(const double* __re
 }
 
 template <int D>
-__global__ void __launch_bounds__(256) fast_derivative_batch(const BatchDeriv* __restrict__ items) {
+__global__ void __launch_bounds__(256) fast_derivative_batch(const BatchDeriv* __restrict__ items,
+                                                             const int32_t* __restrict__ halted) {
+  if (halted && halted[blockIdx.x]) return;
   const BatchDeriv& it = items[blockIdx.x];
   fast_derivative_body<D>(it.partials, it.rows, it.map, it.result, it.deriv);
 }
 
 // Zeroes one evaluation's histogram, marginal, count and overflow flag per
 // workgroup (a kernel rather than memsets, so the sequence can be captured).
-__global__ void __launch_bounds__(256) batch_clear(const FastArgs* __restrict__ fa) {
+__global__ void __launch_bounds__(256) batch_clear(const FastArgs* __restrict__ fa,
+                                                   const int32_t* __restrict__ halted) {
+  if (halted && halted[blockIdx.x]) return;
   const FastArgs& f = fa[blockIdx.x];
   const int BB = f.bins * f.bins;
   for (int i = threadIdx.x; i < BB; i += blockDim.x) f.hist[i] = 0ull;
@@ -1168,23 +1178,26 @@ hipError_t launch_fast_derivative(const double* partials, int rows, const FastMa
 
 hipError_t launch_fast_batch(const BatchLaunch& b, hipStream_t st) {
   const int B = b.bins_max;
-  hipLaunchKernelGGL(batch_clear, dim3(b.n), dim3(256), 0, st, b.fa);
+  hipLaunchKernelGGL(batch_clear, dim3(b.n), dim3(256), 0, st, b.fa, b.halted);
   const size_t lds1 = (size_t)b.lds1;
-  MMI_DISPATCH_FAST(pass1_fast_batch, b.storage, b.dim, b.total_blocks, lds1, st, b.fa, b.ea, b.start, b.n);
-  hipError_t e = launch_finalize_batch(b.fz, b.n, B, st);
+  MMI_DISPATCH_FAST(pass1_fast_batch, b.storage, b.dim, b.total_blocks, lds1, st, b.fa, b.ea, b.start, b.n,
+                    b.halted);
+  hipError_t e = launch_finalize_batch(b.fz, b.n, B, b.halted, st);
   if (e != hipSuccess) return e;
   if (b.want_derivative) {
     const size_t lds2 = ((size_t)B * B + 128) * 4;
     if (b.cls)
-      MMI_DISPATCH_FAST(pass2_cls_batch, b.storage, b.dim, b.total_blocks, lds2, st, b.fa, b.ea, b.start, b.n);
+      MMI_DISPATCH_FAST(pass2_cls_batch, b.storage, b.dim, b.total_blocks, lds2, st, b.fa, b.ea, b.start, b.n,
+                        b.halted);
     else
-      MMI_DISPATCH_FAST(pass2_fast_batch, b.storage, b.dim, b.total_blocks, lds2, st, b.fa, b.ea, b.start, b.n);
+      MMI_DISPATCH_FAST(pass2_fast_batch, b.storage, b.dim, b.total_blocks, lds2, st, b.fa, b.ea, b.start, b.n,
+                        b.halted);
     if (b.dim == 2)
-      hipLaunchKernelGGL(fast_derivative_batch<2>, dim3(b.n), dim3(256), 0, st, b.dv);
+      hipLaunchKernelGGL(fast_derivative_batch<2>, dim3(b.n), dim3(256), 0, st, b.dv, b.halted);
     else
-      hipLaunchKernelGGL(fast_derivative_batch<3>, dim3(b.n), dim3(256), 0, st, b.dv);
+      hipLaunchKernelGGL(fast_derivative_batch<3>, dim3(b.n), dim3(256), 0, st, b.dv, b.halted);
   }
-  hipLaunchKernelGGL(batch_gather, dim3(b.n), dim3(64), 0, st, b.items, b.out, b.out_stride);
+  if (b.out) hipLaunchKernelGGL(batch_gather, dim3(b.n), dim3(64), 0, st, b.items, b.out, b.out_stride);
   return hipGetLastError();
 }
 

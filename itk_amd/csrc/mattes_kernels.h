@@ -260,7 +260,8 @@ struct BatchLaunch {
   const FinalizeArgs* fz;
   const BatchDeriv* dv;
   const BatchOut* items;
-  double* out;  // [n][out_stride]: value, n_valid, status, derivative
+  double* out;  // [n][out_stride]: value, n_valid, status, derivative (NULL: not gathered)
+  const int32_t* halted;  // NULL, or [n]: nonzero skips the evaluation (every kernel of the sequence)
 };
 
 // Uncertain samples a fast-path workgroup can list (more: the evaluation is
@@ -310,7 +311,9 @@ struct Direction {
 hipError_t launch_pass1(const EvalArgs& a, int grid, hipStream_t st);
 hipError_t launch_finalize(const FinalizeArgs& f, hipStream_t st);
 // n finalize workgroups, item lds flags already set with finalize_uses_lds.
-hipError_t launch_finalize_batch(const FinalizeArgs* fz, int n, int bins_max, hipStream_t st);
+// halted: NULL, or one flag per evaluation (nonzero: its workgroups return
+// at once; the device optimizer loop's finished members)
+hipError_t launch_finalize_batch(const FinalizeArgs* fz, int n, int bins_max, const int32_t* halted, hipStream_t st);
 // The whole fast-path sequence for a batch (clear, pass 1, exceptions,
 // finalize, pass 2, exceptions, derivative, gather), in launch order.
 hipError_t launch_fast_batch(const BatchLaunch& b, hipStream_t st);

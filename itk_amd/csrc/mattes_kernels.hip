@@ -206,7 +206,9 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& f) {
 
 __global__ void __launch_bounds__(kFinalizeThreads) finalize_kernel(FinalizeArgs f) { finalize_body(f); }
 
-__global__ void __launch_bounds__(kFinalizeThreads) finalize_batch(const FinalizeArgs* __restrict__ fz) {
+__global__ void __launch_bounds__(kFinalizeThreads) finalize_batch(const FinalizeArgs* __restrict__ fz,
+                                                                   const int32_t* __restrict__ halted) {
+  if (halted && halted[blockIdx.x]) return;
   finalize_body(fz[blockIdx.x]);
 }
 
@@ -373,10 +375,10 @@ hipError_t launch_finalize(const FinalizeArgs& f, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_finalize_batch(const FinalizeArgs* fz, int n, int bins_max, hipStream_t st) {
+hipError_t launch_finalize_batch(const FinalizeArgs* fz, int n, int bins_max, const int32_t* halted, hipStream_t st) {
   const size_t lds =
       finalize_uses_lds(bins_max) ? ((size_t)bins_max * bins_max + 3 * (size_t)bins_max) * sizeof(double) : 0;
-  hipLaunchKernelGGL(finalize_batch, dim3(n), dim3(kFinalizeThreads), lds, st, fz);
+  hipLaunchKernelGGL(finalize_batch, dim3(n), dim3(kFinalizeThreads), lds, st, fz, halted);
   return hipGetLastError();
 }
 
