@@ -16,8 +16,11 @@
  * rounded to float, kept as double, truncated for integers), dense
  * sampling of the virtual domain or a virtual point set, ImageMaskSpatialObject
  * masks, an identity or affine fixed transform, and a moving transform that is
- * an AffineTransform or a cubic BSplineTransform, alone or as the one entry of
- * a CompositeTransform.  Other settings throw at Initialize.
+ * an AffineTransform or a cubic BSplineTransform, alone or as the last entry
+ * of a CompositeTransform whose earlier entries (at most 4) are
+ * MatrixOffsetTransformBase transforms and whose last entry alone is
+ * optimized (ImageRegistrationMethodv4's moving initial transform).  Other
+ * settings throw at Initialize.
  *
  * The base class's Initialize sets up the virtual domain, the transforms and a
  * mapped fixed point set; the moving-image gradient filter runs on the device
@@ -39,6 +42,7 @@
 #include "itkImageMaskSpatialObject.h"
 #include "itkImageRegionConstIterator.h"
 #include "itkImageToImageMetricv4.h"
+#include "itkMatrixOffsetTransformBase.h"
 #include "itk_mattes_hip.hpp"
 
 namespace itk {
@@ -230,11 +234,28 @@ class MattesMutualInformationImageToImageMetricv4HIP
   void PushTransform(bool fixed, const TTransform* t) {
     using Scalar = typename TTransform::ParametersValueType;
     const TransformBaseTemplate<Scalar>* base = t;
+    std::vector<double> stage_m, stage_o;  // moving composite: the entries before the last
+    int32_t n_stages = 0;
     if (const auto* comp = dynamic_cast<const CompositeTransform<Scalar, Dimension>*>(base)) {
-      if (comp->GetNumberOfTransforms() != 1)
-        itkExceptionMacro(<< "the HIP metric takes a CompositeTransform with one transform");
-      base = comp->GetNthTransformConstPointer(0);
+      const SizeValueType n = comp->GetNumberOfTransforms();
+      if (n == 0) itkExceptionMacro(<< "the moving CompositeTransform is empty");
+      if (fixed && n != 1) itkExceptionMacro(<< "the HIP metric takes a fixed CompositeTransform with one transform");
+      if (n - 1 > 4) itkExceptionMacro(<< "the HIP metric takes at most 4 transforms before the optimized one");
+      using MatrixOffset = MatrixOffsetTransformBase<Scalar, Dimension, Dimension>;
+      for (SizeValueType i = 0; i + 1 < n; ++i) {
+        if (comp->GetNthTransformToOptimize(i))
+          itkExceptionMacro(<< "the HIP metric optimizes only the CompositeTransform's most recent transform");
+        const auto* s = dynamic_cast<const MatrixOffset*>(comp->GetNthTransformConstPointer(i));
+        if (!s) itkExceptionMacro(<< "the transforms before the optimized one must be matrix-offset transforms");
+        for (unsigned int r = 0; r < Dimension; ++r) {
+          for (unsigned int c = 0; c < Dimension; ++c) stage_m.push_back(static_cast<double>(s->GetMatrix()[r][c]));
+          stage_o.push_back(static_cast<double>(s->GetOffset()[r]));
+        }
+        ++n_stages;
+      }
+      base = comp->GetNthTransformConstPointer(n - 1);
     }
+    if (!fixed) m_Context->set_moving_initial_transforms(n_stages, stage_m.data(), stage_o.data());
     int32_t kind = MMI_TF_IDENTITY;
     if (dynamic_cast<const AffineTransform<Scalar, Dimension>*>(base))
       kind = MMI_TF_AFFINE;

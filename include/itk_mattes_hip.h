@@ -211,6 +211,18 @@ int mmi_set_moving_transform(mmi_context* ctx, int32_t kind, const double* fixed
                              const double* params, int64_t n_params);
 /* Replace only the moving transform's parameters (optimizer step). */
 int mmi_set_moving_parameters(mmi_context* ctx, const double* params, int64_t n_params);
+/* A composite moving transform (CompositeTransform with only its most recent
+ * transform optimized, as ImageRegistrationMethodv4 builds it from a moving
+ * initial transform: itkImageRegistrationMethodv4.hxx:344-355): n
+ * matrix-offset transforms (MatrixOffsetTransformBase, y = matrix * x +
+ * offset) before the moving transform in the composite's queue, in queue
+ * order.  A point goes through the moving transform first, then through
+ * stages n-1 .. 0 (itkCompositeTransform.hxx:60-71); the derivative takes the
+ * composite's Jacobian, the stages' matrices times the moving transform's
+ * parameter Jacobian (itkCompositeTransform.hxx:467-560).  matrices: n
+ * row-major D x D, offsets: n x D; n = 0 (the default) removes them, at most
+ * 4.  Set after the images.  Such contexts evaluate on the exact FP64 path. */
+int mmi_set_moving_initial_transforms(mmi_context* ctx, int32_t n, const double* matrices, const double* offsets);
 
 /* GetValueAndDerivative (want_derivative = 1) or GetValue (0):
  * ImageToImageMetricv4::GetValueAndDerivative (itkImageToImageMetricv4.hxx:217-231).

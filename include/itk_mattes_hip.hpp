@@ -137,6 +137,11 @@ class Context {
                                    (int64_t)params.size()));
   }
   void set_moving_parameters(const double* params, int64_t n) { check(mmi_set_moving_parameters(ctx_, params, n)); }
+  // a composite moving transform's matrix-offset entries before the moving
+  // transform, in queue order (n = 0: none)
+  void set_moving_initial_transforms(int32_t n, const double* matrices, const double* offsets) {
+    check(mmi_set_moving_initial_transforms(ctx_, n, matrices, offsets));
+  }
   void initialize() { check(mmi_initialize(ctx_)); }
 
   // GetValueAndDerivative (derivative != NULL) or GetValue: the reference's

@@ -61,6 +61,7 @@ SIGNATURES = {
     # per-evaluation calls take plain addresses (ints): building typed ctypes
     # pointers costs microseconds per call
     "mmi_set_moving_parameters": (ctypes.c_int, [_vp, _vp, ctypes.c_int64]),
+    "mmi_set_moving_initial_transforms": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, _dp]),
     "mmi_evaluate": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp]),
     "mmi_evaluate_static": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _vp, _vp]),
     "mmi_get_sample_bins": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int16), ctypes.POINTER(ctypes.c_int16)]),

@@ -281,6 +281,10 @@ struct mmi_context {
   HostTransform ftf, mtf;
   DevTransform dftf{}, dmtf{};
   DevBuf coeffs;
+  // composite moving transform: matrix-offset stages before the moving
+  // transform in the queue (12 doubles each), on the host and the device
+  std::vector<double> init_stages;
+  DevBuf init_dev;
 
   DevBuf hist, nvalid, pdf, fm, mm, pratio, pratio_f, rowsum, partials, sample_fbin, sample_mbin, minmax_out;
   // the evaluation's outputs in one block (one copy to the host): the result,
@@ -849,6 +853,8 @@ void setup_rows_layout(const mmi_context* ctx, FastRows& R) {
 
 bool setup_fast(mmi_context* ctx, FastArgs* F) {
   std::memset(F, 0, sizeof(*F));
+  // a composite moving transform maps through its earlier stages: the FP64 path
+  if (!ctx->init_stages.empty()) return false;
   // integer pixels truncate the interpolated value: the FP64 path
   if (ctx->fixed_kind == kPixelInteger || ctx->moving_kind == kPixelInteger) return false;
   const int D = ctx->fixed_g.dim;
@@ -1609,6 +1615,31 @@ int mmi_set_moving_transform(mmi_context* ctx, int32_t kind, const double* fixed
   return set_transform(ctx, ctx->mtf, ctx->dmtf, true, kind, fixed_params, n_fixed, params, n_params);
 }
 
+int mmi_set_moving_initial_transforms(mmi_context* ctx, int32_t n, const double* matrices, const double* offsets) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (n < 0 || n > kMaxInitialStages)
+    return fail(MMI_ERR_ARGUMENT, "a composite moving transform takes 0 to " + std::to_string(kMaxInitialStages) +
+                                      " transforms before the moving transform");
+  if (n > 0 && (!matrices || !offsets)) return fail(MMI_ERR_ARGUMENT, "NULL argument");
+  const int D = ctx->fixed_g.dim ? ctx->fixed_g.dim : ctx->moving_g.dim;
+  if (n > 0 && D != 2 && D != 3) return fail(MMI_ERR_ARGUMENT, "set the images before the composite stages");
+  std::vector<double> st((size_t)n * 12, 0.0);
+  for (int i = 0; i < n; ++i) {
+    for (int k = 0; k < D * D; ++k) st[12 * i + k] = matrices[(size_t)i * D * D + k];
+    for (int k = 0; k < D; ++k) st[12 * i + 9 + k] = offsets[(size_t)i * D + k];
+    for (int k = 0; k < 12; ++k)
+      if (!std::isfinite(st[12 * i + k])) return fail(MMI_ERR_ARGUMENT, "composite stage is not finite");
+  }
+  MMI_HIP(hipSetDevice(ctx->device));
+  if (n > 0) {
+    MMI_HIP(ctx->init_dev.ensure(sizeof(double) * st.size()));
+    MMI_HIP(hipStreamSynchronize(ctx->stream));
+    MMI_HIP(hipMemcpy(ctx->init_dev.p, st.data(), sizeof(double) * st.size(), hipMemcpyHostToDevice));
+  }
+  ctx->init_stages = st;
+  return MMI_OK;
+}
+
 int mmi_set_moving_parameters(mmi_context* ctx, const double* params, int64_t n_params) {
   if (!ctx || !params) return fail(MMI_ERR_ARGUMENT, "NULL argument");
   if ((int64_t)ctx->mtf.params.size() != n_params)
@@ -1669,6 +1700,8 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
   a.moving = to_dev(ctx->moving_g);
   a.ftf = ctx->dftf;
   a.mtf = ctx->dmtf;
+  a.n_init = (int32_t)(ctx->init_stages.size() / 12);
+  a.init_stages = a.n_init ? ctx->init_dev.as<double>() : nullptr;
   a.fixed_px = fixed_pixels(ctx);
   a.moving_px = moving_pixels(ctx);
   a.texels = texel_pixels(ctx);

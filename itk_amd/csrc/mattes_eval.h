@@ -71,6 +71,22 @@ __device__ __forceinline__ bool eval_sample(const EvalArgs& a, const double* vp,
   const double fval = pixel_value(a.fixed_kind, interp_exact<D, P>(static_cast<const typename P::pixel*>(a.fixed_px), fc));
 
   transform_point<D, MK>(a.mtf, o.vp, o.mp, &o.sup);
+  // CompositeTransform::TransformPoint: the queue's earlier transforms after
+  // the moving one, last to first (itkCompositeTransform.hxx:60-71), each as
+  // MatrixOffsetTransformBase::TransformPoint (matrix * point + offset)
+  for (int st = a.n_init - 1; st >= 0; --st) {
+    const double* m = a.init_stages + 12 * st;
+    double q[3];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      double sum = 0.0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) sum += m[r * D + c] * o.mp[c];
+      q[r] = sum + m[9 + r];
+    }
+#pragma unroll
+    for (int r = 0; r < D; ++r) o.mp[r] = q[r];
+  }
   // the moving mask at the mapped moving point (itkImageToImageMetricv4.hxx:351-362)
   if (a.moving_mask && !mask_inside<D>(*a.moving_mask, o.mp)) return false;
   point_to_cindex<D>(a.moving, o.mp, ci);

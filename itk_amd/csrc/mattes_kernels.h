@@ -194,7 +194,15 @@ struct EvalArgs {
   double* partials;      // [grid * P] (affine)
   double* deriv_accum;   // [P] (B-spline, non-separable geometry)
   float* bspline_g;      // [D][n] (B-spline, separable geometry): per-sample w * gradient, 0 if not counted
+  // composite moving transform: n_init matrix-offset stages (12 doubles each:
+  // row-major matrix, offset) before the moving transform in the
+  // CompositeTransform queue; the mapped point goes through stages n_init-1 .. 0
+  const double* init_stages;
+  int32_t n_init;
+  int32_t pad3_;
 };
+
+constexpr int kMaxInitialStages = 4;
 
 // Support of the cubic B-spline along one axis of a separable geometry (virtual
 // index_to_physical and grid physical_to_index both diagonal): for virtual

@@ -295,6 +295,22 @@ __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
       } else {
         central_difference<D, P>(a, o.mp, gd);
       }
+      // composite moving transform: the Jacobian is the stages' matrices times
+      // the moving transform's (itkCompositeTransform.hxx:467-560), so the
+      // gradient meets the moving transform's Jacobian as M0^T .. Mn-1^T g
+      for (int st = 0; st < a.n_init; ++st) {
+        const double* m = a.init_stages + 12 * st;
+        double h[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          double sum = 0.0;
+#pragma unroll
+          for (int r = 0; r < D; ++r) sum += m[r * D + c] * gd[r];
+          h[c] = sum;
+        }
+#pragma unroll
+        for (int c = 0; c < D; ++c) gd[c] = h[c];
+      }
     }
     if constexpr (MK == 2) {
       // B-spline Jacobian: J[d][node + d*pdim] = weight(node) (BSplTf:594-650)

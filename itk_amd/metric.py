@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _native as N
 from .geometry import ImageGeometry
-from .transforms import TF_AFFINE, TF_BSPLINE
+from .transforms import TF_AFFINE, TF_BSPLINE, TF_COMPOSITE
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -355,11 +355,26 @@ class MattesMutualInformationImageToImageMetricv4:
             N.check(L.mmi_set_virtual_points(ctx, d, None, 0))
         if self._fixed_tf is not None:
             self._push_transform(L.mmi_set_fixed_transform, self._fixed_tf)
-        self._push_transform(L.mmi_set_moving_transform, self._moving_tf)
+        self._push_moving_transform(self._moving_tf)
         N.check(L.mmi_initialize(ctx))
         self._initialized = True
 
+    def _push_moving_transform(self, tf):
+        """The moving transform; a composite as its optimized entry plus the
+        earlier entries as matrix-offset stages."""
+        if tf.kind == TF_COMPOSITE:
+            m, o = tf.initial_stages()
+            m = np.ascontiguousarray(m, dtype=np.float64)
+            o = np.ascontiguousarray(o, dtype=np.float64)
+            N.check(self._L.mmi_set_moving_initial_transforms(self._ctx, m.shape[0], _dptr(m), _dptr(o)))
+            tf = tf.optimized
+        else:
+            N.check(self._L.mmi_set_moving_initial_transforms(self._ctx, 0, None, None))
+        self._push_transform(self._L.mmi_set_moving_transform, tf)
+
     def _push_transform(self, setter, tf):
+        if tf.kind == TF_COMPOSITE:
+            raise ValueError("the fixed transform cannot be a composite")
         fp = transform_fixed_parameters(tf)
         p = np.ascontiguousarray(tf.get_parameters(), dtype=np.float64)
         N.check(setter(self._ctx, tf.kind, _dptr(fp), fp.size, _dptr(p), p.size))

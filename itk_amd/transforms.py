@@ -5,6 +5,9 @@
   followed by the translation; the fixed parameters are the centre.  The
   offset is derived as in ``MatrixOffsetTransformBase::ComputeOffset``
   (``Modules/Core/Transform/include/itkMatrixOffsetTransformBase.hxx:615-631``).
+* ``CompositeTransform`` follows ``itk::CompositeTransform`` as
+  ``ImageRegistrationMethodv4`` sets it up (a moving initial transform, then
+  the optimized one; only the most recent entry is optimized).
 * ``BSplineTransform`` follows ``itk::BSplineTransform<double, D, 3>``:
   parameters are the coefficient images one dimension after another, each
   x-fastest over the grid of ``mesh + 3`` nodes; the grid geometry comes from
@@ -20,6 +23,7 @@ from .geometry import ImageGeometry
 TF_IDENTITY = 0
 TF_AFFINE = 1
 TF_BSPLINE = 2
+TF_COMPOSITE = 3  # Python-side only: the C ABI takes its entries separately
 
 
 class AffineTransform:
@@ -145,3 +149,65 @@ class BSplineTransform:
             self.parameters = self.parameters + update
         else:
             self.parameters = self.parameters + update * factor
+
+
+class CompositeTransform:
+    """itk::CompositeTransform with only its most recent transform optimized
+    (``SetOnlyMostRecentTransformToOptimizeOn``, as
+    ImageRegistrationMethodv4 builds it: itkImageRegistrationMethodv4.hxx:344-355).
+    A point goes through the queue's last transform first, then the earlier
+    ones (itkCompositeTransform.hxx:60-71); the parameters are the last
+    transform's.  The entries before the last are affine (the metric takes
+    them as matrix-offset stages)."""
+    kind = TF_COMPOSITE
+
+    def __init__(self, dim: int, transforms=()):
+        self.dim = dim
+        self.transforms = []
+        for t in transforms:
+            self.add_transform(t)
+
+    def add_transform(self, tf):
+        if tf.dim != self.dim:
+            raise ValueError("transform dimension differs from the composite's")
+        if tf.kind == TF_COMPOSITE:  # FlattenTransformQueue
+            for t in tf.transforms:
+                self.add_transform(t)
+            return
+        self.transforms.append(tf)
+
+    @property
+    def optimized(self):
+        if not self.transforms:
+            raise ValueError("the composite transform is empty")
+        return self.transforms[-1]
+
+    def initial_stages(self):
+        """(matrices [n, D, D], offsets [n, D]) of the entries before the
+        optimized one, in queue order."""
+        stages = self.transforms[:-1]
+        for t in stages:
+            if t.kind != TF_AFFINE:
+                raise ValueError("only affine transforms may precede the optimized transform")
+        m = np.array([t.matrix for t in stages]).reshape(len(stages), self.dim, self.dim)
+        o = np.array([t.offset for t in stages]).reshape(len(stages), self.dim)
+        return m, o
+
+    @property
+    def number_of_parameters(self) -> int:
+        return self.optimized.number_of_parameters
+
+    def set_parameters(self, p):
+        self.optimized.set_parameters(p)
+
+    def get_parameters(self) -> np.ndarray:
+        return self.optimized.get_parameters()
+
+    def update_transform_parameters(self, update, factor: float = 1.0):
+        self.optimized.update_transform_parameters(update, factor)
+
+    def transform_point(self, x) -> np.ndarray:
+        y = np.asarray(x, dtype=np.float64)
+        for t in reversed(self.transforms):
+            y = t.transform_point(y)
+        return y

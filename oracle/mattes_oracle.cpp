@@ -455,6 +455,13 @@ void process_sample(const Problem& pb, const double* vp, int64_t lin, ThreadStat
   double mp[3];
   BSplineEval be;
   tf_point(pb.mtf, D, vp, mp, &be);
+  // CompositeTransform::TransformPoint: the earlier queue entries after it,
+  // in reverse queue order (itkCompositeTransform.hxx:60-71)
+  for (int st = pb.mtf->n_initial - 1; st >= 0; --st) {
+    double q[3];
+    tf_point(&pb.mtf->initial[st], D, mp, q, nullptr);
+    for (int i = 0; i < D; ++i) mp[i] = q[i];
+  }
   if (pb.mmask.g && !pb.mmask.inside(mp)) return;
   point_to_cindex(pb.moving.g, mp, ci);
   if (!inside_buffer(pb.moving.g, ci)) return;
@@ -510,12 +517,63 @@ void process_sample(const Problem& pb, const double* vp, int64_t lin, ThreadStat
     for (int d = 0; d < D; ++d) v[d] = vp[d] - pb.mtf->center[d];
   const int64_t pdim = pb.mtf->kind == ORC_TF_BSPLINE ? grid_nodes(&pb.mtf->grid) : 0;
 
+  // Composite moving transform: ComputeJacobianWithRespectToParameters of the
+  // CompositeTransform (itkCompositeTransform.hxx:467-560): the moving
+  // transform's Jacobian at the virtual point, left-multiplied by the
+  // position Jacobian (the matrix) of each earlier queue entry, last to
+  // first; then the threader's inner product with the gradient
+  // (MattesT:361-371).  Only the Jacobian's non-zero columns are formed (the
+  // others stay zero through the products).
+  std::vector<double> jcomp;  // [D][ncol]
+  std::vector<int64_t> jcol;  // parameter index of each column
+  if (deriv && pb.mtf->n_initial > 0) {
+    if (pb.mtf->kind == ORC_TF_AFFINE) {
+      for (int64_t c = 0; c < D * D + D; ++c) jcol.push_back(c);
+    } else if (be.inside) {
+      for (int k = 0; k < be.nw; ++k)
+        for (int d = 0; d < D; ++d) jcol.push_back(be.node[k] + d * pdim);
+    }
+    const size_t nc = jcol.size();
+    jcomp.assign((size_t)D * nc, 0.0);
+    for (size_t c = 0; c < nc; ++c) {
+      for (int r = 0; r < D; ++r) {
+        double j = 0.0;
+        if (pb.mtf->kind == ORC_TF_AFFINE) {
+          const int64_t mu = jcol[c];
+          if (mu < D * D) j = (mu / D == r) ? v[mu % D] : 0.0;
+          else j = (mu - D * D == r) ? 1.0 : 0.0;
+        } else {
+          j = ((int)(c % D) == r) ? be.weights[c / D] : 0.0;
+        }
+        jcomp[r * nc + c] = j;
+      }
+    }
+    for (int st = pb.mtf->n_initial - 1; st >= 0; --st) {
+      const double* M = pb.mtf->initial[st].matrix;
+      for (size_t c = 0; c < nc; ++c) {
+        double temp[3];
+        for (int r = 0; r < D; ++r) {
+          temp[r] = 0.0;
+          for (int k = 0; k < D; ++k) temp[r] += M[r * D + k] * jcomp[k * nc + c];
+        }
+        for (int r = 0; r < D; ++r) jcomp[r * nc + c] = temp[r];
+      }
+    }
+  }
+
   while (pdfMovingIndex <= pdfMovingIndexMax) {
     pdfRow[pdfMovingIndex] += bspline3(arg);
     if (deriv) {
       const double db = bspline3_deriv(arg);
       double* dp = ts.dpdf.data() + (fIdx * B + pdfMovingIndex) * pb.P;
-      if (pb.mtf->kind == ORC_TF_AFFINE) {
+      if (pb.mtf->n_initial > 0) {
+        const size_t nc = jcol.size();
+        for (size_t c = 0; c < nc; ++c) {
+          double inner = 0.0;
+          for (int d = 0; d < D; ++d) inner += jcomp[d * nc + c] * grad[d];
+          dp[jcol[c]] += inner * db;
+        }
+      } else if (pb.mtf->kind == ORC_TF_AFFINE) {
         // innerProduct = sum_dim J[dim][mu] * grad[dim]; only one row of J is
         // non-zero per mu, so the sum equals that single product exactly.
         for (int b = 0; b < D; ++b)

@@ -34,7 +34,7 @@ typedef struct {
 
 enum { ORC_TF_IDENTITY = 0, ORC_TF_AFFINE = 1, ORC_TF_BSPLINE = 2 };
 
-typedef struct {
+typedef struct orc_transform_s {
   int32_t kind;
   int32_t pad_;
   double matrix[9];  /* affine: row-major */
@@ -42,6 +42,13 @@ typedef struct {
   double center[3];  /* affine: Jacobian is taken about this point */
   orc_geometry grid; /* B-spline coefficient grid, size = mesh + 3 */
   const double* coeffs; /* B-spline parameters, dim * prod(grid.size), dimension-major */
+  /* moving transform of a CompositeTransform whose only optimized transform
+   * is this one (its queue's last entry): the n_initial affine transforms
+   * before it, in queue order (a point goes through this one, then
+   * initial[n_initial - 1] .. initial[0]); NULL / 0 for none */
+  const struct orc_transform_s* initial;
+  int32_t n_initial;
+  int32_t pad2_;
 } orc_transform;
 
 typedef struct {

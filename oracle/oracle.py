@@ -13,7 +13,7 @@ import subprocess
 import numpy as np
 
 from itk_amd.geometry import CGeometry, ImageGeometry
-from itk_amd.transforms import TF_AFFINE, TF_BSPLINE, TF_IDENTITY
+from itk_amd.transforms import TF_AFFINE, TF_BSPLINE, TF_COMPOSITE, TF_IDENTITY
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libmattes_oracle.so")
@@ -29,6 +29,9 @@ class CTransform(ctypes.Structure):
         ("center", ctypes.c_double * 3),
         ("grid", CGeometry),
         ("coeffs", ctypes.POINTER(ctypes.c_double)),
+        ("initial", ctypes.c_void_p),
+        ("n_initial", ctypes.c_int32),
+        ("pad2_", ctypes.c_int32),
     ]
 
 
@@ -78,9 +81,9 @@ class CSampling(ctypes.Structure):
 
 
 def build(force: bool = False) -> str:
-    """Compile the oracle with its Makefile (g++ only)."""
-    if force or not os.path.exists(_LIB_PATH):
-        subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
+    """Compile the oracle with its Makefile (g++ only; make rebuilds it when
+    a source is newer than the library)."""
+    subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
     return _LIB_PATH
 
 
@@ -161,6 +164,19 @@ def make_transform(tf) -> tuple:
         c.kind = TF_IDENTITY
         return c, keep
     d = tf.dim
+    if tf.kind == TF_COMPOSITE:
+        # the optimized (last) entry, with the earlier ones as affine stages
+        c, keep = make_transform(tf.transforms[-1])
+        stages = (CTransform * len(tf.transforms[:-1]))()
+        for i, t in enumerate(tf.transforms[:-1]):
+            if t.kind != TF_AFFINE:
+                raise ValueError("the oracle takes affine transforms before the optimized one")
+            si, _ = make_transform(t)
+            stages[i] = si
+        keep.append(stages)
+        c.initial = ctypes.cast(stages, ctypes.c_void_p)
+        c.n_initial = len(tf.transforms) - 1
+        return c, keep
     c.kind = tf.kind
     if tf.kind == TF_AFFINE:
         m, off = tf.matrix, tf.offset

@@ -18,7 +18,10 @@ def build() -> str:
     return BIN
 
 
-def run(tmpdir, fixed, moving, geom, kind, fixed_params, param_sets, bins=50, use_filter=True, timeout=120):
+def run(tmpdir, fixed, moving, geom, kind, fixed_params, param_sets, bins=50, use_filter=True, timeout=120,
+        initial=()):
+    """initial: AffineTransforms before the moving transform in the
+    composite's queue."""
     d = geom.dim
     pixel = "f64" if fixed.dtype == np.float64 else "f32"
     np.ascontiguousarray(fixed).tofile(os.path.join(tmpdir, "fixed.raw"))
@@ -29,6 +32,9 @@ def run(tmpdir, fixed, moving, geom, kind, fixed_params, param_sets, bins=50, us
              "%d %s" % (len(param_sets[0]), fmt(param_sets[0])),
              "%d %d %d" % (bins, 1 if use_filter else 0, len(param_sets))]
     lines += [fmt(p) for p in param_sets[1:]]
+    if initial:
+        lines.append("%d" % len(initial))
+        lines += ["%s %s" % (fmt(t.center), fmt(t.get_parameters())) for t in initial]
     with open(os.path.join(tmpdir, "spec.txt"), "w") as f:
         f.write("\n".join(lines) + "\n")
     r = subprocess.run([BIN, str(tmpdir)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,

@@ -10,6 +10,9 @@
 //                  kind(affine|bspline) n_fixed fixed[n_fixed] n_params params[n_params]
 //                  bins use_filter(0|1) evaluations
 //                  then evaluations - 1 parameter vectors of n_params values
+//                  then optionally n_initial and, per initial affine transform,
+//                  centre[dim] params[dim*dim+dim] (the composite's entries
+//                  before the moving transform, in queue order)
 // <dir>/fixed.raw, <dir>/moving.raw: pixels in buffer order.
 #include <cstdio>
 #include <fstream>
@@ -26,6 +29,7 @@ struct Spec {
   std::string pixel, kind;
   std::vector<double> size, spacing, origin, fixed, params;
   std::vector<std::vector<double>> more;
+  std::vector<std::vector<double>> initial_center, initial_params;
   int bins = 50, use_filter = 1, evaluations = 1;
 };
 
@@ -52,6 +56,17 @@ Spec read_spec(const std::string& dir) {
     s.more.push_back(p);
   }
   if (!in) throw std::runtime_error("bad spec.txt in " + dir);
+  size_t n_init = 0;
+  if (in >> n_init) {
+    for (size_t i = 0; i < n_init; ++i) {
+      std::vector<double> c, p;
+      vec(c, s.dim);
+      vec(p, s.dim * s.dim + s.dim);
+      s.initial_center.push_back(c);
+      s.initial_params.push_back(p);
+    }
+    if (!in) throw std::runtime_error("bad initial transforms in " + dir + "/spec.txt");
+  }
   return s;
 }
 
@@ -104,7 +119,17 @@ int run(const std::string& dir, const Spec& s) {
   tf->SetParameters(p);
   // the registration method's output transform: a CompositeTransform holding it
   auto composite = itk::CompositeTransform<double, D>::New();
+  for (size_t i = 0; i < s.initial_center.size(); ++i) {
+    auto a = itk::AffineTransform<double, D>::New();
+    itk::Array<double> c(D), q(D * D + D);
+    for (unsigned int k = 0; k < D; ++k) c[k] = s.initial_center[i][k];
+    for (unsigned int k = 0; k < D * D + D; ++k) q[k] = s.initial_params[i][k];
+    a->SetFixedParameters(c);
+    a->SetParameters(q);
+    composite->AddTransform(a.GetPointer());
+  }
   composite->AddTransform(tf.GetPointer());
+  composite->SetOnlyMostRecentTransformToOptimizeOn();
   auto metric = MetricType::New();
   metric->SetFixedImage(fixed.GetPointer());
   metric->SetMovingImage(moving.GetPointer());

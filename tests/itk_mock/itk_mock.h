@@ -277,18 +277,52 @@ class Transform : public TransformBaseTemplate<T> {
   using Pointer = SmartPointer<Transform>;
 };
 
-// AffineTransform: parameters = matrix row-major then translation, fixed
-// parameters = centre (itkMatrixOffsetTransformBase.hxx:420-500).
+// MatrixOffsetTransformBase: y = matrix * x + offset; parameters = matrix
+// row-major then translation, fixed parameters = centre, offset from
+// ComputeOffset (itkMatrixOffsetTransformBase.hxx:420-500, 615-631).
+template <typename T = double, unsigned int NIn = 3, unsigned int NOut = 3>
+class MatrixOffsetTransformBase : public Transform<T, NIn, NOut> {
+ public:
+  using MatrixType = Matrix<T, NOut, NIn>;
+  using OutputVectorType = FixedArray<T, NOut>;
+  const MatrixType& GetMatrix() const { return m_Matrix; }
+  const OutputVectorType& GetOffset() const { return m_Offset; }
+  void SetParameters(const typename TransformBaseTemplate<T>::ParametersType& p) override {
+    TransformBaseTemplate<T>::SetParameters(p);
+    Update();
+  }
+  void SetFixedParameters(const typename TransformBaseTemplate<T>::FixedParametersType& p) override {
+    TransformBaseTemplate<T>::SetFixedParameters(p);
+    Update();
+  }
+
+ private:
+  void Update() {
+    const auto& p = this->m_Parameters;
+    const auto& c = this->m_FixedParameters;
+    if (p.Size() != NOut * NIn + NOut || c.Size() != NIn) return;
+    for (unsigned int i = 0; i < NOut; ++i)
+      for (unsigned int j = 0; j < NIn; ++j) m_Matrix[i][j] = p[i * NIn + j];
+    for (unsigned int i = 0; i < NOut; ++i) {
+      T off = p[NOut * NIn + i] + c[i];
+      for (unsigned int j = 0; j < NIn; ++j) off -= m_Matrix[i][j] * c[j];
+      m_Offset[i] = off;
+    }
+  }
+  MatrixType m_Matrix;
+  OutputVectorType m_Offset;
+};
+
 template <typename T = double, unsigned int D = 3>
-class AffineTransform : public Transform<T, D, D> {
+class AffineTransform : public MatrixOffsetTransformBase<T, D, D> {
  public:
   using Pointer = SmartPointer<AffineTransform>;
   static Pointer New() {
     Pointer p(new AffineTransform());
     typename TransformBaseTemplate<T>::ParametersType q(D * D + D);
     for (unsigned int i = 0; i < D; ++i) q[i * D + i] = 1;
-    p->SetParameters(q);
     p->SetFixedParameters(Array<double>(D));
+    p->SetParameters(q);
     return p;
   }
 };
@@ -315,7 +349,15 @@ class CompositeTransform : public Transform<T, D, D> {
   using Pointer = SmartPointer<CompositeTransform>;
   using TransformType = Transform<T, D, D>;
   static Pointer New() { return Pointer(new CompositeTransform()); }
-  void AddTransform(TransformType* t) { m_Queue.push_back(typename TransformType::Pointer(t)); }
+  // a new entry is optimized (as ITK's PushBackTransform sets it)
+  void AddTransform(TransformType* t) {
+    m_Queue.push_back(typename TransformType::Pointer(t));
+    m_Optimize.push_back(true);
+  }
+  void SetOnlyMostRecentTransformToOptimizeOn() {
+    for (size_t i = 0; i < m_Optimize.size(); ++i) m_Optimize[i] = i + 1 == m_Optimize.size();
+  }
+  bool GetNthTransformToOptimize(SizeValueType n) const { return m_Optimize[n]; }
   SizeValueType GetNumberOfTransforms() const { return m_Queue.size(); }
   const TransformType* GetNthTransformConstPointer(SizeValueType n) const { return m_Queue[n].GetPointer(); }
   const typename TransformBaseTemplate<T>::ParametersType& GetParameters() const override {
@@ -328,6 +370,7 @@ class CompositeTransform : public Transform<T, D, D> {
 
  private:
   std::vector<typename TransformType::Pointer> m_Queue;
+  std::vector<bool> m_Optimize;
 };
 
 template <unsigned int D>

@@ -381,3 +381,79 @@ def test_recursive_gaussian_borders_extend_the_edge(oracle_mod):
             assert rel(gx, refg) >= 0.05
             if mode == "constant":
                 assert rel(sm, ref0) >= 0.05
+
+
+def _rotation_stage(geom, angle, scale, shift):
+    """An affine transform rotating about the image centre (x-y plane),
+    scaled slightly, then shifted: a moving initial transform."""
+    c = geom.index_to_point([(s - 1) / 2.0 for s in geom.size])
+    ca, sa = np.cos(angle), np.sin(angle)
+    m = np.eye(3)
+    m[:2, :2] = [[ca, -sa], [sa, ca]]
+    m = m * scale
+    tf = affine_for(geom)
+    tf.center = np.asarray(c, dtype=np.float64)
+    tf.set_parameters(np.concatenate([m.ravel(), shift]))
+    return tf
+
+
+def test_composite_with_identity_stage_is_the_moving_transform(oracle_mod):
+    """A composite whose earlier entry is the identity affine maps every point
+    and forms every Jacobian column with exact products by 1 and 0: value and
+    derivative equal the moving transform's alone, bit for bit."""
+    from itk_amd.transforms import CompositeTransform
+
+    fixed, moving, geom = blob_problem((30, 28, 26), 41)
+    for tf in (affine_for(geom, affine_params(3, np.random.default_rng(42), scale=1.0)),
+               bspline_for(geom, (3, 3, 2), seed=43, sigma=0.6)):
+        ident = affine_for(geom)
+        comp = CompositeTransform(3, [ident, tf])
+        a = oracle_mod.mattes(fixed, geom, moving, geom, tf, want_samples=True)
+        b = oracle_mod.mattes(fixed, geom, moving, geom, comp, want_samples=True)
+        assert a["value"] == b["value"] and a["n_valid"] == b["n_valid"]
+        assert np.array_equal(a["derivative"], b["derivative"])
+        assert np.array_equal(a["moving_bin"], b["moving_bin"])
+
+
+def test_composite_maps_through_the_earlier_entry(oracle_mod):
+    """The composite's samples are the moving transform's points mapped by
+    the earlier entry: with a pure translation stage of a whole number of
+    voxels on a moving image shifted the same way, every sample's value, bin
+    and the metric value are those of the moving transform alone."""
+    from itk_amd.transforms import CompositeTransform
+
+    fixed, moving, geom = blob_problem((30, 28, 26), 44)
+    tf = affine_for(geom, affine_params(3, np.random.default_rng(45), scale=0.5))
+    shift = np.array([3.0, -2.0, 4.0]) * np.asarray(geom.spacing)
+    stage = affine_for(geom)
+    stage.set_parameters(np.concatenate([np.eye(3).ravel(), shift]))
+    moved = ImageGeometry(size=geom.size, origin=tuple(np.asarray(geom.origin) + shift), spacing=geom.spacing,
+                          direction=geom.direction)
+    a = oracle_mod.mattes(fixed, geom, moving, geom, tf, want_samples=True, derivative=False)
+    b = oracle_mod.mattes(fixed, geom, moving, moved, CompositeTransform(3, [stage, tf]), want_samples=True,
+                          derivative=False)
+    assert a["n_valid"] == b["n_valid"]
+    assert np.array_equal(a["moving_bin"], b["moving_bin"]) and np.array_equal(a["fixed_bin"], b["fixed_bin"])
+    assert b["value"] == pytest.approx(a["value"], rel=1e-12)
+
+
+def test_composite_affine_finite_difference(oracle_mod):
+    """The composite's derivative (stage matrix times the affine Jacobian) is
+    the derivative of its value."""
+    from itk_amd.transforms import CompositeTransform
+
+    fixed, moving, geom, virt = _ramp_problem((32, 30, 28), 46)
+    tf = affine_for(geom, affine_params(3, np.random.default_rng(47), scale=0.5))
+    comp = CompositeTransform(3, [_rotation_stage(geom, 0.08, 1.03, np.array([0.4, -0.3, 0.2])), tf])
+    _fd_check(oracle_mod, fixed, moving, geom, virt, comp, range(12), 1e-6)
+
+
+def test_composite_bspline_finite_difference(oracle_mod):
+    from itk_amd.transforms import CompositeTransform
+
+    fixed, moving, geom, virt = _ramp_problem((28, 28, 28), 48)
+    tf = bspline_for(geom, (2, 2, 2), seed=49, sigma=0.5)
+    comp = CompositeTransform(3, [_rotation_stage(geom, -0.06, 0.98, np.array([-0.3, 0.2, 0.1])), tf])
+    r = oracle_mod.mattes(fixed, geom, moving, geom, comp, virtual_geom=virt)
+    d = r["derivative"]
+    _fd_check(oracle_mod, fixed, moving, geom, virt, comp, np.argsort(-np.abs(d))[:10], 1e-5)
