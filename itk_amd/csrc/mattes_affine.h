@@ -155,6 +155,20 @@ __host__ __device__ inline void rows_thresholds_d(const int32_t* m_start, const 
     R.span_out[k] = (unsigned long long)(hi_out - lo_out);
     R.lo_in[k] = lo_in;
     R.span_in[k] = hi_in > lo_in ? (unsigned long long)(hi_in - lo_in) : 0ull;
+    if (R.cd_on) {
+      // central differences: both half-step points inside iff the index is in
+      // [start, end - 1); the reference rounds the offset point and its index
+      // once more each (a few ulp of S, < e / 8), so the band is 2 e wide
+      const long long ce2 = (long long)ceil(2.0 * ew * 4294967296.0) + 1;
+      const long long a0 = (long long)m_start[k] * 4294967296ll;
+      const long long a1 = (long long)(m_start[k] + m_size[k] - 1) * 4294967296ll;
+      const long long in_lo = a0 + ce2, in_hi = a1 - ce2;
+      const long long out_lo = a0 - ce2, out_hi = a1 + ce2;
+      R.cd_lo_in[k] = in_lo;
+      R.cd_span_in[k] = in_hi > in_lo ? (unsigned long long)(in_hi - in_lo) : 0ull;
+      R.cd_lo_out[k] = out_lo;
+      R.cd_span_out[k] = (unsigned long long)(out_hi - out_lo);
+    }
   }
   *e = ew;
 }

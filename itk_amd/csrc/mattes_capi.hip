@@ -294,6 +294,7 @@ struct mmi_context {
   DevBuf exc_count, exc_list;  // fast-path exception lists
   DevBuf exc_res;              // B-spline: exact results of the listed samples
   DevBuf cls;                  // pass 1's byte per sample for pass 2
+  DevBuf cd_defer;             // central differences on the fast pass 2: per-wavefront deferred samples
   DevBuf ftab;                               // fast path: per-axis fixed cells (coarse levels)
   // grow-only work space of the preparation filters (no device frees, which
   // would synchronise the whole device, while other contexts run)
@@ -849,6 +850,25 @@ void setup_rows_layout(const mmi_context* ctx, FastRows& R) {
   R.dxseg = make_fastdiv((uint32_t)nxseg);
   R.dyblk = make_fastdiv((uint32_t)nyblk);
   R.on = 1;
+  // central differences on the fast pass 2: a moving physical-to-index map
+  // with exact zeros off the diagonal (a half-spacing step along a physical
+  // axis then moves one index coordinate only, by 1/2 to within an ulp)
+  if (ctx->gradient_source != MMI_GRADIENT_IMAGE) {
+    bool diag = true;
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) {
+        const double v = m.physical_to_index[i * D + j];
+        if (i != j && v != 0.0) diag = false;
+        if (i == j && !(std::fabs(std::fabs(v * m.spacing[i]) - 1.0) <= 4.0 * 2.220446049250313e-16)) diag = false;
+      }
+    for (int i = 0; i < D; ++i)
+      if (!(m.spacing[i] > 1e-12)) diag = false;
+    if (diag) {
+      R.cd_on = 1;
+      for (int i = 0; i < D; ++i)
+        R.cd_scale[i] = (float)((m.physical_to_index[i * D + i] > 0.0 ? 1.0 : -1.0) / m.spacing[i]);
+    }
+  }
 }
 
 bool setup_fast(mmi_context* ctx, FastArgs* F) {
@@ -1773,6 +1793,26 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
     }
     fa.partials = b.partials;
   }
+  if (fast && want_derivative && fa.rows.on && fa.rows.cd_on) {
+    if (sb) {
+      fa.rows.cd_on = 0;  // the static entry takes the gradient image only
+    } else {
+      // a wavefront's share of the walks (rows_range): its items times the
+      // samples of an item bound the samples it can defer
+      const FastRows& R = fa.rows;
+      const uint64_t waves = (uint64_t)ctx->grid * 4;
+      uint64_t per_wave = (R.items + waves - 1) / waves;
+      if (fa.xcd_chunks) {
+        const uint64_t chunk = (R.items + kXcd - 1) / kXcd;
+        const uint64_t wx = (uint64_t)(ctx->grid / kXcd) * 4;
+        per_wave = std::max<uint64_t>(per_wave, (chunk + wx - 1) / wx);
+      }
+      const uint64_t cap = per_wave * 64 * (uint64_t)R.walk;
+      MMI_HIP(ctx->cd_defer.ensure(sizeof(uint32_t) * cap * waves));
+      fa.cd_defer = ctx->cd_defer.as<uint32_t>();
+      fa.cd_wave_cap = (uint32_t)cap;
+    }
+  }
   return MMI_OK;
 }
 
@@ -1850,7 +1890,8 @@ retry:
   // pass 1 leaves a byte per sample (fixed bin or skip) for pass 2, which
   // then also takes the moving values from the texels it gathers for the
   // gradient: their value channel must be the level's moving image
-  if (fast && want_derivative && B <= 253 && !(ctx->flags & MMI_FLAG_PASS2_RECLASSIFY) && ctx->texel_values_level) {
+  if (fast && want_derivative && B <= 253 && !(ctx->flags & MMI_FLAG_PASS2_RECLASSIFY) &&
+      (ctx->texel_values_level || (fa.rows.on && fa.rows.cd_on))) {
     if (b.cls) {
       fa.cls = b.cls;
     } else {
@@ -1880,9 +1921,11 @@ retry:
     if (int rc = bspline_separable(ctx, a.virt, D, tabs, &separable)) return rc;
   if (want_derivative) {
     if (time_passes) MMI_HIP(hipEventRecord(ctx->ev[2], st));
-    // the fast pass 2 interpolates the gradient image; with central
-    // differences pass 2 runs on the exact path after the fast pass 1
-    const bool fast2 = fast && ctx->gradient_source == MMI_GRADIENT_IMAGE;
+    // the fast pass 2 interpolates the gradient image, or takes central
+    // differences over row walks after a pass 1 that left its byte per
+    // sample; other central differences run the exact pass 2 after the fast
+    // pass 1
+    const bool fast2 = fast && (ctx->gradient_source == MMI_GRADIENT_IMAGE || (fa.rows.on && fa.rows.cd_on && fa.cls));
     if (fast2 && fa.mk == 2) {
       // B-spline fast path: per-sample factors, then the separable contraction
       MMI_HIP(ctx->bs_g.ensure(sizeof(float) * D * n));

@@ -30,8 +30,6 @@
 
 namespace mmi {
 
-constexpr int kXcd = 8;
-
 // Samples of local workgroup lb of lg (one evaluation's share of a launch).
 template <int D>
 __device__ __forceinline__ void run_range(const FastArgs& f, uint32_t lb, uint32_t lg, uint32_t& begin, uint32_t& end,
@@ -682,9 +680,21 @@ __device__ __forceinline__ void add_moments(double* acc, float w, float3 g, cons
   }
 }
 
+template <int D>
+__device__ __forceinline__ void add_moments_d(double* acc, float w, const double* g, const int32_t* idx) {
+#pragma unroll
+  for (int b = 0; b < D; ++b) {
+    const double gw = (double)w * g[b];
+#pragma unroll
+    for (int j = 0; j < D; ++j) acc[b * D + j] = fma(gw, (double)idx[j], acc[b * D + j]);
+    acc[D * D + b] += gw;
+  }
+}
+
 // Exception samples of one pass-1 workgroup, in sample order (bitonic sort of
-// the list in LDS), summed in a fixed tree: deterministic.
-template <int D, typename P, int MK>
+// the list in LDS), summed in a fixed tree: deterministic.  CD: the gradient
+// by central differences of the moving image (else the gradient image).
+template <int D, typename P, int MK, bool CD = false>
 __device__ __forceinline__ void exceptions_pass2_body(const EvalArgs& a, const FastArgs& f, uint32_t lb,
                                                       uint32_t lg) {
   __shared__ uint32_t keys[256];
@@ -722,8 +732,16 @@ __device__ __forceinline__ void exceptions_pass2_body(const EvalArgs& a, const F
       if (eval_sample<D, 1, P>(a, vp, o)) {
         const float w = parzen_deriv_weight((float)((double)(o.mbin - 1) - o.mterm),
                                             f.pratio_f + o.fbin * a.bins + (o.mbin - 1));
-        if (w != 0.0f)
-          add_moments<D>(acc, w, interp_gradient3<D, P>(static_cast<const typename P::texel*>(a.texels), o.cell), idx);
+        if (w != 0.0f) {
+          if constexpr (CD) {
+            double g[3] = {0.0, 0.0, 0.0};
+            central_difference<D, P>(a, o.mp, g);
+            add_moments_d<D>(acc, w, g, idx);
+          } else {
+            add_moments<D>(acc, w, interp_gradient3<D, P>(static_cast<const typename P::texel*>(a.texels), o.cell),
+                           idx);
+          }
+        }
       }
     }
   }
@@ -892,6 +910,187 @@ __device__ __forceinline__ void pass2_rows_body(const FastArgs& f, const EvalArg
   exceptions_pass2_body<D, P, 1>(a, f, lb, lg);
 }
 
+// Pass 2 over row walks with central differences of the moving image
+// (CentralDifferenceImageFunction::EvaluateSpecialized,
+// itkCentralDifferenceImageFunction.hxx:252-318) on a moving geometry with a
+// diagonal physical-to-index map.  Along axis b the reference's points at
+// -/+ half a spacing are the sample's continuous index -/+ 1/2 along index
+// axis b (the other coordinates unchanged); their cells along b take the
+// positions P0, P1, P3 with P1 the upper (t >= 1/2) or lower (t < 1/2) corner
+// of the sample's own cell, so the sample's 2^D corner values and 2^(D-1)
+// more per axis give the value (interpolated exactly as pass 1 does: the same
+// bins) and the D gradient components, each (v+ - v-) * sign / spacing.  A
+// sample whose half-step points are not certainly inside or outside the
+// buffer is deferred to the wavefront's region of cd_defer and taken after
+// the walks on the exact FP64 path (eval_sample, central_difference), each
+// wavefront's in its own order, so the results do not depend on timing.
+template <int D, typename P>
+__device__ __forceinline__ void pass2_rows_cd_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
+  extern __shared__ float lds_f32[];
+  const int B = f.bins;
+  const int BB = B * B;
+  double* scratch = reinterpret_cast<double*>(lds_f32);  // 4 wavefronts x 12 doubles (128 floats)
+  float* pr = lds_f32 + 128;
+  for (int i = threadIdx.x; i < BB; i += blockDim.x) pr[i] = f.pratio_f[i];
+  __syncthreads();
+  constexpr int PA = D * D + D;
+  double acc[PA];
+#pragma unroll
+  for (int i = 0; i < PA; ++i) acc[i] = 0.0;
+  const FastRows& R = f.rows;
+  const typename P::pixel* img = static_cast<const typename P::pixel*>(f.moving_px);
+  __shared__ uint32_t cd_n[4];  // deferred samples per wavefront
+  const uint32_t wave = threadIdx.x >> 6;
+  uint32_t* defer = f.cd_defer + (int64_t)(lb * 4u + wave) * f.cd_wave_cap;
+  if ((threadIdx.x & 63u) == 0) cd_n[wave] = 0u;
+  uint32_t it, iend, istep;
+  rows_range(f, lb, lg, it, iend, istep);
+  for (; it < iend; it += istep) {
+    RowWalk<D> w;
+    rows_start<D>(f, it, w);
+    float sg[3] = {0.0f, 0.0f, 0.0f}, sy[3] = {0.0f, 0.0f, 0.0f};
+    float yf = (float)w.y;
+    for (int j = 0; j < w.rows; ++j, rows_step<D>(f, w), yf += 1.0f) {
+      if (!w.active) continue;
+      const uint32_t cb = f.cls[w.s];
+      if (cb >= kClsListed) continue;
+      // the sample's cell and weights, as rows_classify
+      int32_t I[3] = {0, 0, 0}, fl[3] = {0, 0, 0};
+      uint32_t off[3] = {0u, 0u, 0u}, stride[3] = {1u, (uint32_t)f.msy, (uint32_t)f.msz};
+      float t[3] = {0.0f, 0.0f, 0.0f}, th[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const long long q = w.q[k];
+        I[k] = (int32_t)(q >> 32);
+        fl[k] = min(max(I[k], f.mstart[k]), f.mend[k]);
+        const bool act = (uint32_t)(I[k] - f.mstart[k]) < (uint32_t)(f.mend[k] - f.mstart[k]);
+        off[k] = act ? stride[k] : 0u;
+        t[k] = (float)(uint32_t)q * 2.3283064365386963e-10f;
+        th[k] = (float)((uint32_t)q ^ 0x80000000u) * 2.3283064365386963e-10f;  // fraction of the index + 1/2
+      }
+      uint32_t cbase = (uint32_t)(fl[0] - f.mstart[0]) + __umul24((uint32_t)f.msy, (uint32_t)(fl[1] - f.mstart[1]));
+      if (D == 3) cbase += __umul24((uint32_t)f.msz, (uint32_t)(fl[2] - f.mstart[2]));
+      // corner values c[i] at cbase + (i&1) off0 + (i>>1&1) off1 + (i>>2&1) off2
+      float c[8];
+#pragma unroll
+      for (int i = 0; i < (1 << D); ++i)
+        c[i] = P::ld32(img, cbase + ((i & 1) ? off[0] : 0u) + ((i & 2) ? off[1] : 0u) + ((i & 4) ? off[2] : 0u));
+      // the value with fast_interp's operations (pass 1's bits)
+      float v;
+      {
+        const float vx00 = fmaf(c[1] - c[0], t[0], c[0]);
+        const float vx10 = fmaf(c[3] - c[2], t[0], c[2]);
+        v = fmaf(vx10 - vx00, t[1], vx00);
+        if (D == 3) {
+          const float vx01 = fmaf(c[5] - c[4], t[0], c[4]);
+          const float vx11 = fmaf(c[7] - c[6], t[0], c[6]);
+          const float vxx1 = fmaf(vx11 - vx01, t[1], vx01);
+          v = fmaf(vxx1 - v, t[2], v);
+        }
+      }
+      const float mterm = fmaf(v, f.inv_mbs, -f.mnm);
+      const int mbin = (int)fminf(fmaxf(mterm, 2.0f), (float)(B - 3));
+      const float arg0 = (float)(mbin - 1) - mterm;
+      const float wt = parzen_deriv_weight_fast(arg0, pr + (int)cb * B + (mbin - 1));
+      if (wt == 0.0f) continue;
+      float g[3] = {0.0f, 0.0f, 0.0f};
+      bool inb[3] = {false, false, false}, unc[3] = {false, false, false};
+      bool any_unc = false;
+#pragma unroll
+      for (int b = 0; b < D; ++b) {
+        const long long q = w.q[b];
+        inb[b] = (unsigned long long)(q - R.cd_lo_in[b]) < R.cd_span_in[b];
+        unc[b] = ((unsigned long long)(q - R.cd_lo_out[b]) < R.cd_span_out[b]) & !inb[b];
+        any_unc = any_unc | unc[b];
+        // the third position along b: P3 = clamp(I + 1) + act(I + 1) when t >= 1/2, P0 = clamp(I - 1) else
+        const bool upper = (uint32_t)q >= 0x80000000u;
+        const int32_t Ie = upper ? I[b] + 1 : I[b] - 1;
+        const int32_t fe = min(max(Ie, f.mstart[b]), f.mend[b]);
+        const int32_t pe = upper ? fe + ((uint32_t)(Ie - f.mstart[b]) < (uint32_t)(f.mend[b] - f.mstart[b]) ? 1 : 0)
+                                 : fe;
+        const uint32_t ebase = (uint32_t)((int32_t)cbase + (pe - fl[b]) * (int32_t)stride[b]);
+        // slabs over the other axes at the low, high and third positions
+        const int o1 = b == 0 ? 1 : 0, o2 = b == 2 ? 1 : 2;  // the other axes (D == 3)
+        float slo, shi, se;
+        if (D == 2) {
+          const int o = 1 - b;
+          const int bit_b = 1 << b, bit_o = 1 << o;
+          slo = fmaf(c[bit_o] - c[0], t[o], c[0]);
+          shi = fmaf(c[bit_b | bit_o] - c[bit_b], t[o], c[bit_b]);
+          const float e0 = P::ld32(img, ebase), e1 = P::ld32(img, ebase + off[o]);
+          se = fmaf(e1 - e0, t[o], e0);
+        } else {
+          const int b1 = 1 << o1, b2 = 1 << o2, bb = 1 << b;
+          const float l0 = fmaf(c[b1] - c[0], t[o1], c[0]);
+          const float l1 = fmaf(c[b1 | b2] - c[b2], t[o1], c[b2]);
+          slo = fmaf(l1 - l0, t[o2], l0);
+          const float h0 = fmaf(c[bb | b1] - c[bb], t[o1], c[bb]);
+          const float h1 = fmaf(c[bb | b1 | b2] - c[bb | b2], t[o1], c[bb | b2]);
+          shi = fmaf(h1 - h0, t[o2], h0);
+          const float e00 = P::ld32(img, ebase), e10 = P::ld32(img, ebase + off[o1]);
+          const float e01 = P::ld32(img, ebase + off[o2]), e11 = P::ld32(img, ebase + off[o1] + off[o2]);
+          const float m0 = fmaf(e10 - e00, t[o1], e00);
+          const float m1 = fmaf(e11 - e01, t[o1], e01);
+          se = fmaf(m1 - m0, t[o2], m0);
+        }
+        const float vm = upper ? fmaf(shi - slo, th[b], slo) : fmaf(slo - se, th[b], se);
+        const float vp = upper ? fmaf(se - shi, th[b], shi) : fmaf(shi - slo, th[b], slo);
+        g[b] = (vp - vm) * R.cd_scale[b];
+      }
+      if (any_unc) {
+        // half-step points near the buffer's edge: after the walks
+        const unsigned long long mask = __ballot(1);
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t first = (uint32_t)__ffsll((long long)mask) - 1u;
+        uint32_t base = 0;
+        if (lane == first) {
+          base = cd_n[wave];
+          cd_n[wave] = base + (uint32_t)__popcll(mask);
+        }
+        base = __shfl(base, (int)first, 64);
+        const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        if (slot < f.cd_wave_cap) defer[slot] = w.s;
+        else atomicOr(f.exc_overflow, 1u);
+        continue;
+      }
+#pragma unroll
+      for (int b = 0; b < D; ++b) {
+        const float gw = inb[b] ? wt * g[b] : 0.0f;
+        sg[b] += gw;
+        sy[b] = fmaf(yf, gw, sy[b]);
+      }
+    }
+    const double x = (double)w.x, z = (double)w.z;
+#pragma unroll
+    for (int b = 0; b < D; ++b) {
+      acc[b * D + 0] = fma((double)sg[b], x, acc[b * D + 0]);
+      acc[b * D + 1] += (double)sy[b];
+      if (D == 3) acc[b * D + 2] = fma((double)sg[b], z, acc[b * D + 2]);
+      acc[D * D + b] += (double)sg[b];
+    }
+  }
+  // the deferred samples, exactly: this wavefront's list, lane by lane
+  __syncthreads();
+  const uint32_t nd = min(cd_n[wave], f.cd_wave_cap);
+  for (uint32_t k = threadIdx.x & 63u; k < nd; k += 64u) {
+    const uint32_t s = defer[k];
+    int32_t idx[3];
+    sample_index<D>(a.virt, s, idx);
+    double vp[3];
+    index_to_point<D>(a.virt, idx, vp);
+    SampleOut<D> o;
+    if (!eval_sample<D, 1, P>(a, vp, o)) continue;
+    const float wt = parzen_deriv_weight((float)((double)(o.mbin - 1) - o.mterm), pr + o.fbin * B + (o.mbin - 1));
+    if (wt == 0.0f) continue;
+    double g[3] = {0.0, 0.0, 0.0};
+    central_difference<D, P>(a, o.mp, g);
+    add_moments_d<D>(acc, wt, g, idx);
+  }
+  block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)lb * PA);
+  // this workgroup's listed samples, exactly, into their own partial row
+  exceptions_pass2_body<D, P, 1, true>(a, f, lb, lg);
+}
+
 // Fixed-order reduction of 2 * nblocks rows of index-space moments, then the
 // map to the affine derivative:
 //   d[b*D + d]   = nFactor * (sum_j V[d][j] M[b][j] + c0[d] M1[b])
@@ -984,6 +1183,11 @@ __global__ void __launch_bounds__(256) pass1_rows_kernel(FastArgs f, EvalArgs a)
 template <int D, typename P>
 __global__ void __launch_bounds__(256) pass2_rows_kernel(FastArgs f, EvalArgs a) {
   pass2_rows_body<D, P>(f, a, blockIdx.x, gridDim.x);
+}
+
+template <int D, typename P>
+__global__ void __launch_bounds__(256) pass2_rows_cd_kernel(FastArgs f, EvalArgs a) {
+  pass2_rows_cd_body<D, P>(f, a, blockIdx.x, gridDim.x);
 }
 
 template <int D, typename P, int MK>
@@ -1156,7 +1360,9 @@ hipError_t launch_fast_pass2(const FastArgs& f, const EvalArgs& a, int dim, int 
   const size_t lds = ((size_t)f.bins * f.bins + 128) * 4;
   // a row-walk pass 1 is followed by a pass 2 that reads its byte per sample
   if (f.rows.on && !f.cls) return hipErrorInvalidValue;
-  if (f.rows.on)
+  if (f.rows.on && f.rows.cd_on)
+    MMI_DISPATCH_FAST(pass2_rows_cd_kernel, f.storage, dim, grid, lds, st, f, a);
+  else if (f.rows.on)
     MMI_DISPATCH_FAST(pass2_rows_kernel, f.storage, dim, grid, lds, st, f, a);
   else if (f.cls)
     MMI_DISPATCH_FAST_MK(pass2_cls_kernel, f.mk, f.storage, dim, grid, lds, st, f, a);

@@ -66,6 +66,19 @@ struct FastRows {
   unsigned long long span_out[3];  // lo_out + span_out and above: certainly outside
   long long lo_in[3];              // [lo_in, lo_in + span_in): certainly inside
   unsigned long long span_in[3];
+  // Central differences of the moving image on the fast pass 2 (cd_on; a
+  // moving geometry with a diagonal physical-to-index map): along axis b the
+  // reference's two points at -/+ half a spacing are the continuous index
+  // -/+ 1/2 along index axis b.  Both are inside the buffer iff the index lies
+  // in [start, end - 1); in fixed point [cd_lo_in, +cd_span_in) certainly
+  // holds it, outside [cd_lo_out, +cd_span_out) it certainly fails, between
+  // the two the reference's own FP64 points decide.
+  int32_t cd_on;
+  float cd_scale[3];  // sign of the index step of a physical + step, over the spacing
+  long long cd_lo_in[3];
+  unsigned long long cd_span_in[3];
+  long long cd_lo_out[3];
+  unsigned long long cd_span_out[3];
 };
 
 struct FastArgs {
@@ -108,6 +121,11 @@ struct FastArgs {
   int16_t* sample_mbin;
   const float* pratio_f;
   double* partials;  // [2 * grid * P]: fast rows then exception rows
+  // rows.cd_on: per wavefront of pass 2 (workgroup * 4 + wave) a region of
+  // cd_wave_cap samples whose central-difference points need the exact test
+  uint32_t* cd_defer;
+  uint32_t cd_wave_cap;
+  int32_t pad_cd_;
   uint32_t* exc_count;
   uint32_t* exc_list;
   uint32_t* exc_overflow;
@@ -135,6 +153,9 @@ struct FastArgs {
   FastMask fmask, mmask;  // fixed mask at the fixed point, moving mask at the mapped moving point
   FastRows rows;          // rows.on: the row-walk kernels (pass 2 then needs cls)
 };
+
+// XCDs of the device: the fast path gives each a contiguous share of the work.
+constexpr int kXcd = 8;
 
 constexpr uint32_t kClsListed = 0xFEu;
 constexpr uint32_t kClsSkip = 0xFFu;

@@ -216,44 +216,6 @@ __global__ void __launch_bounds__(kFinalizeThreads) finalize_batch(const Finaliz
 // pass 2: derivative
 // ---------------------------------------------------------------------------
 
-// Central differences of the moving image at a mapped point
-// (CentralDifferenceImageFunction::EvaluateSpecialized,
-// Modules/Core/ImageFunction/include/itkCentralDifferenceImageFunction.hxx:252-318).
-template <int D, typename P>
-__device__ __forceinline__ void central_difference(const EvalArgs& a, const double* p, double* grad) {
-  const typename P::pixel* img = static_cast<const typename P::pixel*>(a.moving_px);
-  double n1[3], n2[3];
-#pragma unroll
-  for (int i = 0; i < D; ++i) n1[i] = n2[i] = p[i];
-#pragma unroll
-  for (int dim = 0; dim < D; ++dim) {
-    const double offset = 0.5 * a.moving.spacing[dim];
-    double c1[3], c2[3];
-    n1[dim] = p[dim] - offset;
-    point_to_cindex<D>(a.moving, n1, c1);
-    bool ok = inside_buffer<D>(a.moving, c1);
-    if (ok) {
-      n2[dim] = p[dim] + offset;
-      point_to_cindex<D>(a.moving, n2, c2);
-      ok = inside_buffer<D>(a.moving, c2);
-    }
-    if (!ok) {
-      grad[dim] = 0.0;
-    } else {
-      const double delta = n2[dim] - n1[dim];
-      if (delta > 10.0 * 2.220446049250313e-16) {
-        const double v2 = interp_exact<D, P>(img, make_cell<D>(a.moving, c2));
-        const double v1 = interp_exact<D, P>(img, make_cell<D>(a.moving, c1));
-        grad[dim] = (v2 - v1) / delta;
-      } else {
-        grad[dim] = 0.0;
-      }
-    }
-    n1[dim] = p[dim];
-    n2[dim] = p[dim];
-  }
-}
-
 template <int D, int MODE, typename P>
 __global__ void __launch_bounds__(256) pass2_kernel(EvalArgs a) {
   constexpr int MK = MODE == kExactBSpline ? 2 : 1;
