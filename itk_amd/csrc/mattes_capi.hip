@@ -1808,8 +1808,11 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
         per_wave = std::max<uint64_t>(per_wave, (chunk + wx - 1) / wx);
       }
       const uint64_t cap = per_wave * 64 * (uint64_t)R.walk;
-      MMI_HIP(ctx->cd_defer.ensure(sizeof(uint32_t) * cap * waves));
+      // the region, then the per-thread counts
+      const size_t region = sizeof(uint32_t) * cap * waves;
+      MMI_HIP(ctx->cd_defer.ensure(region + sizeof(uint32_t) * waves * 64));
       fa.cd_defer = ctx->cd_defer.as<uint32_t>();
+      fa.cd_nd = fa.cd_defer + cap * waves;
       fa.cd_wave_cap = (uint32_t)cap;
     }
   }

@@ -693,10 +693,13 @@ __device__ __forceinline__ void add_moments_d(double* acc, float w, const double
 
 // Exception samples of one pass-1 workgroup, in sample order (bitonic sort of
 // the list in LDS), summed in a fixed tree: deterministic.  CD: the gradient
-// by central differences of the moving image (else the gradient image).
+// by central differences of the moving image (else the gradient image), and
+// the calling lane's nd deferred samples at defer (pass2_rows_cd_body) as
+// well, in the list's order.
 template <int D, typename P, int MK, bool CD = false>
 __device__ __forceinline__ void exceptions_pass2_body(const EvalArgs& a, const FastArgs& f, uint32_t lb,
-                                                      uint32_t lg) {
+                                                      uint32_t lg, const uint32_t* defer = nullptr,
+                                                      uint32_t nd = 0) {
   __shared__ uint32_t keys[256];
   __shared__ double scratch[4 * 12];
   constexpr int PA = D * D + D;
@@ -743,6 +746,23 @@ __device__ __forceinline__ void exceptions_pass2_body(const EvalArgs& a, const F
           }
         }
       }
+    }
+  }
+  if constexpr (CD) {
+    for (uint32_t k = 0; k < nd; ++k) {
+      const uint32_t s = defer[k];
+      int32_t idx[3];
+      sample_index<D>(a.virt, s, idx);
+      double vp[3];
+      index_to_point<D>(a.virt, idx, vp);
+      SampleOut<D> o;
+      if (!eval_sample<D, 1, P>(a, vp, o)) continue;
+      const float w = parzen_deriv_weight((float)((double)(o.mbin - 1) - o.mterm),
+                                          f.pratio_f + o.fbin * a.bins + (o.mbin - 1));
+      if (w == 0.0f) continue;
+      double g[3] = {0.0, 0.0, 0.0};
+      central_difference<D, P>(a, o.mp, g);
+      add_moments_d<D>(acc, w, g, idx);
     }
   }
   block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)(lg + lb) * PA);
@@ -921,9 +941,9 @@ __device__ __forceinline__ void pass2_rows_body(const FastArgs& f, const EvalArg
 // more per axis give the value (interpolated exactly as pass 1 does: the same
 // bins) and the D gradient components, each (v+ - v-) * sign / spacing.  A
 // sample whose half-step points are not certainly inside or outside the
-// buffer is deferred to the wavefront's region of cd_defer and taken after
-// the walks on the exact FP64 path (eval_sample, central_difference), each
-// wavefront's in its own order, so the results do not depend on timing.
+// buffer is deferred to the lane's region of cd_defer and taken by the tail
+// kernel on the exact FP64 path (eval_sample, central_difference), in the
+// lane's own order, so the results do not depend on timing.
 template <int D, typename P>
 __device__ __forceinline__ void pass2_rows_cd_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
   extern __shared__ float lds_f32[];
@@ -939,10 +959,11 @@ __device__ __forceinline__ void pass2_rows_cd_body(const FastArgs& f, const Eval
   for (int i = 0; i < PA; ++i) acc[i] = 0.0;
   const FastRows& R = f.rows;
   const typename P::pixel* img = static_cast<const typename P::pixel*>(f.moving_px);
-  __shared__ uint32_t cd_n[4];  // deferred samples per wavefront
-  const uint32_t wave = threadIdx.x >> 6;
-  uint32_t* defer = f.cd_defer + (int64_t)(lb * 4u + wave) * f.cd_wave_cap;
-  if ((threadIdx.x & 63u) == 0) cd_n[wave] = 0u;
+  // this lane's region of deferred samples (at most one per row of each of
+  // its walks)
+  const uint32_t lane_cap = f.cd_wave_cap >> 6;
+  uint32_t* defer = f.cd_defer + (int64_t)(lb * 256u + threadIdx.x) * lane_cap;
+  uint32_t nd = 0;
   uint32_t it, iend, istep;
   rows_range(f, lb, lg, it, iend, istep);
   for (; it < iend; it += istep) {
@@ -1039,18 +1060,8 @@ __device__ __forceinline__ void pass2_rows_cd_body(const FastArgs& f, const Eval
       }
       if (any_unc) {
         // half-step points near the buffer's edge: after the walks
-        const unsigned long long mask = __ballot(1);
-        const uint32_t lane = threadIdx.x & 63u;
-        const uint32_t first = (uint32_t)__ffsll((long long)mask) - 1u;
-        uint32_t base = 0;
-        if (lane == first) {
-          base = cd_n[wave];
-          cd_n[wave] = base + (uint32_t)__popcll(mask);
-        }
-        base = __shfl(base, (int)first, 64);
-        const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-        if (slot < f.cd_wave_cap) defer[slot] = w.s;
-        else atomicOr(f.exc_overflow, 1u);
+        if (nd < lane_cap) defer[nd] = w.s;
+        ++nd;
         continue;
       }
 #pragma unroll
@@ -1069,26 +1080,21 @@ __device__ __forceinline__ void pass2_rows_cd_body(const FastArgs& f, const Eval
       acc[D * D + b] += (double)sg[b];
     }
   }
-  // the deferred samples, exactly: this wavefront's list, lane by lane
-  __syncthreads();
-  const uint32_t nd = min(cd_n[wave], f.cd_wave_cap);
-  for (uint32_t k = threadIdx.x & 63u; k < nd; k += 64u) {
-    const uint32_t s = defer[k];
-    int32_t idx[3];
-    sample_index<D>(a.virt, s, idx);
-    double vp[3];
-    index_to_point<D>(a.virt, idx, vp);
-    SampleOut<D> o;
-    if (!eval_sample<D, 1, P>(a, vp, o)) continue;
-    const float wt = parzen_deriv_weight((float)((double)(o.mbin - 1) - o.mterm), pr + o.fbin * B + (o.mbin - 1));
-    if (wt == 0.0f) continue;
-    double g[3] = {0.0, 0.0, 0.0};
-    central_difference<D, P>(a, o.mp, g);
-    add_moments_d<D>(acc, wt, g, idx);
-  }
   block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)lb * PA);
-  // this workgroup's listed samples, exactly, into their own partial row
-  exceptions_pass2_body<D, P, 1, true>(a, f, lb, lg);
+  // this workgroup's listed samples and each lane's deferred ones go to the
+  // exact tail kernel (its FP64 central differences would raise this
+  // kernel's registers)
+  if (nd > lane_cap) atomicOr(f.exc_overflow, 1u);
+  f.cd_nd[lb * 256u + threadIdx.x] = min(nd, lane_cap);
+}
+
+// The exact tail of pass2_rows_cd_body's workgroup lb: its listed samples and
+// its lanes' deferred samples, into partial row lg + lb.
+template <int D, typename P>
+__device__ __forceinline__ void pass2_cd_tail_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
+  const uint32_t lane_cap = f.cd_wave_cap >> 6;
+  const uint32_t t = lb * 256u + threadIdx.x;
+  exceptions_pass2_body<D, P, 1, true>(a, f, lb, lg, f.cd_defer + (int64_t)t * lane_cap, f.cd_nd[t]);
 }
 
 // Fixed-order reduction of 2 * nblocks rows of index-space moments, then the
@@ -1188,6 +1194,11 @@ __global__ void __launch_bounds__(256) pass2_rows_kernel(FastArgs f, EvalArgs a)
 template <int D, typename P>
 __global__ void __launch_bounds__(256) pass2_rows_cd_kernel(FastArgs f, EvalArgs a) {
   pass2_rows_cd_body<D, P>(f, a, blockIdx.x, gridDim.x);
+}
+
+template <int D, typename P>
+__global__ void __launch_bounds__(256) pass2_cd_tail_kernel(FastArgs f, EvalArgs a) {
+  pass2_cd_tail_body<D, P>(f, a, blockIdx.x, gridDim.x);
 }
 
 template <int D, typename P, int MK>
@@ -1360,9 +1371,10 @@ hipError_t launch_fast_pass2(const FastArgs& f, const EvalArgs& a, int dim, int 
   const size_t lds = ((size_t)f.bins * f.bins + 128) * 4;
   // a row-walk pass 1 is followed by a pass 2 that reads its byte per sample
   if (f.rows.on && !f.cls) return hipErrorInvalidValue;
-  if (f.rows.on && f.rows.cd_on)
+  if (f.rows.on && f.rows.cd_on) {
     MMI_DISPATCH_FAST(pass2_rows_cd_kernel, f.storage, dim, grid, lds, st, f, a);
-  else if (f.rows.on)
+    MMI_DISPATCH_FAST(pass2_cd_tail_kernel, f.storage, dim, grid, 0, st, f, a);
+  } else if (f.rows.on)
     MMI_DISPATCH_FAST(pass2_rows_kernel, f.storage, dim, grid, lds, st, f, a);
   else if (f.cls)
     MMI_DISPATCH_FAST_MK(pass2_cls_kernel, f.mk, f.storage, dim, grid, lds, st, f, a);

@@ -121,9 +121,11 @@ struct FastArgs {
   int16_t* sample_mbin;
   const float* pratio_f;
   double* partials;  // [2 * grid * P]: fast rows then exception rows
-  // rows.cd_on: per wavefront of pass 2 (workgroup * 4 + wave) a region of
-  // cd_wave_cap samples whose central-difference points need the exact test
+  // rows.cd_on: per thread of pass 2 (workgroup * 256 + thread) a region of
+  // cd_wave_cap / 64 samples whose central-difference points need the exact
+  // test (cd_wave_cap: the bound of one wavefront's samples)
   uint32_t* cd_defer;
+  uint32_t* cd_nd;  // [grid * 256] deferred samples per thread
   uint32_t cd_wave_cap;
   int32_t pad_cd_;
   uint32_t* exc_count;
