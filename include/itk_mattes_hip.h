@@ -239,6 +239,26 @@ int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, doubl
  * value restores the library's choice.  Takes effect at the next
  * mmi_initialize. */
 int mmi_set_launch_config(mmi_context* ctx, int32_t blocks_per_cu, int32_t rows_walk);
+/* Pass 1's LDS histogram copies (lane i adds to copy i mod copies): 1, 2, 4,
+ * or 0 for the library's choice.  No counterpart in the reference (tuning). */
+int mmi_set_histogram_copies(mmi_context* ctx, int32_t copies);
+/* The launch configuration in effect after mmi_initialize; from_table: 1 when
+ * the tuning table chose the settings the caller left to the library. */
+int mmi_get_launch_config(mmi_context* ctx, int32_t* blocks_per_cu, int32_t* rows_walk, int32_t* hist_copies,
+                          int32_t* from_table);
+/* Launch-configuration table (no counterpart in the reference): text lines
+ *   kind storage dim gradient bins log2_samples blocks_per_cu rows_walk hist_copies
+ * (kind affine|bspline, storage f32|bf16|f16, gradient image|cd; '#' starts
+ * a comment), as the tuner itk_amd/tune.py writes them from timed sweeps.
+ * mmi_initialize takes, for the settings neither mmi_set_launch_config,
+ * mmi_set_histogram_copies nor the environment chose, the row of the
+ * context's transform kind, storage, dimension and gradient source with the
+ * smallest bins >= the context's and the log2 sample count nearest to its
+ * own; no row: the library's defaults.  The table is read at the first
+ * mmi_initialize from $MMI_TUNING_TABLE or tuning/mi355x.tsv beside the
+ * library; this call replaces it (path NULL: that default).  n_rows: rows
+ * read (may be NULL). */
+int mmi_load_tuning_table(const char* path, int32_t* n_rows);
 
 /* mmi_evaluate without device allocation.  No counterpart in the reference:
  * for callers that cannot allocate device memory (code under HIP graph

@@ -71,6 +71,10 @@ SIGNATURES = {
     "mmi_batch_destroy": (ctypes.c_int, [_vp]),
     "mmi_batch_set_graphs": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "mmi_batch_set_step_loop": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
+    "mmi_set_histogram_copies": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mmi_get_launch_config": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                             ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    "mmi_load_tuning_table": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32)]),
     "mmi_batch_get_step_count": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "mmi_batch_evaluate": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_int32, ctypes.c_int32, _dp,
                                           ctypes.POINTER(_dp), ctypes.POINTER(ctypes.c_int64),

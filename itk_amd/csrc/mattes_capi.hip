@@ -1,13 +1,16 @@
 // C ABI of the MI355X Mattes MI v4 metric: device context, image and
 // transform uploads, MattesMutualInformationImageToImageMetricv4::Initialize
 // and GetValue / GetValueAndDerivative on the device.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <limits>
+#include <sstream>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -312,6 +315,11 @@ struct mmi_context {
   int rows_walk = 16;                        // affine fast path: rows per walk (0: sample-strided kernels)
   int default_blocks_per_cu = 4, default_rows_walk = 16;  // the library's choice (environment overrides)
   int hist_copies = 0;                       // pass 1's LDS histogram copies (0: by size)
+  // launch settings the caller (mmi_set_launch_config, mmi_set_histogram_copies)
+  // or the environment chose; the others come from the tuning table
+  bool user_bpc = false, user_walk = false, user_copies = false;
+  bool env_bpc = false, env_walk = false, env_copies = false;  // set by the environment (kept by the API's -1 / 0)
+  int32_t tuned = 0;                         // 1: the last mmi_initialize took a tuning-table row
   int32_t fixed_kind = kPixelFloat, moving_kind = kPixelFloat;  // reference pixel kinds of the uploads
   int max_lds_per_block = 64 * 1024;         // the device's LDS bytes per workgroup
   int grid = 0;
@@ -471,6 +479,100 @@ int resolve_scratch(int device) {
     return fail(MMI_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
   g_scratch_ptr[device] = static_cast<unsigned char*>(p);
   return MMI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launch-configuration table (written by the tuner, itk_amd/tune.py): per
+// workload class the blocks per CU, rows per walk and pass-1 histogram copies
+// that timed fastest.
+// ---------------------------------------------------------------------------
+
+struct TuneRow {
+  int32_t kind;      // MMI_TF_AFFINE / MMI_TF_BSPLINE
+  int32_t storage;   // MMI_F32 / MMI_BF16 / MMI_F16
+  int32_t dim;
+  int32_t gradient;  // MMI_GRADIENT_IMAGE / MMI_GRADIENT_CENTRAL_DIFFERENCE
+  int32_t bins;      // the row serves contexts with at most this many bins
+  int32_t log2n;     // floor(log2(virtual samples)) of the tuned workload
+  int32_t bpc, walk, copies;
+};
+
+std::mutex g_tuning_mu;
+std::vector<TuneRow> g_tuning;
+bool g_tuning_loaded = false;
+
+int hist_copies_default(int bins) { return hist_copies(bins); }
+
+// The table shipped beside the library, or $MMI_TUNING_TABLE.
+std::string default_tuning_path() {
+  if (const char* e = std::getenv("MMI_TUNING_TABLE")) return e;
+  Dl_info info;
+  if (dladdr(reinterpret_cast<void*>(&hist_copies_default), &info) && info.dli_fname) {
+    std::string so = info.dli_fname;
+    const size_t slash = so.find_last_of('/');
+    return (slash == std::string::npos ? std::string(".") : so.substr(0, slash)) + "/tuning/mi355x.tsv";
+  }
+  return "tuning/mi355x.tsv";
+}
+
+int read_tuning_table(const std::string& path, std::vector<TuneRow>& rows) {
+  std::ifstream in(path);
+  if (!in) return fail(MMI_ERR_ARGUMENT, "cannot open tuning table " + path);
+  std::string line;
+  int lineno = 0;
+  while (std::getline(in, line)) {
+    ++lineno;
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line.resize(hash);
+    std::istringstream ss(line);
+    std::string kind, storage, gradient;
+    TuneRow r{};
+    if (!(ss >> kind)) continue;  // blank or comment
+    if (!(ss >> storage >> r.dim >> gradient >> r.bins >> r.log2n >> r.bpc >> r.walk >> r.copies))
+      return fail(MMI_ERR_ARGUMENT, path + ":" + std::to_string(lineno) + ": expected 9 fields");
+    r.kind = kind == "affine" ? MMI_TF_AFFINE : (kind == "bspline" ? MMI_TF_BSPLINE : -1);
+    r.storage = storage == "f32" ? MMI_F32 : (storage == "bf16" ? MMI_BF16 : (storage == "f16" ? MMI_F16 : -1));
+    r.gradient = gradient == "image" ? MMI_GRADIENT_IMAGE : (gradient == "cd" ? MMI_GRADIENT_CENTRAL_DIFFERENCE : -1);
+    if (r.kind < 0 || r.storage < 0 || r.gradient < 0 || (r.dim != 2 && r.dim != 3) || r.bpc < 1 || r.bpc > 64 ||
+        r.walk < 0 || r.walk > 256 || (r.copies != 1 && r.copies != 2 && r.copies != 4))
+      return fail(MMI_ERR_ARGUMENT, path + ":" + std::to_string(lineno) + ": field out of range");
+    rows.push_back(r);
+  }
+  return MMI_OK;
+}
+
+// The table's settings for this context's workload (the smallest bins bucket
+// that holds its bins, then the nearest sample-count bucket), for the
+// settings the caller left to the library; the defaults otherwise.  The
+// table is read once, at the first initialisation.
+void apply_tuning(mmi_context* ctx, int64_t n) {
+  {
+    std::lock_guard<std::mutex> lock(g_tuning_mu);
+    if (!g_tuning_loaded) {
+      g_tuning_loaded = true;
+      std::vector<TuneRow> rows;
+      const std::string saved = g_last_error;
+      if (read_tuning_table(default_tuning_path(), rows) == MMI_OK) g_tuning = rows;
+      g_last_error = saved;  // no table is not an error
+    }
+  }
+  const TuneRow* best = nullptr;
+  int log2n = 0;
+  while (log2n < 62 && (int64_t(1) << (log2n + 1)) <= n) ++log2n;
+  for (const TuneRow& r : g_tuning) {
+    if (r.kind != ctx->mtf.kind || r.storage != ctx->storage || r.dim != ctx->fixed_g.dim ||
+        r.gradient != ctx->gradient_source || r.bins < ctx->bins)
+      continue;
+    if (!best || r.bins < best->bins ||
+        (r.bins == best->bins && (std::abs(r.log2n - log2n) < std::abs(best->log2n - log2n) ||
+                                  (std::abs(r.log2n - log2n) == std::abs(best->log2n - log2n) &&
+                                   r.log2n < best->log2n))))
+      best = &r;
+  }
+  ctx->tuned = best ? 1 : 0;
+  if (!ctx->user_bpc) ctx->blocks_per_cu = best ? best->bpc : ctx->default_blocks_per_cu;
+  if (!ctx->user_walk) ctx->rows_walk = best ? best->walk : ctx->default_rows_walk;
+  if (!ctx->user_copies) ctx->hist_copies = best ? best->copies : 0;
 }
 
 EvalBufs scratch_bufs(unsigned char* base) {
@@ -1197,9 +1299,18 @@ int mmi_context_create(int device, uint32_t flags, mmi_context** out) {
   ctx->device = device;
   ctx->flags = flags;
   if (const char* e = std::getenv("MMI_XCD_CHUNKS")) ctx->xcd_chunks = std::atoi(e) != 0;
-  if (const char* e = std::getenv("MMI_BLOCKS_PER_CU")) ctx->blocks_per_cu = std::max(1, std::min(64, std::atoi(e)));
-  if (const char* e = std::getenv("MMI_ROWS_WALK")) ctx->rows_walk = std::max(0, std::min(256, std::atoi(e)));
-  if (const char* e = std::getenv("MMI_HIST_COPIES")) ctx->hist_copies = std::max(0, std::min(4, std::atoi(e)));
+  if (const char* e = std::getenv("MMI_BLOCKS_PER_CU")) {
+    ctx->blocks_per_cu = std::max(1, std::min(64, std::atoi(e)));
+    ctx->user_bpc = ctx->env_bpc = true;
+  }
+  if (const char* e = std::getenv("MMI_ROWS_WALK")) {
+    ctx->rows_walk = std::max(0, std::min(256, std::atoi(e)));
+    ctx->user_walk = ctx->env_walk = true;
+  }
+  if (const char* e = std::getenv("MMI_HIST_COPIES")) {
+    ctx->hist_copies = std::max(0, std::min(4, std::atoi(e)));
+    ctx->user_copies = ctx->env_copies = ctx->hist_copies != 0;
+  }
   ctx->default_blocks_per_cu = ctx->blocks_per_cu;
   ctx->default_rows_walk = ctx->rows_walk;
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -1472,7 +1583,40 @@ int mmi_set_launch_config(mmi_context* ctx, int32_t blocks_per_cu, int32_t rows_
   if (blocks_per_cu > 64 || rows_walk > 256) return fail(MMI_ERR_ARGUMENT, "launch configuration out of range");
   ctx->blocks_per_cu = blocks_per_cu > 0 ? blocks_per_cu : ctx->default_blocks_per_cu;
   ctx->rows_walk = rows_walk >= 0 ? rows_walk : ctx->default_rows_walk;
+  ctx->user_bpc = blocks_per_cu > 0 || ctx->env_bpc;
+  ctx->user_walk = rows_walk >= 0 || ctx->env_walk;
   ctx->initialized = false;
+  return MMI_OK;
+}
+
+int mmi_set_histogram_copies(mmi_context* ctx, int32_t copies) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (copies != 0 && copies != 1 && copies != 2 && copies != 4)
+    return fail(MMI_ERR_ARGUMENT, "histogram copies must be 0 (the library's choice), 1, 2 or 4");
+  if (copies != 0 || !ctx->env_copies) ctx->hist_copies = copies;
+  ctx->user_copies = copies != 0 || ctx->env_copies;
+  ctx->initialized = false;
+  return MMI_OK;
+}
+
+int mmi_get_launch_config(mmi_context* ctx, int32_t* blocks_per_cu, int32_t* rows_walk, int32_t* hist_copies,
+                          int32_t* from_table) {
+  if (!ctx) return fail(MMI_ERR_ARGUMENT, "ctx is NULL");
+  if (blocks_per_cu) *blocks_per_cu = ctx->blocks_per_cu;
+  if (rows_walk) *rows_walk = ctx->rows_walk;
+  if (hist_copies) *hist_copies = ctx->hist_copies ? ctx->hist_copies : hist_copies_default(ctx->bins);
+  if (from_table) *from_table = ctx->tuned;
+  return MMI_OK;
+}
+
+int mmi_load_tuning_table(const char* path, int32_t* n_rows) {
+  std::vector<TuneRow> rows;
+  const std::string p = path ? std::string(path) : default_tuning_path();
+  if (int rc = read_tuning_table(p, rows)) return rc;
+  std::lock_guard<std::mutex> lock(g_tuning_mu);
+  g_tuning = rows;
+  g_tuning_loaded = true;
+  if (n_rows) *n_rows = (int32_t)rows.size();
   return MMI_OK;
 }
 
@@ -1602,6 +1746,7 @@ int mmi_initialize(mmi_context* ctx) {
   MMI_HIP(ctx->rowsum.ensure(sizeof(double) * B));
   MMI_HIP(ctx->out.ensure(out_bytes((int64_t)ctx->mtf.params.size())));
   const int64_t n = sample_count(ctx);
+  apply_tuning(ctx, n);
   // a multiple of 8: the fast kernels split the samples into one share per XCD
   const int64_t want = std::min<int64_t>((n + 255) / 256, (int64_t)ctx->cu_count * ctx->blocks_per_cu);
   ctx->grid = (int)std::max<int64_t>(8, (want + 7) / 8 * 8);

@@ -55,6 +55,7 @@ class MattesMutualInformationImageToImageMetricv4:
         N.check(self._L.mmi_context_create(device, flags, ctypes.byref(self._ctx)))
         self._bins = 50
         self._launch = (-1, -1)  # library's launch configuration
+        self._hist_copies = 0
         self._out = np.zeros(2)  # [value (double), valid count (int64 bits)]
         self._out_addr = self._out.ctypes.data
         self._fixed = self._moving = None
@@ -152,12 +153,23 @@ class MattesMutualInformationImageToImageMetricv4:
     def GetNumberOfHistogramBins(self) -> int:
         return self._bins
 
-    def SetLaunchConfiguration(self, blocks_per_cu: int = -1, rows_walk: int = -1):
+    def SetLaunchConfiguration(self, blocks_per_cu: int = -1, rows_walk: int = -1, hist_copies: int = 0):
         """Kernel launch configuration (no counterpart in ITK): workgroups per
-        compute unit and rows per walk of the affine row-walk kernels (0: the
-        sample-strided kernels); -1 keeps the library's choice."""
+        compute unit, rows per walk of the affine row-walk kernels (0: the
+        sample-strided kernels) and pass 1's LDS histogram copies; -1 (0 for
+        the copies) keeps the library's choice (the tuning table's row for the
+        workload, else its defaults)."""
         self._launch = (int(blocks_per_cu), int(rows_walk))
+        self._hist_copies = int(hist_copies)
         self._initialized = False
+
+    def GetLaunchConfiguration(self) -> dict:
+        """The launch configuration in effect after Initialize, and whether
+        the tuning table chose it."""
+        v = [ctypes.c_int32() for _ in range(4)]
+        N.check(self._L.mmi_get_launch_config(self._ctx, *[ctypes.byref(x) for x in v]))
+        return {"blocks_per_cu": v[0].value, "rows_walk": v[1].value, "hist_copies": v[2].value,
+                "from_table": bool(v[3].value)}
 
     def SetVirtualDomain(self, geometry: ImageGeometry):
         self._virtual = geometry
@@ -331,6 +343,7 @@ class MattesMutualInformationImageToImageMetricv4:
             self._gradient_dirty = False
         N.check(L.mmi_set_number_of_histogram_bins(ctx, self._bins))
         N.check(L.mmi_set_launch_config(ctx, *self._launch))
+        N.check(L.mmi_set_histogram_copies(ctx, self._hist_copies))
         d = self._fixed[1].dim
         fs = None if self._fixed_sigma is None else np.ascontiguousarray(np.broadcast_to(self._fixed_sigma, (d,)))
         ms = None if self._moving_sigma is None else np.ascontiguousarray(np.broadcast_to(self._moving_sigma, (d,)))
