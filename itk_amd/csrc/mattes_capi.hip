@@ -2718,7 +2718,9 @@ int regular_step_device(mmi_batch* b, mmi_context* const* ctxs, int32_t n, const
   L.halted = d_halt;
   if (int rc = ensure_pinned(reinterpret_cast<void**>(&b->h_flags), &b->h_flags_bytes, sizeof(int32_t) * nf))
     return rc;
-  const int K = b->chunk;
+  // a chunk of at most the call's own budget (no empty iterations when the
+  // caller steps in short calls)
+  const int K = std::min<int>(b->chunk, max_steps);
   auto chunk = [&]() -> hipError_t {
     for (int k = 0; k < K; ++k) {
       hipError_t e = launch_fast_batch(L, b->stream);

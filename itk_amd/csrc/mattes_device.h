@@ -240,6 +240,12 @@ template <typename T>
 __device__ __forceinline__ T at32(const T* p, uint32_t i) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + i * (uint32_t)sizeof(T));
 }
+// The same with the byte offset given (a cell's corners then cost one add
+// each, the scaling done once per sample).
+template <typename T>
+__device__ __forceinline__ T atb(const T* p, uint32_t b) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + b);
+}
 
 struct PxF32 {
   using pixel = float;
@@ -248,6 +254,8 @@ struct PxF32 {
   static __device__ __forceinline__ float4 ldt(const texel* t, int32_t i) { return t[i]; }
   static __device__ __forceinline__ float ld32(const pixel* p, uint32_t i) { return at32(p, i); }
   static __device__ __forceinline__ float4 ldt32(const texel* t, uint32_t i) { return at32(t, i); }
+  static __device__ __forceinline__ float ldb(const pixel* p, uint32_t b) { return atb(p, b); }
+  static __device__ __forceinline__ float4 ldtb(const texel* t, uint32_t b) { return atb(t, b); }
 };
 
 struct PxBF16 {
@@ -262,6 +270,8 @@ struct PxBF16 {
   static __device__ __forceinline__ float4 ldt(const texel* t, int32_t i) { return cvt(t[i]); }
   static __device__ __forceinline__ float ld32(const pixel* p, uint32_t i) { return cv(at32(p, i)); }
   static __device__ __forceinline__ float4 ldt32(const texel* t, uint32_t i) { return cvt(at32(t, i)); }
+  static __device__ __forceinline__ float ldb(const pixel* p, uint32_t b) { return cv(atb(p, b)); }
+  static __device__ __forceinline__ float4 ldtb(const texel* t, uint32_t b) { return cvt(atb(t, b)); }
 };
 
 struct PxF16 {
@@ -275,6 +285,8 @@ struct PxF16 {
   static __device__ __forceinline__ float4 ldt(const texel* t, int32_t i) { return cvt(t[i]); }
   static __device__ __forceinline__ float ld32(const pixel* p, uint32_t i) { return h2f(at32(p, i)); }
   static __device__ __forceinline__ float4 ldt32(const texel* t, uint32_t i) { return cvt(at32(t, i)); }
+  static __device__ __forceinline__ float ldb(const pixel* p, uint32_t b) { return h2f(atb(p, b)); }
+  static __device__ __forceinline__ float4 ldtb(const texel* t, uint32_t b) { return cvt(atb(t, b)); }
 };
 
 // Exact FP64 multilinear value of a float image (the reference's RealType is

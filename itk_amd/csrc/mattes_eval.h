@@ -216,13 +216,16 @@ __device__ __forceinline__ float parzen_deriv_weight_t(float t, const float* pro
   return fmaf(d0, prow[0], fmaf(d1, prow[1], fmaf(d2, prow[2], d3 * prow[3])));
 }
 
-// Fixed-point Parzen weights for either window: closed form when unclamped.
+// Fixed-point Parzen weights for either window: closed form when unclamped
+// (the FP32 products w 2^32 of weights above 2^-8 are integers already, so
+// the conversion truncates without rounding first: below that it moves a
+// weight by less than 2^-32).
 __device__ __forceinline__ void parzen_fixed_fast(float arg0, unsigned int* q) {
   if (arg0 <= -1.0f && arg0 > -2.0f) {
     float w[4];
     parzen_weights_t(-1.0f - arg0, w);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) q[k] = (unsigned int)rintf(w[k] * 4294967296.0f);
+    for (int k = 0; k < 4; ++k) q[k] = (unsigned int)(w[k] * 4294967296.0f);
     q[1] = 0u - q[0] - q[2] - q[3];
   } else {
     parzen_fixed(arg0, q);

@@ -161,15 +161,18 @@ __device__ __forceinline__ void fast_mask(const FastMask& m, double x, double y,
 // channels) instead of the moving values and return the interpolated gradient
 // as well; the value is interpolated with the same operations either way, and
 // every fast-path kernel interpolates through this function.
-template <int D, typename P, bool TEX>
+template <int D, typename P, bool TEX, bool BYTES = false>
 __device__ __forceinline__ float fast_interp(const FastArgs& f, uint32_t cb, uint32_t cx, uint32_t cy, uint32_t cz,
                                              float t0, float t1, float t2, float3* grad) {
   float v;
+  // BYTES: the offsets are byte offsets into the array read
+#define LDT(o) (BYTES ? P::ldtb(tex, (o)) : P::ldt32(tex, (o)))
+#define LDP(o) (BYTES ? P::ldb(img, (o)) : P::ld32(img, (o)))
   if constexpr (TEX) {
     // value as below, gradient as interp_gradient3, from one texel load per corner
     const typename P::texel* tex = static_cast<const typename P::texel*>(f.texels);
-    const float4 a000 = P::ldt32(tex, cb), a100 = P::ldt32(tex, cb + cx);
-    const float4 a010 = P::ldt32(tex, cb + cy), a110 = P::ldt32(tex, cb + cy + cx);
+    const float4 a000 = LDT(cb), a100 = LDT(cb + cx);
+    const float4 a010 = LDT(cb + cy), a110 = LDT(cb + cy + cx);
     const float vx00 = fmaf(a100.x - a000.x, t0, a000.x);
     const float vx10 = fmaf(a110.x - a010.x, t0, a010.x);
     v = fmaf(vx10 - vx00, t1, vx00);
@@ -178,8 +181,8 @@ __device__ __forceinline__ float fast_interp(const FastArgs& f, uint32_t cb, uin
     const float gz0 = fmaf(a100.w - a000.w, t0, a000.w), gz1 = fmaf(a110.w - a010.w, t0, a010.w);
     float3 g = make_float3(fmaf(gx1 - gx0, t1, gx0), fmaf(gy1 - gy0, t1, gy0), fmaf(gz1 - gz0, t1, gz0));
     if (D == 3) {
-      const float4 a001 = P::ldt32(tex, cb + cz), a101 = P::ldt32(tex, cb + cz + cx);
-      const float4 a011 = P::ldt32(tex, cb + cz + cy), a111 = P::ldt32(tex, cb + cz + cy + cx);
+      const float4 a001 = LDT(cb + cz), a101 = LDT(cb + cz + cx);
+      const float4 a011 = LDT(cb + cz + cy), a111 = LDT(cb + cz + cy + cx);
       const float vx01 = fmaf(a101.x - a001.x, t0, a001.x);
       const float vx11 = fmaf(a111.x - a011.x, t0, a011.x);
       const float vxx1 = fmaf(vx11 - vx01, t1, vx01);
@@ -195,21 +198,23 @@ __device__ __forceinline__ float fast_interp(const FastArgs& f, uint32_t cb, uin
     *grad = g;
   } else {
     const typename P::pixel* img = static_cast<const typename P::pixel*>(f.moving_px);
-    const float v000 = P::ld32(img, cb), v100 = P::ld32(img, cb + cx);
-    const float v010 = P::ld32(img, cb + cy), v110 = P::ld32(img, cb + cy + cx);
+    const float v000 = LDP(cb), v100 = LDP(cb + cx);
+    const float v010 = LDP(cb + cy), v110 = LDP(cb + cy + cx);
     const float vx00 = fmaf(v100 - v000, t0, v000);
     const float vx10 = fmaf(v110 - v010, t0, v010);
     v = fmaf(vx10 - vx00, t1, vx00);
     if (D == 3) {
-      const float v001 = P::ld32(img, cb + cz), v101 = P::ld32(img, cb + cz + cx);
-      const float v011 = P::ld32(img, cb + cz + cy);
-      const float v111 = P::ld32(img, cb + cz + cy + cx);
+      const float v001 = LDP(cb + cz), v101 = LDP(cb + cz + cx);
+      const float v011 = LDP(cb + cz + cy);
+      const float v111 = LDP(cb + cz + cy + cx);
       const float vx01 = fmaf(v101 - v001, t0, v001);
       const float vx11 = fmaf(v111 - v011, t0, v011);
       const float vxx1 = fmaf(vx11 - vx01, t1, vx01);
       v = fmaf(vxx1 - v, t2, v);
     }
   }
+#undef LDT
+#undef LDP
   return v;
 }
 
@@ -477,7 +482,7 @@ template <int D>
 struct RowWalk {
   long long q[3];   // moving continuous index, 32.32 fixed point (floor in the high word)
   uint32_t s;       // sample index (virtual buffer order)
-  uint32_t fp;      // fixed pixel element offset
+  uint32_t fp;      // fixed pixel byte offset
   int32_t x, y, z;  // virtual index of the lane's sample (y advances)
   int32_t rows;     // samples of the walk (wavefront-uniform)
   bool active;      // x inside the virtual row
@@ -492,7 +497,7 @@ __device__ __forceinline__ long long to_fixed32(double c) {
   return (long long)(((unsigned long long)(uint32_t)(int32_t)fl << 32) | (unsigned long long)(uint32_t)fr);
 }
 
-template <int D>
+template <int D, typename P>
 __device__ __forceinline__ void rows_start(const FastArgs& f, uint32_t item, RowWalk<D>& w) {
   const FastRows& R = f.rows;
   const uint32_t r = fdiv(item, R.dxseg);
@@ -513,15 +518,16 @@ __device__ __forceinline__ void rows_start(const FastArgs& f, uint32_t item, Row
 #pragma unroll
   for (int k = 0; k < D; ++k) w.q[k] = to_fixed32(ci[k]);
   w.s = (uint32_t)xl + (uint32_t)R.nx * ((uint32_t)y0 + (uint32_t)R.ny * zl);
-  w.fp = (uint32_t)((w.x + f.foff[0]) + f.fsy * (w.y + f.foff[1]) + (D == 3 ? f.fsz * (w.z + f.foff[2]) : 0));
+  w.fp = (uint32_t)((w.x + f.foff[0]) + f.fsy * (w.y + f.foff[1]) + (D == 3 ? f.fsz * (w.z + f.foff[2]) : 0)) *
+         (uint32_t)sizeof(typename P::pixel);
 }
 
-template <int D>
+template <int D, typename P>
 __device__ __forceinline__ void rows_step(const FastArgs& f, RowWalk<D>& w) {
 #pragma unroll
   for (int k = 0; k < D; ++k) w.q[k] += f.rows.qy[k];
   w.s += (uint32_t)f.rows.nx;
-  w.fp += (uint32_t)f.fsy;
+  w.fp += (uint32_t)f.fsy * (uint32_t)sizeof(typename P::pixel);
   ++w.y;
 }
 
@@ -534,6 +540,8 @@ template <int D, typename P, bool TEX = false>
 __device__ __forceinline__ int rows_classify(const FastArgs& f, const RowWalk<D>& w, int& fbin, int& mbin,
                                              float& arg0, float3* grad = nullptr) {
   const FastRows& R = f.rows;
+  // byte offsets into the moving values (or texels): scaled once per sample
+  constexpr uint32_t E = TEX ? (uint32_t)sizeof(typename P::texel) : (uint32_t)sizeof(typename P::pixel);
   bool out = false, uncertain = false;
   int32_t fl[3] = {0, 0, 0};
   uint32_t off[3] = {0u, 0u, 0u};
@@ -546,16 +554,16 @@ __device__ __forceinline__ int rows_classify(const FastArgs& f, const RowWalk<D>
     const int32_t I = (int32_t)(q >> 32);
     fl[k] = min(max(I, f.mstart[k]), f.mend[k]);
     const bool act = (uint32_t)(I - f.mstart[k]) < (uint32_t)(f.mend[k] - f.mstart[k]);
-    const uint32_t stride = k == 0 ? 1u : (k == 1 ? (uint32_t)f.msy : (uint32_t)f.msz);
+    const uint32_t stride = k == 0 ? E : (k == 1 ? (uint32_t)f.msy * E : (uint32_t)f.msz * E);
     off[k] = act ? stride : 0u;
     t[k] = (float)(uint32_t)q * 2.3283064365386963e-10f;  // 2^-32
   }
   // 24-bit products: the host checks the moving extents and strides
   uint32_t cb = (uint32_t)(fl[0] - f.mstart[0]) + __umul24((uint32_t)f.msy, (uint32_t)(fl[1] - f.mstart[1]));
   if (D == 3) cb += __umul24((uint32_t)f.msz, (uint32_t)(fl[2] - f.mstart[2]));
-  const float v = fast_interp<D, P, TEX>(f, cb, off[0], off[1], off[2], t[0], t[1], t[2], grad);
+  const float v = fast_interp<D, P, TEX, true>(f, cb * E, off[0], off[1], off[2], t[0], t[1], t[2], grad);
   // pass 2 (TEX) takes the fixed bin from pass 1's byte: no fixed read
-  const float fv = TEX ? 0.0f : P::ld32(static_cast<const typename P::pixel*>(f.fixed_px), w.fp);
+  const float fv = TEX ? 0.0f : P::ldb(static_cast<const typename P::pixel*>(f.fixed_px), w.fp);
   return fast_bins(f, v, fv, out, uncertain, fbin, mbin, arg0, nullptr);
 }
 
@@ -567,8 +575,8 @@ __device__ __forceinline__ void pass1_rows_body(const FastArgs& f, const EvalArg
   rows_range(f, lb, lg, it, iend, istep);
   for (; it < iend; it += istep) {
     RowWalk<D> w;
-    rows_start<D>(f, it, w);
-    for (int j = 0; j < w.rows; ++j, rows_step<D>(f, w)) {
+    rows_start<D, P>(f, it, w);
+    for (int j = 0; j < w.rows; ++j, rows_step<D, P>(f, w)) {
       if (!w.active) continue;
       int fbin = 0, mbin = 0;
       float arg0 = 0.0f;
@@ -897,10 +905,10 @@ __device__ __forceinline__ void pass2_rows_body(const FastArgs& f, const EvalArg
   rows_range(f, lb, lg, it, iend, istep);
   for (; it < iend; it += istep) {
     RowWalk<D> w;
-    rows_start<D>(f, it, w);
+    rows_start<D, P>(f, it, w);
     float sg[3] = {0.0f, 0.0f, 0.0f}, sy[3] = {0.0f, 0.0f, 0.0f};
     float yf = (float)w.y;
-    for (int j = 0; j < w.rows; ++j, rows_step<D>(f, w), yf += 1.0f) {
+    for (int j = 0; j < w.rows; ++j, rows_step<D, P>(f, w), yf += 1.0f) {
       if (!w.active) continue;
       const uint32_t cb = f.cls[w.s];
       if (cb >= kClsListed) continue;
@@ -968,10 +976,10 @@ __device__ __forceinline__ void pass2_rows_cd_body(const FastArgs& f, const Eval
   rows_range(f, lb, lg, it, iend, istep);
   for (; it < iend; it += istep) {
     RowWalk<D> w;
-    rows_start<D>(f, it, w);
+    rows_start<D, P>(f, it, w);
     float sg[3] = {0.0f, 0.0f, 0.0f}, sy[3] = {0.0f, 0.0f, 0.0f};
     float yf = (float)w.y;
-    for (int j = 0; j < w.rows; ++j, rows_step<D>(f, w), yf += 1.0f) {
+    for (int j = 0; j < w.rows; ++j, rows_step<D, P>(f, w), yf += 1.0f) {
       if (!w.active) continue;
       const uint32_t cb = f.cls[w.s];
       if (cb >= kClsListed) continue;

@@ -199,10 +199,12 @@ class _Lane:
             groups.setdefault(key, []).append(it)
         for group in groups.values():
             states = [it[2] for it in group]
+            before = sum(int(st.iteration) for st in states)
             self.batch.regular_step([it[0].metric for it in group], group[0][1], states, srv.chunk_steps)
             done = []
             with srv.lock:
                 srv.batches += srv.chunk_steps
+                srv.batched_evaluations += sum(int(st.iteration) for st in states) - before
                 srv.max_batch = max(srv.max_batch, len(group))
                 for it, st in zip(group, states):
                     it[2] = st
@@ -231,7 +233,7 @@ class _Lane:
 class RegistrationServer:
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 64 << 30, max_inflight: int = 3,
                  max_wait_s: float = 0.0005, graphs: bool = True, storage: str = "f32", lanes: int = 1,
-                 chunk_steps: int = 5, wait_limit_s: float = 600.0):
+                 chunk_steps: int = 8, wait_limit_s: float = 600.0):
         self.device = device
         # a registration whose evaluation is not served within this time fails
         # (reported in its result) instead of waiting forever
