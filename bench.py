@@ -36,13 +36,17 @@ HBM_PEAK_GBS = 8000.0
 # One compulsory read each of the fixed, moving and gradient (3 components)
 # images per sample: 4 + 4 + 12 bytes in FP32 storage, half in BF16/FP16.
 BYTES_PER_SAMPLE_3D = {"f32": 20, "bf16": 10, "f16": 10}
+# central differences: pass 2 reads the moving value again instead of a texel
+BYTES_PER_SAMPLE_3D_CD = {"f32": 12, "bf16": 6, "f16": 6}
 # HBM bytes per evaluation (FETCH_SIZE + WRITE_SIZE summed over the evaluation's
 # kernels, scaled by the counters' reading for a kernel of known traffic), from the
-# rocprofv3 --pmc runs summarised in profiles/mattes_affine_256cube_fast_path.md
-# and profiles/mattes_bspline_192cube_mesh16.md.  None where not profiled.
+# rocprofv3 --pmc runs summarised in profiles/mattes_affine_256cube_fast_path.md,
+# profiles/mattes_affine_256cube_central_differences.md and
+# profiles/mattes_bspline_192cube_mesh16.md.  None where not profiled.
 TRAFFIC_BYTES_PER_EVAL = {
-    ("mattes_mi_dense_value_and_derivative_affine_256cube", "f32"): 461.4e6,
-    ("mattes_mi_dense_value_and_derivative_bspline_192cube_mesh16", "f32"): 491.1e6,
+    ("mattes_mi_dense_value_and_derivative_affine_256cube", "f32", "image"): 481.7e6,
+    ("mattes_mi_dense_value_and_derivative_affine_256cube", "f32", "cd"): 269.6e6,
+    ("mattes_mi_dense_value_and_derivative_bspline_192cube_mesh16", "f32", "image"): 525.8e6,
 }
 
 
@@ -56,6 +60,9 @@ def parse():
     ap.add_argument("--mesh", type=int, default=16, help="B-spline mesh size per axis")
     ap.add_argument("--storage", choices=("f32", "bf16", "f16"), default="f32",
                     help="device storage of the fixed, moving and gradient images")
+    ap.add_argument("--gradient", choices=("image", "cd"), default="image",
+                    help="moving gradient: the device gradient image (the metric's default) or central "
+                         "differences (the registration method's default; affine workload)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -288,6 +295,7 @@ def pass_breakdown(args, wl, n):
     m.SetFixedImage(wl.problem.fixed, geom)
     m.SetMovingImage(wl.problem.moving, geom)
     m.SetNumberOfHistogramBins(50)
+    m.SetUseMovingImageGradientFilter(args.gradient == "image")
     m.SetMovingTransform(wl.tf)
     m.Initialize()
     p1 = p2 = 0.0
@@ -301,7 +309,7 @@ def pass_breakdown(args, wl, n):
             p2 += t["pass2_ms"] / reps
     m.close()
     px = 4 if args.storage == "f32" else 2
-    b1, b2 = n * 2 * px, n * 4 * px
+    b1, b2 = n * 2 * px, n * (4 if args.gradient == "image" else 1) * px
     out = {"pass1_ms": p1, "pass1_algorithmic_bytes": b1, "pass1_frac": b1 / (p1 * 1e-3) / 1e9 / HBM_PEAK_GBS,
            "pass2_ms": p2, "pass2_algorithmic_bytes": b2, "pass2_frac": b2 / (p2 * 1e-3) / 1e9 / HBM_PEAK_GBS}
     if wl.kind != "affine":
@@ -327,6 +335,7 @@ def main():
     m.SetFixedImage(problem.fixed, geom)
     m.SetMovingImage(problem.moving, geom)
     m.SetNumberOfHistogramBins(50)
+    m.SetUseMovingImageGradientFilter(args.gradient == "image")
     m.SetMovingTransform(tf)
     t0 = time.perf_counter()
     m.Initialize()  # uploads, device gradient filter, intensity ranges
@@ -346,7 +355,7 @@ def main():
     K = args.steps
     ms_step = wall * 1e3 / K
     n = geom.num_pixels
-    alg_bytes = n * BYTES_PER_SAMPLE_3D[args.storage]
+    alg_bytes = n * (BYTES_PER_SAMPLE_3D if args.gradient == "image" else BYTES_PER_SAMPLE_3D_CD)[args.storage]
     dev_ms = t_total / K
     achieved = alg_bytes / (dev_ms * 1e-3) / 1e9
     line = {
@@ -362,13 +371,15 @@ def main():
         "vs_baseline": None,
         "dtype": "%s images, f64 coordinates/PDF" % args.storage,
         "data": "synthetic (seeded blob volumes, see itk_amd/synth.py)",
-        "config": dict({"workload": wl.name + ("" if args.storage == "f32" else "_" + args.storage), "bins": 50,
+        "config": dict({"workload": wl.name + ("" if args.storage == "f32" else "_" + args.storage) +
+                        ("" if args.gradient == "image" else "_central_differences"), "bins": 50,
                         "samples_per_eval": n, "storage": args.storage,
-                        "gradient_source": "recursive Gaussian gradient image (device)",
+                        "gradient_source": "recursive Gaussian gradient image (device)" if args.gradient == "image"
+                        else "central differences of the moving image",
                         "parallelism": "single GPU"}, **wl.extra),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": TRAFFIC_BYTES_PER_EVAL.get((wl.name, args.storage)),
+                     "traffic": TRAFFIC_BYTES_PER_EVAL.get((wl.name, args.storage, args.gradient)),
                      "kernel": "whole evaluation (clear, pass 1, finalize, pass 2, derivative), HIP events on "
                                "the metric stream around each evaluation",
                      "algorithmic_bytes_per_eval": alg_bytes, "device_ms_per_eval": dev_ms},
