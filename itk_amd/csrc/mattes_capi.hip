@@ -297,6 +297,16 @@ struct mmi_context {
   DevBuf exc_count, exc_list;  // fast-path exception lists
   DevBuf exc_res;              // B-spline: exact results of the listed samples
   DevBuf cls;                  // pass 1's byte per sample for pass 2
+  // The exact fixed bin of every dense virtual sample (FastArgs::fbin8):
+  // allocated by mmi_initialize, filled by the level's first row-walk
+  // evaluation (fixed_bins_kernel), valid until the next mmi_initialize.
+  DevBuf fbin8;
+  bool fbin8_valid = false;
+  // The level's moving values and texels with a one-voxel replicated border
+  // (pad_replicate_kernel), for the row walks with the fixed bins; built by
+  // mmi_initialize (pad_ok), texels only with a gradient image (pad_tex).
+  DevBuf mpad, tpad;
+  bool pad_ok = false, pad_tex = false;
   DevBuf cd_defer;             // central differences on the fast pass 2: per-wavefront deferred samples
   DevBuf ftab;                               // fast path: per-axis fixed cells (coarse levels)
   // grow-only work space of the preparation filters (no device frees, which
@@ -973,6 +983,17 @@ void setup_rows_layout(const mmi_context* ctx, FastRows& R) {
   }
 }
 
+// Whether the affine row walks can take the fixed bins from the level's byte
+// map (FastArgs::fbin8): a dense unmasked domain, bins that fit a byte below
+// kClsListed, and texels holding the level's moving image when pass 2 reads
+// the gradient image (it takes the moving values from them).  The same for
+// GetValue and GetValueAndDerivative, so both see the same histogram.
+bool fb_rows_ok(const mmi_context* ctx) {
+  return ctx->mtf.kind == MMI_TF_AFFINE && ctx->bins <= 253 && ctx->n_points == 0 && !ctx->have_mask[0] &&
+         !ctx->have_mask[1] && ctx->pad_ok && ctx->fbin8.bytes >= (size_t)sample_count(ctx) &&
+         (ctx->gradient_source != MMI_GRADIENT_IMAGE || (ctx->texel_values_level && ctx->pad_tex));
+}
+
 bool setup_fast(mmi_context* ctx, FastArgs* F) {
   std::memset(F, 0, sizeof(*F));
   // a composite moving transform maps through its earlier stages: the FP64 path
@@ -1036,7 +1057,8 @@ bool setup_fast(mmi_context* ctx, FastArgs* F) {
   if (!bs) {
     // the affine map, row walks (their steps widen e_ci) and moving bounds, as
     // the device optimizer loop recomputes them after each step
-    if (direct && !mg[0] && !mg[1]) setup_rows_layout(ctx, F->rows);
+    // (fixed samples off the fixed pixel centres: only with the fixed bins)
+    if ((direct || fb_rows_ok(ctx)) && !mg[0] && !mg[1]) setup_rows_layout(ctx, F->rows);
     DevTransform tt;
     std::memset(&tt, 0, sizeof(tt));
     if (!affine_fast_update(affine_inputs(ctx, F->rows), ctx->mtf.params.data(), tt, *F)) return false;
@@ -1344,7 +1366,7 @@ int mmi_context_destroy(mmi_context* ctx) {
                     &ctx->nvalid, &ctx->pdf, &ctx->fm, &ctx->mm, &ctx->pratio, &ctx->pratio_f, &ctx->rowsum, &ctx->out, &ctx->points, &ctx->point_ids, &ctx->mask_px[0], &ctx->mask_px[1],
                     &ctx->mask_desc[0], &ctx->mask_desc[1],
                     &ctx->partials, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->minmax_out,
-                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_res, &ctx->cls, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->bs_rows_t1, &ctx->ftab, &ctx->work_a,
+                    &ctx->exc_count, &ctx->exc_list, &ctx->exc_res, &ctx->cls, &ctx->fbin8, &ctx->mpad, &ctx->tpad, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->bs_rows_t1, &ctx->ftab, &ctx->work_a,
                     &ctx->work_b, &ctx->work_grad};
   for (DevBuf* b : bufs) b->release();
   for (int j = 0; j < 3; ++j) {
@@ -1754,6 +1776,30 @@ int mmi_initialize(mmi_context* ctx) {
     MMI_HIP(ctx->sample_fbin.ensure(sizeof(int16_t) * n));
     MMI_HIP(ctx->sample_mbin.ensure(sizeof(int16_t) * n));
   }
+  // the fixed bins of a dense domain, for the affine row walks (bins fit a
+  // byte below kClsListed)
+  ctx->fbin8_valid = false;
+  ctx->pad_ok = ctx->pad_tex = false;
+  if (ctx->n_points == 0 && B <= 253 && n < (1ll << 31) && ctx->mtf.kind == MMI_TF_AFFINE) {
+    const mmi_geometry& mg = ctx->moving_g;
+    const int64_t px = mg.size[0] + 2, py = mg.size[1] + 2, pz = D == 3 ? mg.size[2] + 2 : 1;
+    const int64_t pn = px * py * pz;
+    const int vb = ctx->active_storage == MMI_F32 ? 4 : 2, tb = ctx->active_storage == MMI_F32 ? 16 : 8;
+    // 32-bit byte offsets and 24-bit index products in the kernels
+    if (pn * tb < (1ll << 32) && px * py < (1ll << 24) && pz < (1ll << 24)) {
+      MMI_HIP(ctx->fbin8.ensure((size_t)n));
+      MMI_HIP(ctx->mpad.ensure((size_t)(pn * vb)));
+      MMI_HIP(launch_pad_replicate(moving_pixels(ctx), ctx->mpad.p, vb, (int)mg.size[0], (int)mg.size[1],
+                                   D == 3 ? (int)mg.size[2] : 1, D, ctx->stream));
+      if (ctx->have_texels) {
+        MMI_HIP(ctx->tpad.ensure((size_t)(pn * tb)));
+        MMI_HIP(launch_pad_replicate(texel_pixels(ctx), ctx->tpad.p, tb, (int)mg.size[0], (int)mg.size[1],
+                                     D == 3 ? (int)mg.size[2] : 1, D, ctx->stream));
+        ctx->pad_tex = true;
+      }
+      ctx->pad_ok = true;
+    }
+  }
   ctx->initialized = true;
   ctx->acc_clean = false;  // the accumulators may have been reallocated
   return MMI_OK;
@@ -1961,6 +2007,21 @@ int prepare_eval(mmi_context* ctx, int32_t want_derivative, EvalArgs& a, FastArg
       fa.cd_wave_cap = (uint32_t)cap;
     }
   }
+  // Row walks over an unmasked dense domain take the fixed bins from the
+  // level's byte map (filled here once per level; a kernel launch, no
+  // allocation).  Pass 2 then classifies on its own and takes the moving
+  // values from the texels, which must hold the level's moving image.
+  if (fast && fa.mk == 1 && fa.rows.on && fb_rows_ok(ctx)) {
+    if (!ctx->fbin8_valid) {
+      MMI_HIP(launch_fixed_bins(a, ctx->fbin8.as<uint8_t>(), (uint32_t)n, std::max(8, ctx->grid), ctx->stream));
+      ctx->fbin8_valid = true;
+    }
+    fa.fbin8 = ctx->fbin8.as<uint8_t>();
+    fa.mpad = ctx->mpad.p;
+    fa.tpad = ctx->pad_tex ? ctx->tpad.p : nullptr;
+    fa.psy = (int32_t)(ctx->moving_g.size[0] + 2);
+    fa.psz = fa.psy * (int32_t)(ctx->moving_g.size[1] + 2);
+  }
   return MMI_OK;
 }
 
@@ -2039,7 +2100,7 @@ retry:
   // then also takes the moving values from the texels it gathers for the
   // gradient: their value channel must be the level's moving image
   if (fast && want_derivative && B <= 253 && !(ctx->flags & MMI_FLAG_PASS2_RECLASSIFY) &&
-      (ctx->texel_values_level || (fa.rows.on && fa.rows.cd_on))) {
+      (ctx->texel_values_level || (fa.rows.on && fa.rows.cd_on)) && !(fa.fbin8 && !fa.rows.cd_on)) {
     if (b.cls) {
       fa.cls = b.cls;
     } else {
@@ -2049,7 +2110,7 @@ retry:
   }
   // a row-walk pass 1 needs a pass 2 that follows its byte per sample (or the
   // exact pass 2 of central differences, which classifies on its own)
-  if (fast && fa.rows.on && want_derivative && !fa.cls && ctx->gradient_source == MMI_GRADIENT_IMAGE)
+  if (fast && fa.rows.on && want_derivative && !fa.cls && !fa.fbin8 && ctx->gradient_source == MMI_GRADIENT_IMAGE)
     fa.rows.on = 0;
   if (fast) {
     MMI_HIP(launch_fast_pass1(fa, a, D, ctx->grid, st));
@@ -2349,7 +2410,7 @@ int stage_fused(mmi_batch* b, mmi_context* const* ctxs, const std::vector<int>& 
   const bool cls = want_derivative != 0;
   for (int k = 0; k < nf; ++k) {
     fa[k].cls = nullptr;
-    if (cls) {
+    if (cls && !fa[k].fbin8) {
       mmi_context* c = ctxs[fused[k]];
       MMI_HIP(c->cls.ensure((size_t)sample_count(c)));
       fa[k].cls = c->cls.as<uint8_t>();

@@ -51,6 +51,31 @@ __device__ __forceinline__ double pixel_value(int32_t kind, double v) {
   return kind == kPixelDouble ? v : (kind == kPixelInteger ? trunc(v) : (double)(float)v);
 }
 
+// TransformAndEvaluateFixedPoint (itkImageToImageMetricv4.hxx:289-320),
+// exact FP64: false when the fixed point is masked out or outside the fixed
+// buffer, else the fixed value assigned to the pixel type.
+template <int D, typename P>
+__device__ __forceinline__ bool fixed_value_exact(const EvalArgs& a, const double* vp, double* fval) {
+  double fp[3], ci[3];
+  fixed_point<D>(a.ftf, vp, fp);
+  // the fixed mask at the mapped fixed point (itkImageToImageMetricv4.hxx:298-309)
+  if (a.fixed_mask && !mask_inside<D>(*a.fixed_mask, fp)) return false;
+  point_to_cindex<D>(a.fixed, fp, ci);
+  if (!inside_buffer<D>(a.fixed, ci)) return false;
+  const Cell fc = make_cell<D>(a.fixed, ci);
+  *fval = pixel_value(a.fixed_kind, interp_exact<D, P>(static_cast<const typename P::pixel*>(a.fixed_px), fc));
+  return true;
+}
+
+// The fixed Parzen window index of a fixed value
+// (ComputeSingleFixedImageParzenWindowIndex,
+// itkMattesMutualInformationImageToImageMetricv4.hxx:571-593, called at
+// itkMattesMutualInformationImageToImageMetricv4GetValueAndDerivativeThreader.hxx:269-270).
+__device__ __forceinline__ int fixed_bin_exact(const EvalArgs& a, double fval) {
+  const double fterm = fval / a.fixed_bin_size - a.fixed_norm_min;
+  return fterm < 2.0 ? 2 : (fterm >= (double)(a.bins - 2) ? a.bins - 3 : (int)fterm);
+}
+
 // ProcessVirtualPoint + Mattes ProcessPoint up to the bins
 // (itkImageToImageMetricv4GetValueAndDerivativeThreaderBase.hxx:174-273,
 // itkImageToImageMetricv4.hxx:289-373,
@@ -61,14 +86,9 @@ template <int D, int MK, typename P>
 __device__ __forceinline__ bool eval_sample(const EvalArgs& a, const double* vp, SampleOut<D>& o) {
 #pragma unroll
   for (int k = 0; k < D; ++k) o.vp[k] = vp[k];
-  double fp[3], ci[3];
-  fixed_point<D>(a.ftf, o.vp, fp);
-  // the fixed mask at the mapped fixed point (itkImageToImageMetricv4.hxx:298-309)
-  if (a.fixed_mask && !mask_inside<D>(*a.fixed_mask, fp)) return false;
-  point_to_cindex<D>(a.fixed, fp, ci);
-  if (!inside_buffer<D>(a.fixed, ci)) return false;
-  const Cell fc = make_cell<D>(a.fixed, ci);
-  const double fval = pixel_value(a.fixed_kind, interp_exact<D, P>(static_cast<const typename P::pixel*>(a.fixed_px), fc));
+  double ci[3];
+  double fval;
+  if (!fixed_value_exact<D, P>(a, o.vp, &fval)) return false;
 
   transform_point<D, MK>(a.mtf, o.vp, o.mp, &o.sup);
   // CompositeTransform::TransformPoint: the queue's earlier transforms after
@@ -99,9 +119,7 @@ __device__ __forceinline__ bool eval_sample(const EvalArgs& a, const double* vp,
   if (mval > a.moving_max) return false;
   const double mterm = mval / a.moving_bin_size - a.moving_norm_min;
   const int mb = mterm < 2.0 ? 2 : (mterm >= (double)(a.bins - 2) ? a.bins - 3 : (int)mterm);
-  const double fterm = fval / a.fixed_bin_size - a.fixed_norm_min;
-  const int fb = fterm < 2.0 ? 2 : (fterm >= (double)(a.bins - 2) ? a.bins - 3 : (int)fterm);
-  o.fbin = fb;
+  o.fbin = fixed_bin_exact(a, fval);
   o.mbin = mb;
   o.mterm = mterm;
   return true;

@@ -28,6 +28,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace mmi {
 
 // Samples of local workgroup lb of lg (one evaluation's share of a launch).
@@ -161,61 +163,98 @@ __device__ __forceinline__ void fast_mask(const FastMask& m, double x, double y,
 // channels) instead of the moving values and return the interpolated gradient
 // as well; the value is interpolated with the same operations either way, and
 // every fast-path kernel interpolates through this function.
+// The multilinear arithmetic of every fast-path interpolation, on corner
+// values already loaded (cXYZ: corner with upper x, y, z flags).  TEX: the
+// corners are texels (value, dI/dx, dI/dy, dI/dz); the value channel takes the
+// same operations as plain values, and the gradient channels those of
+// interp_gradient3.
+__device__ __forceinline__ float mlerp(float c000, float c100, float c010, float c110, float t0, float t1) {
+  const float vx00 = fmaf(c100 - c000, t0, c000);
+  const float vx10 = fmaf(c110 - c010, t0, c010);
+  return fmaf(vx10 - vx00, t1, vx00);
+}
+
+template <int D>
+__device__ __forceinline__ float mlerp_value(float c000, float c100, float c010, float c110, float c001, float c101,
+                                             float c011, float c111, float t0, float t1, float t2) {
+  float v = mlerp(c000, c100, c010, c110, t0, t1);
+  if (D == 3) {
+    const float vx01 = fmaf(c101 - c001, t0, c001);
+    const float vx11 = fmaf(c111 - c011, t0, c011);
+    const float vxx1 = fmaf(vx11 - vx01, t1, vx01);
+    v = fmaf(vxx1 - v, t2, v);
+  }
+  return v;
+}
+
+template <int D>
+__device__ __forceinline__ float mlerp_texels(const float4& a000, const float4& a100, const float4& a010,
+                                              const float4& a110, const float4& a001, const float4& a101,
+                                              const float4& a011, const float4& a111, float t0, float t1, float t2,
+                                              float3* grad) {
+  const float vx00 = fmaf(a100.x - a000.x, t0, a000.x);
+  const float vx10 = fmaf(a110.x - a010.x, t0, a010.x);
+  float v = fmaf(vx10 - vx00, t1, vx00);
+  const float gx0 = fmaf(a100.y - a000.y, t0, a000.y), gx1 = fmaf(a110.y - a010.y, t0, a010.y);
+  const float gy0 = fmaf(a100.z - a000.z, t0, a000.z), gy1 = fmaf(a110.z - a010.z, t0, a010.z);
+  const float gz0 = fmaf(a100.w - a000.w, t0, a000.w), gz1 = fmaf(a110.w - a010.w, t0, a010.w);
+  float3 g = make_float3(fmaf(gx1 - gx0, t1, gx0), fmaf(gy1 - gy0, t1, gy0), fmaf(gz1 - gz0, t1, gz0));
+  if (D == 3) {
+    const float vx01 = fmaf(a101.x - a001.x, t0, a001.x);
+    const float vx11 = fmaf(a111.x - a011.x, t0, a011.x);
+    const float vxx1 = fmaf(vx11 - vx01, t1, vx01);
+    v = fmaf(vxx1 - v, t2, v);
+    const float hx0 = fmaf(a101.y - a001.y, t0, a001.y), hx1 = fmaf(a111.y - a011.y, t0, a011.y);
+    const float hy0 = fmaf(a101.z - a001.z, t0, a001.z), hy1 = fmaf(a111.z - a011.z, t0, a011.z);
+    const float hz0 = fmaf(a101.w - a001.w, t0, a001.w), hz1 = fmaf(a111.w - a011.w, t0, a011.w);
+    const float s0 = fmaf(hx1 - hx0, t1, hx0), s1 = fmaf(hy1 - hy0, t1, hy0), s2 = fmaf(hz1 - hz0, t1, hz0);
+    g.x = fmaf(s0 - g.x, t2, g.x);
+    g.y = fmaf(s1 - g.y, t2, g.y);
+    g.z = fmaf(s2 - g.z, t2, g.z);
+  }
+  *grad = g;
+  return v;
+}
+
+// FP32 multilinear value of the moving image at a cell (corner element offsets
+// cb + {0, cx} + {0, cy} + {0, cz}, weights t0..t2; every offset non-negative
+// and inside the buffer).  TEX: gather the moving texels (value and gradient
+// channels) instead of the moving values and return the interpolated gradient
+// as well; the value is interpolated with the same operations either way, and
+// every fast-path kernel interpolates through mlerp_value / mlerp_texels.
 template <int D, typename P, bool TEX, bool BYTES = false>
 __device__ __forceinline__ float fast_interp(const FastArgs& f, uint32_t cb, uint32_t cx, uint32_t cy, uint32_t cz,
                                              float t0, float t1, float t2, float3* grad) {
-  float v;
   // BYTES: the offsets are byte offsets into the array read
 #define LDT(o) (BYTES ? P::ldtb(tex, (o)) : P::ldt32(tex, (o)))
 #define LDP(o) (BYTES ? P::ldb(img, (o)) : P::ld32(img, (o)))
   if constexpr (TEX) {
-    // value as below, gradient as interp_gradient3, from one texel load per corner
     const typename P::texel* tex = static_cast<const typename P::texel*>(f.texels);
     const float4 a000 = LDT(cb), a100 = LDT(cb + cx);
     const float4 a010 = LDT(cb + cy), a110 = LDT(cb + cy + cx);
-    const float vx00 = fmaf(a100.x - a000.x, t0, a000.x);
-    const float vx10 = fmaf(a110.x - a010.x, t0, a010.x);
-    v = fmaf(vx10 - vx00, t1, vx00);
-    const float gx0 = fmaf(a100.y - a000.y, t0, a000.y), gx1 = fmaf(a110.y - a010.y, t0, a010.y);
-    const float gy0 = fmaf(a100.z - a000.z, t0, a000.z), gy1 = fmaf(a110.z - a010.z, t0, a010.z);
-    const float gz0 = fmaf(a100.w - a000.w, t0, a000.w), gz1 = fmaf(a110.w - a010.w, t0, a010.w);
-    float3 g = make_float3(fmaf(gx1 - gx0, t1, gx0), fmaf(gy1 - gy0, t1, gy0), fmaf(gz1 - gz0, t1, gz0));
+    float4 a001 = a000, a101 = a000, a011 = a000, a111 = a000;
     if (D == 3) {
-      const float4 a001 = LDT(cb + cz), a101 = LDT(cb + cz + cx);
-      const float4 a011 = LDT(cb + cz + cy), a111 = LDT(cb + cz + cy + cx);
-      const float vx01 = fmaf(a101.x - a001.x, t0, a001.x);
-      const float vx11 = fmaf(a111.x - a011.x, t0, a011.x);
-      const float vxx1 = fmaf(vx11 - vx01, t1, vx01);
-      v = fmaf(vxx1 - v, t2, v);
-      const float hx0 = fmaf(a101.y - a001.y, t0, a001.y), hx1 = fmaf(a111.y - a011.y, t0, a011.y);
-      const float hy0 = fmaf(a101.z - a001.z, t0, a001.z), hy1 = fmaf(a111.z - a011.z, t0, a011.z);
-      const float hz0 = fmaf(a101.w - a001.w, t0, a001.w), hz1 = fmaf(a111.w - a011.w, t0, a011.w);
-      const float s0 = fmaf(hx1 - hx0, t1, hx0), s1 = fmaf(hy1 - hy0, t1, hy0), s2 = fmaf(hz1 - hz0, t1, hz0);
-      g.x = fmaf(s0 - g.x, t2, g.x);
-      g.y = fmaf(s1 - g.y, t2, g.y);
-      g.z = fmaf(s2 - g.z, t2, g.z);
+      a001 = LDT(cb + cz);
+      a101 = LDT(cb + cz + cx);
+      a011 = LDT(cb + cz + cy);
+      a111 = LDT(cb + cz + cy + cx);
     }
-    *grad = g;
+    return mlerp_texels<D>(a000, a100, a010, a110, a001, a101, a011, a111, t0, t1, t2, grad);
   } else {
     const typename P::pixel* img = static_cast<const typename P::pixel*>(f.moving_px);
     const float v000 = LDP(cb), v100 = LDP(cb + cx);
     const float v010 = LDP(cb + cy), v110 = LDP(cb + cy + cx);
-    const float vx00 = fmaf(v100 - v000, t0, v000);
-    const float vx10 = fmaf(v110 - v010, t0, v010);
-    v = fmaf(vx10 - vx00, t1, vx00);
+    float v001 = 0.0f, v101 = 0.0f, v011 = 0.0f, v111 = 0.0f;
     if (D == 3) {
-      const float v001 = LDP(cb + cz), v101 = LDP(cb + cz + cx);
-      const float v011 = LDP(cb + cz + cy);
-      const float v111 = LDP(cb + cz + cy + cx);
-      const float vx01 = fmaf(v101 - v001, t0, v001);
-      const float vx11 = fmaf(v111 - v011, t0, v011);
-      const float vxx1 = fmaf(vx11 - vx01, t1, vx01);
-      v = fmaf(vxx1 - v, t2, v);
+      v001 = LDP(cb + cz);
+      v101 = LDP(cb + cz + cx);
+      v011 = LDP(cb + cz + cy);
+      v111 = LDP(cb + cz + cy + cx);
     }
+    return mlerp_value<D>(v000, v100, v010, v110, v001, v101, v011, v111, t0, t1, t2);
   }
 #undef LDT
 #undef LDP
-  return v;
 }
 
 // The intensity part of a fast-path classification, after the coordinate
@@ -531,21 +570,16 @@ __device__ __forceinline__ void rows_step(const FastArgs& f, RowWalk<D>& w) {
   ++w.y;
 }
 
-// fast_classify of a row-walk sample: buffer tests against the fixed-point
+// The cell of a row-walk sample: buffer tests against the fixed-point
 // thresholds, the cell from the high word (clamped into the buffer, the upper
 // corner duplicated outside [start, end - 1] as in the reference's branch
-// rules), the weights from the low word; interpolation and bins as
-// fast_classify.
-template <int D, typename P, bool TEX = false>
-__device__ __forceinline__ int rows_classify(const FastArgs& f, const RowWalk<D>& w, int& fbin, int& mbin,
-                                             float& arg0, float3* grad = nullptr) {
+// rules), the weights from the low word.  cb and off[] are byte offsets into
+// the moving values (E bytes per element) or texels.
+template <int D, uint32_t E>
+__device__ __forceinline__ void rows_cell(const FastArgs& f, const RowWalk<D>& w, bool& out, bool& uncertain,
+                                          uint32_t& cb, uint32_t* off, float* t) {
   const FastRows& R = f.rows;
-  // byte offsets into the moving values (or texels): scaled once per sample
-  constexpr uint32_t E = TEX ? (uint32_t)sizeof(typename P::texel) : (uint32_t)sizeof(typename P::pixel);
-  bool out = false, uncertain = false;
   int32_t fl[3] = {0, 0, 0};
-  uint32_t off[3] = {0u, 0u, 0u};
-  float t[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     const long long q = w.q[k];
@@ -559,9 +593,23 @@ __device__ __forceinline__ int rows_classify(const FastArgs& f, const RowWalk<D>
     t[k] = (float)(uint32_t)q * 2.3283064365386963e-10f;  // 2^-32
   }
   // 24-bit products: the host checks the moving extents and strides
-  uint32_t cb = (uint32_t)(fl[0] - f.mstart[0]) + __umul24((uint32_t)f.msy, (uint32_t)(fl[1] - f.mstart[1]));
+  cb = (uint32_t)(fl[0] - f.mstart[0]) + __umul24((uint32_t)f.msy, (uint32_t)(fl[1] - f.mstart[1]));
   if (D == 3) cb += __umul24((uint32_t)f.msz, (uint32_t)(fl[2] - f.mstart[2]));
-  const float v = fast_interp<D, P, TEX, true>(f, cb * E, off[0], off[1], off[2], t[0], t[1], t[2], grad);
+  cb *= E;
+}
+
+// fast_classify of a row-walk sample: the cell (rows_cell); interpolation and
+// bins as fast_classify.
+template <int D, typename P, bool TEX = false>
+__device__ __forceinline__ int rows_classify(const FastArgs& f, const RowWalk<D>& w, int& fbin, int& mbin,
+                                             float& arg0, float3* grad = nullptr) {
+  // byte offsets into the moving values (or texels): scaled once per sample
+  constexpr uint32_t E = TEX ? (uint32_t)sizeof(typename P::texel) : (uint32_t)sizeof(typename P::pixel);
+  bool out = false, uncertain = false;
+  uint32_t cb = 0u, off[3] = {0u, 0u, 0u};
+  float t[3] = {0.0f, 0.0f, 0.0f};
+  rows_cell<D, E>(f, w, out, uncertain, cb, off, t);
+  const float v = fast_interp<D, P, TEX, true>(f, cb, off[0], off[1], off[2], t[0], t[1], t[2], grad);
   // pass 2 (TEX) takes the fixed bin from pass 1's byte: no fixed read
   const float fv = TEX ? 0.0f : P::ldb(static_cast<const typename P::pixel*>(f.fixed_px), w.fp);
   return fast_bins(f, v, fv, out, uncertain, fbin, mbin, arg0, nullptr);
@@ -582,6 +630,299 @@ __device__ __forceinline__ void pass1_rows_body(const FastArgs& f, const EvalArg
       float arg0 = 0.0f;
       const int c = rows_classify<D, P>(f, w, fbin, mbin, arg0);
       pass1_add(f, L, lb, w.s, c, fbin, mbin, arg0, nvalid);
+    }
+  }
+  pass1_end<D, P, 1>(f, a, L, lb, nvalid);
+}
+
+// ---------------------------------------------------------------------------
+// Row walks with the exact fixed bins (FastArgs::fbin8).  The fixed side of a
+// sample is one byte, and the moving value needs no range test (the moving
+// range is the min/max of every value the interpolation reads).  The moving
+// values and texels are read from copies with a one-voxel replicated border
+// (FastArgs::mpad, tpad), so a sample certainly inside the moving buffer
+// needs no clamped cell: its lower corner lies in [start - 1, end] and the
+// border holds the duplicates of the reference's branch rules.  The moving
+// continuous index is monotone along a walk, so the walk's first and last
+// samples give each lane's class: every sample certainly inside, or every
+// sample certainly outside (one axis below or above the buffer at both
+// ends), or neither.  A wavefront without a lane of the third class takes a
+// loop with no buffer test (outside lanes idle); the others test each sample
+// (the inside test, then, for the few that fail it, the outside test).
+// Either loop gives a sample the same class, value and bins, in both passes.
+// ---------------------------------------------------------------------------
+
+// Parzen weights of an unclamped window, t = mterm - mbin in [0, 1): bins
+// mbin - 1 .. mbin + 2 take B3(1 + t), B3(t), B3(1 - t), B3(2 - t), that is
+// u^3/6, (3t^3 - 6t^2 + 4)/6, (3u^3 - 6u^2 + 4)/6, t^3/6 with u = 1 - t, as
+// 2^32 fixed point truncated; the second takes the remainder, so the four sum
+// to exactly 2^32.
+__device__ __forceinline__ void parzen_fixed_t(float t, unsigned int* q) {
+  constexpr float C = 4294967296.0f;
+  const float u = 1.0f - t;
+  const float t2 = t * t, u2 = u * u;
+  q[0] = (unsigned int)(u2 * (u * (C / 6.0f)));
+  q[2] = (unsigned int)fmaf(u2, fmaf(0.5f * C, u, -C), (2.0f / 3.0f) * C);
+  q[3] = (unsigned int)(t2 * (t * (C / 6.0f)));
+  q[1] = 0u - q[0] - q[2] - q[3];
+}
+
+// The moving bin of a fast value, its certainty (fast_bin) and the Parzen
+// fraction t = mterm - mbin (exact by Sterbenz's lemma for mterm in [mbin,
+// mbin + 1)); t leaves [0, 1) only where the term was clamped.
+__device__ __forceinline__ int fb_bin(const FastArgs& f, float v, bool* uncertain, float* t) {
+  const float mterm = fmaf(v, f.inv_mbs, -f.mnm);
+  const int mbin = fast_bin(mterm, f.bt_m, f.bins, uncertain);
+  *t = mterm - (float)mbin;
+  return mbin;
+}
+
+// The buffer tests of the padded walks, in fixed point relative to the padded
+// origin (start - 1 on every axis): [in_lo, in_lo + in_span) certainly inside,
+// below out_lo or at out_hi and above certainly outside.
+template <int D>
+struct PadTests {
+  long long in_lo[3], out_lo[3], out_hi[3];
+  unsigned long long in_span[3];
+  long long shift[3];  // (start - 1) * 2^32
+};
+
+template <int D>
+__device__ __forceinline__ PadTests<D> pad_tests(const FastArgs& f) {
+  PadTests<D> T;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    T.shift[k] = (long long)((unsigned long long)(uint32_t)(f.mstart[k] - 1) << 32);
+    T.in_lo[k] = f.rows.lo_in[k] - T.shift[k];
+    T.in_span[k] = f.rows.span_in[k];
+    T.out_lo[k] = f.rows.lo_out[k] - T.shift[k];
+    T.out_hi[k] = T.out_lo[k] + (long long)f.rows.span_out[k];
+  }
+  return T;
+}
+
+// The lane's walk relative to the padded origin and its class: 0 every sample
+// certainly inside, 1 every sample certainly outside, 2 neither.
+template <int D>
+__device__ __forceinline__ int pad_walk(const FastArgs& f, const PadTests<D>& T, const RowWalk<D>& w, long long* q) {
+  bool in = true, out = false;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    q[k] = w.q[k] - T.shift[k];
+    const long long q1 = q[k] + (long long)(w.rows - 1) * f.rows.qy[k];
+    in = in & ((unsigned long long)(q[k] - T.in_lo[k]) < T.in_span[k]) &
+         ((unsigned long long)(q1 - T.in_lo[k]) < T.in_span[k]);
+    out = out | ((q[k] < T.out_lo[k]) & (q1 < T.out_lo[k])) | ((q[k] >= T.out_hi[k]) & (q1 >= T.out_hi[k]));
+  }
+  return in ? 0 : (out ? 1 : 2);
+}
+
+template <int D>
+__device__ __forceinline__ bool pad_inside(const PadTests<D>& T, const long long* q) {
+  bool in = true;
+#pragma unroll
+  for (int k = 0; k < D; ++k) in = in & ((unsigned long long)(q[k] - T.in_lo[k]) < T.in_span[k]);
+  return in;
+}
+
+template <int D>
+__device__ __forceinline__ bool pad_outside(const PadTests<D>& T, const long long* q) {
+  bool out = false;
+#pragma unroll
+  for (int k = 0; k < D; ++k) out = out | (q[k] < T.out_lo[k]) | (q[k] >= T.out_hi[k]);
+  return out;
+}
+
+// The cell of a certainly inside sample in the padded copy: byte offset of its
+// lower corner and the weights.
+template <int D, uint32_t E>
+__device__ __forceinline__ uint32_t pad_cell(const FastArgs& f, const long long* q, float* t) {
+  uint32_t cb = (uint32_t)(q[0] >> 32) + __umul24((uint32_t)f.psy, (uint32_t)(q[1] >> 32));
+  if (D == 3) cb += __umul24((uint32_t)f.psz, (uint32_t)(q[2] >> 32));
+#pragma unroll
+  for (int k = 0; k < D; ++k) t[k] = (float)(uint32_t)q[k] * 2.3283064365386963e-10f;  // 2^-32
+  return cb * E;
+}
+
+// Multilinear value (TEX: and gradient) at a padded cell: one address per
+// (y, z) corner pair, the x neighbour an immediate offset; mlerp_value /
+// mlerp_texels as every fast-path interpolation.
+template <int D, typename P, bool TEX>
+__device__ __forceinline__ float pad_interp(const void* base, uint32_t cb, uint32_t sy, uint32_t sz, const float* t,
+                                            float3* grad) {
+  if constexpr (TEX) {
+    using T = typename P::texel;
+    constexpr uint32_t E = (uint32_t)sizeof(T);
+    const char* p00 = static_cast<const char*>(base) + cb;
+    const char* p10 = p00 + sy;
+    const float4 a000 = P::ldtb(reinterpret_cast<const T*>(p00), 0u), a100 = P::ldtb(reinterpret_cast<const T*>(p00), E);
+    const float4 a010 = P::ldtb(reinterpret_cast<const T*>(p10), 0u), a110 = P::ldtb(reinterpret_cast<const T*>(p10), E);
+    float4 a001 = a000, a101 = a000, a011 = a000, a111 = a000;
+    if (D == 3) {
+      const char* p01 = p00 + sz;
+      const char* p11 = p10 + sz;
+      a001 = P::ldtb(reinterpret_cast<const T*>(p01), 0u);
+      a101 = P::ldtb(reinterpret_cast<const T*>(p01), E);
+      a011 = P::ldtb(reinterpret_cast<const T*>(p11), 0u);
+      a111 = P::ldtb(reinterpret_cast<const T*>(p11), E);
+    }
+    return mlerp_texels<D>(a000, a100, a010, a110, a001, a101, a011, a111, t[0], t[1], t[2], grad);
+  } else {
+    using T = typename P::pixel;
+    constexpr uint32_t E = (uint32_t)sizeof(T);
+    const char* p00 = static_cast<const char*>(base) + cb;
+    const char* p10 = p00 + sy;
+    const float v000 = P::ldb(reinterpret_cast<const T*>(p00), 0u), v100 = P::ldb(reinterpret_cast<const T*>(p00), E);
+    const float v010 = P::ldb(reinterpret_cast<const T*>(p10), 0u), v110 = P::ldb(reinterpret_cast<const T*>(p10), E);
+    float v001 = 0.0f, v101 = 0.0f, v011 = 0.0f, v111 = 0.0f;
+    if (D == 3) {
+      const char* p01 = p00 + sz;
+      const char* p11 = p10 + sz;
+      v001 = P::ldb(reinterpret_cast<const T*>(p01), 0u);
+      v101 = P::ldb(reinterpret_cast<const T*>(p01), E);
+      v011 = P::ldb(reinterpret_cast<const T*>(p11), 0u);
+      v111 = P::ldb(reinterpret_cast<const T*>(p11), E);
+    }
+    return mlerp_value<D>(v000, v100, v010, v110, v001, v101, v011, v111, t[0], t[1], t[2]);
+  }
+}
+
+// The loads of one certainly inside sample of a padded walk: its fixed bin,
+// its cell's eight corners (TEX: texels) and weights.
+template <int D, typename P, bool TEX>
+struct PadLoad {
+  using V = typename std::conditional<TEX, float4, float>::type;
+  V c[8];
+  float t[3];
+  uint32_t fb;
+};
+
+template <int D, typename P, bool TEX>
+__device__ __forceinline__ void pad_load(const FastArgs& f, uint32_t s, const long long* q, uint32_t sy, uint32_t sz,
+                                         PadLoad<D, P, TEX>& L) {
+  using T = typename std::conditional<TEX, typename P::texel, typename P::pixel>::type;
+  constexpr uint32_t E = (uint32_t)sizeof(T);
+  L.fb = f.fbin8[s];
+  L.t[0] = L.t[1] = L.t[2] = 0.0f;
+  const uint32_t cb = pad_cell<D, E>(f, q, L.t);
+  const char* p00 = static_cast<const char*>(TEX ? f.tpad : f.mpad) + cb;
+  const char* p10 = p00 + sy;
+  auto ld = [](const char* p, uint32_t o) {
+    if constexpr (TEX)
+      return P::ldtb(reinterpret_cast<const T*>(p), o);
+    else
+      return P::ldb(reinterpret_cast<const T*>(p), o);
+  };
+  L.c[0] = ld(p00, 0u);
+  L.c[1] = ld(p00, E);
+  L.c[2] = ld(p10, 0u);
+  L.c[3] = ld(p10, E);
+  if (D == 3) {
+    const char* p01 = p00 + sz;
+    const char* p11 = p10 + sz;
+    L.c[4] = ld(p01, 0u);
+    L.c[5] = ld(p01, E);
+    L.c[6] = ld(p11, 0u);
+    L.c[7] = ld(p11, E);
+  } else {
+    L.c[4] = L.c[5] = L.c[6] = L.c[7] = L.c[0];
+  }
+}
+
+// The multilinear value (TEX: and gradient) of loaded corners.
+template <int D, typename P, bool TEX>
+__device__ __forceinline__ float pad_value(const PadLoad<D, P, TEX>& L, float3* grad) {
+  if constexpr (TEX)
+    return mlerp_texels<D>(L.c[0], L.c[1], L.c[2], L.c[3], L.c[4], L.c[5], L.c[6], L.c[7], L.t[0], L.t[1], L.t[2],
+                           grad);
+  else
+    return mlerp_value<D>(L.c[0], L.c[1], L.c[2], L.c[3], L.c[4], L.c[5], L.c[6], L.c[7], L.t[0], L.t[1], L.t[2]);
+}
+
+// pass1_add with the Parzen fraction t instead of arg0.
+__device__ __forceinline__ void pass1_add_t(const FastArgs& f, const P1Lds& L, uint32_t lb, uint32_t s, int c,
+                                           uint32_t fbin, int mbin, float t, unsigned int& nvalid) {
+  if (f.cls) f.cls[s] = (uint8_t)(c == 1 ? fbin : (c == 2 ? kClsListed : kClsSkip));
+  if (c == 2) {
+    const unsigned int slot = atomicAdd(&L.counters[1], 1u);
+    if (slot < (unsigned int)kExcCap) f.exc_list[lb * kExcCap + slot] = s;
+    return;
+  }
+  if (f.sample_fbin) {
+    f.sample_fbin[s] = c == 1 ? (int16_t)fbin : (int16_t)-1;
+    f.sample_mbin[s] = c == 1 ? (int16_t)mbin : (int16_t)-1;
+  }
+  if (c == 0) return;
+  ++nvalid;
+  unsigned int q[4];
+  if (t >= 0.0f && t < 1.0f)
+    parzen_fixed_t(t, q);
+  else
+    parzen_fixed(-1.0f - t, q);
+  // mbin is clamped to [2, B-3], so the 4 bins stay in the row
+  unsigned long long* row = L.hist + __umul24(fbin, (uint32_t)f.bins) + (mbin - 1);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) atomicAdd(&row[k], (unsigned long long)q[k]);
+}
+
+// Pass 1 of a certainly inside sample from its loads: value, bins, class,
+// accumulation.
+template <int D, typename P>
+__device__ __forceinline__ void pass1_pad_compute(const FastArgs& f, const P1Lds& L, uint32_t lb, uint32_t s,
+                                                  const PadLoad<D, P, false>& x, unsigned int& nvalid) {
+  const float v = pad_value<D, P, false>(x, nullptr);
+  bool unc = false;
+  float t;
+  const int mbin = fb_bin(f, v, &unc, &t);
+  pass1_add_t(f, L, lb, s, x.fb == kClsSkip ? 0 : (unc ? 2 : 1), x.fb, mbin, t, nvalid);
+}
+
+template <int D, typename P>
+__device__ __forceinline__ void pass1_pad_sample(const FastArgs& f, const P1Lds& L, uint32_t lb, uint32_t s,
+                                                 const long long* q, uint32_t sy, uint32_t sz, unsigned int& nvalid) {
+  PadLoad<D, P, false> x;
+  pad_load<D, P, false>(f, s, q, sy, sz, x);
+  pass1_pad_compute<D, P>(f, L, lb, s, x, nvalid);
+}
+
+template <int D, typename P>
+__device__ __forceinline__ void pass1_rows_fb_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
+  const P1Lds L = pass1_begin(f.bins, f.hist_copies);
+  unsigned int nvalid = 0;
+  constexpr uint32_t E = (uint32_t)sizeof(typename P::pixel);
+  const uint32_t sy = (uint32_t)f.psy * E, sz = (uint32_t)f.psz * E;
+  const uint32_t nx = (uint32_t)f.rows.nx;
+  const PadTests<D> T = pad_tests<D>(f);
+  uint32_t it, iend, istep;
+  rows_range(f, lb, lg, it, iend, istep);
+  for (; it < iend; it += istep) {
+    RowWalk<D> w;
+    rows_start<D, P>(f, it, w);
+    long long q[3] = {0, 0, 0};
+    const int cls = w.active ? pad_walk<D>(f, T, w, q) : 1;
+    uint32_t s = w.s;
+    if (__all(cls != 2)) {
+      if (cls == 0) {
+        for (int j = 0; j < w.rows; ++j, s += nx) {
+          pass1_pad_sample<D, P>(f, L, lb, s, q, sy, sz, nvalid);
+#pragma unroll
+          for (int k = 0; k < D; ++k) q[k] += f.rows.qy[k];
+        }
+      } else if (w.active && (f.cls || f.sample_fbin)) {
+        // a walk outside the buffer: its records say not counted
+        for (int j = 0; j < w.rows; ++j, s += nx) pass1_add_t(f, L, lb, s, 0, 0u, 0, 0.0f, nvalid);
+      }
+    } else {
+      for (int j = 0; j < w.rows; ++j, s += nx) {
+        if (w.active) {
+          if (pad_inside<D>(T, q))
+            pass1_pad_sample<D, P>(f, L, lb, s, q, sy, sz, nvalid);
+          else
+            pass1_add_t(f, L, lb, s, (pad_outside<D>(T, q) || f.fbin8[s] == kClsSkip) ? 0 : 2, 0u, 0, 0.0f, nvalid);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) q[k] += f.rows.qy[k];
+      }
     }
   }
   pass1_end<D, P, 1>(f, a, L, lb, nvalid);
@@ -938,6 +1279,100 @@ __device__ __forceinline__ void pass2_rows_body(const FastArgs& f, const EvalArg
   exceptions_pass2_body<D, P, 1>(a, f, lb, lg);
 }
 
+// Parzen derivative weight from the fraction t (parzen_deriv_weight_fast).
+__device__ __forceinline__ float parzen_deriv_weight_of_t(float t, const float* prow) {
+  if (t >= 0.0f && t < 1.0f) return parzen_deriv_weight_t(t, prow);
+  return parzen_deriv_weight(-1.0f - t, prow);
+}
+
+// Pass 2 of a certainly inside sample from its loads: when pass 1 counted it
+// as certain, its Parzen derivative weight times the interpolated gradient
+// into the walk's FP32 sums (pass2_rows_body).
+template <int D, typename P>
+__device__ __forceinline__ void pass2_pad_compute(const FastArgs& f, const float* pr, const PadLoad<D, P, true>& x,
+                                                  float yf, float* sg, float* sy_) {
+  if (x.fb == kClsSkip) return;
+  float3 g;
+  const float v = pad_value<D, P, true>(x, &g);
+  bool unc = false;
+  float t;
+  const int mbin = fb_bin(f, v, &unc, &t);
+  if (unc) return;
+  const float wt = parzen_deriv_weight_of_t(t, pr + __umul24(x.fb, (uint32_t)f.bins) + (mbin - 1));
+  const float gw[3] = {wt * g.x, wt * g.y, wt * g.z};
+#pragma unroll
+  for (int b = 0; b < D; ++b) {
+    sg[b] += gw[b];
+    sy_[b] = fmaf(yf, gw[b], sy_[b]);
+  }
+}
+
+// Pass 2 over row walks with the exact fixed bins: it classifies each sample
+// as pass 1 did (the same walks, classes and arithmetic), so it needs no byte
+// from pass 1; counted certain samples add their moments as pass2_rows_body.
+template <int D, typename P>
+__device__ __forceinline__ void pass2_rows_fb_body(const FastArgs& f, const EvalArgs& a, uint32_t lb, uint32_t lg) {
+  extern __shared__ float lds_f32[];
+  const int B = f.bins;
+  const int BB = B * B;
+  double* scratch = reinterpret_cast<double*>(lds_f32);  // 4 wavefronts x 12 doubles (128 floats)
+  float* pr = lds_f32 + 128;
+  for (int i = threadIdx.x; i < BB; i += blockDim.x) pr[i] = f.pratio_f[i];
+  __syncthreads();
+  constexpr int PA = D * D + D;
+  double acc[PA];
+#pragma unroll
+  for (int i = 0; i < PA; ++i) acc[i] = 0.0;
+  constexpr uint32_t E = (uint32_t)sizeof(typename P::texel);
+  const uint32_t sy = (uint32_t)f.psy * E, sz = (uint32_t)f.psz * E;
+  const uint32_t nx = (uint32_t)f.rows.nx;
+  const PadTests<D> T = pad_tests<D>(f);
+  uint32_t it, iend, istep;
+  rows_range(f, lb, lg, it, iend, istep);
+  for (; it < iend; it += istep) {
+    RowWalk<D> w;
+    rows_start<D, P>(f, it, w);
+    long long q[3] = {0, 0, 0};
+    const int cls = w.active ? pad_walk<D>(f, T, w, q) : 1;
+    float sg[3] = {0.0f, 0.0f, 0.0f}, sy_[3] = {0.0f, 0.0f, 0.0f};
+    float yf = (float)w.y;
+    uint32_t s = w.s;
+    if (__all(cls != 2)) {
+      if (cls == 0) {
+        for (int j = 0; j < w.rows; ++j, s += nx, yf += 1.0f) {
+          PadLoad<D, P, true> x;
+          pad_load<D, P, true>(f, s, q, sy, sz, x);
+          pass2_pad_compute<D, P>(f, pr, x, yf, sg, sy_);
+#pragma unroll
+          for (int k = 0; k < D; ++k) q[k] += f.rows.qy[k];
+        }
+      }
+    } else {
+      for (int j = 0; j < w.rows; ++j, s += nx, yf += 1.0f) {
+        if (w.active && pad_inside<D>(T, q)) {
+          PadLoad<D, P, true> x;
+          pad_load<D, P, true>(f, s, q, sy, sz, x);
+          pass2_pad_compute<D, P>(f, pr, x, yf, sg, sy_);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) q[k] += f.rows.qy[k];
+      }
+    }
+    // the walk's sums into the thread's FP64 moments (x and z are the walk's)
+    const double x = (double)w.x, z = (double)w.z;
+#pragma unroll
+    for (int b = 0; b < D; ++b) {
+      acc[b * D + 0] = fma((double)sg[b], x, acc[b * D + 0]);
+      acc[b * D + 1] += (double)sy_[b];
+      if (D == 3) acc[b * D + 2] = fma((double)sg[b], z, acc[b * D + 2]);
+      acc[D * D + b] += (double)sg[b];
+    }
+  }
+  block_reduce_array<PA>(acc, scratch, f.partials + (int64_t)lb * PA);
+  // this workgroup's listed samples, exactly, into their own partial row
+  exceptions_pass2_body<D, P, 1>(a, f, lb, lg);
+}
+
 // Pass 2 over row walks with central differences of the moving image
 // (CentralDifferenceImageFunction::EvaluateSpecialized,
 // itkCentralDifferenceImageFunction.hxx:252-318) on a moving geometry with a
@@ -1171,7 +1606,8 @@ __global__ void __launch_bounds__(256) pass1_fast_batch(const FastArgs* __restri
   const int r = batch_item(start, n, blockIdx.x);
   if (halted && halted[r]) return;
   if (fa[r].rows.on)
-    pass1_rows_body<D, P>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+    fa[r].fbin8 ? pass1_rows_fb_body<D, P>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r])
+                : pass1_rows_body<D, P>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
   else
     pass1_fast_body<D, P, 1>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
 }
@@ -1184,19 +1620,42 @@ __global__ void __launch_bounds__(256) pass2_cls_batch(const FastArgs* __restric
   const int r = batch_item(start, n, blockIdx.x);
   if (halted && halted[r]) return;
   if (fa[r].rows.on)
-    pass2_rows_body<D, P>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
+    fa[r].fbin8 ? pass2_rows_fb_body<D, P>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r])
+                : pass2_rows_body<D, P>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
   else
     pass2_fast_body<D, P, 1, true, true>(fa[r], ea[r], blockIdx.x - start[r], start[r + 1] - start[r]);
 }
 
+// The exact fixed Parzen bin of every virtual voxel (the fixed side of
+// eval_sample: fixed point, fixed mask, buffer test, FP64 interpolation, pixel
+// assignment, ComputeSingleFixedImageParzenWindowIndex), or kClsSkip where
+// the fixed side does not count the voxel.  Once per level: the fixed image,
+// its range, the virtual domain and the identity fixed transform do not change
+// between evaluations.
+template <int D, typename P>
+__global__ void __launch_bounds__(256) fixed_bins_kernel(EvalArgs a, uint8_t* __restrict__ out, uint32_t n) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    double vp[3];
+    virtual_point<D>(a, s, vp);
+    double fval;
+    out[s] = fixed_value_exact<D, P>(a, vp, &fval) ? (uint8_t)fixed_bin_exact(a, fval) : (uint8_t)kClsSkip;
+  }
+}
+
 template <int D, typename P>
 __global__ void __launch_bounds__(256) pass1_rows_kernel(FastArgs f, EvalArgs a) {
-  pass1_rows_body<D, P>(f, a, blockIdx.x, gridDim.x);
+  if (f.fbin8)
+    pass1_rows_fb_body<D, P>(f, a, blockIdx.x, gridDim.x);
+  else
+    pass1_rows_body<D, P>(f, a, blockIdx.x, gridDim.x);
 }
 
 template <int D, typename P>
 __global__ void __launch_bounds__(256) pass2_rows_kernel(FastArgs f, EvalArgs a) {
-  pass2_rows_body<D, P>(f, a, blockIdx.x, gridDim.x);
+  if (f.fbin8)
+    pass2_rows_fb_body<D, P>(f, a, blockIdx.x, gridDim.x);
+  else
+    pass2_rows_body<D, P>(f, a, blockIdx.x, gridDim.x);
 }
 
 template <int D, typename P>
@@ -1369,6 +1828,11 @@ hipError_t launch_fast_pass1(const FastArgs& f, const EvalArgs& a, int dim, int 
   return hipGetLastError();
 }
 
+hipError_t launch_fixed_bins(const EvalArgs& a, uint8_t* out, uint32_t n, int grid, hipStream_t st) {
+  MMI_DISPATCH_FAST(fixed_bins_kernel, a.storage, a.virt.dim, grid, 0, st, a, out, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st) {
   // grid (the pass-1 workgroups) is a multiple of 8 = kExcListsPerGroup
   MMI_DISPATCH_FAST_MK(exceptions_pass1_kernel, f.mk, f.storage, a.virt.dim, grid / kExcListsPerGroup, 0, st, a, f);
@@ -1378,8 +1842,9 @@ hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int gri
 hipError_t launch_fast_pass2(const FastArgs& f, const EvalArgs& a, int dim, int grid, hipStream_t st) {
   const size_t lds = ((size_t)f.bins * f.bins + 128) * 4;
   // a row-walk pass 1 is followed by a pass 2 that reads its byte per sample
-  if (f.rows.on && !f.cls) return hipErrorInvalidValue;
+  if (f.rows.on && !f.cls && !f.fbin8) return hipErrorInvalidValue;
   if (f.rows.on && f.rows.cd_on) {
+    if (!f.cls) return hipErrorInvalidValue;
     MMI_DISPATCH_FAST(pass2_rows_cd_kernel, f.storage, dim, grid, lds, st, f, a);
     MMI_DISPATCH_FAST(pass2_cd_tail_kernel, f.storage, dim, grid, 0, st, f, a);
   } else if (f.rows.on)

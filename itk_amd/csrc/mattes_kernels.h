@@ -152,6 +152,23 @@ struct FastArgs {
   // again): the fixed bin of a counted certain sample, kClsListed for a
   // sample on the exact path, kClsSkip for one not counted.
   uint8_t* cls;
+  // The exact fixed Parzen bin of every virtual sample, or kClsSkip where the
+  // fixed side does not count it (fixed_bins_kernel, once per level; NULL:
+  // the kernels read the fixed image).  Set only for row walks over an
+  // unmasked dense domain, where the moving range is the min/max of every
+  // moving value the interpolation reads, so the reference's range test
+  // (itkMattesMutualInformationImageToImageMetricv4GetValueAndDerivativeThreader.hxx:238-245)
+  // passes for every sample inside the moving buffer: an FP64 multilinear
+  // value of corners in [min, max] stays in [min, max] (each lerp
+  // a + (b - a) t with exact b - a and t in [0, 1] rounds between a and b).
+  const uint8_t* fbin8;
+  // With fbin8: the moving values (mpad) and texels (tpad; NULL without a
+  // gradient image) with a one-voxel replicated border (pad_replicate_kernel),
+  // element strides psy, psz: a certainly inside sample's cell, lower corner
+  // in [start - 1, end], needs no clamp there.
+  const void* mpad;
+  const void* tpad;
+  int32_t psy, psz;
   FastMask fmask, mmask;  // fixed mask at the fixed point, moving mask at the mapped moving point
   FastRows rows;          // rows.on: the row-walk kernels (pass 2 then needs cls)
 };
@@ -353,6 +370,8 @@ hipError_t launch_pass2(const EvalArgs& a, int grid, hipStream_t st);
 // samples on the exact path (the EvalArgs) into its LDS histogram.
 hipError_t launch_fast_pass1(const FastArgs& f, const EvalArgs& a, int dim, int grid, hipStream_t st);
 // B-spline fast path: pass 1's listed samples on the exact path.
+// The exact fixed bin of each of the n dense virtual samples (FastArgs::fbin8).
+hipError_t launch_fixed_bins(const EvalArgs& a, uint8_t* out, uint32_t n, int grid, hipStream_t st);
 hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st);
 // Pass 2 of the fast path; affine workgroups end with their listed samples on
 // the exact path (their own partial row).
@@ -378,6 +397,9 @@ hipError_t launch_separable_contract(const float* in_f, const double* in_d, doub
 hipError_t launch_minmax(const void* px, int storage, int64_t n, float* out, int blocks, hipStream_t st);
 // FP32 image -> 16-bit storage (round to nearest even), and FP32 texels -> 16-bit texels.
 hipError_t launch_to_storage(const float* in, void* out, int storage, int64_t n, hipStream_t st);
+// The buffer with a one-element replicated border on each axis (pad_replicate_kernel).
+hipError_t launch_pad_replicate(const void* src, void* dst, int elem_bytes, int nx, int ny, int nz, int D,
+                                hipStream_t st);
 hipError_t launch_texels_to_storage(const float4* in, void* out, int storage, int64_t n, hipStream_t st);
 hipError_t launch_recursive_gaussian(const void* in, bool in_float, double* out, int64_t ln, int64_t stride,
                                      int64_t nlines, const RecCoeffs& c, const RecSink& sink, hipStream_t st);

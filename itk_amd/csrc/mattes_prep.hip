@@ -594,6 +594,55 @@ hipError_t launch_pack_texels(const float* moving, const double* grad, int64_t n
   return hipGetLastError();
 }
 
+// A copy of an image buffer (nx x ny x nz elements, x fastest) with a
+// one-element border on each side of every axis (z only in 3-D), each border
+// element a copy of its nearest buffer element.  A linear interpolation on the
+// copy at any cell whose lower corner lies in [start - 1, end] gives the bits
+// the reference's branch rules give in the buffer: a corner the reference
+// drops is duplicated (lerp(a, a, t) == a), and the border holds exactly that
+// duplicate.
+template <typename T>
+__global__ void pad_replicate_kernel(const T* __restrict__ src, T* __restrict__ dst, int nx, int ny, int nz, int D) {
+  const int px = nx + 2, py = ny + 2, pz = D == 3 ? nz + 2 : 1;
+  const int64_t n = (int64_t)px * py * pz;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % px);
+    const int64_t r = i / px;
+    const int y = (int)(r % py);
+    const int z = (int)(r / py);
+    const int sx = min(max(x - 1, 0), nx - 1), sy = min(max(y - 1, 0), ny - 1);
+    const int sz = D == 3 ? min(max(z - 1, 0), nz - 1) : 0;
+    dst[i] = src[sx + (int64_t)nx * (sy + (int64_t)ny * sz)];
+  }
+}
+
+hipError_t launch_pad_replicate(const void* src, void* dst, int elem_bytes, int nx, int ny, int nz, int D,
+                                hipStream_t st) {
+  const int64_t n = (int64_t)(nx + 2) * (ny + 2) * (D == 3 ? nz + 2 : 1);
+  const dim3 grid((unsigned)std::min<int64_t>((n + 255) / 256, 65536));
+  switch (elem_bytes) {
+    case 2:
+      hipLaunchKernelGGL(pad_replicate_kernel<uint16_t>, grid, dim3(256), 0, st, static_cast<const uint16_t*>(src),
+                         static_cast<uint16_t*>(dst), nx, ny, nz, D);
+      break;
+    case 4:
+      hipLaunchKernelGGL(pad_replicate_kernel<uint32_t>, grid, dim3(256), 0, st, static_cast<const uint32_t*>(src),
+                         static_cast<uint32_t*>(dst), nx, ny, nz, D);
+      break;
+    case 8:
+      hipLaunchKernelGGL(pad_replicate_kernel<uint2>, grid, dim3(256), 0, st, static_cast<const uint2*>(src),
+                         static_cast<uint2*>(dst), nx, ny, nz, D);
+      break;
+    case 16:
+      hipLaunchKernelGGL(pad_replicate_kernel<uint4>, grid, dim3(256), 0, st, static_cast<const uint4*>(src),
+                         static_cast<uint4*>(dst), nx, ny, nz, D);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 // The accumulators an evaluation starts from: joint histogram, valid count,
 // exception-overflow flag (one launch instead of three memsets).
 __global__ void clear_eval_kernel(unsigned long long* __restrict__ hist, int n_hist,

@@ -219,6 +219,25 @@ def test_fast_path_matches_exact_path(oracle_mod, shape, seed):
     assert np.linalg.norm(d0 - d1) <= 1e-5 * np.linalg.norm(d1)
 
 
+@pytest.mark.parametrize("shape,seed", [((48, 44, 40), 330), ((160, 150), 331)])
+def test_plateaus_at_the_range_ends_stay_on_the_fast_path(oracle_mod, shape, seed):
+    """Moving and fixed images clipped at their 10th and 90th percentiles:
+    large plateaus whose interpolated values equal the range ends exactly.
+    The reference counts those samples (a multilinear value of corners in
+    [min, max] stays in [min, max]) and their Parzen terms sit on the clamped
+    edges 2 and B - 2, so the row walks certify them without a range test and
+    the exception lists do not overflow; bins, count, value and derivative
+    equal the oracle's."""
+    fixed, moving, geom = blob_problem(shape, seed)
+    fixed = np.clip(fixed, *np.quantile(fixed, [0.1, 0.9])).astype(np.float32)
+    moving = np.clip(moving, *np.quantile(moving, [0.1, 0.9])).astype(np.float32)
+    grad = oracle_mod.gradient_image(moving, geom)
+    tf = affine_for(geom, affine_params(geom.dim, np.random.default_rng(seed)))
+    m = _gpu_metric(fixed, moving, geom, tf, gradient=grad)
+    _compare(oracle_mod, m, fixed, moving, geom, tf, gradient=grad)
+    assert m.GetLastEvaluationPath() == "fast"
+
+
 def test_full_size_config1_affine(oracle_mod):
     """256^3 float32, 80 blobs, affine: bins, count, value and derivative
     against the multithreaded oracle at full size."""

@@ -129,12 +129,15 @@ def test_mixed_batch_with_a_member_off_the_fused_launch():
 
 
 def test_list_overflow_hands_the_member_back():
-    """A moving image of integer plateaus 0..46 with 50 bins (moving bin size
-    exactly 1): inside a plateau the interpolated value is an integer, its
-    Parzen term sits on a bin edge, every such sample is uncertain and the
-    exception lists overflow; the host redoes those evaluations exactly."""
+    """A moving image of integer plateaus 1..45 with one voxel at 0 and one at
+    46 and 50 bins (moving bin size exactly 1): inside a plateau the
+    interpolated value is an integer, its Parzen term sits on an interior bin
+    edge, every such sample is uncertain and the exception lists overflow; the
+    host redoes those evaluations exactly."""
     fixed, moving, geom = blob_problem((48, 44, 40), 330)
-    q = np.round(np.clip(moving, 0.0, None) / moving.max() * 46.0).astype(np.float32)
+    q = (1.0 + np.round(np.clip(moving, 0.0, None) / moving.max() * 44.0)).astype(np.float32)
+    q.flat[0] = 0.0
+    q.flat[-1] = 46.0
     params = affine_params(3, np.random.default_rng(331), scale=1.0)
     m = _metric(fixed, q, geom, geom, params)
     m.GetValueAndDerivative()
