@@ -312,6 +312,7 @@ struct mmi_context {
   // grow-only work space of the preparation filters (no device frees, which
   // would synchronise the whole device, while other contexts run)
   DevBuf work_a, work_b, work_grad;
+  DevBuf work_c, work_d, work_e, work_f;  // with work_a, work_b: the FP64 volumes of up to 3 filter chains
   bool ftab_valid = false;
   mmi_geometry ftab_key_v{}, ftab_key_f{};
   // separable B-spline derivative: per-sample factors, contraction stages, axis tables
@@ -1133,43 +1134,79 @@ const void* texel_pixels(const mmi_context* ctx) {
 // (itkSmoothingRecursiveGaussianImageFilter.hxx:30-70, 119-131, 199-260):
 // zero-order recursive Gaussian along the last axis, then axes 0 .. D-2, in
 // FP64 (normalise across scale off), cast to FP32 at the end.
-int smooth_image(mmi_context* ctx, const float* in, const mmi_geometry& g, const double* sigma, DevBuf& out) {
-  const int D = g.dim;
-  const int64_t n = num_pixels(g);
-  for (int i = 0; i < D; ++i)
-    if (g.size[i] < 4)
-      return fail(MMI_ERR_ARGUMENT,
-                  "The number of pixels along dimension " + std::to_string(i) +
-                      " is less than 4. This filter requires a minimum of four pixels along the dimension to be "
-                      "processed.");
-  DevBuf& a = ctx->work_a;
-  DevBuf& b = ctx->work_b;
-  MMI_HIP(a.ensure(sizeof(double) * n));
-  MMI_HIP(b.ensure(sizeof(double) * n));
-  MMI_HIP(out.ensure(sizeof(float) * n));
-  // float image in, FP64 between the passes, float image out (the casts are
-  // those of the reference's pipeline, done inside the first and last pass)
-  const void* src = in;
-  double* bufs[2] = {a.as<double>(), b.as<double>()};
+// A chain of D recursive-Gaussian passes over one float image: pass s along
+// axis[s] with coefficients c[s], FP64 between the passes in buf[0] / buf[1],
+// the last pass's values into `last`.
+struct FilterChain {
+  const mmi_geometry* g;
+  const float* in;
+  int axis[3];
+  RecCoeffs c[3];
+  double* buf[2];
+  RecSink last;
+};
+
+// Runs independent chains pass by pass, each pass of every chain in one
+// launch (their lines together fill the device; each line's arithmetic is
+// that of a chain run alone).
+int run_filter_chains(mmi_context* ctx, const FilterChain* ch, int K) {
+  if (K <= 0) return MMI_OK;
+  const int D = ch[0].g->dim;
   for (int step = 0; step < D; ++step) {
-    const int axis = step == 0 ? D - 1 : step - 1;
-    RecCoeffs c;
-    if (!rec_setup(sigma[axis], g.spacing[axis], 0, false, &c))
-      return fail(MMI_ERR_ARGUMENT, "recursive Gaussian: spacing or sigma out of range");
-    int64_t stride = 1;
-    for (int j = 0; j < axis; ++j) stride *= g.size[j];
-    const int64_t ln = g.size[axis];
-    RecSink sink{};
-    sink.mode = kSinkDouble;
-    if (step == D - 1) {
-      sink.mode = kSinkFloat;
-      sink.outf = out.as<float>();
+    RecJobs J;
+    std::memset(&J, 0, sizeof(J));
+    J.n = K;
+    for (int k = 0; k < K; ++k) {
+      const FilterChain& f = ch[k];
+      const mmi_geometry& g = *f.g;
+      const int axis = f.axis[step];
+      int64_t stride = 1;
+      for (int j = 0; j < axis; ++j) stride *= g.size[j];
+      RecJob& r = J.j[k];
+      r.in = step == 0 ? static_cast<const void*>(f.in) : static_cast<const void*>(f.buf[(step - 1) & 1]);
+      r.out = f.buf[step & 1];
+      r.ln = g.size[axis];
+      r.stride = stride;
+      r.nlines = num_pixels(g) / r.ln;
+      r.blocks = (int32_t)((r.nlines + kRecLines - 1) / kRecLines);
+      r.c = f.c[step];
+      r.sink.mode = kSinkDouble;
+      if (step == D - 1) r.sink = f.last;
     }
-    double* dst = bufs[step & 1];
-    MMI_HIP(launch_recursive_gaussian(src, step == 0, dst, ln, stride, n / ln, c, sink, ctx->stream));
-    src = dst;
+    MMI_HIP(launch_recursive_gaussian_jobs(J, step == 0, ctx->stream));
   }
-  MMI_HIP(hipStreamSynchronize(ctx->stream));
+  return MMI_OK;
+}
+
+int check_filter_extent(const mmi_geometry& g, const char* word) {
+  for (int i = 0; i < g.dim; ++i)
+    if (g.size[i] < 4)
+      return fail(MMI_ERR_ARGUMENT, "The number of pixels along " + std::string(word) + " " + std::to_string(i) +
+                                        " is less than 4. This filter requires a minimum of four pixels along the "
+                                        "dimension to be processed.");
+  return MMI_OK;
+}
+
+// SmoothingRecursiveGaussianImageFilter on the device
+// (itkSmoothingRecursiveGaussianImageFilter.hxx:30-70, 119-131, 199-260) as a
+// chain: zero-order recursive Gaussian along the last axis, then axes 0 ..
+// D-2, in FP64 (normalise across scale off), cast to FP32 at the end.
+int smooth_chain(const mmi_geometry& g, const float* in, const double* sigma, double* b0, double* b1, float* out,
+                 FilterChain& f) {
+  if (int rc = check_filter_extent(g, "dimension")) return rc;
+  const int D = g.dim;
+  f.g = &g;
+  f.in = in;
+  f.buf[0] = b0;
+  f.buf[1] = b1;
+  for (int step = 0; step < D; ++step) {
+    f.axis[step] = step == 0 ? D - 1 : step - 1;
+    if (!rec_setup(sigma[f.axis[step]], g.spacing[f.axis[step]], 0, false, &f.c[step]))
+      return fail(MMI_ERR_ARGUMENT, "recursive Gaussian: spacing or sigma out of range");
+  }
+  std::memset(&f.last, 0, sizeof(f.last));
+  f.last.mode = kSinkFloat;
+  f.last.outf = out;
   return MMI_OK;
 }
 
@@ -1367,7 +1404,7 @@ int mmi_context_destroy(mmi_context* ctx) {
                     &ctx->mask_desc[0], &ctx->mask_desc[1],
                     &ctx->partials, &ctx->sample_fbin, &ctx->sample_mbin, &ctx->minmax_out,
                     &ctx->exc_count, &ctx->exc_list, &ctx->exc_res, &ctx->cls, &ctx->fbin8, &ctx->mpad, &ctx->tpad, &ctx->bs_g, &ctx->bs_t1, &ctx->bs_t2, &ctx->bs_rows, &ctx->bs_rows_t1, &ctx->ftab, &ctx->work_a,
-                    &ctx->work_b, &ctx->work_grad};
+                    &ctx->work_b, &ctx->work_grad, &ctx->work_c, &ctx->work_d, &ctx->work_e, &ctx->work_f};
   for (DevBuf* b : bufs) b->release();
   for (int j = 0; j < 3; ++j) {
     ctx->bs_start[j].release();
@@ -1445,55 +1482,41 @@ int mmi_compute_moving_gradient(mmi_context* ctx) {
   const mmi_geometry& g = ctx->moving_g;
   const int D = g.dim;
   const int64_t n = num_pixels(g);
-  for (int i = 0; i < D; ++i)
-    if (g.size[i] < 4)
-      return fail(MMI_ERR_ARGUMENT,
-                  "The number of pixels along direction " + std::to_string(i) +
-                      " is less than 4. This filter requires a minimum of four pixels along the dimension to be "
-                      "processed.");
+  if (int rc = check_filter_extent(g, "direction")) return rc;
   // GradientRecursiveGaussianImageFilter as the metric configures it:
   // sigma = maximum spacing, normalised across scale (I2IM:487-503).
   double sigma = 0.0;
   for (int i = 0; i < D; ++i) sigma = std::max(sigma, g.spacing[i]);
-  DevBuf& a = ctx->work_a;
-  DevBuf& b = ctx->work_b;
+  DevBuf* work[6] = {&ctx->work_a, &ctx->work_b, &ctx->work_c, &ctx->work_d, &ctx->work_e, &ctx->work_f};
+  for (int i = 0; i < 2 * D; ++i) MMI_HIP(work[i]->ensure(sizeof(double) * n));
   DevBuf& grad = ctx->work_grad;
-  MMI_HIP(a.ensure(sizeof(double) * n));
-  MMI_HIP(b.ensure(sizeof(double) * n));
   MMI_HIP(grad.ensure(sizeof(double) * n * D));
-  double* bufs[2] = {a.as<double>(), b.as<double>()};
+  // one chain per component, run together: derivative along dim, then zero
+  // order along the other axes in increasing order (GradRG:214-238); the
+  // float image is widened inside the first pass, FP64 between the passes,
+  // the last pass writes component dim (value / spacing)
+  FilterChain ch[3];
   for (int dim = 0; dim < D; ++dim) {
-    // the float image widened inside the first pass; FP64 between passes
-    const void* src = level_moving(ctx);
-    // derivative along dim, then zero order along the other axes in
-    // increasing order (GradRG:214-238)
-    int order[3] = {dim, -1, -1};
+    FilterChain& f = ch[dim];
+    f.g = &g;
+    f.in = level_moving(ctx);
+    f.buf[0] = work[2 * dim]->as<double>();
+    f.buf[1] = work[2 * dim + 1]->as<double>();
+    f.axis[0] = dim;
     int k = 1;
     for (int o = 0; o < D; ++o)
-      if (o != dim) order[k++] = o;
-    for (int step = 0; step < D; ++step) {
-      const int axis = order[step];
-      RecCoeffs c;
-      if (!rec_setup(sigma, g.spacing[axis], step == 0 ? 1 : 0, true, &c))
+      if (o != dim) f.axis[k++] = o;
+    for (int step = 0; step < D; ++step)
+      if (!rec_setup(sigma, g.spacing[f.axis[step]], step == 0 ? 1 : 0, true, &f.c[step]))
         return fail(MMI_ERR_ARGUMENT, "recursive Gaussian: spacing or sigma out of range");
-      int64_t stride = 1;
-      for (int j = 0; j < axis; ++j) stride *= g.size[j];
-      const int64_t ln = g.size[axis];
-      // the last pass writes gradient component dim (value / spacing)
-      RecSink sink{};
-      sink.mode = kSinkDouble;
-      if (step == D - 1) {
-        sink.mode = kSinkGradient;
-        sink.comp = dim;
-        sink.dim = D;
-        sink.spacing = g.spacing[dim];
-        sink.grad = grad.as<double>();
-      }
-      double* dst = bufs[step & 1];
-      MMI_HIP(launch_recursive_gaussian(src, step == 0, dst, ln, stride, n / ln, c, sink, ctx->stream));
-      src = dst;
-    }
+    std::memset(&f.last, 0, sizeof(f.last));
+    f.last.mode = kSinkGradient;
+    f.last.comp = dim;
+    f.last.dim = D;
+    f.last.spacing = g.spacing[dim];
+    f.last.grad = grad.as<double>();
   }
+  if (int rc = run_filter_chains(ctx, ch, D)) return rc;
   Direction dir{};
   for (int i = 0; i < D * D; ++i) dir.m[i] = g.direction[i];
   MMI_HIP(ctx->texels.ensure(sizeof(float4) * n));
@@ -1671,15 +1694,33 @@ int mmi_initialize(mmi_context* ctx) {
     return fail(MMI_ERR_ARGUMENT, "smoothing takes float images only (other pixel types have no device copy of "
                                   "their smoothed image)");
   ctx->fixed_smoothed = ctx->moving_smoothed = false;
-  if (ctx->fixed_sigma[0] > 0.0) {
-    if (int rc = smooth_image(ctx, ctx->fixed_px.as<float>(), ctx->fixed_g, ctx->fixed_sigma, ctx->fixed_work))
-      return rc;
-    ctx->fixed_smoothed = true;
-  }
-  if (ctx->moving_sigma[0] > 0.0) {
-    if (int rc = smooth_image(ctx, ctx->moving_px.as<float>(), ctx->moving_g, ctx->moving_sigma, ctx->moving_work))
-      return rc;
-    ctx->moving_smoothed = true;
+  {
+    // the fixed and moving smoothing chains together
+    FilterChain ch[2];
+    int K = 0;
+    if (ctx->fixed_sigma[0] > 0.0) {
+      const int64_t nf = num_pixels(ctx->fixed_g);
+      MMI_HIP(ctx->work_a.ensure(sizeof(double) * nf));
+      MMI_HIP(ctx->work_b.ensure(sizeof(double) * nf));
+      MMI_HIP(ctx->fixed_work.ensure(sizeof(float) * nf));
+      if (int rc = smooth_chain(ctx->fixed_g, ctx->fixed_px.as<float>(), ctx->fixed_sigma, ctx->work_a.as<double>(),
+                                ctx->work_b.as<double>(), ctx->fixed_work.as<float>(), ch[K++]))
+        return rc;
+    }
+    if (ctx->moving_sigma[0] > 0.0) {
+      const int64_t nm = num_pixels(ctx->moving_g);
+      MMI_HIP(ctx->work_c.ensure(sizeof(double) * nm));
+      MMI_HIP(ctx->work_d.ensure(sizeof(double) * nm));
+      MMI_HIP(ctx->moving_work.ensure(sizeof(float) * nm));
+      if (int rc = smooth_chain(ctx->moving_g, ctx->moving_px.as<float>(), ctx->moving_sigma,
+                                ctx->work_c.as<double>(), ctx->work_d.as<double>(), ctx->moving_work.as<float>(),
+                                ch[K++]))
+        return rc;
+    }
+    if (int rc = run_filter_chains(ctx, ch, K)) return rc;
+    MMI_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->fixed_smoothed = ctx->fixed_sigma[0] > 0.0;
+    ctx->moving_smoothed = ctx->moving_sigma[0] > 0.0;
   }
   bool stale = !ctx->have_texels;
   if (ctx->have_texels && !ctx->user_texels)

@@ -352,6 +352,26 @@ struct RecSink {
   float* outf;
 };
 
+// One line set of a batched recursive-Gaussian launch: the lines of one pass
+// of one filter chain (a volume, an axis, the pass's coefficients and sink).
+struct RecJob {
+  const void* in;  // float volume (the first pass of a chain) or double volume
+  double* out;
+  int64_t ln, stride, nlines;
+  int32_t blocks;  // workgroups of this job (kRecLines lines each)
+  int32_t pad_;
+  RecCoeffs c;
+  RecSink sink;
+};
+constexpr int kRecJobsMax = 4;
+constexpr int kRecLines = 128;
+// The jobs of one launch (the same pass of independent chains: all take float
+// input, or all double).
+struct RecJobs {
+  RecJob j[kRecJobsMax];
+  int32_t n;
+};
+
 struct Direction {
   double m[9];
 };
@@ -401,6 +421,8 @@ hipError_t launch_to_storage(const float* in, void* out, int storage, int64_t n,
 hipError_t launch_pad_replicate(const void* src, void* dst, int elem_bytes, int nx, int ny, int nz, int D,
                                 hipStream_t st);
 hipError_t launch_texels_to_storage(const float4* in, void* out, int storage, int64_t n, hipStream_t st);
+// Independent recursive-Gaussian passes in one launch (one job per chain).
+hipError_t launch_recursive_gaussian_jobs(const RecJobs& jobs, bool in_float, hipStream_t st);
 hipError_t launch_recursive_gaussian(const void* in, bool in_float, double* out, int64_t ln, int64_t stride,
                                      int64_t nlines, const RecCoeffs& c, const RecSink& sink, hipStream_t st);
 hipError_t launch_clear_eval(unsigned long long* hist, int n_hist, unsigned long long* nvalid, uint32_t* overflow,

@@ -71,9 +71,9 @@ __global__ void minmax_kernel(const typename P::pixel* __restrict__ px, int64_t 
 // value / spacing into one component of D-interleaved gradient vectors, or
 // a float volume), so the chain needs no separate conversion kernels.
 template <typename TIn>
-__global__ void recursive_gaussian_kernel(const TIn* __restrict__ in, double* __restrict__ out, int64_t ln,
-                                          int64_t stride, int64_t nlines, RecCoeffs c, RecSink sink) {
-  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void recursive_gaussian_line(const TIn* __restrict__ in, double* __restrict__ out,
+                                                        int64_t ln, int64_t stride, int64_t nlines,
+                                                        const RecCoeffs& c, const RecSink& sink, int64_t l) {
   if (l >= nlines) return;
   const int64_t lo = l % stride, hi = l / stride;
   const int64_t base = lo + hi * stride * ln;
@@ -186,6 +186,39 @@ __global__ void recursive_gaussian_kernel(const TIn* __restrict__ in, double* __
       xp1 = xi;
     }
   }
+}
+
+template <typename TIn>
+__global__ void recursive_gaussian_kernel(const TIn* __restrict__ in, double* __restrict__ out, int64_t ln,
+                                          int64_t stride, int64_t nlines, RecCoeffs c, RecSink sink) {
+  recursive_gaussian_line<TIn>(in, out, ln, stride, nlines, c, sink, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The same pass of up to kRecJobsMax independent chains in one launch: the
+// workgroups are the jobs' in order, so a pass with few lines per chain still
+// fills the device.
+template <typename TIn>
+__global__ void __launch_bounds__(kRecLines) recursive_gaussian_jobs_kernel(RecJobs J) {
+  uint32_t b = blockIdx.x;
+  int k = 0;
+  while (k + 1 < J.n && b >= (uint32_t)J.j[k].blocks) {
+    b -= (uint32_t)J.j[k].blocks;
+    ++k;
+  }
+  const int64_t l = (int64_t)b * kRecLines + threadIdx.x;
+#define MMI_REC_JOB(K)                                                                                          \
+  recursive_gaussian_line<TIn>(static_cast<const TIn*>(J.j[K].in), J.j[K].out, J.j[K].ln, J.j[K].stride,        \
+                               J.j[K].nlines, J.j[K].c, J.j[K].sink, l)
+  // a workgroup-uniform branch per job keeps the arguments in scalar registers
+  if (k == 0)
+    MMI_REC_JOB(0);
+  else if (k == 1)
+    MMI_REC_JOB(1);
+  else if (k == 2)
+    MMI_REC_JOB(2);
+  else
+    MMI_REC_JOB(3);
+#undef MMI_REC_JOB
 }
 
 // TransformLocalVectorToPhysicalVector (itkImageBase.h:636-653) + packing into
@@ -584,6 +617,20 @@ hipError_t launch_recursive_gaussian(const void* in, bool in_float, double* out,
   else
     hipLaunchKernelGGL(recursive_gaussian_kernel<double>, grid, dim3(128), 0, st, static_cast<const double*>(in),
                        out, ln, stride, nlines, c, sink);
+  return hipGetLastError();
+}
+
+hipError_t launch_recursive_gaussian_jobs(const RecJobs& jobs, bool in_float, hipStream_t st) {
+  if (jobs.n < 1 || jobs.n > kRecJobsMax) return hipErrorInvalidValue;
+  unsigned blocks = 0;
+  for (int k = 0; k < jobs.n; ++k) {
+    if (jobs.j[k].blocks != (int32_t)((jobs.j[k].nlines + kRecLines - 1) / kRecLines)) return hipErrorInvalidValue;
+    blocks += (unsigned)jobs.j[k].blocks;
+  }
+  if (in_float)
+    hipLaunchKernelGGL(recursive_gaussian_jobs_kernel<float>, dim3(blocks), dim3(kRecLines), 0, st, jobs);
+  else
+    hipLaunchKernelGGL(recursive_gaussian_jobs_kernel<double>, dim3(blocks), dim3(kRecLines), 0, st, jobs);
   return hipGetLastError();
 }
 
