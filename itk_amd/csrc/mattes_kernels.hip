@@ -118,26 +118,29 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& f) {
   double* fm = f.lds ? lds_d + BB : f.fm;
   double* mm = f.lds ? lds_d + BB + B : f.mm;
   double* lrow = f.lds ? lds_d + BB + 2 * B : f.rowsum;
-  // Joint PDF sum: exact in 64-bit integers.
+  // pass 1's histogram, read once into the PDF's place (as integers) and left
+  // zeroed for the next evaluation, with its exact 64-bit total
+  unsigned long long* stage = reinterpret_cast<unsigned long long*>(pdf);
   unsigned long long part = 0;
-  for (int i = t; i < BB; i += blockDim.x) part += f.hist[i];
+  for (int i = t; i < BB; i += blockDim.x) {
+    const unsigned long long h = f.hist[i];
+    f.hist[i] = 0ull;
+    stage[i] = h;
+    part += h;
+  }
   part = wave_sum(part);
   if (lane == 0) redu[wid] = part;
+  if (t == 0) {
+    s_nvalid = *f.nvalid;
+    *f.nvalid = 0ull;
+    f.result->exc_overflow = *f.overflow ? 1 : 0;
+    *f.overflow = 0u;
+  }
   __syncthreads();
-  // pass 1's accumulators are left zeroed for the next evaluation once read
-  auto release = [&]() {
-    for (int i = t; i < BB; i += blockDim.x) f.hist[i] = 0ull;
-    if (t == 0) {
-      *f.nvalid = 0ull;
-      *f.overflow = 0u;
-    }
-  };
   if (t == 0) {
     unsigned long long total = 0;
     for (int w = 0; w < nw; ++w) total += redu[w];
-    const unsigned long long nvalid = *f.nvalid;
-    s_nvalid = nvalid;
-    f.result->exc_overflow = *f.overflow ? 1 : 0;
+    const unsigned long long nvalid = s_nvalid;
     const double jointPDFSum = (double)total / kFixedPointScale;
     f.result->n_valid = nvalid;
     f.result->joint_pdf_sum = jointPDFSum;
@@ -146,34 +149,45 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& f) {
   }
   __syncthreads();
   const double nf = s_norm;
-  if (nf == 0.0) {
-    release();
-    return;
-  }
+  if (nf == 0.0) return;
   const double closeToZero = 2.220446049250313e-16;
-  for (int i = t; i < BB; i += blockDim.x) {
-    const double v = ((double)f.hist[i] / kFixedPointScale) * nf;
-    pdf[i] = v;
-    if (f.lds) f.pdf[i] = v;
-  }
   // fixed marginal (the reference counts each valid sample once in its fixed
   // bin): the Parzen weights of a counted sample sum to 2^32 within a few
   // units, so a row's total rounded to a multiple of 2^32 is the row's count
-  // exactly (below 2^30 samples)
-  for (int i = t; i < B; i += blockDim.x) {
+  // exactly (below 2^30 samples).  Sixteen lanes per row, an exact integer
+  // sum.
+  for (int r = t >> 4; r < B; r += blockDim.x >> 4) {  // r is uniform over each group of sixteen lanes
     unsigned long long rs = 0ull;
-    for (int m = 0; m < B; ++m) rs += f.hist[i * B + m];
-    const double fmv = (double)((rs + (1ull << 31)) >> 32) * nf;
-    fm[i] = fmv;
-    if (f.lds) f.fm[i] = fmv;
-    lrow[i] = fmv > closeToZero ? log(fmv) : 0.0;  // log of the fixed marginal, once per row
+    for (int m = t & 15; m < B; m += 16) rs += stage[r * B + m];
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) rs += __shfl_xor(rs, o, 16);
+    if ((t & 15) == 0) {
+      const double fmv = (double)((rs + (1ull << 31)) >> 32) * nf;
+      fm[r] = fmv;
+      if (f.lds) f.fm[r] = fmv;
+      lrow[r] = fmv > closeToZero ? log(fmv) : 0.0;  // log of the fixed marginal, once per row
+    }
   }
   __syncthreads();
-  release();
-  // moving marginal: column sums in the reference's row order (Mattes:405-417)
+  for (int i = t; i < BB; i += blockDim.x) {
+    const double v = ((double)stage[i] / kFixedPointScale) * nf;
+    pdf[i] = v;
+    if (f.lds) f.pdf[i] = v;
+  }
+  __syncthreads();
+  // moving marginal: column sums in the reference's row order (Mattes:405-417),
+  // the loads of a few rows issued ahead of their sums
   for (int m = t; m < B; m += blockDim.x) {
     double acc = 0.0;
-    for (int r = 0; r < B; ++r) acc += pdf[r * B + m];
+    int r = 0;
+    for (; r + 8 <= B; r += 8) {
+      double v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = pdf[(r + k) * B + m];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc += v[k];
+    }
+    for (; r < B; ++r) acc += pdf[r * B + m];
     mm[m] = acc;
     if (f.lds) f.mm[m] = acc;
   }
