@@ -332,6 +332,11 @@ int mmi_batch_destroy(mmi_batch* batch);
 int mmi_batch_set_graphs(mmi_batch* batch, int32_t enabled);
 int mmi_batch_evaluate(mmi_batch* batch, mmi_context* const* ctxs, int32_t n, int32_t want_derivative,
                        double* values, double* const* derivatives, int64_t* n_valid, int32_t* status);
+/* The message of the context's last failed evaluation ("" when it succeeded;
+ * never NULL), also when a batch evaluated it on another thread: the text of
+ * the itk::ExceptionObject the reference's GetValueAndDerivative would throw
+ * (no separate counterpart in the reference, whose exceptions carry it). */
+const char* mmi_context_last_error(const mmi_context* ctx);
 /* Device time of the last fused launch and how many contexts it held. */
 int mmi_batch_get_last_timing(mmi_batch* batch, float* device_ms, int32_t* n_fused);
 

@@ -37,6 +37,7 @@ _gp = ctypes.POINTER(CGeometry)
 # name -> (restype, argtypes); the exported C ABI, declaration for declaration.
 SIGNATURES = {
     "mmi_last_error": (ctypes.c_char_p, []),
+    "mmi_context_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "mmi_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(_vp)]),
     "mmi_context_destroy": (ctypes.c_int, [_vp]),
     "mmi_set_fixed_image": (ctypes.c_int, [_vp, _gp, ctypes.c_int32, _vp]),

@@ -64,7 +64,9 @@ class BatchEvaluator:
             err = None
             m._n_valid = int(nvalid[i])
             if status[i] != N.MMI_OK:
-                err = N.MetricError(int(status[i]), "evaluation failed with status %d" % status[i])
+                # the message this metric's own evaluation would raise
+                msg = self._L.mmi_context_last_error(m._ctx).decode()
+                err = N.MetricError(int(status[i]), msg or "evaluation failed with status %d" % status[i])
             else:
                 m._value = float(values[i])
             out.append((float(values[i]), derivs[i], int(nvalid[i]), err))

@@ -335,6 +335,7 @@ struct mmi_context {
   int max_lds_per_block = 64 * 1024;         // the device's LDS bytes per workgroup
   int grid = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::string last_error;  // the last failed evaluation's message ("" after a success)
   // pinned host staging: B-spline coefficients up, result and derivative down
   void* h_coeffs = nullptr;
   size_t h_coeffs_bytes = 0;
@@ -2276,8 +2277,12 @@ extern "C" {
 
 int mmi_evaluate(mmi_context* ctx, int32_t want_derivative, double* value, double* derivative, int64_t* n_valid,
                  double* joint_pdf, double* fixed_marginal) {
-  return evaluate_impl(ctx, want_derivative, value, derivative, n_valid, joint_pdf, fixed_marginal, nullptr);
+  const int rc = evaluate_impl(ctx, want_derivative, value, derivative, n_valid, joint_pdf, fixed_marginal, nullptr);
+  if (ctx) ctx->last_error = rc == MMI_OK ? std::string() : g_last_error;
+  return rc;
 }
+
+const char* mmi_context_last_error(const mmi_context* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
 
 int mmi_evaluate_static(mmi_context* ctx, int32_t want_derivative, double* value, double* derivative,
                         int64_t* n_valid) {
@@ -2664,9 +2669,11 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
       n_valid[i] = (int64_t)o[1];
       if (st != 0) {
         status[i] = fail(MMI_ERR_DEGENERATE, degenerate_message(st));
+        c->last_error = degenerate_message(st);
         continue;
       }
       status[i] = MMI_OK;
+      c->last_error.clear();
       values[i] = o[0];
       if (want_derivative && derivatives[i])
         std::memcpy(derivatives[i], o + 3, sizeof(double) * c->mtf.params.size());
@@ -2877,7 +2884,10 @@ int regular_step_device(mmi_batch* b, mmi_context* const* ctxs, int32_t n, const
     // the context takes the parameters the device reached (its host set-up
     // computes the map the device computed)
     if (int rc = mmi_set_moving_parameters(ctxs[i], it.params, it.P)) return rc;
-    if (it.dstatus) (void)fail(MMI_ERR_DEGENERATE, degenerate_message(it.dstatus));
+    if (it.dstatus) {
+      (void)fail(MMI_ERR_DEGENERATE, degenerate_message(it.dstatus));
+      ctxs[i]->last_error = degenerate_message(it.dstatus);
+    }
     budget[i] = it.handback != kStepOnDevice ? max_steps - it.steps : 0;
   }
   return MMI_OK;

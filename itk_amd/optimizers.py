@@ -204,7 +204,9 @@ class RegularStepGradientDescentOptimizerv4(_OptimizerBase):
         if st.stop == N.MMI_STOP_METRIC_ERROR:
             self.stop_condition = StopCondition.COSTFUNCTION_ERROR
             self.stop_description += "Metric error during optimization"
-            raise N.MetricError(int(st.status), "metric evaluation failed during optimization (status %d)"
+            # the message the metric's own evaluation gave (its exception text)
+            msg = N.load().mmi_context_last_error(self.metric._ctx).decode()
+            raise N.MetricError(int(st.status), msg or "metric evaluation failed during optimization (status %d)"
                                 % st.status)
         if st.stop == N.MMI_STOP_MAXIMUM_NUMBER_OF_ITERATIONS:
             self.stop_condition = StopCondition.MAXIMUM_NUMBER_OF_ITERATIONS

@@ -46,9 +46,14 @@ class RegistrationRequest:
     learning_rate: float = 2.0
 
     def device_bytes(self) -> int:
-        """Resident estimate: FP32 uploads and smoothed levels (16 B), gradient
-        texels (16 B), the gradient filter's FP64 temporaries (40 B) per voxel."""
-        return int(self.geometry.num_pixels) * 72 + (64 << 20)
+        """Resident estimate of the registration's context: per voxel the FP32
+        uploads and smoothed levels (16 B), the gradient texels (16 B), the
+        filter chains' FP64 volumes (6 x 8 B) and FP64 gradient (24 B), the
+        per-sample bytes (fixed bins, classes: 2 B), and for an affine
+        registration the row walks' bordered texels and values (20 B), for a
+        B-spline one its per-sample derivative factors (12 B); plus 64 MB."""
+        per = 16 + 16 + 48 + 24 + 2 + (20 if self.kind == "affine" else 12)
+        return int(self.geometry.num_pixels) * per + (64 << 20)
 
 
 def build_registration(req: RegistrationRequest, metric, native_loop: bool = True):
@@ -253,7 +258,10 @@ class RegistrationServer:
         self.max_batch = 0
         self.peak_inflight = 0
         self.stopping = False
-        self.pool = []  # idle metrics: contexts whose device buffers only grow
+        # idle metrics: contexts whose device buffers only grow; their resident
+        # bytes (the largest registration each has held) stay in `used`
+        self.pool = []
+        self.peak_used = 0
 
     # -- called from worker threads ----------------------------------------------
     def _evaluate(self, job, want_derivative: bool):
@@ -296,11 +304,33 @@ class RegistrationServer:
                 job.lane.running.discard(job)
                 job.lane.cv.notify()
             self.running -= 1
-            self.used -= job.req.device_bytes()
             self.main_cv.notify()
 
     def _now(self) -> float:
         return time.perf_counter() - self.t0
+
+    def _pooled_metric(self, need: int):
+        """The idle metric a registration of `need` resident bytes takes (the
+        smallest that holds it, else the largest, which grows; None: a new
+        one) and the bytes `used` grows by.  Other idle metrics are released,
+        largest first, while the registration would not fit the budget."""
+        fits = [m for m in self.pool if m._resident_bytes >= need]
+        if fits:
+            metric = min(fits, key=lambda m: m._resident_bytes)
+        elif self.pool:
+            metric = max(self.pool, key=lambda m: m._resident_bytes)
+        else:
+            metric = None
+        extra = need if metric is None else max(0, need - metric._resident_bytes)
+        while self.used + extra > self.budget:
+            others = [m for m in self.pool if m is not metric]
+            if not others:
+                break
+            victim = max(others, key=lambda m: m._resident_bytes)
+            self.pool.remove(victim)
+            self.used -= victim._resident_bytes
+            victim.close()
+        return metric, extra
 
     def run(self, requests):
         """Serve the requests (arrival times relative to now); returns the results."""
@@ -318,21 +348,27 @@ class RegistrationServer:
                 now = self._now()
                 # admission: arrived, within budget and slots, in arrival order;
                 # each registration joins the lane with fewer running ones
-                while queue and queue[0].arrival <= now and self.running < self.max_inflight and (
-                        self.used + queue[0].device_bytes() <= self.budget or self.running == 0):
-                    req = queue.pop(0)
+                while queue and queue[0].arrival <= now and self.running < self.max_inflight:
+                    req = queue[0]
+                    need = req.device_bytes()
+                    metric, extra = self._pooled_metric(need)
+                    if self.used + extra > self.budget and self.running > 0:
+                        break
+                    queue.pop(0)
                     # affine levels share the lanes' fused RegularStep loops;
                     # a B-spline registration (L-BFGS-B steps on the host between
                     # evaluations) runs on its own stream, so no lane waits for it
                     lane = min(lanes, key=lambda ln: len(ln.running)) if req.kind == "affine" else None
-                    if self.pool:
-                        metric = self.pool.pop()
+                    if metric is not None:
+                        self.pool.remove(metric)
                     else:
                         metric = MattesMutualInformationImageToImageMetricv4(device=self.device)
                         metric.SetImageStorage(self.storage)
+                    metric._resident_bytes = max(getattr(metric, "_resident_bytes", 0), need)
                     job = _Job(self, lane, req, metric)
                     job.admitted = now
-                    self.used += req.device_bytes()
+                    self.used += extra
+                    self.peak_used = max(self.peak_used, self.used)
                     self.running += 1
                     if lane is not None:
                         lane.running.add(job)
@@ -350,6 +386,7 @@ class RegistrationServer:
         for m in self.pool:
             m.close()
         self.pool = []
+        self.used = 0
         return sorted(self.results, key=lambda r: r.rid)
 
 

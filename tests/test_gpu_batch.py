@@ -78,7 +78,8 @@ def test_mixed_batch_and_degenerate_member():
     p = far.get_parameters()
     p[-3:] += 1e4
     far.set_parameters(p)
-    members.append(_metric(fixed, moving, geom, far))          # every sample outside: degenerate
+    members.append(_metric(fixed, moving, geom, far))          # every sample outside: degenerate (fused)
+    members.append(_metric(fixed, moving, geom, far, cd=True))  # the same off the fused launch (worker thread)
     be = BatchEvaluator()
     got = be.evaluate(members)
     assert be.last_timing()["fused"] == 3
@@ -87,9 +88,14 @@ def test_mixed_batch_and_degenerate_member():
         v1, d1, nv1, _ = _alone(m)
         assert (v, nv) == (v1, nv1)
         assert np.array_equal(d, d1)
-    assert got[4][3] is not None and got[4][3].code == MMI_ERR_DEGENERATE
-    with pytest.raises(MetricError):
-        members[4].GetValueAndDerivative()
+    for k in (4, 5):
+        err = got[k][3]
+        assert err is not None and err.code == MMI_ERR_DEGENERATE
+        # the batch reports the reference's exception text, as the member's own evaluation raises it
+        with pytest.raises(MetricError) as alone:
+            members[k].GetValueAndDerivative()
+        assert str(err) == str(alone.value)
+        assert "All samples map outside moving image buffer" in str(err)
     be.close()
 
 

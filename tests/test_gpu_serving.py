@@ -52,6 +52,8 @@ def test_admission_respects_hbm_budget():
     server = RegistrationServer(hbm_budget_bytes=budget, max_inflight=8)  # the budget, not the cap, limits
     out = server.run(reqs)
     assert server.peak_inflight == 1
+    # pooled contexts keep their buffers: the resident total never exceeds the budget
+    assert server.peak_used <= budget
     alone, _ = run_sequential(reqs)
     for a, b in zip(alone, out):
         assert np.array_equal(a.parameters, b.parameters)
