@@ -94,19 +94,27 @@ __device__ __forceinline__ void fast_cindex(const FastArgs& f, const int32_t* id
 #pragma unroll
     for (int k = 0; k < D; ++k) vp[k] = f.vorg[k] + f.vsp[k] * (double)idx[k];
     const int xl = idx[0] - f.vstart[0], yl = idx[1] - f.vstart[1], zl = D == 3 ? idx[2] - f.vstart[2] : 0;
-    const bool in = f.bs_inside[0][xl] && f.bs_inside[1][yl] && (D < 3 || f.bs_inside[2][zl]);
+    // The three flags, the weights and the coefficient line are read without
+    // waiting for one another: an index outside the transform's region has
+    // support start 0 and zero weights, and every virtual row has its line,
+    // so the reads are in bounds for every sample; the flags then select.
+    const uint32_t in_x = f.bs_inside[0][xl], in_y = f.bs_inside[1][yl];
+    const uint32_t in_z = D == 3 ? f.bs_inside[2][zl] : 1u;
+    const bool in = (in_x & in_y & in_z) != 0u;
     *bsin = in;
-    if (in) {
-      const double* w = f.bs_w[0] + 4 * xl;
-      const double* row = f.bs_rows + ((int64_t)(zl * f.vsize1 + yl) * D) * f.bs_gx + f.bs_start[0][xl];
+    const double* w = f.bs_w[0] + 4 * xl;
+    const double* row = f.bs_rows + ((int64_t)(zl * f.vsize1 + yl) * D) * f.bs_gx + f.bs_start[0][xl];
+    double wv[4], rv[3][4];
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        const double* r = row + d * f.bs_gx;
-        mp[d] = (w[0] * r[0] + w[1] * r[1] + w[2] * r[2] + w[3] * r[3]) + vp[d];
-      }
-    } else {
+    for (int k = 0; k < 4; ++k) wv[k] = w[k];
 #pragma unroll
-      for (int d = 0; d < D; ++d) mp[d] = vp[d];
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) rv[d][k] = row[d * f.bs_gx + k];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double disp = wv[0] * rv[d][0] + wv[1] * rv[d][1] + wv[2] * rv[d][2] + wv[3] * rv[d][3];
+      mp[d] = in ? disp + vp[d] : vp[d];
     }
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -1287,19 +1295,25 @@ __device__ __forceinline__ float parzen_deriv_weight_of_t(float t, const float* 
 
 // Pass 2 of a certainly inside sample from its loads: when pass 1 counted it
 // as certain, its Parzen derivative weight times the interpolated gradient
-// into the walk's FP32 sums (pass2_rows_body).
+// into the walk's FP32 sums (pass2_rows_body).  The fixed bin's byte is only
+// tested once the value is interpolated, and a sample that does not count
+// adds an exact zero: the byte and the texels are then in flight together
+// (a test before the value would let the texel loads wait for the byte).
+// Every sample here is certainly inside the moving buffer, so its texel loads
+// are in bounds whatever its byte says.
 template <int D, typename P>
 __device__ __forceinline__ void pass2_pad_compute(const FastArgs& f, const float* pr, const PadLoad<D, P, true>& x,
                                                   float yf, float* sg, float* sy_) {
-  if (x.fb == kClsSkip) return;
   float3 g;
   const float v = pad_value<D, P, true>(x, &g);
   bool unc = false;
   float t;
   const int mbin = fb_bin(f, v, &unc, &t);
-  if (unc) return;
-  const float wt = parzen_deriv_weight_of_t(t, pr + __umul24(x.fb, (uint32_t)f.bins) + (mbin - 1));
-  const float gw[3] = {wt * g.x, wt * g.y, wt * g.z};
+  const bool counted = x.fb != kClsSkip && !unc;
+  // a row of the table for every sample (mbin is within [2, B - 3])
+  const uint32_t row = counted ? x.fb : 0u;
+  const float wt = parzen_deriv_weight_of_t(t, pr + __umul24(row, (uint32_t)f.bins) + (mbin - 1));
+  const float gw[3] = {counted ? wt * g.x : 0.0f, counted ? wt * g.y : 0.0f, counted ? wt * g.z : 0.0f};
 #pragma unroll
   for (int b = 0; b < D; ++b) {
     sg[b] += gw[b];
@@ -1416,8 +1430,11 @@ __device__ __forceinline__ void pass2_rows_cd_body(const FastArgs& f, const Eval
     float yf = (float)w.y;
     for (int j = 0; j < w.rows; ++j, rows_step<D, P>(f, w), yf += 1.0f) {
       if (!w.active) continue;
+      // Every load of the sample is issued before any test of loaded data
+      // (pass 1's byte, the weight): the cells are clamped into the buffer,
+      // so the loads are in bounds for every sample, and a sample that does
+      // not count is dropped at the end.
       const uint32_t cb = f.cls[w.s];
-      if (cb >= kClsListed) continue;
       // the sample's cell and weights, as rows_classify
       int32_t I[3] = {0, 0, 0}, fl[3] = {0, 0, 0};
       uint32_t off[3] = {0u, 0u, 0u}, stride[3] = {1u, (uint32_t)f.msy, (uint32_t)f.msz};
@@ -1439,6 +1456,34 @@ __device__ __forceinline__ void pass2_rows_cd_body(const FastArgs& f, const Eval
 #pragma unroll
       for (int i = 0; i < (1 << D); ++i)
         c[i] = P::ld32(img, cbase + ((i & 1) ? off[0] : 0u) + ((i & 2) ? off[1] : 0u) + ((i & 4) ? off[2] : 0u));
+      // the third position along each axis b: P3 = clamp(I + 1) + act(I + 1)
+      // when t >= 1/2, P0 = clamp(I - 1) else; its slab over the other axes
+      // (e[b][0] at the cell's lower corners, then + off of the first other
+      // axis, + off of the second, + both)
+      bool upper[3] = {false, false, false};
+      float e[3][4];
+#pragma unroll
+      for (int b = 0; b < D; ++b) {
+        const long long q = w.q[b];
+        upper[b] = (uint32_t)q >= 0x80000000u;
+        const int32_t Ie = upper[b] ? I[b] + 1 : I[b] - 1;
+        const int32_t fe = min(max(Ie, f.mstart[b]), f.mend[b]);
+        const int32_t pe = upper[b] ? fe + ((uint32_t)(Ie - f.mstart[b]) < (uint32_t)(f.mend[b] - f.mstart[b]) ? 1 : 0)
+                                    : fe;
+        const uint32_t ebase = (uint32_t)((int32_t)cbase + (pe - fl[b]) * (int32_t)stride[b]);
+        if (D == 2) {
+          const int o = 1 - b;
+          e[b][0] = P::ld32(img, ebase);
+          e[b][1] = P::ld32(img, ebase + off[o]);
+          e[b][2] = e[b][3] = 0.0f;
+        } else {
+          const int o1 = b == 0 ? 1 : 0, o2 = b == 2 ? 1 : 2;  // the other axes
+          e[b][0] = P::ld32(img, ebase);
+          e[b][1] = P::ld32(img, ebase + off[o1]);
+          e[b][2] = P::ld32(img, ebase + off[o2]);
+          e[b][3] = P::ld32(img, ebase + off[o1] + off[o2]);
+        }
+      }
       // the value with fast_interp's operations (pass 1's bits)
       float v;
       {
@@ -1455,35 +1500,29 @@ __device__ __forceinline__ void pass2_rows_cd_body(const FastArgs& f, const Eval
       const float mterm = fmaf(v, f.inv_mbs, -f.mnm);
       const int mbin = (int)fminf(fmaxf(mterm, 2.0f), (float)(B - 3));
       const float arg0 = (float)(mbin - 1) - mterm;
-      const float wt = parzen_deriv_weight_fast(arg0, pr + (int)cb * B + (mbin - 1));
-      if (wt == 0.0f) continue;
+      // listed and not counted samples read row 0 of the table and are dropped
+      const bool listed = cb >= kClsListed;
+      const float wt = parzen_deriv_weight_fast(arg0, pr + (int)(listed ? 0u : cb) * B + (mbin - 1));
+      const bool counted = !listed && wt != 0.0f;
       float g[3] = {0.0f, 0.0f, 0.0f};
-      bool inb[3] = {false, false, false}, unc[3] = {false, false, false};
+      bool inb[3] = {false, false, false};
       bool any_unc = false;
 #pragma unroll
       for (int b = 0; b < D; ++b) {
         const long long q = w.q[b];
         inb[b] = (unsigned long long)(q - R.cd_lo_in[b]) < R.cd_span_in[b];
-        unc[b] = ((unsigned long long)(q - R.cd_lo_out[b]) < R.cd_span_out[b]) & !inb[b];
-        any_unc = any_unc | unc[b];
-        // the third position along b: P3 = clamp(I + 1) + act(I + 1) when t >= 1/2, P0 = clamp(I - 1) else
-        const bool upper = (uint32_t)q >= 0x80000000u;
-        const int32_t Ie = upper ? I[b] + 1 : I[b] - 1;
-        const int32_t fe = min(max(Ie, f.mstart[b]), f.mend[b]);
-        const int32_t pe = upper ? fe + ((uint32_t)(Ie - f.mstart[b]) < (uint32_t)(f.mend[b] - f.mstart[b]) ? 1 : 0)
-                                 : fe;
-        const uint32_t ebase = (uint32_t)((int32_t)cbase + (pe - fl[b]) * (int32_t)stride[b]);
+        const bool unc = ((unsigned long long)(q - R.cd_lo_out[b]) < R.cd_span_out[b]) & !inb[b];
+        any_unc = any_unc | unc;
         // slabs over the other axes at the low, high and third positions
-        const int o1 = b == 0 ? 1 : 0, o2 = b == 2 ? 1 : 2;  // the other axes (D == 3)
         float slo, shi, se;
         if (D == 2) {
           const int o = 1 - b;
           const int bit_b = 1 << b, bit_o = 1 << o;
           slo = fmaf(c[bit_o] - c[0], t[o], c[0]);
           shi = fmaf(c[bit_b | bit_o] - c[bit_b], t[o], c[bit_b]);
-          const float e0 = P::ld32(img, ebase), e1 = P::ld32(img, ebase + off[o]);
-          se = fmaf(e1 - e0, t[o], e0);
+          se = fmaf(e[b][1] - e[b][0], t[o], e[b][0]);
         } else {
+          const int o1 = b == 0 ? 1 : 0, o2 = b == 2 ? 1 : 2;  // the other axes
           const int b1 = 1 << o1, b2 = 1 << o2, bb = 1 << b;
           const float l0 = fmaf(c[b1] - c[0], t[o1], c[0]);
           const float l1 = fmaf(c[b1 | b2] - c[b2], t[o1], c[b2]);
@@ -1491,16 +1530,15 @@ __device__ __forceinline__ void pass2_rows_cd_body(const FastArgs& f, const Eval
           const float h0 = fmaf(c[bb | b1] - c[bb], t[o1], c[bb]);
           const float h1 = fmaf(c[bb | b1 | b2] - c[bb | b2], t[o1], c[bb | b2]);
           shi = fmaf(h1 - h0, t[o2], h0);
-          const float e00 = P::ld32(img, ebase), e10 = P::ld32(img, ebase + off[o1]);
-          const float e01 = P::ld32(img, ebase + off[o2]), e11 = P::ld32(img, ebase + off[o1] + off[o2]);
-          const float m0 = fmaf(e10 - e00, t[o1], e00);
-          const float m1 = fmaf(e11 - e01, t[o1], e01);
+          const float m0 = fmaf(e[b][1] - e[b][0], t[o1], e[b][0]);
+          const float m1 = fmaf(e[b][3] - e[b][2], t[o1], e[b][2]);
           se = fmaf(m1 - m0, t[o2], m0);
         }
-        const float vm = upper ? fmaf(shi - slo, th[b], slo) : fmaf(slo - se, th[b], se);
-        const float vp = upper ? fmaf(se - shi, th[b], shi) : fmaf(shi - slo, th[b], slo);
+        const float vm = upper[b] ? fmaf(shi - slo, th[b], slo) : fmaf(slo - se, th[b], se);
+        const float vp = upper[b] ? fmaf(se - shi, th[b], shi) : fmaf(shi - slo, th[b], slo);
         g[b] = (vp - vm) * R.cd_scale[b];
       }
+      if (!counted) continue;
       if (any_unc) {
         // half-step points near the buffer's edge: after the walks
         if (nd < lane_cap) defer[nd] = w.s;
