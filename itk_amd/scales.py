@@ -35,27 +35,46 @@ class RegistrationParameterScalesFromPhysicalShift:
         if self.virtual is None:
             raise ValueError("virtual domain is not set")
 
-    def _corners(self):
+    def _corners(self) -> np.ndarray:
+        """The virtual domain's 2^D corner points, one per row."""
         g = self.virtual
         d = g.dim
         pts = []
         for i in range(1 << d):
             idx = [g.start[k] + (((i >> k) & 1) * (g.size[k] - 1)) for k in range(d)]
             pts.append(g.index_to_point(idx))
-        return pts
+        return np.array(pts)
 
-    def _max_shift(self, delta) -> float:
+    @staticmethod
+    def _map(tf, pts: np.ndarray) -> np.ndarray:
+        """TransformPoint of every row: y[r] = (sum_c M[r][c] x[c]) + offset[r],
+        summed in c order (element for element the transform's own arithmetic)."""
+        m = tf.matrix
+        off = tf.offset
+        d = pts.shape[1]
+        y = np.empty_like(pts)
+        for r in range(d):
+            s = np.zeros(pts.shape[0])
+            for c in range(d):
+                s = s + m[r, c] * pts[:, c]
+            y[:, r] = s + off[r]
+        return y
+
+    def _max_shift(self, delta, pts=None, before=None) -> float:
         tf = self.transform
+        if pts is None:
+            pts = self._corners()
+        if before is None:
+            before = self._map(tf, pts)
         old = tf.get_parameters()
-        pts = self._corners()
-        before = [tf.transform_point(p) for p in pts]
         tf.update_transform_parameters(delta, 1.0)
-        after = [tf.transform_point(p) for p in pts]
+        after = self._map(tf, pts)
         tf.set_parameters(old)
-        m = 0.0
-        for a, b in zip(after, before):
-            m = max(m, float(np.sqrt(np.sum((np.asarray(a) - np.asarray(b)) ** 2))))
-        return m
+        e = after - before
+        sq = e[:, 0] * e[:, 0]
+        for k in range(1, e.shape[1]):
+            sq = sq + e[:, k] * e[:, k]
+        return float(np.max(np.sqrt(sq))) if sq.size else 0.0
 
     def EstimateScales(self) -> np.ndarray:
         self._check()
@@ -63,10 +82,12 @@ class RegistrationParameterScalesFromPhysicalShift:
         eps = np.finfo(np.float64).eps
         scales = np.zeros(P)
         min_nonzero = np.finfo(np.float64).max
+        pts = self._corners()
+        before = self._map(self.transform, pts)
         for i in range(P):
             delta = np.zeros(P)
             delta[i] = self.small_parameter_variation
-            s = self._max_shift(delta)
+            s = self._max_shift(delta, pts, before)
             scales[i] = s
             if eps < s < min_nonzero:
                 min_nonzero = s
