@@ -2383,6 +2383,33 @@ void drop_step_graph(mmi_batch* b) {
   b->step_sig.clear();
 }
 
+// One capture of `work` on the stream, instantiated.  False, with no graph,
+// when the capture fails: another host thread may make a call that waits for
+// the whole device (a free, a device synchronisation) while the capture is
+// open, which invalidates it.  The caller then runs the work without a graph
+// and captures again on its next call; the failed capture's error is cleared
+// (it is not the work's).
+template <typename Work>
+bool capture_graph(hipStream_t st, Work&& work, hipGraph_t* graph, hipGraphExec_t* exec) {
+  *graph = nullptr;
+  *exec = nullptr;
+  bool ok = false;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+    const hipError_t e = work();
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(st, &g);
+    if (e == hipSuccess && e2 == hipSuccess && g && hipGraphInstantiate(exec, g, nullptr, nullptr, 0) == hipSuccess) {
+      *graph = g;
+      ok = true;
+    } else {
+      if (g) (void)hipGraphDestroy(g);
+      *exec = nullptr;
+    }
+  }
+  (void)hipGetLastError();
+  return ok;
+}
+
 void drop_graph(mmi_batch* b) {
   drop_eval_graph(b);
   drop_step_graph(b);
@@ -2599,6 +2626,14 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
     L.out_stride = stride;
     L.out = b->d_out.as<double>();
     MMI_HIP(hipEventRecord(b->ev0, b->stream));
+    auto work = [&]() -> hipError_t {
+      hipError_t e = hipMemcpyAsync(b->d_stage.p, b->h_stage, bytes, hipMemcpyHostToDevice, b->stream);
+      if (e == hipSuccess) e = launch_fast_batch(L, b->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(b->h_out, b->d_out.p, sizeof(double) * stride * nf, hipMemcpyDeviceToHost, b->stream);
+      return e;
+    };
+    bool replayed = false;
     if (b->use_graph) {
       const std::vector<int64_t> sig = {nf, dim, storage, L.want_derivative, total, stride, bins_max, L.lds1, L.cls,
                                         (int64_t)bytes,
@@ -2606,28 +2641,14 @@ int mmi_batch_evaluate(mmi_batch* b, mmi_context* const* ctxs, int32_t n, int32_
                                         (int64_t)(intptr_t)b->h_stage, (int64_t)(intptr_t)b->h_out};
       if (!b->exec || sig != b->sig) {
         drop_eval_graph(b);
-        MMI_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e = hipMemcpyAsync(b->d_stage.p, b->h_stage, bytes, hipMemcpyHostToDevice, b->stream);
-        if (e == hipSuccess) e = launch_fast_batch(L, b->stream);
-        if (e == hipSuccess)
-          e = hipMemcpyAsync(b->h_out, b->d_out.p, sizeof(double) * stride * nf, hipMemcpyDeviceToHost, b->stream);
-        hipGraph_t g = nullptr;
-        const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
-        if (e != hipSuccess || e2 != hipSuccess) {
-          if (g) (void)hipGraphDestroy(g);
-          return fail(MMI_ERR_DEVICE, std::string("HIP graph capture failed: ") +
-                                          hipGetErrorString(e != hipSuccess ? e : e2));
-        }
-        b->graph = g;
-        MMI_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-        b->sig = sig;
+        if (capture_graph(b->stream, work, &b->graph, &b->exec)) b->sig = sig;
       }
-      MMI_HIP(hipGraphLaunch(b->exec, b->stream));
-    } else {
-      MMI_HIP(hipMemcpyAsync(b->d_stage.p, b->h_stage, bytes, hipMemcpyHostToDevice, b->stream));
-      MMI_HIP(launch_fast_batch(L, b->stream));
-      MMI_HIP(hipMemcpyAsync(b->h_out, b->d_out.p, sizeof(double) * stride * nf, hipMemcpyDeviceToHost, b->stream));
+      if (b->exec) {
+        MMI_HIP(hipGraphLaunch(b->exec, b->stream));
+        replayed = true;
+      }
     }
+    if (!replayed) MMI_HIP(work());
     MMI_HIP(hipEventRecord(b->ev1, b->stream));
   }
   // the other contexts, each on its own stream from its own host thread, while
@@ -2844,24 +2865,13 @@ int regular_step_device(mmi_batch* b, mmi_context* const* ctxs, int32_t n, const
                                       (int64_t)(intptr_t)b->d_stage.p, (int64_t)(intptr_t)b->h_flags};
     if (!b->step_exec || sig != b->step_sig) {
       drop_step_graph(b);
-      MMI_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      const hipError_t e = chunk();
-      hipGraph_t g = nullptr;
-      const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
-      if (e != hipSuccess || e2 != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        return fail(MMI_ERR_DEVICE, std::string("HIP graph capture failed: ") +
-                                        hipGetErrorString(e != hipSuccess ? e : e2));
-      }
-      b->step_graph = g;
-      MMI_HIP(hipGraphInstantiate(&b->step_exec, b->step_graph, nullptr, nullptr, 0));
-      b->step_sig = sig;
+      if (capture_graph(b->stream, chunk, &b->step_graph, &b->step_exec)) b->step_sig = sig;
     }
   }
   MMI_HIP(hipEventRecord(b->ev0, b->stream));
   const int chunks = (max_steps + K - 1) / K;
   for (int c = 0; c < chunks; ++c) {
-    if (b->use_graph) {
+    if (b->use_graph && b->step_exec) {
       MMI_HIP(hipGraphLaunch(b->step_exec, b->stream));
     } else {
       MMI_HIP(chunk());

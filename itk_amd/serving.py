@@ -187,10 +187,27 @@ class _Lane:
                 items, self.pending = self.pending, []
                 self.first = time.perf_counter() if self.optimizing else None
                 opt = list(self.optimizing)
+            # a failure of a launch fails the registrations it served (their
+            # threads raise it) instead of stopping the lane with them waiting
             if opt:
-                self._run_optimizing(opt)
+                try:
+                    self._run_optimizing(opt)
+                except Exception as e:
+                    with srv.lock:  # the levels not finished before the failure
+                        failed = [it for it in opt if it in self.optimizing]
+                        for it in failed:
+                            self.optimizing.remove(it)
+                    for it in failed:
+                        it.append(e)
+                        it[3].set()
             if items:
-                self._run_batch(items)
+                try:
+                    self._run_batch(items)
+                except Exception as e:
+                    for job, _, slot in items:
+                        if not slot["event"].is_set():
+                            slot["value"], slot["deriv"], slot["err"] = None, None, e
+                            slot["event"].set()
 
     def _run_optimizing(self, jobs):
         """A few RegularStep iterations of every registration optimizing in this
@@ -291,6 +308,8 @@ class RegistrationServer:
             lane.cv.notify()
         if not it[3].wait(self.wait_limit):
             raise RuntimeError("batched optimizer level not served within %.0f s" % self.wait_limit)
+        if len(it) > 4:
+            raise it[4]
         job.evaluations += int(it[2].iteration)  # one evaluation per iteration
         return it[2]
 

@@ -67,3 +67,54 @@ def test_new_context_when_nothing_is_pooled():
     s = _server(1000, [])
     metric, extra = s._pooled_metric(600)
     assert metric is None and extra == 600
+
+
+class _Job:
+    def __init__(self, lane):
+        self.lane, self.evaluations, self.metric = lane, 0, object()
+
+
+class _FailingBatch:
+    def regular_step(self, *a, **k):
+        raise RuntimeError("launch failed")
+
+    def evaluate(self, *a, **k):
+        raise RuntimeError("launch failed")
+
+    def close(self):
+        pass
+
+
+def test_a_failed_launch_fails_its_registrations_instead_of_stopping_the_lane():
+    """A launch that raises in a batching lane is delivered to the
+    registrations it served; the lane keeps serving the next ones."""
+    import threading
+    import types
+
+    import pytest
+
+    from itk_amd.serving import _Lane
+
+    s = RegistrationServer(hbm_budget_bytes=1 << 30, wait_limit_s=10.0)
+    lane = object.__new__(_Lane)  # no device: the lane's own fields with a batch that fails
+    lane.server, lane.batch = s, _FailingBatch()
+    lane.cv = threading.Condition(s.lock)
+    lane.pending, lane.first, lane.optimizing, lane.running = [], None, [], set()
+    lane.thread = threading.Thread(target=lane._loop, daemon=True)
+    lane.thread.start()
+    job = _Job(lane)
+    with s.lock:
+        lane.running.add(job)
+    opts = types.SimpleNamespace(learning_rate=1.0, minimum_step_length=1e-3, relaxation_factor=0.5,
+                                 gradient_magnitude_tolerance=1e-4, number_of_iterations=10)
+    for _ in range(2):
+        with pytest.raises(RuntimeError, match="launch failed"):
+            s._optimize(job, opts, types.SimpleNamespace(iteration=0, stop=0))
+        with pytest.raises(RuntimeError, match="launch failed"):
+            s._evaluate(job, True)
+    with s.lock:
+        s.stopping = True
+        lane.running.clear()
+        lane.cv.notify()
+    lane.thread.join(5.0)
+    assert not lane.thread.is_alive()
