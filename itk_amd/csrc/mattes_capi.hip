@@ -341,6 +341,11 @@ struct mmi_context {
   size_t h_coeffs_bytes = 0;
   void* h_out = nullptr;
   size_t h_out_bytes = 0;
+  // the fast affine evaluation's result block, written by its last kernel:
+  // coherent pinned memory and its device address
+  void* h_res = nullptr;
+  char* h_res_dev = nullptr;
+  size_t h_res_bytes = 0;
   float last_total_ms = 0, last_pass1_ms = 0, last_pass2_ms = 0;
   int32_t last_path = 0;  // 0 exact, 1 fast, 2 fast redone exactly
   bool acc_clean = false;  // pass-1 accumulators known to be zero (finalize zeroes them)
@@ -415,6 +420,9 @@ namespace {
 // P derivative values.
 static_assert(sizeof(DevResult) % 8 == 0, "derivative alignment");
 size_t out_bytes(int64_t P) { return sizeof(DevResult) + 16 + sizeof(double) * (size_t)P; }
+// parameters of the largest transform the fast affine derivative writes
+// straight to host memory (a 3-D affine map)
+constexpr int64_t kMaxOutParams = 12;
 DevResult* dev_result(const mmi_context* ctx) { return ctx->out.as<DevResult>(); }
 double* dev_deriv(const mmi_context* ctx) {
   return reinterpret_cast<double*>(static_cast<char*>(ctx->out.p) + sizeof(DevResult) + 16);
@@ -1418,6 +1426,7 @@ int mmi_context_destroy(mmi_context* ctx) {
     if (ev) (void)hipEventDestroy(ev);
   if (ctx->h_coeffs) (void)hipHostFree(ctx->h_coeffs);
   if (ctx->h_out) (void)hipHostFree(ctx->h_out);
+  if (ctx->h_res) (void)hipHostFree(ctx->h_res);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MMI_OK;
@@ -2117,7 +2126,10 @@ int evaluate_impl(mmi_context* ctx, int32_t want_derivative, double* value, doub
   // An event record costs the queue a few microseconds (a timestamp with a
   // release), so pass 1 and pass 2 are timed apart only on request.
   const bool time_passes = (ctx->flags & 4u) != 0;
+  // the fast affine derivative kernel wrote the result block to h_res
+  bool direct_out = false;
 retry:
+  direct_out = false;
   MMI_HIP(hipEventRecord(ctx->ev[0], st));
   // pass 1's accumulators (histogram, count, overflow word) are zero between
   // evaluations of a context: finalize leaves them so; a clear kernel runs
@@ -2186,7 +2198,20 @@ retry:
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     } else if (fast2) {
       MMI_HIP(launch_fast_pass2(fa, a, D, ctx->grid, st));
-      MMI_HIP(launch_fast_derivative(fa.partials, 2 * ctx->grid, fast_map(ctx), D, f.result, b.deriv(), st));
+      // the context's evaluation: the derivative kernel also writes the
+      // result block into coherent pinned memory (no copy after it)
+      if (!sb && !ctx->h_res_dev) {
+        if (ctx->h_res) (void)hipHostFree(ctx->h_res);
+        ctx->h_res = nullptr;
+        MMI_HIP(hipHostMalloc(&ctx->h_res, out_bytes(kMaxOutParams), hipHostMallocCoherent));
+        ctx->h_res_bytes = out_bytes(kMaxOutParams);
+        void* dp = nullptr;
+        MMI_HIP(hipHostGetDevicePointer(&dp, ctx->h_res, 0));
+        ctx->h_res_dev = static_cast<char*>(dp);
+      }
+      direct_out = !sb && (int64_t)P <= kMaxOutParams;
+      MMI_HIP(launch_fast_derivative(fa.partials, 2 * ctx->grid, fast_map(ctx), D, f.result, b.deriv(),
+                                     direct_out ? ctx->h_res_dev : nullptr, st));
       MMI_HIP(hipEventRecord(ctx->ev[3], st));
     } else if (ctx->mtf.kind == MMI_TF_AFFINE) {
       if (!sb) {
@@ -2224,7 +2249,7 @@ retry:
     MMI_HIP(hipMemcpyAsync(&r, b.out, sizeof(DevResult), hipMemcpyDeviceToHost, st));
     if (want_derivative && derivative)
       MMI_HIP(hipMemcpyAsync(derivative, b.deriv(), sizeof(double) * P, hipMemcpyDeviceToHost, st));
-  } else {
+  } else if (!direct_out) {
     if (int rc = ensure_host_pinned(&ctx->h_out, &ctx->h_out_bytes, out_bytes(P))) return rc;
     MMI_HIP(hipMemcpyAsync(ctx->h_out, b.out, ob, hipMemcpyDeviceToHost, st));
   }
@@ -2234,7 +2259,7 @@ retry:
   MMI_HIP(hipStreamSynchronize(st));
   ctx->acc_clean = !sb;  // finalize ran (it zeroes the accumulators it read)
   if (!sb) {
-    const char* hout = static_cast<const char*>(ctx->h_out);
+    const char* hout = static_cast<const char*>(direct_out ? ctx->h_res : ctx->h_out);
     std::memcpy(&r, hout, sizeof(DevResult));
     if (want_derivative && derivative)
       std::memcpy(derivative, hout + sizeof(DevResult) + 16, sizeof(double) * P);

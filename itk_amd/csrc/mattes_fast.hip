@@ -1585,7 +1585,8 @@ __device__ __forceinline__ void pass2_cd_tail_body(const FastArgs& f, const Eval
 // with V the virtual index-to-physical matrix and c0 = virtual origin - centre.
 template <int D>
 __device__ __forceinline__ void fast_derivative_body(const double* __restrict__ partials, int rows, const FastMap& map,
-                                                     const DevResult* __restrict__ result, double* __restrict__ deriv) {
+                                                     const DevResult* __restrict__ result, double* __restrict__ deriv,
+                                                     char* host_out = nullptr) {
   constexpr int PA = D * D + D;
   __shared__ double scratch[4 * PA];
   __shared__ double mom[PA];
@@ -1608,6 +1609,16 @@ __device__ __forceinline__ void fast_derivative_body(const double* __restrict__ 
         deriv[b * D + d] = s * nf;
       }
       deriv[D * D + b] = m1 * nf;
+    }
+    if (host_out) {
+      // the result block and the derivative straight into the caller's
+      // pinned block (its layout: DevResult, 16 bytes, the derivative), so
+      // no copy follows the kernel
+      DevResult* hr = reinterpret_cast<DevResult*>(host_out);
+      *hr = *result;
+      double* hd = reinterpret_cast<double*>(host_out + sizeof(DevResult) + 16);
+      for (int k = 0; k < PA; ++k) hd[k] = deriv[k];
+      __threadfence_system();
     }
   }
 }
@@ -1749,8 +1760,8 @@ __global__ void __launch_bounds__(256) pass2_fast_batch(const FastArgs* __restri
 template <int D>
 __global__ void __launch_bounds__(256) fast_derivative_kernel(const double* __restrict__ partials, int rows,
                                                               FastMap map, const DevResult* __restrict__ result,
-                                                              double* __restrict__ deriv) {
-  fast_derivative_body<D>(partials, rows, map, result, deriv);
+                                                              double* __restrict__ deriv, char* host_out) {
+  fast_derivative_body<D>(partials, rows, map, result, deriv, host_out);
 }
 
 template <int D>
@@ -1897,11 +1908,11 @@ hipError_t launch_fast_pass2(const FastArgs& f, const EvalArgs& a, int dim, int 
 }
 
 hipError_t launch_fast_derivative(const double* partials, int rows, const FastMap& map, int dim,
-                                  const DevResult* r, double* deriv, hipStream_t st) {
+                                  const DevResult* r, double* deriv, char* host_out, hipStream_t st) {
   if (dim == 2)
-    hipLaunchKernelGGL(fast_derivative_kernel<2>, dim3(1), dim3(256), 0, st, partials, rows, map, r, deriv);
+    hipLaunchKernelGGL(fast_derivative_kernel<2>, dim3(1), dim3(256), 0, st, partials, rows, map, r, deriv, host_out);
   else
-    hipLaunchKernelGGL(fast_derivative_kernel<3>, dim3(1), dim3(256), 0, st, partials, rows, map, r, deriv);
+    hipLaunchKernelGGL(fast_derivative_kernel<3>, dim3(1), dim3(256), 0, st, partials, rows, map, r, deriv, host_out);
   return hipGetLastError();
 }
 

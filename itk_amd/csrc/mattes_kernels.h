@@ -389,16 +389,19 @@ hipError_t launch_pass2(const EvalArgs& a, int grid, hipStream_t st);
 // Pass 1 of the fast path; each workgroup ends by evaluating its own uncertain
 // samples on the exact path (the EvalArgs) into its LDS histogram.
 hipError_t launch_fast_pass1(const FastArgs& f, const EvalArgs& a, int dim, int grid, hipStream_t st);
-// B-spline fast path: pass 1's listed samples on the exact path.
 // The exact fixed bin of each of the n dense virtual samples (FastArgs::fbin8).
 hipError_t launch_fixed_bins(const EvalArgs& a, uint8_t* out, uint32_t n, int grid, hipStream_t st);
+// B-spline fast path: pass 1's listed samples on the exact path.
 hipError_t launch_fast_exceptions1(const EvalArgs& a, const FastArgs& f, int grid, hipStream_t st);
 // Pass 2 of the fast path; affine workgroups end with their listed samples on
 // the exact path (their own partial row).
 hipError_t launch_fast_pass2(const FastArgs& f, const EvalArgs& a, int dim, int grid, hipStream_t st);
-// B-spline fast path: pass 2's listed samples on the exact path.
+// The affine derivative from the fast pass 2's partial rows (fixed-order
+// reduction, then the index-space moments mapped to the parameters); with
+// host_out, the result block and the derivative are also written into that
+// pinned host block (out_bytes layout).
 hipError_t launch_fast_derivative(const double* partials, int rows, const FastMap& map, int dim,
-                                  const DevResult* r, double* deriv, hipStream_t st);
+                                  const DevResult* r, double* deriv, char* host_out, hipStream_t st);
 hipError_t launch_derivative_reduce(const double* partials, int nblocks, int P, const DevResult* r, double* deriv,
                                     hipStream_t st);
 hipError_t launch_derivative_scale(double* deriv, int64_t P, const DevResult* r, hipStream_t st);
