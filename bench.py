@@ -44,9 +44,9 @@ BYTES_PER_SAMPLE_3D_CD = {"f32": 12, "bf16": 6, "f16": 6}
 # profiles/mattes_affine_256cube_central_differences.md and
 # profiles/mattes_bspline_192cube_mesh16.md.  None where not profiled.
 TRAFFIC_BYTES_PER_EVAL = {
-    ("mattes_mi_dense_value_and_derivative_affine_256cube", "f32", "image"): 417.1e6,
-    ("mattes_mi_dense_value_and_derivative_affine_256cube", "f32", "cd"): 218.8e6,
-    ("mattes_mi_dense_value_and_derivative_bspline_192cube_mesh16", "f32", "image"): 527.1e6,
+    ("mattes_mi_dense_value_and_derivative_affine_256cube", "f32", "image"): 432.3e6,
+    ("mattes_mi_dense_value_and_derivative_affine_256cube", "f32", "cd"): 214.7e6,
+    ("mattes_mi_dense_value_and_derivative_bspline_192cube_mesh16", "f32", "image"): 541.4e6,
 }
 
 
