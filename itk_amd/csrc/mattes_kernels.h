@@ -426,8 +426,6 @@ hipError_t launch_pad_replicate(const void* src, void* dst, int elem_bytes, int 
 hipError_t launch_texels_to_storage(const float4* in, void* out, int storage, int64_t n, hipStream_t st);
 // Independent recursive-Gaussian passes in one launch (one job per chain).
 hipError_t launch_recursive_gaussian_jobs(const RecJobs& jobs, bool in_float, hipStream_t st);
-hipError_t launch_recursive_gaussian(const void* in, bool in_float, double* out, int64_t ln, int64_t stride,
-                                     int64_t nlines, const RecCoeffs& c, const RecSink& sink, hipStream_t st);
 hipError_t launch_clear_eval(unsigned long long* hist, int n_hist, unsigned long long* nvalid, uint32_t* overflow,
                              hipStream_t st);
 // Per-workgroup min/max over the masked voxels of an image (mask) or over the

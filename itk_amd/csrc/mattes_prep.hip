@@ -188,27 +188,196 @@ __device__ __forceinline__ void recursive_gaussian_line(const TIn* __restrict__ 
   }
 }
 
+// The same pass over lines along x (stride 1), which lie one after another in
+// memory: with a line per thread every load and store of a wavefront would
+// touch 64 lines.  The workgroup's kRecLines consecutive lines form one block;
+// their elements pass through LDS kRecX at a time, read and written by
+// consecutive threads along each line, and each thread runs its line's
+// recursion from its LDS row with recursive_gaussian_line's arithmetic (the
+// same values in the same order, so the same bits).  The first and last
+// elements of a line, which start the two recursions, are read directly.
+constexpr int kRecX = 8;              // elements of each line per LDS stage
+constexpr int kRecPitch = kRecX + 1;  // LDS row pitch in doubles
+constexpr size_t kRecRowsLdsBytes = 2 * kRecLines * kRecPitch * sizeof(double);
+static_assert(kRecX == 8 && kRecLines == 128, "the staging index split assumes 8 elements of 128 lines");
+
+// Elements [i0, i0 + n) of the block's nl lines into an LDS tile.
 template <typename TIn>
-__global__ void recursive_gaussian_kernel(const TIn* __restrict__ in, double* __restrict__ out, int64_t ln,
-                                          int64_t stride, int64_t nlines, RecCoeffs c, RecSink sink) {
-  recursive_gaussian_line<TIn>(in, out, ln, stride, nlines, c, sink, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+__device__ __forceinline__ void rec_stage(double* tile, const TIn* __restrict__ src, int64_t blk, int64_t ln,
+                                          int64_t i0, int n, int nl) {
+  for (int r = threadIdx.x; r < kRecLines * kRecX; r += kRecLines) {
+    const int j = r >> 3, k = r & 7;
+    if (j < nl && k < n) tile[j * kRecPitch + k] = (double)src[blk + (int64_t)j * ln + i0 + k];
+  }
 }
+
+// A final value of element e where the pass's sink puts it.
+__device__ __forceinline__ void rec_sink_store(const RecSink& sink, double* __restrict__ out, int64_t e, double v) {
+  if (sink.mode == kSinkGradient)
+    sink.grad[e * sink.dim + sink.comp] = v / sink.spacing;
+  else if (sink.mode == kSinkFloat)
+    sink.outf[e] = (float)v;
+  else
+    out[e] = v;
+}
+
+template <typename TIn>
+__device__ __forceinline__ void recursive_gaussian_rows(const TIn* __restrict__ in, double* __restrict__ out,
+                                                        int64_t ln, int64_t nlines, const RecCoeffs& cref,
+                                                        const RecSink& sref, int64_t l0, double* tx, double* ty) {
+  // by value: the coefficients and sink then stay in scalar registers
+  const RecCoeffs c = cref;
+  const RecSink sink = sref;
+  const int t = threadIdx.x;
+  const bool live = l0 + t < nlines;
+  const int nl = (int)min((int64_t)kRecLines, nlines - l0);
+  const int64_t blk = l0 * ln;
+  const int64_t base = blk + (int64_t)t * ln;  // this thread's line
+  double* rx = tx + t * kRecPitch;
+  double* ry = ty + t * kRecPitch;
+  // causal
+  double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0, xm1 = 0.0, xm2 = 0.0, xm3 = 0.0;
+  if (live) {
+    const double v1 = (double)in[base];
+    const double d1 = (double)in[base + 1], d2 = (double)in[base + 2], d3 = (double)in[base + 3];
+    double q0 = v1 * c.N0 + v1 * c.N1 + v1 * c.N2 + v1 * c.N3;
+    double q1 = d1 * c.N0 + v1 * c.N1 + v1 * c.N2 + v1 * c.N3;
+    double q2 = d2 * c.N0 + d1 * c.N1 + v1 * c.N2 + v1 * c.N3;
+    double q3 = d3 * c.N0 + d2 * c.N1 + d1 * c.N2 + v1 * c.N3;
+    q0 -= v1 * c.BN1 + v1 * c.BN2 + v1 * c.BN3 + v1 * c.BN4;
+    q1 -= q0 * c.D1 + v1 * c.BN2 + v1 * c.BN3 + v1 * c.BN4;
+    q2 -= q1 * c.D1 + q0 * c.D2 + v1 * c.BN3 + v1 * c.BN4;
+    q3 -= q2 * c.D1 + q1 * c.D2 + q0 * c.D3 + v1 * c.BN4;
+    out[base] = q0;
+    out[base + 1] = q1;
+    out[base + 2] = q2;
+    out[base + 3] = q3;
+    o0 = q3;
+    o1 = q2;
+    o2 = q1;
+    o3 = q0;
+    xm1 = d3;
+    xm2 = d2;
+    xm3 = d1;
+  }
+  for (int64_t i0 = 4; i0 < ln; i0 += kRecX) {
+    const int n = (int)min((int64_t)kRecX, ln - i0);
+    __syncthreads();  // the previous stage's tiles are consumed
+    rec_stage<TIn>(tx, in, blk, ln, i0, n, nl);
+    __syncthreads();
+    if (live) {
+      for (int k = 0; k < n; ++k) {
+        const double xi = rx[k];
+        double q = xi * c.N0 + xm1 * c.N1 + xm2 * c.N2 + xm3 * c.N3;
+        q -= o0 * c.D1 + o1 * c.D2 + o2 * c.D3 + o3 * c.D4;
+        ry[k] = q;
+        o3 = o2;
+        o2 = o1;
+        o1 = o0;
+        o0 = q;
+        xm3 = xm2;
+        xm2 = xm1;
+        xm1 = xi;
+      }
+    }
+    __syncthreads();
+    for (int r = t; r < kRecLines * kRecX; r += kRecLines) {
+      const int j = r >> 3, k = r & 7;
+      if (j < nl && k < n) out[blk + (int64_t)j * ln + i0 + k] = ty[j * kRecPitch + k];
+    }
+  }
+  __syncthreads();  // every line's causal values are in place
+  // anti-causal
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, xp1 = 0.0, xp2 = 0.0, xp3 = 0.0;
+  if (live) {
+    const double v2 = (double)in[base + ln - 1];
+    const double e1 = (double)in[base + ln - 1], e2 = (double)in[base + ln - 2], e3 = (double)in[base + ln - 3];
+    double r1 = v2 * c.M1 + v2 * c.M2 + v2 * c.M3 + v2 * c.M4;
+    double r2 = e1 * c.M1 + v2 * c.M2 + v2 * c.M3 + v2 * c.M4;
+    double r3 = e2 * c.M1 + e1 * c.M2 + v2 * c.M3 + v2 * c.M4;
+    double r4 = e3 * c.M1 + e2 * c.M2 + e1 * c.M3 + v2 * c.M4;
+    r1 -= v2 * c.BM1 + v2 * c.BM2 + v2 * c.BM3 + v2 * c.BM4;
+    r2 -= r1 * c.D1 + v2 * c.BM2 + v2 * c.BM3 + v2 * c.BM4;
+    r3 -= r2 * c.D1 + r1 * c.D2 + v2 * c.BM3 + v2 * c.BM4;
+    r4 -= r3 * c.D1 + r2 * c.D2 + r1 * c.D3 + v2 * c.BM4;
+    const double y1 = out[base + ln - 1], y2 = out[base + ln - 2];
+    const double y3 = out[base + ln - 3], y4 = out[base + ln - 4];
+    rec_sink_store(sink, out, base + ln - 1, y1 + r1);
+    rec_sink_store(sink, out, base + ln - 2, y2 + r2);
+    rec_sink_store(sink, out, base + ln - 3, y3 + r3);
+    rec_sink_store(sink, out, base + ln - 4, y4 + r4);
+    s0 = r4;
+    s1 = r3;
+    s2 = r2;
+    s3 = r1;
+    xp1 = e3;
+    xp2 = e2;
+    xp3 = e1;
+  }
+  for (int64_t i0 = ln - 4; i0 > 0; i0 -= kRecX) {
+    // i runs i0, i0 - 1, ..., i0 - n + 1 (> 0): it reads data[i] and the
+    // causal value at i - 1 and finishes element i - 1
+    const int n = (int)min((int64_t)kRecX, i0);
+    const int64_t lo = i0 - n;
+    __syncthreads();
+    rec_stage<TIn>(tx, in, blk, ln, lo + 1, n, nl);
+    rec_stage<double>(ty, out, blk, ln, lo, n, nl);
+    __syncthreads();
+    if (live) {
+      for (int k = 0; k < n; ++k) {
+        const double xi = rx[n - 1 - k];
+        double q = xi * c.M1 + xp1 * c.M2 + xp2 * c.M3 + xp3 * c.M4;
+        q -= s0 * c.D1 + s1 * c.D2 + s2 * c.D3 + s3 * c.D4;
+        ry[n - 1 - k] = ry[n - 1 - k] + q;
+        s3 = s2;
+        s2 = s1;
+        s1 = s0;
+        s0 = q;
+        xp3 = xp2;
+        xp2 = xp1;
+        xp1 = xi;
+      }
+    }
+    __syncthreads();
+    for (int r = t; r < kRecLines * kRecX; r += kRecLines) {
+      const int j = r >> 3, k = r & 7;
+      if (j < nl && k < n) rec_sink_store(sink, out, blk + (int64_t)j * ln + lo + k, ty[j * kRecPitch + k]);
+    }
+  }
+}
+
 
 // The same pass of up to kRecJobsMax independent chains in one launch: the
 // workgroups are the jobs' in order, so a pass with few lines per chain still
 // fills the device.
 template <typename TIn>
 __global__ void __launch_bounds__(kRecLines) recursive_gaussian_jobs_kernel(RecJobs J) {
+  // the workgroup's job, with constant indices into the argument (an index
+  // held in a register would put the whole argument block in scratch)
+  static_assert(kRecJobsMax == 4, "one test per job boundary below");
   uint32_t b = blockIdx.x;
   int k = 0;
-  while (k + 1 < J.n && b >= (uint32_t)J.j[k].blocks) {
-    b -= (uint32_t)J.j[k].blocks;
-    ++k;
+  if (J.n > 1 && b >= (uint32_t)J.j[0].blocks) {
+    b -= (uint32_t)J.j[0].blocks;
+    k = 1;
+    if (J.n > 2 && b >= (uint32_t)J.j[1].blocks) {
+      b -= (uint32_t)J.j[1].blocks;
+      k = 2;
+      if (J.n > 3 && b >= (uint32_t)J.j[2].blocks) {
+        b -= (uint32_t)J.j[2].blocks;
+        k = 3;
+      }
+    }
   }
-  const int64_t l = (int64_t)b * kRecLines + threadIdx.x;
+  const int64_t l0 = (int64_t)b * kRecLines;
+  extern __shared__ double rec_lds[];  // kRecRowsLdsBytes when the launch holds a pass along x
 #define MMI_REC_JOB(K)                                                                                          \
-  recursive_gaussian_line<TIn>(static_cast<const TIn*>(J.j[K].in), J.j[K].out, J.j[K].ln, J.j[K].stride,        \
-                               J.j[K].nlines, J.j[K].c, J.j[K].sink, l)
+  if (J.j[K].stride == 1)                                                                                        \
+    recursive_gaussian_rows<TIn>(static_cast<const TIn*>(J.j[K].in), J.j[K].out, J.j[K].ln, J.j[K].nlines,       \
+                                 J.j[K].c, J.j[K].sink, l0, rec_lds, rec_lds + kRecLines * kRecPitch);          \
+  else                                                                                                           \
+    recursive_gaussian_line<TIn>(static_cast<const TIn*>(J.j[K].in), J.j[K].out, J.j[K].ln, J.j[K].stride,      \
+                                 J.j[K].nlines, J.j[K].c, J.j[K].sink, l0 + threadIdx.x)
   // a workgroup-uniform branch per job keeps the arguments in scalar registers
   if (k == 0)
     MMI_REC_JOB(0);
@@ -608,18 +777,6 @@ hipError_t launch_texels_to_storage(const float4* in, void* out, int storage, in
   return hipGetLastError();
 }
 
-hipError_t launch_recursive_gaussian(const void* in, bool in_float, double* out, int64_t ln, int64_t stride,
-                                     int64_t nlines, const RecCoeffs& c, const RecSink& sink, hipStream_t st) {
-  const dim3 grid((unsigned)((nlines + 127) / 128));
-  if (in_float)
-    hipLaunchKernelGGL(recursive_gaussian_kernel<float>, grid, dim3(128), 0, st, static_cast<const float*>(in), out,
-                       ln, stride, nlines, c, sink);
-  else
-    hipLaunchKernelGGL(recursive_gaussian_kernel<double>, grid, dim3(128), 0, st, static_cast<const double*>(in),
-                       out, ln, stride, nlines, c, sink);
-  return hipGetLastError();
-}
-
 hipError_t launch_recursive_gaussian_jobs(const RecJobs& jobs, bool in_float, hipStream_t st) {
   if (jobs.n < 1 || jobs.n > kRecJobsMax) return hipErrorInvalidValue;
   unsigned blocks = 0;
@@ -627,10 +784,13 @@ hipError_t launch_recursive_gaussian_jobs(const RecJobs& jobs, bool in_float, hi
     if (jobs.j[k].blocks != (int32_t)((jobs.j[k].nlines + kRecLines - 1) / kRecLines)) return hipErrorInvalidValue;
     blocks += (unsigned)jobs.j[k].blocks;
   }
+  size_t lds = 0;
+  for (int k = 0; k < jobs.n; ++k)
+    if (jobs.j[k].stride == 1) lds = kRecRowsLdsBytes;
   if (in_float)
-    hipLaunchKernelGGL(recursive_gaussian_jobs_kernel<float>, dim3(blocks), dim3(kRecLines), 0, st, jobs);
+    hipLaunchKernelGGL(recursive_gaussian_jobs_kernel<float>, dim3(blocks), dim3(kRecLines), lds, st, jobs);
   else
-    hipLaunchKernelGGL(recursive_gaussian_jobs_kernel<double>, dim3(blocks), dim3(kRecLines), 0, st, jobs);
+    hipLaunchKernelGGL(recursive_gaussian_jobs_kernel<double>, dim3(blocks), dim3(kRecLines), lds, st, jobs);
   return hipGetLastError();
 }
 
