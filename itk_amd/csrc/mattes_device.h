@@ -112,15 +112,6 @@ __device__ __forceinline__ double bspline3(double u) {
   return 0.0;
 }
 
-// BSplineDerivativeKernelFunction<3> (itkBSplineDerivativeKernelFunction.h:157-180).
-__device__ __forceinline__ double bspline3_deriv(double u) {
-  if (u >= 0.0 && u < 1.0) return -2.0 * u + 1.5 * u * u;
-  if (u > -1.0 && u < 0.0) return -2.0 * u - 1.5 * u * u;
-  if (u >= 1.0 && u < 2.0) return -2.0 + 2.0 * u - 0.5 * u * u;
-  if (u > -2.0 && u <= -1.0) return 2.0 + 2.0 * u + 0.5 * u * u;
-  return 0.0;
-}
-
 // ImageBase::TransformIndexToPhysicalPoint (itkImageBase.h:594-604).
 template <int D>
 __device__ __forceinline__ void index_to_point(const DevGeom& g, const int32_t* idx, double* p) {

@@ -752,49 +752,6 @@ __device__ __forceinline__ uint32_t pad_cell(const FastArgs& f, const long long*
   return cb * E;
 }
 
-// Multilinear value (TEX: and gradient) at a padded cell: one address per
-// (y, z) corner pair, the x neighbour an immediate offset; mlerp_value /
-// mlerp_texels as every fast-path interpolation.
-template <int D, typename P, bool TEX>
-__device__ __forceinline__ float pad_interp(const void* base, uint32_t cb, uint32_t sy, uint32_t sz, const float* t,
-                                            float3* grad) {
-  if constexpr (TEX) {
-    using T = typename P::texel;
-    constexpr uint32_t E = (uint32_t)sizeof(T);
-    const char* p00 = static_cast<const char*>(base) + cb;
-    const char* p10 = p00 + sy;
-    const float4 a000 = P::ldtb(reinterpret_cast<const T*>(p00), 0u), a100 = P::ldtb(reinterpret_cast<const T*>(p00), E);
-    const float4 a010 = P::ldtb(reinterpret_cast<const T*>(p10), 0u), a110 = P::ldtb(reinterpret_cast<const T*>(p10), E);
-    float4 a001 = a000, a101 = a000, a011 = a000, a111 = a000;
-    if (D == 3) {
-      const char* p01 = p00 + sz;
-      const char* p11 = p10 + sz;
-      a001 = P::ldtb(reinterpret_cast<const T*>(p01), 0u);
-      a101 = P::ldtb(reinterpret_cast<const T*>(p01), E);
-      a011 = P::ldtb(reinterpret_cast<const T*>(p11), 0u);
-      a111 = P::ldtb(reinterpret_cast<const T*>(p11), E);
-    }
-    return mlerp_texels<D>(a000, a100, a010, a110, a001, a101, a011, a111, t[0], t[1], t[2], grad);
-  } else {
-    using T = typename P::pixel;
-    constexpr uint32_t E = (uint32_t)sizeof(T);
-    const char* p00 = static_cast<const char*>(base) + cb;
-    const char* p10 = p00 + sy;
-    const float v000 = P::ldb(reinterpret_cast<const T*>(p00), 0u), v100 = P::ldb(reinterpret_cast<const T*>(p00), E);
-    const float v010 = P::ldb(reinterpret_cast<const T*>(p10), 0u), v110 = P::ldb(reinterpret_cast<const T*>(p10), E);
-    float v001 = 0.0f, v101 = 0.0f, v011 = 0.0f, v111 = 0.0f;
-    if (D == 3) {
-      const char* p01 = p00 + sz;
-      const char* p11 = p10 + sz;
-      v001 = P::ldb(reinterpret_cast<const T*>(p01), 0u);
-      v101 = P::ldb(reinterpret_cast<const T*>(p01), E);
-      v011 = P::ldb(reinterpret_cast<const T*>(p11), 0u);
-      v111 = P::ldb(reinterpret_cast<const T*>(p11), E);
-    }
-    return mlerp_value<D>(v000, v100, v010, v110, v001, v101, v011, v111, t[0], t[1], t[2]);
-  }
-}
-
 // The loads of one certainly inside sample of a padded walk: its fixed bin,
 // its cell's eight corners (TEX: texels) and weights.
 template <int D, typename P, bool TEX>
