@@ -317,6 +317,19 @@ def pass_breakdown(args, wl, n):
     return out
 
 
+def copy_bandwidth_gbs():
+    """Device-to-device copy rate of a 1 GiB buffer (the library's plain copy
+    probe, read + written bytes over its HIP-event time, best of 10): the
+    practical HBM ceiling the roofline fraction can also be read against."""
+    import ctypes
+
+    from itk_amd import _native as N
+
+    g = ctypes.c_double(0.0)
+    N.check(N.load().mmi_measure_copy_bandwidth(0, 1 << 30, 10, ctypes.byref(g)))
+    return g.value
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -387,6 +400,9 @@ def main():
     }
     m.close()
     line["roofline"]["passes"] = pass_breakdown(args, wl, n)
+    copy = copy_bandwidth_gbs()
+    line["roofline"]["copy_gbs"] = copy
+    line["roofline"]["frac_of_copy"] = achieved / copy if copy else None
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     if served is not None:

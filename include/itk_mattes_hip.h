@@ -307,6 +307,13 @@ int mmi_get_last_timing(mmi_context* ctx, float* total_ms, float* pass1_ms, floa
  * not on the fast path), and the samples per evaluation. */
 int mmi_get_exception_count(mmi_context* ctx, int64_t* n_exact, int64_t* n_samples);
 
+/* Diagnostics (no counterpart in the reference): the device-to-device copy
+ * rate of the device, in GB/s (bytes read + written over the kernel time of a
+ * plain 16-byte-per-lane copy of `bytes` bytes, the best of `reps` launches
+ * timed with HIP events): the practical HBM ceiling beside the published
+ * peak.  Allocates and frees 2 x bytes of device memory. */
+int mmi_measure_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, double* gbs);
+
 /* Diagnostics (no counterpart in the reference): which kernels the last
  * mmi_evaluate ran: 0 the exact FP64 path, 1 the certified fast path, 2 the
  * fast path redone on the exact path after an exception-list overflow. */

@@ -87,6 +87,8 @@ SIGNATURES = {
     "mmi_get_moving_parameters": (ctypes.c_int, [_vp, _dp, ctypes.c_int64]),
     "mmi_get_last_path": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "mmi_get_exception_count": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    "mmi_measure_copy_bandwidth": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
+                                                  ctypes.POINTER(ctypes.c_double)]),
     "mmi_get_last_timing": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                            ctypes.POINTER(ctypes.c_float)]),
 }

@@ -3043,6 +3043,48 @@ int mmi_get_exception_count(mmi_context* ctx, int64_t* n_exact, int64_t* n_sampl
   return MMI_OK;
 }
 
+int mmi_measure_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, double* gbs) {
+  if (!gbs || bytes < 16 || reps < 1) return fail(MMI_ERR_ARGUMENT, "copy probe: bytes >= 16, reps >= 1, gbs set");
+  MMI_HIP(hipSetDevice(device));
+  int cus = 0;
+  MMI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  const int64_t n16 = bytes / 16;
+  DevBuf a, b;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = MMI_OK;
+  double best = 0.0;
+  auto run = [&]() -> hipError_t {
+    hipError_t e = a.ensure((size_t)n16 * 16);
+    if (e == hipSuccess) e = b.ensure((size_t)n16 * 16);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(a.p, 0, (size_t)n16 * 16, st);
+    const int blocks = std::max(1, cus) * 8;
+    if (e == hipSuccess) e = launch_copy_probe(a.p, b.p, n16, blocks, st);  // warm-up
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+      e = hipEventRecord(e0, st);
+      if (e == hipSuccess) e = launch_copy_probe(a.p, b.p, n16, blocks, st);
+      if (e == hipSuccess) e = hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e == hipSuccess && ms > 0.0f) best = std::max(best, 2.0 * (double)n16 * 16.0 / (ms * 1e-3) / 1e9);
+    }
+    return e;
+  };
+  const hipError_t e = run();
+  if (e != hipSuccess) rc = fail(MMI_ERR_DEVICE, std::string("copy probe: ") + hipGetErrorString(e));
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  a.release();
+  b.release();
+  *gbs = best;
+  return rc;
+}
+
 int mmi_get_last_path(mmi_context* ctx, int32_t* path) {
   if (!ctx || !path) return fail(MMI_ERR_ARGUMENT, "NULL argument");
   *path = ctx->last_path;

@@ -420,6 +420,8 @@ hipError_t launch_separable_contract(const float* in_f, const double* in_d, doub
 hipError_t launch_minmax(const void* px, int storage, int64_t n, float* out, int blocks, hipStream_t st);
 // FP32 image -> 16-bit storage (round to nearest even), and FP32 texels -> 16-bit texels.
 hipError_t launch_to_storage(const float* in, void* out, int storage, int64_t n, hipStream_t st);
+// A copy of n16 16-byte elements by `blocks` workgroups of 256 (copy-rate probe).
+hipError_t launch_copy_probe(const void* in, void* out, int64_t n16, int blocks, hipStream_t st);
 // The buffer with a one-element replicated border on each axis (pad_replicate_kernel).
 hipError_t launch_pad_replicate(const void* src, void* dst, int elem_bytes, int nx, int ny, int nz, int D,
                                 hipStream_t st);

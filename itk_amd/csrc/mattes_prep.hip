@@ -801,6 +801,19 @@ hipError_t launch_pack_texels(const float* moving, const double* grad, int64_t n
   return hipGetLastError();
 }
 
+// A plain copy, 16 bytes per lane and grid-stride (the copy-rate probe).
+__global__ void __launch_bounds__(256) copy_probe_kernel(const uint4* __restrict__ in, uint4* __restrict__ out,
+                                                         int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
+
+hipError_t launch_copy_probe(const void* in, void* out, int64_t n16, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL(copy_probe_kernel, dim3((unsigned)blocks), dim3(256), 0, st, static_cast<const uint4*>(in),
+                     static_cast<uint4*>(out), n16);
+  return hipGetLastError();
+}
+
 // A copy of an image buffer (nx x ny x nz elements, x fastest) with a
 // one-element border on each side of every axis (z only in 3-D), each border
 // element a copy of its nearest buffer element.  A linear interpolation on the

@@ -70,3 +70,17 @@ def test_table_row_sets_the_launch_configuration(tmp_path):
     finally:
         _load(None)
         m.close()
+
+
+def test_copy_probe_reports_a_rate():
+    """The copy-rate diagnostic the benchmark reports beside the roofline:
+    positive and below the device's published 8 TB/s."""
+    import ctypes
+
+    from itk_amd import _native as N
+
+    g = ctypes.c_double(0.0)
+    N.check(N.load().mmi_measure_copy_bandwidth(0, 64 << 20, 3, ctypes.byref(g)))
+    assert 100.0 < g.value < 8000.0
+    with pytest.raises(N.MetricError):
+        N.check(N.load().mmi_measure_copy_bandwidth(0, 8, 3, ctypes.byref(g)))
