@@ -448,7 +448,7 @@ __device__ __forceinline__ void transform_point(const DevTransform& t, const dou
     double acc[3] = {0.0, 0.0, 0.0};
     constexpr int NR = D == 3 ? 16 : 4;
     const int32_t gx = t.grid.size[0], gxy = t.grid.size[0] * t.grid.size[1];
-#pragma unroll 4
+#pragma unroll
     for (int r = 0; r < NR; ++r) {
       const int32_t row = sup->start[0] + gx * (sup->start[1] + (r & 3)) + (D == 3 ? gxy * (sup->start[2] + (r >> 2)) : 0);
       double c[3][4];
